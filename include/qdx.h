@@ -1,0 +1,174 @@
+/*
+ * qdx.h - C ABI of the MI355X grid-propagation library (libqdx.so).
+ *
+ * The library replaces, for one GPU, the part of QDng that propagates grid
+ * wave functions: the Chebyshev (OCheby) and general split-operator (OGSPO)
+ * propagators, the Hamiltonian they apply (OSum / OMultistate of
+ * OGridNablaSq, OGridPotential, OGridDipole) and the per-step observables of
+ * the Reporter.  Every entry point names the QDng interface it stands in for
+ * (paths relative to the QDng source tree).
+ *
+ * Conventions
+ *   - All functions return QDX_OK (0) or a negative qdx_status; the message of
+ *     the last failure on the calling thread is qdx_last_error().
+ *   - Host arrays are borrowed for the duration of the call only.  Complex
+ *     arrays are interleaved (re, im) doubles in QDng's layout: dimension 0
+ *     fastest (GridSystem.h:26), multistate vectors state-major [S][N]
+ *     (WFMultistate.cpp:43-140).
+ *   - The library owns all device memory.  Handles are destroyed explicitly.
+ *   - One host thread per context at a time; each context owns its stream.
+ */
+#ifndef QDX_H
+#define QDX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum qdx_status {
+    QDX_OK = 0,
+    QDX_ERR_PARAM = -1,       /* QDng EParamProblem (tools/Exception.h:115-146)  */
+    QDX_ERR_INCOMPAT = -2,    /* QDng EIncompatible (tools/Exception.h:97-104)   */
+    QDX_ERR_OVERFLOW = -3,    /* QDng EOverflow (tools/Exception.h:148-154)      */
+    QDX_ERR_MEMORY = -4,      /* QDng EMemError (tools/Exception.h:76-84)        */
+    QDX_ERR_DEVICE = -5,      /* HIP runtime failure                            */
+    QDX_ERR_STATE = -6        /* call order violated (Operator.h:48-58)          */
+} qdx_status;
+
+typedef enum qdx_precision {
+    QDX_FP64 = 0,        /* plain path: every array complex128 / float64        */
+    QDX_FP32_BULK = 1    /* FFT, recurrence and operator tables in FP32; Bessel
+                            coefficients, reductions and the accumulated psi in
+                            FP64                                                 */
+} qdx_precision;
+
+typedef struct qdx_ctx qdx_ctx;
+typedef struct qdx_wf qdx_wf;
+typedef struct qdx_ham qdx_ham;
+typedef struct qdx_prop qdx_prop;
+
+/* Grid geometry, QDng GridSystem (GridSystem.h:29-118, GridSystem.proto:6-17):
+ * dx = (xmax - xmin) / (N - 1), both end points are grid points. */
+typedef struct qdx_grid {
+    int32_t ndims;          /* 1..3 on the device path                           */
+    int32_t N[4];
+    double xmin[4];
+    double xmax[4];
+} qdx_grid;
+
+/* ---- library / context --------------------------------------------------- */
+const char *qdx_last_error(void);
+const char *qdx_version(void);
+/* Bind a context (stream, plans, twiddle tables) to HIP device `device`. */
+int qdx_ctx_create(int device, qdx_ctx **out);
+int qdx_ctx_synchronize(qdx_ctx *ctx);
+int qdx_ctx_destroy(qdx_ctx *ctx);
+
+/* ---- wave functions (WaveFunction.h:34-314, WFGridCartesian.cpp:16-178,
+ *      WFMultistate.cpp:43-300) ------------------------------------------- */
+int qdx_wf_create(qdx_ctx *ctx, const qdx_grid *grid, int32_t nstates, qdx_wf **out);
+/* state < 0: all states ([S][N] array); else one state ([N] array) */
+int qdx_wf_upload(qdx_wf *wf, int32_t state, const double *c128);
+int qdx_wf_download(qdx_wf *wf, int32_t state, double *c128);
+int qdx_wf_copy(qdx_wf *dst, const qdx_wf *src);
+/* QDng WaveFunction::Norm() = sqrt(<psi|psi>) with prod(dx) weights; per_state
+ * (nullable, S entries) receives each state's squared norm (Reporter.cpp:115). */
+int qdx_wf_norm(qdx_wf *wf, double *norm, double *per_state_pop);
+/* <a|b> = sum conj(a) b prod(dx), summed over states (WFGridCartesian.cpp:62-116) */
+int qdx_wf_overlap(qdx_wf *a, qdx_wf *b, double out[2]);
+/* psi *= 1/Norm() (WaveFunction.h:103-106) */
+int qdx_wf_normalize(qdx_wf *wf);
+int qdx_wf_destroy(qdx_wf *wf);
+
+/* ---- Hamiltonian ----------------------------------------------------------
+ * multistate = 0: QDng OSum of grid terms acting on one state
+ *                 (OSum.cpp:28-60); nstates must be 1.
+ * multistate = 1: QDng OMultistate (OMultistate.cpp:34-232): element (r,c) is
+ *                 the sum of the terms added at (r,c); a lone off-diagonal
+ *                 element is mirrored to (c,r); a missing diagonal acts as the
+ *                 identity when unity != 0 (QDng default).                     */
+int qdx_ham_create(qdx_ctx *ctx, const qdx_grid *grid, int32_t nstates, int32_t multistate,
+                   int32_t unity, qdx_ham **out);
+/* ctx may be NULL: a host-only Hamiltonian that supports qdx_ham_bounds and
+ * qdx_cheby_params but no device operation. */
+/* OGridNablaSq (OGridNablaSq.cpp:28-111): -1/(2 m_d) d^2/dx_d^2 on state s */
+int qdx_ham_add_kinetic(qdx_ham *h, int32_t state, const double *mass_per_dim);
+/* OGridPotential (OGridPotential.cpp:124-148): V(x) */
+int qdx_ham_add_potential(qdx_ham *h, int32_t row, int32_t col, const double *V);
+/* OGridDipole (OGridDipole.cpp:36-60) + Laser (Laser.cpp:125-128):
+ * -mu(x) * laser[step] * scale */
+int qdx_ham_add_dipole(qdx_ham *h, int32_t row, int32_t col, const double *mu,
+                       const double *laser, int32_t nt, double scale);
+/* Emin / Emax with QDng's rules (OSum.cpp:28-51, OMultistate.cpp:163-204) */
+int qdx_ham_bounds(qdx_ham *h, double *emin, double *emax);
+/* dst = H(step) src (Operator::Apply, Operator.h:276) */
+int qdx_ham_apply(qdx_ham *h, qdx_wf *dst, qdx_wf *src, int64_t step);
+/* Re <psi|H(step)|psi> (Operator::Expec, Operator.h:249-252) */
+int qdx_ham_expec(qdx_ham *h, qdx_wf *psi, int64_t step, double *out);
+int qdx_ham_destroy(qdx_ham *h);
+
+/* ---- host-side parameter rules (no device needed) ------------------------ */
+/* BesselJ0 (linalg/Bessel.cpp:22-58): J_k(x), k = 0..n-1, and the number of
+ * trailing values zeroed by underflow. */
+int qdx_bessel_j0(int32_t n, double x, double *out, int32_t *nz);
+
+/* ---- propagators --------------------------------------------------------- */
+typedef struct qdx_cheby_info {
+    int32_t order;        /* series length after QDng's truncation              */
+    int32_t nz;           /* Bessel underflow count removed (OCheby.cpp:451)   */
+    double offset;        /* (Emax+Emin)/2   (OCheby.cpp:406-408)               */
+    double scaling;       /* |Emax-Emin|/2   (OCheby.cpp:413-416)               */
+    double prec;          /* break criterion (OCheby.cpp:179-182)               */
+    double emin, emax;
+} qdx_cheby_info;
+
+/* OCheby::Init (OCheby.cpp:358-493), real-time forward propagation.
+ * order <= 0: QDng's automatic order; prec < 0: QDng default 1e-10;
+ * scaling <= 0: automatic.  The Hamiltonian is borrowed and must outlive the
+ * propagator. */
+int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double scaling,
+                     int32_t precision, qdx_prop **out);
+int qdx_cheby_info_get(qdx_prop *p, qdx_cheby_info *info);
+/* The scalars and coefficients OCheby::Init would choose for this Hamiltonian,
+ * without allocating device memory (h may have been created with ctx NULL).
+ * coeff (nullable) receives min(coeff_len, order) values. */
+int qdx_cheby_params(qdx_ham *h, double dt, int32_t order, double prec, double scaling,
+                     qdx_cheby_info *info, double *coeff, int32_t coeff_len);
+/* Bessel-weighted coefficients c_k (order doubles, OCheby.cpp:476-480) */
+int qdx_cheby_coeffs(qdx_prop *p, double *coeff);
+
+/* OGSPO (OGSPO.cpp:30-139): split operators A, B, C, ... each a qdx_ham
+ * holding one kind of term (kinetic, potential or dipole) per matrix element,
+ * QDng's GridNablaSq / GridPotential / GridDipole / DMultistate. */
+int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precision,
+                    qdx_prop **out);
+
+typedef struct qdx_step_stats {
+    int64_t n_hpsi;        /* H applications executed (Chebyshev: 1 + recurrence
+                              iterations actually run; GSPO: steps)              */
+    double norm2;          /* <psi|psi> after the last step                      */
+    double nconv;          /* Chebyshev: |norm0 - accnorm| / norm0 of the last
+                              executed order of the last step (OCheby.cpp:179)  */
+} qdx_step_stats;
+
+/* OPropagator::Apply(psi) at clock step `step` (OCheby.cpp:306-322,
+ * OGSPO.cpp:114-139). */
+int qdx_prop_apply(qdx_prop *p, qdx_wf *psi, int64_t step, qdx_step_stats *stats);
+
+/* ProgPropa::Run step loop (ProgPropa.cpp:317-326) for steps
+ * first_step .. first_step+nsteps-1 without host round trips.  Before each
+ * step the Reporter observables are recorded (Reporter.cpp:65-158):
+ *   norm2[i]     = <psi|psi>            (nullable)
+ *   autocorr[2i] = <psi(t)|psi0>        (nullable; needs psi0)
+ * n_hpsi_per_step (nullable) receives the H-application count of each step. */
+int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, qdx_wf *psi0,
+                 double *norm2, double *autocorr, int32_t *n_hpsi_per_step,
+                 qdx_step_stats *stats);
+int qdx_prop_destroy(qdx_prop *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,261 @@
+"""ctypes front end of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker of the HIP path.  The product
+package (qdng_amd) never imports this module.
+
+Vectors are numpy complex128 arrays; a multistate vector is state-major
+(shape [S, *grid] flattened, dimension 0 fastest inside a state, i.e. the
+numpy shape is (S, N_{d-1}, ..., N_0) in C order).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile next to this file."""
+    cmd = ["make", "-C", _HERE] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class ChebyInfo(C.Structure):
+    _fields_ = [("order", C.c_int), ("nz", C.c_int), ("offset", C.c_double),
+                ("scaling", C.c_double), ("prec", C.c_double),
+                ("emin", C.c_double), ("emax", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        i32p = C.POINTER(C.c_int32)
+        L.orc_last_error.restype = C.c_char_p
+        sig = {
+            "orc_set_threads": [C.c_int],
+            "orc_fft": [C.c_int, i32p, dp, C.c_int],
+            "orc_bessel_j": [C.c_int, C.c_double, dp, ip],
+            "orc_grid": [C.c_int, i32p, dp, dp],
+            "orc_op_kinetic": [C.c_int, dp],
+            "orc_op_potential": [C.c_int, dp],
+            "orc_op_dipole": [C.c_int, dp, dp, C.c_int, C.c_double],
+            "orc_op_sum": [ip, C.c_int],
+            "orc_op_multistate": [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int],
+            "orc_op_bounds": [C.c_int, dp, dp],
+            "orc_op_apply": [C.c_int, C.c_int, dp, dp],
+            "orc_op_expec": [C.c_int, C.c_int, dp, C.c_int, dp],
+            "orc_overlap": [C.c_int, C.c_int, dp, dp, dp],
+            "orc_expec_position": [C.c_int, C.c_int, dp, dp],
+            "orc_expec_momentum": [C.c_int, C.c_int, dp, dp],
+            "orc_cheby_create": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                 C.c_double, C.POINTER(ChebyInfo)],
+            "orc_cheby_coeffs": [C.c_int, dp],
+            "orc_cheby_step": [C.c_int, C.c_int, dp, ip],
+            "orc_gspo_create": [ip, C.c_int, C.c_int, C.c_double],
+            "orc_gspo_step": [C.c_int, C.c_int, dp],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.orc_reset.restype = None
+        _lib = L
+    return _lib
+
+
+def _chk(rc: int) -> int:
+    if rc < 0:
+        raise OracleError(lib().orc_last_error().decode())
+    return rc
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(int(n))
+
+
+def reset() -> None:
+    lib().orc_reset()
+
+
+def fft(data: np.ndarray, dims, sign: int) -> np.ndarray:
+    """Unnormalised ND FFT over the QDng layout; dims = (N0, N1, ...)."""
+    x = _c128(data).copy()
+    N = (C.c_int32 * len(dims))(*dims)
+    _chk(lib().orc_fft(len(dims), N, _dp(x), int(sign)))
+    return x
+
+
+def bessel_j(n: int, x: float):
+    out = np.zeros(n)
+    nz = C.c_int(0)
+    _chk(lib().orc_bessel_j(int(n), float(x), _dp(out), C.byref(nz)))
+    return out, nz.value
+
+
+class Grid:
+    def __init__(self, N, xmin, xmax):
+        self.N = [int(v) for v in N]
+        self.xmin = [float(v) for v in xmin]
+        self.xmax = [float(v) for v in xmax]
+        nd = len(self.N)
+        self.id = _chk(lib().orc_grid(nd, (C.c_int32 * nd)(*self.N), (C.c_double * nd)(*self.xmin),
+                                      (C.c_double * nd)(*self.xmax)))
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.N))
+
+    def dx(self, d: int) -> float:
+        return (self.xmax[d] - self.xmin[d]) / (self.N[d] - 1)
+
+    def axis(self, d: int) -> np.ndarray:
+        return self.xmin[d] + self.dx(d) * np.arange(self.N[d])
+
+
+class Op:
+    def __init__(self, oid: int, nstates: int = 1, grid: Grid | None = None):
+        self.id = oid
+        self.nstates = nstates
+        self.grid = grid
+
+    def bounds(self):
+        a, b = C.c_double(), C.c_double()
+        _chk(lib().orc_op_bounds(self.id, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def apply(self, psi: np.ndarray, step: int = 0) -> np.ndarray:
+        src = _c128(psi)
+        dst = np.zeros_like(src)
+        _chk(lib().orc_op_apply(self.id, int(step), _dp(src), _dp(dst)))
+        return dst
+
+    def expec(self, psi: np.ndarray, step: int = 0) -> float:
+        src = _c128(psi)
+        out = C.c_double()
+        _chk(lib().orc_op_expec(self.id, int(step), _dp(src), self.nstates, C.byref(out)))
+        return out.value
+
+
+def kinetic(grid: Grid, mass) -> Op:
+    m = _f64(np.atleast_1d(mass))
+    return Op(_chk(lib().orc_op_kinetic(grid.id, _dp(m))), 1, grid)
+
+
+def potential(grid: Grid, V) -> Op:
+    v = _f64(V).ravel()
+    assert v.size == grid.size
+    return Op(_chk(lib().orc_op_potential(grid.id, _dp(v))), 1, grid)
+
+
+def dipole(grid: Grid, mu, laser, scale: float = 1.0) -> Op:
+    m = _f64(mu).ravel()
+    e = _f64(laser).ravel()
+    assert m.size == grid.size
+    return Op(_chk(lib().orc_op_dipole(grid.id, _dp(m), _dp(e), e.size, float(scale))), 1, grid)
+
+
+def op_sum(*ops: Op) -> Op:
+    ids = (C.c_int * len(ops))(*[o.id for o in ops])
+    return Op(_chk(lib().orc_op_sum(ids, len(ops))), ops[0].nstates, ops[0].grid)
+
+
+def multistate(nstates: int, elements: dict, unity: bool = True, dmulti: bool = False) -> Op:
+    """elements: {(row, col): Op}; off-diagonals are mirrored as in QDng."""
+    keys = list(elements.keys())
+    n = len(keys)
+    rows = (C.c_int * n)(*[k[0] for k in keys])
+    cols = (C.c_int * n)(*[k[1] for k in keys])
+    ids = (C.c_int * n)(*[elements[k].id for k in keys])
+    grid = next(iter(elements.values())).grid
+    return Op(_chk(lib().orc_op_multistate(int(nstates), rows, cols, ids, n, int(unity), int(dmulti))),
+              nstates, grid)
+
+
+def overlap(grid: Grid, a: np.ndarray, b: np.ndarray, nstates: int = 1) -> complex:
+    out = np.zeros(2)
+    _chk(lib().orc_overlap(grid.id, nstates, _dp(_c128(a)), _dp(_c128(b)), _dp(out)))
+    return complex(out[0], out[1])
+
+
+def norm2(grid: Grid, psi: np.ndarray, nstates: int = 1) -> float:
+    return overlap(grid, psi, psi, nstates).real
+
+
+def expec_position(grid: Grid, psi: np.ndarray, dim: int = 0) -> float:
+    out = C.c_double()
+    _chk(lib().orc_expec_position(grid.id, dim, _dp(_c128(psi)), C.byref(out)))
+    return out.value
+
+
+def expec_momentum(grid: Grid, psi: np.ndarray, dim: int = 0) -> float:
+    out = C.c_double()
+    _chk(lib().orc_expec_momentum(grid.id, dim, _dp(_c128(psi)), C.byref(out)))
+    return out.value
+
+
+class Cheby:
+    """Real-time Chebyshev propagator (QDng OCheby, forward)."""
+
+    def __init__(self, H: Op, dt: float, order: int = 0, prec: float = -1.0, scaling: float = 0.0):
+        info = ChebyInfo()
+        self.id = _chk(lib().orc_cheby_create(H.id, H.nstates, float(dt), int(order), float(prec),
+                                              float(scaling), C.byref(info)))
+        self.info = info
+        self.H = H
+        self.order = info.order
+        self.offset = info.offset
+        self.scaling = info.scaling
+        self.prec = info.prec
+
+    def coeffs(self) -> np.ndarray:
+        out = np.zeros(self.order)
+        _chk(lib().orc_cheby_coeffs(self.id, _dp(out)))
+        return out
+
+    def step(self, psi: np.ndarray, step: int = 0):
+        """Advance psi in place by one step; returns the number of H applications."""
+        assert psi.dtype == np.complex128 and psi.flags.c_contiguous
+        n = C.c_int(0)
+        _chk(lib().orc_cheby_step(self.id, int(step), _dp(psi), C.byref(n)))
+        return n.value
+
+
+class GSPO:
+    """General split-operator propagator (QDng OGSPO)."""
+
+    def __init__(self, ops, dt: float, nstates: int = 1):
+        ids = (C.c_int * len(ops))(*[o.id for o in ops])
+        self.id = _chk(lib().orc_gspo_create(ids, len(ops), int(nstates), float(dt)))
+
+    def step(self, psi: np.ndarray, step: int = 0) -> None:
+        assert psi.dtype == np.complex128 and psi.flags.c_contiguous
+        _chk(lib().orc_gspo_step(self.id, int(step), _dp(psi)))

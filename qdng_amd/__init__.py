@@ -1,0 +1,340 @@
+"""qdng_amd: MI355X grid-propagation path of QDng behind a C ABI (libqdx.so).
+
+This module is a thin ctypes binding of include/qdx.h.  Everything numeric
+runs in libqdx.so on the GPU; there is no CPU fallback: if the library is
+missing or no HIP device is present, the calls raise.
+
+Arrays follow QDng's layout: complex128, dimension 0 fastest, multistate
+vectors state-major.  In numpy terms a wave function on an (N0, N1, N2) grid
+with S states is an array of shape (S, N2, N1, N0) in C order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqdx.so")
+
+FP64 = 0
+FP32_BULK = 1
+_PREC = {"fp64": FP64, "fp32": FP32_BULK, "fp32_bulk": FP32_BULK, FP64: FP64, FP32_BULK: FP32_BULK}
+
+_lib = None
+
+
+class QdxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"qdx error {code}: {msg}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("ndims", C.c_int32), ("N", C.c_int32 * 4), ("xmin", C.c_double * 4),
+                ("xmax", C.c_double * 4)]
+
+
+class ChebyInfo(C.Structure):
+    _fields_ = [("order", C.c_int32), ("nz", C.c_int32), ("offset", C.c_double),
+                ("scaling", C.c_double), ("prec", C.c_double), ("emin", C.c_double),
+                ("emax", C.c_double)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("n_hpsi", C.c_int64), ("norm2", C.c_double), ("nconv", C.c_double)]
+
+
+EXPORTS = [
+    "qdx_last_error", "qdx_version", "qdx_ctx_create", "qdx_ctx_synchronize", "qdx_ctx_destroy",
+    "qdx_wf_create", "qdx_wf_upload", "qdx_wf_download", "qdx_wf_copy", "qdx_wf_norm",
+    "qdx_wf_overlap", "qdx_wf_normalize", "qdx_wf_destroy", "qdx_ham_create",
+    "qdx_ham_add_kinetic", "qdx_ham_add_potential", "qdx_ham_add_dipole", "qdx_ham_bounds",
+    "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
+    "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
+    "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy",
+]
+
+
+def lib():
+    """Load libqdx.so (built by qdng_amd/build.py); raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python qdng_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    i32 = C.c_int32
+    L.qdx_last_error.restype = C.c_char_p
+    L.qdx_version.restype = C.c_char_p
+    sig = {
+        "qdx_ctx_create": [C.c_int, C.POINTER(vp)],
+        "qdx_ctx_synchronize": [vp],
+        "qdx_ctx_destroy": [vp],
+        "qdx_wf_create": [vp, C.POINTER(Grid), i32, C.POINTER(vp)],
+        "qdx_wf_upload": [vp, i32, dp],
+        "qdx_wf_download": [vp, i32, dp],
+        "qdx_wf_copy": [vp, vp],
+        "qdx_wf_norm": [vp, dp, dp],
+        "qdx_wf_overlap": [vp, vp, dp],
+        "qdx_wf_normalize": [vp],
+        "qdx_wf_destroy": [vp],
+        "qdx_ham_create": [vp, C.POINTER(Grid), i32, i32, i32, C.POINTER(vp)],
+        "qdx_ham_add_kinetic": [vp, i32, dp],
+        "qdx_ham_add_potential": [vp, i32, i32, dp],
+        "qdx_ham_add_dipole": [vp, i32, i32, dp, dp, i32, C.c_double],
+        "qdx_ham_bounds": [vp, dp, dp],
+        "qdx_ham_apply": [vp, vp, vp, C.c_int64],
+        "qdx_ham_expec": [vp, vp, C.c_int64, dp],
+        "qdx_ham_destroy": [vp],
+        "qdx_bessel_j0": [i32, C.c_double, dp, C.POINTER(i32)],
+        "qdx_cheby_create": [vp, C.c_double, i32, C.c_double, C.c_double, i32, C.POINTER(vp)],
+        "qdx_cheby_info_get": [vp, C.POINTER(ChebyInfo)],
+        "qdx_cheby_params": [vp, C.c_double, i32, C.c_double, C.c_double, C.POINTER(ChebyInfo), dp, i32],
+        "qdx_cheby_coeffs": [vp, dp],
+        "qdx_gspo_create": [C.POINTER(vp), i32, C.c_double, i32, C.POINTER(vp)],
+        "qdx_prop_apply": [vp, vp, C.c_int64, C.POINTER(StepStats)],
+        "qdx_prop_run": [vp, vp, C.c_int64, C.c_int64, vp, dp, dp, C.POINTER(i32), C.POINTER(StepStats)],
+        "qdx_prop_destroy": [vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _chk(rc: int) -> None:
+    if rc != 0:
+        raise QdxError(rc, lib().qdx_last_error().decode())
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_grid(N, xmin, xmax) -> Grid:
+    g = Grid()
+    g.ndims = len(N)
+    for d in range(len(N)):
+        g.N[d] = int(N[d])
+        g.xmin[d] = float(xmin[d])
+        g.xmax[d] = float(xmax[d])
+    return g
+
+
+def bessel_j0(n: int, x: float):
+    out = np.zeros(n)
+    nz = C.c_int32(0)
+    _chk(lib().qdx_bessel_j0(int(n), float(x), _dp(out), C.byref(nz)))
+    return out, nz.value
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        _chk(lib().qdx_ctx_create(int(device), C.byref(self.h)))
+        self._destroy = lib().qdx_ctx_destroy
+
+    def synchronize(self):
+        _chk(lib().qdx_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            self._destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WaveFunction:
+    """WFGridCartesian / WFMultistate on the device (complex128)."""
+
+    def __init__(self, ctx: Context, N, xmin, xmax, nstates: int = 1):
+        self.ctx = ctx
+        self.N = [int(v) for v in N]
+        self.grid = make_grid(N, xmin, xmax)
+        self.S = int(nstates)
+        self.h = C.c_void_p()
+        _chk(lib().qdx_wf_create(ctx.h, C.byref(self.grid), self.S, C.byref(self.h)))
+        self._destroy = lib().qdx_wf_destroy
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.N))
+
+    def upload(self, psi: np.ndarray, state: int = -1) -> "WaveFunction":
+        a = np.ascontiguousarray(psi, dtype=np.complex128)
+        want = self.size * (self.S if state < 0 else 1)
+        if a.size != want:
+            raise ValueError(f"expected {want} complex values, got {a.size}")
+        _chk(lib().qdx_wf_upload(self.h, int(state), _dp(a)))
+        return self
+
+    def download(self, state: int = -1) -> np.ndarray:
+        out = np.zeros(self.size * (self.S if state < 0 else 1), dtype=np.complex128)
+        _chk(lib().qdx_wf_download(self.h, int(state), _dp(out)))
+        return out
+
+    def copy_from(self, other: "WaveFunction") -> None:
+        _chk(lib().qdx_wf_copy(self.h, other.h))
+
+    def norm(self):
+        n = C.c_double()
+        per = np.zeros(self.S)
+        _chk(lib().qdx_wf_norm(self.h, C.byref(n), _dp(per)))
+        return n.value, per
+
+    def overlap(self, other: "WaveFunction") -> complex:
+        out = np.zeros(2)
+        _chk(lib().qdx_wf_overlap(self.h, other.h, _dp(out)))
+        return complex(out[0], out[1])
+
+    def normalize(self) -> None:
+        _chk(lib().qdx_wf_normalize(self.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Hamiltonian:
+    """OSum (multistate=False) or OMultistate of grid terms."""
+
+    def __init__(self, ctx: Context | None, N, xmin, xmax, nstates: int = 1,
+                 multistate: bool = False, unity: bool = True):
+        self.ctx = ctx
+        self.N = [int(v) for v in N]
+        self.grid = make_grid(N, xmin, xmax)
+        self.S = int(nstates)
+        self.h = C.c_void_p()
+        _chk(lib().qdx_ham_create(ctx.h if ctx else None, C.byref(self.grid), self.S,
+                                  int(bool(multistate)), int(bool(unity)), C.byref(self.h)))
+        self._destroy = lib().qdx_ham_destroy
+
+    @property
+    def size(self) -> int:
+        return int(np.prod(self.N))
+
+    def add_kinetic(self, state: int, mass) -> "Hamiltonian":
+        m = np.ascontiguousarray(np.atleast_1d(mass), dtype=np.float64)
+        _chk(lib().qdx_ham_add_kinetic(self.h, int(state), _dp(m)))
+        return self
+
+    def add_potential(self, row: int, col: int, V) -> "Hamiltonian":
+        v = np.ascontiguousarray(V, dtype=np.float64).ravel()
+        if v.size != self.size:
+            raise ValueError("potential size does not match the grid")
+        _chk(lib().qdx_ham_add_potential(self.h, int(row), int(col), _dp(v)))
+        return self
+
+    def add_dipole(self, row: int, col: int, mu, laser, scale: float = 1.0) -> "Hamiltonian":
+        m = np.ascontiguousarray(mu, dtype=np.float64).ravel()
+        e = np.ascontiguousarray(laser, dtype=np.float64).ravel()
+        if m.size != self.size:
+            raise ValueError("dipole size does not match the grid")
+        _chk(lib().qdx_ham_add_dipole(self.h, int(row), int(col), _dp(m), _dp(e), e.size, float(scale)))
+        return self
+
+    def bounds(self):
+        a, b = C.c_double(), C.c_double()
+        _chk(lib().qdx_ham_bounds(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def apply(self, dst: WaveFunction, src: WaveFunction, step: int = 0) -> None:
+        _chk(lib().qdx_ham_apply(self.h, dst.h, src.h, int(step)))
+
+    def expec(self, psi: WaveFunction, step: int = 0) -> float:
+        out = C.c_double()
+        _chk(lib().qdx_ham_expec(self.h, psi.h, int(step), C.byref(out)))
+        return out.value
+
+    def cheby_params(self, dt: float, order: int = 0, prec: float = -1.0, scaling: float = 0.0):
+        info = ChebyInfo()
+        coeff = np.zeros(4096)
+        _chk(lib().qdx_cheby_params(self.h, float(dt), int(order), float(prec), float(scaling),
+                                    C.byref(info), _dp(coeff), coeff.size))
+        return info, coeff[:info.order].copy()
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class _Propagator:
+    h: C.c_void_p
+
+    def apply(self, psi: WaveFunction, step: int = 0) -> StepStats:
+        st = StepStats()
+        _chk(lib().qdx_prop_apply(self.h, psi.h, int(step), C.byref(st)))
+        return st
+
+    def run(self, psi: WaveFunction, first_step: int, nsteps: int, psi0: WaveFunction | None = None,
+            observables: bool = True):
+        """ProgPropa step loop on the device; returns (norm2, autocorr, n_hpsi, stats)."""
+        norm2 = np.zeros(max(nsteps, 1))
+        ac = np.zeros(2 * max(nsteps, 1))
+        nh = np.zeros(max(nsteps, 1), dtype=np.int32)
+        st = StepStats()
+        want_ac = observables and psi0 is not None
+        _chk(lib().qdx_prop_run(self.h, psi.h, int(first_step), int(nsteps),
+                                psi0.h if psi0 is not None else None,
+                                _dp(norm2) if observables else None,
+                                _dp(ac) if want_ac else None,
+                                nh.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(st)))
+        return (norm2[:nsteps], (ac[0::2] + 1j * ac[1::2])[:nsteps], nh[:nsteps], st)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Cheby(_Propagator):
+    """OCheby (real time, forward)."""
+
+    def __init__(self, H: Hamiltonian, dt: float, order: int = 0, prec: float = -1.0,
+                 scaling: float = 0.0, precision="fp64"):
+        self.H = H
+        self.h = C.c_void_p()
+        _chk(lib().qdx_cheby_create(H.h, float(dt), int(order), float(prec), float(scaling),
+                                    _PREC[precision], C.byref(self.h)))
+        self._destroy = lib().qdx_prop_destroy
+        self.info = ChebyInfo()
+        _chk(lib().qdx_cheby_info_get(self.h, C.byref(self.info)))
+        self.order = self.info.order
+
+    def coeffs(self) -> np.ndarray:
+        out = np.zeros(self.order)
+        _chk(lib().qdx_cheby_coeffs(self.h, _dp(out)))
+        return out
+
+
+class GSPO(_Propagator):
+    """OGSPO with split operators A, B, C, ... (each a Hamiltonian of one term kind)."""
+
+    def __init__(self, ops, dt: float, precision="fp64"):
+        self.ops = list(ops)
+        arr = (C.c_void_p * len(self.ops))(*[o.h.value for o in self.ops])
+        self.h = C.c_void_p()
+        _chk(lib().qdx_gspo_create(arr, len(self.ops), float(dt), _PREC[precision], C.byref(self.h)))
+        self._destroy = lib().qdx_prop_destroy

@@ -1,0 +1,1343 @@
+// Host runtime of libqdx.so: handles, QDng's operator rules (bounds,
+// multistate mirroring, Chebyshev order and coefficients, GSPO operator
+// sequence) and the step drivers that enqueue the gfx950 kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qdx.h"
+#include "qdx_impl.hpp"
+
+using namespace qdx;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct QdxError : std::runtime_error {
+    int code;
+    QdxError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const std::string &msg) { throw QdxError(code, msg); }
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) fail(QDX_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return QDX_OK;
+    } catch (const QdxError &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return QDX_ERR_MEMORY;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return QDX_ERR_DEVICE;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device memory owned by RAII holders
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) return;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(QDX_ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// geometry (GridSystem.cpp:207-211, Kspace.h:20-23)
+struct Grid {
+    int nd = 1;
+    int N[4] = {1, 1, 1, 1};
+    double xmin[4] = {0, 0, 0, 0}, xmax[4] = {0, 0, 0, 0};
+    long size() const {
+        long s = 1;
+        for (int d = 0; d < nd; ++d) s *= N[d];
+        return s;
+    }
+    double dx(int d) const { return (xmax[d] - xmin[d]) / double(N[d] - 1); }
+    double dk(int d) const { return (2 * M_PI) / (xmax[d] - xmin[d] + dx(d)); }
+    double weight() const {
+        double w = 1;
+        for (int d = 0; d < nd; ++d) w *= dx(d);
+        return w;
+    }
+    bool operator==(const Grid &o) const {
+        if (nd != o.nd) return false;
+        for (int d = 0; d < nd; ++d) {  // GridSystem.cpp:248-261, GRID_EPS 1e-10
+            if (N[d] != o.N[d]) return false;
+            if (std::fabs(xmin[d] - o.xmin[d]) > 1e-10 || std::fabs(xmax[d] - o.xmax[d]) > 1e-10) return false;
+        }
+        return true;
+    }
+    GridDev dev() const {
+        GridDev g;
+        g.nd = nd;
+        for (int d = 0; d < 3; ++d) g.N[d] = d < nd ? N[d] : 1;
+        g.size = size();
+        return g;
+    }
+};
+
+bool is_smooth(int n) {  // 2^a 3^b
+    if (n < 1) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+Grid make_grid(const qdx_grid *g) {
+    if (!g) fail(QDX_ERR_PARAM, "grid is NULL");
+    if (g->ndims < 1 || g->ndims > 3) fail(QDX_ERR_PARAM, "device path supports 1..3 grid dimensions");
+    Grid r;
+    r.nd = g->ndims;
+    for (int d = 0; d < r.nd; ++d) {
+        if (g->N[d] < 2) fail(QDX_ERR_PARAM, "Zero points grid defined");
+        if (g->xmax[d] - g->xmin[d] <= 0) fail(QDX_ERR_PARAM, "Zero length grid defined");
+        if (!is_smooth(g->N[d])) fail(QDX_ERR_PARAM, "grid sizes must be of the form 2^a 3^b");
+        if (g->N[d] > kMaxLine) fail(QDX_ERR_PARAM, "grid dimension longer than 4096 points");
+        r.N[d] = g->N[d];
+        r.xmin[d] = g->xmin[d];
+        r.xmax[d] = g->xmax[d];
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// radix plans and twiddle tables exp(-2 pi i q / L)
+std::vector<int> radix_plan(int L) {
+    std::vector<int> r;
+    int m = L;
+    while (m % 8 == 0) { r.push_back(8); m /= 8; }
+    if (m % 4 == 0) { r.push_back(4); m /= 4; }
+    if (m % 2 == 0) { r.push_back(2); m /= 2; }
+    while (m % 3 == 0) { r.push_back(3); m /= 3; }
+    if (m != 1) fail(QDX_ERR_PARAM, "line length is not 2^a 3^b");
+    if ((int)r.size() > kMaxStages) fail(QDX_ERR_PARAM, "too many radix stages");
+    return r;
+}
+
+}  // namespace
+
+struct qdx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int refs = 0;          // live wave functions, Hamiltonians and propagators
+    bool closing = false;  // qdx_ctx_destroy called while children were alive
+    std::map<int, std::unique_ptr<DevBuf>> tw64, tw32;
+    DevBuf partials;   // reduction scratch
+    DevBuf scal;       // small device result area
+    double *pinned = nullptr;
+
+    LinePlan plan(int L, bool fp32) {
+        auto &tab = fp32 ? tw32 : tw64;
+        auto it = tab.find(L);
+        if (it == tab.end()) {
+            std::vector<double> t(2 * L);
+            for (int q = 0; q < L; ++q) {
+                long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)L;
+                t[2 * q] = (double)cosl(a);
+                t[2 * q + 1] = (double)sinl(a);
+            }
+            auto buf = std::make_unique<DevBuf>(fp32 ? 2 * L * sizeof(float) : 2 * L * sizeof(double));
+            if (fp32) {
+                std::vector<float> f(t.begin(), t.end());
+                HIPCHK(hipMemcpy(buf->p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+            } else {
+                HIPCHK(hipMemcpy(buf->p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+            }
+            it = tab.emplace(L, std::move(buf)).first;
+        }
+        LinePlan p;
+        p.L = L;
+        auto r = radix_plan(L);
+        p.nst = (int)r.size();
+        for (int i = 0; i < p.nst; ++i) p.rad[i] = r[i];
+        p.tw = it->second->p;
+        return p;
+    }
+};
+
+namespace {
+void ctx_free(qdx_ctx *ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+void ctx_acquire(qdx_ctx *ctx) {
+    if (ctx) ctx->refs++;
+}
+void ctx_release(qdx_ctx *ctx) {
+    if (!ctx) return;
+    if (--ctx->refs == 0 && ctx->closing) ctx_free(ctx);
+}
+}  // namespace
+
+struct qdx_wf {
+    qdx_ctx *ctx = nullptr;
+    Grid grid;
+    int S = 1;
+    DevBuf d;   // complex128 [S][N]
+    double *ptr() const { return d.as<double>(); }
+    long n() const { return grid.size(); }
+};
+
+namespace {
+
+enum TermKind { T_KIN = 0, T_POT = 1, T_DIP = 2 };
+
+struct Term {
+    TermKind kind;
+    int row, col;
+    std::vector<double> mass;     // kinetic
+    std::vector<double> v;        // potential / dipole mu
+    std::vector<double> laser;    // dipole field
+    double scale = 1.0;
+};
+
+struct Elem {             // applied matrix element after QDng's mirroring
+    std::vector<int> terms;
+};
+
+void sym_eigen(int n, std::vector<double> a, std::vector<double> &ev) {
+    // cyclic Jacobi rotations; S <= 4 here
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += a[p * n + q] * a[p * n + q];
+        if (off < 1e-300) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = a[p * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                double theta = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = c * akp - s * akq;
+                    a[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = c * apk - s * aqk;
+                    a[q * n + k] = s * apk + c * aqk;
+                }
+            }
+    }
+    ev.resize(n);
+    for (int i = 0; i < n; ++i) ev[i] = a[i * n + i];
+}
+
+}  // namespace
+
+struct qdx_ham {
+    qdx_ctx *ctx = nullptr;
+    Grid grid;
+    int S = 1;
+    bool multistate = false, unity = true;
+    std::vector<Term> terms;
+    // finalised state
+    bool final_ = false;
+    bool hermitian = true;
+    std::vector<Elem> elem;        // S*S applied elements
+    std::vector<bool> given;       // S*S elements added by the caller
+    // device arrays (FP64 always, FP32 on demand)
+    struct DevSet {
+        std::vector<std::unique_ptr<DevBuf>> bufs;
+        HamDev<double> h64;
+        HamDev<float> h32;
+        bool have32 = false;
+    } dev;
+    DevBuf tk;                     // kinetic tables, FP64
+    DevBuf X64;                    // scratch for ham_apply
+    DevBuf tmp;                    // complex128 scratch for expec
+
+    void check_open() const {
+        if (final_) fail(QDX_ERR_STATE, "Hamiltonian is already in use; add terms before the first use");
+    }
+
+    // QDng OMultistate::Add / Init mirroring (OMultistate.cpp:34-53, 99-142)
+    void finalize() {
+        if (final_) return;
+        if (terms.empty()) fail(QDX_ERR_PARAM, multistate ? "Multistate operator is empty" : "Sum Operator is empty");
+        given.assign(S * S, false);
+        for (auto &t : terms) given[t.row * S + t.col] = true;
+        hermitian = true;
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j)
+                if (i != j && given[i * S + j] && given[j * S + i]) hermitian = false;
+        elem.assign(S * S, Elem());
+        for (int k = 0; k < (int)terms.size(); ++k) {
+            const Term &t = terms[k];
+            elem[t.row * S + t.col].terms.push_back(k);
+            if (hermitian && t.row != t.col) elem[t.col * S + t.row].terms.push_back(k);
+        }
+        final_ = true;
+        upload();
+    }
+
+    // bounds of one matrix element: the sum of its terms (OSum.cpp:28-51)
+    double term_emax(const Term &t) const {
+        switch (t.kind) {
+        case T_KIN: {  // OGridNablaSq.cpp:89-101
+            double T = 0;
+            for (int d = 0; d < grid.nd; ++d)
+                if (t.mass[d] > 0) T += 1 / (t.mass[d] * grid.dx(d) * grid.dx(d));
+            return T * (M_PI * M_PI / 2);
+        }
+        case T_POT: return *std::max_element(t.v.begin(), t.v.end());
+        case T_DIP:   // OGridDipole.cpp:36-39
+            return *std::max_element(t.v.begin(), t.v.end()) * *std::max_element(t.laser.begin(), t.laser.end());
+        }
+        return 0;
+    }
+    double term_emin(const Term &t) const {
+        if (t.kind == T_POT) return *std::min_element(t.v.begin(), t.v.end());
+        return 0;
+    }
+    void bounds(double &emin, double &emax) {
+        finalize();
+        if (!multistate) {
+            emin = emax = 0;
+            for (auto &t : terms) {
+                emax += term_emax(t);
+                emin += term_emin(t);
+            }
+            return;
+        }
+        // OMultistate.cpp:163-204: eigenvalues of |Emax_ij| and |Emin_ij|
+        std::vector<double> mx(S * S, 0.0), mn(S * S, 0.0), ev;
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) {
+                const Elem &e = elem[i * S + j];
+                if (e.terms.empty()) continue;
+                double a = 0, b = 0;
+                for (int k : e.terms) {
+                    a += term_emax(terms[k]);
+                    b += term_emin(terms[k]);
+                }
+                mx[i * S + j] = std::fabs(a);
+                mn[i * S + j] = std::fabs(b);
+            }
+        sym_eigen(S, mx, ev);
+        emax = *std::max_element(ev.begin(), ev.end());
+        sym_eigen(S, mn, ev);
+        emin = *std::min_element(ev.begin(), ev.end());
+    }
+
+    // upload the applied matrix: W_ij = sum of potentials, mu_ij one dipole,
+    // kinetic tables per state and dimension (Kspace.cpp:22-50)
+    std::vector<std::vector<double>> hostW, hostMu;
+    void upload() {
+        if (!ctx) return;
+        const long n = grid.size();
+        HamDev<double> &h = dev.h64;
+        h.S = S;
+        h.inv_n = 1.0 / double(n);
+        hostW.assign(S * S, {});
+        hostMu.assign(S * S, {});
+        std::vector<double> tkh;
+        std::vector<long> tk_off(S * 3, -1);
+        for (int i = 0; i < S; ++i) {
+            h.unity[i] = 0;
+            for (int j = 0; j < S; ++j) {
+                const Elem &e = elem[i * S + j];
+                std::vector<double> &W = hostW[i * S + j];
+                std::vector<double> &Mu = hostMu[i * S + j];
+                const Term *dip = nullptr;
+                std::vector<double> kin_sum;
+                for (int k : e.terms) {
+                    const Term &t = terms[k];
+                    if (t.kind == T_POT) {
+                        if (W.empty()) W.assign(n, 0.0);
+                        for (long x = 0; x < n; ++x) W[x] += t.v[x];
+                    } else if (t.kind == T_DIP) {
+                        if (dip && (dip->laser != t.laser || dip->scale != t.scale))
+                            fail(QDX_ERR_INCOMPAT, "two dipole terms with different fields on one element");
+                        if (Mu.empty()) Mu.assign(n, 0.0);
+                        for (long x = 0; x < n; ++x) Mu[x] += t.v[x];
+                        dip = &t;
+                    } else {
+                        if (i != j) fail(QDX_ERR_INCOMPAT, "kinetic operator on an off-diagonal element");
+                        // per-dimension tables, summed over kinetic terms of the state
+                        long base = (long)tkh.size();
+                        for (int d = 0; d < grid.nd; ++d) {
+                            const int Nx = grid.N[d];
+                            double dp = grid.dk(d);
+                            dp *= dp;
+                            dp *= -(-1. / 2. / t.mass[d]);
+                            std::vector<double> k1(Nx);
+                            if (Nx % 2 == 0) {
+                                for (int q = 0; q < Nx / 2; ++q) {
+                                    k1[q] = (double(q) * double(q)) * dp;
+                                    k1[Nx - q - 1] = (double(q + 1) * double(q + 1)) * dp;
+                                }
+                            } else {
+                                k1[0] = 0;
+                                for (int q = 1; q < (Nx + 1) / 2; ++q) {
+                                    k1[q] = (double(q) * double(q)) * dp;
+                                    k1[Nx - q] = (double(q) * double(q)) * dp;
+                                }
+                            }
+                            if (tk_off[i * 3 + d] < 0) {
+                                tk_off[i * 3 + d] = (long)tkh.size();
+                                tkh.insert(tkh.end(), k1.begin(), k1.end());
+                            } else {
+                                for (int q = 0; q < Nx; ++q) tkh[tk_off[i * 3 + d] + q] += k1[q];
+                            }
+                        }
+                        (void)base;
+                    }
+                }
+                if (dip) h.Edip[i][j] = 0;  // set per step
+                if (i == j && e.terms.empty() && multistate && unity) h.unity[i] = 1.0;
+            }
+        }
+        // device copies
+        tk.alloc(std::max<size_t>(tkh.size(), 1) * sizeof(double));
+        if (!tkh.empty()) HIPCHK(hipMemcpy(tk.p, tkh.data(), tkh.size() * sizeof(double), hipMemcpyHostToDevice));
+        for (int i = 0; i < S; ++i)
+            for (int d = 0; d < 3; ++d)
+                h.tk[i][d] = (d < grid.nd && tk_off[i * 3 + d] >= 0) ? tk.as<double>() + tk_off[i * 3 + d] : nullptr;
+        for (int i = 0; i < S; ++i) {
+            if (h.tk[i][0] == nullptr)
+                for (int d = 0; d < 3; ++d) h.tk[i][d] = nullptr;
+            for (int j = 0; j < S; ++j) {
+                h.W[i][j] = nullptr;
+                h.mu[i][j] = nullptr;
+                if (!hostW[i * S + j].empty()) h.W[i][j] = upload64(hostW[i * S + j]);
+                if (!hostMu[i * S + j].empty()) h.mu[i][j] = upload64(hostMu[i * S + j]);
+            }
+        }
+    }
+    std::map<const std::vector<double> *, const double *> up64_cache;
+    const double *upload64(const std::vector<double> &v) {
+        auto b = std::make_unique<DevBuf>(v.size() * sizeof(double));
+        HIPCHK(hipMemcpy(b->p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        const double *p = b->as<double>();
+        dev.bufs.push_back(std::move(b));
+        return p;
+    }
+    const float *upload32(const std::vector<double> &v) {
+        std::vector<float> f(v.begin(), v.end());
+        auto b = std::make_unique<DevBuf>(f.size() * sizeof(float));
+        HIPCHK(hipMemcpy(b->p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        const float *p = b->as<float>();
+        dev.bufs.push_back(std::move(b));
+        return p;
+    }
+    void ensure32() {
+        finalize();
+        if (dev.have32) return;
+        HamDev<float> &f = dev.h32;
+        const HamDev<double> &d = dev.h64;
+        f.S = d.S;
+        f.inv_n = d.inv_n;
+        for (int i = 0; i < S; ++i) {
+            f.unity[i] = d.unity[i];
+            for (int k = 0; k < 3; ++k) f.tk[i][k] = d.tk[i][k];
+            for (int j = 0; j < S; ++j) {
+                f.W[i][j] = hostW[i * S + j].empty() ? nullptr : upload32(hostW[i * S + j]);
+                f.mu[i][j] = hostMu[i * S + j].empty() ? nullptr : upload32(hostMu[i * S + j]);
+                f.Edip[i][j] = 0;
+            }
+        }
+        dev.have32 = true;
+    }
+
+    // -laser[step] * scale for each applied dipole element (Laser.cpp:125-128)
+    template <class R>
+    void set_fields(HamDev<R> &h, long step) const {
+        for (int i = 0; i < S; ++i)
+            for (int j = 0; j < S; ++j) {
+                h.Edip[i][j] = 0;
+                for (int k : elem[i * S + j].terms) {
+                    const Term &t = terms[k];
+                    if (t.kind != T_DIP) continue;
+                    if (step < 0 || step >= (long)t.laser.size())
+                        fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
+                    h.Edip[i][j] = -t.laser[step] * t.scale;
+                }
+            }
+    }
+    bool time_dependent() const {
+        for (auto &t : terms)
+            if (t.kind == T_DIP) return true;
+        return false;
+    }
+    long min_laser() const {
+        long m = -1;
+        for (auto &t : terms)
+            if (t.kind == T_DIP) m = (m < 0) ? (long)t.laser.size() : std::min(m, (long)t.laser.size());
+        return m;
+    }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Bessel J_k(x), k = 0..n-1 by Miller's backward recurrence in extended
+// precision, normalised with J_0 + 2 sum_{k>=1} J_{2k} = 1.  QDng obtains the
+// same values from SLATEC ZBESJ (Bessel.cpp:22-58); its underflow count NZ is
+// reproduced as the number of trailing values below exp(-ELIM),
+// ELIM = 2.303 (1021 log10 2 - 3) (zbesj.f:183-188).
+int bessel_j0(int n, double x, std::vector<double> &out) {
+    out.assign(n, 0.0);
+    if (n < 1) fail(QDX_ERR_PARAM, "Bessel: n must be >= 1");
+    if (x == 0) {
+        out[0] = 1.0;
+        return 0;
+    }
+    const long double ax = std::fabs((long double)x);
+    // start far enough above max(n, x) that J_m is negligible
+    int m = std::max(n, (int)ax) + 16;
+    while ((long double)m * std::log(2.0L * m / (2.718281828459045L * ax)) < 120.0L) m += 8;
+    if (m % 2) ++m;
+    std::vector<long double> j(m + 2, 0.0L);
+    j[m + 1] = 0.0L;
+    j[m] = 1e-300L;
+    for (int k = m; k >= 1; --k) j[k - 1] = (2.0L * k / ax) * j[k] - j[k + 1];
+    long double norm = j[0];
+    for (int k = 2; k <= m; k += 2) norm += 2.0L * j[k];
+    for (int k = 0; k < n; ++k) {
+        long double v = j[k] / norm;
+        if (x < 0 && (k & 1)) v = -v;
+        out[k] = (double)v;
+    }
+    const double elim = 2.303 * (1021.0 * 0.30102999566398120 - 3.0);
+    const double ulim = std::exp(-elim);
+    int nz = 0;
+    for (int k = n - 1; k >= 0 && std::fabs(out[k]) < ulim; --k) {
+        out[k] = 0.0;
+        ++nz;
+    }
+    return nz;
+}
+
+struct ChebyParams {
+    int order = 0, nz = 0;
+    double offset = 0, scaling = 1, prec = 1e-10, emin = 0, emax = 0;
+    std::vector<double> coeff;
+};
+
+// OCheby::Init (OCheby.cpp:382-493), real time, forward
+ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double prec, double scaling_param) {
+    if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
+    ChebyParams c;
+    h->bounds(c.emin, c.emax);
+    c.offset = (c.emax + c.emin) / 2;
+    c.scaling = std::fabs(c.emax - c.emin) / 2;
+    int order = order_param > 0 ? order_param : 0;
+    if (order <= 0) order = std::abs(int(20 * c.scaling * dt));
+    if (order < 10) order = 10;
+    if (scaling_param > 0) c.scaling = scaling_param;
+    if (order_param > 0) order = order_param;
+    if (order < 3) fail(QDX_ERR_PARAM, "Chebychev recursion must be at least 3");
+    std::vector<double> b;
+    int zeroes = bessel_j0(order, c.scaling * dt, b);
+    order -= zeroes;
+    if (order_param <= 0) {
+        int i = 2;
+        while (i < order && std::fabs(b[i - 1] - b[i - 2]) > 1e-32) i++;
+        order = i;
+    }
+    if (order > 0 && order <= int(c.scaling * dt)) fail(QDX_ERR_PARAM, "Chebychev recursion order is to small");
+    c.order = order;
+    c.nz = zeroes;
+    c.coeff.assign(order, 0.0);
+    c.coeff[0] = b[0];
+    for (int i = 1; i < order; ++i) c.coeff[i] = 2.0 * b[i];
+    if (prec >= 0) c.prec = prec;
+    return c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// propagators
+struct qdx_prop {
+    enum Kind { CHEBY, GSPO } kind;
+    qdx_ctx *ctx = nullptr;
+    int precision = QDX_FP64;
+    double dt = 0;
+    Grid grid;
+    int S = 1;
+    // Chebyshev
+    qdx_ham *ham = nullptr;
+    ChebyParams cp;
+    DevBuf phiA, phiB, X, ctl, partials, lastorders;
+    // GSPO
+    std::vector<qdx_ham *> ops;
+    DevBuf psi32, psi0_32;
+    DevBuf obs;   // device observables of qdx_prop_run
+
+    virtual ~qdx_prop() = default;
+};
+
+namespace {
+
+qdx_wf *check_wf(qdx_wf *w) {
+    if (!w || !w->ctx) fail(QDX_ERR_PARAM, "Invalid WaveFunction");
+    return w;
+}
+
+template <class R>
+void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
+    qdx_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const bool fp32 = std::is_same<R, float>::value;
+    const GridDev g = p->grid.dev();
+    qdx_ham *H = p->ham;
+    HamDev<R> h;
+    if constexpr (std::is_same<R, float>::value) h = H->dev.h32;
+    else h = H->dev.h64;
+    H->set_fields(h, step);
+    LinePlan plans[3];
+    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan(p->grid.N[d], fp32);
+    R *A = p->phiA.as<R>(), *B = p->phiB.as<R>();
+    R *X = p->grid.nd > 1 ? p->X.as<R>() : nullptr;
+    ChebyCtl *ctl = p->ctl.as<ChebyCtl>();
+    double *part = p->partials.as<double>();
+    const int S = p->S;
+    Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, X, ctl, part);
+
+    auto cols = [&]() {
+        if (p->grid.nd == 2) {
+            Launch<R>::col_pass(s, g, S, 1, 2, plans[1], h, 0.0, X, ctl);
+        } else if (p->grid.nd == 3) {
+            Launch<R>::col_pass(s, g, S, 1, 0, plans[1], h, 0.0, X, ctl);
+            Launch<R>::col_pass(s, g, S, 2, 2, plans[2], h, 0.0, X, ctl);
+            Launch<R>::col_pass(s, g, S, 1, 1, plans[1], h, 0.0, X, ctl);
+        }
+    };
+    const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
+    ChebyOrder ord;
+    ord.offset = p->cp.offset;
+    ord.inv_scaling = 1.0 / p->cp.scaling;
+    ord.prec = p->cp.prec;
+    const std::complex<double> c0 = p->cp.coeff[0] * ph, c1 = p->cp.coeff[1] * ph;
+    ord.k = 1;
+    ord.c0r = c0.real();
+    ord.c0i = c0.imag();
+    ord.ckr = c1.real();
+    ord.cki = c1.imag();
+    cols();
+    Launch<R>::cheby_row(s, g, h, plans[0], ord, A, B, X, psi->ptr(), ctl, part);
+    for (int k = 2; k < p->cp.order; ++k) {
+        const std::complex<double> ck = p->cp.coeff[k] * ph;
+        ord.k = k;
+        ord.ckr = ck.real();
+        ord.cki = ck.imag();
+        cols();
+        R *pm1 = (k - 1) % 2 == 0 ? A : B;
+        R *pm2 = k % 2 == 0 ? A : B;
+        Launch<R>::cheby_row(s, g, h, plans[0], ord, pm1, pm2, X, psi->ptr(), ctl, part);
+    }
+}
+
+// --- GSPO -------------------------------------------------------------------
+struct SplitOp {
+    bool kinetic = false;
+    qdx_ham *op = nullptr;
+    double tau = 0;
+};
+
+std::vector<SplitOp> gspo_sequence(qdx_prop *p) {
+    // OGSPO.cpp:119-138: ops n-1..1 with dt/2, then 0 with dt, then 1..n-1 with dt/2
+    std::vector<SplitOp> seq;
+    const int n = (int)p->ops.size();
+    auto mk = [&](int i, double sf) {
+        SplitOp o;
+        o.op = p->ops[i];
+        o.kinetic = o.op->terms[0].kind == T_KIN;
+        o.tau = p->dt * sf;
+        return o;
+    };
+    for (int i = n - 1; i > 0; --i) seq.push_back(mk(i, 0.5));
+    for (int i = 0; i < n; ++i) seq.push_back(mk(i, i == 0 ? 1.0 : 0.5));
+    return seq;
+}
+
+template <class R>
+XSplitDev<R> xsplit(qdx_ham *op, double tau, long step) {
+    XSplitDev<R> x;
+    x.S = op->S;
+    x.tau = tau;
+    x.coupled = 0;
+    HamDev<R> h;
+    if constexpr (std::is_same<R, float>::value) h = op->dev.h32;
+    else h = op->dev.h64;
+    op->set_fields(h, step);
+    for (int i = 0; i < op->S; ++i)
+        for (int j = 0; j < op->S; ++j) {
+            x.V[i][j] = h.W[i][j];
+            x.mu[i][j] = h.mu[i][j];
+            x.Edip[i][j] = h.Edip[i][j];
+            if (i != j && (h.W[i][j] || h.mu[i][j])) x.coupled = 1;
+        }
+    return x;
+}
+
+template <class R>
+void gspo_step(qdx_prop *p, R *psi, long step) {
+    qdx_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const bool fp32 = std::is_same<R, float>::value;
+    const GridDev g = p->grid.dev();
+    LinePlan plans[3];
+    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan(p->grid.N[d], fp32);
+    auto seq = gspo_sequence(p);
+    XGroupDev<R> pending;
+    pending.n = 0;
+    size_t i = 0;
+    while (i < seq.size()) {
+        const SplitOp &o = seq[i];
+        if (!o.kinetic) {
+            if (pending.n == kMaxXOps) {
+                Launch<R>::gspo_xonly(s, g, p->S, pending, psi);
+                pending.n = 0;
+            }
+            pending.op[pending.n++] = xsplit<R>(o.op, o.tau, step);
+            ++i;
+            continue;
+        }
+        // kinetic split operator: collect the x-space operators that follow it
+        XGroupDev<R> post;
+        post.n = 0;
+        size_t j = i + 1;
+        while (j < seq.size() && !seq[j].kinetic && post.n < kMaxXOps) {
+            post.op[post.n++] = xsplit<R>(seq[j].op, seq[j].tau, step);
+            ++j;
+        }
+        HamDev<R> kin;
+        if constexpr (std::is_same<R, float>::value) kin = o.op->dev.h32;
+        else kin = o.op->dev.h64;
+        if (p->grid.nd == 1) {
+            Launch<R>::gspo_1d(s, g, p->S, plans[0], kin, o.tau, pending, post, psi);
+        } else {
+            R *X = p->X.as<R>();
+            Launch<R>::gspo_row_pre(s, g, p->S, plans[0], pending, psi, X);
+            if (p->grid.nd == 2) {
+                Launch<R>::col_pass(s, g, p->S, 1, 3, plans[1], kin, o.tau, X, nullptr);
+            } else {
+                Launch<R>::col_pass(s, g, p->S, 1, 0, plans[1], kin, o.tau, X, nullptr);
+                Launch<R>::col_pass(s, g, p->S, 2, 3, plans[2], kin, o.tau, X, nullptr);
+                Launch<R>::col_pass(s, g, p->S, 1, 1, plans[1], kin, o.tau, X, nullptr);
+            }
+            Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi);
+        }
+        pending.n = 0;
+        i = j;
+    }
+    if (pending.n > 0) Launch<R>::gspo_xonly(s, g, p->S, pending, psi);
+}
+
+void prop_step(qdx_prop *p, qdx_wf *psi, long step) {
+    if (p->kind == qdx_prop::CHEBY) {
+        if (p->precision == QDX_FP32_BULK) cheby_step<float>(p, psi, step);
+        else cheby_step<double>(p, psi, step);
+        // record executed order count of this step
+    } else {
+        if (p->precision == QDX_FP32_BULK) gspo_step<float>(p, p->psi32.as<float>(), step);
+        else gspo_step<double>(p, psi->ptr(), step);
+    }
+}
+
+void check_prop_wf(qdx_prop *p, qdx_wf *psi) {
+    check_wf(psi);
+    if (psi->ctx != p->ctx) fail(QDX_ERR_INCOMPAT, "wave function belongs to another context");
+    if (!(psi->grid == p->grid) || psi->S != p->S)
+        fail(QDX_ERR_INCOMPAT, "GridPotential parameters are not compatible with wave function");
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+const char *qdx_last_error(void) { return g_err.c_str(); }
+const char *qdx_version(void) { return "qdx 0.1 (gfx950)"; }
+
+int qdx_ctx_create(int device, qdx_ctx **out) {
+    return guarded([&] {
+        if (!out) fail(QDX_ERR_PARAM, "out is NULL");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) fail(QDX_ERR_DEVICE, "no HIP device available");
+        if (device < 0 || device >= n) fail(QDX_ERR_PARAM, "device index out of range");
+        HIPCHK(hipSetDevice(device));
+        auto c = std::make_unique<qdx_ctx>();
+        c->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->partials.alloc(kMaxPartials * sizeof(double));
+        c->scal.alloc(4096 * sizeof(double));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 4096 * sizeof(double), hipHostMallocDefault));
+        *out = c.release();
+    });
+}
+
+int qdx_ctx_synchronize(qdx_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) fail(QDX_ERR_PARAM, "ctx is NULL");
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int qdx_ctx_destroy(qdx_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        // children keep the context alive until the last one is destroyed
+        ctx->closing = true;
+        if (ctx->refs == 0) ctx_free(ctx);
+    });
+}
+
+int qdx_wf_create(qdx_ctx *ctx, const qdx_grid *grid, int32_t nstates, qdx_wf **out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(QDX_ERR_PARAM, "NULL argument");
+        if (nstates < 1 || nstates > kMaxStates) fail(QDX_ERR_PARAM, "number of states must be 1..4");
+        auto w = std::make_unique<qdx_wf>();
+        w->ctx = ctx;
+        w->grid = make_grid(grid);
+        w->S = nstates;
+        w->d.alloc((size_t)nstates * w->grid.size() * 2 * sizeof(double));
+        HIPCHK(hipMemsetAsync(w->d.p, 0, w->d.bytes, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ctx_acquire(ctx);
+        *out = w.release();
+    });
+}
+
+int qdx_wf_upload(qdx_wf *wf, int32_t state, const double *c128) {
+    return guarded([&] {
+        check_wf(wf);
+        if (!c128) fail(QDX_ERR_PARAM, "NULL array");
+        const size_t one = wf->n() * 2 * sizeof(double);
+        if (state >= wf->S) fail(QDX_ERR_PARAM, "state index out of range");
+        double *dst = state < 0 ? wf->ptr() : wf->ptr() + (size_t)state * wf->n() * 2;
+        HIPCHK(hipMemcpyAsync(dst, c128, state < 0 ? one * wf->S : one, hipMemcpyHostToDevice, wf->ctx->stream));
+        HIPCHK(hipStreamSynchronize(wf->ctx->stream));
+    });
+}
+
+int qdx_wf_download(qdx_wf *wf, int32_t state, double *c128) {
+    return guarded([&] {
+        check_wf(wf);
+        if (!c128) fail(QDX_ERR_PARAM, "NULL array");
+        const size_t one = wf->n() * 2 * sizeof(double);
+        if (state >= wf->S) fail(QDX_ERR_PARAM, "state index out of range");
+        const double *src = state < 0 ? wf->ptr() : wf->ptr() + (size_t)state * wf->n() * 2;
+        HIPCHK(hipMemcpyAsync(c128, src, state < 0 ? one * wf->S : one, hipMemcpyDeviceToHost, wf->ctx->stream));
+        HIPCHK(hipStreamSynchronize(wf->ctx->stream));
+    });
+}
+
+int qdx_wf_copy(qdx_wf *dst, const qdx_wf *src) {
+    return guarded([&] {
+        check_wf(dst);
+        if (!src || !(src->grid == dst->grid) || src->S != dst->S)
+            fail(QDX_ERR_INCOMPAT, "wave functions differ in grid or states");
+        HIPCHK(hipMemcpyAsync(dst->d.p, src->d.p, dst->d.bytes, hipMemcpyDeviceToDevice, dst->ctx->stream));
+        HIPCHK(hipStreamSynchronize(dst->ctx->stream));
+    });
+}
+
+static void overlap_states(qdx_wf *a, qdx_wf *b, std::vector<double> &per) {
+    qdx_ctx *c = a->ctx;
+    launch_overlap(c->stream, a->ptr(), b->ptr(), a->n(), a->S, c->partials.as<double>(), c->scal.as<double>());
+    HIPCHK(hipMemcpyAsync(c->pinned, c->scal.p, 2 * a->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    per.assign(c->pinned, c->pinned + 2 * a->S);
+}
+
+int qdx_wf_norm(qdx_wf *wf, double *norm, double *per_state_pop) {
+    return guarded([&] {
+        check_wf(wf);
+        std::vector<double> per;
+        overlap_states(wf, wf, per);
+        const double w = wf->grid.weight();
+        double tot = 0;
+        for (int s = 0; s < wf->S; ++s) {   // WFMultistate.cpp:158-174
+            const double p = per[2 * s] * w;
+            if (per_state_pop) per_state_pop[s] = p;
+            tot += p;
+        }
+        if (norm) *norm = std::sqrt(tot);
+        if (!std::isfinite(tot)) fail(QDX_ERR_OVERFLOW, "Norm is not a number");
+    });
+}
+
+int qdx_wf_overlap(qdx_wf *a, qdx_wf *b, double out[2]) {
+    return guarded([&] {
+        check_wf(a);
+        check_wf(b);
+        if (!(a->grid == b->grid) || a->S != b->S) fail(QDX_ERR_INCOMPAT, "wave functions differ in grid or states");
+        std::vector<double> per;
+        overlap_states(a, b, per);
+        const double w = a->grid.weight();
+        double re = 0, im = 0;
+        for (int s = 0; s < a->S; ++s) {
+            re += per[2 * s] * w;
+            im += per[2 * s + 1] * w;
+        }
+        out[0] = re;
+        out[1] = im;
+    });
+}
+
+int qdx_wf_normalize(qdx_wf *wf) {
+    return guarded([&] {
+        check_wf(wf);
+        std::vector<double> per;
+        overlap_states(wf, wf, per);
+        double tot = 0;
+        for (int s = 0; s < wf->S; ++s) tot += per[2 * s] * wf->grid.weight();
+        const double f = 1.0 / std::sqrt(tot);
+        // host round trip is fine here: normalisation happens once per job
+        std::vector<double> h((size_t)wf->S * wf->n() * 2);
+        HIPCHK(hipMemcpy(h.data(), wf->ptr(), h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (auto &v : h) v *= f;
+        HIPCHK(hipMemcpy(wf->ptr(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    });
+}
+
+int qdx_wf_destroy(qdx_wf *wf) {
+    return guarded([&] {
+        if (!wf) return;
+        qdx_ctx *c = wf->ctx;
+        (void)hipStreamSynchronize(c->stream);
+        delete wf;
+        ctx_release(c);
+    });
+}
+
+int qdx_ham_create(qdx_ctx *ctx, const qdx_grid *grid, int32_t nstates, int32_t multistate, int32_t unity,
+                   qdx_ham **out) {
+    return guarded([&] {
+        if (!out) fail(QDX_ERR_PARAM, "out is NULL");
+        if (nstates < 1 || nstates > kMaxStates) fail(QDX_ERR_PARAM, "number of states must be 1..4");
+        if (!multistate && nstates != 1) fail(QDX_ERR_PARAM, "a Sum Hamiltonian acts on one state");
+        auto h = std::make_unique<qdx_ham>();
+        h->ctx = ctx;
+        h->grid = make_grid(grid);
+        h->S = nstates;
+        h->multistate = multistate != 0;
+        h->unity = unity != 0;
+        ctx_acquire(ctx);
+        *out = h.release();
+    });
+}
+
+static void check_elem(qdx_ham *h, int row, int col) {
+    if (!h) fail(QDX_ERR_PARAM, "Hamiltonian is NULL");
+    h->check_open();
+    if (row < 0 || col < 0 || row >= h->S || col >= h->S) fail(QDX_ERR_OVERFLOW, "Multistate index out of range");
+}
+
+int qdx_ham_add_kinetic(qdx_ham *h, int32_t state, const double *mass) {
+    return guarded([&] {
+        check_elem(h, state, state);
+        if (!mass) fail(QDX_ERR_PARAM, "mass is NULL");
+        Term t;
+        t.kind = T_KIN;
+        t.row = t.col = state;
+        bool eff = false;
+        for (int d = 0; d < h->grid.nd; ++d) {   // OGridNablaSq.cpp:46-55
+            if (mass[d] == 0) fail(QDX_ERR_PARAM, "Zero mass defined");
+            if (mass[d] > 0) eff = true;
+            t.mass.push_back(mass[d]);
+        }
+        if (!eff) fail(QDX_ERR_PARAM, "Nabla^2 Operator is empty (no masses defined)");
+        h->terms.push_back(t);
+    });
+}
+
+int qdx_ham_add_potential(qdx_ham *h, int32_t row, int32_t col, const double *V) {
+    return guarded([&] {
+        check_elem(h, row, col);
+        if (!V) fail(QDX_ERR_PARAM, "V is NULL");
+        Term t;
+        t.kind = T_POT;
+        t.row = row;
+        t.col = col;
+        t.v.assign(V, V + h->grid.size());
+        for (double x : t.v)
+            if (!std::isfinite(x)) fail(QDX_ERR_PARAM, "potential holds non-finite values");
+        h->terms.push_back(t);
+    });
+}
+
+int qdx_ham_add_dipole(qdx_ham *h, int32_t row, int32_t col, const double *mu, const double *laser, int32_t nt,
+                       double scale) {
+    return guarded([&] {
+        check_elem(h, row, col);
+        if (!mu || !laser) fail(QDX_ERR_PARAM, "NULL array");
+        if (nt < 1) fail(QDX_ERR_PARAM, "Laser field has zero size");
+        Term t;
+        t.kind = T_DIP;
+        t.row = row;
+        t.col = col;
+        t.v.assign(mu, mu + h->grid.size());
+        t.laser.assign(laser, laser + nt);
+        t.scale = scale;
+        h->terms.push_back(t);
+    });
+}
+
+int qdx_ham_bounds(qdx_ham *h, double *emin, double *emax) {
+    return guarded([&] {
+        if (!h || !emin || !emax) fail(QDX_ERR_PARAM, "NULL argument");
+        h->bounds(*emin, *emax);
+    });
+}
+
+static void ham_apply_impl(qdx_ham *h, qdx_wf *dst, qdx_wf *src, long step) {
+    if (!h || !h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
+    check_wf(dst);
+    check_wf(src);
+    if (!(src->grid == h->grid) || src->S != h->S || !(dst->grid == h->grid) || dst->S != h->S)
+        fail(QDX_ERR_INCOMPAT, "wave function does not match the Hamiltonian");
+    if (dst == src) fail(QDX_ERR_PARAM, "in-place apply is not supported");
+    h->finalize();
+    HamDev<double> hd = h->dev.h64;
+    h->set_fields(hd, step);
+    LinePlan plans[3];
+    for (int d = 0; d < h->grid.nd; ++d) plans[d] = h->ctx->plan(h->grid.N[d], false);
+    if (h->grid.nd > 1 && !h->X64.p) h->X64.alloc((size_t)h->S * h->grid.size() * 2 * sizeof(double));
+    Launch<double>::ham_apply(h->ctx->stream, h->grid.dev(), hd, plans, src->ptr(), dst->ptr(), h->X64.as<double>());
+}
+
+int qdx_ham_apply(qdx_ham *h, qdx_wf *dst, qdx_wf *src, int64_t step) {
+    return guarded([&] {
+        ham_apply_impl(h, dst, src, step);
+        HIPCHK(hipStreamSynchronize(h->ctx->stream));
+    });
+}
+
+int qdx_ham_expec(qdx_ham *h, qdx_wf *psi, int64_t step, double *out) {
+    return guarded([&] {
+        if (!out) fail(QDX_ERR_PARAM, "out is NULL");
+        check_wf(psi);
+        qdx_wf tmp;
+        tmp.ctx = psi->ctx;
+        tmp.grid = psi->grid;
+        tmp.S = psi->S;
+        tmp.d.alloc(psi->d.bytes);
+        ham_apply_impl(h, &tmp, psi, step);
+        std::vector<double> per;
+        overlap_states(psi, &tmp, per);
+        double re = 0;
+        for (int s = 0; s < psi->S; ++s) re += per[2 * s];
+        *out = re * psi->grid.weight();   // Operator.h:226-252
+    });
+}
+
+int qdx_ham_destroy(qdx_ham *h) {
+    return guarded([&] {
+        if (!h) return;
+        qdx_ctx *c = h->ctx;
+        if (c) (void)hipStreamSynchronize(c->stream);
+        delete h;
+        ctx_release(c);
+    });
+}
+
+int qdx_bessel_j0(int32_t n, double x, double *out, int32_t *nz) {
+    return guarded([&] {
+        if (!out) fail(QDX_ERR_PARAM, "out is NULL");
+        std::vector<double> b;
+        int z = bessel_j0(n, x, b);
+        std::copy(b.begin(), b.end(), out);
+        if (nz) *nz = z;
+    });
+}
+
+int qdx_cheby_params(qdx_ham *h, double dt, int32_t order, double prec, double scaling, qdx_cheby_info *info,
+                     double *coeff, int32_t coeff_len) {
+    return guarded([&] {
+        if (!h || !info) fail(QDX_ERR_PARAM, "NULL argument");
+        ChebyParams c = cheby_params(h, dt, order, prec, scaling);
+        info->order = c.order;
+        info->nz = c.nz;
+        info->offset = c.offset;
+        info->scaling = c.scaling;
+        info->prec = c.prec;
+        info->emin = c.emin;
+        info->emax = c.emax;
+        if (coeff)
+            for (int i = 0; i < std::min<int>(coeff_len, c.order); ++i) coeff[i] = c.coeff[i];
+    });
+}
+
+int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double scaling, int32_t precision,
+                     qdx_prop **out) {
+    return guarded([&] {
+        if (!h || !out) fail(QDX_ERR_PARAM, "NULL argument");
+        if (!h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        auto p = std::make_unique<qdx_prop>();
+        p->kind = qdx_prop::CHEBY;
+        p->ctx = h->ctx;
+        p->precision = precision;
+        p->dt = dt;
+        p->grid = h->grid;
+        p->S = h->S;
+        p->ham = h;
+        p->cp = cheby_params(h, dt, order, prec, scaling);
+        if (precision == QDX_FP32_BULK) h->ensure32();
+        const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
+        const size_t nbytes = (size_t)p->S * p->grid.size() * es;
+        p->phiA.alloc(nbytes);
+        p->phiB.alloc(nbytes);
+        if (p->grid.nd > 1) p->X.alloc(nbytes);
+        p->ctl.alloc(sizeof(ChebyCtl));
+        HIPCHK(hipMemset(p->ctl.p, 0, sizeof(ChebyCtl)));
+        p->partials.alloc(kMaxPartials * sizeof(double));
+        HIPCHK(hipDeviceSynchronize());
+        ctx_acquire(p->ctx);
+        *out = p.release();
+    });
+}
+
+int qdx_cheby_info_get(qdx_prop *p, qdx_cheby_info *info) {
+    return guarded([&] {
+        if (!p || !info || p->kind != qdx_prop::CHEBY) fail(QDX_ERR_PARAM, "not a Chebyshev propagator");
+        info->order = p->cp.order;
+        info->nz = p->cp.nz;
+        info->offset = p->cp.offset;
+        info->scaling = p->cp.scaling;
+        info->prec = p->cp.prec;
+        info->emin = p->cp.emin;
+        info->emax = p->cp.emax;
+    });
+}
+
+int qdx_cheby_coeffs(qdx_prop *p, double *coeff) {
+    return guarded([&] {
+        if (!p || !coeff || p->kind != qdx_prop::CHEBY) fail(QDX_ERR_PARAM, "not a Chebyshev propagator");
+        std::copy(p->cp.coeff.begin(), p->cp.coeff.end(), coeff);
+    });
+}
+
+int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precision, qdx_prop **out) {
+    return guarded([&] {
+        if (!ops || !out) fail(QDX_ERR_PARAM, "NULL argument");
+        if (nops < 1) fail(QDX_ERR_PARAM, "GSPO is empty!");
+        if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        auto p = std::make_unique<qdx_prop>();
+        p->kind = qdx_prop::GSPO;
+        p->precision = precision;
+        p->dt = dt;
+        for (int i = 0; i < nops; ++i) {
+            qdx_ham *o = ops[i];
+            if (!o || !o->ctx) fail(QDX_ERR_PARAM, "GSPO misses an operator. Sequence must be A, B, C, ...");
+            o->finalize();
+            if (i == 0) {
+                p->ctx = o->ctx;
+                p->grid = o->grid;
+                p->S = o->S;
+            } else if (!(o->grid == p->grid) || o->S != p->S || o->ctx != p->ctx) {
+                fail(QDX_ERR_INCOMPAT, "GSPO operators differ in grid or states");
+            }
+            // ODSpace requirement: one operator kind per split operator
+            const TermKind k0 = o->terms[0].kind;
+            for (auto &t : o->terms) {
+                if ((t.kind == T_KIN) != (k0 == T_KIN))
+                    fail(QDX_ERR_INCOMPAT, "GSPO: Operator has no diagonal representation");
+            }
+            bool coupled = false, has_dip = false;
+            for (auto &t : o->terms) {
+                if (t.row != t.col) coupled = true;
+                if (t.kind == T_DIP) has_dip = true;
+            }
+            if (o->multistate && has_dip)
+                fail(QDX_ERR_INCOMPAT, "GSPO: multistate dipole split operators are not supported");
+            if (coupled && o->S > 2) fail(QDX_ERR_INCOMPAT, "GSPO: coupled split operators need S <= 2");
+            if (precision == QDX_FP32_BULK) o->ensure32();
+            p->ops.push_back(o);
+        }
+        const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
+        const size_t nbytes = (size_t)p->S * p->grid.size() * es;
+        if (p->grid.nd > 1) p->X.alloc(nbytes);
+        if (precision == QDX_FP32_BULK) p->psi32.alloc(nbytes);
+        p->partials.alloc(kMaxPartials * sizeof(double));
+        ctx_acquire(p->ctx);
+        *out = p.release();
+    });
+}
+
+int qdx_prop_apply(qdx_prop *p, qdx_wf *psi, int64_t step, qdx_step_stats *stats) {
+    return guarded([&] {
+        if (!p) fail(QDX_ERR_PARAM, "propagator is NULL");
+        check_prop_wf(p, psi);
+        hipStream_t s = p->ctx->stream;
+        if (p->kind == qdx_prop::GSPO && p->precision == QDX_FP32_BULK)
+            launch_convert_c128_to_c64(s, psi->ptr(), p->psi32.as<float>(), (long)p->S * psi->n());
+        prop_step(p, psi, step);
+        if (p->kind == qdx_prop::GSPO && p->precision == QDX_FP32_BULK)
+            launch_convert_c64_to_c128(s, p->psi32.as<float>(), psi->ptr(), (long)p->S * psi->n());
+        if (stats) {
+            if (p->kind == qdx_prop::CHEBY) {
+                ChebyCtl c;
+                HIPCHK(hipMemcpyAsync(&c, p->ctl.p, sizeof(c), hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                stats->n_hpsi = c.last_order;
+                stats->nconv = std::fabs(c.norm0 - c.accnorm) / c.norm0;
+            } else {
+                stats->n_hpsi = 1;
+                stats->nconv = 0;
+            }
+            std::vector<double> per;
+            overlap_states(psi, psi, per);
+            double tot = 0;
+            for (int k = 0; k < psi->S; ++k) tot += per[2 * k];
+            stats->norm2 = tot * psi->grid.weight();
+        } else {
+            HIPCHK(hipStreamSynchronize(s));
+        }
+    });
+}
+
+int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, qdx_wf *psi0, double *norm2,
+                 double *autocorr, int32_t *n_hpsi_per_step, qdx_step_stats *stats) {
+    return guarded([&] {
+        if (!p) fail(QDX_ERR_PARAM, "propagator is NULL");
+        check_prop_wf(p, psi);
+        if (nsteps < 0) fail(QDX_ERR_PARAM, "negative step count");
+        if (autocorr && !psi0) fail(QDX_ERR_PARAM, "autocorrelation needs psi0");
+        if (psi0) {
+            check_wf(psi0);
+            if (!(psi0->grid == psi->grid) || psi0->S != psi->S) fail(QDX_ERR_INCOMPAT, "psi0 does not match psi");
+        }
+        hipStream_t s = p->ctx->stream;
+        const int S = p->S;
+        const long n = psi->n();
+        const bool g32 = p->kind == qdx_prop::GSPO && p->precision == QDX_FP32_BULK;
+        // device observables: per step, per state (re, im) of <psi|psi> and <psi|psi0>
+        const size_t per_step = 4 * S;
+        p->obs.alloc(std::max<size_t>(1, (size_t)nsteps * per_step + nsteps) * sizeof(double));
+        double *obs = p->obs.as<double>();
+        int *orders = reinterpret_cast<int *>(obs + nsteps * per_step);
+        if (g32) {
+            launch_convert_c128_to_c64(s, psi->ptr(), p->psi32.as<float>(), (long)S * n);
+            if (autocorr) {
+                p->psi0_32.alloc((size_t)S * n * 2 * sizeof(float));
+                launch_convert_c128_to_c64(s, psi0->ptr(), p->psi0_32.as<float>(), (long)S * n);
+            }
+        }
+        double *part = p->partials.as<double>();
+        for (int64_t i = 0; i < nsteps; ++i) {
+            double *o = obs + i * per_step;
+            if (norm2) {
+                if (g32) launch_overlap_c64(s, p->psi32.as<float>(), p->psi32.as<float>(), n, S, part, o);
+                else launch_overlap(s, psi->ptr(), psi->ptr(), n, S, part, o);
+            }
+            if (autocorr) {
+                if (g32) launch_overlap_c64(s, p->psi32.as<float>(), p->psi0_32.as<float>(), n, S, part, o + 2 * S);
+                else launch_overlap(s, psi->ptr(), psi0->ptr(), n, S, part, o + 2 * S);
+            }
+            prop_step(p, psi, first_step + i);
+            if (p->kind == qdx_prop::CHEBY)
+                HIPCHK(hipMemcpyAsync(orders + i, &p->ctl.as<ChebyCtl>()->last_order, sizeof(int),
+                                      hipMemcpyDeviceToDevice, s));
+        }
+        if (g32) launch_convert_c64_to_c128(s, p->psi32.as<float>(), psi->ptr(), (long)S * n);
+        std::vector<double> host((size_t)nsteps * per_step + nsteps);
+        if (nsteps > 0)
+            HIPCHK(hipMemcpyAsync(host.data(), obs, host.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const double w = psi->grid.weight();
+        int64_t total = 0;
+        const int *hord = reinterpret_cast<const int *>(host.data() + nsteps * per_step);
+        for (int64_t i = 0; i < nsteps; ++i) {
+            const double *o = host.data() + i * per_step;
+            if (norm2) {
+                double t = 0;
+                for (int k = 0; k < S; ++k) t += o[2 * k];
+                norm2[i] = t * w;
+                if (!std::isfinite(norm2[i])) fail(QDX_ERR_OVERFLOW, "Norm is not a number");
+            }
+            if (autocorr) {
+                double re = 0, im = 0;
+                for (int k = 0; k < S; ++k) {
+                    re += o[2 * S + 2 * k];
+                    im += o[2 * S + 2 * k + 1];
+                }
+                autocorr[2 * i] = re * w;
+                autocorr[2 * i + 1] = im * w;
+            }
+            const int cnt = p->kind == qdx_prop::CHEBY ? hord[i] : 1;
+            if (n_hpsi_per_step) n_hpsi_per_step[i] = cnt;
+            total += cnt;
+        }
+        if (stats) {
+            stats->n_hpsi = total;
+            stats->nconv = 0;
+            if (p->kind == qdx_prop::CHEBY) {
+                ChebyCtl c;
+                HIPCHK(hipMemcpy(&c, p->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
+                stats->nconv = std::fabs(c.norm0 - c.accnorm) / c.norm0;
+            }
+            std::vector<double> per;
+            overlap_states(psi, psi, per);
+            double t = 0;
+            for (int k = 0; k < S; ++k) t += per[2 * k];
+            stats->norm2 = t * w;
+        }
+    });
+}
+
+int qdx_prop_destroy(qdx_prop *p) {
+    return guarded([&] {
+        if (!p) return;
+        qdx_ctx *c = p->ctx;
+        if (c) (void)hipStreamSynchronize(c->stream);
+        delete p;
+        ctx_release(c);
+    });
+}
+
+}  // extern "C"

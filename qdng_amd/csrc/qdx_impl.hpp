@@ -1,0 +1,141 @@
+// Internal declarations shared by the host runtime (qdx_host.cpp) and the
+// gfx950 kernels (qdx_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace qdx {
+
+constexpr int kMaxStates = 4;      // states handled by the fused kernels
+constexpr int kMaxStages = 16;     // radix stages of one line transform
+constexpr int kMaxLine = 4096;     // longest line transform held in LDS
+
+// ---------------------------------------------------------------------------
+// Radix plan of one line length L = 2^a 3^b: radix-8 stages first, then one
+// radix-4 or radix-2 stage, then radix-3 stages.  `tw` points to the device
+// table exp(-2 pi i q / L), q in [0, L), in the precision of the kernel.
+struct LinePlan {
+    int L = 0;
+    int nst = 0;
+    int rad[kMaxStages] = {};
+    const void *tw = nullptr;
+};
+
+// Grid as the kernels see it: N[d] points in dimension d, dimension 0
+// contiguous.  Unused dimensions have N = 1.
+struct GridDev {
+    int nd = 1;
+    int N[3] = {1, 1, 1};
+    long size = 1;
+};
+
+// Hamiltonian H = T + W(x) - E(t) mu(x) as the fused kernels apply it to a
+// multistate vector phi[S][N] (OSum / OMultistate of grid terms):
+//   (H phi)_i = T_i phi_i + sum_j (W_ij(x) + unity_i delta_ij) phi_j
+//               + sum_j (-E_ij mu_ij(x)) phi_j
+// tk[i][d] is the per-dimension kinetic table of state i (nullptr: no kinetic
+// term), its value at k summed over dimensions and multiplied by inv_n is
+// T_i(k)/N as in Diff::DnDxn.  W[i][j] / mu[i][j] are device arrays of the
+// kernel precision or nullptr.  Edip[i][j] = -laser[step] * scale.
+template <class R>
+struct HamDev {
+    int S = 1;
+    double inv_n = 1.0;
+    const double *tk[kMaxStates][3] = {};
+    const R *W[kMaxStates][kMaxStates] = {};
+    const R *mu[kMaxStates][kMaxStates] = {};
+    double Edip[kMaxStates][kMaxStates] = {};
+    double unity[kMaxStates] = {};
+};
+
+// Per-propagator device control block for the Chebyshev series.
+struct ChebyCtl {
+    double norm0;        // raw_norm(psi) at the start of the step
+    double accnorm;      // sum |psi|^2 after the last executed order
+    int done;            // series converged (OCheby.cpp:179-182)
+    int last_order;      // index i of the last executed recurrence iteration
+    unsigned ticket;     // arrival counter of the last-block reduction
+    int pad;
+};
+
+constexpr int kMaxPartials = 8192;
+
+// Chebyshev scalars for one order (OCheby.cpp:97-122).
+struct ChebyOrder {
+    int k;               // order index (1 for the prologue)
+    double offset;       // Re offset (energy shift)
+    double inv_scaling;  // 1 / R
+    double c0r, c0i;     // order 1: coeff[0] * exp(-i offset dt)
+    double ckr, cki;     // coeff[k] * exp(-i offset dt)
+    double prec;         // break threshold
+};
+
+// One x-space split operator of GSPO: exp(c * W(x)) with c = -i tau, W the
+// S x S real symmetric matrix of this operator at each grid point.
+template <class R>
+struct XSplitDev {
+    int S = 1;
+    int coupled = 0;     // off-diagonal elements present
+    double tau = 0;      // dt * sf
+    const R *V[kMaxStates][kMaxStates] = {};   // potential arrays
+    const R *mu[kMaxStates][kMaxStates] = {};  // dipole arrays
+    double Edip[kMaxStates][kMaxStates] = {};  // -laser[step]*scale
+};
+
+constexpr int kMaxXOps = 4;
+template <class R>
+struct XGroupDev {
+    int n = 0;
+    XSplitDev<R> op[kMaxXOps];
+};
+
+// ---------------------------------------------------------------------------
+// Launchers implemented in qdx_kernels.hip.  All run on `stream`.
+template <class R>
+struct Launch {
+    // psi (complex128) -> phi (complex R); norm0 into ctl; resets ctl.
+    // For nd >= 2 also X = FFT_0(phi).
+    static void cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
+                           const double *psi, R *phi0, R *X, ChebyCtl *ctl, double *partials);
+    // One order of the series; prologue when ord.k == 1.
+    //   nd == 1: Tphi from phi_{k-1} in LDS; nd >= 2: Tphi = IFFT_0(X) and
+    //   X <- FFT_0(phi_k) afterwards.
+    static void cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
+                          const ChebyOrder &ord, const R *phim1, R *phim2, R *X, double *psi,
+                          ChebyCtl *ctl, double *partials);
+    // strided line pass along dimension `dim` (1 or 2) on X, in place.
+    // mode 0: forward, 1: inverse, 2: forward * T/N inverse (kinetic),
+    // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator)
+    static void col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode,
+                         const LinePlan &p, const HamDev<R> &h, double tau, R *X,
+                         const ChebyCtl *ctl);
+    // dst = H src (both complex128) through the same kernels; X scratch
+    static void ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h,
+                          const LinePlan *plans, const double *src, double *dst, R *X);
+    // GSPO: one step; psi complex R (state-major)
+    static void gspo_row_pre(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
+                             const XGroupDev<R> &pre, const R *psi_in, R *X);
+    static void gspo_row_post(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
+                              const XGroupDev<R> &post, const R *X, R *psi_out);
+    static void gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
+                        const HamDev<R> &kin, double tau_kin, const XGroupDev<R> &pre,
+                        const XGroupDev<R> &post, R *psi);
+    static void gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp,
+                           R *psi);
+};
+
+// precision-independent helpers
+void launch_convert_c128_to_c64(hipStream_t s, const double *src, float *dst, long n);
+void launch_convert_c64_to_c128(hipStream_t s, const float *src, double *dst, long n);
+// Deterministic FP64 reductions over a complex128 vector set [S][n]:
+// out[s] = sum_j conj(a_sj) b_sj  (re, im pairs), S <= kMaxStates
+void launch_overlap(hipStream_t s, const double *a, const double *b, long n, int S, double *partials,
+                    double *out);
+void launch_overlap_c64(hipStream_t s, const float *a, const float *b, long n, int S,
+                        double *partials, double *out);
+
+}  // namespace qdx
