@@ -1,5 +1,6 @@
 // Device building blocks for gfx950: complex arithmetic, radix-2/3/4/8
-// Stockham stages on LDS-resident lines, wave64 reductions.
+// Stockham line transforms with register-resident first/last stages, wave64
+// reductions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -100,86 +101,169 @@ struct Bfly<8, R> {
 };
 
 // ---------------------------------------------------------------------------
-// One Stockham stage on `nlines` lines of length L held in LDS.  Element q of
-// line l sits at s[l * lstr + q * estr].  Thread (line, lt) handles
-// butterflies j = lt + p * TL.  The stage reads all its inputs into
-// registers, synchronises, and writes its outputs in place.
-template <int RR, class R>
-__device__ __forceinline__ void fft_stage(cx<R> *s, int L, int Ns, int sign, const cx<R> *__restrict__ tw,
-                                          int line, int lt, int TL, int lstr, int estr, bool active) {
-    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
-    const int m = L / RR;
-    cx<R> v[Q][RR];
-    cx<R> *base = s + line * lstr;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int j = lt + q * TL;
-        if (active && j < m) {
-#pragma unroll
-            for (int t = 0; t < RR; ++t) v[q][t] = base[(j + t * m) * estr];
-        }
-    }
-    __syncthreads();
-    const int tstep = L / (Ns * RR);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int j = lt + q * TL;
-        if (active && j < m) {
-            const int k = j % Ns;
-            if (k != 0) {
-#pragma unroll
-                for (int t = 1; t < RR; ++t) {
-                    cx<R> w = tw[k * t * tstep];
-                    if (sign > 0) w.i = -w.i;
-                    v[q][t] = cmul(v[q][t], w);
-                }
-            }
-            Bfly<RR, R>::run(v[q], sign);
-            const int ob = (j - k) * RR + k;
-#pragma unroll
-            for (int t = 0; t < RR; ++t) base[(ob + t * Ns) * estr] = v[q][t];
-        }
-    }
-    __syncthreads();
+// Register-resident Stockham transform of one line of length L = 8*TL.
+//
+// Thread lt (0 <= lt < TL) of a line owns elements lt + q*TL, q = 0..7, in
+// v[q].  In the Stockham formulation the first stage (span 1) reads exactly
+// those elements and the last stage (span L/r) writes exactly those elements,
+// so both run on registers; only the middle stages go through LDS, using two
+// buffers alternately (one barrier per stage).  The plan's first and last
+// radices are 8, 4 or 2; middle stages may also be radix 3.
+//
+// LDS addressing of one line: element i of buffer a / b lives at
+// a[idx(i)] / b[idx(i)], idx(i) = (pad ? i + i/8 : i) * es.
+template <class R>
+struct LdsLine {
+    cx<R> *a;
+    cx<R> *b;
+    int es;
+    bool pad;
+    __device__ __forceinline__ int idx(int i) const { return (pad ? i + (i >> 3) : i) * es; }
+};
+
+template <class R>
+__device__ __forceinline__ cx<R> twiddle(const cx<R> *tw, int q, int sign) {
+    cx<R> w = tw[q];
+    if (sign > 0) w.i = -w.i;
+    return w;
 }
 
-// Full transform of LDS-resident lines (unnormalised, sign -1 forward).
-// rowmajor: line l occupies s[l*L .. l*L+L); otherwise element q of line l is
-// at s[q*nlines + l] (a tile of adjacent columns).
-// Requires blockDim.x >= nlines * ceil(L/8).  All threads must call it.
-template <class R>
-__device__ void fft_lines(cx<R> *s, const LinePlan &p, int nlines, bool rowmajor, int sign) {
-    const int L = p.L;
-    if (L == 1) return;
-    const int TL = (L + 7) / 8;
-    const int tid = threadIdx.x;
-    int line, lt, lstr, estr;
-    bool active;
-    if (rowmajor) {
-        line = tid / TL;
-        lt = tid % TL;
-        active = line < nlines;
-        lstr = L;
-        estr = 1;
-    } else {
-        line = tid % nlines;
-        lt = tid / nlines;
-        active = lt < TL;
-        lstr = 1;
-        estr = nlines;
-    }
-    if (!active) line = 0;
-    const cx<R> *tw = reinterpret_cast<const cx<R> *>(p.tw);
-    int Ns = 1;
-    for (int st = 0; st < p.nst; ++st) {
-        const int r = p.rad[st];
-        switch (r) {
-        case 8: fft_stage<8, R>(s, L, Ns, sign, tw, line, lt, TL, lstr, estr, active); break;
-        case 4: fft_stage<4, R>(s, L, Ns, sign, tw, line, lt, TL, lstr, estr, active); break;
-        case 2: fft_stage<2, R>(s, L, Ns, sign, tw, line, lt, TL, lstr, estr, active); break;
-        default: fft_stage<3, R>(s, L, Ns, sign, tw, line, lt, TL, lstr, estr, active); break;
+template <int RR, class R>
+__device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const LdsLine<R> &ln, int L, int Ns,
+                                          int TL, int lt, int sign, const cx<R> *tw) {
+    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
+    const int m = L / RR;
+    const int tstep = L / (Ns * RR);
+    const bool pow2 = (Ns & (Ns - 1)) == 0;
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+        const int j = lt + p * TL;
+        if (j < m) {
+            const int k = pow2 ? (j & (Ns - 1)) : (j % Ns);
+            cx<R> x[RR];
+#pragma unroll
+            for (int t = 0; t < RR; ++t) x[t] = in[ln.idx(j + t * m)];
+            if (k != 0) {
+#pragma unroll
+                for (int t = 1; t < RR; ++t) x[t] = cmul(x[t], twiddle(tw, k * t * tstep, sign));
+            }
+            Bfly<RR, R>::run(x, sign);
+            const int ob = (j - k) * RR + k;
+#pragma unroll
+            for (int t = 0; t < RR; ++t) out[ln.idx(ob + t * Ns)] = x[t];
         }
+    }
+}
+
+// copy the twiddle table of plan p into LDS (all threads of the block)
+template <class R>
+__device__ __forceinline__ void stage_twiddles(cx<R> *dst, const LinePlan &p) {
+    const cx<R> *src = reinterpret_cast<const cx<R> *>(p.tw);
+    for (int i = threadIdx.x; i < p.L; i += blockDim.x) dst[i] = src[i];
+}
+
+// Full unnormalised transform (sign -1 forward).  Every thread of the block
+// must call it (it synchronises); threads without a line pass active=false.
+// tw: the table exp(-2 pi i q / L), q in [0, L), usually an LDS copy made by
+// stage_twiddles (the barrier at the start of fft8 publishes it).
+template <class R>
+__device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const LinePlan &p, const cx<R> *tw,
+                                     int sign, int lt, bool active) {
+    const int L = p.L, TL = L >> 3;
+    if (p.nst == 1) {   // L == 8
+        if (active) Bfly<8, R>::run(v, sign);
+        return;
+    }
+    __syncthreads();   // previous readers of the LDS buffers are done
+    // first stage: span 1, no twiddles, registers -> buffer a
+    const int r0 = p.rad[0];
+    if (active) {
+        if (r0 == 8) {
+            Bfly<8, R>::run(v, sign);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) ln.a[ln.idx(lt * 8 + t)] = v[t];
+        } else if (r0 == 4) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                cx<R> x[4] = {v[q], v[q + 2], v[q + 4], v[q + 6]};
+                Bfly<4, R>::run(x, sign);
+                const int j = lt + q * TL;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) ln.a[ln.idx(j * 4 + t)] = x[t];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cx<R> x[2] = {v[q], v[q + 4]};
+                Bfly<2, R>::run(x, sign);
+                const int j = lt + q * TL;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) ln.a[ln.idx(j * 2 + t)] = x[t];
+            }
+        }
+    }
+    __syncthreads();
+    int Ns = r0;
+    const cx<R> *in = ln.a;
+    cx<R> *out = ln.b;
+    for (int st = 1; st < p.nst - 1; ++st) {
+        const int r = p.rad[st];
+        if (active) {
+            switch (r) {
+            case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
+            case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
+            case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
+            default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
+            }
+        }
+        __syncthreads();
+        const cx<R> *tmp = in;
+        in = out;
+        out = const_cast<cx<R> *>(tmp);
         Ns *= r;
+    }
+    // last stage: span L/r, buffer -> registers in the owned layout
+    const int rl = p.rad[p.nst - 1];
+    if (!active) return;
+    if (rl == 8) {
+        const int j = lt;
+        cx<R> x[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) x[t] = in[ln.idx(j + t * Ns)];
+        if (j != 0) {
+#pragma unroll
+            for (int t = 1; t < 8; ++t) x[t] = cmul(x[t], twiddle(tw, j * t, sign));
+        }
+        Bfly<8, R>::run(x, sign);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[t] = x[t];
+    } else if (rl == 4) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int j = lt + q * TL;
+            cx<R> x[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x[t] = in[ln.idx(j + t * Ns)];
+            if (j != 0) {
+#pragma unroll
+                for (int t = 1; t < 4; ++t) x[t] = cmul(x[t], twiddle(tw, j * t, sign));
+            }
+            Bfly<4, R>::run(x, sign);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[q + 2 * t] = x[t];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = lt + q * TL;
+            cx<R> x[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) x[t] = in[ln.idx(j + t * Ns)];
+            if (j != 0) x[1] = cmul(x[1], twiddle(tw, j, sign));
+            Bfly<2, R>::run(x, sign);
+            v[q] = x[0];
+            v[q + 4] = x[1];
+        }
     }
 }
 
@@ -206,33 +290,29 @@ __device__ __forceinline__ double block_sum(double v, double *red /* >= 16 doubl
     return t;
 }
 
-// Last-block completion of a grid-wide reduction: every block stores its
-// partial, the block that arrives last sums the partials in index order.
-// Returns true (in all threads of that block) for the last block; `total`
-// then holds the sum in thread 0.
-__device__ __forceinline__ bool last_block_sum(double part, double *partials, unsigned *ticket,
-                                               double &total, int *flag_lds) {
-    const int nb = gridDim.x * gridDim.y;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[bid], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned t = atomicAdd(ticket, 1u);
-        *flag_lds = (t == (unsigned)(nb - 1));
+// Sum of n partials in a fixed order that does not depend on the block size:
+// lane l of wave 0 adds partials l, l+64, ..., then a fixed shuffle tree.
+// The result is broadcast to the whole block through *bcast.
+__device__ __forceinline__ double fixed_sum(const double *p, int n, double *bcast) {
+    if (threadIdx.x < 64) {
+        double v = 0;
+        for (int i = threadIdx.x; i < n; i += 64) v += p[i];
+        v = wave_sum(v);
+        if (threadIdx.x == 0) *bcast = v;
     }
     __syncthreads();
-    const bool last = *flag_lds != 0;
-    if (last) {
-        __threadfence();
-        double v = 0;
-        for (int i = threadIdx.x; i < nb; i += blockDim.x)
-            v += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // fixed-order combination: per-thread strided sums, then block_sum
-        __shared__ double red[16];
-        total = block_sum(v, red);
-        if (threadIdx.x == 0) *ticket = 0u;
-    }
-    return last;
+    const double r = *bcast;
+    __syncthreads();
+    return r;
+}
+
+// true when the previous order met |norm0 - accnorm| / norm0 < prec
+__device__ __forceinline__ bool prev_converged(const ConvCheck &c, double *bcast, double *acc_out) {
+    if (c.k_prev < 2) return false;
+    const double n0 = *c.norm0;
+    const double acc = fixed_sum(c.part, c.nb, bcast);
+    *acc_out = acc;
+    return fabs(n0 - acc) / n0 < c.prec;
 }
 
 }  // namespace qdx

@@ -23,6 +23,7 @@ struct LinePlan {
     int nst = 0;
     int rad[kMaxStages] = {};
     const void *tw = nullptr;
+    int tw_lds = 1;    // kernels stage the table in LDS (set by the launcher)
 };
 
 // Grid as the kernels see it: N[d] points in dimension d, dimension 0
@@ -55,13 +56,24 @@ struct HamDev {
 // Per-propagator device control block for the Chebyshev series.
 struct ChebyCtl {
     double norm0;        // raw_norm(psi) at the start of the step
-    double accnorm;      // sum |psi|^2 after the last executed order
+    double accnorm;      // sum |psi|^2 of the order that ended the series
     int done;            // series converged (OCheby.cpp:179-182)
     int last_order;      // index i of the last executed recurrence iteration
-    unsigned ticket;     // arrival counter of the last-block reduction
-    int pad;
+    unsigned pad0;
+    int pad1;
 };
 
+// Break test of the previous order, evaluated at the start of a kernel from
+// the per-block partial sums that the previous row kernel stored.  Every
+// kernel sums the partials in the same fixed order (lane-strided in wave 0,
+// then a fixed shuffle tree), so all kernels reach the same decision.
+struct ConvCheck {
+    const double *part = nullptr;   // partials of order k_prev
+    const double *norm0 = nullptr;  // sum |psi|^2 at step start (ChebyCtl::norm0)
+    int nb = 0;
+    int k_prev = 0;                 // < 2: nothing to test
+    double prec = 0;
+};
 constexpr int kMaxPartials = 8192;
 
 // Chebyshev scalars for one order (OCheby.cpp:97-122).
@@ -99,20 +111,24 @@ template <class R>
 struct Launch {
     // psi (complex128) -> phi (complex R); norm0 into ctl; resets ctl.
     // For nd >= 2 also X = FFT_0(phi).
-    static void cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
-                           const double *psi, R *phi0, R *X, ChebyCtl *ctl, double *partials);
+    // returns the number of partial sums written (one per block)
+    static int cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
+                          const double *psi, R *phi0, R *X, ChebyCtl *ctl, double *partials);
     // One order of the series; prologue when ord.k == 1.
     //   nd == 1: Tphi from phi_{k-1} in LDS; nd >= 2: Tphi = IFFT_0(X) and
     //   X <- FFT_0(phi_k) afterwards.
-    static void cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                          const ChebyOrder &ord, const R *phim1, R *phim2, R *X, double *psi,
-                          ChebyCtl *ctl, double *partials);
+    // init_part/nb_init: partials of the init kernel, summed into
+    // ctl->norm0 by the prologue (ord.k == 1)
+    static int cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
+                         const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X,
+                         double *psi, ChebyCtl *ctl, double *partials, const double *init_part,
+                         int nb_init);
     // strided line pass along dimension `dim` (1 or 2) on X, in place.
     // mode 0: forward, 1: inverse, 2: forward * T/N inverse (kinetic),
     // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator)
     static void col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode,
                          const LinePlan &p, const HamDev<R> &h, double tau, R *X,
-                         const ChebyCtl *ctl);
+                         const ChebyCtl *ctl, const ConvCheck &cc);
     // dst = H src (both complex128) through the same kernels; X scratch
     static void ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h,
                           const LinePlan *plans, const double *src, double *dst, R *X);

@@ -1,7 +1,7 @@
 // gfx950 kernels of the grid-propagation hot path.
 //
 // Chebyshev (OCheby::real_time_step, OCheby.cpp:92-197) runs per order as
-//   nd == 1 : one kernel  -- load phi_{k-1} lines into LDS, FFT, *T/N, IFFT,
+//   nd == 1 : one kernel  -- phi_{k-1} line into registers, FFT, *T/N, IFFT,
 //             fused potential/dipole/coupling multiply, recurrence, FP64
 //             accumulation of psi and of sum |psi|^2, device-side break test;
 //   nd >= 2 : line passes -- strided FFT/T/IFFT passes along dimensions
@@ -10,11 +10,16 @@
 //             back into X for the next order.
 // GSPO (OGSPO.cpp:114-139) uses the same line machinery with the x-space
 // split operators exp(-i tau W(x)) evaluated per point in registers.
+//
+// Thread layout of every line kernel: a line of L = 8*TL points is handled by
+// TL threads, thread lt owning points lt + q*TL, q = 0..7 (see fft8).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <string>
 
 #include "qdx_device.hpp"
 #include "qdx_impl.hpp"
@@ -31,36 +36,17 @@ namespace qdx {
 // ---------------------------------------------------------------------------
 // helpers
 
-// kinetic value T_i(k) at a flat k-space index (dimension 0 fastest), summed
-// over dimensions in dimension order as Diff::_InitKspace does (Diff.cpp:120-139)
+// kinetic value T_i(k) at k-space indices (k0, k1, k2), summed over dimensions
+// in dimension order as Diff::_InitKspace does (Diff.cpp:120-139)
 template <class R>
-__device__ __forceinline__ double kin_sum(const HamDev<R> &h, int st, const GridDev &g, long idx) {
+__device__ __forceinline__ double kin_sum(const HamDev<R> &h, int st, const GridDev &g, int k0, int k1, int k2) {
     const double *const *tk = h.tk[st];
-    const int k0 = (int)(idx % g.N[0]);
     double t = 0.0 + tk[0][k0];
     if (g.nd > 1) {
-        const long r = idx / g.N[0];
-        t += tk[1][(int)(r % g.N[1])];
-        if (g.nd > 2) t += tk[2][(int)(r / g.N[1])];
+        t += tk[1][k1];
+        if (g.nd > 2) t += tk[2][k2];
     }
     return t;
-}
-
-// (H_x phi)_i: potential, unity and dipole part at grid index gi
-template <class R, int S>
-__device__ __forceinline__ void xspace_apply(const HamDev<R> &h, long gi, const cx<double> (&p)[S],
-                                             cx<double> (&hp)[S]) {
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-#pragma unroll
-        for (int j = 0; j < S; ++j) {
-            double w = (i == j) ? h.unity[i] : 0.0;
-            if (h.W[i][j]) w += (double)h.W[i][j][gi];
-            if (h.mu[i][j]) w += h.Edip[i][j] * (double)h.mu[i][j][gi];
-            hp[i].r += w * p[j].r;
-            hp[i].i += w * p[j].i;
-        }
-    }
 }
 
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -70,295 +56,395 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (b % 8) * (nb / 8) + (b / 8);
 }
 
+// Row-block geometry: rb rows of dimension 0 for each of S states; line
+// l = st * rb + rr.  Two padded LDS buffers per line.
+template <class R>
+struct RowBlock {
+    int L, TL, rb, nl, line, lt, st, rr;
+    long nrows, row;
+    bool active;
+    LdsLine<R> ln;
+    cx<R> *tw;       // LDS twiddle table after the line buffers (if twl)
+    cx<R> *after;    // first free LDS element after the table
+    bool twl;        // twiddle table staged in LDS (else read from the plan)
+    __device__ RowBlock(const GridDev &g, int rb_, int S, cx<R> *smem, const LinePlan &p) {
+        L = g.N[0];
+        TL = L >> 3;
+        rb = rb_;
+        nl = S * rb;
+        nrows = g.size / L;
+        line = threadIdx.x / TL;
+        lt = threadIdx.x % TL;
+        st = line / rb;
+        rr = line % rb;
+        row = (long)blockIdx.x * rb + rr;
+        active = line < nl && row < nrows;
+        if (!active) {
+            st = 0;
+            rr = 0;
+        }
+        const int lp = L + (L >> 3);
+        const int l = active ? line : 0;
+        ln.a = smem + (long)l * lp;
+        ln.b = smem + (long)(nl + l) * lp;
+        ln.es = 1;
+        ln.pad = true;
+        tw = smem + (long)2 * nl * lp;
+        twl = p.tw_lds;
+        after = twl ? tw + L : tw;
+    }
+    // stage the twiddle table (when it lives in LDS)
+    __device__ void stage(const LinePlan &p) {
+        if (twl) stage_twiddles(tw, p);
+    }
+    // transform with the LDS or the global twiddle table; both calls inline
+    __device__ void fft(cx<R> (&v)[8], const LinePlan &p, int sign) {
+        if (twl) fft8<R>(v, ln, p, tw, sign, lt, active);
+        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active);
+    }
+    __device__ long gidx(int q) const { return row * L + lt + q * TL; }
+};
+
+template <class R>
+static size_t row_lds_bytes(int L, int nl, bool twl) {
+    return ((size_t)2 * nl * (L + L / 8) + (twl ? L : 0)) * sizeof(cx<R>);
+}
+
 // ---------------------------------------------------------------------------
 // Chebyshev init: phi0 = psi, norm0 = sum |psi|^2, X = FFT_0(phi0) (nd >= 2)
-template <class R, int S>
-__global__ void k_cheby_init(GridDev g, LinePlan p0, int rb, const cx<double> *__restrict__ psi,
-                             cx<R> *__restrict__ phi0, cx<R> *__restrict__ X, ChebyCtl *ctl,
-                             double *partials) {
+template <class R, int S, bool HAS_X>
+__global__ void __launch_bounds__(512) k_cheby_init(GridDev g, LinePlan p0, int rb, const cx<double> *__restrict__ psi,
+                                                     cx<R> *__restrict__ phi0, cx<R> *__restrict__ X, ChebyCtl *ctl,
+                                                     double *partials) {
     extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    __shared__ int flag;
-    const int L = g.N[0];
-    const long nrows = g.size / L;
-    const long row0 = (long)blockIdx.x * rb;
-    const int nl = S * rb;
-    double acc = 0;
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        if (row < nrows) {
-            const long gi = (long)st * g.size + row * L + x;
-            cx<double> v = psi[gi];
-            acc += v.r * v.r + v.i * v.i;
-            cx<R> vr = from_d<R>(v);
-            phi0[gi] = vr;
-            s[line * L + x] = vr;
-        }
-    }
-    if (X) {
-        __syncthreads();
-        fft_lines<R>(s, p0, nl, true, -1);
-        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-            const int line = e / L, x = e % L;
-            const int st = line / rb, rr = line % rb;
-            const long row = row0 + rr;
-            if (row < nrows) X[(long)st * g.size + row * L + x] = s[line * L + x];
-        }
-    }
     __shared__ double red[16];
-    double part = block_sum(acc, red);
-    double total = 0;
-    if (last_block_sum(part, partials, &ctl->ticket, total, &flag) && threadIdx.x == 0) {
-        ctl->norm0 = total;
-        ctl->accnorm = total;
-        ctl->done = 0;
-        ctl->last_order = 1;
+    RowBlock<R> B(g, rb, S, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    const long base = (long)B.st * g.size;
+    double acc = 0;
+    cx<R> v[8];
+    if (B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const cx<double> x = psi[base + B.gidx(q)];
+            acc += x.r * x.r + x.i * x.i;
+            v[q] = from_d<R>(x);
+            phi0[base + B.gidx(q)] = v[q];
+        }
+    }
+    if (HAS_X) {
+        B.stage(p0);
+        B.fft(v, p0, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) X[base + B.gidx(q)] = v[q];
+        }
+    }
+    const double part = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = part;
+        if (blockIdx.x == 0) {
+            ctl->done = 0;
+            ctl->last_order = 1;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Chebyshev row kernel (see file header).  PRO: prologue (order 1).
 template <class R, int S, bool ONE_D, bool PRO>
-__global__ void k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord, int rb,
-                            const cx<R> *__restrict__ phim1, cx<R> *__restrict__ phim2,
-                            cx<R> *__restrict__ X, cx<double> *__restrict__ psi, ChebyCtl *ctl,
-                            double *partials) {
+__global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord,
+                                                    ConvCheck cc, int rb, const cx<R> *__restrict__ phim1,
+                                                    cx<R> *__restrict__ phim2, cx<R> *__restrict__ X,
+                                                    cx<double> *__restrict__ psi, ChebyCtl *ctl,
+                                                    double *__restrict__ partials, const double *init_part,
+                                                    int nb_init) {
     if (ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    __shared__ int flag;
-    const int L = g.N[0];
-    const long nrows = g.size / L;
-    const long row0 = (long)blockIdx.x * rb;
-    const int nl = S * rb;
+    __shared__ double red[16];
+    RowBlock<R> B(g, rb, S, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    const int st = B.st;
+    const long base = (long)st * g.size;
 
-    // 1. T phi_{k-1} into LDS
-    const cx<R> *src = ONE_D ? phim1 : X;
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        cx<R> v = mk<R>(0, 0);
-        if (row < nrows) v = src[(long)st * g.size + row * L + x];
-        s[line * L + x] = v;
-    }
-    __syncthreads();
-    if (ONE_D) {
-        fft_lines<R>(s, p0, nl, true, -1);
-        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-            const int line = e / L, x = e % L;
-            const int st = line / rb;
-            const double t = h.tk[st][0] ? (0.0 + h.tk[st][0][x]) * h.inv_n : 0.0;
-            cx<R> v = s[line * L + x];
-            s[line * L + x] = mk<R>((R)((double)v.r * t), (R)((double)v.i * t));
-        }
-        __syncthreads();
-    }
-    fft_lines<R>(s, p0, nl, true, +1);
-
-    // 2. fused epilogue per grid point, all states
-    double acc = 0;
-    const cx<double> ck = mk<double>(ord.ckr, ord.cki);
-    const cx<double> c0 = mk<double>(ord.c0r, ord.c0i);
-    for (int e = threadIdx.x; e < rb * L; e += blockDim.x) {
-        const int rr = e / L, x = e % L;
-        const long row = row0 + rr;
-        if (row >= nrows) continue;
-        const long gi = row * L + x;
-        cx<double> p[S], hp[S];
+    // phi operands of the epilogue are requested before anything else; psi
+    // and the operator arrays are read in the epilogue itself
+    cx<R> v[8], pm1[S][8];
+    if (B.active) {
+        const cx<R> *src = ONE_D ? phim1 : X;
 #pragma unroll
-        for (int st = 0; st < S; ++st) {
-            p[st] = to_d(phim1[(long)st * g.size + gi]);
-            hp[st] = to_d(s[(st * rb + rr) * L + x]);
-            if (!h.tk[st][0]) hp[st] = mk<double>(0, 0);
-        }
-        xspace_apply<R, S>(h, gi, p, hp);
+        for (int q = 0; q < 8; ++q) v[q] = src[base + B.gidx(q)];
 #pragma unroll
-        for (int st = 0; st < S; ++st) {
-            const long si = (long)st * g.size + gi;
-            // b = (H phi - offset phi) / R
-            cx<double> b = mk<double>((hp[st].r - ord.offset * p[st].r) * ord.inv_scaling,
-                                      (hp[st].i - ord.offset * p[st].i) * ord.inv_scaling);
-            cx<double> pk;
-            cx<double> ps = psi[si];
-            if (PRO) {
-                pk = mk<double>(b.i, -b.r);                       // * (-i)
-                ps = cmul(c0, ps) + cmul(ck, pk);
-            } else {
-                const cx<double> pm2 = to_d(phim2[si]);
-                pk = mk<double>(2.0 * b.i + pm2.r, -2.0 * b.r + pm2.i);   // * (-2i) + phi_{k-2}
-                ps = ps + cmul(ck, pk);
-                acc += ps.r * ps.r + ps.i * ps.i;
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) pm1[j][q] = phim1[(long)j * g.size + B.gidx(q)];
+    }
+    // this thread's row of the operator matrix (runtime state index, read once)
+    const double *tk0 = h.tk[st][0];
+    const bool has_t = tk0 != nullptr;
+    double tq[8];
+    if (ONE_D && B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_n : 0.0;
+    }
+    B.stage(p0);
+    if (PRO) {
+        // sum |psi|^2 of the step start, in the fixed order every kernel uses
+        if (blockIdx.x == 0) {
+            const double n0 = fixed_sum(init_part, nb_init, red);
+            if (threadIdx.x == 0) {
+                ctl->norm0 = n0;
+                ctl->last_order = 1;
             }
-            psi[si] = ps;
-            const cx<R> pkr = from_d<R>(pk);
-            phim2[si] = pkr;
-            s[(st * rb + rr) * L + x] = pkr;
+        }
+    } else {
+        double acc_prev;
+        if (prev_converged(cc, red, &acc_prev)) {
+            // the previous order ended the series (OCheby.cpp:179-182)
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ctl->done = 1;
+                ctl->accnorm = acc_prev;
+            }
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->last_order = ord.k;
+    }
+    const R *Wp[S];
+    const R *Mp[S];
+    double Ed[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        Wp[j] = h.W[st][j];
+        Mp[j] = h.mu[st][j];
+        Ed[j] = h.Edip[st][j];
+    }
+    const double uni = h.unity[st];
+    if (ONE_D) {
+        B.fft(v, p0, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
         }
     }
-    // 3. X <- FFT_0(phi_k) for the next order's strided passes
+    B.fft(v, p0, +1);
+
+    // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j.
+    // All loads are issued before the first store of the epilogue.
+    double acc = 0;
+    if (B.active) {
+        const cx<double> ck = mk<double>(ord.ckr, ord.cki);
+        const cx<double> c0 = mk<double>(ord.c0r, ord.c0i);
+        cx<double> ps[8];
+        cx<R> m2[8];
+        double wv[S][8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long gi = B.gidx(q);
+            ps[q] = psi[base + gi];
+            if (!PRO) m2[q] = phim2[base + gi];
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                double w = (j == st) ? uni : 0.0;
+                if (Wp[j]) w += (double)Wp[j][gi];
+                if (Mp[j]) w += Ed[j] * (double)Mp[j][gi];
+                wv[j][q] = w;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long gi = B.gidx(q);
+            cx<double> hp = has_t ? to_d(v[q]) : mk<double>(0, 0);
+            cx<double> p = to_d(pm1[0][q]);
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const double w = wv[j][q];
+                const cx<double> pj = to_d(pm1[j][q]);
+                if (j == st) p = pj;
+                hp.r += w * pj.r;
+                hp.i += w * pj.i;
+            }
+            // b = (H phi - offset phi) / R
+            const cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
+                                            (hp.i - ord.offset * p.i) * ord.inv_scaling);
+            cx<double> pk;
+            cx<double> s = ps[q];
+            if (PRO) {
+                pk = mk<double>(b.i, -b.r);                               // * (-i)
+                s = cmul(c0, s) + cmul(ck, pk);
+            } else {
+                const cx<double> mm = to_d(m2[q]);
+                pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);    // * (-2i) + phi_{k-2}
+                s = s + cmul(ck, pk);
+                acc += s.r * s.r + s.i * s.i;
+            }
+            psi[base + gi] = s;
+            v[q] = from_d<R>(pk);
+            phim2[base + gi] = v[q];
+        }
+    }
+    // X <- FFT_0(phi_k) for the next order's strided passes
     if (!ONE_D) {
-        __syncthreads();
-        fft_lines<R>(s, p0, nl, true, -1);
-        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-            const int line = e / L, x = e % L;
-            const int st = line / rb, rr = line % rb;
-            const long row = row0 + rr;
-            if (row < nrows) X[(long)st * g.size + row * L + x] = s[line * L + x];
+        B.fft(v, p0, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) X[base + B.gidx(q)] = v[q];
         }
     }
     if (PRO) return;
-    // 4. convergence test on sum |psi|^2 (OCheby.cpp:179-182)
-    __shared__ double red[16];
-    double part = block_sum(acc, red);
-    double total = 0;
-    if (last_block_sum(part, partials, &ctl->ticket, total, &flag) && threadIdx.x == 0) {
-        ctl->accnorm = total;
-        ctl->last_order = ord.k;
-        const double nconv = fabs(ctl->norm0 - total) / ctl->norm0;
-        if (nconv < ord.prec) ctl->done = 1;
-    }
+    // partial sum |psi|^2 of this block; the next kernels test convergence
+    const double part = block_sum(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = part;
 }
 
 // ---------------------------------------------------------------------------
-// Strided line pass along dimension `dim` on X (in place).  A block owns a
-// tile of W adjacent dimension-0 positions for one state and one index of the
-// remaining dimension; lines are stored column-major in LDS so that the W
-// lanes of one butterfly read adjacent words.
+// Strided line pass along dimension `dim` (1 or 2) on X, in place.  A block
+// owns a tile of W adjacent dimension-0 positions for one state and one index
+// of the remaining dimension; the W lanes of a butterfly touch W adjacent
+// words in LDS and W adjacent complex values in memory.
+// MODE 0: forward, 1: inverse, 2: forward * T/N inverse,
+//      3: forward * exp(-i tau T)/N inverse.
 template <class R, int MODE>
-__global__ void k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
-                      cx<R> *__restrict__ X, const ChebyCtl *ctl) {
+__global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
+                                              cx<R> *__restrict__ X, const ChebyCtl *ctl, ConvCheck cc) {
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
+    cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
+    __shared__ double bc;
     const int N0 = g.N[0];
     const int L = g.N[dim];
+    const int TL = L >> 3;
     const long estride = dim == 1 ? N0 : (long)N0 * g.N[1];
-    // remaining dimension (the one that is neither 0 nor dim)
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     const long ostride = dim == 1 ? (long)N0 * g.N[1] : N0;
     const int tiles0 = N0 / W;
-    const int nb = gridDim.x;
-    const int b = xcd_remap(blockIdx.x, nb);
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int c0 = (b % tiles0) * W;
     const int rest = b / tiles0;
     const int oth = rest % nother;
     const int st = rest / nother;
-    cx<R> *base = X + (long)st * g.size + (long)oth * ostride + c0;
+    const int c = threadIdx.x % W;
+    const int lt = threadIdx.x / W;
+    const bool active = lt < TL;
+    cx<R> *base = X + (long)st * g.size + (long)oth * ostride + c0 + c;
+    LdsLine<R> ln;
+    ln.a = smem + c;
+    ln.b = smem + (long)W * L + c;
+    ln.es = W;
+    ln.pad = false;
+    cx<R> *twl = smem + (long)2 * W * L;
+    cx<R> v[8];
+    const bool tw_in_lds = p.tw_lds;
+    auto FFT = [&](int sign) {
+        if (tw_in_lds) fft8<R>(v, ln, p, twl, sign, lt, active);
+        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active);
+    };
 
-    for (int e = threadIdx.x; e < W * L; e += blockDim.x) {
-        const int q = e / W, c = e % W;
-        s[q * W + c] = base[q * estride + c];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = base[(long)(lt + q * TL) * estride];
     }
-    __syncthreads();
-    const int sign1 = MODE == 1 ? +1 : -1;
-    fft_lines<R>(s, p, W, false, sign1);
+    // kinetic table values of this thread's elements, requested early
+    double tq[8];
+    if (MODE >= 2 && active) {
+        const bool has_t = h.tk[st][0] != nullptr;
+        const int k0 = c0 + c;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int kd = lt + q * TL;
+            tq[q] = has_t ? kin_sum<R>(h, st, g, k0, dim == 1 ? kd : oth, dim == 2 ? kd : oth) : 0.0;
+        }
+    }
+    if (tw_in_lds) stage_twiddles(twl, p);
+    {
+        double a;
+        if (prev_converged(cc, &bc, &a)) return;
+    }
+    FFT(MODE == 1 ? +1 : -1);
     if (MODE >= 2) {
-        for (int e = threadIdx.x; e < W * L; e += blockDim.x) {
-            const int q = e / W, c = e % W;
-            // k-space indices of this element
-            long kidx;
-            if (g.nd == 2) kidx = (long)q * N0 + c0 + c;
-            else kidx = (dim == 2) ? ((long)q * g.N[1] + oth) * N0 + c0 + c
-                                   : ((long)oth * g.N[1] + q) * N0 + c0 + c;
-            const double tt = h.tk[st][0] ? kin_sum<R>(h, st, g, kidx) : 0.0;
-            cx<R> v = s[q * W + c];
-            if (MODE == 2) {
-                const double t = tt * h.inv_n;
-                s[q * W + c] = mk<R>((R)((double)v.r * t), (R)((double)v.i * t));
-            } else {
-                // exp(-i tau T) / N (OGSPO.cpp:37-41 folds 1/N into the exponential)
-                double sn, cs;
-                sincos(-tau * tt, &sn, &cs);
-                cx<double> w = mk<double>(cs * h.inv_n, sn * h.inv_n);
-                s[q * W + c] = from_d<R>(cmul(to_d(v), w));
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double tt = tq[q];
+                if (MODE == 2) {
+                    const double t = tt * h.inv_n;
+                    v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
+                } else {
+                    // exp(-i tau T) / N (OGSPO.cpp:37-41 folds 1/N into the exponential)
+                    double sn, cs;
+                    sincos(-tau * tt, &sn, &cs);
+                    v[q] = from_d<R>(cmul(to_d(v[q]), mk<double>(cs * h.inv_n, sn * h.inv_n)));
+                }
             }
         }
-        __syncthreads();
-        fft_lines<R>(s, p, W, false, +1);
+        FFT(+1);
     }
-    for (int e = threadIdx.x; e < W * L; e += blockDim.x) {
-        const int q = e / W, c = e % W;
-        base[q * estride + c] = s[q * W + c];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * estride] = v[q];
     }
 }
 
 // ---------------------------------------------------------------------------
-// H applied to a complex128 vector: forward row pass, strided passes, row
-// epilogue.  Reuses the Chebyshev row machinery with plain output.
-template <class R, int S, bool ONE_D>
-__global__ void k_apply_row(GridDev g, HamDev<R> h, LinePlan p0, int rb, const cx<double> *__restrict__ src,
-                            cx<R> *__restrict__ X, cx<double> *__restrict__ dst) {
+// H applied to a complex128 vector (Operator::Apply): forward row pass,
+// strided passes, row epilogue.
+template <class R, int S>
+__global__ void __launch_bounds__(512) k_row_fwd(GridDev g, LinePlan p0, int rb, const cx<double> *__restrict__ src,
+                                                  cx<R> *__restrict__ X) {
     extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    const int L = g.N[0];
-    const long nrows = g.size / L;
-    const long row0 = (long)blockIdx.x * rb;
-    const int nl = S * rb;
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        cx<R> v = mk<R>(0, 0);
-        if (row < nrows) v = ONE_D ? from_d<R>(src[(long)st * g.size + row * L + x]) : X[(long)st * g.size + row * L + x];
-        s[line * L + x] = v;
-    }
-    __syncthreads();
-    if (ONE_D) {
-        fft_lines<R>(s, p0, nl, true, -1);
-        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-            const int line = e / L, x = e % L;
-            const int st = line / rb;
-            const double t = h.tk[st][0] ? (0.0 + h.tk[st][0][x]) * h.inv_n : 0.0;
-            cx<R> v = s[line * L + x];
-            s[line * L + x] = mk<R>((R)((double)v.r * t), (R)((double)v.i * t));
-        }
-        __syncthreads();
-    }
-    fft_lines<R>(s, p0, nl, true, +1);
-    for (int e = threadIdx.x; e < rb * L; e += blockDim.x) {
-        const int rr = e / L, x = e % L;
-        const long row = row0 + rr;
-        if (row >= nrows) continue;
-        const long gi = row * L + x;
-        cx<double> p[S], hp[S];
+    RowBlock<R> B(g, rb, S, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    const long base = (long)B.st * g.size;
+    cx<R> v[8];
+    if (B.active) {
 #pragma unroll
-        for (int st = 0; st < S; ++st) {
-            p[st] = src[(long)st * g.size + gi];
-            hp[st] = h.tk[st][0] ? to_d(s[(st * rb + rr) * L + x]) : mk<double>(0, 0);
-        }
-        xspace_apply<R, S>(h, gi, p, hp);
+        for (int q = 0; q < 8; ++q) v[q] = from_d<R>(src[base + B.gidx(q)]);
+    }
+    B.stage(p0);
+    B.fft(v, p0, -1);
+    if (B.active) {
 #pragma unroll
-        for (int st = 0; st < S; ++st) dst[(long)st * g.size + gi] = hp[st];
+        for (int q = 0; q < 8; ++q) X[base + B.gidx(q)] = v[q];
     }
 }
 
-template <class R, int S>
-__global__ void k_row_fwd(GridDev g, LinePlan p0, int rb, const cx<double> *__restrict__ src,
-                          cx<R> *__restrict__ X) {
+template <class R, int S, bool ONE_D>
+__global__ void __launch_bounds__(512) k_apply_row(GridDev g, HamDev<R> h, LinePlan p0, int rb,
+                                                    const cx<double> *__restrict__ src, const cx<R> *__restrict__ X,
+                                                    cx<double> *__restrict__ dst) {
     extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    const int L = g.N[0];
-    const long nrows = g.size / L;
-    const long row0 = (long)blockIdx.x * rb;
-    const int nl = S * rb;
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        cx<R> v = mk<R>(0, 0);
-        if (row < nrows) v = from_d<R>(src[(long)st * g.size + row * L + x]);
-        s[line * L + x] = v;
+    RowBlock<R> B(g, rb, S, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    const int st = B.st;
+    const long base = (long)st * g.size;
+    cx<R> v[8];
+    if (B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = ONE_D ? from_d<R>(src[base + B.gidx(q)]) : X[base + B.gidx(q)];
     }
-    __syncthreads();
-    fft_lines<R>(s, p0, nl, true, -1);
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        if (row < nrows) X[(long)st * g.size + row * L + x] = s[line * L + x];
+    B.stage(p0);
+    const bool has_t = h.tk[st][0] != nullptr;
+    if (ONE_D) {
+        B.fft(v, p0, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double t = has_t ? (0.0 + h.tk[st][0][B.lt + q * B.TL]) * h.inv_n : 0.0;
+                v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
+            }
+        }
+    }
+    B.fft(v, p0, +1);
+    if (!B.active) return;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const long gi = B.gidx(q);
+        cx<double> hp = has_t ? to_d(v[q]) : mk<double>(0, 0);
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            double w = (j == st) ? h.unity[st] : 0.0;
+            if (h.W[st][j]) w += (double)h.W[st][j][gi];
+            if (h.mu[st][j]) w += h.Edip[st][j] * (double)h.mu[st][j][gi];
+            const cx<double> pj = src[(long)j * g.size + gi];
+            hp.r += w * pj.r;
+            hp.i += w * pj.i;
+        }
+        dst[base + gi] = hp;
     }
 }
 
@@ -398,11 +484,10 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
         double sm, cm;
         sincos(-op.tau * m, &sm, &cm);
         const cx<double> ph = mk<double>(cm, sm);
-        // U = ph * [[c - i sc d, -i sc b], [-i sc b, c + i sc d]]
         const cx<double> u00 = mk<double>(c, -sc * d), u11 = mk<double>(c, sc * d);
         const cx<double> u01 = mk<double>(0, -sc * b);
-        cx<double> a0 = cmul(u00, v[0]) + cmul(u01, v[1]);
-        cx<double> a1 = cmul(u01, v[0]) + cmul(u11, v[1]);
+        const cx<double> a0 = cmul(u00, v[0]) + cmul(u01, v[1]);
+        const cx<double> a1 = cmul(u01, v[0]) + cmul(u11, v[1]);
         v[0] = cmul(ph, a0);
         v[1] = cmul(ph, a1);
     }
@@ -413,115 +498,99 @@ __device__ __forceinline__ void xgroup_point(const XGroupDev<R> &grp, long gi, c
     for (int o = 0; o < grp.n; ++o) xsplit_point<R, S>(grp.op[o], gi, v);
 }
 
-// pre-ops then FFT_0 into X (nd >= 2)
-template <class R, int S>
-__global__ void k_gspo_row_pre(GridDev g, LinePlan p0, int rb, XGroupDev<R> pre,
-                               const cx<R> *__restrict__ psi, cx<R> *__restrict__ X) {
+// GSPO kinetic split step over rows.  ONE_D: psi -> pre -> FFT -> exp(-i tau
+// T)/N -> IFFT -> post -> psi in one pass.  Otherwise PHASE 0: psi -> pre ->
+// FFT_0 -> X; PHASE 1: X -> IFFT_0 -> post -> psi.  The x-space operators act
+// on all states of a point at once, so the S lines of a row are exchanged
+// through LDS around them.
+template <class R, int S, bool ONE_D, int PHASE>
+__global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, LinePlan p0, int rb,
+                                                   XGroupDev<R> pre, XGroupDev<R> post, cx<R> *__restrict__ psi,
+                                                   cx<R> *__restrict__ X) {
     extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    const int L = g.N[0];
-    const long nrows = g.size / L;
-    const long row0 = (long)blockIdx.x * rb;
-    const int nl = S * rb;
-    for (int e = threadIdx.x; e < rb * L; e += blockDim.x) {
-        const int rr = e / L, x = e % L;
-        const long row = row0 + rr;
-        cx<double> v[S];
-        if (row < nrows) {
-            const long gi = row * L + x;
+    cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
+    RowBlock<R> B(g, rb, S, smem, p0);
+    const int st = B.st;
+    const long base = (long)st * g.size;
+    cx<R> v[8];
+    B.stage(p0);
+    // exchange area after the transform buffers and the twiddle table: [S][rb][L]
+    cx<R> *xa = B.after;
+    auto xspace = [&](const XGroupDev<R> &grp) {
+        if (grp.n == 0) return;
+        if (S == 1) {
+            if (B.active) {
 #pragma unroll
-            for (int st = 0; st < S; ++st) v[st] = to_d(psi[(long)st * g.size + gi]);
-            xgroup_point<R, S>(pre, gi, v);
-        } else {
-#pragma unroll
-            for (int st = 0; st < S; ++st) v[st] = mk<double>(0, 0);
+                for (int q = 0; q < 8; ++q) {
+                    cx<double> w[S];
+                    w[0] = to_d(v[q]);
+                    xgroup_point<R, S>(grp, B.gidx(q), w);
+                    v[q] = from_d<R>(w[0]);
+                }
+            }
+            return;
         }
+        __syncthreads();
+        if (B.active) {
 #pragma unroll
-        for (int st = 0; st < S; ++st) s[(st * rb + rr) * L + x] = from_d<R>(v[st]);
-    }
-    __syncthreads();
-    fft_lines<R>(s, p0, nl, true, -1);
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        if (row < nrows) X[(long)st * g.size + row * L + x] = s[line * L + x];
-    }
-}
-
-// IFFT_0 of X then post-ops into psi (nd >= 2)
-template <class R, int S>
-__global__ void k_gspo_row_post(GridDev g, LinePlan p0, int rb, XGroupDev<R> post,
-                                const cx<R> *__restrict__ X, cx<R> *__restrict__ psi) {
-    extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    const int L = g.N[0];
-    const long nrows = g.size / L;
-    const long row0 = (long)blockIdx.x * rb;
-    const int nl = S * rb;
-    for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
-        const int line = e / L, x = e % L;
-        const int st = line / rb, rr = line % rb;
-        const long row = row0 + rr;
-        cx<R> v = mk<R>(0, 0);
-        if (row < nrows) v = X[(long)st * g.size + row * L + x];
-        s[line * L + x] = v;
-    }
-    __syncthreads();
-    fft_lines<R>(s, p0, nl, true, +1);
-    for (int e = threadIdx.x; e < rb * L; e += blockDim.x) {
-        const int rr = e / L, x = e % L;
-        const long row = row0 + rr;
-        if (row >= nrows) continue;
-        const long gi = row * L + x;
-        cx<double> v[S];
-#pragma unroll
-        for (int st = 0; st < S; ++st) v[st] = to_d(s[(st * rb + rr) * L + x]);
-        xgroup_point<R, S>(post, gi, v);
-#pragma unroll
-        for (int st = 0; st < S; ++st) psi[(long)st * g.size + gi] = from_d<R>(v[st]);
-    }
-}
-
-// whole GSPO step for nd == 1 (pre, FFT, exp(-i tau T)/N, IFFT, post)
-template <class R, int S>
-__global__ void k_gspo_1d(GridDev g, LinePlan p0, HamDev<R> kin, double tau, XGroupDev<R> pre,
-                          XGroupDev<R> post, cx<R> *__restrict__ psi) {
-    extern __shared__ __align__(16) char smem_raw[];
-    cx<R> *s = reinterpret_cast<cx<R> *>(smem_raw);
-    const int L = g.N[0];
-    for (int x = threadIdx.x; x < L; x += blockDim.x) {
-        cx<double> v[S];
-#pragma unroll
-        for (int st = 0; st < S; ++st) v[st] = to_d(psi[(long)st * g.size + x]);
-        xgroup_point<R, S>(pre, x, v);
-#pragma unroll
-        for (int st = 0; st < S; ++st) s[st * L + x] = from_d<R>(v[st]);
-    }
-    __syncthreads();
-    fft_lines<R>(s, p0, S, true, -1);
-    for (int e = threadIdx.x; e < S * L; e += blockDim.x) {
-        const int st = e / L, x = e % L;
-        if (!kin.tk[st][0]) {
-            // no kinetic term on this state: exp(0)/N keeps the round trip exact
-            cx<R> v = s[e];
-            s[e] = from_d<R>(mk<double>((double)v.r * kin.inv_n, (double)v.i * kin.inv_n));
-            continue;
+            for (int q = 0; q < 8; ++q) xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL] = v[q];
         }
-        const double tt = 0.0 + kin.tk[st][0][x];
-        double sn, cs;
-        sincos(-tau * tt, &sn, &cs);
-        s[e] = from_d<R>(cmul(to_d(s[e]), mk<double>(cs * kin.inv_n, sn * kin.inv_n)));
+        __syncthreads();
+        // the points of row rr are shared out among the S lines' threads
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int x = B.lt + q * B.TL;
+                if (x % S != st) continue;
+                cx<double> w[S];
+#pragma unroll
+                for (int s2 = 0; s2 < S; ++s2) w[s2] = to_d(xa[((long)s2 * B.rb + B.rr) * B.L + x]);
+                xgroup_point<R, S>(grp, B.row * B.L + x, w);
+#pragma unroll
+                for (int s2 = 0; s2 < S; ++s2) xa[((long)s2 * B.rb + B.rr) * B.L + x] = from_d<R>(w[s2]);
+            }
+        }
+        __syncthreads();
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL];
+        }
+    };
+    if (PHASE == 0 || ONE_D) {
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = psi[base + B.gidx(q)];
+        }
+        xspace(pre);
+        B.fft(v, p0, -1);
+        if (!ONE_D) {
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) X[base + B.gidx(q)] = v[q];
+            }
+            return;
+        }
+        if (B.active) {
+            const bool has_t = kin.tk[st][0] != nullptr;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double tt = has_t ? 0.0 + kin.tk[st][0][B.lt + q * B.TL] : 0.0;
+                double sn, cs;
+                sincos(-tau * tt, &sn, &cs);
+                v[q] = from_d<R>(cmul(to_d(v[q]), mk<double>(cs * kin.inv_n, sn * kin.inv_n)));
+            }
+        }
+    } else {
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = X[base + B.gidx(q)];
+        }
     }
-    __syncthreads();
-    fft_lines<R>(s, p0, S, true, +1);
-    for (int x = threadIdx.x; x < L; x += blockDim.x) {
-        cx<double> v[S];
+    B.fft(v, p0, +1);
+    xspace(post);
+    if (B.active) {
 #pragma unroll
-        for (int st = 0; st < S; ++st) v[st] = to_d(s[st * L + x]);
-        xgroup_point<R, S>(post, x, v);
-#pragma unroll
-        for (int st = 0; st < S; ++st) psi[(long)st * g.size + x] = from_d<R>(v[st]);
+        for (int q = 0; q < 8; ++q) psi[base + B.gidx(q)] = v[q];
     }
 }
 
@@ -591,185 +660,83 @@ __global__ void k_c64_to_c128(const float *__restrict__ s, double *__restrict__ 
 
 static int round64(int n) {
     const int r = ((n + 63) / 64) * 64;
-    if (r > 1024) throw std::runtime_error("line block needs more than 1024 threads");
+    if (r > 512) throw std::runtime_error("line block needs more than 512 threads");
     return r;
 }
 
-// rows per block so that a row block has >= 256 threads of butterflies, but
-// the grid keeps at least ~1024 blocks where the problem allows it
+// rows per block: grow the block towards 256 threads while the grid keeps at
+// least 512 blocks
 static int rows_per_block(const GridDev &g, int S, int L) {
     const long nrows = g.size / L;
-    const int TL = (L + 7) / 8;
+    const int TL = L / 8;
     int rb = 1;
-    while (rb * 2 <= nrows && S * rb * 2 * TL <= 256 && nrows / (rb * 2) >= 1024) rb *= 2;
+    while (rb * 2 <= nrows && S * rb * 2 * TL <= 256 && nrows / (rb * 2) >= 512) rb *= 2;
     return rb;
 }
 
+constexpr size_t kLdsBytes = 160 * 1024;
+
 template <class K>
 static void set_lds(K kernel, size_t bytes) {
-    if (bytes > 160 * 1024) throw std::runtime_error("line block exceeds the 160 KiB LDS of a gfx950 CU");
+    if (bytes > kLdsBytes) throw std::runtime_error("line block exceeds the 160 KiB LDS of a gfx950 CU");
     if (bytes > 65536)
         QDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
-static int col_tile(const GridDev &g, int L, size_t elem) {
-    // widest tile (<= 16 lanes, <= 64 KiB LDS) that still leaves >= 2048 blocks
+// tile width of the strided passes: adjacent dimension-0 positions per block.
+// Prefer >= 64-byte segments, keep >= 256 blocks when possible.
+static int col_tile(const GridDev &g, int S, int dim, int L, size_t elem) {
+    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     int W = 16;
-    while (W > 1 && ((size_t)W * L * elem > 65536 || g.N[0] % W != 0)) W /= 2;
-    while (W > 1 && (g.size / L) / W < 2048) W /= 2;
-    if (g.N[0] % W != 0) W = 1;
+    auto fits = [&](int w) {
+        return g.N[0] % w == 0 && ((size_t)2 * w * L + L) * elem <= 72 * 1024 && w * (L / 8) <= 512;
+    };
+    while (W > 1 && !fits(W)) W /= 2;
+    const int wmin = std::max(1, (int)(64 / elem));
+    while (W > wmin && (long)(g.N[0] / W) * nother * S < 256 && fits(W / 2)) W /= 2;
     return W;
 }
 
-template <class R, int S>
-static void cheby_init_S(hipStream_t st, const GridDev &g, const LinePlan &p0, const double *psi, R *phi0,
-                         R *X, ChebyCtl *ctl, double *partials) {
-    const int L = g.N[0];
-    const int rb = rows_per_block(g, S, L);
-    const long nrows = g.size / L;
-    const int nb = (int)((nrows + rb - 1) / rb);
-    if (nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
-    const int threads = std::max(64, round64(S * rb * ((L + 7) / 8)));
-    const size_t lds = (size_t)S * rb * L * sizeof(cx<R>);
-    set_lds(k_cheby_init<R, S>, lds);
-    hipLaunchKernelGGL((k_cheby_init<R, S>), dim3(nb), dim3(threads), lds, st, g, p0, rb,
-                       reinterpret_cast<const cx<double> *>(psi), reinterpret_cast<cx<R> *>(phi0),
-                       reinterpret_cast<cx<R> *>(X), ctl, partials);
-    QDX_HIP(hipGetLastError());
-}
+struct RowLaunch {
+    int rb, nb, threads;
+    size_t lds;
+};
 
 template <class R>
-void Launch<R>::cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const double *psi,
-                           R *phi0, R *X, ChebyCtl *ctl, double *partials) {
-    switch (S) {
-    case 1: cheby_init_S<R, 1>(s, g, p0, psi, phi0, X, ctl, partials); break;
-    case 2: cheby_init_S<R, 2>(s, g, p0, psi, phi0, X, ctl, partials); break;
-    case 3: cheby_init_S<R, 3>(s, g, p0, psi, phi0, X, ctl, partials); break;
-    case 4: cheby_init_S<R, 4>(s, g, p0, psi, phi0, X, ctl, partials); break;
-    default: throw std::runtime_error("unsupported number of states");
-    }
-}
-
-template <class R, int S, bool ONE_D, bool PRO>
-static void cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                         const ChebyOrder &ord, const R *phim1, R *phim2, R *X, double *psi, ChebyCtl *ctl,
-                         double *partials) {
+static RowLaunch row_launch(const GridDev &g, int S, LinePlan &p0, bool extra_exchange = false) {
     const int L = g.N[0];
-    const int rb = rows_per_block(g, S, L);
+    RowLaunch r;
+    r.rb = rows_per_block(g, S, L);
     const long nrows = g.size / L;
-    const int nb = (int)((nrows + rb - 1) / rb);
-    if (nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
-    const int threads = std::max(64, round64(S * rb * ((L + 7) / 8)));
-    const size_t lds = (size_t)S * rb * L * sizeof(cx<R>);
-    set_lds(k_cheby_row<R, S, ONE_D, PRO>, lds);
-    hipLaunchKernelGGL((k_cheby_row<R, S, ONE_D, PRO>), dim3(nb), dim3(threads), lds, st, g, h, p0, ord, rb,
-                       reinterpret_cast<const cx<R> *>(phim1), reinterpret_cast<cx<R> *>(phim2),
-                       reinterpret_cast<cx<R> *>(X), reinterpret_cast<cx<double> *>(psi), ctl, partials);
-    QDX_HIP(hipGetLastError());
+    r.nb = (int)((nrows + r.rb - 1) / r.rb);
+    r.threads = std::max(64, round64(S * r.rb * (L / 8)));
+    const size_t extra = extra_exchange ? (size_t)S * r.rb * L * sizeof(cx<R>) : 0;
+    // keep the twiddle table in LDS when it fits next to the line buffers
+    p0.tw_lds = row_lds_bytes<R>(L, S * r.rb, true) + extra <= kLdsBytes ? 1 : 0;
+    r.lds = row_lds_bytes<R>(L, S * r.rb, p0.tw_lds != 0) + extra;
+    return r;
 }
 
 template <class R, int S>
-static void cheby_row_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                        const ChebyOrder &ord, const R *phim1, R *phim2, R *X, double *psi, ChebyCtl *ctl,
-                        double *partials) {
-    const bool one = g.nd == 1;
-    const bool pro = ord.k == 1;
-    if (one && pro) cheby_row_SK<R, S, true, true>(st, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials);
-    else if (one) cheby_row_SK<R, S, true, false>(st, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials);
-    else if (pro) cheby_row_SK<R, S, false, true>(st, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials);
-    else cheby_row_SK<R, S, false, false>(st, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials);
-}
-
-template <class R>
-void Launch<R>::cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                          const ChebyOrder &ord, const R *phim1, R *phim2, R *X, double *psi, ChebyCtl *ctl,
-                          double *partials) {
-    switch (h.S) {
-    case 1: cheby_row_S<R, 1>(s, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials); break;
-    case 2: cheby_row_S<R, 2>(s, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials); break;
-    case 3: cheby_row_S<R, 3>(s, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials); break;
-    case 4: cheby_row_S<R, 4>(s, g, h, p0, ord, phim1, phim2, X, psi, ctl, partials); break;
-    default: throw std::runtime_error("unsupported number of states");
-    }
-}
-
-template <class R>
-void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &p,
-                         const HamDev<R> &h, double tau, R *X, const ChebyCtl *ctl) {
-    const int L = g.N[dim];
-    const int W = col_tile(g, L, sizeof(cx<R>));
-    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
-    const int nb = (g.N[0] / W) * nother * S;
-    const int threads = std::max(64, round64(W * ((L + 7) / 8)));
-    const size_t lds = (size_t)W * L * sizeof(cx<R>);
-    cx<R> *x = reinterpret_cast<cx<R> *>(X);
-    switch (mode) {
-    case 0:
-        set_lds(k_col<R, 0>, lds);
-        hipLaunchKernelGGL((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, p, h, tau, x, ctl);
-        break;
-    case 1:
-        set_lds(k_col<R, 1>, lds);
-        hipLaunchKernelGGL((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, p, h, tau, x, ctl);
-        break;
-    case 2:
-        set_lds(k_col<R, 2>, lds);
-        hipLaunchKernelGGL((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, p, h, tau, x, ctl);
-        break;
-    default:
-        set_lds(k_col<R, 3>, lds);
-        hipLaunchKernelGGL((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, p, h, tau, x, ctl);
-        break;
-    }
-    QDX_HIP(hipGetLastError());
-}
-
-template <class R, int S>
-static void ham_apply_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan *plans,
-                        const double *src, double *dst, R *X) {
-    const int L = g.N[0];
-    const int rb = rows_per_block(g, S, L);
-    const long nrows = g.size / L;
-    const int nb = (int)((nrows + rb - 1) / rb);
-    const int threads = std::max(64, round64(S * rb * ((L + 7) / 8)));
-    const size_t lds = (size_t)S * rb * L * sizeof(cx<R>);
-    const cx<double> *s = reinterpret_cast<const cx<double> *>(src);
-    cx<double> *d = reinterpret_cast<cx<double> *>(dst);
-    if (g.nd == 1) {
-        set_lds(k_apply_row<R, S, true>, lds);
-        hipLaunchKernelGGL((k_apply_row<R, S, true>), dim3(nb), dim3(threads), lds, st, g, h, plans[0], rb, s,
-                           reinterpret_cast<cx<R> *>(X), d);
-        QDX_HIP(hipGetLastError());
-        return;
-    }
-    set_lds(k_row_fwd<R, S>, lds);
-    hipLaunchKernelGGL((k_row_fwd<R, S>), dim3(nb), dim3(threads), lds, st, g, plans[0], rb, s,
-                       reinterpret_cast<cx<R> *>(X));
-    QDX_HIP(hipGetLastError());
-    if (g.nd == 2) {
-        Launch<R>::col_pass(st, g, S, 1, 2, plans[1], h, 0.0, X, nullptr);
+static int cheby_init_S(hipStream_t st, const GridDev &g, const LinePlan &plan, const double *psi, R *phi0,
+                        R *X, ChebyCtl *ctl, double *partials) {
+    LinePlan p0 = plan;
+    RowLaunch r = row_launch<R>(g, S, p0);
+    if (r.nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
+    auto cpsi = reinterpret_cast<const cx<double> *>(psi);
+    auto cphi = reinterpret_cast<cx<R> *>(phi0);
+    auto cx_ = reinterpret_cast<cx<R> *>(X);
+    if (X) {
+        set_lds(k_cheby_init<R, S, true>, r.lds);
+        hipLaunchKernelGGL((k_cheby_init<R, S, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, p0, r.rb, cpsi,
+                           cphi, cx_, ctl, partials);
     } else {
-        Launch<R>::col_pass(st, g, S, 1, 0, plans[1], h, 0.0, X, nullptr);
-        Launch<R>::col_pass(st, g, S, 2, 2, plans[2], h, 0.0, X, nullptr);
-        Launch<R>::col_pass(st, g, S, 1, 1, plans[1], h, 0.0, X, nullptr);
+        hipLaunchKernelGGL((k_cheby_init<R, S, false>), dim3(r.nb), dim3(r.threads), 0, st, g, p0, r.rb, cpsi, cphi,
+                           cx_, ctl, partials);
     }
-    set_lds(k_apply_row<R, S, false>, lds);
-    hipLaunchKernelGGL((k_apply_row<R, S, false>), dim3(nb), dim3(threads), lds, st, g, h, plans[0], rb, s,
-                       reinterpret_cast<cx<R> *>(X), d);
     QDX_HIP(hipGetLastError());
-}
-
-template <class R>
-void Launch<R>::ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan *plans,
-                          const double *src, double *dst, R *X) {
-    switch (h.S) {
-    case 1: ham_apply_S<R, 1>(s, g, h, plans, src, dst, X); break;
-    case 2: ham_apply_S<R, 2>(s, g, h, plans, src, dst, X); break;
-    case 3: ham_apply_S<R, 3>(s, g, h, plans, src, dst, X); break;
-    case 4: ham_apply_S<R, 4>(s, g, h, plans, src, dst, X); break;
-    default: throw std::runtime_error("unsupported number of states");
-    }
+    return r.nb;
 }
 
 #define QDX_DISPATCH_S(S_, CALL)                                              \
@@ -782,49 +749,155 @@ void Launch<R>::ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h, c
     }
 
 template <class R>
-void Launch<R>::gspo_row_pre(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const XGroupDev<R> &pre,
-                             const R *psi, R *X) {
-    const int L = g.N[0];
+int Launch<R>::cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const double *psi, R *phi0,
+                          R *X, ChebyCtl *ctl, double *partials) {
+    int nb = 0;
+    QDX_DISPATCH_S(S, (nb = cheby_init_S<R, SS>(s, g, p0, psi, phi0, X, ctl, partials)));
+    return nb;
+}
+
+template <class R, int S, bool ONE_D, bool PRO>
+static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &plan,
+                        const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                        ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
+    LinePlan p0 = plan;
+    RowLaunch r = row_launch<R>(g, S, p0);
+    if (r.nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
+    set_lds(k_cheby_row<R, S, ONE_D, PRO>, r.lds);
+    hipLaunchKernelGGL((k_cheby_row<R, S, ONE_D, PRO>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
+                       r.rb, reinterpret_cast<const cx<R> *>(phim1), reinterpret_cast<cx<R> *>(phim2),
+                       reinterpret_cast<cx<R> *>(X), reinterpret_cast<cx<double> *>(psi), ctl, partials, init_part,
+                       nb_init);
+    QDX_HIP(hipGetLastError());
+    return r.nb;
+}
+
+template <class R, int S>
+static int cheby_row_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
+                       const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                       ChebyCtl *ctl, double *partials, const double *ip, int ni) {
+    const bool one = g.nd == 1;
+    const bool pro = ord.k == 1;
+    if (one && pro)
+        return cheby_row_SK<R, S, true, true>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
+    if (one) return cheby_row_SK<R, S, true, false>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
+    if (pro) return cheby_row_SK<R, S, false, true>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
+    return cheby_row_SK<R, S, false, false>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
+}
+
+template <class R>
+int Launch<R>::cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
+                         const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                         ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
+    int nb = 0;
+    QDX_DISPATCH_S(h.S, (nb = cheby_row_S<R, SS>(s, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials,
+                                                 init_part, nb_init)));
+    return nb;
+}
+
+template <class R>
+void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
+                         const HamDev<R> &h, double tau, R *X, const ChebyCtl *ctl, const ConvCheck &cc) {
+    const int L = g.N[dim];
+    const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
+    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
+    const int nb = (g.N[0] / W) * nother * S;
+    const int threads = std::max(64, round64(W * (L / 8)));
+    const size_t lds = ((size_t)2 * W * L + L) * sizeof(cx<R>);
+    cx<R> *x = reinterpret_cast<cx<R> *>(X);
+    LinePlan pl = plan;
+    pl.tw_lds = 1;
+    switch (mode) {
+    case 0:
+        set_lds(k_col<R, 0>, lds);
+        hipLaunchKernelGGL((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        break;
+    case 1:
+        set_lds(k_col<R, 1>, lds);
+        hipLaunchKernelGGL((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        break;
+    case 2:
+        set_lds(k_col<R, 2>, lds);
+        hipLaunchKernelGGL((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        break;
+    default:
+        set_lds(k_col<R, 3>, lds);
+        hipLaunchKernelGGL((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        break;
+    }
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R, int S>
+static void ham_apply_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan *plans_in,
+                        const double *src, double *dst, R *X) {
+    LinePlan plans[3] = {plans_in[0], g.nd > 1 ? plans_in[1] : LinePlan(), g.nd > 2 ? plans_in[2] : LinePlan()};
+    RowLaunch r = row_launch<R>(g, S, plans[0]);
+    auto s = reinterpret_cast<const cx<double> *>(src);
+    auto d = reinterpret_cast<cx<double> *>(dst);
+    auto x = reinterpret_cast<cx<R> *>(X);
+    if (g.nd == 1) {
+        set_lds(k_apply_row<R, S, true>, r.lds);
+        hipLaunchKernelGGL((k_apply_row<R, S, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s,
+                           x, d);
+        QDX_HIP(hipGetLastError());
+        return;
+    }
+    set_lds(k_row_fwd<R, S>, r.lds);
+    hipLaunchKernelGGL((k_row_fwd<R, S>), dim3(r.nb), dim3(r.threads), r.lds, st, g, plans[0], r.rb, s, x);
+    QDX_HIP(hipGetLastError());
+    if (g.nd == 2) {
+        Launch<R>::col_pass(st, g, S, 1, 2, plans[1], h, 0.0, X, nullptr, ConvCheck());
+    } else {
+        Launch<R>::col_pass(st, g, S, 1, 0, plans[1], h, 0.0, X, nullptr, ConvCheck());
+        Launch<R>::col_pass(st, g, S, 2, 2, plans[2], h, 0.0, X, nullptr, ConvCheck());
+        Launch<R>::col_pass(st, g, S, 1, 1, plans[1], h, 0.0, X, nullptr, ConvCheck());
+    }
+    set_lds(k_apply_row<R, S, false>, r.lds);
+    hipLaunchKernelGGL((k_apply_row<R, S, false>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s, x,
+                       d);
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void Launch<R>::ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan *plans,
+                          const double *src, double *dst, R *X) {
+    QDX_DISPATCH_S(h.S, (ham_apply_S<R, SS>(s, g, h, plans, src, dst, X)));
+}
+
+template <class R, bool ONE_D, int PHASE>
+static void gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                            const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X) {
     QDX_DISPATCH_S(S, {
-        const int rb = rows_per_block(g, SS, L);
-        const int nb = (int)((g.size / L + rb - 1) / rb);
-        const int threads = std::max(64, round64(SS * rb * ((L + 7) / 8)));
-        const size_t lds = (size_t)SS * rb * L * sizeof(cx<R>);
-        set_lds(k_gspo_row_pre<R, SS>, lds);
-        hipLaunchKernelGGL((k_gspo_row_pre<R, SS>), dim3(nb), dim3(threads), lds, s, g, p0, rb, pre,
-                           reinterpret_cast<const cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
+        LinePlan p0 = plan;
+        RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
+        set_lds(k_gspo_row<R, SS, ONE_D, PHASE>, r.lds);
+        hipLaunchKernelGGL((k_gspo_row<R, SS, ONE_D, PHASE>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, p0,
+                           r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
     });
     QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void Launch<R>::gspo_row_pre(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const XGroupDev<R> &pre,
+                             const R *psi, R *X) {
+    HamDev<R> none;
+    XGroupDev<R> empty;
+    gspo_row_launch<R, false, 0>(s, g, S, none, 0.0, p0, pre, empty, const_cast<R *>(psi), X);
 }
 
 template <class R>
 void Launch<R>::gspo_row_post(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const XGroupDev<R> &post,
                               const R *X, R *psi) {
-    const int L = g.N[0];
-    QDX_DISPATCH_S(S, {
-        const int rb = rows_per_block(g, SS, L);
-        const int nb = (int)((g.size / L + rb - 1) / rb);
-        const int threads = std::max(64, round64(SS * rb * ((L + 7) / 8)));
-        const size_t lds = (size_t)SS * rb * L * sizeof(cx<R>);
-        set_lds(k_gspo_row_post<R, SS>, lds);
-        hipLaunchKernelGGL((k_gspo_row_post<R, SS>), dim3(nb), dim3(threads), lds, s, g, p0, rb, post,
-                           reinterpret_cast<const cx<R> *>(X), reinterpret_cast<cx<R> *>(psi));
-    });
-    QDX_HIP(hipGetLastError());
+    HamDev<R> none;
+    XGroupDev<R> empty;
+    gspo_row_launch<R, false, 1>(s, g, S, none, 0.0, p0, empty, post, psi, const_cast<R *>(X));
 }
 
 template <class R>
 void Launch<R>::gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const HamDev<R> &kin,
                         double tau, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi) {
-    const int L = g.N[0];
-    QDX_DISPATCH_S(S, {
-        const int threads = std::max(64, round64(SS * ((L + 7) / 8)));
-        const size_t lds = (size_t)SS * L * sizeof(cx<R>);
-        set_lds(k_gspo_1d<R, SS>, lds);
-        hipLaunchKernelGGL((k_gspo_1d<R, SS>), dim3(1), dim3(threads), lds, s, g, p0, kin, tau, pre, post,
-                           reinterpret_cast<cx<R> *>(psi));
-    });
-    QDX_HIP(hipGetLastError());
+    gspo_row_launch<R, true, 0>(s, g, S, kin, tau, p0, pre, post, psi, nullptr);
 }
 
 template <class R>

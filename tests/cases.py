@@ -129,7 +129,9 @@ class Problem:
     def gpu_wf(self, ctx, psi=None):
         import qdng_amd as Q
         w = Q.WaveFunction(ctx, self.N, self.xmin, self.xmax, self.S)
-        w.upload(self.psi0 if psi is None else psi)
+        data = self.psi0 if psi is None else psi
+        if data is not None:
+            w.upload(data)
         return w
 
 

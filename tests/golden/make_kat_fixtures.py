@@ -82,9 +82,13 @@ def main(root):
         if name not in golden:
             continue
         toks = line.split()
+        # a trailing [0-9]* in the grep pattern means "printed value starts
+        # with these digits" (truncation); otherwise the digits are complete
         vals = [t.replace("[0-9]*", "").replace("[0-9]", "") for t in toks]
+        prefix = [("[0-9]" in t) for t in toks]
         golden[name]["expec_post"] = {"time": float(vals[0]),
-                                      "values": [v for v in vals[1:]]}
+                                      "values": [v for v in vals[1:]],
+                                      "prefix": prefix[1:]}
 
     np.savez_compressed(os.path.join(HERE, "qdng_kat.npz"), **arrays)
     with open(os.path.join(HERE, "qdng_kat.json"), "w") as f:

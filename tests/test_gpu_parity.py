@@ -1,0 +1,294 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Bars: FP64 path <= 1e-10 relative L2 against the oracle on the same inputs
+(the H application itself to ~1e-13); FP32-bulk path <= 1e-6 relative L2 on
+the short precision-variant runs; identical Chebyshev series lengths per step
+(the break test of OCheby.cpp:179-182 is evaluated on the device).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+def note(msg):
+    print("[gpu case]", msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(autouse=True)
+def _fresh():
+    O.reset()
+    yield
+    O.reset()
+
+
+# ---------------------------------------------------------------------------
+# H psi for every supported line length (FFT parity)
+
+SIZES_1D = [8, 16, 24, 32, 48, 64, 72, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096]
+
+
+@pytest.mark.parametrize("N", SIZES_1D)
+def test_apply_1d_all_lengths(ctx, N):
+    note(f"apply 1D N={N}")
+    p = cases.harmonic_1d(N, L=6.0)
+    rng = np.random.default_rng(N)
+    psi = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    H = p.gpu_ham(ctx)
+    src, dst = p.gpu_wf(ctx, psi), p.gpu_wf(ctx)
+    H.apply(dst, src)
+    assert rel(dst.download(), p.oracle_ham().apply(psi)) < 1e-12
+
+
+@pytest.mark.parametrize("dims,S", [([64, 48], 1), ([32, 96], 2), ([128, 128], 2), ([24, 16, 32], 1),
+                                    ([32, 32, 32], 2), ([16, 64, 8], 1)])
+def test_apply_multi_d(ctx, dims, S):
+    note(f"apply dims={dims} S={S}")
+    p = cases.Problem(dims, [-4.0] * len(dims), [4.0] * len(dims), nstates=S)
+    ax = p.axes()
+    r2 = sum(a ** 2 for a in ax)
+    for s in range(S):
+        p.kinetic(s, [1000.0 + 500 * d + 100 * s for d in range(len(dims))])
+        p.potential(s, s, 0.05 * r2 + 0.01 * s)
+    if S == 2:
+        p.potential(0, 1, 0.02 * np.exp(-r2))
+    rng = np.random.default_rng(7)
+    psi = rng.standard_normal(p.size * S) + 1j * rng.standard_normal(p.size * S)
+    H = p.gpu_ham(ctx)
+    src, dst = p.gpu_wf(ctx, psi), p.gpu_wf(ctx)
+    H.apply(dst, src)
+    assert rel(dst.download(), p.oracle_ham().apply(psi)) < 1e-12
+
+
+def test_apply_dipole_coupling_and_unity(ctx):
+    note("apply: dipole coupling, missing diagonal acts as identity")
+    p = cases.Problem([64], [-3.0], [3.0], nstates=3)
+    x = p.axes()[0]
+    p.kinetic(0, 900.0).potential(0, 0, 0.1 * x ** 2)
+    p.potential(2, 2, 0.2 + 0.0 * x)                     # state 1 has no diagonal term
+    laser = np.sin(0.3 * np.arange(20))
+    p.dipole(1, 0, x * np.exp(-x ** 2), laser, 0.7)
+    p.dipole(2, 2, x, laser[::-1].copy(), 1.0)
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal(p.size * 3) + 1j * rng.standard_normal(p.size * 3)
+    H = p.gpu_ham(ctx)
+    Ho = p.oracle_ham()
+    src, dst = p.gpu_wf(ctx, psi), p.gpu_wf(ctx)
+    for step in (0, 7, 19):
+        H.apply(dst, src, step)
+        assert rel(dst.download(), Ho.apply(psi, step)) < 1e-12
+    assert H.expec(src, 4) == pytest.approx(Ho.expec(psi, 4), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# observables
+
+def test_norm_overlap_normalize(ctx):
+    note("observables")
+    p = cases.morse_2state_2d(64)
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal(p.size * 2) + 1j * rng.standard_normal(p.size * 2)
+    b = rng.standard_normal(p.size * 2) + 1j * rng.standard_normal(p.size * 2)
+    wa, wb = p.gpu_wf(ctx, a), p.gpu_wf(ctx, b)
+    g = p.oracle_grid()
+    n, per = wa.norm()
+    assert n ** 2 == pytest.approx(O.norm2(g, a, 2), rel=1e-13)
+    assert per[1] == pytest.approx(O.norm2(g, a[p.size:]), rel=1e-13)
+    assert wa.overlap(wb) == pytest.approx(O.overlap(g, a, b, 2), rel=1e-13)
+    wa.normalize()
+    assert wa.norm()[0] == pytest.approx(1.0, rel=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# Chebyshev parity
+
+def cheby_pair(ctx, p, dt, prec, precision="fp64", steps=3):
+    import qdng_amd as Q
+    U = Q.Cheby(p.gpu_ham(ctx), dt, prec=prec, precision=precision)
+    Uo = O.Cheby(p.oracle_ham(), dt, prec=prec)
+    assert U.order == Uo.order
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    counts = []
+    for i in range(steps):
+        st = U.apply(w, i)
+        n = Uo.step(psi, i)
+        counts.append((st.n_hpsi, n))
+    return w.download(), psi, counts
+
+
+@pytest.mark.parametrize("name,builder,dt,prec,steps", [
+    ("harmonic_1d", lambda: cases.harmonic_1d(256), 10.0, 1e-12, 20),
+    ("laser_1d", lambda: cases.laser_1d(512), 1.0, 1e-12, 40),
+    ("morse_2state_2d", lambda: cases.morse_2state_2d(96), 5.0, 1e-10, 4),
+    ("anharmonic_3d", lambda: cases.anharmonic_3d(32), 3.5, 1e-10, 3),
+    ("anharmonic_3d_long", lambda: cases.anharmonic_3d(24), 37.0, 1e-10, 1),
+])
+def test_cheby_fp64_parity(ctx, name, builder, dt, prec, steps):
+    note(f"cheby fp64 {name}")
+    p = builder()
+    got, ref, counts = cheby_pair(ctx, p, dt, prec, "fp64", steps)
+    assert rel(got, ref) < 1e-10
+    assert all(a == b for a, b in counts), counts
+
+
+@pytest.mark.parametrize("name,builder,dt", [
+    ("anharmonic_3d", lambda: cases.anharmonic_3d(64), 3.5),
+    ("morse_2state_2d", lambda: cases.morse_2state_2d(128), 5.0),
+])
+def test_cheby_fp32_bulk_accuracy(ctx, name, builder, dt):
+    note(f"cheby fp32-bulk {name}")
+    p = builder()
+    got, ref, counts = cheby_pair(ctx, p, dt, 1e-10, "fp32", 3)
+    assert rel(got, ref) < 1e-6
+
+
+def test_run_matches_stepwise_and_oracle_observables(ctx):
+    note("prop_run observables")
+    import qdng_amd as Q
+    p = cases.laser_1d(256)
+    U = Q.Cheby(p.gpu_ham(ctx), 1.0, prec=1e-12)
+    Uo = O.Cheby(p.oracle_ham(), 1.0, prec=1e-12)
+    w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
+    n2, ac, nh, st = U.run(w, 0, 30, w0)
+    g = p.oracle_grid()
+    psi = p.psi0.copy()
+    for i in range(30):
+        assert n2[i] == pytest.approx(O.norm2(g, psi), rel=1e-12)
+        assert abs(ac[i] - O.overlap(g, psi, p.psi0)) < 1e-11
+        assert nh[i] == Uo.step(psi, i)
+    assert rel(w.download(), psi) < 1e-10
+    assert st.n_hpsi == nh.sum()
+
+
+# ---------------------------------------------------------------------------
+# GSPO parity
+
+def test_gspo_fp64_coupled_2d(ctx):
+    note("gspo coupled 2D")
+    import qdng_amd as Q
+    p = cases.morse_2state_2d(64)
+    G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], 5.0)
+    Go = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], 5.0, 2)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for i in range(10):
+        G.apply(w, i)
+        Go.step(psi, i)
+    assert rel(w.download(), psi) < 1e-12
+
+
+def test_gspo_fp64_laser_1d_and_3d(ctx):
+    note("gspo laser 1D, 3D")
+    import qdng_amd as Q
+    p = cases.laser_1d(384)
+    ops = [p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"]), p.gpu_split_op(ctx, ["dip"])]
+    G = Q.GSPO(ops, 1.0)
+    Go = O.GSPO([p._oracle_term(t) for t in p.terms], 1.0, 1)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for i in range(50):
+        G.apply(w, i)
+        Go.step(psi, i)
+    assert rel(w.download(), psi) < 1e-12
+    O.reset()
+    p = cases.anharmonic_3d(32)
+    G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], 3.5)
+    Go = O.GSPO([p._oracle_term(t) for t in p.terms], 3.5, 1)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for i in range(5):
+        G.apply(w, i)
+        Go.step(psi, i)
+    assert rel(w.download(), psi) < 1e-12
+
+
+def test_gspo_potential_first_ordering(ctx):
+    # A = V (full step), B = T (half steps on both sides): OGSPO.cpp:119-138
+    note("gspo A=V B=T")
+    import qdng_amd as Q
+    p = cases.harmonic_1d(128)
+    G = Q.GSPO([p.gpu_split_op(ctx, ["pot"]), p.gpu_split_op(ctx, ["kin"])], 10.0)
+    Go = O.GSPO([p._oracle_term(p.terms[1]), p._oracle_term(p.terms[0])], 10.0, 1)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for i in range(20):
+        G.apply(w, i)
+        Go.step(psi, i)
+    assert rel(w.download(), psi) < 1e-12
+
+
+def test_gspo_fp32_bulk(ctx):
+    note("gspo fp32-bulk")
+    import qdng_amd as Q
+    p = cases.morse_2state_2d(128)
+    G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], 5.0, precision="fp32")
+    Go = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], 5.0, 2)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    G.run(w, 0, 10, observables=False)
+    for i in range(10):
+        Go.step(psi, i)
+    assert rel(w.download(), psi) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# QDng regression tests end to end on the device (tests/checktests.sh)
+
+@pytest.mark.parametrize("name", cases.KAT_NAMES)
+def test_qdng_regression_on_device(ctx, name):
+    note(f"KAT {name}")
+    import qdng_amd as Q
+    p, kind, dt, steps, prec = cases.kat_problem(name)
+    gold = cases.kat_data()[1]["golden"][name]["report"]
+    w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
+    if kind == "cheby":
+        H = p.gpu_ham(ctx)
+        U = Q.Cheby(H, dt, prec=prec)
+        Hrep = H
+    else:
+        if p.S == 1:
+            ops = [p.gpu_split_op(ctx, [k]) for k in ("kin", "pot", "dip") if any(t[0] == k for t in p.terms)]
+        else:
+            ops = [p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])]
+        U = Q.GSPO(ops, dt)
+        Hrep = p.gpu_ham(ctx)
+    n2, ac, nh, st = U.run(w, 0, steps - 1, w0)
+    n, per = w.norm()
+    energy = Hrep.expec(w, steps - 1)
+    autoc = w.overlap(w0)
+    assert abs(n ** 2 - gold["population"]) < 5e-7
+    for got, want in zip(per if p.S > 1 else [], gold["pops"]):
+        assert abs(got - want) < 5e-7
+    assert abs(energy - gold["energy"]) < 6e-9, (energy, gold["energy"])
+    assert abs(autoc.real - gold["autocorr"][0]) < 6e-9
+    assert abs(autoc.imag - gold["autocorr"][1]) < 6e-9
+
+
+# ---------------------------------------------------------------------------
+# error paths
+
+def test_incompatible_wave_function_is_rejected(ctx):
+    note("error paths")
+    import qdng_amd as Q
+    p = cases.harmonic_1d(128)
+    U = Q.Cheby(p.gpu_ham(ctx), 10.0)
+    q = cases.harmonic_1d(256)
+    with pytest.raises(Q.QdxError):
+        U.apply(q.gpu_wf(ctx), 0)
+    lp = cases.laser_1d(128, nt=10)
+    U2 = Q.Cheby(lp.gpu_ham(ctx), 1.0)
+    w = lp.gpu_wf(ctx)
+    with pytest.raises(Q.QdxError):
+        U2.apply(w, 10)           # clock step beyond the laser field (Laser.cpp:110-112)
