@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X grid-propagation path (QDng Chebyshev / GSPO).
+
+Workload (one GPU): 2D grid of 512 x 512 points on [-10, 10]^2, two electronic
+states with masses (1836, 12000) per dimension, V1 a 2D Morse surface
+(De = 0.2, a = 1), V2 a displaced harmonic surface (k = 0.1, shift 1.5,
++0.05 Eh), coupling V12 = 0.01 exp(-r^2), both surfaces capped at 1 Eh;
+Gaussian psi0 (sigma 0.5) on state 1 centred at a seed-1234 random point in
+[-1, 1]^2.  Chebyshev propagation, dt = 5 a.u., QDng's automatic order and
+default break test (prec 1e-10), plain FP64 path.  One step = one Chebyshev
+time step of the two-state wave function, inputs resident in HBM.
+
+metric: grid-pt * H*psi / s = sum over steps of N_points * states * executed
+H applications (1 + recurrence iterations actually run), divided by the wall
+time of the timed region.
+
+Also reported (object "extra"): the FP32-bulk path on the same workload, the
+GSPO path (grid-pt * step / s), and the relative L2 difference to the FP64
+CPU oracle after a few steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "grid-pt H·ψ/s + %HBM roofline, FP32 vs FP64; p50/p99 batched-job latency"
+UNIT = "grid-pt*Hpsi/s"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per grid point per state per executed order of the row
+# kernel (streaming model): phi_{k-1} + phi_{k-2} + phi_k + psi read/write +
+# V_ii + V_ij (coupling); the prologue (order 1) reads no phi_{k-2}
+B_ROW = {"fp64": (16 + 16 + 16 + 32 + 8 + 8), "fp32": (8 + 8 + 8 + 32 + 4 + 4)}
+B_ROW_PRO = {"fp64": B_ROW["fp64"] - 16, "fp32": B_ROW["fp32"] - 8}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def sync(ctx):
+    ctx.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def build_problem(N=512):
+    import cases
+    return cases.morse_2state_2d(N)
+
+
+def run_cheby(ctx, p, precision, steps, warmup, profile):
+    import qdng_amd as Q
+    H = p.gpu_ham(ctx)
+    U = Q.Cheby(H, 5.0, prec=1e-10, precision=precision)
+    w = p.gpu_wf(ctx)
+    if warmup:
+        U.run(w, 0, warmup, observables=False)
+    sync(ctx)
+    if profile:
+        ctx.profile(True)
+    t0 = time.perf_counter()
+    _, _, nh, st = U.run(w, warmup, steps, observables=False)
+    sync(ctx)
+    t1 = time.perf_counter()
+    rowk = ctx.kernel_time(Q.K_CHEBY_ROW) if profile else None
+    linek = ctx.kernel_time(Q.K_LINE_PASS) if profile else None
+    if profile:
+        ctx.profile(False)
+    return {"seconds": t1 - t0, "n_hpsi": nh, "order": U.order, "row": rowk, "line": linek}
+
+
+def run_gspo(ctx, p, precision, steps, warmup):
+    import qdng_amd as Q
+    G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], 5.0, precision=precision)
+    w = p.gpu_wf(ctx)
+    if warmup:
+        G.run(w, 0, warmup, observables=False)
+    sync(ctx)
+    t0 = time.perf_counter()
+    G.run(w, warmup, steps, observables=False)
+    sync(ctx)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(p, steps):
+    """FP64 oracle on the host cores for `steps` Chebyshev steps from psi0."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    O.reset()
+    U = O.Cheby(p.oracle_ham(), 5.0, prec=1e-10)
+    psi = p.psi0.copy()
+    t0 = time.perf_counter()
+    nh = 0
+    for i in range(steps):
+        nh += U.step(psi, i)
+    dt = time.perf_counter() - t0
+    return psi, nh, dt, threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import qdng_amd as Q
+    ctx = Q.Context(0)
+    p = build_problem(512)
+    npts = p.size * p.S
+    log(f"workload 512x512 x 2 states, {args.warmup} warmup + {args.steps} timed steps")
+
+    # ---- headline: FP64 Chebyshev.  Pass 1 times the steps; pass 2 repeats
+    # them with HIP events on every launch (attached to the dispatch packets)
+    # for the per-kernel roofline.
+    r = run_cheby(ctx, p, "fp64", args.steps, args.warmup, profile=False)
+    nh = r["n_hpsi"]
+    hpsi = int(nh.sum())
+    value = npts * hpsi / r["seconds"]
+    rp = run_cheby(ctx, p, "fp64", args.steps, args.warmup, profile=True)
+    assert int(rp["n_hpsi"].sum()) == hpsi
+    launches, row_ms = rp["row"]
+    row_bytes = p.size * p.S * (B_ROW["fp64"] * (hpsi - args.steps) + B_ROW_PRO["fp64"] * args.steps)
+    achieved = row_bytes / (row_ms * 1e-3) / 1e9
+    log(f"fp64: {value:.4g} grid-pt*Hpsi/s, {r['seconds'] * 1e3 / args.steps:.3f} ms/step, "
+        f"orders/step {nh.min()}-{nh.max()}, row kernel {launches} launches avg {row_ms / launches * 1e3:.2f} us")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["seconds"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "chebyshev_2d_512x512_2state_fp64",
+            "grid": [512, 512], "box": [-10.0, 10.0], "states": 2, "masses": [1836.0, 12000.0],
+            "dt": 5.0, "prec": 1e-10, "order": r["order"],
+            "hpsi_per_step": [int(nh.min()), int(nh.max())],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "qdx::k_cheby_row<double, 2, false, false>",
+            "launches": launches,
+            "avg_launch_us": row_ms / launches * 1e3,
+            "algorithmic_bytes_per_launch": row_bytes / launches,
+            "bytes_model": "96 B per grid point and state per executed order (80 B for order 1); "
+                           "early-exit launches after the break count 0 B",
+            "measured": "HIP events attached to every launch (hipExtLaunchKernelGGL) over a second "
+                        "timed pass of the same steps",
+            "ms_per_step_profiled_pass": rp["seconds"] * 1e3 / args.steps,
+        },
+    }
+
+    # ---- CPU baseline: FP64 oracle on a bounded sample of the same workload --
+    if args.cpu_steps > 0:
+        psi_cpu, nh_cpu, dt_cpu, threads = cpu_baseline(p, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": npts * nh_cpu / dt_cpu,
+            "unit": UNIT,
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {args.cpu_steps} Chebyshev steps of the same workload on the FP64 oracle "
+                      f"(oracle/qdx_oracle.cpp, OpenMP {threads} threads), {dt_cpu:.1f} s",
+        }
+        # parity of the same steps on the GPU
+        U = Q.Cheby(p.gpu_ham(ctx), 5.0, prec=1e-10)
+        w = p.gpu_wf(ctx)
+        U.run(w, 0, args.cpu_steps, observables=False)
+        g = w.download()
+        out.setdefault("extra", {})["l2_fp64_vs_oracle"] = float(np.linalg.norm(g - psi_cpu) / np.linalg.norm(psi_cpu))
+        if not args.no_extra:
+            U32 = Q.Cheby(p.gpu_ham(ctx), 5.0, prec=1e-10, precision="fp32")
+            w32 = p.gpu_wf(ctx)
+            U32.run(w32, 0, args.cpu_steps, observables=False)
+            g32 = w32.download()
+            out["extra"]["l2_fp32_bulk_vs_oracle"] = float(np.linalg.norm(g32 - psi_cpu) / np.linalg.norm(psi_cpu))
+
+    # ---- variants: FP32-bulk Chebyshev, GSPO -------------------------------
+    if not args.no_extra:
+        ex = out.setdefault("extra", {})
+        r32 = run_cheby(ctx, p, "fp32", args.steps, args.warmup, profile=True)
+        h32 = int(r32["n_hpsi"].sum())
+        l32, ms32 = r32["row"]
+        b32 = p.size * p.S * (B_ROW["fp32"] * (h32 - args.steps) + B_ROW_PRO["fp32"] * args.steps)
+        ex["fp32_bulk"] = {
+            "value": npts * h32 / r32["seconds"], "unit": UNIT,
+            "ms_per_step": r32["seconds"] * 1e3 / args.steps,
+            "hpsi_per_step": [int(r32["n_hpsi"].min()), int(r32["n_hpsi"].max())],
+            "row_kernel_achieved_GBs": b32 / (ms32 * 1e-3) / 1e9,
+            "row_kernel_avg_launch_us": ms32 / l32 * 1e3,
+            "vs_fp64": (npts * h32 / r32["seconds"]) / value,
+        }
+        log(f"fp32-bulk: {ex['fp32_bulk']['value']:.4g} grid-pt*Hpsi/s")
+        gsteps = max(args.steps // 4, 1)
+        for prec in ("fp64", "fp32"):
+            t = run_gspo(ctx, p, prec, gsteps, min(args.warmup, 10))
+            ex[f"gspo_{prec}"] = {"value": npts * gsteps / t, "unit": "grid-pt*step/s",
+                                  "ms_per_step": t * 1e3 / gsteps, "steps": gsteps}
+            log(f"gspo {prec}: {npts * gsteps / t:.4g} grid-pt*step/s")
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

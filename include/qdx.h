@@ -66,6 +66,21 @@ int qdx_ctx_create(int device, qdx_ctx **out);
 int qdx_ctx_synchronize(qdx_ctx *ctx);
 int qdx_ctx_destroy(qdx_ctx *ctx);
 
+/* Per-launch kernel timing with HIP events on the context's stream.  QDng
+ * itself only reports per-program wall/CPU seconds (CmdHandler.cpp:225-233);
+ * these counters serve the roofline accounting of the benchmark.
+ * qdx_ctx_profile(ctx, 1) clears the counters and starts recording,
+ * qdx_ctx_profile(ctx, 0) stops; qdx_ctx_kernel_time sums one class. */
+typedef enum qdx_kernel_class {
+    QDX_K_CHEBY_ROW = 0,    /* Chebyshev order: IFFT_0 + fused recurrence + FFT_0 */
+    QDX_K_LINE_PASS = 1,    /* strided FFT passes along dimensions >= 1          */
+    QDX_K_CHEBY_INIT = 2,   /* step start: psi -> phi_0, FFT_0, sum |psi|^2        */
+    QDX_K_GSPO_ROW = 3,     /* GSPO x-space split operators + FFT_0 / IFFT_0       */
+    QDX_NUM_KERNEL_CLASSES = 4
+} qdx_kernel_class;
+int qdx_ctx_profile(qdx_ctx *ctx, int32_t on);
+int qdx_ctx_kernel_time(qdx_ctx *ctx, int32_t kernel_class, int64_t *launches, double *total_ms);
+
 /* ---- wave functions (WaveFunction.h:34-314, WFGridCartesian.cpp:16-178,
  *      WFMultistate.cpp:43-300) ------------------------------------------- */
 int qdx_wf_create(qdx_ctx *ctx, const qdx_grid *grid, int32_t nstates, qdx_wf **out);

@@ -53,8 +53,10 @@ EXPORTS = [
     "qdx_ham_add_kinetic", "qdx_ham_add_potential", "qdx_ham_add_dipole", "qdx_ham_bounds",
     "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
     "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
-    "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy",
+    "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
 ]
+
+K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW = 0, 1, 2, 3
 
 
 def lib():
@@ -99,6 +101,8 @@ def lib():
         "qdx_prop_apply": [vp, vp, C.c_int64, C.POINTER(StepStats)],
         "qdx_prop_run": [vp, vp, C.c_int64, C.c_int64, vp, dp, dp, C.POINTER(i32), C.POINTER(StepStats)],
         "qdx_prop_destroy": [vp],
+        "qdx_ctx_profile": [vp, i32],
+        "qdx_ctx_kernel_time": [vp, i32, C.POINTER(C.c_int64), dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -142,6 +146,17 @@ class Context:
 
     def synchronize(self):
         _chk(lib().qdx_ctx_synchronize(self.h))
+
+    def profile(self, on: bool = True) -> None:
+        """Start (clearing the counters) or stop per-launch HIP-event timing."""
+        _chk(lib().qdx_ctx_profile(self.h, int(bool(on))))
+
+    def kernel_time(self, kernel_class: int):
+        """(launches, total milliseconds) of one kernel class since profile(True)."""
+        n = C.c_int64()
+        ms = C.c_double()
+        _chk(lib().qdx_ctx_kernel_time(self.h, int(kernel_class), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def close(self):
         if self.h:

@@ -183,6 +183,37 @@ struct qdx_ctx {
     DevBuf scal;       // small device result area
     double *pinned = nullptr;
 
+    // per-launch HIP-event timing (qdx_ctx_profile): a fixed pool of events;
+    // when it is used up the recorded spans are folded into the totals
+    static constexpr size_t kEventPool = 8192;
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    size_t next_event = 0;
+    std::vector<std::pair<size_t, size_t>> spans[QDX_NUM_KERNEL_CLASSES];
+    double acc_ms[QDX_NUM_KERNEL_CLASSES] = {};
+    int64_t acc_n[QDX_NUM_KERNEL_CLASSES] = {};
+    void fold() {
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int c = 0; c < QDX_NUM_KERNEL_CLASSES; ++c) {
+            for (auto &sp : spans[c]) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, events[sp.first], events[sp.second]));
+                acc_ms[c] += ms;
+                acc_n[c] += 1;
+            }
+            spans[c].clear();
+        }
+        next_event = 0;
+    }
+    // index of a fresh event; the caller records it right away
+    size_t take_event() {
+        if (events.empty()) {
+            events.resize(kEventPool);
+            for (auto &e : events) HIPCHK(hipEventCreate(&e));
+        }
+        return next_event++;
+    }
+
     LinePlan plan(int L, bool fp32) {
         auto &tab = fp32 ? tw32 : tw64;
         auto it = tab.find(L);
@@ -213,9 +244,32 @@ struct qdx_ctx {
 };
 
 namespace {
+// brackets one launch with events when the context is profiling
+// Times exactly one kernel launch: the events ride on that launch's dispatch
+// packet (hipExtLaunchKernelGGL), so the span is the kernel's own execution.
+struct KTimer {
+    qdx_ctx *c;
+    int cls;
+    size_t a = 0, b = 0;
+    KTimer(qdx_ctx *ctx, int kind) : c(ctx), cls(kind) {
+        if (!c->profiling) return;
+        if (c->next_event + 2 > qdx_ctx::kEventPool) c->fold();
+        a = c->take_event();
+        b = c->take_event();
+        g_launch_timing.start = c->events[a];
+        g_launch_timing.stop = c->events[b];
+    }
+    ~KTimer() {
+        if (!c->profiling) return;
+        if (!g_launch_timing.start) c->spans[cls].push_back({a, b});   // consumed by the launch
+        g_launch_timing = LaunchTiming();
+    }
+};
+
 void ctx_free(qdx_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     ctx->pinned = nullptr;
     (void)hipStreamDestroy(ctx->stream);
@@ -573,6 +627,7 @@ int bessel_j0(int n, double x, std::vector<double> &out) {
 }
 
 constexpr double kFp32PrecFloor = 1e-8;
+constexpr double kFp32CoeffFloor = 1e-9;
 
 struct ChebyParams {
     int order = 0, nz = 0;
@@ -664,18 +719,26 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     double *slot[2] = {part, part + kMaxPartials};
     double *init = part + 2 * kMaxPartials;
     const int S = p->S;
-    const int nb_init = Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, X, ctl, init);
+    int nb_init;
+    {
+        KTimer t(ctx, QDX_K_CHEBY_INIT);
+        nb_init = Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, X, ctl, init);
+    }
 
     ConvCheck cc;
     cc.norm0 = &ctl->norm0;
     cc.prec = p->cp.prec;
+    auto col = [&](int dim, int mode, const ConvCheck &c) {
+        KTimer t(ctx, QDX_K_LINE_PASS);
+        Launch<R>::col_pass(s, g, S, dim, mode, plans[dim], h, 0.0, X, ctl, c);
+    };
     auto cols = [&](const ConvCheck &c) {
         if (p->grid.nd == 2) {
-            Launch<R>::col_pass(s, g, S, 1, 2, plans[1], h, 0.0, X, ctl, c);
+            col(1, 2, c);
         } else if (p->grid.nd == 3) {
-            Launch<R>::col_pass(s, g, S, 1, 0, plans[1], h, 0.0, X, ctl, c);
-            Launch<R>::col_pass(s, g, S, 2, 2, plans[2], h, 0.0, X, ctl, c);
-            Launch<R>::col_pass(s, g, S, 1, 1, plans[1], h, 0.0, X, ctl, c);
+            col(1, 0, c);
+            col(2, 2, c);
+            col(1, 1, c);
         }
     };
     const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
@@ -690,7 +753,10 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     ord.ckr = c1.real();
     ord.cki = c1.imag();
     cols(cc);   // k_prev = 0: no test
-    Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, A, B, X, psi->ptr(), ctl, slot[1], init, nb_init);
+    {
+        KTimer t(ctx, QDX_K_CHEBY_ROW);
+        Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, A, B, X, psi->ptr(), ctl, slot[1], init, nb_init);
+    }
     int nb_prev = 0;
     for (int k = 2; k < p->cp.order; ++k) {
         const std::complex<double> ck = p->cp.coeff[k] * ph;
@@ -703,6 +769,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
         cols(cc);
         R *pm1 = (k - 1) % 2 == 0 ? A : B;
         R *pm2 = k % 2 == 0 ? A : B;
+        KTimer t(ctx, QDX_K_CHEBY_ROW);
         nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, pm1, pm2, X, psi->ptr(), ctl, slot[k % 2], init,
                                        nb_init);
     }
@@ -788,18 +855,29 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
         if constexpr (std::is_same<R, float>::value) kin = o.op->dev.h32;
         else kin = o.op->dev.h64;
         if (p->grid.nd == 1) {
+            KTimer t(ctx, QDX_K_GSPO_ROW);
             Launch<R>::gspo_1d(s, g, p->S, plans[0], kin, o.tau, pending, post, psi);
         } else {
             R *X = p->X.as<R>();
-            Launch<R>::gspo_row_pre(s, g, p->S, plans[0], pending, psi, X);
-            if (p->grid.nd == 2) {
-                Launch<R>::col_pass(s, g, p->S, 1, 3, plans[1], kin, o.tau, X, nullptr, ConvCheck());
-            } else {
-                Launch<R>::col_pass(s, g, p->S, 1, 0, plans[1], kin, o.tau, X, nullptr, ConvCheck());
-                Launch<R>::col_pass(s, g, p->S, 2, 3, plans[2], kin, o.tau, X, nullptr, ConvCheck());
-                Launch<R>::col_pass(s, g, p->S, 1, 1, plans[1], kin, o.tau, X, nullptr, ConvCheck());
+            {
+                KTimer t(ctx, QDX_K_GSPO_ROW);
+                Launch<R>::gspo_row_pre(s, g, p->S, plans[0], pending, psi, X);
             }
-            Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi);
+            auto col = [&](int dim, int mode) {
+                KTimer t(ctx, QDX_K_LINE_PASS);
+                Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, o.tau, X, nullptr, ConvCheck());
+            };
+            if (p->grid.nd == 2) {
+                col(1, 3);
+            } else {
+                col(1, 0);
+                col(2, 3);
+                col(1, 1);
+            }
+            {
+                KTimer t(ctx, QDX_K_GSPO_ROW);
+                Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi);
+            }
         }
         pending.n = 0;
         i = j;
@@ -865,6 +943,29 @@ int qdx_ctx_create(int device, qdx_ctx **out) {
         c->scal.alloc(4096 * sizeof(double));
         HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&c->pinned), 4096 * sizeof(double), hipHostMallocDefault));
         *out = c.release();
+    });
+}
+
+int qdx_ctx_profile(qdx_ctx *ctx, int32_t on) {
+    return guarded([&] {
+        if (!ctx) fail(QDX_ERR_PARAM, "ctx is NULL");
+        if (ctx->profiling) ctx->fold();
+        ctx->profiling = on != 0;
+        if (ctx->profiling)
+            for (int c = 0; c < QDX_NUM_KERNEL_CLASSES; ++c) {
+                ctx->acc_ms[c] = 0;
+                ctx->acc_n[c] = 0;
+            }
+    });
+}
+
+int qdx_ctx_kernel_time(qdx_ctx *ctx, int32_t kernel_class, int64_t *launches, double *total_ms) {
+    return guarded([&] {
+        if (!ctx || !launches || !total_ms) fail(QDX_ERR_PARAM, "NULL argument");
+        if (kernel_class < 0 || kernel_class >= QDX_NUM_KERNEL_CLASSES) fail(QDX_ERR_PARAM, "unknown kernel class");
+        ctx->fold();
+        *launches = ctx->acc_n[kernel_class];
+        *total_ms = ctx->acc_ms[kernel_class];
     });
 }
 
@@ -1177,8 +1278,16 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         p->ham = h;
         p->cp = cheby_params(h, dt, order, prec, scaling);
         // FP32-bulk series: sum |psi|^2 of FP32 terms cannot resolve norm
-        // changes much below 1e-9, so the break test uses at least 1e-8.
-        if (precision == QDX_FP32_BULK) p->cp.prec = std::max(p->cp.prec, kFp32PrecFloor);
+        // changes much below 1e-9, so the break test uses at least 1e-8, and
+        // the series stops where the coefficients fall below FP32 resolution
+        // (|c_k| < 1e-9 for all remaining k beyond R dt).
+        if (precision == QDX_FP32_BULK) {
+            p->cp.prec = std::max(p->cp.prec, kFp32PrecFloor);
+            const double x = p->cp.scaling * dt;
+            int cut = p->cp.order;
+            while (cut > 3 && cut - 1 > x && std::fabs(p->cp.coeff[cut - 1]) < kFp32CoeffFloor) --cut;
+            p->cp.order = cut;
+        }
         if (precision == QDX_FP32_BULK) h->ensure32();
         const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
         const size_t nbytes = (size_t)p->S * p->grid.size() * es;

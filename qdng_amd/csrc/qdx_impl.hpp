@@ -105,6 +105,13 @@ struct XGroupDev {
     XSplitDev<R> op[kMaxXOps];
 };
 
+// Events attached to the dispatch packet of the next kernel launch (set by the
+// host's profiling timer, consumed and cleared by that launch).
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchTiming g_launch_timing;
+
 // ---------------------------------------------------------------------------
 // Launchers implemented in qdx_kernels.hip.  All run on `stream`.
 template <class R>

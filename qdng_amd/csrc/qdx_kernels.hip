@@ -13,6 +13,7 @@
 //
 // Thread layout of every line kernel: a line of L = 8*TL points is handled by
 // TL threads, thread lt owning points lt + q*TL, q = 0..7 (see fft8).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +32,20 @@ namespace qdx {
         hipError_t e_ = (call);                                                              \
         if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") +         \
                                                        hipGetErrorString(e_) + " at " #call); \
+    } while (0)
+
+thread_local LaunchTiming g_launch_timing;
+
+// kernel launch that carries the profiling events of g_launch_timing, if any
+#define QDX_LAUNCH(kernel, grid, block, lds, stream, ...)                                          \
+    do {                                                                                          \
+        if (g_launch_timing.start) {                                                              \
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, g_launch_timing.start,        \
+                                  g_launch_timing.stop, 0, __VA_ARGS__);                          \
+            g_launch_timing = LaunchTiming();                                                     \
+        } else {                                                                                  \
+            hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                    \
+        }                                                                                         \
     } while (0)
 
 // ---------------------------------------------------------------------------
@@ -729,10 +744,10 @@ static int cheby_init_S(hipStream_t st, const GridDev &g, const LinePlan &plan, 
     auto cx_ = reinterpret_cast<cx<R> *>(X);
     if (X) {
         set_lds(k_cheby_init<R, S, true>, r.lds);
-        hipLaunchKernelGGL((k_cheby_init<R, S, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, p0, r.rb, cpsi,
+        QDX_LAUNCH((k_cheby_init<R, S, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, p0, r.rb, cpsi,
                            cphi, cx_, ctl, partials);
     } else {
-        hipLaunchKernelGGL((k_cheby_init<R, S, false>), dim3(r.nb), dim3(r.threads), 0, st, g, p0, r.rb, cpsi, cphi,
+        QDX_LAUNCH((k_cheby_init<R, S, false>), dim3(r.nb), dim3(r.threads), 0, st, g, p0, r.rb, cpsi, cphi,
                            cx_, ctl, partials);
     }
     QDX_HIP(hipGetLastError());
@@ -764,7 +779,7 @@ static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, co
     RowLaunch r = row_launch<R>(g, S, p0);
     if (r.nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
     set_lds(k_cheby_row<R, S, ONE_D, PRO>, r.lds);
-    hipLaunchKernelGGL((k_cheby_row<R, S, ONE_D, PRO>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
+    QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
                        r.rb, reinterpret_cast<const cx<R> *>(phim1), reinterpret_cast<cx<R> *>(phim2),
                        reinterpret_cast<cx<R> *>(X), reinterpret_cast<cx<double> *>(psi), ctl, partials, init_part,
                        nb_init);
@@ -810,19 +825,19 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     switch (mode) {
     case 0:
         set_lds(k_col<R, 0>, lds);
-        hipLaunchKernelGGL((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
         break;
     case 1:
         set_lds(k_col<R, 1>, lds);
-        hipLaunchKernelGGL((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
         break;
     case 2:
         set_lds(k_col<R, 2>, lds);
-        hipLaunchKernelGGL((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
         break;
     default:
         set_lds(k_col<R, 3>, lds);
-        hipLaunchKernelGGL((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
         break;
     }
     QDX_HIP(hipGetLastError());
@@ -838,13 +853,13 @@ static void ham_apply_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, co
     auto x = reinterpret_cast<cx<R> *>(X);
     if (g.nd == 1) {
         set_lds(k_apply_row<R, S, true>, r.lds);
-        hipLaunchKernelGGL((k_apply_row<R, S, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s,
+        QDX_LAUNCH((k_apply_row<R, S, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s,
                            x, d);
         QDX_HIP(hipGetLastError());
         return;
     }
     set_lds(k_row_fwd<R, S>, r.lds);
-    hipLaunchKernelGGL((k_row_fwd<R, S>), dim3(r.nb), dim3(r.threads), r.lds, st, g, plans[0], r.rb, s, x);
+    QDX_LAUNCH((k_row_fwd<R, S>), dim3(r.nb), dim3(r.threads), r.lds, st, g, plans[0], r.rb, s, x);
     QDX_HIP(hipGetLastError());
     if (g.nd == 2) {
         Launch<R>::col_pass(st, g, S, 1, 2, plans[1], h, 0.0, X, nullptr, ConvCheck());
@@ -854,7 +869,7 @@ static void ham_apply_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, co
         Launch<R>::col_pass(st, g, S, 1, 1, plans[1], h, 0.0, X, nullptr, ConvCheck());
     }
     set_lds(k_apply_row<R, S, false>, r.lds);
-    hipLaunchKernelGGL((k_apply_row<R, S, false>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s, x,
+    QDX_LAUNCH((k_apply_row<R, S, false>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s, x,
                        d);
     QDX_HIP(hipGetLastError());
 }
@@ -872,7 +887,7 @@ static void gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev
         LinePlan p0 = plan;
         RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
         set_lds(k_gspo_row<R, SS, ONE_D, PHASE>, r.lds);
-        hipLaunchKernelGGL((k_gspo_row<R, SS, ONE_D, PHASE>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, p0,
+        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, p0,
                            r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
     });
     QDX_HIP(hipGetLastError());
@@ -905,7 +920,7 @@ void Launch<R>::gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupD
     const int threads = 256;
     const int nb = (int)((g.size + threads - 1) / threads);
     QDX_DISPATCH_S(S, {
-        hipLaunchKernelGGL((k_gspo_xonly<R, SS>), dim3(nb), dim3(threads), 0, s, g, grp,
+        QDX_LAUNCH((k_gspo_xonly<R, SS>), dim3(nb), dim3(threads), 0, s, g, grp,
                            reinterpret_cast<cx<R> *>(psi));
     });
     QDX_HIP(hipGetLastError());
@@ -917,14 +932,14 @@ template struct Launch<float>;
 void launch_convert_c128_to_c64(hipStream_t s, const double *src, float *dst, long n) {
     const long n2 = 2 * n;
     const int nb = (int)std::min<long>((n2 + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_c128_to_c64, dim3(nb), dim3(256), 0, s, src, dst, n2);
+    QDX_LAUNCH(k_c128_to_c64, dim3(nb), dim3(256), 0, s, src, dst, n2);
     QDX_HIP(hipGetLastError());
 }
 
 void launch_convert_c64_to_c128(hipStream_t s, const float *src, double *dst, long n) {
     const long n2 = 2 * n;
     const int nb = (int)std::min<long>((n2 + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_c64_to_c128, dim3(nb), dim3(256), 0, s, src, dst, n2);
+    QDX_LAUNCH(k_c64_to_c128, dim3(nb), dim3(256), 0, s, src, dst, n2);
     QDX_HIP(hipGetLastError());
 }
 
@@ -933,10 +948,10 @@ static void overlap_impl(hipStream_t s, const R *a, const R *b, long n, int S, d
     const int threads = 256;
     int nbx = (int)std::min<long>((n + threads - 1) / threads, 1024);
     if (nbx < 1) nbx = 1;
-    hipLaunchKernelGGL((k_overlap_stage1<R>), dim3(nbx, S), dim3(threads), 0, s,
+    QDX_LAUNCH((k_overlap_stage1<R>), dim3(nbx, S), dim3(threads), 0, s,
                        reinterpret_cast<const cx<R> *>(a), reinterpret_cast<const cx<R> *>(b), n, partials);
     QDX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
+    QDX_LAUNCH(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
     QDX_HIP(hipGetLastError());
 }
 

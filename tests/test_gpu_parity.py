@@ -117,7 +117,10 @@ def cheby_pair(ctx, p, dt, prec, precision="fp64", steps=3):
     import qdng_amd as Q
     U = Q.Cheby(p.gpu_ham(ctx), dt, prec=prec, precision=precision)
     Uo = O.Cheby(p.oracle_ham(), dt, prec=prec)
-    assert U.order == Uo.order
+    if precision == "fp64":
+        assert U.order == Uo.order
+    else:
+        assert U.order <= Uo.order
     w = p.gpu_wf(ctx)
     psi = p.psi0.copy()
     counts = []
