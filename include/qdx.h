@@ -76,7 +76,9 @@ typedef enum qdx_kernel_class {
     QDX_K_LINE_PASS = 1,    /* strided FFT passes along dimensions >= 1          */
     QDX_K_CHEBY_INIT = 2,   /* step start: psi -> phi_0, FFT_0, sum |psi|^2        */
     QDX_K_GSPO_ROW = 3,     /* GSPO x-space split operators + FFT_0 / IFFT_0       */
-    QDX_NUM_KERNEL_CLASSES = 4
+    QDX_K_BATCH_ROW = 4,    /* batched jobs: row tiles of every job (one launch)  */
+    QDX_K_BATCH_COL = 5,    /* batched jobs: dimension-1 kinetic pass, 2D jobs     */
+    QDX_NUM_KERNEL_CLASSES = 6
 } qdx_kernel_class;
 int qdx_ctx_profile(qdx_ctx *ctx, int32_t on);
 int qdx_ctx_kernel_time(qdx_ctx *ctx, int32_t kernel_class, int64_t *launches, double *total_ms);
@@ -182,6 +184,50 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
                  double *norm2, double *autocorr, int32_t *n_hpsi_per_step,
                  qdx_step_stats *stats);
 int qdx_prop_destroy(qdx_prop *p);
+
+/* ---- batched jobs ---------------------------------------------------------
+ * Many independent single-state Chebyshev propagations (1D or 2D grids of
+ * different sizes, each with its own potential, dipole and laser field) kept
+ * resident in HBM and advanced together: one tick is one time step
+ * (OCheby::real_time_step, OCheby.cpp:92-197) of every active job, each pass
+ * of the series one launch over the tiles of all jobs.  QDng runs one
+ * propagation per program (ProgPropa::Run, ProgPropa.cpp:317-326); a job here
+ * is exactly that propagation with its own series length and break test, so
+ * its result does not depend on the other jobs in the batch.
+ *
+ * Device memory of the jobs comes from one arena of `hbm_budget` bytes
+ * allocated by qdx_batch_create; qdx_batch_submit fails with QDX_ERR_MEMORY
+ * (nothing allocated) when the job does not fit, so callers can queue it.
+ * Grids: 1D N <= 4096, 2D N0 <= 4096 and N1 <= 2048, N = 8 * 2^a * 3^b. */
+typedef struct qdx_batch qdx_batch;
+typedef struct qdx_job_desc {
+    qdx_ham *ham;          /* one state: kinetic, potential and dipole terms
+                              (OSum of OGridNablaSq, OGridPotential,
+                              OGridDipole); created with ctx NULL or not      */
+    double dt;
+    int32_t nsteps;
+    int32_t order;         /* 0: OCheby's automatic order                      */
+    double prec;           /* < 0: OCheby's default 1e-10                       */
+    const double *psi0;    /* complex128 initial state on the ham's grid        */
+} qdx_job_desc;
+
+int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_batch **out);
+/* device bytes the job would take from the arena */
+int qdx_batch_footprint(qdx_batch *b, const qdx_job_desc *job, int64_t *bytes);
+/* copy the job's data to the device; it starts at the next tick */
+int qdx_batch_submit(qdx_batch *b, const qdx_job_desc *job, int64_t *id);
+/* enqueue `nticks` ticks (asynchronous; qdx_ctx_synchronize waits) */
+int qdx_batch_run(qdx_batch *b, int32_t nticks);
+/* steps still to run (0: finished once the enqueued ticks complete), and,
+ * after a synchronisation, H applications and steps executed so far */
+int qdx_batch_job_state(qdx_batch *b, int64_t id, int64_t *steps_left, int64_t *n_hpsi, int64_t *steps_done);
+/* wait for the enqueued ticks and copy the job's psi (complex128) */
+int qdx_batch_fetch(qdx_batch *b, int64_t id, double *psi);
+/* wait for the enqueued ticks and return the job's memory to the arena */
+int qdx_batch_release(qdx_batch *b, int64_t id);
+/* jobs held, arena bytes in use, ticks enqueued so far */
+int qdx_batch_usage(qdx_batch *b, int32_t *jobs, int64_t *bytes_used, int64_t *ticks);
+int qdx_batch_destroy(qdx_batch *b);
 
 #ifdef __cplusplus
 }

@@ -54,9 +54,17 @@ EXPORTS = [
     "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
     "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
     "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
+    "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
+    "qdx_batch_job_state", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
+    "qdx_batch_destroy",
 ]
 
-K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW = 0, 1, 2, 3
+K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL = 0, 1, 2, 3, 4, 5
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("ham", C.c_void_p), ("dt", C.c_double), ("nsteps", C.c_int32), ("order", C.c_int32),
+                ("prec", C.c_double), ("psi0", C.POINTER(C.c_double))]
 
 
 def lib():
@@ -103,6 +111,16 @@ def lib():
         "qdx_prop_destroy": [vp],
         "qdx_ctx_profile": [vp, i32],
         "qdx_ctx_kernel_time": [vp, i32, C.POINTER(C.c_int64), dp],
+        "qdx_batch_create": [vp, i32, C.c_int64, C.POINTER(vp)],
+        "qdx_batch_footprint": [vp, C.POINTER(JobDesc), C.POINTER(C.c_int64)],
+        "qdx_batch_submit": [vp, C.POINTER(JobDesc), C.POINTER(C.c_int64)],
+        "qdx_batch_run": [vp, i32],
+        "qdx_batch_job_state": [vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int64)],
+        "qdx_batch_fetch": [vp, C.c_int64, dp],
+        "qdx_batch_release": [vp, C.c_int64],
+        "qdx_batch_usage": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+        "qdx_batch_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -353,3 +371,89 @@ class GSPO(_Propagator):
         self.h = C.c_void_p()
         _chk(lib().qdx_gspo_create(arr, len(self.ops), float(dt), _PREC[precision], C.byref(self.h)))
         self._destroy = lib().qdx_prop_destroy
+
+
+class Batch:
+    """Batched independent single-state Chebyshev jobs (qdx_batch_*).
+
+    One tick advances every active job by one time step; jobs are admitted
+    while they fit into the HBM budget (submit raises QdxError with code
+    QDX_ERR_MEMORY = -4 otherwise)."""
+
+    ERR_MEMORY = -4
+
+    def __init__(self, ctx: Context, precision="fp32", hbm_budget: int = 8 << 30):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _chk(lib().qdx_batch_create(ctx.h, _PREC[precision], int(hbm_budget), C.byref(self.h)))
+        self._destroy = lib().qdx_batch_destroy
+        self._keep = {}
+        self.sizes = {}
+
+    @staticmethod
+    def _desc(H: "Hamiltonian", dt, nsteps, psi0, order=0, prec=-1.0):
+        psi = np.ascontiguousarray(psi0, dtype=np.complex128).ravel()
+        d = JobDesc()
+        d.ham = H.h.value
+        d.dt = float(dt)
+        d.nsteps = int(nsteps)
+        d.order = int(order)
+        d.prec = float(prec)
+        d.psi0 = psi.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+        return d, psi
+
+    def footprint(self, H, dt, nsteps, psi0, order=0, prec=-1.0) -> int:
+        d, psi = self._desc(H, dt, nsteps, psi0, order, prec)
+        out = C.c_int64()
+        _chk(lib().qdx_batch_footprint(self.h, C.byref(d), C.byref(out)))
+        return out.value
+
+    def submit(self, H, dt, nsteps, psi0, order=0, prec=-1.0) -> int:
+        d, psi = self._desc(H, dt, nsteps, psi0, order, prec)
+        jid = C.c_int64()
+        _chk(lib().qdx_batch_submit(self.h, C.byref(d), C.byref(jid)))
+        self.sizes[jid.value] = psi.size
+        return jid.value
+
+    def try_submit(self, H, dt, nsteps, psi0, order=0, prec=-1.0):
+        """submit, or None when the job does not fit into the free budget"""
+        try:
+            return self.submit(H, dt, nsteps, psi0, order, prec)
+        except QdxError as e:
+            if e.code == self.ERR_MEMORY:
+                return None
+            raise
+
+    def run(self, nticks: int = 1) -> None:
+        _chk(lib().qdx_batch_run(self.h, int(nticks)))
+
+    def state(self, jid: int, device: bool = True):
+        """(steps_left, n_hpsi, steps_done); device=False skips the device read"""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        if device:
+            _chk(lib().qdx_batch_job_state(self.h, int(jid), C.byref(a), C.byref(b), C.byref(c)))
+        else:
+            _chk(lib().qdx_batch_job_state(self.h, int(jid), C.byref(a), None, None))
+        return a.value, b.value, c.value
+
+    def fetch(self, jid: int) -> np.ndarray:
+        out = np.zeros(self.sizes[jid], dtype=np.complex128)
+        _chk(lib().qdx_batch_fetch(self.h, int(jid), out.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def release(self, jid: int) -> None:
+        _chk(lib().qdx_batch_release(self.h, int(jid)))
+        self.sizes.pop(jid, None)
+
+    def usage(self):
+        n, b, t = C.c_int32(), C.c_int64(), C.c_int64()
+        _chk(lib().qdx_batch_usage(self.h, C.byref(n), C.byref(b), C.byref(t)))
+        return n.value, b.value, t.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqdx.so")
-SOURCES = ["qdx_kernels.hip", "qdx_host.cpp"]
+SOURCES = ["qdx_kernels.hip", "qdx_batch.hip", "qdx_host.cpp", "qdx_batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed-FP32 instructions are switched off for all device code.
@@ -15,14 +15,17 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(CSRC, src.rsplit(".", 1)[0] + ".o")
+    objs, procs = [], []
+    for src in SOURCES:   # translation units compile in parallel
+        obj = os.path.join(CSRC, src.replace(".", "_") + ".o")
         cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, pr in procs if pr.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-shared", "-o", LIB + ".tmp"] + objs
     if verbose:
         print(" ".join(cmd), flush=True)

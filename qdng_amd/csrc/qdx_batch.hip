@@ -1,0 +1,365 @@
+// gfx950 kernels of the batched-jobs runtime.
+//
+// A tick advances every active job by one Chebyshev time step
+// (OCheby::real_time_step, OCheby.cpp:92-197).  Each pass is one launch over
+// the tiles of all jobs: a block reads its tile, then its job's descriptor,
+// and runs the same register-resident line transforms and fused epilogue as
+// the single-propagation kernels, on that job's arrays.  Jobs keep their own
+// series length and break test: blocks of a job that has converged, or whose
+// series is shorter than the order of the launch, return at once.
+//
+// Row tiles: rb lines of dimension 0 of one job (one line for 1D jobs).
+//   nd == 1: FFT, *T/N, IFFT of phi_{k-1} in registers/LDS;
+//   nd == 2: IFFT_0 of X (transformed along dimension 1 by the column pass),
+//            and X <- FFT_0(phi_k) after the epilogue.
+// Column tiles (2D jobs): W adjacent columns, forward * T/N * inverse along
+// dimension 1 on X.
+#include <hip/hip_runtime.h>
+
+#include "qdx_batch.hpp"
+#include "qdx_device.hpp"
+
+namespace qdx {
+
+// geometry of one row tile
+template <class R>
+struct BRowTile {
+    int L, TL, line, lt;
+    long row;
+    bool active;
+    LdsLine<R> ln;
+    cx<R> *tw;
+    __device__ BRowTile(const BJob<R> &J, int row0, cx<R> *smem) {
+        L = J.N0;
+        TL = L >> 3;
+        line = threadIdx.x / TL;
+        lt = threadIdx.x - line * TL;
+        row = (long)row0 + line;
+        active = line < J.rb && row < J.N1;
+        const int lp = L + (L >> 3);
+        const int l = active ? line : 0;
+        ln.a = smem + (long)l * lp;
+        ln.b = smem + (long)(J.rb + l) * lp;
+        ln.es = 1;
+        ln.pad = true;
+        tw = smem + (long)2 * J.rb * lp;
+    }
+    __device__ long gidx(int q) const { return row * L + lt + q * TL; }
+};
+
+template <class R>
+__device__ __forceinline__ void bfft(cx<R> (&v)[8], const BRowTile<R> &B, const LinePlan &p, bool twl, int sign) {
+    if (twl) fft8<R>(v, B.ln, p, B.tw, sign, B.lt, B.active);
+    else fft8<R>(v, B.ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, B.lt, B.active);
+}
+
+// PHASE 0: init (phi_0 = psi, sum |psi|^2 of the step start, X = FFT_0 phi_0)
+// PHASE 1: prologue (order 1); PHASE 2: order k >= 2
+template <class R, int NT, int PHASE>
+__global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
+                                             const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BTile t = tiles[b];
+    const BJob<R> J = jobs[t.job];
+    const int step = *tick_p - J.start_tick;
+    if (step < 0 || step >= J.nsteps) return;
+    if (PHASE == 2 && (k >= J.order || J.ctl->done)) return;
+    extern __shared__ __align__(16) char smem_raw[];
+    __shared__ double red[16];
+    BRowTile<R> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
+    const LinePlan &p0 = plans[J.plan0];
+    const int local = b - J.rt0;   // tile index within the job
+    cx<double> *psi = reinterpret_cast<cx<double> *>(J.psi);
+    cx<R> *A = reinterpret_cast<cx<R> *>(J.phiA);
+    cx<R> *Bf = reinterpret_cast<cx<R> *>(J.phiB);
+    cx<R> *X = reinterpret_cast<cx<R> *>(J.X);
+    const bool two_d = J.nd == 2;
+
+    if (PHASE == 0) {
+        double acc = 0;
+        cx<R> v[8];
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const cx<double> x = psi[B.gidx(q)];
+                acc += x.r * x.r + x.i * x.i;
+                v[q] = from_d<R>(x);
+                A[B.gidx(q)] = v[q];
+            }
+        }
+        if (two_d) {
+            if (twl) stage_twiddles(B.tw, p0);
+            bfft(v, B, p0, twl, -1);
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) X[B.gidx(q)] = v[q];
+            }
+        }
+        const double part = block_sum(acc, red);
+        if (threadIdx.x == 0) {
+            J.part[2 * J.nrt + local] = part;
+            if (t.first == 0) {
+                J.ctl->done = 0;
+                J.ctl->last_order = 1;
+            }
+        }
+        return;
+    }
+
+    const cx<R> *phim1 = (k - 1) % 2 == 0 ? A : Bf;
+    cx<R> *phik = k % 2 == 0 ? A : Bf;   // phi_k replaces phi_{k-2}
+    cx<R> v[8], pm1[8];
+    if (B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) pm1[q] = phim1[B.gidx(q)];
+        if (two_d) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = X[B.gidx(q)];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = pm1[q];
+        }
+    }
+    const bool has_t = J.tk0 != nullptr;
+    double tq[8];
+    if (!two_d && B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + J.tk0[B.lt + q * B.TL]) * J.inv_n : 0.0;
+    }
+    if (twl) stage_twiddles(B.tw, p0);
+    if (PHASE == 1) {
+        if (t.first == 0) {
+            const double n0 = fixed_sum(J.part + 2 * J.nrt, J.nrt, red);
+            if (threadIdx.x == 0) {
+                J.ctl->norm0 = n0;
+                J.ctl->last_order = 1;
+            }
+        }
+    } else {
+        ConvCheck cc;
+        cc.part = J.part + ((k - 1) % 2) * J.nrt;
+        cc.norm0 = &J.ctl->norm0;
+        cc.nb = J.nrt;
+        cc.k_prev = k - 1;
+        cc.prec = J.prec;
+        double acc_prev;
+        if (prev_converged(cc, red, &acc_prev)) {
+            if (t.first == 0 && threadIdx.x == 0) {
+                J.ctl->done = 1;
+                J.ctl->accnorm = acc_prev;
+            }
+            return;
+        }
+        if (t.first == 0 && threadIdx.x == 0) J.ctl->last_order = k;
+    }
+    const double Ed = J.nt > 0 ? -J.laser[step] * J.lscale : 0.0;
+    if (!two_d) {
+        bfft(v, B, p0, twl, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+        }
+    }
+    bfft(v, B, p0, twl, +1);
+
+    double acc = 0;
+    if (B.active) {
+        const double *cf = J.coef;
+        const cx<double> ck = mk<double>(cf[2 * (PHASE == 1 ? 1 : k)], cf[2 * (PHASE == 1 ? 1 : k) + 1]);
+        const cx<double> c0 = mk<double>(cf[0], cf[1]);
+        cx<double> ps[8];
+        cx<R> m2[8];
+        double wv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long gi = B.gidx(q);
+            ps[q] = psi[gi];
+            if (PHASE == 2) m2[q] = phik[gi];
+            double w = 0.0;
+            if (J.V) w += (double)J.V[gi];
+            if (J.mu) w += Ed * (double)J.mu[gi];
+            wv[q] = w;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long gi = B.gidx(q);
+            cx<double> hp = has_t ? to_d(v[q]) : mk<double>(0, 0);
+            const cx<double> p = to_d(pm1[q]);
+            hp.r += wv[q] * p.r;
+            hp.i += wv[q] * p.i;
+            const cx<double> bb = mk<double>((hp.r - J.offset * p.r) * J.inv_scaling,
+                                             (hp.i - J.offset * p.i) * J.inv_scaling);
+            cx<double> pk;
+            cx<double> s = ps[q];
+            if (PHASE == 1) {
+                pk = mk<double>(bb.i, -bb.r);
+                s = cmul(c0, s) + cmul(ck, pk);
+            } else {
+                const cx<double> mm = to_d(m2[q]);
+                pk = mk<double>(2.0 * bb.i + mm.r, -2.0 * bb.r + mm.i);
+                s = s + cmul(ck, pk);
+                acc += s.r * s.r + s.i * s.i;
+            }
+            psi[gi] = s;
+            v[q] = from_d<R>(pk);
+            phik[gi] = v[q];
+        }
+    }
+    if (two_d) {
+        bfft(v, B, p0, twl, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) X[B.gidx(q)] = v[q];
+        }
+    }
+    if (PHASE == 1) return;
+    const double part = block_sum(acc, red);
+    if (threadIdx.x == 0) J.part[(k % 2) * J.nrt + local] = part;
+}
+
+// kinetic pass along dimension 1 of a 2D job: X <- IFFT_1(T/N * FFT_1(X))
+template <class R>
+__global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
+                                              const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BTile t = tiles[b];
+    const BJob<R> J = jobs[t.job];
+    const int step = *tick_p - J.start_tick;
+    if (step < 0 || step >= J.nsteps) return;
+    if (k >= 2 && (k >= J.order || J.ctl->done)) return;
+    extern __shared__ __align__(16) char smem_raw[];
+    cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
+    __shared__ double bc;
+    const LinePlan &p = plans[J.plan1];
+    const int N0 = J.N0, L = J.N1, TL = L >> 3, W = J.W;
+    const int c = threadIdx.x % W;
+    const int lt = threadIdx.x / W;
+    const int col = t.first + c;
+    const bool active = lt < TL && col < N0;
+    cx<R> *base = reinterpret_cast<cx<R> *>(J.X) + col;
+    LdsLine<R> ln;
+    ln.a = smem + c;
+    ln.b = smem + (long)W * L + c;
+    ln.es = W;
+    ln.pad = false;
+    cx<R> *twt = smem + (long)2 * W * L;
+    cx<R> v[8];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = base[(long)(lt + q * TL) * N0];
+    }
+    double tq[8];
+    if (active) {
+        const bool has_t = J.tk0 != nullptr;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            double tt = 0.0;
+            if (has_t) {
+                tt = 0.0 + J.tk0[col];
+                tt += J.tk1[lt + q * TL];
+            }
+            tq[q] = tt;
+        }
+    }
+    if (twl) stage_twiddles(twt, p);
+    if (k >= 3) {
+        ConvCheck cc;
+        cc.part = J.part + ((k - 1) % 2) * J.nrt;
+        cc.norm0 = &J.ctl->norm0;
+        cc.nb = J.nrt;
+        cc.k_prev = k - 1;
+        cc.prec = J.prec;
+        double a;
+        if (prev_converged(cc, &bc, &a)) return;
+    }
+    const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
+    fft8<R>(v, ln, p, tws, -1, lt, active);
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double tn = tq[q] * J.inv_n;
+            v[q] = mk<R>((R)((double)v[q].r * tn), (R)((double)v[q].i * tn));
+        }
+    }
+    fft8<R>(v, ln, p, tws, +1, lt, active);
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * N0] = v[q];
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_btick_end(const BJob<R> *__restrict__ jobs, int njobs, int *tick) {
+    const int tk = *tick;
+    for (int j = threadIdx.x; j < njobs; j += blockDim.x) {
+        const BJob<R> &J = jobs[j];
+        const int step = tk - J.start_tick;
+        if (step >= 0 && step < J.nsteps) {
+            J.stats[0] += J.ctl->last_order;
+            J.stats[1] += 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *tick = tk + 1;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+// raise the kernel's dynamic-LDS limit when a launch needs more than 64 KiB
+// (one attribute call per kernel instance and new maximum)
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+    static size_t allowed = 65536;
+    if (bytes <= allowed) return;
+    QDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes));
+    allowed = bytes;
+}
+
+template <class R, int NT, int PHASE>
+static void row_launch(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                       const LinePlan *plans, const int *tick, size_t lds, bool twl) {
+    allow_lds(k_brow<R, NT, PHASE>, lds);
+    QDX_LAUNCH((k_brow<R, NT, PHASE>), dim3(ntiles), dim3(NT), lds, s, jobs, tiles, plans, k, tick, twl ? 1 : 0);
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void BatchLaunch<R>::row(hipStream_t s, int NT, int phase, int k, const BJob<R> *jobs, const BTile *tiles,
+                         int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
+    if (ntiles <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("row tile exceeds the 160 KiB LDS of a gfx950 CU");
+    if (NT == 256) {
+        if (phase == 0) row_launch<R, 256, 0>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+        else if (phase == 1) row_launch<R, 256, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+        else row_launch<R, 256, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+    } else if (NT == 512) {
+        if (phase == 0) row_launch<R, 512, 0>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+        else if (phase == 1) row_launch<R, 512, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+        else row_launch<R, 512, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+    } else {
+        throw std::runtime_error("row tiles use 256 or 512 threads");
+    }
+}
+
+template <class R>
+void BatchLaunch<R>::col(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                         const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
+    if (ntiles <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("column tile exceeds the 160 KiB LDS of a gfx950 CU");
+    allow_lds(k_bcol<R>, lds);
+    QDX_LAUNCH((k_bcol<R>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void BatchLaunch<R>::tick_end(hipStream_t s, const BJob<R> *jobs, int njobs, int *tick) {
+    hipLaunchKernelGGL((k_btick_end<R>), dim3(1), dim3(256), 0, s, jobs, njobs, tick);
+    QDX_HIP(hipGetLastError());
+}
+
+template struct BatchLaunch<double>;
+template struct BatchLaunch<float>;
+
+}  // namespace qdx

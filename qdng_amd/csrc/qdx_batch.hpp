@@ -1,0 +1,74 @@
+// Device-side tables of the batched-jobs runtime: many independent
+// single-state Chebyshev propagations (1D or 2D grids of different sizes,
+// each with its own potential, dipole and laser field) advanced together, one
+// time step per tick, every pass of the series covering all active jobs in a
+// single launch.  Shared by qdx_batch.cpp (host) and qdx_batch.hip (kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "qdx_impl.hpp"
+
+namespace qdx {
+
+// One job as the kernels see it.  H = T + V(x) - E(t) mu(x) on an nd = 1 or
+// 2 grid; the Chebyshev scalars follow OCheby::Init / real_time_step
+// (OCheby.cpp:92-197, 358-493) for this job alone.
+template <class R>
+struct BJob {
+    int nd;                 // 1 or 2
+    int N0, N1;             // N1 = 1 when nd == 1
+    int plan0, plan1;       // indices into the plan table
+    int rb;                 // rows (lines of dimension 0) per row tile
+    int rt0, nrt;           // first row tile of the job, row tiles of the job
+    int W;                  // columns per column tile (nd == 2)
+    int order;              // series length (OCheby order after truncation)
+    int start_tick;         // tick of the job's step 0
+    int nsteps;
+    int nt;                 // laser samples (0: no dipole term)
+    long size;              // N0 * N1
+    double inv_n;           // 1 / size
+    double offset;          // (Emax + Emin) / 2
+    double inv_scaling;     // 1 / R
+    double prec;            // break threshold
+    double lscale;          // dipole scale
+    const double *coef;     // (re, im): coeff[k] * exp(-i offset dt), k < order
+    const double *tk0;      // kinetic table of dimension 0 (FP64), nullptr: no T
+    const double *tk1;      // dimension 1
+    const R *V;             // potential, nullptr: none
+    const R *mu;            // dipole moment, nullptr: none
+    const double *laser;    // field per step
+    double *psi;            // complex128 [size]
+    R *phiA, *phiB;         // phi_k for even / odd k (complex R)
+    R *X;                   // FFT_0 scratch (nd == 2)
+    double *part;           // 3 * nrt partial sums: [k % 2], [2] = step start
+    ChebyCtl *ctl;
+    long long *stats;       // [0] H applications, [1] steps done
+};
+
+// a row tile: rows row0 .. row0 + rb - 1 of job `job`; a column tile:
+// columns c0 .. c0 + W - 1
+struct BTile {
+    int job;
+    int first;
+};
+
+// Launchers (qdx_batch.hip).  `tick` is a device counter advanced by
+// batch_tick_end, so a tick's launch sequence does not change from tick to
+// tick and can be replayed from a captured graph.
+template <class R>
+struct BatchLaunch {
+    // init (phase 0), prologue (phase 1) or order k >= 2 (phase 2) over the
+    // row tiles [0, ntiles); NT = 256 or 512 threads per block
+    static void row(hipStream_t s, int NT, int phase, int k, const BJob<R> *jobs, const BTile *tiles,
+                    int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool tw_lds);
+    // kinetic pass along dimension 1 of the 2D jobs' X (forward * T/N * inverse)
+    static void col(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                    const LinePlan *plans, const int *tick, size_t lds, bool tw_lds);
+    // per-job bookkeeping at the end of a tick; advances *tick
+    static void tick_end(hipStream_t s, const BJob<R> *jobs, int njobs, int *tick);
+};
+
+}  // namespace qdx

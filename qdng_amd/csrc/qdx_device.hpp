@@ -3,11 +3,43 @@
 // reductions.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
 
 #include "qdx_impl.hpp"
 
 namespace qdx {
+
+#define QDX_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") +         \
+                                                       hipGetErrorString(e_) + " at " #call); \
+    } while (0)
+
+// kernel launch that carries the profiling events of g_launch_timing, if any
+#define QDX_LAUNCH(kernel, grid, block, lds, stream, ...)                                          \
+    do {                                                                                          \
+        if (g_launch_timing.start) {                                                              \
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, g_launch_timing.start,        \
+                                  g_launch_timing.stop, 0, __VA_ARGS__);                          \
+            g_launch_timing = LaunchTiming();                                                     \
+        } else {                                                                                  \
+            hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                    \
+        }                                                                                         \
+    } while (0)
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    // 8 XCDs receive blocks round robin (block b on XCD b % 8); give each XCD
+    // a contiguous range of tiles so that neighbouring tiles share one L2.
+    // XCD x runs nb/8 blocks, plus one when x < nb % 8.
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 
 template <class R>
 struct alignas(2 * sizeof(R)) cx {

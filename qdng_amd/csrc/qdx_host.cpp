@@ -14,659 +14,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/qdx.h"
-#include "qdx_impl.hpp"
-
-using namespace qdx;
-
-namespace {
-
-thread_local std::string g_err;
-
-struct QdxError : std::runtime_error {
-    int code;
-    QdxError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-[[noreturn]] void fail(int code, const std::string &msg) { throw QdxError(code, msg); }
-
-#define HIPCHK(call)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (call);                                                                   \
-        if (e_ != hipSuccess) fail(QDX_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-template <class F>
-int guarded(F &&f) {
-    try {
-        f();
-        return QDX_OK;
-    } catch (const QdxError &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_err = "host allocation failed";
-        return QDX_ERR_MEMORY;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return QDX_ERR_DEVICE;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// device memory owned by RAII holders
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    explicit DevBuf(size_t n) { alloc(n); }
-    void alloc(size_t n) {
-        release();
-        if (n == 0) return;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            p = nullptr;
-            fail(QDX_ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-        }
-        bytes = n;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    ~DevBuf() { release(); }
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    template <class T>
-    T *as() const { return static_cast<T *>(p); }
-};
-
-// ---------------------------------------------------------------------------
-// geometry (GridSystem.cpp:207-211, Kspace.h:20-23)
-struct Grid {
-    int nd = 1;
-    int N[4] = {1, 1, 1, 1};
-    double xmin[4] = {0, 0, 0, 0}, xmax[4] = {0, 0, 0, 0};
-    long size() const {
-        long s = 1;
-        for (int d = 0; d < nd; ++d) s *= N[d];
-        return s;
-    }
-    double dx(int d) const { return (xmax[d] - xmin[d]) / double(N[d] - 1); }
-    double dk(int d) const { return (2 * M_PI) / (xmax[d] - xmin[d] + dx(d)); }
-    double weight() const {
-        double w = 1;
-        for (int d = 0; d < nd; ++d) w *= dx(d);
-        return w;
-    }
-    bool operator==(const Grid &o) const {
-        if (nd != o.nd) return false;
-        for (int d = 0; d < nd; ++d) {  // GridSystem.cpp:248-261, GRID_EPS 1e-10
-            if (N[d] != o.N[d]) return false;
-            if (std::fabs(xmin[d] - o.xmin[d]) > 1e-10 || std::fabs(xmax[d] - o.xmax[d]) > 1e-10) return false;
-        }
-        return true;
-    }
-    GridDev dev() const {
-        GridDev g;
-        g.nd = nd;
-        for (int d = 0; d < 3; ++d) g.N[d] = d < nd ? N[d] : 1;
-        g.size = size();
-        return g;
-    }
-};
-
-bool is_smooth(int n) {  // 2^a 3^b
-    if (n < 1) return false;
-    while (n % 2 == 0) n /= 2;
-    while (n % 3 == 0) n /= 3;
-    return n == 1;
-}
-
-Grid make_grid(const qdx_grid *g) {
-    if (!g) fail(QDX_ERR_PARAM, "grid is NULL");
-    if (g->ndims < 1 || g->ndims > 3) fail(QDX_ERR_PARAM, "device path supports 1..3 grid dimensions");
-    Grid r;
-    r.nd = g->ndims;
-    for (int d = 0; d < r.nd; ++d) {
-        if (g->N[d] < 2) fail(QDX_ERR_PARAM, "Zero points grid defined");
-        if (g->xmax[d] - g->xmin[d] <= 0) fail(QDX_ERR_PARAM, "Zero length grid defined");
-        if (!is_smooth(g->N[d]) || g->N[d] % 8 != 0)
-            fail(QDX_ERR_PARAM, "grid sizes must be of the form 8 * 2^a * 3^b");
-        if (g->N[d] > kMaxLine) fail(QDX_ERR_PARAM, "grid dimension longer than 4096 points");
-        r.N[d] = g->N[d];
-        r.xmin[d] = g->xmin[d];
-        r.xmax[d] = g->xmax[d];
-    }
-    return r;
-}
-
-// ---------------------------------------------------------------------------
-// radix plans and twiddle tables exp(-2 pi i q / L).  L = 2^a 3^b with
-// a >= 3; the first and the last stage take a radix 8, 4 or 2 (they run on
-// registers in fft8), radix-3 stages sit in the middle.
-std::vector<int> radix_plan(int L) {
-    int a = 0, b = 0, m = L;
-    while (m % 2 == 0) { ++a; m /= 2; }
-    while (m % 3 == 0) { ++b; m /= 3; }
-    if (m != 1 || a < 3) fail(QDX_ERR_PARAM, "line length must be 8 * 2^a * 3^b");
-    auto pw = [](int e) { return e >= 3 ? 8 : (e == 2 ? 4 : 2); };
-    std::vector<int> r;
-    if (b == 0) {
-        while (a >= 3) { r.push_back(8); a -= 3; }
-        if (a > 0) r.push_back(pw(a));
-    } else {
-        const int first = a >= 4 ? 8 : 4;
-        a -= first == 8 ? 3 : 2;
-        const int last = pw(a);
-        a -= last == 8 ? 3 : (last == 4 ? 2 : 1);
-        r.push_back(first);
-        while (a >= 3) { r.push_back(8); a -= 3; }
-        if (a > 0) r.push_back(pw(a));
-        for (int i = 0; i < b; ++i) r.push_back(3);
-        r.push_back(last);
-    }
-    if ((int)r.size() > kMaxStages) fail(QDX_ERR_PARAM, "too many radix stages");
-    return r;
-}
-
-}  // namespace
-
-struct qdx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int refs = 0;          // live wave functions, Hamiltonians and propagators
-    bool closing = false;  // qdx_ctx_destroy called while children were alive
-    std::map<int, std::unique_ptr<DevBuf>> tw64, tw32;
-    DevBuf partials;   // reduction scratch
-    DevBuf scal;       // small device result area
-    double *pinned = nullptr;
-
-    // per-launch HIP-event timing (qdx_ctx_profile): a fixed pool of events;
-    // when it is used up the recorded spans are folded into the totals
-    static constexpr size_t kEventPool = 8192;
-    bool profiling = false;
-    std::vector<hipEvent_t> events;
-    size_t next_event = 0;
-    std::vector<std::pair<size_t, size_t>> spans[QDX_NUM_KERNEL_CLASSES];
-    double acc_ms[QDX_NUM_KERNEL_CLASSES] = {};
-    int64_t acc_n[QDX_NUM_KERNEL_CLASSES] = {};
-    void fold() {
-        HIPCHK(hipStreamSynchronize(stream));
-        for (int c = 0; c < QDX_NUM_KERNEL_CLASSES; ++c) {
-            for (auto &sp : spans[c]) {
-                float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, events[sp.first], events[sp.second]));
-                acc_ms[c] += ms;
-                acc_n[c] += 1;
-            }
-            spans[c].clear();
-        }
-        next_event = 0;
-    }
-    // index of a fresh event; the caller records it right away
-    size_t take_event() {
-        if (events.empty()) {
-            events.resize(kEventPool);
-            for (auto &e : events) HIPCHK(hipEventCreate(&e));
-        }
-        return next_event++;
-    }
-
-    LinePlan plan(int L, bool fp32) {
-        auto &tab = fp32 ? tw32 : tw64;
-        auto it = tab.find(L);
-        if (it == tab.end()) {
-            std::vector<double> t(2 * L);
-            for (int q = 0; q < L; ++q) {
-                long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)L;
-                t[2 * q] = (double)cosl(a);
-                t[2 * q + 1] = (double)sinl(a);
-            }
-            auto buf = std::make_unique<DevBuf>(fp32 ? 2 * L * sizeof(float) : 2 * L * sizeof(double));
-            if (fp32) {
-                std::vector<float> f(t.begin(), t.end());
-                HIPCHK(hipMemcpy(buf->p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
-            } else {
-                HIPCHK(hipMemcpy(buf->p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-            }
-            it = tab.emplace(L, std::move(buf)).first;
-        }
-        LinePlan p;
-        p.L = L;
-        auto r = radix_plan(L);
-        p.nst = (int)r.size();
-        for (int i = 0; i < p.nst; ++i) p.rad[i] = r[i];
-        p.tw = it->second->p;
-        return p;
-    }
-};
-
-namespace {
-// brackets one launch with events when the context is profiling
-// Times exactly one kernel launch: the events ride on that launch's dispatch
-// packet (hipExtLaunchKernelGGL), so the span is the kernel's own execution.
-struct KTimer {
-    qdx_ctx *c;
-    int cls;
-    size_t a = 0, b = 0;
-    KTimer(qdx_ctx *ctx, int kind) : c(ctx), cls(kind) {
-        if (!c->profiling) return;
-        if (c->next_event + 2 > qdx_ctx::kEventPool) c->fold();
-        a = c->take_event();
-        b = c->take_event();
-        g_launch_timing.start = c->events[a];
-        g_launch_timing.stop = c->events[b];
-    }
-    ~KTimer() {
-        if (!c->profiling) return;
-        if (!g_launch_timing.start) c->spans[cls].push_back({a, b});   // consumed by the launch
-        g_launch_timing = LaunchTiming();
-    }
-};
-
-void ctx_free(qdx_ctx *ctx) {
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    ctx->pinned = nullptr;
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-}
-void ctx_acquire(qdx_ctx *ctx) {
-    if (ctx) ctx->refs++;
-}
-void ctx_release(qdx_ctx *ctx) {
-    if (!ctx) return;
-    if (--ctx->refs == 0 && ctx->closing) ctx_free(ctx);
-}
-}  // namespace
-
-struct qdx_wf {
-    qdx_ctx *ctx = nullptr;
-    Grid grid;
-    int S = 1;
-    DevBuf d;   // complex128 [S][N]
-    double *ptr() const { return d.as<double>(); }
-    long n() const { return grid.size(); }
-};
-
-namespace {
-
-enum TermKind { T_KIN = 0, T_POT = 1, T_DIP = 2 };
-
-struct Term {
-    TermKind kind;
-    int row, col;
-    std::vector<double> mass;     // kinetic
-    std::vector<double> v;        // potential / dipole mu
-    std::vector<double> laser;    // dipole field
-    double scale = 1.0;
-};
-
-struct Elem {             // applied matrix element after QDng's mirroring
-    std::vector<int> terms;
-};
-
-void sym_eigen(int n, std::vector<double> a, std::vector<double> &ev) {
-    // cyclic Jacobi rotations; S <= 4 here
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0;
-        for (int p = 0; p < n; ++p)
-            for (int q = p + 1; q < n; ++q) off += a[p * n + q] * a[p * n + q];
-        if (off < 1e-300) break;
-        for (int p = 0; p < n; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                double apq = a[p * n + q];
-                if (std::fabs(apq) < 1e-300) continue;
-                double theta = (a[q * n + q] - a[p * n + p]) / (2 * apq);
-                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
-                double c = 1 / std::sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < n; ++k) {
-                    double akp = a[k * n + p], akq = a[k * n + q];
-                    a[k * n + p] = c * akp - s * akq;
-                    a[k * n + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; ++k) {
-                    double apk = a[p * n + k], aqk = a[q * n + k];
-                    a[p * n + k] = c * apk - s * aqk;
-                    a[q * n + k] = s * apk + c * aqk;
-                }
-            }
-    }
-    ev.resize(n);
-    for (int i = 0; i < n; ++i) ev[i] = a[i * n + i];
-}
-
-}  // namespace
-
-struct qdx_ham {
-    qdx_ctx *ctx = nullptr;
-    Grid grid;
-    int S = 1;
-    bool multistate = false, unity = true;
-    std::vector<Term> terms;
-    // finalised state
-    bool final_ = false;
-    bool hermitian = true;
-    std::vector<Elem> elem;        // S*S applied elements
-    std::vector<bool> given;       // S*S elements added by the caller
-    // device arrays (FP64 always, FP32 on demand)
-    struct DevSet {
-        std::vector<std::unique_ptr<DevBuf>> bufs;
-        HamDev<double> h64;
-        HamDev<float> h32;
-        bool have32 = false;
-    } dev;
-    DevBuf tk;                     // kinetic tables, FP64
-    DevBuf X64;                    // scratch for ham_apply
-    DevBuf tmp;                    // complex128 scratch for expec
-
-    void check_open() const {
-        if (final_) fail(QDX_ERR_STATE, "Hamiltonian is already in use; add terms before the first use");
-    }
-
-    // QDng OMultistate::Add / Init mirroring (OMultistate.cpp:34-53, 99-142)
-    void finalize() {
-        if (final_) return;
-        if (terms.empty()) fail(QDX_ERR_PARAM, multistate ? "Multistate operator is empty" : "Sum Operator is empty");
-        given.assign(S * S, false);
-        for (auto &t : terms) given[t.row * S + t.col] = true;
-        hermitian = true;
-        for (int i = 0; i < S; ++i)
-            for (int j = 0; j < S; ++j)
-                if (i != j && given[i * S + j] && given[j * S + i]) hermitian = false;
-        elem.assign(S * S, Elem());
-        for (int k = 0; k < (int)terms.size(); ++k) {
-            const Term &t = terms[k];
-            elem[t.row * S + t.col].terms.push_back(k);
-            if (hermitian && t.row != t.col) elem[t.col * S + t.row].terms.push_back(k);
-        }
-        final_ = true;
-        upload();
-    }
-
-    // bounds of one matrix element: the sum of its terms (OSum.cpp:28-51)
-    double term_emax(const Term &t) const {
-        switch (t.kind) {
-        case T_KIN: {  // OGridNablaSq.cpp:89-101
-            double T = 0;
-            for (int d = 0; d < grid.nd; ++d)
-                if (t.mass[d] > 0) T += 1 / (t.mass[d] * grid.dx(d) * grid.dx(d));
-            return T * (M_PI * M_PI / 2);
-        }
-        case T_POT: return *std::max_element(t.v.begin(), t.v.end());
-        case T_DIP:   // OGridDipole.cpp:36-39
-            return *std::max_element(t.v.begin(), t.v.end()) * *std::max_element(t.laser.begin(), t.laser.end());
-        }
-        return 0;
-    }
-    double term_emin(const Term &t) const {
-        if (t.kind == T_POT) return *std::min_element(t.v.begin(), t.v.end());
-        return 0;
-    }
-    void bounds(double &emin, double &emax) {
-        finalize();
-        if (!multistate) {
-            emin = emax = 0;
-            for (auto &t : terms) {
-                emax += term_emax(t);
-                emin += term_emin(t);
-            }
-            return;
-        }
-        // OMultistate.cpp:163-204: eigenvalues of |Emax_ij| and |Emin_ij|
-        std::vector<double> mx(S * S, 0.0), mn(S * S, 0.0), ev;
-        for (int i = 0; i < S; ++i)
-            for (int j = 0; j < S; ++j) {
-                const Elem &e = elem[i * S + j];
-                if (e.terms.empty()) continue;
-                double a = 0, b = 0;
-                for (int k : e.terms) {
-                    a += term_emax(terms[k]);
-                    b += term_emin(terms[k]);
-                }
-                mx[i * S + j] = std::fabs(a);
-                mn[i * S + j] = std::fabs(b);
-            }
-        sym_eigen(S, mx, ev);
-        emax = *std::max_element(ev.begin(), ev.end());
-        sym_eigen(S, mn, ev);
-        emin = *std::min_element(ev.begin(), ev.end());
-    }
-
-    // upload the applied matrix: W_ij = sum of potentials, mu_ij one dipole,
-    // kinetic tables per state and dimension (Kspace.cpp:22-50)
-    std::vector<std::vector<double>> hostW, hostMu;
-    void upload() {
-        if (!ctx) return;
-        const long n = grid.size();
-        HamDev<double> &h = dev.h64;
-        h.S = S;
-        h.inv_n = 1.0 / double(n);
-        hostW.assign(S * S, {});
-        hostMu.assign(S * S, {});
-        std::vector<double> tkh;
-        std::vector<long> tk_off(S * 3, -1);
-        for (int i = 0; i < S; ++i) {
-            h.unity[i] = 0;
-            for (int j = 0; j < S; ++j) {
-                const Elem &e = elem[i * S + j];
-                std::vector<double> &W = hostW[i * S + j];
-                std::vector<double> &Mu = hostMu[i * S + j];
-                const Term *dip = nullptr;
-                std::vector<double> kin_sum;
-                for (int k : e.terms) {
-                    const Term &t = terms[k];
-                    if (t.kind == T_POT) {
-                        if (W.empty()) W.assign(n, 0.0);
-                        for (long x = 0; x < n; ++x) W[x] += t.v[x];
-                    } else if (t.kind == T_DIP) {
-                        if (dip && (dip->laser != t.laser || dip->scale != t.scale))
-                            fail(QDX_ERR_INCOMPAT, "two dipole terms with different fields on one element");
-                        if (Mu.empty()) Mu.assign(n, 0.0);
-                        for (long x = 0; x < n; ++x) Mu[x] += t.v[x];
-                        dip = &t;
-                    } else {
-                        if (i != j) fail(QDX_ERR_INCOMPAT, "kinetic operator on an off-diagonal element");
-                        // per-dimension tables, summed over kinetic terms of the state
-                        long base = (long)tkh.size();
-                        for (int d = 0; d < grid.nd; ++d) {
-                            const int Nx = grid.N[d];
-                            double dp = grid.dk(d);
-                            dp *= dp;
-                            dp *= -(-1. / 2. / t.mass[d]);
-                            std::vector<double> k1(Nx);
-                            if (Nx % 2 == 0) {
-                                for (int q = 0; q < Nx / 2; ++q) {
-                                    k1[q] = (double(q) * double(q)) * dp;
-                                    k1[Nx - q - 1] = (double(q + 1) * double(q + 1)) * dp;
-                                }
-                            } else {
-                                k1[0] = 0;
-                                for (int q = 1; q < (Nx + 1) / 2; ++q) {
-                                    k1[q] = (double(q) * double(q)) * dp;
-                                    k1[Nx - q] = (double(q) * double(q)) * dp;
-                                }
-                            }
-                            if (tk_off[i * 3 + d] < 0) {
-                                tk_off[i * 3 + d] = (long)tkh.size();
-                                tkh.insert(tkh.end(), k1.begin(), k1.end());
-                            } else {
-                                for (int q = 0; q < Nx; ++q) tkh[tk_off[i * 3 + d] + q] += k1[q];
-                            }
-                        }
-                        (void)base;
-                    }
-                }
-                if (dip) h.Edip[i][j] = 0;  // set per step
-                if (i == j && e.terms.empty() && multistate && unity) h.unity[i] = 1.0;
-            }
-        }
-        // device copies
-        tk.alloc(std::max<size_t>(tkh.size(), 1) * sizeof(double));
-        if (!tkh.empty()) HIPCHK(hipMemcpy(tk.p, tkh.data(), tkh.size() * sizeof(double), hipMemcpyHostToDevice));
-        for (int i = 0; i < S; ++i)
-            for (int d = 0; d < 3; ++d)
-                h.tk[i][d] = (d < grid.nd && tk_off[i * 3 + d] >= 0) ? tk.as<double>() + tk_off[i * 3 + d] : nullptr;
-        for (int i = 0; i < S; ++i) {
-            if (h.tk[i][0] == nullptr)
-                for (int d = 0; d < 3; ++d) h.tk[i][d] = nullptr;
-            for (int j = 0; j < S; ++j) {
-                h.W[i][j] = nullptr;
-                h.mu[i][j] = nullptr;
-                if (!hostW[i * S + j].empty()) h.W[i][j] = upload64(hostW[i * S + j]);
-                if (!hostMu[i * S + j].empty()) h.mu[i][j] = upload64(hostMu[i * S + j]);
-            }
-        }
-    }
-    std::map<const std::vector<double> *, const double *> up64_cache;
-    const double *upload64(const std::vector<double> &v) {
-        auto b = std::make_unique<DevBuf>(v.size() * sizeof(double));
-        HIPCHK(hipMemcpy(b->p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
-        const double *p = b->as<double>();
-        dev.bufs.push_back(std::move(b));
-        return p;
-    }
-    const float *upload32(const std::vector<double> &v) {
-        std::vector<float> f(v.begin(), v.end());
-        auto b = std::make_unique<DevBuf>(f.size() * sizeof(float));
-        HIPCHK(hipMemcpy(b->p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
-        const float *p = b->as<float>();
-        dev.bufs.push_back(std::move(b));
-        return p;
-    }
-    void ensure32() {
-        finalize();
-        if (dev.have32) return;
-        HamDev<float> &f = dev.h32;
-        const HamDev<double> &d = dev.h64;
-        f.S = d.S;
-        f.inv_n = d.inv_n;
-        for (int i = 0; i < S; ++i) {
-            f.unity[i] = d.unity[i];
-            for (int k = 0; k < 3; ++k) f.tk[i][k] = d.tk[i][k];
-            for (int j = 0; j < S; ++j) {
-                f.W[i][j] = hostW[i * S + j].empty() ? nullptr : upload32(hostW[i * S + j]);
-                f.mu[i][j] = hostMu[i * S + j].empty() ? nullptr : upload32(hostMu[i * S + j]);
-                f.Edip[i][j] = 0;
-            }
-        }
-        dev.have32 = true;
-    }
-
-    // -laser[step] * scale for each applied dipole element (Laser.cpp:125-128)
-    template <class R>
-    void set_fields(HamDev<R> &h, long step) const {
-        for (int i = 0; i < S; ++i)
-            for (int j = 0; j < S; ++j) {
-                h.Edip[i][j] = 0;
-                for (int k : elem[i * S + j].terms) {
-                    const Term &t = terms[k];
-                    if (t.kind != T_DIP) continue;
-                    if (step < 0 || step >= (long)t.laser.size())
-                        fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
-                    h.Edip[i][j] = -t.laser[step] * t.scale;
-                }
-            }
-    }
-    bool time_dependent() const {
-        for (auto &t : terms)
-            if (t.kind == T_DIP) return true;
-        return false;
-    }
-    long min_laser() const {
-        long m = -1;
-        for (auto &t : terms)
-            if (t.kind == T_DIP) m = (m < 0) ? (long)t.laser.size() : std::min(m, (long)t.laser.size());
-        return m;
-    }
-};
-
-namespace {
-
-// ---------------------------------------------------------------------------
-// Bessel J_k(x), k = 0..n-1 by Miller's backward recurrence in extended
-// precision, normalised with J_0 + 2 sum_{k>=1} J_{2k} = 1.  QDng obtains the
-// same values from SLATEC ZBESJ (Bessel.cpp:22-58); its underflow count NZ is
-// reproduced as the number of trailing values below exp(-ELIM),
-// ELIM = 2.303 (1021 log10 2 - 3) (zbesj.f:183-188).
-int bessel_j0(int n, double x, std::vector<double> &out) {
-    out.assign(n, 0.0);
-    if (n < 1) fail(QDX_ERR_PARAM, "Bessel: n must be >= 1");
-    if (x == 0) {
-        out[0] = 1.0;
-        return 0;
-    }
-    const long double ax = std::fabs((long double)x);
-    // start far enough above max(n, x) that J_m is negligible
-    int m = std::max(n, (int)ax) + 16;
-    while ((long double)m * std::log(2.0L * m / (2.718281828459045L * ax)) < 120.0L) m += 8;
-    if (m % 2) ++m;
-    std::vector<long double> j(m + 2, 0.0L);
-    j[m + 1] = 0.0L;
-    j[m] = 1e-300L;
-    for (int k = m; k >= 1; --k) j[k - 1] = (2.0L * k / ax) * j[k] - j[k + 1];
-    long double norm = j[0];
-    for (int k = 2; k <= m; k += 2) norm += 2.0L * j[k];
-    for (int k = 0; k < n; ++k) {
-        long double v = j[k] / norm;
-        if (x < 0 && (k & 1)) v = -v;
-        out[k] = (double)v;
-    }
-    const double elim = 2.303 * (1021.0 * 0.30102999566398120 - 3.0);
-    const double ulim = std::exp(-elim);
-    int nz = 0;
-    for (int k = n - 1; k >= 0 && std::fabs(out[k]) < ulim; --k) {
-        out[k] = 0.0;
-        ++nz;
-    }
-    return nz;
-}
-
-constexpr double kFp32PrecFloor = 1e-8;
-constexpr double kFp32CoeffFloor = 1e-9;
-
-struct ChebyParams {
-    int order = 0, nz = 0;
-    double offset = 0, scaling = 1, prec = 1e-10, emin = 0, emax = 0;
-    std::vector<double> coeff;
-};
-
-// OCheby::Init (OCheby.cpp:382-493), real time, forward
-ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double prec, double scaling_param) {
-    if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
-    ChebyParams c;
-    h->bounds(c.emin, c.emax);
-    c.offset = (c.emax + c.emin) / 2;
-    c.scaling = std::fabs(c.emax - c.emin) / 2;
-    int order = order_param > 0 ? order_param : 0;
-    if (order <= 0) order = std::abs(int(20 * c.scaling * dt));
-    if (order < 10) order = 10;
-    if (scaling_param > 0) c.scaling = scaling_param;
-    if (order_param > 0) order = order_param;
-    if (order < 3) fail(QDX_ERR_PARAM, "Chebychev recursion must be at least 3");
-    std::vector<double> b;
-    int zeroes = bessel_j0(order, c.scaling * dt, b);
-    order -= zeroes;
-    if (order_param <= 0) {
-        int i = 2;
-        while (i < order && std::fabs(b[i - 1] - b[i - 2]) > 1e-32) i++;
-        order = i;
-    }
-    if (order > 0 && order <= int(c.scaling * dt)) fail(QDX_ERR_PARAM, "Chebychev recursion order is to small");
-    c.order = order;
-    c.nz = zeroes;
-    c.coeff.assign(order, 0.0);
-    c.coeff[0] = b[0];
-    for (int i = 1; i < order; ++i) c.coeff[i] = 2.0 * b[i];
-    if (prec >= 0) c.prec = prec;
-    return c;
-}
-
-}  // namespace
+#include "qdx_host_internal.hpp"
 
 // ---------------------------------------------------------------------------
 // propagators
@@ -1277,17 +625,7 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         p->S = h->S;
         p->ham = h;
         p->cp = cheby_params(h, dt, order, prec, scaling);
-        // FP32-bulk series: sum |psi|^2 of FP32 terms cannot resolve norm
-        // changes much below 1e-9, so the break test uses at least 1e-8, and
-        // the series stops where the coefficients fall below FP32 resolution
-        // (|c_k| < 1e-9 for all remaining k beyond R dt).
-        if (precision == QDX_FP32_BULK) {
-            p->cp.prec = std::max(p->cp.prec, kFp32PrecFloor);
-            const double x = p->cp.scaling * dt;
-            int cut = p->cp.order;
-            while (cut > 3 && cut - 1 > x && std::fabs(p->cp.coeff[cut - 1]) < kFp32CoeffFloor) --cut;
-            p->cp.order = cut;
-        }
+        if (precision == QDX_FP32_BULK) fp32_series(p->cp, dt);
         if (precision == QDX_FP32_BULK) h->ensure32();
         const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
         const size_t nbytes = (size_t)p->S * p->grid.size() * es;

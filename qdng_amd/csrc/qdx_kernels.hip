@@ -27,26 +27,8 @@
 
 namespace qdx {
 
-#define QDX_HIP(call)                                                                        \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") +         \
-                                                       hipGetErrorString(e_) + " at " #call); \
-    } while (0)
-
 thread_local LaunchTiming g_launch_timing;
 
-// kernel launch that carries the profiling events of g_launch_timing, if any
-#define QDX_LAUNCH(kernel, grid, block, lds, stream, ...)                                          \
-    do {                                                                                          \
-        if (g_launch_timing.start) {                                                              \
-            hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, g_launch_timing.start,        \
-                                  g_launch_timing.stop, 0, __VA_ARGS__);                          \
-            g_launch_timing = LaunchTiming();                                                     \
-        } else {                                                                                  \
-            hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                    \
-        }                                                                                         \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -62,13 +44,6 @@ __device__ __forceinline__ double kin_sum(const HamDev<R> &h, int st, const Grid
         if (g.nd > 2) t += tk[2][k2];
     }
     return t;
-}
-
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-    // 8 XCDs receive blocks round robin; give each XCD a contiguous range of
-    // tiles so that neighbouring tiles share one L2.
-    if (nb % 8 != 0) return b;
-    return (b % 8) * (nb / 8) + (b / 8);
 }
 
 // Row-block geometry: rb rows of dimension 0 for each of S states; line

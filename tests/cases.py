@@ -252,3 +252,58 @@ def laser_1d(N=512, L=10.0, mass=1836.0, De=0.1, a=1.0, E0=0.01, w=0.05, nt=400,
     p.kinetic(0, mass).potential(0, 0, V).dipole(0, 0, mu, E)
     p.psi0 = normalized(p, gaussian(p, [0.3], 0.4))
     return p
+
+
+# ---------------------------------------------------------------------------
+# batched uneven jobs: laser-driven 1D and 2D wave packets
+
+SIZES_1D_BATCH = [128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096]
+SIZES_2D_BATCH = [64, 96, 128, 192, 256, 384]
+
+
+def driven_job(N, rng, steps, dt=1.0):
+    """One laser-driven job: box [-L, L] (L ~ U[8, 12]) per dimension, mass
+    ~ U[1000, 3000]; Morse (De ~ U[0.05, 0.2], a ~ U[0.5, 1.5], capped at
+    1 Eh) or harmonic (k ~ U[0.02, 0.2]) with equal odds, summed over the
+    dimensions; dipole mu = x exp(-x^2/8) along dimension 0; field
+    E(t) = E0 sin^2(pi t / T) cos(w t), E0 ~ U[0.001, 0.02], w ~ U[0.005, 0.2],
+    T = steps * dt, sampled at t = n dt; Gaussian psi0 (sigma ~ U[0.3, 0.6])
+    centred at ~ U[-0.5, 0.5] per dimension."""
+    N = list(N)
+    nd = len(N)
+    L = rng.uniform(8, 12)
+    mass = rng.uniform(1000, 3000)
+    p = Problem(N, [-L] * nd, [L] * nd)
+    ax = p.axes()
+    if rng.uniform() < 0.5:
+        De, a = rng.uniform(0.05, 0.2), rng.uniform(0.5, 1.5)
+        V = sum(De * (1 - np.exp(-a * x)) ** 2 for x in ax)
+    else:
+        k = rng.uniform(0.02, 0.2)
+        V = sum(0.5 * k * x ** 2 for x in ax)
+    V = np.minimum(V, 1.0)
+    x = ax[0]
+    mu = x * np.exp(-x ** 2 / 8)
+    E0, w = rng.uniform(0.001, 0.02), rng.uniform(0.005, 0.2)
+    t = np.arange(steps) * dt
+    T = steps * dt
+    E = E0 * np.sin(np.pi * t / T) ** 2 * np.cos(w * t)
+    p.kinetic(0, [mass] * nd).potential(0, 0, V).dipole(0, 0, mu, E)
+    c = rng.uniform(-0.5, 0.5, size=nd)
+    p.psi0 = normalized(p, gaussian(p, c, rng.uniform(0.3, 0.6)))
+    return p
+
+
+def batch_jobs(n1d=256, n2d=256, steps=500, seed=2026, sizes_1d=None, sizes_2d=None):
+    """The batched workload: n1d 1D jobs with N drawn from SIZES_1D_BATCH and
+    n2d 2D jobs N x N with N drawn from SIZES_2D_BATCH (seeded)."""
+    rng = np.random.default_rng(seed)
+    s1 = SIZES_1D_BATCH if sizes_1d is None else sizes_1d
+    s2 = SIZES_2D_BATCH if sizes_2d is None else sizes_2d
+    jobs = []
+    for _ in range(n1d):
+        jobs.append(driven_job([int(rng.choice(s1))], rng, steps))
+    for _ in range(n2d):
+        n = int(rng.choice(s2))
+        jobs.append(driven_job([n, n], rng, steps))
+    return jobs

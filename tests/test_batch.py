@@ -1,0 +1,143 @@
+"""Batched independent Chebyshev jobs (qdx_batch_*).
+
+Each job of a batch is one QDng propagation (OCheby real-time steps of its
+own Hamiltonian, ProgPropa.cpp:317-326): its result must equal the
+single-propagation path's and, in FP64, the oracle's; it must not depend on
+the other jobs in the batch or on the tick it started at; jobs are admitted
+only while they fit into the batch's HBM budget.
+"""
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+def note(msg):
+    print("[gpu case]", msg, flush=True)
+
+
+def mixed_jobs(steps, seed=5):
+    """1D lines on both row-tile block sizes (<= 2048 and 3072 / 4096 points),
+    2D jobs with square and non-square grids, all laser driven."""
+    rng = np.random.default_rng(seed)
+    shapes = [[128], [384], [3072], [4096], [64, 64], [96, 64], [64, 192], [128, 128]]
+    return [cases.driven_job(s, rng, steps) for s in shapes]
+
+
+def single_path(ctx, p, dt, steps, precision):
+    import qdng_amd as Q
+    U = Q.Cheby(p.gpu_ham(ctx), dt, prec=1e-10, precision=precision)
+    w = p.gpu_wf(ctx)
+    _, _, nh, _ = U.run(w, 0, steps, observables=False)
+    return w.download(), int(nh.sum())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_matches_single_path(ctx, precision):
+    import qdng_amd as Q
+    steps = 12
+    jobs = mixed_jobs(steps)
+    note(f"batch vs single path, {precision}, {len(jobs)} jobs")
+    B = Q.Batch(ctx, precision, 1 << 30)
+    ids = [B.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in jobs]
+    B.run(steps)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    for p, jid in zip(jobs, ids):
+        left, nh, done = B.state(jid)
+        got = B.fetch(jid)
+        ref, nh_ref = single_path(ctx, p, 1.0, steps, precision)
+        note(f"  job N={p.N} nhpsi {nh}/{nh_ref} rel {rel(got, ref):.2e}")
+        assert left == 0 and done == steps
+        if precision == "fp64":
+            assert nh == nh_ref
+        assert rel(got, ref) < tol
+
+
+def test_batch_fp64_matches_oracle(ctx):
+    import qdng_amd as Q
+    steps = 6
+    jobs = mixed_jobs(steps, seed=11)
+    pick = [jobs[1], jobs[5]]                    # 1D N=384 and 2D 96 x 64
+    note("batch fp64 vs oracle")
+    B = Q.Batch(ctx, "fp64", 1 << 30)
+    ids = [B.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in pick]
+    B.run(steps)
+    for p, jid in zip(pick, ids):
+        U = O.Cheby(p.oracle_ham(), 1.0, prec=1e-10)
+        psi = p.psi0.copy()
+        nh = sum(U.step(psi, i) for i in range(steps))
+        _, nh_b, _ = B.state(jid)
+        assert nh_b == nh
+        assert rel(B.fetch(jid), psi) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_job_result_same_alone_and_in_mix(ctx, precision):
+    import qdng_amd as Q
+    steps = 10
+    jobs = mixed_jobs(steps + 20, seed=3)
+    target = jobs[6]                              # 2D 64 x 192
+    note(f"alone vs mix, {precision}")
+    A = Q.Batch(ctx, precision, 1 << 30)
+    a = A.submit(target.gpu_ham(None), 1.0, steps, target.psi0)
+    A.run(steps)
+    alone = A.fetch(a)
+    _, nh_alone, _ = A.state(a)
+    M = Q.Batch(ctx, precision, 1 << 30)
+    others = [M.submit(p.gpu_ham(None), 1.0, steps + 20, p.psi0) for p in jobs if p is not target]
+    M.run(3)                                      # the target joins a running batch
+    t = M.submit(target.gpu_ham(None), 1.0, steps, target.psi0)
+    M.run(4)
+    M.release(others[0])                          # the mix changes while it runs
+    M.run(steps)
+    mixed = M.fetch(t)
+    _, nh_mix, done = M.state(t)
+    assert done == steps
+    assert nh_mix == nh_alone
+    assert np.array_equal(mixed, alone)
+
+
+def test_admission_by_footprint(ctx):
+    import qdng_amd as Q
+    steps = 4
+    jobs = mixed_jobs(steps, seed=9)
+    note("admission by footprint")
+    sizes = []
+    probe = Q.Batch(ctx, "fp32", 1 << 30)
+    for p in jobs:
+        sizes.append(probe.footprint(p.gpu_ham(None), 1.0, steps, p.psi0))
+    budget = sizes[4] + sizes[5] + 4096
+    B = Q.Batch(ctx, "fp32", budget)
+    a = B.try_submit(jobs[4].gpu_ham(None), 1.0, steps, jobs[4].psi0)
+    b = B.try_submit(jobs[5].gpu_ham(None), 1.0, steps, jobs[5].psi0)
+    assert a is not None and b is not None
+    assert B.try_submit(jobs[7].gpu_ham(None), 1.0, steps, jobs[7].psi0) is None   # 128 x 128 does not fit
+    n, used, _ = B.usage()
+    assert n == 2 and used <= budget
+    B.run(steps)
+    B.release(a)
+    B.release(b)
+    c = B.try_submit(jobs[4].gpu_ham(None), 1.0, steps, jobs[4].psi0)
+    assert c is not None
+    B.run(steps)
+    assert B.state(c)[2] == steps
+
+
+def test_batch_rejects_unsupported_jobs(ctx):
+    import qdng_amd as Q
+    p = cases.morse_2state_2d(64)
+    B = Q.Batch(ctx, "fp64", 1 << 26)
+    with pytest.raises(Q.QdxError) as e:
+        B.submit(p.gpu_ham(None), 1.0, 2, p.psi0)
+    assert e.value.code == -2
+    q = cases.laser_1d(128, nt=5)
+    with pytest.raises(Q.QdxError) as e:
+        B.submit(q.gpu_ham(None), 1.0, 10, q.psi0)   # laser shorter than the run
+    assert e.value.code == -1
