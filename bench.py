@@ -96,6 +96,89 @@ def run_gspo(ctx, p, precision, steps, warmup):
     return time.perf_counter() - t0
 
 
+# batched jobs, single state with a laser dipole: phi_{k-1}, phi_{k-2}, phi_k,
+# psi read/write, V and mu per grid point and executed order
+B_BATCH = {"fp64": 16 * 3 + 32 + 8 + 8, "fp32": 8 * 3 + 32 + 4 + 4}
+B_BATCH_M2 = {"fp64": 16, "fp32": 8}   # the prologue reads no phi_{k-2}
+
+
+def run_batched(ctx, precision, steps, prof_ticks, seed=2026):
+    """512 laser-driven jobs (256 1D, 256 2D, sizes drawn from seed 2026),
+    dt = 1; one tick = one step of every job.  Returns throughput, kernel
+    timings and the final states."""
+    import cases
+    import qdng_amd as Q
+    jobs = cases.batch_jobs(steps=steps + prof_ticks + 1, seed=seed)
+    B = Q.Batch(ctx, precision, 16 << 30)
+    ids = [B.submit(p.gpu_ham(None), 1.0, steps + prof_ticks + 1, p.psi0) for p in jobs]
+    B.run(1)                       # first tick builds the tables
+    sync(ctx)
+    h0 = np.array([B.state(j)[1] for j in ids])
+    t0 = time.perf_counter()
+    B.run(steps)
+    sync(ctx)
+    t1 = time.perf_counter()
+    h1 = np.array([B.state(j)[1] for j in ids])
+    sizes = np.array([p.size for p in jobs])
+    work = float((sizes * (h1 - h0)).sum())
+    ctx.profile(True)
+    B.run(prof_ticks)
+    sync(ctx)
+    h2 = np.array([B.state(j)[1] for j in ids])
+    rowk = ctx.kernel_time(Q.K_BATCH_ROW)
+    colk = ctx.kernel_time(Q.K_BATCH_COL)
+    linek = ctx.kernel_time(Q.K_BATCH_LINE)
+    ctx.profile(False)
+    two_d = np.array([p.nd == 2 for p in jobs])
+    alg = sizes * (B_BATCH[precision] * (h2 - h1) - B_BATCH_M2[precision] * prof_ticks)
+    return {"seconds": t1 - t0, "work": work, "row": rowk, "col": colk, "line": linek,
+            "row_bytes": float(alg[two_d].sum()), "line_bytes": float(alg[~two_d].sum()),
+            "work_2d": float((sizes * (h1 - h0))[two_d].sum()),
+            "jobs": jobs, "ids": ids, "batch": B, "points": int(sizes.sum())}
+
+
+def batched_extra(ctx, steps, prof_ticks):
+    out = {}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        r = run_batched(ctx, prec, steps, prof_ticks)
+        res[prec] = r
+        n, ms = r["row"]
+        nc, msc = r["col"]
+        nl, msl = r["line"]
+        ach = r["row_bytes"] / (ms * 1e-3) / 1e9
+        value = r["work"] / r["seconds"]
+        out[prec] = {
+            "value": value, "unit": UNIT, "ms_per_tick": r["seconds"] * 1e3 / steps,
+            "effective_GBs": B_BATCH[prec] * value / 1e9,
+            "effective_frac": B_BATCH[prec] * value / 1e9 / HBM_PEAK_GBS,
+            "value_2d_jobs": r["work_2d"] / r["seconds"],
+            "row_kernel": {"achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "launches": n,
+                           "avg_launch_us": ms / max(n, 1) * 1e3, "jobs": "2D"},
+            "col_kernel": {"launches": nc, "avg_launch_us": msc / max(nc, 1) * 1e3},
+            "line_kernel": {"launches": nl, "avg_launch_us": msl / max(nl, 1) * 1e3, "jobs": "1D",
+                            "algorithmic_GBs": r["line_bytes"] / max(msl * 1e-3, 1e-12) / 1e9,
+                            "note": "whole step per block with the line in registers: the algorithmic "
+                                    "bytes are not HBM traffic"},
+        }
+        log(f"batched {prec}: {value:.4g} grid-pt*Hpsi/s, {r['seconds'] * 1e3 / steps:.2f} ms/tick, "
+            f"row kernel {ach:.0f} GB/s")
+    out["fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    b64, b32 = res["fp64"]["batch"], res["fp32"]["batch"]
+    errs = []
+    for j64, j32 in zip(res["fp64"]["ids"], res["fp32"]["ids"]):
+        a, b = b64.fetch(j64), b32.fetch(j32)
+        errs.append(np.linalg.norm(b - a) / np.linalg.norm(a))
+    w = int(np.argmax(errs))
+    out["l2_worst_job"] = {"N": res["fp64"]["jobs"][w].N, "l2": float(errs[w])}
+    out["l2_fp32_vs_fp64_max"] = float(max(errs))
+    out["l2_fp32_vs_fp64_median"] = float(np.median(errs))
+    out["workload"] = {"jobs": 512, "points": res["fp64"]["points"], "dt": 1.0,
+                       "timed_ticks": steps, "profiled_ticks": prof_ticks,
+                       "bytes_model": "96 B (fp64) / 64 B (fp32) per grid point per executed order"}
+    return out
+
+
 def cpu_baseline(p, steps):
     """FP64 oracle on the host cores for `steps` Chebyshev steps from psi0."""
     from oracle import oracle as O
@@ -119,6 +202,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=100, help="timed ticks of the batched-jobs workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -230,6 +314,8 @@ def main():
             ex[f"gspo_{prec}"] = {"value": npts * gsteps / t, "unit": "grid-pt*step/s",
                                   "ms_per_step": t * 1e3 / gsteps, "steps": gsteps}
             log(f"gspo {prec}: {npts * gsteps / t:.4g} grid-pt*step/s")
+        if args.batch_steps > 0:
+            ex["batched"] = batched_extra(ctx, args.batch_steps, 5)
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -78,7 +78,8 @@ typedef enum qdx_kernel_class {
     QDX_K_GSPO_ROW = 3,     /* GSPO x-space split operators + FFT_0 / IFFT_0       */
     QDX_K_BATCH_ROW = 4,    /* batched jobs: row tiles of every job (one launch)  */
     QDX_K_BATCH_COL = 5,    /* batched jobs: dimension-1 kinetic pass, 2D jobs     */
-    QDX_NUM_KERNEL_CLASSES = 6
+    QDX_K_BATCH_LINE = 6,   /* batched jobs: whole time step of each 1D job        */
+    QDX_NUM_KERNEL_CLASSES = 7
 } qdx_kernel_class;
 int qdx_ctx_profile(qdx_ctx *ctx, int32_t on);
 int qdx_ctx_kernel_time(qdx_ctx *ctx, int32_t kernel_class, int64_t *launches, double *total_ms);
@@ -186,11 +187,12 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
 int qdx_prop_destroy(qdx_prop *p);
 
 /* ---- batched jobs ---------------------------------------------------------
- * Many independent single-state Chebyshev propagations (1D or 2D grids of
- * different sizes, each with its own potential, dipole and laser field) kept
- * resident in HBM and advanced together: one tick is one time step
- * (OCheby::real_time_step, OCheby.cpp:92-197) of every active job, each pass
- * of the series one launch over the tiles of all jobs.  QDng runs one
+ * Many independent propagations (1D or 2D grids of different sizes, each
+ * with its own potential, dipole and laser field): single-state Chebyshev
+ * jobs and GSPO jobs on one or two coupled states, kept resident in HBM and
+ * advanced together.  One tick is one time step of every active job
+ * (OCheby::real_time_step, OCheby.cpp:92-197; OGSPO::Apply,
+ * OGSPO.cpp:114-139); each pass is one launch over the tiles of all jobs.  QDng runs one
  * propagation per program (ProgPropa::Run, ProgPropa.cpp:317-326); a job here
  * is exactly that propagation with its own series length and break test, so
  * its result does not depend on the other jobs in the batch.
@@ -198,17 +200,24 @@ int qdx_prop_destroy(qdx_prop *p);
  * Device memory of the jobs comes from one arena of `hbm_budget` bytes
  * allocated by qdx_batch_create; qdx_batch_submit fails with QDX_ERR_MEMORY
  * (nothing allocated) when the job does not fit, so callers can queue it.
- * Grids: 1D N <= 4096, 2D N0 <= 4096 and N1 <= 2048, N = 8 * 2^a * 3^b. */
+ * Grids: 1D or 2D, each N <= 4096 and of the form 8 * 2^a * 3^b.  One block
+ * runs a 1D job's whole step with the line held in registers. */
 typedef struct qdx_batch qdx_batch;
 typedef struct qdx_job_desc {
-    qdx_ham *ham;          /* one state: kinetic, potential and dipole terms
-                              (OSum of OGridNablaSq, OGridPotential,
-                              OGridDipole); created with ctx NULL or not      */
-    double dt;
+    qdx_ham *ham;          /* Chebyshev job: one state, kinetic, potential and
+                              dipole terms (OSum of OGridNablaSq,
+                              OGridPotential, OGridDipole)                     */
+    qdx_ham *const *ops;   /* GSPO job (ham NULL): split operators {T, W}, T
+                              kinetic terms only, W potentials on 1 or 2
+                              states (+ a dipole on one state); OGSPO with
+                              operators A = T, B = W (OGSPO.cpp:30-139)        */
+    int32_t nops;          /* 2 for a GSPO job, 0 otherwise                    */
     int32_t nsteps;
-    int32_t order;         /* 0: OCheby's automatic order                      */
-    double prec;           /* < 0: OCheby's default 1e-10                       */
-    const double *psi0;    /* complex128 initial state on the ham's grid        */
+    double dt;
+    int32_t order;         /* Chebyshev: 0 = OCheby's automatic order          */
+    int32_t reserved;
+    double prec;           /* Chebyshev: < 0 = OCheby's default 1e-10          */
+    const double *psi0;    /* complex128 initial state [S][N]                  */
 } qdx_job_desc;
 
 int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_batch **out);
@@ -219,14 +228,31 @@ int qdx_batch_submit(qdx_batch *b, const qdx_job_desc *job, int64_t *id);
 /* enqueue `nticks` ticks (asynchronous; qdx_ctx_synchronize waits) */
 int qdx_batch_run(qdx_batch *b, int32_t nticks);
 /* steps still to run (0: finished once the enqueued ticks complete), and,
- * after a synchronisation, H applications and steps executed so far */
+ * after a synchronisation, H applications (GSPO: steps) and steps executed */
 int qdx_batch_job_state(qdx_batch *b, int64_t id, int64_t *steps_left, int64_t *n_hpsi, int64_t *steps_done);
-/* wait for the enqueued ticks and copy the job's psi (complex128) */
+/* wait for the enqueued ticks and copy the job's psi (complex128, [S][N]) */
 int qdx_batch_fetch(qdx_batch *b, int64_t id, double *psi);
 /* wait for the enqueued ticks and return the job's memory to the arena */
 int qdx_batch_release(qdx_batch *b, int64_t id);
 /* jobs held, arena bytes in use, ticks enqueued so far */
 int qdx_batch_usage(qdx_batch *b, int32_t *jobs, int64_t *bytes_used, int64_t *ticks);
+
+/* Continuous batching over a stream of requests.  Request i arrives at
+ * arrival_s[i] seconds after the call starts (non-decreasing).  Arrived
+ * requests are admitted in arrival order as soon as the arena has room for
+ * them (and fewer than max_active jobs are held, if max_active > 0); every
+ * tick advances all admitted jobs by one step, so the work of jobs admitted
+ * at different times overlaps on the device; a job leaves after its last
+ * step and its memory is reused.  psi_out (nullable, entries nullable)
+ * receives each final psi (complex128).  Returns when all requests are done;
+ * a request larger than the whole budget fails with QDX_ERR_MEMORY. */
+typedef struct qdx_serve_result {
+    double admit_s;        /* admission, seconds after the start (host clock)  */
+    double finish_s;       /* end of the job's last step (device clock)        */
+    int64_t n_hpsi;        /* H applications (GSPO: steps)                      */
+} qdx_serve_result;
+int qdx_batch_serve(qdx_batch *b, const qdx_job_desc *jobs, const double *arrival_s, int32_t n,
+                    int32_t max_active, double *const *psi_out, qdx_serve_result *out);
 int qdx_batch_destroy(qdx_batch *b);
 
 #ifdef __cplusplus

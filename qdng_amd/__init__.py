@@ -56,15 +56,20 @@ EXPORTS = [
     "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
     "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
     "qdx_batch_job_state", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
-    "qdx_batch_destroy",
+    "qdx_batch_serve", "qdx_batch_destroy",
 ]
 
-K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL = 0, 1, 2, 3, 4, 5
+K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
 
 
 class JobDesc(C.Structure):
-    _fields_ = [("ham", C.c_void_p), ("dt", C.c_double), ("nsteps", C.c_int32), ("order", C.c_int32),
-                ("prec", C.c_double), ("psi0", C.POINTER(C.c_double))]
+    _fields_ = [("ham", C.c_void_p), ("ops", C.POINTER(C.c_void_p)), ("nops", C.c_int32), ("nsteps", C.c_int32),
+                ("dt", C.c_double), ("order", C.c_int32), ("reserved", C.c_int32), ("prec", C.c_double),
+                ("psi0", C.POINTER(C.c_double))]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double), ("n_hpsi", C.c_int64)]
 
 
 def lib():
@@ -120,6 +125,7 @@ def lib():
         "qdx_batch_fetch": [vp, C.c_int64, dp],
         "qdx_batch_release": [vp, C.c_int64],
         "qdx_batch_usage": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+        "qdx_batch_serve": [vp, C.POINTER(JobDesc), dp, i32, i32, C.POINTER(dp), C.POINTER(ServeResult)],
         "qdx_batch_destroy": [vp],
     }
     for name, args in sig.items():
@@ -391,38 +397,77 @@ class Batch:
         self.sizes = {}
 
     @staticmethod
-    def _desc(H: "Hamiltonian", dt, nsteps, psi0, order=0, prec=-1.0):
+    def job(H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None):
+        """A job description: Chebyshev with Hamiltonian H (one state), or
+        GSPO with split operators ops = [T, W]."""
         psi = np.ascontiguousarray(psi0, dtype=np.complex128).ravel()
         d = JobDesc()
-        d.ham = H.h.value
+        keep = [psi]
+        if ops is not None:
+            arr = (C.c_void_p * len(ops))(*[o.h.value for o in ops])
+            keep.append(arr)
+            keep.extend(ops)
+            d.ham = None
+            d.ops = C.cast(arr, C.POINTER(C.c_void_p))
+            d.nops = len(ops)
+        else:
+            keep.append(H)
+            d.ham = H.h.value
+            d.nops = 0
         d.dt = float(dt)
         d.nsteps = int(nsteps)
         d.order = int(order)
         d.prec = float(prec)
         d.psi0 = psi.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
-        return d, psi
+        return d, keep
 
-    def footprint(self, H, dt, nsteps, psi0, order=0, prec=-1.0) -> int:
-        d, psi = self._desc(H, dt, nsteps, psi0, order, prec)
+    def footprint(self, H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None) -> int:
+        d, keep = self.job(H, dt, nsteps, psi0, order, prec, ops)
         out = C.c_int64()
         _chk(lib().qdx_batch_footprint(self.h, C.byref(d), C.byref(out)))
         return out.value
 
-    def submit(self, H, dt, nsteps, psi0, order=0, prec=-1.0) -> int:
-        d, psi = self._desc(H, dt, nsteps, psi0, order, prec)
+    def submit(self, H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None) -> int:
+        d, keep = self.job(H, dt, nsteps, psi0, order, prec, ops)
         jid = C.c_int64()
         _chk(lib().qdx_batch_submit(self.h, C.byref(d), C.byref(jid)))
-        self.sizes[jid.value] = psi.size
+        self.sizes[jid.value] = keep[0].size
         return jid.value
 
-    def try_submit(self, H, dt, nsteps, psi0, order=0, prec=-1.0):
+    def try_submit(self, H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None):
         """submit, or None when the job does not fit into the free budget"""
         try:
-            return self.submit(H, dt, nsteps, psi0, order, prec)
+            return self.submit(H, dt, nsteps, psi0, order, prec, ops)
         except QdxError as e:
             if e.code == self.ERR_MEMORY:
                 return None
             raise
+
+    def serve(self, jobs, arrivals, max_active: int = 0, want_psi: bool = False):
+        """Continuous batching over requests (kwargs dicts for job()) arriving
+        at `arrivals` seconds; returns (admit_s, finish_s, n_hpsi[, psi list])."""
+        n = len(jobs)
+        descs = (JobDesc * max(n, 1))()
+        keep = []
+        for i, kw in enumerate(jobs):
+            d, k = self.job(**kw)
+            descs[i] = d
+            keep.append(k)
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        res = (ServeResult * max(n, 1))()
+        outs = None
+        psis = []
+        if want_psi:
+            outs = (C.POINTER(C.c_double) * max(n, 1))()
+            for i, k in enumerate(keep):
+                a = np.zeros(k[0].size, dtype=np.complex128)
+                psis.append(a)
+                outs[i] = a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
+        _chk(lib().qdx_batch_serve(self.h, descs, _dp(arr), n, int(max_active), outs, res))
+        admit = np.array([res[i].admit_s for i in range(n)])
+        finish = np.array([res[i].finish_s for i in range(n)])
+        nh = np.array([res[i].n_hpsi for i in range(n)])
+        return (admit, finish, nh, psis) if want_psi else (admit, finish, nh)
 
     def run(self, nticks: int = 1) -> None:
         _chk(lib().qdx_batch_run(self.h, int(nticks)))

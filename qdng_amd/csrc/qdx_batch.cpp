@@ -1,10 +1,13 @@
 // Host runtime of the batched-jobs path (include/qdx.h, "batched jobs"):
 // device arena, job admission by footprint, the job/tile tables the batched
-// kernels read, and the per-tick launch sequence.
+// kernels read, the per-tick launch sequence and the continuous-batching
+// serve loop.
 #include "qdx_batch.hpp"
 #include "qdx_host_internal.hpp"
 
+#include <chrono>
 #include <deque>
+#include <thread>
 
 namespace {
 
@@ -56,9 +59,10 @@ struct Arena {
     char *base() const { return buf.as<char>(); }
 };
 
-// Sub-buffers of one job inside its arena block.
+// Sub-buffers of one job inside its arena block (offsets; the job's flags say
+// which exist).
 struct JobLayout {
-    size_t psi, A, B, X, V, mu, tk0, tk1, laser, coef, part, ctl, stats, bytes;
+    size_t psi, A, B, X, V, mu, V11, V01, tk0, tk1, tk0s1, tk1s1, laser, coef, part, ctl, stats, bytes;
 };
 
 // Tiling of one job: row tiles of rb lines (NT threads), column tiles of W
@@ -67,18 +71,24 @@ struct JobTiling {
     int NT = 256, rb = 1, nrt = 1, W = 1, nct = 0;
 };
 
+enum JobKind { K_CHEBY = 0, K_GSPO = 1 };
+
 struct BatchJob {
     int64_t id = 0;
+    int kind = K_CHEBY;
+    int S = 1;
     Grid grid;
-    ChebyParams cp;
+    ChebyParams cp;          // Chebyshev jobs
+    double dt = 0;
     int nsteps = 0;
     int64_t start_tick = 0;
     size_t off = 0;
     JobLayout lay{};
     JobTiling tile;
-    bool has_V = false, has_mu = false, has_T = false;
+    bool has_V = false, has_mu = false, has_T = false, has_V11 = false, has_V01 = false, has_T1 = false;
     double lscale = 1.0;
     int nt = 0;
+    int64_t last_tick() const { return start_tick + nsteps - 1; }
     bool active_at(int64_t t) const { return t >= start_tick && t < start_tick + nsteps; }
 };
 
@@ -101,7 +111,91 @@ JobTiling job_tiling(const Grid &g) {
     return t;
 }
 
-JobLayout job_layout(const Grid &g, const JobTiling &t, int order, int nt, size_t rsz, bool has_mu) {
+// Host arrays of one job.
+struct JobHost {
+    std::vector<double> V, mu, V11, V01, tk0, tk1, tk0s1, tk1s1, laser;
+    double lscale = 1.0;
+    bool has_T = false, has_T1 = false;
+};
+
+void add_kinetic(JobHost &j, const Grid &g, const Term &t) {
+    for (int d = 0; d < g.nd; ++d) {
+        std::vector<double> &dst = t.row == 0 ? (d == 0 ? j.tk0 : j.tk1) : (d == 0 ? j.tk0s1 : j.tk1s1);
+        const std::vector<double> k1 = kspace_d2(g, d, t.mass[d]);
+        if (dst.empty()) dst = k1;
+        else
+            for (int q = 0; q < g.N[d]; ++q) dst[q] += k1[q];
+    }
+    (t.row == 0 ? j.has_T : j.has_T1) = true;
+}
+
+void add_field(std::vector<double> &dst, const std::vector<double> &v) {
+    if (dst.empty()) dst.assign(v.size(), 0.0);
+    for (size_t x = 0; x < v.size(); ++x) dst[x] += v[x];
+}
+
+// Chebyshev job: H = OSum of grid terms on one state
+JobHost cheby_host(qdx_ham *h, int nsteps) {
+    JobHost j;
+    const std::vector<double> *las = nullptr;
+    for (const Term &t : h->terms) {
+        if (t.row != 0 || t.col != 0) fail(QDX_ERR_INCOMPAT, "batched Chebyshev jobs hold one state");
+        if (t.kind == T_POT) {
+            add_field(j.V, t.v);
+        } else if (t.kind == T_DIP) {
+            if (las && (*las != t.laser || j.lscale != t.scale))
+                fail(QDX_ERR_INCOMPAT, "two dipole terms with different fields on one element");
+            add_field(j.mu, t.v);
+            las = &t.laser;
+            j.lscale = t.scale;
+        } else {
+            add_kinetic(j, h->grid, t);
+        }
+    }
+    if (las) {
+        if ((long)las->size() < nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
+        j.laser = *las;
+    }
+    return j;
+}
+
+// GSPO job: operators {T, W} (OGSPO with A = kinetic, B = potential)
+JobHost gspo_host(qdx_ham *T, qdx_ham *W, int nsteps) {
+    JobHost j;
+    const int S = T->S;
+    for (const Term &t : T->terms) {
+        if (t.kind != T_KIN) fail(QDX_ERR_INCOMPAT, "the first GSPO operator must hold kinetic terms only");
+        add_kinetic(j, T->grid, t);
+    }
+    const std::vector<double> *las = nullptr;
+    bool c01 = false, c10 = false;
+    for (const Term &t : W->terms) {
+        if (t.kind == T_KIN) fail(QDX_ERR_INCOMPAT, "the second GSPO operator must hold potentials only");
+        if (t.kind == T_DIP) {
+            if (S != 1) fail(QDX_ERR_INCOMPAT, "batched GSPO jobs take a dipole on a single state only");
+            if (las && (*las != t.laser || j.lscale != t.scale))
+                fail(QDX_ERR_INCOMPAT, "two dipole terms with different fields on one element");
+            add_field(j.mu, t.v);
+            las = &t.laser;
+            j.lscale = t.scale;
+            continue;
+        }
+        if (t.row == 0 && t.col == 0) add_field(j.V, t.v);
+        else if (t.row == 1 && t.col == 1) add_field(j.V11, t.v);
+        else {
+            (t.row == 0 ? c01 : c10) = true;
+            add_field(j.V01, t.v);
+        }
+    }
+    if (c01 && c10) fail(QDX_ERR_INCOMPAT, "batched GSPO jobs take one coupling element (mirrored)");
+    if (las) {
+        if ((long)las->size() < nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
+        j.laser = *las;
+    }
+    return j;
+}
+
+JobLayout job_layout(const BatchJob &J, size_t rsz) {
     JobLayout l{};
     size_t o = 0;
     auto take = [&](size_t n) {
@@ -109,18 +203,29 @@ JobLayout job_layout(const Grid &g, const JobTiling &t, int order, int nt, size_
         o += Arena::round(std::max<size_t>(n, 1));
         return at;
     };
-    const size_t n = (size_t)g.size();
-    l.psi = take(16 * n);
-    l.A = take(2 * rsz * n);
-    l.B = take(2 * rsz * n);
-    l.X = g.nd == 2 ? take(2 * rsz * n) : 0;
-    l.V = take(rsz * n);
-    l.mu = has_mu ? take(rsz * n) : 0;
+    const Grid &g = J.grid;
+    const size_t n = (size_t)g.size(), S = (size_t)J.S;
+    if (J.kind == K_CHEBY) {
+        l.psi = take(16 * n);
+        l.A = take(2 * rsz * n);
+        l.B = take(2 * rsz * n);
+        l.coef = take(16 * (size_t)J.cp.order);
+        l.part = take(8 * 3 * (size_t)J.tile.nrt);
+    } else {
+        l.psi = take(2 * rsz * n * S);
+    }
+    if (g.nd == 2) l.X = take(2 * rsz * n * S);
+    if (J.has_V) l.V = take(rsz * n);
+    if (J.has_mu) l.mu = take(rsz * n);
+    if (J.has_V11) l.V11 = take(rsz * n);
+    if (J.has_V01) l.V01 = take(rsz * n);
     l.tk0 = take(8 * (size_t)g.N[0]);
-    l.tk1 = g.nd == 2 ? take(8 * (size_t)g.N[1]) : 0;
-    l.laser = take(8 * (size_t)std::max(nt, 1));
-    l.coef = take(16 * (size_t)order);
-    l.part = take(8 * 3 * (size_t)t.nrt);
+    if (g.nd == 2) l.tk1 = take(8 * (size_t)g.N[1]);
+    if (J.has_T1) {
+        l.tk0s1 = take(8 * (size_t)g.N[0]);
+        if (g.nd == 2) l.tk1s1 = take(8 * (size_t)g.N[1]);
+    }
+    if (J.has_mu) l.laser = take(8 * (size_t)std::max(J.nt, 1));
     l.ctl = take(sizeof(ChebyCtl));
     l.stats = take(2 * sizeof(long long));
     l.bytes = o;
@@ -135,55 +240,47 @@ size_t col_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw) {
     const int L = g.N[1];
     return ((size_t)2 * t.W * L + (tw ? L : 0)) * esz;
 }
+size_t line_lds(const Grid &g, size_t esz, bool tw) {
+    const int L = g.N[0];
+    return ((size_t)2 * (L + L / 8) + (tw ? L : 0)) * esz;
+}
 
 constexpr size_t kLds = 160 * 1024;
 
-// Host arrays of one job from its Hamiltonian (OSum of grid terms).
-struct JobHost {
-    std::vector<double> V, mu, tk0, tk1, laser;
-    double lscale = 1.0;
-    bool has_T = false;
+// LDS bytes of one launch: the largest tile, with the twiddle table staged in
+// LDS when that still fits (else all blocks read the table from memory)
+struct LdsNeed {
+    size_t with = 0, without = 0;
+    void add(size_t w, size_t wo) {
+        with = std::max(with, w);
+        without = std::max(without, wo);
+    }
+    bool tw() const { return with <= kLds; }
+    size_t bytes() const { return tw() ? with : without; }
 };
 
-JobHost job_host(qdx_ham *h, int nsteps) {
-    JobHost j;
-    const Grid &g = h->grid;
-    const long n = g.size();
-    const std::vector<double> *las = nullptr;
-    for (const Term &t : h->terms) {
-        if (t.row != 0 || t.col != 0) fail(QDX_ERR_INCOMPAT, "batched jobs hold one state");
-        if (t.kind == T_POT) {
-            if (j.V.empty()) j.V.assign(n, 0.0);
-            for (long x = 0; x < n; ++x) j.V[x] += t.v[x];
-        } else if (t.kind == T_DIP) {
-            if (las && (*las != t.laser || j.lscale != t.scale))
-                fail(QDX_ERR_INCOMPAT, "two dipole terms with different fields on one element");
-            if (j.mu.empty()) j.mu.assign(n, 0.0);
-            for (long x = 0; x < n; ++x) j.mu[x] += t.v[x];
-            las = &t.laser;
-            j.lscale = t.scale;
-        } else {
-            for (int d = 0; d < g.nd; ++d) {
-                std::vector<double> &dst = d == 0 ? j.tk0 : j.tk1;
-                const std::vector<double> k1 = kspace_d2(g, d, t.mass[d]);
-                if (dst.empty()) dst = k1;
-                else
-                    for (int q = 0; q < g.N[d]; ++q) dst[q] += k1[q];
-            }
-            j.has_T = true;
-        }
-    }
-    if (las) {
-        if ((long)las->size() < nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
-        j.laser = *las;
-    }
-    return j;
-}
+// One launch list: tiles (or job indices) with its LDS need and, for the
+// per-order Chebyshev lists, (order, tile count) prefix marks.
+struct TileList {
+    std::vector<BTile> tiles;
+    std::vector<int> jobs;
+    LdsNeed lds;
+    std::vector<std::pair<int, int>> pre;
+    int n2d = 0;   // GSPO row lists: the 2D jobs' tiles come first
+};
 
-// Double-buffered device tables: the set written for a new job mix is not
-// the one the ticks already enqueued read.
+enum ListId { L_ROW256, L_ROW512, L_COL, L_LINE256, L_LINE512, L_GROW256, L_GROW512, L_GCOL, L_COUNT };
+
+bool is_line_list(int i) { return i == L_LINE256 || i == L_LINE512; }
+
+// Device tables, double-buffered: the set written for a new job mix is not
+// the one the ticks already enqueued read.  Uploads go through pinned staging
+// memory, so a rebuild does not wait for the device.
 struct TableSet {
-    DevBuf jobs, r256, r512, col, plans;
+    DevBuf dev;
+    char *host = nullptr;
+    size_t cap = 0;
+    size_t off_jobs = 0, off_plans = 0, off_list[L_COUNT] = {};
     hipEvent_t last_use = nullptr;
     bool used = false;
 };
@@ -201,11 +298,16 @@ struct qdx_batch {
     TableSet sets[2];
     int cur = 0;
     bool dirty = true;
-    // current tables
-    std::vector<int64_t> order_ids;        // job index -> id
-    int n256 = 0, n512 = 0, ncol = 0;
-    size_t lds256 = 0, lds512 = 0, ldsc = 0;
-    bool tw256 = true, tw512 = true, twc = true;
+    int njobs_tab = 0;
+    int count[L_COUNT] = {};
+    size_t lds[L_COUNT] = {};
+    bool twl[L_COUNT] = {};
+    std::vector<std::pair<int, int>> pre[L_COUNT];
+    int n2d[L_COUNT] = {};
+    // 1D Chebyshev jobs run on a side stream that forks from and joins the
+    // main stream every tick
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     size_t rsz() const { return precision == QDX_FP32_BULK ? sizeof(float) : sizeof(double); }
 };
 
@@ -216,25 +318,34 @@ void build_tables(qdx_batch *b) {
     const bool fp32 = std::is_same<R, float>::value;
     const size_t esz = 2 * sizeof(R);
     std::vector<BJob<R>> tj;
-    std::vector<BTile> r256, r512, cols;
+    TileList L[L_COUNT];
     std::vector<LinePlan> plans;
     std::map<int, int> pidx;
-    auto plan_of = [&](int L) {
-        auto it = pidx.find(L);
+    auto plan_of = [&](int len) {
+        auto it = pidx.find(len);
         if (it != pidx.end()) return it->second;
-        plans.push_back(b->ctx->plan(L, fp32));
-        pidx[L] = (int)plans.size() - 1;
+        plans.push_back(b->ctx->plan(len, fp32));
+        pidx[len] = (int)plans.size() - 1;
         return (int)plans.size() - 1;
     };
-    b->order_ids.clear();
-    size_t l256 = 0, l256n = 0, l512 = 0, l512n = 0, lc = 0, lcn = 0;
+    // Chebyshev jobs by decreasing series length (prefix launches per order),
+    // GSPO 2D jobs before GSPO 1D jobs
+    std::vector<const BatchJob *> order;
+    for (auto &kv : b->jobs) order.push_back(&kv.second);
+    std::stable_sort(order.begin(), order.end(), [](const BatchJob *x, const BatchJob *y) {
+        if (x->kind != y->kind) return x->kind < y->kind;
+        if (x->kind == K_CHEBY) return x->cp.order > y->cp.order;
+        return x->grid.nd > y->grid.nd;
+    });
     char *base = b->arena.base();
-    for (auto &kv : b->jobs) {
-        const BatchJob &J = kv.second;
+    for (const BatchJob *jp : order) {
+        const BatchJob &J = *jp;
         const int ji = (int)tj.size();
-        b->order_ids.push_back(J.id);
         BJob<R> d{};
         char *jb = base + J.off;
+        auto at = [&](size_t off) { return jb + off; };
+        d.kind = J.kind;
+        d.S = J.S;
         d.nd = J.grid.nd;
         d.N0 = J.grid.N[0];
         d.N1 = J.grid.nd == 2 ? J.grid.N[1] : 1;
@@ -243,116 +354,196 @@ void build_tables(qdx_batch *b) {
         d.rb = J.tile.rb;
         d.nrt = J.tile.nrt;
         d.W = J.tile.W;
-        d.order = J.cp.order;
+        d.order = J.kind == K_CHEBY ? J.cp.order : 0;
         d.start_tick = (int)J.start_tick;
         d.nsteps = J.nsteps;
         d.nt = J.nt;
         d.size = J.grid.size();
         d.inv_n = 1.0 / double(d.size);
-        d.offset = J.cp.offset;
-        d.inv_scaling = 1.0 / J.cp.scaling;
-        d.prec = J.cp.prec;
+        d.tau = J.dt;
         d.lscale = J.lscale;
-        d.coef = reinterpret_cast<const double *>(jb + J.lay.coef);
-        d.tk0 = J.has_T ? reinterpret_cast<const double *>(jb + J.lay.tk0) : nullptr;
-        d.tk1 = J.has_T && J.grid.nd == 2 ? reinterpret_cast<const double *>(jb + J.lay.tk1) : nullptr;
-        d.V = J.has_V ? reinterpret_cast<const R *>(jb + J.lay.V) : nullptr;
-        d.mu = J.has_mu ? reinterpret_cast<const R *>(jb + J.lay.mu) : nullptr;
-        d.laser = J.nt > 0 ? reinterpret_cast<const double *>(jb + J.lay.laser) : nullptr;
-        d.psi = reinterpret_cast<double *>(jb + J.lay.psi);
-        d.phiA = reinterpret_cast<R *>(jb + J.lay.A);
-        d.phiB = reinterpret_cast<R *>(jb + J.lay.B);
-        d.X = J.grid.nd == 2 ? reinterpret_cast<R *>(jb + J.lay.X) : nullptr;
-        d.part = reinterpret_cast<double *>(jb + J.lay.part);
-        d.ctl = reinterpret_cast<ChebyCtl *>(jb + J.lay.ctl);
-        d.stats = reinterpret_cast<long long *>(jb + J.lay.stats);
-        std::vector<BTile> &rl = J.tile.NT == 256 ? r256 : r512;
-        d.rt0 = (int)rl.size();
-        for (int r = 0; r < J.tile.nrt; ++r) rl.push_back(BTile{ji, r * J.tile.rb});
-        const size_t lt = row_lds(J.grid, J.tile, esz, true), ln = row_lds(J.grid, J.tile, esz, false);
-        if (J.tile.NT == 256) {
-            l256 = std::max(l256, lt);
-            l256n = std::max(l256n, ln);
+        d.tk0 = J.has_T ? reinterpret_cast<const double *>(at(J.lay.tk0)) : nullptr;
+        d.tk1 = J.has_T && J.grid.nd == 2 ? reinterpret_cast<const double *>(at(J.lay.tk1)) : nullptr;
+        d.tk0s1 = J.has_T1 ? reinterpret_cast<const double *>(at(J.lay.tk0s1)) : nullptr;
+        d.tk1s1 = J.has_T1 && J.grid.nd == 2 ? reinterpret_cast<const double *>(at(J.lay.tk1s1)) : nullptr;
+        d.V = J.has_V ? reinterpret_cast<const R *>(at(J.lay.V)) : nullptr;
+        d.mu = J.has_mu ? reinterpret_cast<const R *>(at(J.lay.mu)) : nullptr;
+        d.V11 = J.has_V11 ? reinterpret_cast<const R *>(at(J.lay.V11)) : nullptr;
+        d.V01 = J.has_V01 ? reinterpret_cast<const R *>(at(J.lay.V01)) : nullptr;
+        d.laser = J.has_mu ? reinterpret_cast<const double *>(at(J.lay.laser)) : nullptr;
+        d.X = J.grid.nd == 2 ? reinterpret_cast<R *>(at(J.lay.X)) : nullptr;
+        d.ctl = reinterpret_cast<ChebyCtl *>(at(J.lay.ctl));
+        d.stats = reinterpret_cast<long long *>(at(J.lay.stats));
+        if (J.kind == K_CHEBY) {
+            d.offset = J.cp.offset;
+            d.inv_scaling = 1.0 / J.cp.scaling;
+            d.prec = J.cp.prec;
+            d.coef = reinterpret_cast<const double *>(at(J.lay.coef));
+            d.psi = reinterpret_cast<double *>(at(J.lay.psi));
+            d.phiA = reinterpret_cast<R *>(at(J.lay.A));
+            d.phiB = reinterpret_cast<R *>(at(J.lay.B));
+            d.part = reinterpret_cast<double *>(at(J.lay.part));
+            if (J.grid.nd == 1) {
+                TileList &l = L[J.tile.NT == 256 ? L_LINE256 : L_LINE512];
+                l.jobs.push_back(ji);
+                l.lds.add(line_lds(J.grid, esz, true), line_lds(J.grid, esz, false));
+            } else {
+                TileList &l = L[J.tile.NT == 256 ? L_ROW256 : L_ROW512];
+                d.rt0 = (int)l.tiles.size();
+                for (int r = 0; r < J.tile.nrt; ++r) l.tiles.push_back(BTile{ji, r * J.tile.rb});
+                l.pre.push_back({J.cp.order, (int)l.tiles.size()});
+                l.lds.add(row_lds(J.grid, J.tile, esz, true), row_lds(J.grid, J.tile, esz, false));
+                TileList &c = L[L_COL];
+                for (int k = 0; k < J.tile.nct; ++k) c.tiles.push_back(BTile{ji, k * J.tile.W});
+                c.pre.push_back({J.cp.order, (int)c.tiles.size()});
+                c.lds.add(col_lds(J.grid, J.tile, esz, true), col_lds(J.grid, J.tile, esz, false));
+            }
         } else {
-            l512 = std::max(l512, lt);
-            l512n = std::max(l512n, ln);
-        }
-        if (J.grid.nd == 2) {
-            for (int c = 0; c < J.tile.nct; ++c) cols.push_back(BTile{ji, c * J.tile.W});
-            lc = std::max(lc, col_lds(J.grid, J.tile, esz, true));
-            lcn = std::max(lcn, col_lds(J.grid, J.tile, esz, false));
+            d.psiR = reinterpret_cast<R *>(at(J.lay.psi));
+            TileList &l = L[J.tile.NT == 256 ? L_GROW256 : L_GROW512];
+            d.rt0 = (int)l.tiles.size();
+            for (int r = 0; r < J.tile.nrt; ++r) l.tiles.push_back(BTile{ji, r * J.tile.rb});
+            if (J.grid.nd == 2) l.n2d = (int)l.tiles.size();
+            l.lds.add(row_lds(J.grid, J.tile, esz, true), row_lds(J.grid, J.tile, esz, false));
+            if (J.grid.nd == 2) {
+                TileList &c = L[L_GCOL];
+                for (int s = 0; s < J.S; ++s)
+                    for (int k = 0; k < J.tile.nct; ++k) c.tiles.push_back(BTile{ji, s * d.N0 + k * J.tile.W});
+                c.lds.add(col_lds(J.grid, J.tile, esz, true), col_lds(J.grid, J.tile, esz, false));
+            }
         }
         tj.push_back(d);
     }
-    b->tw256 = l256 <= kLds;
-    b->lds256 = b->tw256 ? l256 : l256n;
-    b->tw512 = l512 <= kLds;
-    b->lds512 = b->tw512 ? l512 : l512n;
-    b->twc = lc <= kLds;
-    b->ldsc = b->twc ? lc : lcn;
-    if (b->lds256 > kLds || b->lds512 > kLds || b->ldsc > kLds) fail(QDX_ERR_PARAM, "job tile exceeds the LDS");
-    b->n256 = (int)r256.size();
-    b->n512 = (int)r512.size();
-    b->ncol = (int)cols.size();
+    for (int i = 0; i < L_COUNT; ++i) {
+        if (L[i].lds.bytes() > kLds) fail(QDX_ERR_PARAM, "job tile exceeds the LDS");
+        b->count[i] = (int)(is_line_list(i) ? L[i].jobs.size() : L[i].tiles.size());
+        b->lds[i] = L[i].lds.bytes();
+        b->twl[i] = L[i].lds.tw();
+        b->pre[i] = L[i].pre;
+        b->n2d[i] = L[i].n2d;
+    }
+    b->njobs_tab = (int)tj.size();
 
-    // write the tables into the set the enqueued ticks do not read
+    // stage into pinned memory, one async copy into the set not in use
     const int nxt = 1 - b->cur;
     TableSet &ts = b->sets[nxt];
     if (ts.used) HIPCHK(hipEventSynchronize(ts.last_use));
-    hipStream_t s = b->ctx->stream;
-    auto put = [&](DevBuf &dst, const void *src, size_t bytes) {
-        if (bytes == 0) return;
-        if (dst.bytes < bytes) dst.alloc(bytes * 2);
-        HIPCHK(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyHostToDevice, s));
+    size_t o = 0;
+    auto slot = [&](size_t bytes) {
+        const size_t at = o;
+        o += Arena::round(std::max<size_t>(bytes, 1));
+        return at;
     };
-    put(ts.jobs, tj.data(), tj.size() * sizeof(BJob<R>));
-    put(ts.r256, r256.data(), r256.size() * sizeof(BTile));
-    put(ts.r512, r512.data(), r512.size() * sizeof(BTile));
-    put(ts.col, cols.data(), cols.size() * sizeof(BTile));
-    put(ts.plans, plans.data(), plans.size() * sizeof(LinePlan));
-    // the copies read pageable host memory: wait before the vectors go away
-    HIPCHK(hipStreamSynchronize(s));
+    ts.off_jobs = slot(tj.size() * sizeof(BJob<R>));
+    ts.off_plans = slot(plans.size() * sizeof(LinePlan));
+    for (int i = 0; i < L_COUNT; ++i)
+        ts.off_list[i] = slot(is_line_list(i) ? L[i].jobs.size() * sizeof(int) : L[i].tiles.size() * sizeof(BTile));
+    if (o > ts.cap) {
+        if (ts.host) HIPCHK(hipHostFree(ts.host));
+        ts.host = nullptr;
+        ts.cap = 0;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ts.host), 2 * o, hipHostMallocDefault));
+        ts.cap = 2 * o;
+        ts.dev.alloc(2 * o);
+    }
+    auto put = [&](size_t off, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(ts.host + off, src, bytes);
+    };
+    put(ts.off_jobs, tj.data(), tj.size() * sizeof(BJob<R>));
+    put(ts.off_plans, plans.data(), plans.size() * sizeof(LinePlan));
+    for (int i = 0; i < L_COUNT; ++i) {
+        if (is_line_list(i)) put(ts.off_list[i], L[i].jobs.data(), L[i].jobs.size() * sizeof(int));
+        else put(ts.off_list[i], L[i].tiles.data(), L[i].tiles.size() * sizeof(BTile));
+    }
+    HIPCHK(hipMemcpyAsync(ts.dev.p, ts.host, o, hipMemcpyHostToDevice, b->ctx->stream));
     b->cur = nxt;
     b->dirty = false;
+}
+
+// tiles of the Chebyshev jobs whose series reaches order k
+int tiles_for(const std::vector<std::pair<int, int>> &pre, int k) {
+    int n = 0;
+    for (const auto &e : pre) {
+        if (e.first <= k) break;
+        n = e.second;
+    }
+    return n;
 }
 
 template <class R>
 void enqueue_tick(qdx_batch *b) {
     if (b->dirty) build_tables<R>(b);
     const int64_t T = b->ticks;
-    bool any256 = false, any512 = false, any2d = false;
+    bool cheb2d = false, cheb1d = false, gspo = false;
     int kmax = 0;
     for (auto &kv : b->jobs) {
         const BatchJob &J = kv.second;
         if (!J.active_at(T)) continue;
-        (J.tile.NT == 256 ? any256 : any512) = true;
-        if (J.grid.nd == 2) any2d = true;
-        kmax = std::max(kmax, J.cp.order);
+        if (J.kind == K_GSPO) {
+            gspo = true;
+        } else if (J.grid.nd == 1) {
+            cheb1d = true;
+        } else {
+            cheb2d = true;
+            kmax = std::max(kmax, J.cp.order);
+        }
     }
     TableSet &ts = b->sets[b->cur];
     hipStream_t s = b->ctx->stream;
-    const BJob<R> *jobs = ts.jobs.as<BJob<R>>();
-    const LinePlan *plans = ts.plans.as<LinePlan>();
+    char *tb = ts.dev.as<char>();
+    const BJob<R> *jobs = reinterpret_cast<const BJob<R> *>(tb + ts.off_jobs);
+    const LinePlan *plans = reinterpret_cast<const LinePlan *>(tb + ts.off_plans);
+    auto tiles = [&](int i) { return reinterpret_cast<const BTile *>(tb + ts.off_list[i]); };
+    auto list = [&](int i) { return reinterpret_cast<const int *>(tb + ts.off_list[i]); };
     const int *tick = b->dtick.as<int>();
     qdx_ctx *ctx = b->ctx;
-    auto rows = [&](int phase, int k) {
-        if (any256) {
-            KTimer t(ctx, QDX_K_BATCH_ROW);
-            BatchLaunch<R>::row(s, 256, phase, k, jobs, ts.r256.as<BTile>(), b->n256, plans, tick, b->lds256,
-                                b->tw256);
+    if (cheb1d) {
+        // the 1D jobs' steps overlap the other jobs' passes
+        HIPCHK(hipEventRecord(b->fork, s));
+        HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
+        for (int i : {L_LINE256, L_LINE512}) {
+            if (b->count[i] == 0) continue;
+            KTimer t(ctx, QDX_K_BATCH_LINE);
+            BatchLaunch<R>::line(b->side, i == L_LINE256 ? 256 : 512, jobs, list(i), b->count[i], plans, tick,
+                                 b->lds[i], b->twl[i]);
         }
-        if (any512) {
+        HIPCHK(hipEventRecord(b->join, b->side));
+    }
+    if (gspo) {
+        for (int i : {L_GROW256, L_GROW512}) {
+            if (b->count[i] == 0) continue;
             KTimer t(ctx, QDX_K_BATCH_ROW);
-            BatchLaunch<R>::row(s, 512, phase, k, jobs, ts.r512.as<BTile>(), b->n512, plans, tick, b->lds512,
-                                b->tw512);
+            BatchLaunch<R>::gspo_row(s, i == L_GROW256 ? 256 : 512, 0, jobs, tiles(i), b->count[i], plans, tick,
+                                     b->lds[i], b->twl[i]);
+        }
+        if (b->count[L_GCOL] > 0) {
+            KTimer t(ctx, QDX_K_BATCH_COL);
+            BatchLaunch<R>::gspo_col(s, jobs, tiles(L_GCOL), b->count[L_GCOL], plans, tick, b->lds[L_GCOL],
+                                     b->twl[L_GCOL]);
+        }
+        for (int i : {L_GROW256, L_GROW512}) {
+            if (b->n2d[i] == 0) continue;
+            KTimer t(ctx, QDX_K_BATCH_ROW);
+            BatchLaunch<R>::gspo_row(s, i == L_GROW256 ? 256 : 512, 1, jobs, tiles(i), b->n2d[i], plans, tick,
+                                     b->lds[i], b->twl[i]);
+        }
+    }
+    auto rows = [&](int phase, int k) {
+        for (int i : {L_ROW256, L_ROW512}) {
+            const int n = tiles_for(b->pre[i], k);
+            if (n == 0) continue;
+            KTimer t(ctx, QDX_K_BATCH_ROW);
+            BatchLaunch<R>::row(s, i == L_ROW256 ? 256 : 512, phase, k, jobs, tiles(i), n, plans, tick, b->lds[i],
+                                b->twl[i]);
         }
     };
     auto cols = [&](int k) {
-        if (!any2d) return;
+        const int n = tiles_for(b->pre[L_COL], k);
+        if (n == 0) return;
         KTimer t(ctx, QDX_K_BATCH_COL);
-        BatchLaunch<R>::col(s, k, jobs, ts.col.as<BTile>(), b->ncol, plans, tick, b->ldsc, b->twc);
+        BatchLaunch<R>::col(s, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL]);
     };
-    if (kmax > 0) {
+    if (cheb2d) {
         rows(0, 0);
         cols(1);
         rows(1, 1);
@@ -361,11 +552,17 @@ void enqueue_tick(qdx_batch *b) {
             rows(2, k);
         }
     }
-    BatchLaunch<R>::tick_end(s, jobs, (int)b->order_ids.size(), b->dtick.as<int>());
+    if (cheb1d) HIPCHK(hipStreamWaitEvent(s, b->join, 0));
+    BatchLaunch<R>::tick_end(s, jobs, b->njobs_tab, b->dtick.as<int>());
     if (!ts.last_use) HIPCHK(hipEventCreateWithFlags(&ts.last_use, hipEventDisableTiming));
     HIPCHK(hipEventRecord(ts.last_use, s));
     ts.used = true;
     b->ticks++;
+}
+
+void tick(qdx_batch *b) {
+    if (b->precision == QDX_FP32_BULK) enqueue_tick<float>(b);
+    else enqueue_tick<double>(b);
 }
 
 BatchJob &find_job(qdx_batch *b, int64_t id) {
@@ -374,38 +571,56 @@ BatchJob &find_job(qdx_batch *b, int64_t id) {
     return it->second;
 }
 
-void check_desc(const qdx_job_desc *d) {
-    if (!d || !d->ham || !d->psi0) fail(QDX_ERR_PARAM, "NULL argument");
-    qdx_ham *h = d->ham;
-    if (h->S != 1 || h->multistate) fail(QDX_ERR_INCOMPAT, "batched jobs hold one state");
-    if (h->grid.nd > 2) fail(QDX_ERR_INCOMPAT, "batched jobs are 1D or 2D");
-    if (h->grid.nd == 2 && h->grid.N[1] > 2048) fail(QDX_ERR_PARAM, "2D jobs need N1 <= 2048");
-    if (d->nsteps < 1) fail(QDX_ERR_PARAM, "nsteps must be >= 1");
-    if (d->dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
+void check_grid(const Grid &g) {
+    if (g.nd > 2) fail(QDX_ERR_INCOMPAT, "batched jobs are 1D or 2D");
 }
 
-// job parameters without touching the device
+// job parameters and host arrays, without touching the device
 BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
-    check_desc(d);
-    qdx_ham *h = d->ham;
+    if (!d || !d->psi0) fail(QDX_ERR_PARAM, "NULL argument");
+    if (d->nsteps < 1) fail(QDX_ERR_PARAM, "nsteps must be >= 1");
+    if (d->dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
     BatchJob J;
-    J.grid = h->grid;
     J.nsteps = d->nsteps;
-    host = job_host(h, d->nsteps);
-    J.cp = cheby_params(h, d->dt, d->order, d->prec, 0.0);
-    if (b->precision == QDX_FP32_BULK) fp32_series(J.cp, d->dt);
+    J.dt = d->dt;
+    if (d->nops == 0) {
+        qdx_ham *h = d->ham;
+        if (!h) fail(QDX_ERR_PARAM, "NULL argument");
+        if (h->S != 1 || h->multistate) fail(QDX_ERR_INCOMPAT, "batched Chebyshev jobs hold one state");
+        J.kind = K_CHEBY;
+        J.S = 1;
+        J.grid = h->grid;
+        check_grid(J.grid);
+        host = cheby_host(h, d->nsteps);
+        J.cp = cheby_params(h, d->dt, d->order, d->prec, 0.0);
+        if (b->precision == QDX_FP32_BULK) fp32_series(J.cp, d->dt);
+    } else {
+        if (d->nops != 2 || !d->ops || !d->ops[0] || !d->ops[1])
+            fail(QDX_ERR_INCOMPAT, "batched GSPO jobs take the split operators {T, W}");
+        qdx_ham *T = d->ops[0], *W = d->ops[1];
+        if (!(T->grid == W->grid) || T->S != W->S) fail(QDX_ERR_INCOMPAT, "GSPO operators on different grids");
+        if (T->S > 2) fail(QDX_ERR_INCOMPAT, "batched GSPO jobs hold one or two states");
+        J.kind = K_GSPO;
+        J.S = T->S;
+        J.grid = T->grid;
+        check_grid(J.grid);
+        host = gspo_host(T, W, d->nsteps);
+    }
     J.tile = job_tiling(J.grid);
     J.has_V = !host.V.empty();
     J.has_mu = !host.mu.empty();
+    J.has_V11 = !host.V11.empty();
+    J.has_V01 = !host.V01.empty();
     J.has_T = host.has_T;
+    J.has_T1 = host.has_T1;
     J.lscale = host.lscale;
     J.nt = J.has_mu ? (int)host.laser.size() : 0;
-    J.lay = job_layout(J.grid, J.tile, J.cp.order, J.nt, b->rsz(), J.has_mu);
+    J.lay = job_layout(J, b->rsz());
     return J;
 }
 
 template <class R>
-void upload_job(qdx_batch *b, BatchJob &J, const JobHost &host, const double *psi0, double dt) {
+void upload_job(qdx_batch *b, BatchJob &J, const JobHost &host, const double *psi0) {
     char *jb = b->arena.base() + J.off;
     const long n = J.grid.size();
     auto put = [&](size_t off, const void *src, size_t bytes) {
@@ -419,8 +634,22 @@ void upload_job(qdx_batch *b, BatchJob &J, const JobHost &host, const double *ps
             put(off, v.data(), v.size() * sizeof(double));
         }
     };
-    put(J.lay.psi, psi0, 16 * (size_t)n);
+    if (J.kind == K_CHEBY) {
+        put(J.lay.psi, psi0, 16 * (size_t)n);
+        std::vector<double> c(2 * J.cp.order);
+        const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * J.dt * J.cp.offset);
+        for (int k = 0; k < J.cp.order; ++k) {
+            const std::complex<double> v = J.cp.coeff[k] * ph;
+            c[2 * k] = v.real();
+            c[2 * k + 1] = v.imag();
+        }
+        put(J.lay.coef, c.data(), c.size() * sizeof(double));
+    } else {
+        putR(J.lay.psi, std::vector<double>(psi0, psi0 + 2 * (size_t)n * J.S));
+    }
     if (J.has_V) putR(J.lay.V, host.V);
+    if (J.has_V11) putR(J.lay.V11, host.V11);
+    if (J.has_V01) putR(J.lay.V01, host.V01);
     if (J.has_mu) {
         putR(J.lay.mu, host.mu);
         put(J.lay.laser, host.laser.data(), host.laser.size() * sizeof(double));
@@ -429,16 +658,65 @@ void upload_job(qdx_batch *b, BatchJob &J, const JobHost &host, const double *ps
         put(J.lay.tk0, host.tk0.data(), host.tk0.size() * sizeof(double));
         if (J.grid.nd == 2) put(J.lay.tk1, host.tk1.data(), host.tk1.size() * sizeof(double));
     }
-    std::vector<double> c(2 * J.cp.order);
-    const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * dt * J.cp.offset);
-    for (int k = 0; k < J.cp.order; ++k) {
-        const std::complex<double> v = J.cp.coeff[k] * ph;
-        c[2 * k] = v.real();
-        c[2 * k + 1] = v.imag();
+    if (J.has_T1) {
+        put(J.lay.tk0s1, host.tk0s1.data(), host.tk0s1.size() * sizeof(double));
+        if (J.grid.nd == 2) put(J.lay.tk1s1, host.tk1s1.data(), host.tk1s1.size() * sizeof(double));
     }
-    put(J.lay.coef, c.data(), c.size() * sizeof(double));
-    HIPCHK(hipMemset(jb + J.lay.ctl, 0, sizeof(ChebyCtl)));
-    HIPCHK(hipMemset(jb + J.lay.stats, 0, 2 * sizeof(long long)));
+    const ChebyCtl c0{};
+    put(J.lay.ctl, &c0, sizeof(c0));
+    const long long z[2] = {0, 0};
+    put(J.lay.stats, z, sizeof(z));
+}
+
+// admit a job; false when it does not fit into the free part of the arena
+bool try_submit(qdx_batch *b, const qdx_job_desc *d, int64_t &id) {
+    JobHost host;
+    BatchJob J = plan_job(b, d, host);
+    if (!b->arena.alloc(J.lay.bytes, J.off)) return false;
+    try {
+        if (b->precision == QDX_FP32_BULK) upload_job<float>(b, J, host, d->psi0);
+        else upload_job<double>(b, J, host, d->psi0);
+    } catch (...) {
+        b->arena.release(J.off, J.lay.bytes);
+        throw;
+    }
+    J.id = b->next_id++;
+    J.start_tick = b->ticks;
+    id = J.id;
+    b->jobs[id] = std::move(J);
+    b->dirty = true;
+    return true;
+}
+
+// psi of a job whose ticks have completed (complex128)
+void copy_psi(qdx_batch *b, const BatchJob &J, double *out) {
+    const char *src = b->arena.base() + J.off + J.lay.psi;
+    const size_t n = (size_t)J.grid.size() * J.S;
+    if (J.kind == K_CHEBY || b->precision == QDX_FP64) {
+        HIPCHK(hipMemcpy(out, src, 16 * n, hipMemcpyDeviceToHost));
+        return;
+    }
+    std::vector<float> f(2 * n);
+    HIPCHK(hipMemcpy(f.data(), src, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 2 * n; ++i) out[i] = f[i];
+}
+
+void read_stats(qdx_batch *b, const BatchJob &J, long long st[2]) {
+    HIPCHK(hipMemcpy(st, b->arena.base() + J.off + J.lay.stats, 2 * sizeof(long long), hipMemcpyDeviceToHost));
+}
+
+// return a job's memory; the caller has made sure no enqueued tick still uses it
+void drop_job(qdx_batch *b, int64_t id) {
+    BatchJob &J = find_job(b, id);
+    b->arena.release(J.off, J.lay.bytes);
+    b->jobs.erase(id);
+    b->dirty = true;
+}
+
+bool any_steps_left(const qdx_batch *b) {
+    for (auto &kv : b->jobs)
+        if (kv.second.active_at(b->ticks)) return true;
+    return false;
 }
 
 }  // namespace
@@ -457,6 +735,9 @@ int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_ba
         b->arena.init((size_t)hbm_budget);
         b->dtick.alloc(sizeof(int));
         HIPCHK(hipMemset(b->dtick.p, 0, sizeof(int)));
+        HIPCHK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
         HIPCHK(hipDeviceSynchronize());
         ctx_acquire(ctx);
         *out = b.release();
@@ -468,29 +749,16 @@ int qdx_batch_footprint(qdx_batch *b, const qdx_job_desc *job, int64_t *bytes) {
         if (!b || !bytes) fail(QDX_ERR_PARAM, "NULL argument");
         JobHost host;
         const BatchJob J = plan_job(b, job, host);
-        *bytes = (int64_t)J.lay.bytes;
+        *bytes = (int64_t)Arena::round(J.lay.bytes);
     });
 }
 
 int qdx_batch_submit(qdx_batch *b, const qdx_job_desc *job, int64_t *id) {
     return guarded([&] {
         if (!b || !id) fail(QDX_ERR_PARAM, "NULL argument");
-        JobHost host;
-        BatchJob J = plan_job(b, job, host);
-        if (!b->arena.alloc(J.lay.bytes, J.off))
+        HIPCHK(hipSetDevice(b->ctx->device));
+        if (!try_submit(b, job, *id))
             fail(QDX_ERR_MEMORY, "job does not fit into the free part of the batch's HBM budget");
-        try {
-            if (b->precision == QDX_FP32_BULK) upload_job<float>(b, J, host, job->psi0, job->dt);
-            else upload_job<double>(b, J, host, job->psi0, job->dt);
-        } catch (...) {
-            b->arena.release(J.off, J.lay.bytes);
-            throw;
-        }
-        J.id = b->next_id++;
-        J.start_tick = b->ticks;
-        b->jobs[J.id] = J;
-        b->dirty = true;
-        *id = J.id;
     });
 }
 
@@ -499,10 +767,7 @@ int qdx_batch_run(qdx_batch *b, int32_t nticks) {
         if (!b) fail(QDX_ERR_PARAM, "NULL argument");
         if (nticks < 0) fail(QDX_ERR_PARAM, "nticks must be >= 0");
         HIPCHK(hipSetDevice(b->ctx->device));
-        for (int i = 0; i < nticks; ++i) {
-            if (b->precision == QDX_FP32_BULK) enqueue_tick<float>(b);
-            else enqueue_tick<double>(b);
-        }
+        for (int i = 0; i < nticks; ++i) tick(b);
     });
 }
 
@@ -512,10 +777,9 @@ int qdx_batch_job_state(qdx_batch *b, int64_t id, int64_t *steps_left, int64_t *
         BatchJob &J = find_job(b, id);
         if (steps_left) *steps_left = std::max<int64_t>(0, J.start_tick + J.nsteps - b->ticks);
         if (n_hpsi || steps_done) {
-            long long st[2];
-            HIPCHK(hipMemcpyAsync(st, b->arena.base() + J.off + J.lay.stats, sizeof(st), hipMemcpyDeviceToHost,
-                                  b->ctx->stream));
             HIPCHK(hipStreamSynchronize(b->ctx->stream));
+            long long st[2];
+            read_stats(b, J, st);
             if (n_hpsi) *n_hpsi = st[0];
             if (steps_done) *steps_done = st[1];
         }
@@ -527,19 +791,16 @@ int qdx_batch_fetch(qdx_batch *b, int64_t id, double *psi) {
         if (!b || !psi) fail(QDX_ERR_PARAM, "NULL argument");
         BatchJob &J = find_job(b, id);
         HIPCHK(hipStreamSynchronize(b->ctx->stream));
-        HIPCHK(hipMemcpy(psi, b->arena.base() + J.off + J.lay.psi, 16 * (size_t)J.grid.size(),
-                         hipMemcpyDeviceToHost));
+        copy_psi(b, J, psi);
     });
 }
 
 int qdx_batch_release(qdx_batch *b, int64_t id) {
     return guarded([&] {
         if (!b) fail(QDX_ERR_PARAM, "NULL argument");
-        BatchJob &J = find_job(b, id);
+        find_job(b, id);
         HIPCHK(hipStreamSynchronize(b->ctx->stream));
-        b->arena.release(J.off, J.lay.bytes);
-        b->jobs.erase(id);
-        b->dirty = true;
+        drop_job(b, id);
     });
 }
 
@@ -552,13 +813,116 @@ int qdx_batch_usage(qdx_batch *b, int32_t *njobs, int64_t *bytes_used, int64_t *
     });
 }
 
+int qdx_batch_serve(qdx_batch *b, const qdx_job_desc *jobs, const double *arrival_s, int32_t n,
+                    int32_t max_active, double *const *psi_out, qdx_serve_result *out) {
+    std::vector<hipEvent_t> pool;
+    hipEvent_t ev0 = nullptr;
+    int rc = guarded([&] {
+        if (!b || (n > 0 && (!jobs || !arrival_s || !out))) fail(QDX_ERR_PARAM, "NULL argument");
+        for (int i = 1; i < n; ++i)
+            if (arrival_s[i] < arrival_s[i - 1]) fail(QDX_ERR_PARAM, "arrival times must not decrease");
+        HIPCHK(hipSetDevice(b->ctx->device));
+        hipStream_t s = b->ctx->stream;
+        using clk = std::chrono::steady_clock;
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipEventCreate(&ev0));
+        const auto t0 = clk::now();
+        HIPCHK(hipEventRecord(ev0, s));
+        auto now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+        struct Flight {
+            int64_t tick;
+            hipEvent_t ev;
+        };
+        std::deque<Flight> flights;
+        std::deque<int> pending;
+        std::map<int64_t, int> active;              // job id -> request
+        std::multimap<int64_t, int64_t> ending;     // last tick -> job id
+        constexpr size_t kAhead = 4;                // ticks in flight before the host waits
+        int next = 0, done = 0;
+        while (done < n) {
+            const double t = now();
+            while (next < n && arrival_s[next] <= t) pending.push_back(next++);
+            while (!pending.empty() && (max_active <= 0 || (int)active.size() < max_active)) {
+                const int i = pending.front();
+                int64_t id;
+                if (!try_submit(b, &jobs[i], id)) break;
+                out[i].admit_s = now();
+                active[id] = i;
+                ending.insert({b->jobs[id].last_tick(), id});
+                pending.pop_front();
+            }
+            const bool work = any_steps_left(b);
+            if (!work && flights.empty()) {
+                if (!pending.empty() && active.empty())
+                    fail(QDX_ERR_MEMORY, "a request does not fit into the whole HBM budget");
+                if (next < n) {
+                    const double wait = arrival_s[next] - now();
+                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
+                }
+                continue;
+            }
+            if (work) {
+                tick(b);
+                hipEvent_t ev;
+                if (pool.empty()) {
+                    HIPCHK(hipEventCreate(&ev));
+                } else {
+                    ev = pool.back();
+                    pool.pop_back();
+                }
+                HIPCHK(hipEventRecord(ev, s));
+                flights.push_back({b->ticks - 1, ev});
+            }
+            // retire the jobs whose last tick has completed; wait only when
+            // too many ticks are in flight or nothing else can be enqueued
+            while (!flights.empty()) {
+                Flight &f = flights.front();
+                const bool block = flights.size() > kAhead || !work;
+                const hipError_t q = block ? hipEventSynchronize(f.ev) : hipEventQuery(f.ev);
+                if (q == hipErrorNotReady) break;
+                HIPCHK(q);
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev0, f.ev));
+                auto range = ending.equal_range(f.tick);
+                for (auto it = range.first; it != range.second; ++it) {
+                    const int64_t id = it->second;
+                    const int i = active[id];
+                    const BatchJob &J = b->jobs[id];
+                    long long st[2];
+                    read_stats(b, J, st);
+                    out[i].finish_s = ms * 1e-3;
+                    out[i].n_hpsi = st[0];
+                    if (psi_out && psi_out[i]) copy_psi(b, J, psi_out[i]);
+                    drop_job(b, id);
+                    active.erase(id);
+                    ++done;
+                }
+                ending.erase(range.first, range.second);
+                pool.push_back(f.ev);
+                flights.pop_front();
+                if (block && flights.size() <= kAhead && work) break;
+            }
+        }
+        HIPCHK(hipStreamSynchronize(s));
+    });
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    return rc;
+}
+
 int qdx_batch_destroy(qdx_batch *b) {
     return guarded([&] {
         if (!b) return;
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
-        for (auto &ts : b->sets)
+        (void)hipStreamSynchronize(b->side);
+        for (auto &ts : b->sets) {
             if (ts.last_use) (void)hipEventDestroy(ts.last_use);
+            if (ts.host) (void)hipHostFree(ts.host);
+        }
+        if (b->fork) (void)hipEventDestroy(b->fork);
+        if (b->join) (void)hipEventDestroy(b->join);
+        if (b->side) (void)hipStreamDestroy(b->side);
         qdx_ctx *ctx = b->ctx;
         delete b;
         ctx_release(ctx);

@@ -295,12 +295,316 @@ __global__ void __launch_bounds__(256) k_btick_end(const BJob<R> *__restrict__ j
         const BJob<R> &J = jobs[j];
         const int step = tk - J.start_tick;
         if (step >= 0 && step < J.nsteps) {
-            J.stats[0] += J.ctl->last_order;
+            J.stats[0] += J.kind == 1 ? 1 : J.ctl->last_order;
             J.stats[1] += 1;
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) *tick = tk + 1;
+}
+
+// One whole time step of a 1D job in one block: the line (L <= 4096 points,
+// 8 per thread) stays in registers through all orders of the series; only the
+// FFT stages exchange through LDS.  Same arithmetic, order by order, as the
+// per-order kernels (init, prologue, recurrence with the break test after
+// each order), so the results agree with them bit for bit.
+template <class R, int NT>
+__global__ void __launch_bounds__(NT) k_bline(const BJob<R> *__restrict__ jobs, const int *__restrict__ list,
+                                              const LinePlan *__restrict__ plans, const int *tick_p, int twl) {
+    const BJob<R> J = jobs[list[blockIdx.x]];
+    const int step = *tick_p - J.start_tick;
+    if (step < 0 || step >= J.nsteps) return;
+    extern __shared__ __align__(16) char smem_raw[];
+    __shared__ double red[16];
+    __shared__ double bc;
+    cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
+    const LinePlan &p = plans[J.plan0];
+    const int L = J.N0, TL = L >> 3, lt = threadIdx.x;
+    const bool active = lt < TL;
+    LdsLine<R> ln;
+    ln.a = smem;
+    ln.b = smem + (L + (L >> 3));
+    ln.es = 1;
+    ln.pad = true;
+    cx<R> *twt = smem + 2 * (L + (L >> 3));
+    if (twl) stage_twiddles(twt, p);
+    const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
+    cx<double> *psi = reinterpret_cast<cx<double> *>(J.psi);
+
+    cx<double> ps[8];
+    cx<R> pm1[8], pm2[8], v[8];
+    double wv[8], tq[8];
+    const bool has_t = J.tk0 != nullptr;
+    const double Ed = J.nt > 0 ? -J.laser[step] * J.lscale : 0.0;
+    double acc = 0;
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int x = lt + q * TL;
+            ps[q] = psi[x];
+            acc += ps[q].r * ps[q].r + ps[q].i * ps[q].i;
+            pm1[q] = from_d<R>(ps[q]);
+            double w = 0.0;
+            if (J.V) w += (double)J.V[x];
+            if (J.mu) w += Ed * (double)J.mu[x];
+            wv[q] = w;
+            tq[q] = has_t ? (0.0 + J.tk0[x]) * J.inv_n : 0.0;
+        }
+    }
+    // sum |psi|^2 at the step start, as the init kernel's single partial
+    double norm0 = block_sum(acc, red);
+    if (threadIdx.x == 0) bc = norm0;
+    __syncthreads();
+    norm0 = bc;
+    const cx<double> c0 = mk<double>(J.coef[0], J.coef[1]);
+    int last = 1;
+    for (int k = 1; k < J.order; ++k) {
+        // T phi_{k-1} (OGridNablaSq through Kspace: FFT, *T/N, IFFT)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = pm1[q];
+        fft8<R>(v, ln, p, tws, -1, lt, active);
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+        }
+        fft8<R>(v, ln, p, tws, +1, lt, active);
+        const cx<double> ck = mk<double>(J.coef[2 * k], J.coef[2 * k + 1]);
+        acc = 0;
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                cx<double> hp = has_t ? to_d(v[q]) : mk<double>(0, 0);
+                const cx<double> ph = to_d(pm1[q]);
+                hp.r += wv[q] * ph.r;
+                hp.i += wv[q] * ph.i;
+                const cx<double> bb = mk<double>((hp.r - J.offset * ph.r) * J.inv_scaling,
+                                                 (hp.i - J.offset * ph.i) * J.inv_scaling);
+                cx<double> pk;
+                if (k == 1) {
+                    pk = mk<double>(bb.i, -bb.r);
+                    ps[q] = cmul(c0, ps[q]) + cmul(ck, pk);
+                } else {
+                    const cx<double> mm = to_d(pm2[q]);
+                    pk = mk<double>(2.0 * bb.i + mm.r, -2.0 * bb.r + mm.i);
+                    ps[q] = ps[q] + cmul(ck, pk);
+                    acc += ps[q].r * ps[q].r + ps[q].i * ps[q].i;
+                }
+                pm2[q] = pm1[q];
+                pm1[q] = from_d<R>(pk);
+            }
+        }
+        last = k;
+        // break test after order k (OCheby.cpp:179-182); off when prec == 0
+        if (k >= 2 && J.prec > 0) {
+            double a = block_sum(acc, red);
+            if (threadIdx.x == 0) bc = a;
+            __syncthreads();
+            a = bc;
+            if (fabs(norm0 - a) / norm0 < J.prec) break;
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) psi[lt + q * TL] = ps[q];
+    }
+    if (threadIdx.x == 0) {
+        J.ctl->norm0 = norm0;
+        J.ctl->last_order = last;
+        J.ctl->done = 1;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// GSPO jobs
+
+// psi <- exp(-i tau W) psi on the S <= 2 states of one grid point, W real
+// symmetric: per-state phases, or for coupled states
+// e^{-i tau m} [cos(tau O) - i sin(tau O)/O (W - m)], m = (W00 + W11)/2,
+// O = sqrt(((W00 - W11)/2)^2 + W01^2) (ODMultistate.cpp:109-159 diagonalises W
+// per point through TransformMSD.cpp:31-101; same operator)
+__device__ __forceinline__ void wsplit(int S, bool coupled, double tau, double w00, double w11, double w01,
+                                       cx<double> &a, cx<double> &b) {
+    if (S == 1 || !coupled) {
+        double sn, cs;
+        sincos(-tau * w00, &sn, &cs);
+        a = cmul(a, mk<double>(cs, sn));
+        if (S == 2) {
+            sincos(-tau * w11, &sn, &cs);
+            b = cmul(b, mk<double>(cs, sn));
+        }
+        return;
+    }
+    const double m = 0.5 * (w00 + w11);
+    const double d = 0.5 * (w00 - w11);
+    const double O = sqrt(d * d + w01 * w01);
+    const double c = cos(tau * O);
+    const double sc = O > 0 ? sin(tau * O) / O : tau;
+    double sm, cm;
+    sincos(-tau * m, &sm, &cm);
+    const cx<double> ph = mk<double>(cm, sm);
+    const cx<double> u00 = mk<double>(c, -sc * d), u11 = mk<double>(c, sc * d);
+    const cx<double> u01 = mk<double>(0, -sc * w01);
+    const cx<double> a0 = cmul(u00, a) + cmul(u01, b);
+    const cx<double> a1 = cmul(u01, a) + cmul(u11, b);
+    a = cmul(ph, a0);
+    b = cmul(ph, a1);
+}
+
+template <class R>
+__device__ __forceinline__ void wsplit_point(const BJob<R> &J, long gi, double Ed, double tau, cx<R> &v0, cx<R> &v1) {
+    double w00 = 0.0, w11 = 0.0, w01 = 0.0;
+    if (J.V) w00 += (double)J.V[gi];
+    if (J.mu) w00 += Ed * (double)J.mu[gi];
+    if (J.S == 2) {
+        if (J.V11) w11 += (double)J.V11[gi];
+        if (J.V01) w01 += (double)J.V01[gi];
+    }
+    cx<double> a = to_d(v0), b = J.S == 2 ? to_d(v1) : mk<double>(0, 0);
+    wsplit(J.S, J.V01 != nullptr, tau, w00, w11, w01, a, b);
+    v0 = from_d<R>(a);
+    if (J.S == 2) v1 = from_d<R>(b);
+}
+
+// exp(-i tau T(k))/N with T summed over dimensions in dimension order; the
+// 1/N of the inverse transform folded in (OGSPO.cpp:37-41)
+__device__ __forceinline__ cx<double> kin_phase(double tt, double tau, double inv_n) {
+    double sn, cs;
+    sincos(-tau * tt, &sn, &cs);
+    return mk<double>(cs * inv_n, sn * inv_n);
+}
+
+// PHASE 0: psi -> exp(-i tau/2 W) -> FFT_0 -> X (2D), or the whole step of a
+// 1D job; PHASE 1: X -> IFFT_0 -> exp(-i tau/2 W) -> psi (2D jobs).  A thread
+// carries both states of its points, transforming them one after the other.
+template <class R, int NT, int PHASE>
+__global__ void __launch_bounds__(NT) k_bgspo_row(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
+                                                  const LinePlan *__restrict__ plans, const int *tick_p, int twl) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BTile t = tiles[b];
+    const BJob<R> J = jobs[t.job];
+    const int step = *tick_p - J.start_tick;
+    if (step < 0 || step >= J.nsteps) return;
+    if (PHASE == 1 && J.nd == 1) return;
+    extern __shared__ __align__(16) char smem_raw[];
+    BRowTile<R> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
+    const LinePlan &p0 = plans[J.plan0];
+    if (twl) stage_twiddles(B.tw, p0);
+    const bool two = J.S == 2;
+    cx<R> *psi = reinterpret_cast<cx<R> *>(J.psiR);
+    cx<R> *X = reinterpret_cast<cx<R> *>(J.X);
+    const double Ed = J.nt > 0 ? -J.laser[step] * J.lscale : 0.0;
+    const double th = 0.5 * J.tau;
+    cx<R> v0[8], v1[8];
+    if (PHASE == 0) {
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                v0[q] = psi[B.gidx(q)];
+                if (two) v1[q] = psi[J.size + B.gidx(q)];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) wsplit_point(J, B.gidx(q), Ed, th, v0[q], v1[q]);
+        }
+        bfft(v0, B, p0, twl, -1);
+        if (two) bfft(v1, B, p0, twl, -1);
+        if (J.nd == 2) {
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    X[B.gidx(q)] = v0[q];
+                    if (two) X[J.size + B.gidx(q)] = v1[q];
+                }
+            }
+            return;
+        }
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int x = B.lt + q * B.TL;
+                const double t0 = J.tk0 ? 0.0 + J.tk0[x] : 0.0;
+                v0[q] = from_d<R>(cmul(to_d(v0[q]), kin_phase(t0, J.tau, J.inv_n)));
+                if (two) {
+                    const double t1 = J.tk0s1 ? 0.0 + J.tk0s1[x] : 0.0;
+                    v1[q] = from_d<R>(cmul(to_d(v1[q]), kin_phase(t1, J.tau, J.inv_n)));
+                }
+            }
+        }
+    } else {
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                v0[q] = X[B.gidx(q)];
+                if (two) v1[q] = X[J.size + B.gidx(q)];
+            }
+        }
+    }
+    bfft(v0, B, p0, twl, +1);
+    if (two) bfft(v1, B, p0, twl, +1);
+    if (B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            wsplit_point(J, B.gidx(q), Ed, th, v0[q], v1[q]);
+            psi[B.gidx(q)] = v0[q];
+            if (two) psi[J.size + B.gidx(q)] = v1[q];
+        }
+    }
+}
+
+// GSPO kinetic split operator along dimension 1 of a 2D job's X (one state
+// per tile): forward * exp(-i tau T)/N * inverse
+template <class R>
+__global__ void __launch_bounds__(256) k_bgspo_col(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
+                                                   const LinePlan *__restrict__ plans, const int *tick_p, int twl) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const BTile t = tiles[b];
+    const BJob<R> J = jobs[t.job];
+    const int step = *tick_p - J.start_tick;
+    if (step < 0 || step >= J.nsteps) return;
+    extern __shared__ __align__(16) char smem_raw[];
+    cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
+    const LinePlan &p = plans[J.plan1];
+    const int N0 = J.N0, L = J.N1, TL = L >> 3, W = J.W;
+    const int st = t.first / N0;
+    const int c = threadIdx.x % W;
+    const int lt = threadIdx.x / W;
+    const int col = t.first - st * N0 + c;
+    const bool active = lt < TL && col < N0;
+    cx<R> *base = reinterpret_cast<cx<R> *>(J.X) + (long)st * J.size + col;
+    LdsLine<R> ln;
+    ln.a = smem + c;
+    ln.b = smem + (long)W * L + c;
+    ln.es = W;
+    ln.pad = false;
+    cx<R> *twt = smem + (long)2 * W * L;
+    const double *tk0 = st == 0 ? J.tk0 : J.tk0s1;
+    const double *tk1 = st == 0 ? J.tk1 : J.tk1s1;
+    cx<R> v[8];
+    double tq[8];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            v[q] = base[(long)(lt + q * TL) * N0];
+            double tt = 0.0;
+            if (tk0) {
+                tt = 0.0 + tk0[col];
+                tt += tk1[lt + q * TL];
+            }
+            tq[q] = tt;
+        }
+    }
+    if (twl) stage_twiddles(twt, p);
+    const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
+    fft8<R>(v, ln, p, tws, -1, lt, active);
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = from_d<R>(cmul(to_d(v[q]), kin_phase(tq[q], J.tau, J.inv_n)));
+    }
+    fft8<R>(v, ln, p, tws, +1, lt, active);
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * N0] = v[q];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -350,6 +654,61 @@ void BatchLaunch<R>::col(hipStream_t s, int k, const BJob<R> *jobs, const BTile 
     if (lds > 160 * 1024) throw std::runtime_error("column tile exceeds the 160 KiB LDS of a gfx950 CU");
     allow_lds(k_bcol<R>, lds);
     QDX_LAUNCH((k_bcol<R>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void BatchLaunch<R>::line(hipStream_t s, int NT, const BJob<R> *jobs, const int *list, int n, const LinePlan *plans,
+                          const int *tick, size_t lds, bool tw_lds) {
+    if (n <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("1D job exceeds the 160 KiB LDS of a gfx950 CU");
+    if (NT == 256) {
+        allow_lds(k_bline<R, 256>, lds);
+        QDX_LAUNCH((k_bline<R, 256>), dim3(n), dim3(256), lds, s, jobs, list, plans, tick, tw_lds ? 1 : 0);
+    } else if (NT == 512) {
+        allow_lds(k_bline<R, 512>, lds);
+        QDX_LAUNCH((k_bline<R, 512>), dim3(n), dim3(512), lds, s, jobs, list, plans, tick, tw_lds ? 1 : 0);
+    } else {
+        throw std::runtime_error("1D jobs use 256 or 512 threads");
+    }
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void BatchLaunch<R>::gspo_row(hipStream_t s, int NT, int phase, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                              const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
+    if (ntiles <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("row tile exceeds the 160 KiB LDS of a gfx950 CU");
+    const int twl = tw_lds ? 1 : 0;
+    if (NT == 256) {
+        if (phase == 0) {
+            allow_lds(k_bgspo_row<R, 256, 0>, lds);
+            QDX_LAUNCH((k_bgspo_row<R, 256, 0>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, tick, twl);
+        } else {
+            allow_lds(k_bgspo_row<R, 256, 1>, lds);
+            QDX_LAUNCH((k_bgspo_row<R, 256, 1>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, tick, twl);
+        }
+    } else if (NT == 512) {
+        if (phase == 0) {
+            allow_lds(k_bgspo_row<R, 512, 0>, lds);
+            QDX_LAUNCH((k_bgspo_row<R, 512, 0>), dim3(ntiles), dim3(512), lds, s, jobs, tiles, plans, tick, twl);
+        } else {
+            allow_lds(k_bgspo_row<R, 512, 1>, lds);
+            QDX_LAUNCH((k_bgspo_row<R, 512, 1>), dim3(ntiles), dim3(512), lds, s, jobs, tiles, plans, tick, twl);
+        }
+    } else {
+        throw std::runtime_error("row tiles use 256 or 512 threads");
+    }
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void BatchLaunch<R>::gspo_col(hipStream_t s, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                              const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
+    if (ntiles <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("column tile exceeds the 160 KiB LDS of a gfx950 CU");
+    allow_lds(k_bgspo_col<R>, lds);
+    QDX_LAUNCH((k_bgspo_col<R>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, tick, tw_lds ? 1 : 0);
     QDX_HIP(hipGetLastError());
 }
 

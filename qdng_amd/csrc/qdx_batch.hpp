@@ -18,6 +18,8 @@ namespace qdx {
 // (OCheby.cpp:92-197, 358-493) for this job alone.
 template <class R>
 struct BJob {
+    int kind;               // 0: Chebyshev (one state), 1: GSPO (S = 1 or 2)
+    int S;                  // states
     int nd;                 // 1 or 2
     int N0, N1;             // N1 = 1 when nd == 1
     int plan0, plan1;       // indices into the plan table
@@ -46,10 +48,18 @@ struct BJob {
     double *part;           // 3 * nrt partial sums: [k % 2], [2] = step start
     ChebyCtl *ctl;
     long long *stats;       // [0] H applications, [1] steps done
+    // GSPO (OGSPO.cpp:114-139 with split operators T and W): one step is
+    // exp(-i dt/2 W) exp(-i dt T) exp(-i dt/2 W); W = [[V, V01], [V01, V11]]
+    // (minus E(t) mu on a single state); psi is held as complex R in psiR.
+    double tau;             // dt
+    R *psiR;                // complex R [S][size]
+    const R *V11, *V01;     // second diagonal element and coupling (nullptr: none)
+    const double *tk0s1;    // kinetic tables of state 1
+    const double *tk1s1;
 };
 
 // a row tile: rows row0 .. row0 + rb - 1 of job `job`; a column tile:
-// columns c0 .. c0 + W - 1
+// columns c0 .. c0 + W - 1 of state first / N0 (c0 = first % N0)
 struct BTile {
     int job;
     int first;
@@ -67,6 +77,17 @@ struct BatchLaunch {
     // kinetic pass along dimension 1 of the 2D jobs' X (forward * T/N * inverse)
     static void col(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
                     const LinePlan *plans, const int *tick, size_t lds, bool tw_lds);
+    // one whole time step of each 1D job in `list` (one block per job)
+    static void line(hipStream_t s, int NT, const BJob<R> *jobs, const int *list, int n, const LinePlan *plans,
+                     const int *tick, size_t lds, bool tw_lds);
+    // GSPO jobs: phase 0 = exp(-i tau/2 W) then FFT_0 into X (2D) or the
+    // whole step (1D); phase 1 = IFFT_0 of X then exp(-i tau/2 W) (2D)
+    static void gspo_row(hipStream_t s, int NT, int phase, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                         const LinePlan *plans, const int *tick, size_t lds, bool tw_lds);
+    // GSPO kinetic split operator along dimension 1 of the 2D jobs' X:
+    // forward * exp(-i tau T)/N * inverse
+    static void gspo_col(hipStream_t s, const BJob<R> *jobs, const BTile *tiles, int ntiles, const LinePlan *plans,
+                         const int *tick, size_t lds, bool tw_lds);
     // per-job bookkeeping at the end of a tick; advances *tick
     static void tick_end(hipStream_t s, const BJob<R> *jobs, int njobs, int *tick);
 };

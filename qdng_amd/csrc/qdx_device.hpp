@@ -340,7 +340,7 @@ __device__ __forceinline__ double fixed_sum(const double *p, int n, double *bcas
 
 // true when the previous order met |norm0 - accnorm| / norm0 < prec
 __device__ __forceinline__ bool prev_converged(const ConvCheck &c, double *bcast, double *acc_out) {
-    if (c.k_prev < 2) return false;
+    if (c.k_prev < 2 || !(c.prec > 0)) return false;   // prec 0: fixed-length series
     const double n0 = *c.norm0;
     const double acc = fixed_sum(c.part, c.nb, bcast);
     *acc_out = acc;

@@ -197,7 +197,7 @@ struct qdx_ctx {
     double acc_ms[QDX_NUM_KERNEL_CLASSES] = {};
     int64_t acc_n[QDX_NUM_KERNEL_CLASSES] = {};
     void fold() {
-        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipDeviceSynchronize());   // timed launches may sit on side streams
         for (int c = 0; c < QDX_NUM_KERNEL_CLASSES; ++c) {
             for (auto &sp : spans[c]) {
                 float ms = 0;
@@ -641,8 +641,7 @@ inline int bessel_j0(int n, double x, std::vector<double> &out) {
     return nz;
 }
 
-inline constexpr double kFp32PrecFloor = 1e-8;
-inline constexpr double kFp32CoeffFloor = 1e-9;
+inline constexpr double kFp32CoeffFloor = 1e-10;
 
 struct ChebyParams {
     int order = 0, nz = 0;
@@ -681,12 +680,14 @@ inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double p
     return c;
 }
 
-// FP32-bulk series: sum |psi|^2 of FP32 terms cannot resolve norm changes
-// much below 1e-9, so the break test uses at least 1e-8, and the series stops
-// where the coefficients fall below FP32 resolution (|c_k| < 1e-9 for all
-// remaining k beyond R dt).
+// FP32-bulk series.  The norm of a sum of FP32 terms wanders by ~1e-7 from
+// term to term, so QDng's break test (|norm0 - sum |psi|^2| / norm0 < prec)
+// can neither reach 1e-10 nor be trusted at a looser threshold (it may fire
+// while the series is still far from converged).  The FP32 series therefore
+// runs to a fixed length instead: every term up to the last k beyond R dt
+// with |c_k| >= 1e-10, the norm test switched off (prec = 0).
 inline void fp32_series(ChebyParams &cp, double dt) {
-    cp.prec = std::max(cp.prec, kFp32PrecFloor);
+    cp.prec = 0.0;
     const double x = cp.scaling * dt;
     int cut = cp.order;
     while (cut > 3 && cut - 1 > x && std::fabs(cp.coeff[cut - 1]) < kFp32CoeffFloor) --cut;

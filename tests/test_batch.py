@@ -141,3 +141,116 @@ def test_batch_rejects_unsupported_jobs(ctx):
     with pytest.raises(Q.QdxError) as e:
         B.submit(q.gpu_ham(None), 1.0, 10, q.psi0)   # laser shorter than the run
     assert e.value.code == -1
+
+
+# ---------------------------------------------------------------------------
+# GSPO jobs and the serve loop
+
+def gspo_jobs(steps, seed=21):
+    """two coupled states on a 2D grid, one laser-driven 1D state, one 2D state"""
+    rng = np.random.default_rng(seed)
+    a = cases.morse_2state_2d(64)
+    b = cases.driven_job([256], rng, steps)
+    c = cases.driven_job([96, 64], rng, steps)
+    return [a, b, c]
+
+
+def split_ops(p, ctx):
+    return [p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot", "dip"])]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_gspo_matches_single_path(ctx, precision):
+    import qdng_amd as Q
+    steps = 9
+    jobs = gspo_jobs(steps)
+    note(f"batched GSPO vs single path, {precision}")
+    B = Q.Batch(ctx, precision, 1 << 30)
+    ids = [B.submit(dt=1.0, nsteps=steps, psi0=p.psi0, ops=split_ops(p, None)) for p in jobs]
+    B.run(steps)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    for p, jid in zip(jobs, ids):
+        G = Q.GSPO(split_ops(p, ctx), 1.0, precision=precision)
+        w = p.gpu_wf(ctx)
+        G.run(w, 0, steps, observables=False)
+        ref = w.download()
+        got = B.fetch(jid)
+        left, nh, done = B.state(jid)
+        note(f"  job N={p.N} S={p.S} rel {rel(got, ref):.2e}")
+        assert left == 0 and done == steps and nh == steps
+        assert rel(got, ref) < tol
+
+
+def test_batch_gspo_fp64_matches_oracle(ctx):
+    import qdng_amd as Q
+    steps = 5
+    p = cases.morse_2state_2d(48)
+    note("batched GSPO fp64 vs oracle, coupled 2D")
+    B = Q.Batch(ctx, "fp64", 1 << 28)
+    jid = B.submit(dt=2.0, nsteps=steps, psi0=p.psi0, ops=split_ops(p, None))
+    B.run(steps)
+    Go = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], 2.0, p.S)
+    psi = p.psi0.copy()
+    for i in range(steps):
+        Go.step(psi, i)
+    assert rel(B.fetch(jid), psi) < 1e-12
+
+
+def serve_mix(steps_lo=4, steps_hi=14, seed=31):
+    rng = np.random.default_rng(seed)
+    shapes = [[128], [64, 64], [384], [96, 64], [4096], [64, 128], [192], [128, 128]]
+    reqs = []
+    for i, sh in enumerate(shapes):
+        n = int(rng.integers(steps_lo, steps_hi))
+        p = cases.driven_job(sh, rng, n)
+        reqs.append(("cheby", p, n))
+        if i % 3 == 0:
+            q = cases.morse_2state_2d(64)
+            reqs.append(("gspo", q, n))
+    return reqs
+
+
+def req_kwargs(kind, p, n):
+    if kind == "cheby":
+        return dict(H=p.gpu_ham(None), dt=1.0, nsteps=n, psi0=p.psi0)
+    return dict(dt=1.0, nsteps=n, psi0=p.psi0, ops=split_ops(p, None))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_serve_results_same_alone_and_in_mix(ctx, precision):
+    import qdng_amd as Q
+    reqs = serve_mix()
+    note(f"serve loop, {len(reqs)} requests, {precision}")
+    arrivals = np.cumsum(np.full(len(reqs), 2e-3))           # overlapping lifetimes
+    S = Q.Batch(ctx, precision, 1 << 30)
+    admit, finish, nh, psis = S.serve([req_kwargs(*r) for r in reqs], arrivals, want_psi=True)
+    assert np.all(admit >= arrivals - 1e-9)
+    assert np.all(finish >= admit - 1e-3)
+    assert S.usage()[0] == 0 and S.usage()[1] == 0           # all memory returned
+    for (kind, p, n), got, h in zip(reqs, psis, nh):
+        A = Q.Batch(ctx, precision, 1 << 28)
+        a = A.submit(**req_kwargs(kind, p, n))
+        A.run(n)
+        alone = A.fetch(a)
+        assert A.state(a)[1] == h
+        assert np.array_equal(got, alone), (kind, p.N)
+
+
+def test_serve_admission_queues_on_budget(ctx):
+    import qdng_amd as Q
+    reqs = serve_mix(seed=32)
+    probe = Q.Batch(ctx, "fp32", 1 << 28)
+    sizes = [probe.footprint(**req_kwargs(*r)) for r in reqs]
+    budget = max(sizes) + min(sizes) + 4096                   # at most two jobs at a time
+    note(f"serve loop with a budget of {budget} bytes")
+    S = Q.Batch(ctx, "fp32", budget)
+    arrivals = np.zeros(len(reqs))                            # all at once: most of them wait
+    admit, finish, nh = S.serve([req_kwargs(*r) for r in reqs], arrivals)
+    assert np.all(np.diff(admit) >= 0)                        # admitted in arrival order
+    assert np.all(nh > 0)
+    order = np.argsort(finish)
+    assert finish[order[-1]] > finish[order[0]]
+    with pytest.raises(Q.QdxError) as e:                      # larger than the whole budget
+        T = Q.Batch(ctx, "fp32", min(sizes))
+        T.serve([req_kwargs(*reqs[4])], [0.0])
+    assert e.value.code == -4
