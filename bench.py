@@ -179,6 +179,63 @@ def batched_extra(ctx, steps, prof_ticks):
     return out
 
 
+def serve_requests(lam, n_jobs, seed=7):
+    """Poisson arrivals (rate lam, seed 7): 90 % laser-driven Chebyshev jobs of
+    the batched distribution (half 1D, half 2D), 10 % two-state 256 x 256 GSPO
+    jobs (coupled Morse / harmonic surfaces); steps ~ U[200, 5000], dt = 1."""
+    import cases
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
+    reqs, meta = [], []
+    for i in range(n_jobs):
+        steps = int(rng.integers(200, 5001))
+        if rng.uniform() < 0.1:
+            p = cases.morse_2state_2d(256, seed=int(rng.integers(1 << 30)))
+            ops = [p.gpu_split_op(None, ["kin"]), p.gpu_split_op(None, ["pot"])]
+            reqs.append(dict(dt=1.0, nsteps=steps, psi0=p.psi0, ops=ops))
+            meta.append(("gspo", p.size * p.S))
+        else:
+            if rng.uniform() < 0.5:
+                p = cases.driven_job([int(rng.choice(cases.SIZES_1D_BATCH))], rng, steps)
+            else:
+                n = int(rng.choice(cases.SIZES_2D_BATCH))
+                p = cases.driven_job([n, n], rng, steps)
+            reqs.append(dict(H=p.gpu_ham(None), dt=1.0, nsteps=steps, psi0=p.psi0))
+            meta.append(("cheby", p.size))
+    return arrivals, reqs, meta
+
+
+def serving_extra(ctx, lams, n_jobs):
+    import qdng_amd as Q
+    out = {"mix": "90% batched-distribution Chebyshev jobs, 10% two-state 256x256 GSPO jobs, "
+                  "steps U[200, 5000], dt 1, Poisson arrivals (seed 7)", "jobs_per_rate": n_jobs, "rates": {}}
+    for lam in lams:
+        arrivals, reqs, meta = serve_requests(lam, n_jobs)
+        pts = np.array([m[1] for m in meta], dtype=float)
+        r = {}
+        finals = {}
+        for prec in ("fp32", "fp64"):
+            B = Q.Batch(ctx, prec, 32 << 30)
+            t0 = time.perf_counter()
+            admit, finish, nh, psis = B.serve(reqs, arrivals, want_psi=True)
+            wall = time.perf_counter() - t0
+            lat = finish - arrivals
+            span = finish.max() - arrivals.min()
+            r[prec] = {"throughput": float((pts * nh).sum() / span), "unit": "grid-pt*Hpsi/s (GSPO: grid-pt*step)",
+                       "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                       "mean_queue_wait_s": float((admit - arrivals).mean()), "span_s": float(span),
+                       "wall_s": wall}
+            finals[prec] = psis
+            log(f"serve lam={lam} {prec}: {r[prec]['throughput']:.3g}, p50 {r[prec]['p50_latency_s']:.3f} s, "
+                f"p99 {r[prec]['p99_latency_s']:.3f} s")
+        errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(finals["fp32"], finals["fp64"])]
+        r["l2_fp32_vs_fp64_median"] = float(np.median(errs))
+        r["l2_fp32_vs_fp64_max"] = float(np.max(errs))
+        r["fp32_vs_fp64_throughput"] = r["fp32"]["throughput"] / r["fp64"]["throughput"]
+        out["rates"][str(lam)] = r
+    return out
+
+
 def cpu_baseline(p, steps):
     """FP64 oracle on the host cores for `steps` Chebyshev steps from psi0."""
     from oracle import oracle as O
@@ -203,6 +260,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10, help="oracle steps timed for cpu_baseline")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=100, help="timed ticks of the batched-jobs workload")
+    ap.add_argument("--serve-jobs", type=int, default=96, help="requests per arrival rate of the serving run")
+    ap.add_argument("--serve-rates", default="50,500,2000", help="Poisson arrival rates (jobs/s), comma separated")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -316,6 +375,8 @@ def main():
             log(f"gspo {prec}: {npts * gsteps / t:.4g} grid-pt*step/s")
         if args.batch_steps > 0:
             ex["batched"] = batched_extra(ctx, args.batch_steps, 5)
+        if args.serve_jobs > 0:
+            ex["serving"] = serving_extra(ctx, [float(x) for x in args.serve_rates.split(",")], args.serve_jobs)
     print(json.dumps(out), flush=True)
     ctx.close()
 
