@@ -43,6 +43,21 @@ B_ROW = {"fp64": (16 + 16 + 16 + 32 + 8 + 8), "fp32": (8 + 8 + 8 + 32 + 4 + 4)}
 B_ROW_PRO = {"fp64": B_ROW["fp64"] - 16, "fp32": B_ROW["fp32"] - 8}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cheby_row_2d_512x512_2state_fp64_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per row-kernel launch from the committed FETCH_SIZE /
+    WRITE_SIZE profile of this workload (None if it is absent)."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -261,6 +276,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=100, help="timed ticks of the batched-jobs workload")
     ap.add_argument("--serve-jobs", type=int, default=96, help="requests per arrival rate of the serving run")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB device buffer 4 times (reference for FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--serve-rates", default="50,500,2000", help="Poisson arrival rates (jobs/s), comma separated")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -270,6 +287,13 @@ def main():
 
     import qdng_amd as Q
     ctx = Q.Context(0)
+    if args.calibrate_copy:
+        a = Q.WaveFunction(ctx, [4096, 4096], [-1.0, -1.0], [1.0, 1.0], 1)
+        b = Q.WaveFunction(ctx, [4096, 4096], [-1.0, -1.0], [1.0, 1.0], 1)
+        for _ in range(4):
+            b.copy_from(a)
+        log("copied 4 x 256 MiB device to device")
+        return
     p = build_problem(512)
     npts = p.size * p.S
     log(f"workload 512x512 x 2 states, {args.warmup} warmup + {args.steps} timed steps")
@@ -314,7 +338,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": "profiles/cheby_row_2d_512x512_2state_fp64_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.000 "
+                              "+ WRITE_SIZE x 1.000, calibrated on a 256 MiB device copy)",
             "kernel": "qdx::k_cheby_row<double, 2, false, false>",
             "launches": launches,
             "avg_launch_us": row_ms / launches * 1e3,
