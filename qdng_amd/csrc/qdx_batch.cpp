@@ -360,6 +360,8 @@ void build_tables(qdx_batch *b) {
         d.nt = J.nt;
         d.size = J.grid.size();
         d.inv_n = 1.0 / double(d.size);
+        d.inv_n0 = 1.0 / double(d.N0);
+        d.inv_n1 = 1.0 / double(d.N1);
         d.tau = J.dt;
         d.lscale = J.lscale;
         d.tk0 = J.has_T ? reinterpret_cast<const double *>(at(J.lay.tk0)) : nullptr;
@@ -594,6 +596,7 @@ BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
         host = cheby_host(h, d->nsteps);
         J.cp = cheby_params(h, d->dt, d->order, d->prec, 0.0);
         if (b->precision == QDX_FP32_BULK) fp32_series(J.cp, d->dt);
+        J.cp.order = launch_window(J.cp, d->dt);   // orders launched (and coefficients uploaded)
     } else {
         if (d->nops != 2 || !d->ops || !d->ops[0] || !d->ops[1])
             fail(QDX_ERR_INCOMPAT, "batched GSPO jobs take the split operators {T, W}");

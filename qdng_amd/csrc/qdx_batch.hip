@@ -8,12 +8,12 @@
 // series length and break test: blocks of a job that has converged, or whose
 // series is shorter than the order of the launch, return at once.
 //
-// Row tiles: rb lines of dimension 0 of one job (one line for 1D jobs).
-//   nd == 1: FFT, *T/N, IFFT of phi_{k-1} in registers/LDS;
-//   nd == 2: IFFT_0 of X (transformed along dimension 1 by the column pass),
-//            and X <- FFT_0(phi_k) after the epilogue.
-// Column tiles (2D jobs): W adjacent columns, forward * T/N * inverse along
-// dimension 1 on X.
+// The kinetic operator is separable, T(k) = T_0(k_0) + T_1(k_1), so T phi is
+// the sum of 1D filters IFFT_d(T_d/N_d FFT_d phi) along each dimension.
+// Row tiles: rb lines of dimension 0 of one job; the dimension-0 filter of
+// phi_{k-1} runs in registers/LDS, the dimension-1 part Y comes from the
+// column pass.  Column tiles (2D jobs): W adjacent columns of phi_{k-1},
+// filtered along dimension 1 into Y (held in X).
 #include <hip/hip_runtime.h>
 
 #include "qdx_batch.hpp"
@@ -77,22 +77,12 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
 
     if (PHASE == 0) {
         double acc = 0;
-        cx<R> v[8];
         if (B.active) {
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const cx<double> x = psi[B.gidx(q)];
                 acc += x.r * x.r + x.i * x.i;
-                v[q] = from_d<R>(x);
-                A[B.gidx(q)] = v[q];
-            }
-        }
-        if (two_d) {
-            if (twl) stage_twiddles(B.tw, p0);
-            bfft(v, B, p0, twl, -1);
-            if (B.active) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) X[B.gidx(q)] = v[q];
+                A[B.gidx(q)] = from_d<R>(x);
             }
         }
         const double part = block_sum(acc, red);
@@ -108,23 +98,18 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
 
     const cx<R> *phim1 = (k - 1) % 2 == 0 ? A : Bf;
     cx<R> *phik = k % 2 == 0 ? A : Bf;   // phi_k replaces phi_{k-2}
-    cx<R> v[8], pm1[8];
+    // only the transformed line stays live across the transforms; the
+    // epilogue operands are read afterwards
+    cx<R> v[8];
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) pm1[q] = phim1[B.gidx(q)];
-        if (two_d) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = X[B.gidx(q)];
-        } else {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = pm1[q];
-        }
+        for (int q = 0; q < 8; ++q) v[q] = phim1[B.gidx(q)];
     }
     const bool has_t = J.tk0 != nullptr;
     double tq[8];
-    if (!two_d && B.active) {
+    if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + J.tk0[B.lt + q * B.TL]) * J.inv_n : 0.0;
+        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + J.tk0[B.lt + q * B.TL]) * J.inv_n0 : 0.0;
     }
     if (twl) stage_twiddles(B.tw, p0);
     if (PHASE == 1) {
@@ -153,63 +138,79 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
         if (t.first == 0 && threadIdx.x == 0) J.ctl->last_order = k;
     }
     const double Ed = J.nt > 0 ? -J.laser[step] * J.lscale : 0.0;
-    if (!two_d) {
-        bfft(v, B, p0, twl, -1);
-        if (B.active) {
+    bfft(v, B, p0, twl, -1);
+    if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
-        }
+        for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
     bfft(v, B, p0, twl, +1);
 
+    // psi accumulation.  With the norm break test (prec > 0) every order adds
+    // its term and stores psi.  The fixed-length FP32 series (prec == 0)
+    // updates psi only at orders 1, 4, 7, ... and at the last order, adding
+    // the terms since the previous update from the stored phi_{k-2},
+    // phi_{k-1} and the new phi_k, in order.
+    const bool defer = !(J.prec > 0);
+    int nterm = 1;                                  // terms added to psi at this order
+    if (PHASE == 2 && defer) {
+        const int m = (k - 1) % 3;
+        nterm = m == 0 ? 3 : (k == J.order - 1 ? m : 0);
+    }
+    const bool upd = nterm > 0;
     double acc = 0;
     if (B.active) {
         const double *cf = J.coef;
         const cx<double> ck = mk<double>(cf[2 * (PHASE == 1 ? 1 : k)], cf[2 * (PHASE == 1 ? 1 : k) + 1]);
         const cx<double> c0 = mk<double>(cf[0], cf[1]);
-        cx<double> ps[8];
-        cx<R> m2[8];
-        double wv[8];
+        cx<double> cm1 = mk<double>(0, 0), cm2 = mk<double>(0, 0);
+        if (PHASE == 2 && nterm >= 2) cm1 = mk<double>(cf[2 * (k - 1)], cf[2 * (k - 1) + 1]);
+        if (PHASE == 2 && nterm >= 3) cm2 = mk<double>(cf[2 * (k - 2)], cf[2 * (k - 2) + 1]);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const long gi = B.gidx(q);
-            ps[q] = psi[gi];
-            if (PHASE == 2) m2[q] = phik[gi];
-            double w = 0.0;
-            if (J.V) w += (double)J.V[gi];
-            if (J.mu) w += Ed * (double)J.mu[gi];
-            wv[q] = w;
-        }
+        for (int h0 = 0; h0 < 8; h0 += 4) {
+            cx<double> ps[4];
+            cx<R> m2[4], yv[4], pm[4];
+            double wv[4];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const long gi = B.gidx(q);
-            cx<double> hp = has_t ? to_d(v[q]) : mk<double>(0, 0);
-            const cx<double> p = to_d(pm1[q]);
-            hp.r += wv[q] * p.r;
-            hp.i += wv[q] * p.i;
-            const cx<double> bb = mk<double>((hp.r - J.offset * p.r) * J.inv_scaling,
-                                             (hp.i - J.offset * p.i) * J.inv_scaling);
-            cx<double> pk;
-            cx<double> s = ps[q];
-            if (PHASE == 1) {
-                pk = mk<double>(bb.i, -bb.r);
-                s = cmul(c0, s) + cmul(ck, pk);
-            } else {
-                const cx<double> mm = to_d(m2[q]);
-                pk = mk<double>(2.0 * bb.i + mm.r, -2.0 * bb.r + mm.i);
-                s = s + cmul(ck, pk);
-                acc += s.r * s.r + s.i * s.i;
+            for (int u = 0; u < 4; ++u) {
+                const long gi = B.gidx(h0 + u);
+                if (upd) ps[u] = psi[gi];
+                pm[u] = phim1[gi];
+                if (PHASE == 2) m2[u] = phik[gi];
+                if (two_d) yv[u] = X[gi];
+                double w = 0.0;
+                if (J.V) w += (double)J.V[gi];
+                if (J.mu) w += Ed * (double)J.mu[gi];
+                wv[u] = w;
             }
-            psi[gi] = s;
-            v[q] = from_d<R>(pk);
-            phik[gi] = v[q];
-        }
-    }
-    if (two_d) {
-        bfft(v, B, p0, twl, -1);
-        if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) X[B.gidx(q)] = v[q];
+            for (int u = 0; u < 4; ++u) {
+                const int q = h0 + u;
+                const long gi = B.gidx(q);
+                cx<double> hp = mk<double>(0, 0);
+                if (has_t) hp = two_d ? to_d(v[q]) + to_d(yv[u]) : to_d(v[q]);
+                const cx<double> p = to_d(pm[u]);
+                hp.r += wv[u] * p.r;
+                hp.i += wv[u] * p.i;
+                const cx<double> bb = mk<double>((hp.r - J.offset * p.r) * J.inv_scaling,
+                                                 (hp.i - J.offset * p.i) * J.inv_scaling);
+                cx<double> pk;
+                if (PHASE == 1) {
+                    pk = mk<double>(bb.i, -bb.r);
+                    psi[gi] = cmul(c0, ps[u]) + cmul(ck, pk);
+                } else {
+                    const cx<double> mm = to_d(m2[u]);
+                    pk = mk<double>(2.0 * bb.i + mm.r, -2.0 * bb.r + mm.i);
+                    if (upd) {
+                        cx<double> sv = ps[u];
+                        if (nterm >= 3) sv = sv + cmul(cm2, mm);
+                        if (nterm >= 2) sv = sv + cmul(cm1, p);
+                        sv = sv + cmul(ck, pk);
+                        acc += sv.r * sv.r + sv.i * sv.i;
+                        psi[gi] = sv;
+                    }
+                }
+                phik[gi] = from_d<R>(pk);
+            }
         }
     }
     if (PHASE == 1) return;
@@ -217,7 +218,8 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
     if (threadIdx.x == 0) J.part[(k % 2) * J.nrt + local] = part;
 }
 
-// kinetic pass along dimension 1 of a 2D job: X <- IFFT_1(T/N * FFT_1(X))
+// dimension-1 part of the kinetic operator of a 2D job:
+// Y = IFFT_1(T_1/N_1 FFT_1 phi_{k-1}), held in X
 template <class R>
 __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
                                               const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
@@ -237,6 +239,7 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     const int col = t.first + c;
     const bool active = lt < TL && col < N0;
     cx<R> *base = reinterpret_cast<cx<R> *>(J.X) + col;
+    const cx<R> *in = reinterpret_cast<const cx<R> *>((k - 1) % 2 == 0 ? J.phiA : J.phiB) + col;
     LdsLine<R> ln;
     ln.a = smem + c;
     ln.b = smem + (long)W * L + c;
@@ -246,20 +249,13 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     cx<R> v[8];
     if (active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = base[(long)(lt + q * TL) * N0];
+        for (int q = 0; q < 8; ++q) v[q] = in[(long)(lt + q * TL) * N0];
     }
     double tq[8];
     if (active) {
-        const bool has_t = J.tk0 != nullptr;
+        const bool has_t = J.tk1 != nullptr;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            double tt = 0.0;
-            if (has_t) {
-                tt = 0.0 + J.tk0[col];
-                tt += J.tk1[lt + q * TL];
-            }
-            tq[q] = tt;
-        }
+        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + J.tk1[lt + q * TL]) * J.inv_n1 : 0.0;
     }
     if (twl) stage_twiddles(twt, p);
     if (k >= 3) {
@@ -276,10 +272,7 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     fft8<R>(v, ln, p, tws, -1, lt, active);
     if (active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const double tn = tq[q] * J.inv_n;
-            v[q] = mk<R>((R)((double)v[q].r * tn), (R)((double)v[q].i * tn));
-        }
+        for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
     fft8<R>(v, ln, p, tws, +1, lt, active);
     if (active) {

@@ -32,6 +32,7 @@ struct BJob {
     int nt;                 // laser samples (0: no dipole term)
     long size;              // N0 * N1
     double inv_n;           // 1 / size
+    double inv_n0, inv_n1;  // 1 / N0, 1 / N1: T = T_0 + T_1 acts as 1D filters
     double offset;          // (Emax + Emin) / 2
     double inv_scaling;     // 1 / R
     double prec;            // break threshold

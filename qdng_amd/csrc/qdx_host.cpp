@@ -28,6 +28,7 @@ struct qdx_prop {
     // Chebyshev
     qdx_ham *ham = nullptr;
     ChebyParams cp;
+    int k_launch = 0;   // orders launched per step (launch_window)
     DevBuf phiA, phiB, X, ctl, partials, lastorders;
     int last_nb = 0, nb_init = 0;   // partial counts of the last Chebyshev step
     // GSPO
@@ -70,23 +71,18 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     int nb_init;
     {
         KTimer t(ctx, QDX_K_CHEBY_INIT);
-        nb_init = Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, X, ctl, init);
+        nb_init = Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, nullptr, ctl, init);
     }
 
     ConvCheck cc;
     cc.norm0 = &ctl->norm0;
     cc.prec = p->cp.prec;
-    auto col = [&](int dim, int mode, const ConvCheck &c) {
-        KTimer t(ctx, QDX_K_LINE_PASS);
-        Launch<R>::col_pass(s, g, S, dim, mode, plans[dim], h, 0.0, X, ctl, c);
-    };
-    auto cols = [&](const ConvCheck &c) {
-        if (p->grid.nd == 2) {
-            col(1, 2, c);
-        } else if (p->grid.nd == 3) {
-            col(1, 0, c);
-            col(2, 2, c);
-            col(1, 1, c);
+    // T = sum_d T_d: the kinetic parts along dimensions >= 1 of phi_{k-1} go
+    // to X; the row kernel adds the dimension-0 part
+    auto cols = [&](const R *pm1, const ConvCheck &c) {
+        for (int d = 1; d < p->grid.nd; ++d) {
+            KTimer t(ctx, QDX_K_LINE_PASS);
+            Launch<R>::kin_pass(s, g, S, d, d > 1, plans[d], h, pm1, X, ctl, c);
         }
     };
     const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
@@ -100,23 +96,35 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     ord.c0i = c0.imag();
     ord.ckr = c1.real();
     ord.cki = c1.imag();
-    cols(cc);   // k_prev = 0: no test
+    cols(A, cc);   // k_prev = 0: no test
     {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
         Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, A, B, X, psi->ptr(), ctl, slot[1], init, nb_init);
     }
     int nb_prev = 0;
-    for (int k = 2; k < p->cp.order; ++k) {
+    for (int k = 2; k < p->k_launch; ++k) {
         const std::complex<double> ck = p->cp.coeff[k] * ph;
         ord.k = k;
         ord.ckr = ck.real();
         ord.cki = ck.imag();
+        // fixed-length series (no break test): psi is updated every third
+        // order and at the last one with the terms since the previous update
+        ord.nterm = 1;
+        if (!(p->cp.prec > 0)) {
+            const int m = (k - 1) % 3;
+            ord.nterm = m == 0 ? 3 : (k == p->k_launch - 1 ? m : 0);
+            const std::complex<double> c1 = p->cp.coeff[k - 1] * ph, c2 = p->cp.coeff[k - 2] * ph;
+            ord.cm1r = c1.real();
+            ord.cm1i = c1.imag();
+            ord.cm2r = c2.real();
+            ord.cm2i = c2.imag();
+        }
         cc.k_prev = k - 1;
         cc.part = slot[(k - 1) % 2];
         cc.nb = nb_prev;
-        cols(cc);
         R *pm1 = (k - 1) % 2 == 0 ? A : B;
         R *pm2 = k % 2 == 0 ? A : B;
+        cols(pm1, cc);
         KTimer t(ctx, QDX_K_CHEBY_ROW);
         nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, pm1, pm2, X, psi->ptr(), ctl, slot[k % 2], init,
                                        nb_init);
@@ -626,6 +634,7 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         p->ham = h;
         p->cp = cheby_params(h, dt, order, prec, scaling);
         if (precision == QDX_FP32_BULK) fp32_series(p->cp, dt);
+        p->k_launch = launch_window(p->cp, dt);
         if (precision == QDX_FP32_BULK) h->ensure32();
         const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
         const size_t nbytes = (size_t)p->S * p->grid.size() * es;

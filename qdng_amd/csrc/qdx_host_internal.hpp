@@ -476,6 +476,7 @@ struct qdx_ham {
         HamDev<double> &h = dev.h64;
         h.S = S;
         h.inv_n = 1.0 / double(n);
+        for (int d = 0; d < 3; ++d) h.inv_nd[d] = d < grid.nd ? 1.0 / double(grid.N[d]) : 1.0;
         hostW.assign(S * S, {});
         hostMu.assign(S * S, {});
         std::vector<double> tkh;
@@ -560,6 +561,7 @@ struct qdx_ham {
         const HamDev<double> &d = dev.h64;
         f.S = d.S;
         f.inv_n = d.inv_n;
+        for (int k = 0; k < 3; ++k) f.inv_nd[k] = d.inv_nd[k];
         for (int i = 0; i < S; ++i) {
             f.unity[i] = d.unity[i];
             for (int k = 0; k < 3; ++k) f.tk[i][k] = d.tk[i][k];
@@ -686,6 +688,20 @@ inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double p
 // while the series is still far from converged).  The FP32 series therefore
 // runs to a fixed length instead: every term up to the last k beyond R dt
 // with |c_k| >= 1e-10, the norm test switched off (prec = 0).
+// Orders to launch per step.  With the norm break test on (prec > 0) the
+// series ends where |norm0 - sum |psi|^2| / norm0 < prec; launches are not
+// issued for the trailing orders whose coefficients are all below
+// 1e-4 * prec (beyond R dt): in QDng those orders only run when the norm
+// test has not yet fired, and their terms together change psi by less than
+// 1e-4 * prec relative.  Returns the series length when prec == 0.
+inline int launch_window(const ChebyParams &cp, double dt) {
+    if (!(cp.prec > 0)) return cp.order;
+    const double x = cp.scaling * dt, floor = 1e-4 * cp.prec;
+    int cut = cp.order;
+    while (cut > 3 && cut - 1 > x && std::fabs(cp.coeff[cut - 1]) < floor) --cut;
+    return cut;
+}
+
 inline void fp32_series(ChebyParams &cp, double dt) {
     cp.prec = 0.0;
     const double x = cp.scaling * dt;

@@ -46,6 +46,7 @@ template <class R>
 struct HamDev {
     int S = 1;
     double inv_n = 1.0;
+    double inv_nd[3] = {1.0, 1.0, 1.0};   // 1 / N_d: T = sum_d T_d acts as 1D filters
     const double *tk[kMaxStates][3] = {};
     const R *W[kMaxStates][kMaxStates] = {};
     const R *mu[kMaxStates][kMaxStates] = {};
@@ -84,6 +85,11 @@ struct ChebyOrder {
     double c0r, c0i;     // order 1: coeff[0] * exp(-i offset dt)
     double ckr, cki;     // coeff[k] * exp(-i offset dt)
     double prec;         // break threshold
+    // psi update at this order: the last `nterm` terms (k-2, k-1, k), 0 for
+    // none (the fixed-length FP32 series updates psi every third order)
+    int nterm = 1;
+    double cm1r = 0, cm1i = 0;   // coeff[k-1] * exp(-i offset dt)
+    double cm2r = 0, cm2i = 0;   // coeff[k-2] * exp(-i offset dt)
 };
 
 // One x-space split operator of GSPO: exp(c * W(x)) with c = -i tau, W the
@@ -121,9 +127,9 @@ struct Launch {
     // returns the number of partial sums written (one per block)
     static int cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
                           const double *psi, R *phi0, R *X, ChebyCtl *ctl, double *partials);
-    // One order of the series; prologue when ord.k == 1.
-    //   nd == 1: Tphi from phi_{k-1} in LDS; nd >= 2: Tphi = IFFT_0(X) and
-    //   X <- FFT_0(phi_k) afterwards.
+    // One order of the series; prologue when ord.k == 1.  T phi_{k-1} is
+    // the dimension-0 filter IFFT_0(T_0/N_0 FFT_0 phi_{k-1}) in registers,
+    // plus (nd >= 2) the other dimensions' part Y that kin_pass left in X.
     // init_part/nb_init: partials of the init kernel, summed into
     // ctl->norm0 by the prologue (ord.k == 1)
     static int cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
@@ -135,6 +141,11 @@ struct Launch {
     // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator)
     static void col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode,
                          const LinePlan &p, const HamDev<R> &h, double tau, R *X,
+                         const ChebyCtl *ctl, const ConvCheck &cc);
+    // kinetic part along dimension `dim` of the separable T = sum_d T_d:
+    // Y = IFFT_d(T_d/N_d FFT_d(src)), or Y += that when `accumulate`
+    static void kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate,
+                         const LinePlan &p, const HamDev<R> &h, const R *src, R *Y,
                          const ChebyCtl *ctl, const ConvCheck &cc);
     // dst = H src (both complex128) through the same kernels; X scratch
     static void ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h,

@@ -155,25 +155,21 @@ __global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LineP
     const int st = B.st;
     const long base = (long)st * g.size;
 
-    // phi operands of the epilogue are requested before anything else; psi
-    // and the operator arrays are read in the epilogue itself
-    cx<R> v[8], pm1[S][8];
+    // the line of phi_{k-1} this thread transforms; the epilogue operands
+    // (psi, phi_{k-2}, Y, phi_{k-1} of every state, W) are read after the
+    // transforms, so that only the line stays live across them
+    cx<R> v[8];
     if (B.active) {
-        const cx<R> *src = ONE_D ? phim1 : X;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = src[base + B.gidx(q)];
-#pragma unroll
-        for (int j = 0; j < S; ++j)
-#pragma unroll
-            for (int q = 0; q < 8; ++q) pm1[j][q] = phim1[(long)j * g.size + B.gidx(q)];
+        for (int q = 0; q < 8; ++q) v[q] = phim1[base + B.gidx(q)];
     }
     // this thread's row of the operator matrix (runtime state index, read once)
     const double *tk0 = h.tk[st][0];
     const bool has_t = tk0 != nullptr;
     double tq[8];
-    if (ONE_D && B.active) {
+    if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_n : 0.0;
+        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_nd[0] : 0.0;
     }
     B.stage(p0);
     if (PRO) {
@@ -207,75 +203,80 @@ __global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LineP
         Ed[j] = h.Edip[st][j];
     }
     const double uni = h.unity[st];
-    if (ONE_D) {
-        B.fft(v, p0, -1);
-        if (B.active) {
+    // T_0 part along dimension 0 (Kspace filter, 1/N_0 folded into tq)
+    B.fft(v, p0, -1);
+    if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
-        }
+        for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
     B.fft(v, p0, +1);
 
-    // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j.
-    // All loads are issued before the first store of the epilogue.
+    // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j,
+    // in two halves of four points: each half reads all its operands before
+    // its first store
+    const int nterm = PRO ? 1 : ord.nterm;
+    const bool upd = nterm > 0;
     double acc = 0;
     if (B.active) {
         const cx<double> ck = mk<double>(ord.ckr, ord.cki);
         const cx<double> c0 = mk<double>(ord.c0r, ord.c0i);
-        cx<double> ps[8];
-        cx<R> m2[8];
-        double wv[S][8];
+        const cx<double> cm1 = mk<double>(ord.cm1r, ord.cm1i), cm2 = mk<double>(ord.cm2r, ord.cm2i);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const long gi = B.gidx(q);
-            ps[q] = psi[base + gi];
-            if (!PRO) m2[q] = phim2[base + gi];
+        for (int h0 = 0; h0 < 8; h0 += 4) {
+            cx<double> ps[4];
+            cx<R> m2[4], yv[4], pm[S][4];
+            double wv[S][4];
 #pragma unroll
-            for (int j = 0; j < S; ++j) {
-                double w = (j == st) ? uni : 0.0;
-                if (Wp[j]) w += (double)Wp[j][gi];
-                if (Mp[j]) w += Ed[j] * (double)Mp[j][gi];
-                wv[j][q] = w;
+            for (int u = 0; u < 4; ++u) {
+                const long gi = B.gidx(h0 + u);
+                if (upd) ps[u] = psi[base + gi];
+                if (!PRO) m2[u] = phim2[base + gi];
+                if (!ONE_D) yv[u] = X[base + gi];
+#pragma unroll
+                for (int j = 0; j < S; ++j) {
+                    pm[j][u] = phim1[(long)j * g.size + gi];
+                    double w = (j == st) ? uni : 0.0;
+                    if (Wp[j]) w += (double)Wp[j][gi];
+                    if (Mp[j]) w += Ed[j] * (double)Mp[j][gi];
+                    wv[j][u] = w;
+                }
             }
-        }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const long gi = B.gidx(q);
-            cx<double> hp = has_t ? to_d(v[q]) : mk<double>(0, 0);
-            cx<double> p = to_d(pm1[0][q]);
+            for (int u = 0; u < 4; ++u) {
+                const int q = h0 + u;
+                const long gi = B.gidx(q);
+                cx<double> hp = mk<double>(0, 0);
+                if (has_t) hp = ONE_D ? to_d(v[q]) : to_d(v[q]) + to_d(yv[u]);
+                cx<double> p = to_d(pm[0][u]);
 #pragma unroll
-            for (int j = 0; j < S; ++j) {
-                const double w = wv[j][q];
-                const cx<double> pj = to_d(pm1[j][q]);
-                if (j == st) p = pj;
-                hp.r += w * pj.r;
-                hp.i += w * pj.i;
+                for (int j = 0; j < S; ++j) {
+                    const double w = wv[j][u];
+                    const cx<double> pj = to_d(pm[j][u]);
+                    if (j == st) p = pj;
+                    hp.r += w * pj.r;
+                    hp.i += w * pj.i;
+                }
+                // b = (H phi - offset phi) / R
+                const cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
+                                                (hp.i - ord.offset * p.i) * ord.inv_scaling);
+                cx<double> pk;
+                if (PRO) {
+                    pk = mk<double>(b.i, -b.r);                               // * (-i)
+                    psi[base + gi] = cmul(c0, ps[u]) + cmul(ck, pk);
+                } else {
+                    const cx<double> mm = to_d(m2[u]);
+                    pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);    // * (-2i) + phi_{k-2}
+                    if (upd) {
+                        cx<double> sv = ps[u];
+                        if (nterm >= 3) sv = sv + cmul(cm2, mm);
+                        if (nterm >= 2) sv = sv + cmul(cm1, p);
+                        sv = sv + cmul(ck, pk);
+                        acc += sv.r * sv.r + sv.i * sv.i;
+                        psi[base + gi] = sv;
+                    }
+                }
+                phim2[base + gi] = from_d<R>(pk);
             }
-            // b = (H phi - offset phi) / R
-            const cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
-                                            (hp.i - ord.offset * p.i) * ord.inv_scaling);
-            cx<double> pk;
-            cx<double> s = ps[q];
-            if (PRO) {
-                pk = mk<double>(b.i, -b.r);                               // * (-i)
-                s = cmul(c0, s) + cmul(ck, pk);
-            } else {
-                const cx<double> mm = to_d(m2[q]);
-                pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);    // * (-2i) + phi_{k-2}
-                s = s + cmul(ck, pk);
-                acc += s.r * s.r + s.i * s.i;
-            }
-            psi[base + gi] = s;
-            v[q] = from_d<R>(pk);
-            phim2[base + gi] = v[q];
-        }
-    }
-    // X <- FFT_0(phi_k) for the next order's strided passes
-    if (!ONE_D) {
-        B.fft(v, p0, -1);
-        if (B.active) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) X[base + B.gidx(q)] = v[q];
         }
     }
     if (PRO) return;
@@ -290,10 +291,13 @@ __global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LineP
 // of the remaining dimension; the W lanes of a butterfly touch W adjacent
 // words in LDS and W adjacent complex values in memory.
 // MODE 0: forward, 1: inverse, 2: forward * T/N inverse,
-//      3: forward * exp(-i tau T)/N inverse.
+//      3: forward * exp(-i tau T)/N inverse,
+//      4: X = IFFT(T_dim/N_dim FFT(src)), the dimension's part of the
+//         separable kinetic operator, 5: X += the same.
 template <class R, int MODE>
 __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
-                                              cx<R> *__restrict__ X, const ChebyCtl *ctl, ConvCheck cc) {
+                                              cx<R> *__restrict__ X, const cx<R> *__restrict__ src,
+                                              const ChebyCtl *ctl, ConvCheck cc) {
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
@@ -320,7 +324,7 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     ln.es = W;
     ln.pad = false;
     cx<R> *twl = smem + (long)2 * W * L;
-    cx<R> v[8];
+    cx<R> v[8], yv[8];
     const bool tw_in_lds = p.tw_lds;
     auto FFT = [&](int sign) {
         if (tw_in_lds) fft8<R>(v, ln, p, twl, sign, lt, active);
@@ -328,8 +332,13 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     };
 
     if (active) {
+        const cx<R> *in = MODE >= 4 ? src + (base - X) : base;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = base[(long)(lt + q * TL) * estride];
+        for (int q = 0; q < 8; ++q) v[q] = in[(long)(lt + q * TL) * estride];
+        if (MODE == 5) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) yv[q] = base[(long)(lt + q * TL) * estride];
+        }
     }
     // kinetic table values of this thread's elements, requested early
     double tq[8];
@@ -339,7 +348,8 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int kd = lt + q * TL;
-            tq[q] = has_t ? kin_sum<R>(h, st, g, k0, dim == 1 ? kd : oth, dim == 2 ? kd : oth) : 0.0;
+            if (MODE >= 4) tq[q] = has_t ? (0.0 + h.tk[st][dim][kd]) * h.inv_nd[dim] : 0.0;
+            else tq[q] = has_t ? kin_sum<R>(h, st, g, k0, dim == 1 ? kd : oth, dim == 2 ? kd : oth) : 0.0;
         }
     }
     if (tw_in_lds) stage_twiddles(twl, p);
@@ -353,7 +363,9 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const double tt = tq[q];
-                if (MODE == 2) {
+                if (MODE >= 4) {
+                    v[q] = mk<R>((R)((double)v[q].r * tt), (R)((double)v[q].i * tt));
+                } else if (MODE == 2) {
                     const double t = tt * h.inv_n;
                     v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
                 } else {
@@ -367,6 +379,10 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
         FFT(+1);
     }
     if (active) {
+        if (MODE == 5) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = v[q] + yv[q];
+        }
 #pragma unroll
         for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * estride] = v[q];
     }
@@ -797,23 +813,47 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     cx<R> *x = reinterpret_cast<cx<R> *>(X);
     LinePlan pl = plan;
     pl.tw_lds = 1;
+    const cx<R> *none = nullptr;
     switch (mode) {
     case 0:
         set_lds(k_col<R, 0>, lds);
-        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     case 1:
         set_lds(k_col<R, 1>, lds);
-        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     case 2:
         set_lds(k_col<R, 2>, lds);
-        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     default:
         set_lds(k_col<R, 3>, lds);
-        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, ctl, cc);
+        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
+    }
+    QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void Launch<R>::kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate, const LinePlan &plan,
+                         const HamDev<R> &h, const R *src, R *Y, const ChebyCtl *ctl, const ConvCheck &cc) {
+    const int L = g.N[dim];
+    const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
+    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
+    const int nb = (g.N[0] / W) * nother * S;
+    const int threads = std::max(64, round64(W * (L / 8)));
+    const size_t lds = ((size_t)2 * W * L + L) * sizeof(cx<R>);
+    cx<R> *y = reinterpret_cast<cx<R> *>(Y);
+    const cx<R> *in = reinterpret_cast<const cx<R> *>(src);
+    LinePlan pl = plan;
+    pl.tw_lds = 1;
+    if (accumulate) {
+        set_lds(k_col<R, 5>, lds);
+        QDX_LAUNCH((k_col<R, 5>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+    } else {
+        set_lds(k_col<R, 4>, lds);
+        QDX_LAUNCH((k_col<R, 4>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
     }
     QDX_HIP(hipGetLastError());
 }

@@ -48,7 +48,11 @@ def test_batch_matches_single_path(ctx, precision):
     B = Q.Batch(ctx, precision, 1 << 30)
     ids = [B.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in jobs]
     B.run(steps)
-    tol = 1e-12 if precision == "fp64" else 1e-6
+    # FP32 paths: the 1D jobs run whole steps in registers (psi updated every
+    # order), the single path updates psi every third order from the stored
+    # FP32 terms; both carry the FP32 transform error, ~5e-6 relative to FP64
+    # for the 3072-point job's 58-term series over 12 steps
+    tol = 1e-12 if precision == "fp64" else 2e-5
     for p, jid in zip(jobs, ids):
         left, nh, done = B.state(jid)
         got = B.fetch(jid)
