@@ -232,17 +232,20 @@ JobLayout job_layout(const BatchJob &J, size_t rsz) {
     return l;
 }
 
+// LDS of the tiles: padded line buffers (L + L/8 entries per line) and the
+// padded twiddle copy (L + L/8 entries) when it is staged
+size_t lp(int L) { return (size_t)L + L / 8; }
 size_t row_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw) {
     const int L = g.N[0];
-    return ((size_t)2 * t.rb * (L + L / 8) + (tw ? L : 0)) * esz;
+    return (2 * t.rb * lp(L) + (tw ? lp(L) : 0)) * esz;
 }
 size_t col_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw) {
     const int L = g.N[1];
-    return ((size_t)2 * t.W * L + (tw ? L : 0)) * esz;
+    return (2 * t.W * lp(L) + (tw ? lp(L) : 0)) * esz;
 }
 size_t line_lds(const Grid &g, size_t esz, bool tw) {
     const int L = g.N[0];
-    return ((size_t)2 * (L + L / 8) + (tw ? L : 0)) * esz;
+    return (2 * lp(L) + (tw ? lp(L) : 0)) * esz;
 }
 
 constexpr size_t kLds = 160 * 1024;

@@ -49,8 +49,8 @@ struct BRowTile {
 
 template <class R>
 __device__ __forceinline__ void bfft(cx<R> (&v)[8], const BRowTile<R> &B, const LinePlan &p, bool twl, int sign) {
-    if (twl) fft8<R>(v, B.ln, p, B.tw, sign, B.lt, B.active);
-    else fft8<R>(v, B.ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, B.lt, B.active);
+    if (twl) fft8<R>(v, B.ln, p, B.tw, sign, B.lt, B.active, kTwLds);
+    else fft8<R>(v, B.ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, B.lt, B.active, kTwMem);
 }
 
 // PHASE 0: init (phi_0 = psi, sum |psi|^2 of the step start, X = FFT_0 phi_0)
@@ -242,10 +242,10 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     const cx<R> *in = reinterpret_cast<const cx<R> *>((k - 1) % 2 == 0 ? J.phiA : J.phiB) + col;
     LdsLine<R> ln;
     ln.a = smem + c;
-    ln.b = smem + (long)W * L + c;
+    ln.b = smem + (long)W * tw_lds_len(L) + c;
     ln.es = W;
-    ln.pad = false;
-    cx<R> *twt = smem + (long)2 * W * L;
+    ln.pad = true;
+    cx<R> *twt = smem + (long)2 * W * tw_lds_len(L);
     cx<R> v[8];
     if (active) {
 #pragma unroll
@@ -269,12 +269,13 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
         if (prev_converged(cc, &bc, &a)) return;
     }
     const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
-    fft8<R>(v, ln, p, tws, -1, lt, active);
+    const int tsh = twl ? kTwLds : kTwMem;
+    fft8<R>(v, ln, p, tws, -1, lt, active, tsh);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
-    fft8<R>(v, ln, p, tws, +1, lt, active);
+    fft8<R>(v, ln, p, tws, +1, lt, active, tsh);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * N0] = v[q];
@@ -322,6 +323,7 @@ __global__ void __launch_bounds__(NT) k_bline(const BJob<R> *__restrict__ jobs, 
     cx<R> *twt = smem + 2 * (L + (L >> 3));
     if (twl) stage_twiddles(twt, p);
     const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
+    const int tsh = twl ? kTwLds : kTwMem;
     cx<double> *psi = reinterpret_cast<cx<double> *>(J.psi);
 
     cx<double> ps[8];
@@ -355,12 +357,12 @@ __global__ void __launch_bounds__(NT) k_bline(const BJob<R> *__restrict__ jobs, 
         // T phi_{k-1} (OGridNablaSq through Kspace: FFT, *T/N, IFFT)
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = pm1[q];
-        fft8<R>(v, ln, p, tws, -1, lt, active);
+        fft8<R>(v, ln, p, tws, -1, lt, active, tsh);
         if (active) {
 #pragma unroll
             for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
         }
-        fft8<R>(v, ln, p, tws, +1, lt, active);
+        fft8<R>(v, ln, p, tws, +1, lt, active, tsh);
         const cx<double> ck = mk<double>(J.coef[2 * k], J.coef[2 * k + 1]);
         acc = 0;
         if (active) {
@@ -566,10 +568,10 @@ __global__ void __launch_bounds__(256) k_bgspo_col(const BJob<R> *__restrict__ j
     cx<R> *base = reinterpret_cast<cx<R> *>(J.X) + (long)st * J.size + col;
     LdsLine<R> ln;
     ln.a = smem + c;
-    ln.b = smem + (long)W * L + c;
+    ln.b = smem + (long)W * tw_lds_len(L) + c;
     ln.es = W;
-    ln.pad = false;
-    cx<R> *twt = smem + (long)2 * W * L;
+    ln.pad = true;
+    cx<R> *twt = smem + (long)2 * W * tw_lds_len(L);
     const double *tk0 = st == 0 ? J.tk0 : J.tk0s1;
     const double *tk1 = st == 0 ? J.tk1 : J.tk1s1;
     cx<R> v[8];
@@ -588,12 +590,13 @@ __global__ void __launch_bounds__(256) k_bgspo_col(const BJob<R> *__restrict__ j
     }
     if (twl) stage_twiddles(twt, p);
     const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
-    fft8<R>(v, ln, p, tws, -1, lt, active);
+    const int tsh = twl ? kTwLds : kTwMem;
+    fft8<R>(v, ln, p, tws, -1, lt, active, tsh);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = from_d<R>(cmul(to_d(v[q]), kin_phase(tq[q], J.tau, J.inv_n)));
     }
-    fft8<R>(v, ln, p, tws, +1, lt, active);
+    fft8<R>(v, ln, p, tws, +1, lt, active, tsh);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * N0] = v[q];

@@ -153,16 +153,18 @@ struct LdsLine {
     __device__ __forceinline__ int idx(int i) const { return (pad ? i + (i >> 3) : i) * es; }
 };
 
+// twiddle q of a table; tsh = 3 for the padded LDS copy (entry q at
+// q + q/8, see stage_twiddles), 31 for the plain table in memory
 template <class R>
-__device__ __forceinline__ cx<R> twiddle(const cx<R> *tw, int q, int sign) {
-    cx<R> w = tw[q];
+__device__ __forceinline__ cx<R> twiddle(const cx<R> *tw, int q, int sign, int tsh) {
+    cx<R> w = tw[q + (q >> tsh)];
     if (sign > 0) w.i = -w.i;
     return w;
 }
 
 template <int RR, class R>
 __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const LdsLine<R> &ln, int L, int Ns,
-                                          int TL, int lt, int sign, const cx<R> *tw) {
+                                          int TL, int lt, int sign, const cx<R> *tw, int tsh) {
     constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
     const int m = L / RR;
     const int tstep = L / (Ns * RR);
@@ -177,7 +179,7 @@ __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const Lds
             for (int t = 0; t < RR; ++t) x[t] = in[ln.idx(j + t * m)];
             if (k != 0) {
 #pragma unroll
-                for (int t = 1; t < RR; ++t) x[t] = cmul(x[t], twiddle(tw, k * t * tstep, sign));
+                for (int t = 1; t < RR; ++t) x[t] = cmul(x[t], twiddle(tw, k * t * tstep, sign, tsh));
             }
             Bfly<RR, R>::run(x, sign);
             const int ob = (j - k) * RR + k;
@@ -187,20 +189,26 @@ __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const Lds
     }
 }
 
-// copy the twiddle table of plan p into LDS (all threads of the block)
+// copy the twiddle table of plan p into LDS (all threads of the block),
+// entry q at q + q/8: the strided reads of a stage (q = k t L/(Ns r)) then
+// fall on different banks.  The copy takes L + L/8 entries.
 template <class R>
 __device__ __forceinline__ void stage_twiddles(cx<R> *dst, const LinePlan &p) {
     const cx<R> *src = reinterpret_cast<const cx<R> *>(p.tw);
-    for (int i = threadIdx.x; i < p.L; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < p.L; i += blockDim.x) dst[i + (i >> 3)] = src[i];
 }
+constexpr int kTwLds = 3;    // twiddle shift of the padded LDS copy
+constexpr int kTwMem = 31;   // twiddle shift of the table in memory
+__host__ __device__ constexpr int tw_lds_len(int L) { return L + (L >> 3); }
 
 // Full unnormalised transform (sign -1 forward).  Every thread of the block
 // must call it (it synchronises); threads without a line pass active=false.
-// tw: the table exp(-2 pi i q / L), q in [0, L), usually an LDS copy made by
-// stage_twiddles (the barrier at the start of fft8 publishes it).
+// tw: the table exp(-2 pi i q / L), q in [0, L): the padded LDS copy made by
+// stage_twiddles (tsh = kTwLds; the barrier at the start of fft8 publishes
+// it) or the table in memory (tsh = kTwMem).
 template <class R>
 __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const LinePlan &p, const cx<R> *tw,
-                                     int sign, int lt, bool active) {
+                                     int sign, int lt, bool active, int tsh) {
     const int L = p.L, TL = L >> 3;
     if (p.nst == 1) {   // L == 8
         if (active) Bfly<8, R>::run(v, sign);
@@ -242,10 +250,10 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
         const int r = p.rad[st];
         if (active) {
             switch (r) {
-            case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
-            case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
-            case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
-            default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lt, sign, tw); break;
+            case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
+            case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
+            case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
+            default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
             }
         }
         __syncthreads();
@@ -264,7 +272,7 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
         for (int t = 0; t < 8; ++t) x[t] = in[ln.idx(j + t * Ns)];
         if (j != 0) {
 #pragma unroll
-            for (int t = 1; t < 8; ++t) x[t] = cmul(x[t], twiddle(tw, j * t, sign));
+            for (int t = 1; t < 8; ++t) x[t] = cmul(x[t], twiddle(tw, j * t, sign, tsh));
         }
         Bfly<8, R>::run(x, sign);
 #pragma unroll
@@ -278,7 +286,7 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             for (int t = 0; t < 4; ++t) x[t] = in[ln.idx(j + t * Ns)];
             if (j != 0) {
 #pragma unroll
-                for (int t = 1; t < 4; ++t) x[t] = cmul(x[t], twiddle(tw, j * t, sign));
+                for (int t = 1; t < 4; ++t) x[t] = cmul(x[t], twiddle(tw, j * t, sign, tsh));
             }
             Bfly<4, R>::run(x, sign);
 #pragma unroll
@@ -291,7 +299,7 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             cx<R> x[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) x[t] = in[ln.idx(j + t * Ns)];
-            if (j != 0) x[1] = cmul(x[1], twiddle(tw, j, sign));
+            if (j != 0) x[1] = cmul(x[1], twiddle(tw, j, sign, tsh));
             Bfly<2, R>::run(x, sign);
             v[q] = x[0];
             v[q + 4] = x[1];

@@ -81,7 +81,7 @@ struct RowBlock {
         ln.pad = true;
         tw = smem + (long)2 * nl * lp;
         twl = p.tw_lds;
-        after = twl ? tw + L : tw;
+        after = twl ? tw + tw_lds_len(L) : tw;
     }
     // stage the twiddle table (when it lives in LDS)
     __device__ void stage(const LinePlan &p) {
@@ -89,15 +89,15 @@ struct RowBlock {
     }
     // transform with the LDS or the global twiddle table; both calls inline
     __device__ void fft(cx<R> (&v)[8], const LinePlan &p, int sign) {
-        if (twl) fft8<R>(v, ln, p, tw, sign, lt, active);
-        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active);
+        if (twl) fft8<R>(v, ln, p, tw, sign, lt, active, kTwLds);
+        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active, kTwMem);
     }
     __device__ long gidx(int q) const { return row * L + lt + q * TL; }
 };
 
 template <class R>
 static size_t row_lds_bytes(int L, int nl, bool twl) {
-    return ((size_t)2 * nl * (L + L / 8) + (twl ? L : 0)) * sizeof(cx<R>);
+    return ((size_t)2 * nl * (L + L / 8) + (twl ? tw_lds_len(L) : 0)) * sizeof(cx<R>);
 }
 
 // ---------------------------------------------------------------------------
@@ -141,8 +141,10 @@ __global__ void __launch_bounds__(512) k_cheby_init(GridDev g, LinePlan p0, int 
 
 // ---------------------------------------------------------------------------
 // Chebyshev row kernel (see file header).  PRO: prologue (order 1).
-template <class R, int S, bool ONE_D, bool PRO>
-__global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord,
+// NT: the largest block the instance is launched with (256 leaves twice the
+// registers per thread of 512).
+template <class R, int S, bool ONE_D, bool PRO, int NT>
+__global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord,
                                                     ConvCheck cc, int rb, const cx<R> *__restrict__ phim1,
                                                     cx<R> *__restrict__ phim2, cx<R> *__restrict__ X,
                                                     cx<double> *__restrict__ psi, ChebyCtl *ctl,
@@ -203,6 +205,78 @@ __global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LineP
         Ed[j] = h.Edip[st][j];
     }
     const double uni = h.unity[st];
+    // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j,
+    // in two halves of four points.  The first half's operands are requested
+    // before the transforms (their latency hides behind them), the second
+    // half's right after.
+    const int nterm = PRO ? 1 : ord.nterm;
+    const bool upd = nterm > 0;
+    struct Half {
+        cx<double> ps[4];
+        cx<R> m2[4], yv[4], pm[S][4];
+        double wv[S][4];
+    };
+    auto load_half = [&](Half &H, int h0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long gi = B.gidx(h0 + u);
+            if (upd) H.ps[u] = psi[base + gi];
+            if (!PRO) H.m2[u] = phim2[base + gi];
+            if (!ONE_D) H.yv[u] = X[base + gi];
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                H.pm[j][u] = phim1[(long)j * g.size + gi];
+                double w = (j == st) ? uni : 0.0;
+                if (Wp[j]) w += (double)Wp[j][gi];
+                if (Mp[j]) w += Ed[j] * (double)Mp[j][gi];
+                H.wv[j][u] = w;
+            }
+        }
+    };
+    double acc = 0;
+    const cx<double> ck = mk<double>(ord.ckr, ord.cki);
+    const cx<double> c0 = mk<double>(ord.c0r, ord.c0i);
+    const cx<double> cm1 = mk<double>(ord.cm1r, ord.cm1i), cm2 = mk<double>(ord.cm2r, ord.cm2i);
+    auto finish_half = [&](const Half &H, int h0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = h0 + u;
+            const long gi = B.gidx(q);
+            cx<double> hp = mk<double>(0, 0);
+            if (has_t) hp = ONE_D ? to_d(v[q]) : to_d(v[q]) + to_d(H.yv[u]);
+            cx<double> p = to_d(H.pm[0][u]);
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const double w = H.wv[j][u];
+                const cx<double> pj = to_d(H.pm[j][u]);
+                if (j == st) p = pj;
+                hp.r += w * pj.r;
+                hp.i += w * pj.i;
+            }
+            // b = (H phi - offset phi) / R
+            const cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
+                                            (hp.i - ord.offset * p.i) * ord.inv_scaling);
+            cx<double> pk;
+            if (PRO) {
+                pk = mk<double>(b.i, -b.r);                               // * (-i)
+                psi[base + gi] = cmul(c0, H.ps[u]) + cmul(ck, pk);
+            } else {
+                const cx<double> mm = to_d(H.m2[u]);
+                pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);    // * (-2i) + phi_{k-2}
+                if (upd) {
+                    cx<double> sv = H.ps[u];
+                    if (nterm >= 3) sv = sv + cmul(cm2, mm);
+                    if (nterm >= 2) sv = sv + cmul(cm1, p);
+                    sv = sv + cmul(ck, pk);
+                    acc += sv.r * sv.r + sv.i * sv.i;
+                    psi[base + gi] = sv;
+                }
+            }
+            phim2[base + gi] = from_d<R>(pk);
+        }
+    };
+    Half H0, H1;
+    if (B.active) load_half(H0, 0);
     // T_0 part along dimension 0 (Kspace filter, 1/N_0 folded into tq)
     B.fft(v, p0, -1);
     if (B.active) {
@@ -210,74 +284,10 @@ __global__ void __launch_bounds__(512) k_cheby_row(GridDev g, HamDev<R> h, LineP
         for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
     B.fft(v, p0, +1);
-
-    // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j,
-    // in two halves of four points: each half reads all its operands before
-    // its first store
-    const int nterm = PRO ? 1 : ord.nterm;
-    const bool upd = nterm > 0;
-    double acc = 0;
     if (B.active) {
-        const cx<double> ck = mk<double>(ord.ckr, ord.cki);
-        const cx<double> c0 = mk<double>(ord.c0r, ord.c0i);
-        const cx<double> cm1 = mk<double>(ord.cm1r, ord.cm1i), cm2 = mk<double>(ord.cm2r, ord.cm2i);
-#pragma unroll
-        for (int h0 = 0; h0 < 8; h0 += 4) {
-            cx<double> ps[4];
-            cx<R> m2[4], yv[4], pm[S][4];
-            double wv[S][4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const long gi = B.gidx(h0 + u);
-                if (upd) ps[u] = psi[base + gi];
-                if (!PRO) m2[u] = phim2[base + gi];
-                if (!ONE_D) yv[u] = X[base + gi];
-#pragma unroll
-                for (int j = 0; j < S; ++j) {
-                    pm[j][u] = phim1[(long)j * g.size + gi];
-                    double w = (j == st) ? uni : 0.0;
-                    if (Wp[j]) w += (double)Wp[j][gi];
-                    if (Mp[j]) w += Ed[j] * (double)Mp[j][gi];
-                    wv[j][u] = w;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = h0 + u;
-                const long gi = B.gidx(q);
-                cx<double> hp = mk<double>(0, 0);
-                if (has_t) hp = ONE_D ? to_d(v[q]) : to_d(v[q]) + to_d(yv[u]);
-                cx<double> p = to_d(pm[0][u]);
-#pragma unroll
-                for (int j = 0; j < S; ++j) {
-                    const double w = wv[j][u];
-                    const cx<double> pj = to_d(pm[j][u]);
-                    if (j == st) p = pj;
-                    hp.r += w * pj.r;
-                    hp.i += w * pj.i;
-                }
-                // b = (H phi - offset phi) / R
-                const cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
-                                                (hp.i - ord.offset * p.i) * ord.inv_scaling);
-                cx<double> pk;
-                if (PRO) {
-                    pk = mk<double>(b.i, -b.r);                               // * (-i)
-                    psi[base + gi] = cmul(c0, ps[u]) + cmul(ck, pk);
-                } else {
-                    const cx<double> mm = to_d(m2[u]);
-                    pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);    // * (-2i) + phi_{k-2}
-                    if (upd) {
-                        cx<double> sv = ps[u];
-                        if (nterm >= 3) sv = sv + cmul(cm2, mm);
-                        if (nterm >= 2) sv = sv + cmul(cm1, p);
-                        sv = sv + cmul(ck, pk);
-                        acc += sv.r * sv.r + sv.i * sv.i;
-                        psi[base + gi] = sv;
-                    }
-                }
-                phim2[base + gi] = from_d<R>(pk);
-            }
-        }
+        load_half(H1, 4);
+        finish_half(H0, 0);
+        finish_half(H1, 4);
     }
     if (PRO) return;
     // partial sum |psi|^2 of this block; the next kernels test convergence
@@ -320,15 +330,15 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     cx<R> *base = X + (long)st * g.size + (long)oth * ostride + c0 + c;
     LdsLine<R> ln;
     ln.a = smem + c;
-    ln.b = smem + (long)W * L + c;
+    ln.b = smem + (long)W * tw_lds_len(L) + c;
     ln.es = W;
-    ln.pad = false;
-    cx<R> *twl = smem + (long)2 * W * L;
+    ln.pad = true;
+    cx<R> *twl = smem + (long)2 * W * tw_lds_len(L);
     cx<R> v[8], yv[8];
     const bool tw_in_lds = p.tw_lds;
     auto FFT = [&](int sign) {
-        if (tw_in_lds) fft8<R>(v, ln, p, twl, sign, lt, active);
-        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active);
+        if (tw_in_lds) fft8<R>(v, ln, p, twl, sign, lt, active, kTwLds);
+        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active, kTwMem);
     };
 
     if (active) {
@@ -690,13 +700,18 @@ static void set_lds(K kernel, size_t bytes) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// LDS of a strided pass: two padded buffers of W lines and the twiddle copy
+static size_t col_lds_bytes(int W, int L, size_t elem) {
+    return ((size_t)2 * W * tw_lds_len(L) + tw_lds_len(L)) * elem;
+}
+
 // tile width of the strided passes: adjacent dimension-0 positions per block.
 // Prefer >= 64-byte segments, keep >= 256 blocks when possible.
 static int col_tile(const GridDev &g, int S, int dim, int L, size_t elem) {
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     int W = 16;
     auto fits = [&](int w) {
-        return g.N[0] % w == 0 && ((size_t)2 * w * L + L) * elem <= 72 * 1024 && w * (L / 8) <= 512;
+        return g.N[0] % w == 0 && col_lds_bytes(w, L, elem) <= 96 * 1024 && w * (L / 8) <= 512;
     };
     while (W > 1 && !fits(W)) W /= 2;
     const int wmin = std::max(1, (int)(64 / elem));
@@ -769,11 +784,19 @@ static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, co
     LinePlan p0 = plan;
     RowLaunch r = row_launch<R>(g, S, p0);
     if (r.nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
-    set_lds(k_cheby_row<R, S, ONE_D, PRO>, r.lds);
-    QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
-                       r.rb, reinterpret_cast<const cx<R> *>(phim1), reinterpret_cast<cx<R> *>(phim2),
-                       reinterpret_cast<cx<R> *>(X), reinterpret_cast<cx<double> *>(psi), ctl, partials, init_part,
-                       nb_init);
+    auto cphim1 = reinterpret_cast<const cx<R> *>(phim1);
+    auto cphim2 = reinterpret_cast<cx<R> *>(phim2);
+    auto cX = reinterpret_cast<cx<R> *>(X);
+    auto cpsi = reinterpret_cast<cx<double> *>(psi);
+    if (r.threads <= 256) {
+        set_lds(k_cheby_row<R, S, ONE_D, PRO, 256>, r.lds);
+        QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 256>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
+                   r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
+    } else {
+        set_lds(k_cheby_row<R, S, ONE_D, PRO, 512>, r.lds);
+        QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 512>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
+                   r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
+    }
     QDX_HIP(hipGetLastError());
     return r.nb;
 }
@@ -809,7 +832,7 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     const int nb = (g.N[0] / W) * nother * S;
     const int threads = std::max(64, round64(W * (L / 8)));
-    const size_t lds = ((size_t)2 * W * L + L) * sizeof(cx<R>);
+    const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *x = reinterpret_cast<cx<R> *>(X);
     LinePlan pl = plan;
     pl.tw_lds = 1;
@@ -843,7 +866,7 @@ void Launch<R>::kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool a
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     const int nb = (g.N[0] / W) * nother * S;
     const int threads = std::max(64, round64(W * (L / 8)));
-    const size_t lds = ((size_t)2 * W * L + L) * sizeof(cx<R>);
+    const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *y = reinterpret_cast<cx<R> *>(Y);
     const cx<R> *in = reinterpret_cast<const cx<R> *>(src);
     LinePlan pl = plan;
