@@ -36,5 +36,21 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+REPO = os.path.dirname(HERE)
+CPP_TESTS = [("tests/cpp/test_qdlib.cpp", "tests/cpp/test_qdlib")]
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """Host-side C++ test programs of the QDng-style classes
+    (include/qdx_qdlib.hpp), linked against libqdx.so."""
+    for src, exe in CPP_TESTS:
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(REPO, "include"), os.path.join(REPO, src),
+               "-L" + HERE, "-lqdx", "-Wl,-rpath,$ORIGIN/../../qdng_amd", "-o", os.path.join(REPO, exe)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    build_cpp_tests(verbose="-v" in sys.argv)

@@ -1,0 +1,301 @@
+// Tests of the QDng-style C++ classes (include/qdx_qdlib.hpp) on the GPU,
+// written after QDng's CppUnit tests with grid operators in place of the
+// level-system ones:
+//   TestCheby (src/tests/TestCheby.cpp:60-125): parameters after Init and a
+//     two-state system whose upper state turns by -i over dt = pi;
+//   OGSPOTest (src/tests/OGSPOTest.cpp:96-121): a coupling of 1/2 moves the
+//     whole population over dt = pi (half a Rabi cycle);
+//   OGridNablaSqTest (src/tests/OGridNablaSqTest.cpp:76-137): the periodic
+//     sine is an eigenfunction with eigenvalue 2 (pi / (L + dx))^2 / (2 m);
+// plus a harmonic-oscillator propagation (Chebyshev against GSPO, norm) and
+// QDng's exceptions for bad parameters.
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "qdx_qdlib.hpp"
+
+using namespace qdxlib;
+
+static int g_fail = 0, g_run = 0;
+#define LOOSE_EPS 1e-10
+#define ROUGH_EPS 1e-6
+
+#define ASSERT_NEAR(exp, got, eps)                                                              \
+    do {                                                                                        \
+        const double e_ = (exp), g_ = (got);                                                    \
+        if (!(std::fabs(e_ - g_) <= (eps))) {                                                   \
+            std::printf("  FAIL %s:%d: expected %.17g got %.17g\n", __FILE__, __LINE__, e_, g_); \
+            throw 1;                                                                            \
+        }                                                                                       \
+    } while (0)
+#define ASSERT_TRUE(c)                                                          \
+    do {                                                                        \
+        if (!(c)) {                                                             \
+            std::printf("  FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);          \
+            throw 1;                                                            \
+        }                                                                       \
+    } while (0)
+template <class E, class F>
+static void assert_throws(F f, int line) {
+    try {
+        f();
+    } catch (const E &) {
+        return;
+    }
+    std::printf("  FAIL line %d: expected exception not thrown\n", line);
+    throw 1;
+}
+#define ASSERT_THROW(E, stmt) assert_throws<E>([&] { stmt; }, __LINE__)
+
+static void run(const char *name, const std::function<void()> &f) {
+    ++g_run;
+    std::printf("[cpp] %s\n", name);
+    std::fflush(stdout);
+    try {
+        f();
+    } catch (int) {
+        ++g_fail;
+    } catch (const std::exception &e) {
+        std::printf("  FAIL: exception %s\n", e.what());
+        ++g_fail;
+    }
+}
+
+static qdx_grid line_grid(int N, double xmin, double xmax) {
+    qdx_grid g{};
+    g.ndims = 1;
+    g.N[0] = N;
+    g.xmin[0] = xmin;
+    g.xmax[0] = xmax;
+    return g;
+}
+
+// two states on 8 points, V00 = 0, V11 = 1/2 (TestCheby's matrix)
+struct TwoLevel {
+    OGridPotential v00, v11;
+    OMultistate H;
+    TwoLevel() {
+        v00.SetData(std::vector<double>(8, 0.0));
+        v11.SetData(std::vector<double>(8, 0.5));
+        H.Add(&v00, 0, 0);
+        H.Add(&v11, 1, 1);
+    }
+};
+
+int main() {
+    const qdx_grid g8 = line_grid(8, -1, 1);
+
+    run("TestCheby::API_Test", [&] {
+        TwoLevel sys;
+        QDClock clock;
+        clock.Dt(M_PI);
+        clock.Steps(2);
+        OCheby U;
+        U.Clock(&clock);
+        ASSERT_TRUE(U.Name() == "OCheby");
+        U.Add("hamiltonian", &sys.H);
+        WFMultistate psi0;
+        psi0.Init(g8, 2);
+        ParamContainer pm;
+        U.Init(pm);
+        U.Init(&psi0);
+        double d;
+        ASSERT_TRUE(U.Params().GetValue("prec", d));
+        ASSERT_NEAR(1e-10, d, 1e-30);
+        U.Params().GetValue("offset.imag", d);
+        ASSERT_NEAR(0, d, 1e-30);
+        U.Params().GetValue("offset.real", d);
+        ASSERT_NEAR(0.25, d, 1e-30);
+        U.Params().GetValue("scaling", d);
+        ASSERT_NEAR(0.25, d, 1e-30);
+    });
+
+    run("TestCheby::NUMERIC_Test", [&] {
+        TwoLevel sys;
+        QDClock clock;
+        clock.Dt(M_PI);
+        OCheby U;
+        U.Clock(&clock);
+        U.Add("hamiltonian", &sys.H);
+        WFMultistate psi0, psi1;
+        psi0.Init(g8, 2);
+        psi1.Init(g8, 2);
+        for (int x = 0; x < 8; ++x) {
+            psi0.at(0, x) = 1;
+            psi0.at(1, x) = 0;
+        }
+        ParamContainer pm;
+        U.Init(pm);
+        U.Init(&psi0);
+        // psi = (1, 0): nothing happens
+        U.Apply(&psi1, &psi0);
+        for (int x = 0; x < 8; ++x) {
+            ASSERT_NEAR(1, psi1.at(0, x).real(), LOOSE_EPS);
+            ASSERT_NEAR(0, psi1.at(0, x).imag(), LOOSE_EPS);
+            ASSERT_NEAR(0, std::abs(psi1.at(1, x)), LOOSE_EPS);
+        }
+        // psi = (0, 1): exp(-i pi/2) = -i
+        for (int x = 0; x < 8; ++x) {
+            psi0.at(0, x) = 0;
+            psi0.at(1, x) = 1;
+        }
+        U.Apply(&psi1, &psi0);
+        for (int x = 0; x < 8; ++x) {
+            ASSERT_NEAR(0, std::abs(psi1.at(0, x)), LOOSE_EPS);
+            ASSERT_NEAR(0, psi1.at(1, x).real(), LOOSE_EPS);
+            ASSERT_NEAR(-1, psi1.at(1, x).imag(), LOOSE_EPS);
+        }
+    });
+
+    run("OGSPOTest::NUMERIC_Test_H0", [&] {
+        OGridPotential c01;
+        c01.SetData(std::vector<double>(8, 0.5));
+        OMultistate W;
+        W.Add(&c01, 0, 1);
+        ParamContainer ms;
+        ms.SetValue("unity", false);
+        W.Init(ms);
+        QDClock clock;
+        clock.Dt(M_PI);
+        OGSPO U;
+        U.Clock(&clock);
+        ASSERT_TRUE(U.Name() == "OGSPO");
+        U.Add("A", &W);
+        WFMultistate psi0, psi1;
+        psi0.Init(g8, 2);
+        psi1.Init(g8, 2);
+        for (int x = 0; x < 8; ++x) psi0.at(0, x) = 1;
+        ParamContainer pm;
+        U.Init(pm);
+        U.Init(&psi0);
+        U.Apply(&psi1, &psi0);
+        for (int x = 0; x < 8; ++x) {
+            ASSERT_NEAR(0, std::norm(psi1.at(0, x)), LOOSE_EPS);
+            ASSERT_NEAR(1, std::norm(psi1.at(1, x)), LOOSE_EPS);
+        }
+        psi1 = psi0;
+        U.Apply(&psi1);
+        for (int x = 0; x < 8; ++x) {
+            ASSERT_NEAR(0, std::norm(psi1.at(0, x)), LOOSE_EPS);
+            ASSERT_NEAR(1, std::norm(psi1.at(1, x)), LOOSE_EPS);
+        }
+    });
+
+    for (int N : {64, 96}) {
+        run(N == 64 ? "OGridNablaSqTest::NUMERIC_Test (N = 64)" : "OGridNablaSqTest::NUMERIC_Test (N = 96)", [&] {
+            const double L = 10, dx = L / (N - 1);
+            const double pre = 2 * (M_PI / (L + dx)) * (M_PI / (L + dx));
+            ParamContainer gp;
+            gp.SetValue("dims", 1);
+            gp.SetValue("N0", N);
+            gp.SetValue("xmin0", -5.0);
+            gp.SetValue("xmax0", 5.0);
+            WFGridCartesian wf;
+            wf.Init(gp);
+            std::unique_ptr<WFGridCartesian> wf2(wf.NewInstance());
+            for (int i = 0; i < N; ++i) wf[i] = std::sin(2 * M_PI * i / N);   // period L + dx
+            OGridNablaSq T;
+            ASSERT_TRUE(T.Name() == "OGridNablaSq");
+            ParamContainer p;
+            p.SetValue("mass", 1.0);
+            T.Init(p);
+            OSum H;
+            H.Add(&T);
+            H.Apply(wf2.get(), &wf);
+            for (int i = 0; i < N; ++i) {
+                ASSERT_NEAR(pre * wf[i].real(), (*wf2)[i].real(), LOOSE_EPS);
+                ASSERT_NEAR(pre * wf[i].imag(), (*wf2)[i].imag(), LOOSE_EPS);
+            }
+            // Emax of the kinetic operator: (pi^2 / 2) / (m dx^2)
+            ASSERT_NEAR(M_PI * M_PI / 2 / (dx * dx), H.Emax().real(), 1e-9);
+        });
+    }
+
+    run("harmonic oscillator: OCheby and OGSPO agree, norm and energy conserved", [&] {
+        const int N = 256;
+        ParamContainer gp;
+        gp.SetValue("dims", 1);
+        gp.SetValue("N0", N);
+        gp.SetValue("xmin0", -8.0);
+        gp.SetValue("xmax0", 8.0);
+        WFGridCartesian a, b;
+        a.Init(gp);
+        b.Init(gp);
+        const double dx = 16.0 / (N - 1);
+        std::vector<double> V(N);
+        for (int i = 0; i < N; ++i) {
+            const double x = -8 + i * dx;
+            V[i] = 0.05 * x * x;
+            a[i] = b[i] = std::exp(-(x - 1) * (x - 1));
+        }
+        a.Normalize();
+        b.Normalize();
+        OGridNablaSq T;
+        ParamContainer p;
+        p.SetValue("mass", 1836.0);
+        T.Init(p);
+        OGridPotential pot;
+        pot.SetData(V);
+        OSum H;
+        H.Add(&T);
+        H.Add(&pot);
+        OSum Tk;
+        Tk.Add(&T);
+        OSum Vp;
+        Vp.Add(&pot);
+        QDClock clock;
+        clock.Dt(2.0);
+        OCheby U;
+        U.Clock(&clock);
+        U.Add("hamiltonian", &H);
+        OGSPO G;
+        G.Clock(&clock);
+        G.Add("A", &Tk);
+        G.Add("B", &Vp);
+        const double e0 = H.Expec(&a).real();
+        for (int s = 0; s < 50; ++s) {
+            clock.TimeStep(s);
+            U.Apply(&a);
+            G.Apply(&b);
+        }
+        // the series stops at a norm change below prec = 1e-10 per step
+        ASSERT_NEAR(1.0, a.Norm(), 1e-8);
+        ASSERT_NEAR(e0, H.Expec(&a).real(), 1e-8);
+        const double ov = std::abs(a * b);
+        ASSERT_NEAR(1.0, ov, 1e-4);   // second-order splitting error of GSPO at dt = 2
+    });
+
+    run("exceptions", [&] {
+        ParamContainer p;
+        p.SetValue("mass", 0.0);
+        OGridNablaSq T;
+        ASSERT_THROW(EParamProblem, T.Init(p));
+        ParamContainer gp;
+        gp.SetValue("dims", 1);
+        gp.SetValue("N0", 100);   // not 8 * 2^a * 3^b
+        gp.SetValue("xmin0", -1.0);
+        gp.SetValue("xmax0", 1.0);
+        WFGridCartesian wf;
+        ASSERT_THROW(EParamProblem, wf.Init(gp));
+        OCheby U;
+        QDClock clock;
+        clock.Dt(1.0);
+        U.Clock(&clock);
+        WFMultistate psi;
+        psi.Init(line_grid(8, -1, 1), 2);
+        ASSERT_THROW(EParamProblem, U.Init(&psi));          // no hamiltonian
+        TwoLevel sys;
+        OCheby U2;
+        U2.Clock(&clock);
+        U2.Add("hamiltonian", &sys.H);
+        WFMultistate three;
+        three.Init(line_grid(8, -1, 1), 3);
+        ASSERT_THROW(EIncompatible, U2.Init(&three));       // 2x2 operator on 3 states
+    });
+
+    std::printf("[cpp] %d tests, %d failed\n", g_run, g_fail);
+    return g_fail == 0 ? 0 : 1;
+}
