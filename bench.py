@@ -123,19 +123,28 @@ def run_batched(ctx, precision, steps, prof_ticks, seed=2026):
     timings and the final states."""
     import cases
     import qdng_amd as Q
-    jobs = cases.batch_jobs(steps=steps + prof_ticks + 1, seed=seed)
+    eager_ticks = max(1, min(20, steps))
+    total = steps + eager_ticks + prof_ticks + 2
+    jobs = cases.batch_jobs(steps=total, seed=seed)
     B = Q.Batch(ctx, precision, 16 << 30)
-    ids = [B.submit(p.gpu_ham(None), 1.0, steps + prof_ticks + 1, p.psi0) for p in jobs]
-    B.run(1)                       # first tick builds the tables
+    ids = [B.submit(p.gpu_ham(None), 1.0, total, p.psi0) for p in jobs]
+    B.run(2)                       # tables built, tick graph captured
     sync(ctx)
     h0 = np.array([B.state(j)[1] for j in ids])
     t0 = time.perf_counter()
-    B.run(steps)
+    B.run(steps)                   # graph replays
     sync(ctx)
     t1 = time.perf_counter()
     h1 = np.array([B.state(j)[1] for j in ids])
     sizes = np.array([p.size for p in jobs])
     work = float((sizes * (h1 - h0)).sum())
+    graphs = B.graph_stats()
+    B.set_graphs(False)            # the same ticks launched kernel by kernel
+    te0 = time.perf_counter()
+    B.run(eager_ticks)
+    sync(ctx)
+    eager_ms = (time.perf_counter() - te0) * 1e3 / eager_ticks
+    h1 = np.array([B.state(j)[1] for j in ids])
     ctx.profile(True)
     B.run(prof_ticks)
     sync(ctx)
@@ -149,7 +158,8 @@ def run_batched(ctx, precision, steps, prof_ticks, seed=2026):
     return {"seconds": t1 - t0, "work": work, "row": rowk, "col": colk, "line": linek,
             "row_bytes": float(alg[two_d].sum()), "line_bytes": float(alg[~two_d].sum()),
             "work_2d": float((sizes * (h1 - h0))[two_d].sum()),
-            "jobs": jobs, "ids": ids, "batch": B, "points": int(sizes.sum())}
+            "jobs": jobs, "ids": ids, "batch": B, "points": int(sizes.sum()),
+            "graphs": graphs, "eager_ms_per_tick": eager_ms}
 
 
 def batched_extra(ctx, steps, prof_ticks):
@@ -168,6 +178,8 @@ def batched_extra(ctx, steps, prof_ticks):
             "effective_GBs": B_BATCH[prec] * value / 1e9,
             "effective_frac": B_BATCH[prec] * value / 1e9 / HBM_PEAK_GBS,
             "value_2d_jobs": r["work_2d"] / r["seconds"],
+            "tick_graphs": {"captured": r["graphs"][0], "replays": r["graphs"][1],
+                            "ms_per_tick_kernel_by_kernel": r["eager_ms_per_tick"]},
             "row_kernel": {"achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "launches": n,
                            "avg_launch_us": ms / max(n, 1) * 1e3, "jobs": "2D"},
             "col_kernel": {"launches": nc, "avg_launch_us": msc / max(nc, 1) * 1e3},

@@ -236,6 +236,16 @@ int qdx_batch_fetch(qdx_batch *b, int64_t id, double *psi);
 int qdx_batch_release(qdx_batch *b, int64_t id);
 /* jobs held, arena bytes in use, ticks enqueued so far */
 int qdx_batch_usage(qdx_batch *b, int32_t *jobs, int64_t *bytes_used, int64_t *ticks);
+/* Tick replay through hipGraphs (default on).  The second tick of one shape
+ * (same job mix, same active job classes, same longest series) is captured
+ * into a graph and later ticks of that shape replay it; the kernels read the
+ * tick from a device counter, so one graph serves every such tick.  A new
+ * job mix (submit, release) starts new graphs.  Off: every tick is launched
+ * kernel by kernel.  Results are the same either way.  No counterpart in
+ * QDng (its propagators run on the host). */
+int qdx_batch_set_graphs(qdx_batch *b, int32_t enable);
+/* graphs captured and graph replays so far (either may be NULL) */
+int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed);
 
 /* Continuous batching over a stream of requests.  Request i arrives at
  * arrival_s[i] seconds after the call starts (non-decreasing).  Arrived

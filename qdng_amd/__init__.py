@@ -56,7 +56,7 @@ EXPORTS = [
     "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
     "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
     "qdx_batch_job_state", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
-    "qdx_batch_serve", "qdx_batch_destroy",
+    "qdx_batch_serve", "qdx_batch_set_graphs", "qdx_batch_graph_stats", "qdx_batch_destroy",
 ]
 
 K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
@@ -126,6 +126,8 @@ def lib():
         "qdx_batch_release": [vp, C.c_int64],
         "qdx_batch_usage": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "qdx_batch_serve": [vp, C.POINTER(JobDesc), dp, i32, i32, C.POINTER(dp), C.POINTER(ServeResult)],
+        "qdx_batch_set_graphs": [vp, i32],
+        "qdx_batch_graph_stats": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "qdx_batch_destroy": [vp],
     }
     for name, args in sig.items():
@@ -494,6 +496,17 @@ class Batch:
         n, b, t = C.c_int32(), C.c_int64(), C.c_int64()
         _chk(lib().qdx_batch_usage(self.h, C.byref(n), C.byref(b), C.byref(t)))
         return n.value, b.value, t.value
+
+    def set_graphs(self, enable: bool) -> None:
+        """replay ticks from captured hipGraphs (default) or launch every
+        kernel of a tick from the host"""
+        _chk(lib().qdx_batch_set_graphs(self.h, 1 if enable else 0))
+
+    def graph_stats(self):
+        """(graphs captured, graph replays) so far"""
+        c, r = C.c_int64(), C.c_int64()
+        _chk(lib().qdx_batch_graph_stats(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def __del__(self):
         try:

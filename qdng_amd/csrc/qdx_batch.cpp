@@ -8,6 +8,7 @@
 #include <chrono>
 #include <deque>
 #include <thread>
+#include <tuple>
 
 namespace {
 
@@ -311,6 +312,20 @@ struct qdx_batch {
     // main stream every tick
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // Tick graphs.  The launch sequence of a tick depends only on the table
+    // set (its generation `gen`), which job classes are active and the longest
+    // active series; the kernels take the tick from the device counter.  Key:
+    // (gen, class flags, kmax).  The first tick of a key runs eagerly (it also
+    // raises the kernels' LDS limits), the second is captured and replayed.
+    struct GraphSlot {
+        int seen = 0;
+        int set = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::tuple<uint64_t, int, int>, GraphSlot> graphs;
+    uint64_t gen = 0;
+    bool use_graphs = true;
+    int64_t n_captured = 0, n_replayed = 0;
     size_t rsz() const { return precision == QDX_FP32_BULK ? sizeof(float) : sizeof(double); }
 };
 
@@ -433,6 +448,16 @@ void build_tables(qdx_batch *b) {
     const int nxt = 1 - b->cur;
     TableSet &ts = b->sets[nxt];
     if (ts.used) HIPCHK(hipEventSynchronize(ts.last_use));
+    // graphs that read set nxt have completed (its last use has passed); the
+    // uncaptured keys of older mixes are not looked up again
+    for (auto it = b->graphs.begin(); it != b->graphs.end();) {
+        if (it->second.exec && it->second.set != nxt) {
+            ++it;
+            continue;
+        }
+        if (it->second.exec) HIPCHK(hipGraphExecDestroy(it->second.exec));
+        it = b->graphs.erase(it);
+    }
     size_t o = 0;
     auto slot = [&](size_t bytes) {
         const size_t at = o;
@@ -463,6 +488,7 @@ void build_tables(qdx_batch *b) {
     HIPCHK(hipMemcpyAsync(ts.dev.p, ts.host, o, hipMemcpyHostToDevice, b->ctx->stream));
     b->cur = nxt;
     b->dirty = false;
+    b->gen++;
 }
 
 // tiles of the Chebyshev jobs whose series reaches order k
@@ -502,63 +528,98 @@ void enqueue_tick(qdx_batch *b) {
     auto list = [&](int i) { return reinterpret_cast<const int *>(tb + ts.off_list[i]); };
     const int *tick = b->dtick.as<int>();
     qdx_ctx *ctx = b->ctx;
-    if (cheb1d) {
-        // the 1D jobs' steps overlap the other jobs' passes
-        HIPCHK(hipEventRecord(b->fork, s));
-        HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
-        for (int i : {L_LINE256, L_LINE512}) {
-            if (b->count[i] == 0) continue;
-            KTimer t(ctx, QDX_K_BATCH_LINE);
-            BatchLaunch<R>::line(b->side, i == L_LINE256 ? 256 : 512, jobs, list(i), b->count[i], plans, tick,
-                                 b->lds[i], b->twl[i]);
-        }
-        HIPCHK(hipEventRecord(b->join, b->side));
-    }
-    if (gspo) {
-        for (int i : {L_GROW256, L_GROW512}) {
-            if (b->count[i] == 0) continue;
-            KTimer t(ctx, QDX_K_BATCH_ROW);
-            BatchLaunch<R>::gspo_row(s, i == L_GROW256 ? 256 : 512, 0, jobs, tiles(i), b->count[i], plans, tick,
+    auto body = [&] {
+        if (cheb1d) {
+            // the 1D jobs' steps overlap the other jobs' passes
+            HIPCHK(hipEventRecord(b->fork, s));
+            HIPCHK(hipStreamWaitEvent(b->side, b->fork, 0));
+            for (int i : {L_LINE256, L_LINE512}) {
+                if (b->count[i] == 0) continue;
+                KTimer t(ctx, QDX_K_BATCH_LINE);
+                BatchLaunch<R>::line(b->side, i == L_LINE256 ? 256 : 512, jobs, list(i), b->count[i], plans, tick,
                                      b->lds[i], b->twl[i]);
+            }
+            HIPCHK(hipEventRecord(b->join, b->side));
         }
-        if (b->count[L_GCOL] > 0) {
+        if (gspo) {
+            for (int i : {L_GROW256, L_GROW512}) {
+                if (b->count[i] == 0) continue;
+                KTimer t(ctx, QDX_K_BATCH_ROW);
+                BatchLaunch<R>::gspo_row(s, i == L_GROW256 ? 256 : 512, 0, jobs, tiles(i), b->count[i], plans, tick,
+                                         b->lds[i], b->twl[i]);
+            }
+            if (b->count[L_GCOL] > 0) {
+                KTimer t(ctx, QDX_K_BATCH_COL);
+                BatchLaunch<R>::gspo_col(s, jobs, tiles(L_GCOL), b->count[L_GCOL], plans, tick, b->lds[L_GCOL],
+                                         b->twl[L_GCOL]);
+            }
+            for (int i : {L_GROW256, L_GROW512}) {
+                if (b->n2d[i] == 0) continue;
+                KTimer t(ctx, QDX_K_BATCH_ROW);
+                BatchLaunch<R>::gspo_row(s, i == L_GROW256 ? 256 : 512, 1, jobs, tiles(i), b->n2d[i], plans, tick,
+                                         b->lds[i], b->twl[i]);
+            }
+        }
+        auto rows = [&](int phase, int k) {
+            for (int i : {L_ROW256, L_ROW512}) {
+                const int n = tiles_for(b->pre[i], k);
+                if (n == 0) continue;
+                KTimer t(ctx, QDX_K_BATCH_ROW);
+                BatchLaunch<R>::row(s, i == L_ROW256 ? 256 : 512, phase, k, jobs, tiles(i), n, plans, tick, b->lds[i],
+                                    b->twl[i]);
+            }
+        };
+        auto cols = [&](int k) {
+            const int n = tiles_for(b->pre[L_COL], k);
+            if (n == 0) return;
             KTimer t(ctx, QDX_K_BATCH_COL);
-            BatchLaunch<R>::gspo_col(s, jobs, tiles(L_GCOL), b->count[L_GCOL], plans, tick, b->lds[L_GCOL],
-                                     b->twl[L_GCOL]);
+            BatchLaunch<R>::col(s, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL]);
+        };
+        if (cheb2d) {
+            rows(0, 0);
+            cols(1);
+            rows(1, 1);
+            for (int k = 2; k < kmax; ++k) {
+                cols(k);
+                rows(2, k);
+            }
         }
-        for (int i : {L_GROW256, L_GROW512}) {
-            if (b->n2d[i] == 0) continue;
-            KTimer t(ctx, QDX_K_BATCH_ROW);
-            BatchLaunch<R>::gspo_row(s, i == L_GROW256 ? 256 : 512, 1, jobs, tiles(i), b->n2d[i], plans, tick,
-                                     b->lds[i], b->twl[i]);
+        if (cheb1d) HIPCHK(hipStreamWaitEvent(s, b->join, 0));
+        BatchLaunch<R>::tick_end(s, jobs, b->njobs_tab, b->dtick.as<int>());
+    };
+    if (!b->use_graphs || ctx->profiling) {
+        body();   // launches carry the profiling events
+    } else {
+        const int flags = (cheb2d ? 1 : 0) | (cheb1d ? 2 : 0) | (gspo ? 4 : 0);
+        auto &g = b->graphs[std::make_tuple(b->gen, flags, kmax)];
+        g.set = b->cur;
+        if (g.exec) {
+            HIPCHK(hipGraphLaunch(g.exec, s));
+            b->n_replayed++;
+        } else if (g.seen++ == 0) {
+            body();
+        } else {
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(s, &dead);
+                if (dead) (void)hipGraphDestroy(dead);
+                throw;
+            }
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamEndCapture(s, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(e);
+            g.exec = exec;
+            b->n_captured++;
+            HIPCHK(hipGraphLaunch(exec, s));
+            b->n_replayed++;
         }
     }
-    auto rows = [&](int phase, int k) {
-        for (int i : {L_ROW256, L_ROW512}) {
-            const int n = tiles_for(b->pre[i], k);
-            if (n == 0) continue;
-            KTimer t(ctx, QDX_K_BATCH_ROW);
-            BatchLaunch<R>::row(s, i == L_ROW256 ? 256 : 512, phase, k, jobs, tiles(i), n, plans, tick, b->lds[i],
-                                b->twl[i]);
-        }
-    };
-    auto cols = [&](int k) {
-        const int n = tiles_for(b->pre[L_COL], k);
-        if (n == 0) return;
-        KTimer t(ctx, QDX_K_BATCH_COL);
-        BatchLaunch<R>::col(s, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL]);
-    };
-    if (cheb2d) {
-        rows(0, 0);
-        cols(1);
-        rows(1, 1);
-        for (int k = 2; k < kmax; ++k) {
-            cols(k);
-            rows(2, k);
-        }
-    }
-    if (cheb1d) HIPCHK(hipStreamWaitEvent(s, b->join, 0));
-    BatchLaunch<R>::tick_end(s, jobs, b->njobs_tab, b->dtick.as<int>());
     if (!ts.last_use) HIPCHK(hipEventCreateWithFlags(&ts.last_use, hipEventDisableTiming));
     HIPCHK(hipEventRecord(ts.last_use, s));
     ts.used = true;
@@ -916,12 +977,29 @@ int qdx_batch_serve(qdx_batch *b, const qdx_job_desc *jobs, const double *arriva
     return rc;
 }
 
+int qdx_batch_set_graphs(qdx_batch *b, int32_t enable) {
+    return guarded([&] {
+        if (!b) fail(QDX_ERR_PARAM, "NULL argument");
+        b->use_graphs = enable != 0;
+    });
+}
+
+int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed) {
+    return guarded([&] {
+        if (!b) fail(QDX_ERR_PARAM, "NULL argument");
+        if (captured) *captured = b->n_captured;
+        if (replayed) *replayed = b->n_replayed;
+    });
+}
+
 int qdx_batch_destroy(qdx_batch *b) {
     return guarded([&] {
         if (!b) return;
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
         (void)hipStreamSynchronize(b->side);
+        for (auto &kv : b->graphs)
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
         for (auto &ts : b->sets) {
             if (ts.last_use) (void)hipEventDestroy(ts.last_use);
             if (ts.host) (void)hipHostFree(ts.host);

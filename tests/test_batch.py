@@ -200,6 +200,42 @@ def test_batch_gspo_fp64_matches_oracle(ctx):
     assert rel(B.fetch(jid), psi) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replay_matches_eager_ticks(ctx, precision):
+    """Ticks replayed from captured hipGraphs leave the bits that ticks
+    launched kernel by kernel leave, with host work between the replays
+    (fetches of a running job) and across changes of the job mix."""
+    import qdng_amd as Q
+    steps = 8
+    cheb = mixed_jobs(steps + 6, seed=41)
+    gsp = gspo_jobs(steps + 6, seed=42)
+
+    def drive(graphs):
+        B = Q.Batch(ctx, precision, 1 << 30)
+        B.set_graphs(graphs)
+        ids = [B.submit(p.gpu_ham(None), 1.0, steps + 6, p.psi0) for p in cheb[:5]]
+        ids += [B.submit(dt=1.0, nsteps=steps + 6, psi0=p.psi0, ops=split_ops(p, None)) for p in gsp]
+        snaps = []
+        for _ in range(4):
+            B.run(1)
+            snaps.append(B.fetch(ids[4]))
+        late = [B.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in cheb[5:]]
+        B.run(3)
+        B.release(ids[0])
+        B.run(steps + 6)
+        out = [B.fetch(j) for j in ids[1:] + late]
+        return snaps + out, B.graph_stats()
+
+    note(f"graph replay vs eager ticks, {precision}")
+    eager, g0 = drive(False)
+    graph, g1 = drive(True)
+    note(f"  graphs captured {g1[0]}, replays {g1[1]}")
+    assert g0 == (0, 0)
+    assert g1[0] >= 2 and g1[1] > g1[0]
+    for a, b in zip(eager, graph):
+        assert np.array_equal(a, b)
+
+
 def serve_mix(steps_lo=4, steps_hi=14, seed=31):
     rng = np.random.default_rng(seed)
     shapes = [[128], [64, 64], [384], [96, 64], [4096], [64, 128], [192], [128, 128]]
