@@ -117,17 +117,32 @@ B_BATCH = {"fp64": 16 * 3 + 32 + 8 + 8, "fp32": 8 * 3 + 32 + 4 + 4}
 B_BATCH_M2 = {"fp64": 16, "fp32": 8}   # the prologue reads no phi_{k-2}
 
 
-def run_batched(ctx, precision, steps, prof_ticks, seed=2026):
+def run_batched(ctx, precision, steps, prof_ticks, seed=2026, tune=False):
     """512 laser-driven jobs (256 1D, 256 2D, sizes drawn from seed 2026),
-    dt = 1; one tick = one step of every job.  Returns throughput, kernel
+    dt = 1; one tick = one step of every job.  tune: run the autotuner on
+    every 2D shape before the jobs are submitted.  Returns throughput, kernel
     timings and the final states."""
     import cases
     import qdng_amd as Q
     eager_ticks = max(1, min(20, steps))
-    total = steps + eager_ticks + prof_ticks + 2
+    total = steps + eager_ticks + prof_ticks + 2 + (13 if tune else 0)
     jobs = cases.batch_jobs(steps=total, seed=seed)
     B = Q.Batch(ctx, precision, 16 << 30)
+    tuning = {}
+    tune_s = 0.0
+    if tune:
+        t0 = time.perf_counter()
+        for n in sorted({p.N[0] for p in jobs if p.nd == 2}):
+            tuning[f"{n}x{n}"] = B.autotune(n, n)
+        tune_s = time.perf_counter() - t0
     ids = [B.submit(p.gpu_ham(None), 1.0, total, p.psi0) for p in jobs]
+    lds = None
+    if tune:
+        B.run(1)
+        t0 = time.perf_counter()
+        ms = B.autotune_lds()      # 12 ticks of the jobs themselves
+        tune_s += time.perf_counter() - t0
+        lds = {"ms_per_tick": dict(zip(["2,2", "1,2", "2,1", "1,1"], ms)), "chosen": B.lds_schedule()}
     B.run(2)                       # tables built, tick graph captured
     sync(ctx)
     h0 = np.array([B.state(j)[1] for j in ids])
@@ -159,15 +174,23 @@ def run_batched(ctx, precision, steps, prof_ticks, seed=2026):
             "row_bytes": float(alg[two_d].sum()), "line_bytes": float(alg[~two_d].sum()),
             "work_2d": float((sizes * (h1 - h0))[two_d].sum()),
             "jobs": jobs, "ids": ids, "batch": B, "points": int(sizes.sum()),
-            "graphs": graphs, "eager_ms_per_tick": eager_ms}
+            "graphs": graphs, "eager_ms_per_tick": eager_ms, "tuning": tuning, "tune_s": tune_s, "lds": lds}
+
+
+TUNING_FILES = {}   # precision -> tuning file written by the batched run (read by the serving run)
 
 
 def batched_extra(ctx, steps, prof_ticks):
+    import tempfile
     out = {}
     res = {}
     for prec in ("fp64", "fp32"):
-        r = run_batched(ctx, prec, steps, prof_ticks)
+        r = run_batched(ctx, prec, steps, prof_ticks, tune=True)
         res[prec] = r
+        fd, path = tempfile.mkstemp(prefix=f"qdx_tuning_{prec}_", suffix=".txt")
+        os.close(fd)
+        r["batch"].save_tuning(path)
+        TUNING_FILES[prec] = path
         n, ms = r["row"]
         nc, msc = r["col"]
         nl, msl = r["line"]
@@ -180,6 +203,11 @@ def batched_extra(ctx, steps, prof_ticks):
             "value_2d_jobs": r["work_2d"] / r["seconds"],
             "tick_graphs": {"captured": r["graphs"][0], "replays": r["graphs"][1],
                             "ms_per_tick_kernel_by_kernel": r["eager_ms_per_tick"]},
+            "autotune": {"seconds": r["tune_s"], "lds_schedule": r["lds"],
+                         "shapes": {k: {"radix": v["radix0"], "tiling": v["tiling"],
+                                        "row_us_default": v["row_us_default"], "row_us": v["row_us"],
+                                        "col_us_default": v["col_us_default"], "col_us": v["col_us"],
+                                        "candidates": v["candidates"]} for k, v in r["tuning"].items()}},
             "row_kernel": {"achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "launches": n,
                            "avg_launch_us": ms / max(n, 1) * 1e3, "jobs": "2D"},
             "col_kernel": {"launches": nc, "avg_launch_us": msc / max(nc, 1) * 1e3},
@@ -243,6 +271,8 @@ def serving_extra(ctx, lams, n_jobs):
         finals = {}
         for prec in ("fp32", "fp64"):
             B = Q.Batch(ctx, prec, 32 << 30)
+            if prec in TUNING_FILES:
+                B.load_tuning(TUNING_FILES[prec])   # the batched run's schedules
             t0 = time.perf_counter()
             admit, finish, nh, psis = B.serve(reqs, arrivals, want_psi=True)
             wall = time.perf_counter() - t0

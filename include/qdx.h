@@ -247,6 +247,81 @@ int qdx_batch_set_graphs(qdx_batch *b, int32_t enable);
 /* graphs captured and graph replays so far (either may be NULL) */
 int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed);
 
+/* ---- schedules and the autotuner ----------------------------------------
+ * QDng's counterpart is FFTW's measured planning and its wisdom file
+ * (src/fft/FFTGlobal.cpp:41-80: "fftwOptimize" selects FFTW_PATIENT, plans
+ * are read from "wisdom" and written to "wisdomsave"; src/fft/fft.cpp:95-110):
+ * the autotuner measures schedules on the device, the tuning file plays the
+ * part of the wisdom file. */
+
+/* Radix plan of the line transforms of length L in one precision on a
+ * context: the sequence of radices (product L; first and last 8, 4 or 2;
+ * middle 8, 4, 2 or 3; at most 16).  A plan set here is used by the
+ * propagators and Hamiltonians created and the batch jobs submitted
+ * afterwards on this context (a batch job keeps the plans it was submitted
+ * with).  Different plans give the same transform up to rounding. */
+int qdx_ctx_set_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, const int32_t *radix, int32_t nst);
+/* the plan in use for length L (radix: room for 16) */
+int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *radix, int32_t *nst);
+
+/* Tile shape of the batched kernels for 2D Chebyshev jobs of one grid
+ * shape.  Row tiles: row_lines lines of dimension 0 (N0/8 threads each) in a
+ * block of row_threads (256 or 512) threads.  Column tiles: col_width
+ * columns (N1/8 threads each, col_width * N1/8 <= 256, col_width divides
+ * N0). */
+typedef struct qdx_tiling {
+    int32_t row_threads;
+    int32_t row_lines;
+    int32_t col_width;
+} qdx_tiling;
+/* the tiling jobs of shape n0 x n1 get when submitted */
+int qdx_batch_get_tiling(qdx_batch *b, int32_t n0, int32_t n1, qdx_tiling *t);
+/* set it for jobs of that shape submitted afterwards (checked: thread and
+ * LDS limits of the kernels) */
+int qdx_batch_set_tiling(qdx_batch *b, int32_t n0, int32_t n1, const qdx_tiling *t);
+
+/* Autotuner for 2D jobs of shape n0 x n1 in the batch's precision.  Runs
+ * synthetic fixed-length jobs of that shape (about 2 M grid points) in a
+ * context and batch of its own on the same device, and times the row and
+ * column kernels with events on their dispatch packets: first the radix
+ * plans of N0 and N1 (each candidate with at most one stage more than the
+ * fewest), then the row tilings, then the column tilings.  The winners are
+ * installed: the plans on the batch's context (qdx_ctx_set_radix_plan), the
+ * tiling on the batch (qdx_batch_set_tiling), for jobs submitted afterwards.
+ * Times are per tick of the synthetic jobs, in microseconds. */
+typedef struct qdx_tune_result {
+    qdx_tiling tiling;
+    int32_t radix0[16], nst0;      /* plan of N0 */
+    int32_t radix1[16], nst1;      /* plan of N1 */
+    double row_us_default, row_us; /* row kernels: default schedule, chosen */
+    double col_us_default, col_us; /* column kernels */
+    int32_t candidates;            /* schedules timed */
+} qdx_tune_result;
+int qdx_batch_autotune(qdx_batch *b, int32_t n0, int32_t n1, qdx_tune_result *out);
+/* tune each new 2D job shape when its first job is submitted (default off) */
+int qdx_batch_set_autotune(qdx_batch *b, int32_t enable);
+/* LDS schedule of the batch's 2D Chebyshev row and column tiles: LDS
+ * buffers per line, 2 (the FFT's middle stages alternate between two
+ * buffers, one barrier per stage) or 1 (in place: half the LDS, one more
+ * barrier per middle stage).  Results are bit-identical either way.
+ * Default: rows 1 / columns 2 in FP32-bulk, rows 2 / columns 1 in FP64. */
+int qdx_batch_set_lds_schedule(qdx_batch *b, int32_t row_bufs, int32_t col_bufs);
+int qdx_batch_get_lds_schedule(qdx_batch *b, int32_t *row_bufs, int32_t *col_bufs);
+/* Time the four LDS schedules on the batch's current jobs (three ticks each;
+ * they are ordinary steps of the jobs, which advance by 12 steps) and keep
+ * the fastest.  ms_per_tick (nullable, 4 entries) receives the tick times of
+ * (rows, columns) = (2,2), (1,2), (2,1), (1,1).  QDX_ERR_STATE when no job
+ * is active, an active job has fewer than 12 steps left (every schedule is
+ * timed on the same work) or the context is profiling. */
+int qdx_batch_autotune_lds(qdx_batch *b, double *ms_per_tick);
+/* Write / read the batch's tilings, its LDS schedule and its context's
+ * radix plans of the batch's precision as text ("tile N0 N1 row_threads
+ * row_lines col_width", "lds row_bufs col_bufs", "plan L r1 r2 ..."), so a
+ * later process can skip the tuning.  A file with a malformed or invalid
+ * line is rejected whole. */
+int qdx_batch_tuning_save(qdx_batch *b, const char *path);
+int qdx_batch_tuning_load(qdx_batch *b, const char *path);
+
 /* Continuous batching over a stream of requests.  Request i arrives at
  * arrival_s[i] seconds after the call starts (non-decreasing).  Arrived
  * requests are admitted in arrival order as soon as the arena has room for

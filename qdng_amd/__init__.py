@@ -57,6 +57,9 @@ EXPORTS = [
     "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
     "qdx_batch_job_state", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
     "qdx_batch_serve", "qdx_batch_set_graphs", "qdx_batch_graph_stats", "qdx_batch_destroy",
+    "qdx_ctx_set_radix_plan", "qdx_ctx_get_radix_plan", "qdx_batch_get_tiling", "qdx_batch_set_tiling",
+    "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
+    "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
 ]
 
 K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
@@ -70,6 +73,26 @@ class JobDesc(C.Structure):
 
 class ServeResult(C.Structure):
     _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double), ("n_hpsi", C.c_int64)]
+
+
+class Tiling(C.Structure):
+    _fields_ = [("row_threads", C.c_int32), ("row_lines", C.c_int32), ("col_width", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tiling", Tiling), ("radix0", C.c_int32 * 16), ("nst0", C.c_int32),
+                ("radix1", C.c_int32 * 16), ("nst1", C.c_int32),
+                ("row_us_default", C.c_double), ("row_us", C.c_double),
+                ("col_us_default", C.c_double), ("col_us", C.c_double), ("candidates", C.c_int32)]
+
+    def as_dict(self):
+        return {"tiling": self.tiling.as_dict(), "radix0": list(self.radix0[:self.nst0]),
+                "radix1": list(self.radix1[:self.nst1]), "row_us_default": self.row_us_default,
+                "row_us": self.row_us, "col_us_default": self.col_us_default, "col_us": self.col_us,
+                "candidates": self.candidates}
 
 
 def lib():
@@ -127,6 +150,17 @@ def lib():
         "qdx_batch_usage": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "qdx_batch_serve": [vp, C.POINTER(JobDesc), dp, i32, i32, C.POINTER(dp), C.POINTER(ServeResult)],
         "qdx_batch_set_graphs": [vp, i32],
+        "qdx_ctx_set_radix_plan": [vp, i32, i32, C.POINTER(i32), i32],
+        "qdx_ctx_get_radix_plan": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)],
+        "qdx_batch_get_tiling": [vp, i32, i32, C.POINTER(Tiling)],
+        "qdx_batch_set_tiling": [vp, i32, i32, C.POINTER(Tiling)],
+        "qdx_batch_autotune": [vp, i32, i32, C.POINTER(TuneResult)],
+        "qdx_batch_set_autotune": [vp, i32],
+        "qdx_batch_set_lds_schedule": [vp, i32, i32],
+        "qdx_batch_get_lds_schedule": [vp, C.POINTER(i32), C.POINTER(i32)],
+        "qdx_batch_autotune_lds": [vp, dp],
+        "qdx_batch_tuning_save": [vp, C.c_char_p],
+        "qdx_batch_tuning_load": [vp, C.c_char_p],
         "qdx_batch_graph_stats": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "qdx_batch_destroy": [vp],
     }
@@ -176,6 +210,17 @@ class Context:
     def profile(self, on: bool = True) -> None:
         """Start (clearing the counters) or stop per-launch HIP-event timing."""
         _chk(lib().qdx_ctx_profile(self.h, int(bool(on))))
+
+    def set_radix_plan(self, L: int, radix, precision="fp64") -> None:
+        """radix sequence of the length-L line transforms (qdx_ctx_set_radix_plan)"""
+        r = (C.c_int32 * len(radix))(*radix)
+        _chk(lib().qdx_ctx_set_radix_plan(self.h, _PREC[precision], int(L), r, len(radix)))
+
+    def radix_plan(self, L: int, precision="fp64"):
+        r = (C.c_int32 * 16)()
+        n = C.c_int32()
+        _chk(lib().qdx_ctx_get_radix_plan(self.h, _PREC[precision], int(L), r, C.byref(n)))
+        return list(r[:n.value])
 
     def kernel_time(self, kernel_class: int):
         """(launches, total milliseconds) of one kernel class since profile(True)."""
@@ -496,6 +541,49 @@ class Batch:
         n, b, t = C.c_int32(), C.c_int64(), C.c_int64()
         _chk(lib().qdx_batch_usage(self.h, C.byref(n), C.byref(b), C.byref(t)))
         return n.value, b.value, t.value
+
+    def tiling(self, n0: int, n1: int) -> dict:
+        """tile shape and LDS schedule that 2D jobs of shape n0 x n1 get"""
+        t = Tiling()
+        _chk(lib().qdx_batch_get_tiling(self.h, int(n0), int(n1), C.byref(t)))
+        return t.as_dict()
+
+    def set_tiling(self, n0: int, n1: int, row_threads: int, row_lines: int, col_width: int) -> None:
+        t = Tiling(row_threads, row_lines, col_width)
+        _chk(lib().qdx_batch_set_tiling(self.h, int(n0), int(n1), C.byref(t)))
+
+    def lds_schedule(self):
+        """(row, column) LDS buffers per line: 2 ping-pong, 1 in place"""
+        r, c = C.c_int32(), C.c_int32()
+        _chk(lib().qdx_batch_get_lds_schedule(self.h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    def set_lds_schedule(self, row_bufs: int, col_bufs: int) -> None:
+        _chk(lib().qdx_batch_set_lds_schedule(self.h, int(row_bufs), int(col_bufs)))
+
+    def autotune_lds(self):
+        """time the four LDS schedules on the current jobs (they advance 12
+        steps) and keep the fastest; returns ms per tick of (2,2), (1,2),
+        (2,1), (1,1)"""
+        ms = (C.c_double * 4)()
+        _chk(lib().qdx_batch_autotune_lds(self.h, ms))
+        return list(ms)
+
+    def autotune(self, n0: int, n1: int) -> dict:
+        """time the schedules of 2D jobs of shape n0 x n1 on the device and
+        install the fastest (qdx_batch_autotune)"""
+        r = TuneResult()
+        _chk(lib().qdx_batch_autotune(self.h, int(n0), int(n1), C.byref(r)))
+        return r.as_dict()
+
+    def set_autotune(self, enable: bool) -> None:
+        _chk(lib().qdx_batch_set_autotune(self.h, 1 if enable else 0))
+
+    def save_tuning(self, path: str) -> None:
+        _chk(lib().qdx_batch_tuning_save(self.h, os.fsencode(path)))
+
+    def load_tuning(self, path: str) -> None:
+        _chk(lib().qdx_batch_tuning_load(self.h, os.fsencode(path)))
 
     def set_graphs(self, enable: bool) -> None:
         """replay ticks from captured hipGraphs (default) or launch every

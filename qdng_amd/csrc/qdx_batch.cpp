@@ -87,6 +87,7 @@ struct BatchJob {
     JobLayout lay{};
     JobTiling tile;
     bool has_V = false, has_mu = false, has_T = false, has_V11 = false, has_V01 = false, has_T1 = false;
+    std::vector<int> rad0, rad1;   // radix plans of N0 / N1, fixed at submission
     double lscale = 1.0;
     int nt = 0;
     int64_t last_tick() const { return start_tick + nsteps - 1; }
@@ -211,7 +212,8 @@ JobLayout job_layout(const BatchJob &J, size_t rsz) {
         l.A = take(2 * rsz * n);
         l.B = take(2 * rsz * n);
         l.coef = take(16 * (size_t)J.cp.order);
-        l.part = take(8 * 3 * (size_t)J.tile.nrt);
+        // room for the partials of one-line row tiles, so any tiling fits
+        l.part = take(8 * 3 * (size_t)std::max(J.tile.nrt, J.grid.nd == 2 ? J.grid.N[1] : 1));
     } else {
         l.psi = take(2 * rsz * n * S);
     }
@@ -236,13 +238,14 @@ JobLayout job_layout(const BatchJob &J, size_t rsz) {
 // LDS of the tiles: padded line buffers (L + L/8 entries per line) and the
 // padded twiddle copy (L + L/8 entries) when it is staged
 size_t lp(int L) { return (size_t)L + L / 8; }
-size_t row_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw) {
+// lb: LDS buffers per line (2 ping-pong, 1 in place)
+size_t row_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw, int lb = 2) {
     const int L = g.N[0];
-    return (2 * t.rb * lp(L) + (tw ? lp(L) : 0)) * esz;
+    return (lb * t.rb * lp(L) + (tw ? lp(L) : 0)) * esz;
 }
-size_t col_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw) {
+size_t col_lds(const Grid &g, const JobTiling &t, size_t esz, bool tw, int lb = 2) {
     const int L = g.N[1];
-    return (2 * t.W * lp(L) + (tw ? lp(L) : 0)) * esz;
+    return (lb * t.W * lp(L) + (tw ? lp(L) : 0)) * esz;
 }
 size_t line_lds(const Grid &g, size_t esz, bool tw) {
     const int L = g.N[0];
@@ -326,6 +329,14 @@ struct qdx_batch {
     uint64_t gen = 0;
     bool use_graphs = true;
     int64_t n_captured = 0, n_replayed = 0;
+    // tilings of 2D Chebyshev job shapes (N0, N1), set or tuned
+    std::map<std::pair<int, int>, JobTiling> tilings;
+    bool autotune = false;
+    // LDS buffers per line of the Chebyshev row / column tiles of all 2D jobs
+    // (2: ping-pong, 1: in place; bit-identical results).  Default: in place
+    // where it raises the kernel's register-limited occupancy (FP32 rows
+    // 3 -> 4 waves per SIMD, FP64 columns 2 -> 3).
+    int lbr = 2, lbc = 2;
     size_t rsz() const { return precision == QDX_FP32_BULK ? sizeof(float) : sizeof(double); }
 };
 
@@ -338,12 +349,12 @@ void build_tables(qdx_batch *b) {
     std::vector<BJob<R>> tj;
     TileList L[L_COUNT];
     std::vector<LinePlan> plans;
-    std::map<int, int> pidx;
-    auto plan_of = [&](int len) {
-        auto it = pidx.find(len);
+    std::map<std::pair<int, std::vector<int>>, int> pidx;
+    auto plan_of = [&](int len, const std::vector<int> &rad) {
+        auto it = pidx.find({len, rad});
         if (it != pidx.end()) return it->second;
-        plans.push_back(b->ctx->plan(len, fp32));
-        pidx[len] = (int)plans.size() - 1;
+        plans.push_back(b->ctx->plan(len, fp32, rad));
+        pidx[{len, rad}] = (int)plans.size() - 1;
         return (int)plans.size() - 1;
     };
     // Chebyshev jobs by decreasing series length (prefix launches per order),
@@ -367,8 +378,8 @@ void build_tables(qdx_batch *b) {
         d.nd = J.grid.nd;
         d.N0 = J.grid.N[0];
         d.N1 = J.grid.nd == 2 ? J.grid.N[1] : 1;
-        d.plan0 = plan_of(d.N0);
-        d.plan1 = J.grid.nd == 2 ? plan_of(d.N1) : 0;
+        d.plan0 = plan_of(d.N0, J.rad0);
+        d.plan1 = J.grid.nd == 2 ? plan_of(d.N1, J.rad1) : 0;
         d.rb = J.tile.rb;
         d.nrt = J.tile.nrt;
         d.W = J.tile.W;
@@ -412,11 +423,11 @@ void build_tables(qdx_batch *b) {
                 d.rt0 = (int)l.tiles.size();
                 for (int r = 0; r < J.tile.nrt; ++r) l.tiles.push_back(BTile{ji, r * J.tile.rb});
                 l.pre.push_back({J.cp.order, (int)l.tiles.size()});
-                l.lds.add(row_lds(J.grid, J.tile, esz, true), row_lds(J.grid, J.tile, esz, false));
+                l.lds.add(row_lds(J.grid, J.tile, esz, true, b->lbr), row_lds(J.grid, J.tile, esz, false, b->lbr));
                 TileList &c = L[L_COL];
                 for (int k = 0; k < J.tile.nct; ++k) c.tiles.push_back(BTile{ji, k * J.tile.W});
                 c.pre.push_back({J.cp.order, (int)c.tiles.size()});
-                c.lds.add(col_lds(J.grid, J.tile, esz, true), col_lds(J.grid, J.tile, esz, false));
+                c.lds.add(col_lds(J.grid, J.tile, esz, true, b->lbc), col_lds(J.grid, J.tile, esz, false, b->lbc));
             }
         } else {
             d.psiR = reinterpret_cast<R *>(at(J.lay.psi));
@@ -565,15 +576,15 @@ void enqueue_tick(qdx_batch *b) {
                 const int n = tiles_for(b->pre[i], k);
                 if (n == 0) continue;
                 KTimer t(ctx, QDX_K_BATCH_ROW);
-                BatchLaunch<R>::row(s, i == L_ROW256 ? 256 : 512, phase, k, jobs, tiles(i), n, plans, tick, b->lds[i],
-                                    b->twl[i]);
+                BatchLaunch<R>::row(s, i == L_ROW256 ? 256 : 512, b->lbr, phase, k, jobs, tiles(i), n, plans, tick,
+                                    b->lds[i], b->twl[i]);
             }
         };
         auto cols = [&](int k) {
             const int n = tiles_for(b->pre[L_COL], k);
             if (n == 0) return;
             KTimer t(ctx, QDX_K_BATCH_COL);
-            BatchLaunch<R>::col(s, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL]);
+            BatchLaunch<R>::col(s, b->lbc, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL]);
         };
         if (cheb2d) {
             rows(0, 0);
@@ -673,7 +684,14 @@ BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
         check_grid(J.grid);
         host = gspo_host(T, W, d->nsteps);
     }
+    const bool fp32 = b->precision == QDX_FP32_BULK;
+    J.rad0 = b->ctx->radix_of(J.grid.N[0], fp32);
+    if (J.grid.nd == 2) J.rad1 = b->ctx->radix_of(J.grid.N[1], fp32);
     J.tile = job_tiling(J.grid);
+    if (J.kind == K_CHEBY && J.grid.nd == 2) {
+        auto it = b->tilings.find({J.grid.N[0], J.grid.N[1]});
+        if (it != b->tilings.end()) J.tile = it->second;
+    }
     J.has_V = !host.V.empty();
     J.has_mu = !host.mu.empty();
     J.has_V11 = !host.V11.empty();
@@ -735,8 +753,13 @@ void upload_job(qdx_batch *b, BatchJob &J, const JobHost &host, const double *ps
     put(J.lay.stats, z, sizeof(z));
 }
 
+void tune_shape(qdx_batch *b, int n0, int n1, qdx_tune_result *res);
+
 // admit a job; false when it does not fit into the free part of the arena
 bool try_submit(qdx_batch *b, const qdx_job_desc *d, int64_t &id) {
+    if (b->autotune && d && d->nops == 0 && d->ham && d->ham->grid.nd == 2 &&
+        !b->tilings.count({d->ham->grid.N[0], d->ham->grid.N[1]}))
+        tune_shape(b, d->ham->grid.N[0], d->ham->grid.N[1], nullptr);
     JobHost host;
     BatchJob J = plan_job(b, d, host);
     if (!b->arena.alloc(J.lay.bytes, J.off)) return false;
@@ -786,6 +809,307 @@ bool any_steps_left(const qdx_batch *b) {
     return false;
 }
 
+// ---------------------------------------------------------------------------
+// Schedules and the autotuner
+
+Grid shape_grid(int n0, int n1) {
+    Grid g;
+    g.nd = 2;
+    g.N[0] = n0;
+    g.N[1] = n1;
+    check_grid(g);
+    radix_plan(n0);   // N = 8 * 2^a * 3^b
+    radix_plan(n1);
+    if (n0 > kMaxLine || n1 > kMaxLine) fail(QDX_ERR_PARAM, "batched lines hold at most 4096 points");
+    return g;
+}
+
+// a tiling from its public description, checked against the kernels' limits
+JobTiling tiling_from(const Grid &g, const qdx_tiling &q, size_t esz) {
+    const int TL = g.N[0] / 8, TL1 = g.N[1] / 8;
+    if (q.row_threads != 256 && q.row_threads != 512) fail(QDX_ERR_PARAM, "row tiles use 256 or 512 threads");
+    if (q.row_lines < 1 || q.row_lines * TL > q.row_threads || q.row_lines > g.N[1])
+        fail(QDX_ERR_PARAM, "row_lines * N0/8 must not exceed row_threads (and row_lines <= N1)");
+    if (q.col_width < 1 || q.col_width * TL1 > 256 || g.N[0] % q.col_width != 0)
+        fail(QDX_ERR_PARAM, "col_width * N1/8 must not exceed 256 and col_width must divide N0");
+    JobTiling t;
+    t.NT = q.row_threads;
+    t.rb = q.row_lines;
+    t.nrt = (g.N[1] + t.rb - 1) / t.rb;
+    t.W = q.col_width;
+    t.nct = g.N[0] / t.W;
+    if (row_lds(g, t, esz, false) > kLds || col_lds(g, t, esz, false) > kLds)
+        fail(QDX_ERR_PARAM, "tile exceeds the 160 KiB LDS of a gfx950 CU");
+    return t;
+}
+
+qdx_tiling tiling_to(const JobTiling &t) {
+    qdx_tiling q;
+    q.row_threads = t.NT;
+    q.row_lines = t.rb;
+    q.col_width = t.W;
+    return q;
+}
+
+void rc_check(int rc) {
+    if (rc != QDX_OK) fail(rc, qdx_last_error());
+}
+
+// K synthetic 2D jobs of one shape in a context and batch of their own, for
+// timing: kinetic term (mass 2000 per dimension), harmonic potential, dipole
+// x exp(-x^2/8) in a sine field, Gaussian psi0, a fixed 16-term series
+// without the break test (every tick runs the same orders), about 2 M grid
+// points in all.
+struct TuneBench {
+    static constexpr int kTicks = 4, kSteps = 1000;
+    qdx_ctx *tc = nullptr;
+    qdx_batch *tb = nullptr;
+    qdx_ham *h = nullptr;
+    int K = 1;
+    TuneBench(int device, int precision, const Grid &g) {
+        rc_check(qdx_ctx_create(device, &tc));
+        qdx_grid q{};
+        q.ndims = 2;
+        for (int d = 0; d < 2; ++d) {
+            q.N[d] = g.N[d];
+            q.xmin[d] = -10.0;
+            q.xmax[d] = 10.0;
+        }
+        rc_check(qdx_ham_create(nullptr, &q, 1, 0, 1, &h));
+        const double mass[2] = {2000.0, 2000.0};
+        rc_check(qdx_ham_add_kinetic(h, 0, mass));
+        const long n = g.size();
+        std::vector<double> V(n), mu(n), psi(2 * n), laser(kSteps);
+        const double dx0 = 20.0 / (g.N[0] - 1), dx1 = 20.0 / (g.N[1] - 1);
+        for (int j = 0; j < g.N[1]; ++j)
+            for (int i = 0; i < g.N[0]; ++i) {
+                const double x = -10.0 + i * dx0, y = -10.0 + j * dx1;
+                const long p = (long)j * g.N[0] + i;
+                V[p] = 0.02 * (x * x + y * y);
+                mu[p] = x * std::exp(-x * x / 8);
+                psi[2 * p] = std::exp(-(x - 1) * (x - 1) - y * y);
+                psi[2 * p + 1] = 0.0;
+            }
+        for (int t = 0; t < kSteps; ++t) laser[t] = 0.01 * std::sin(0.05 * t);
+        rc_check(qdx_ham_add_potential(h, 0, 0, V.data()));
+        rc_check(qdx_ham_add_dipole(h, 0, 0, mu.data(), laser.data(), kSteps, 1.0));
+        double emin = 0, emax = 0;
+        rc_check(qdx_ham_bounds(h, &emin, &emax));
+        qdx_job_desc d{};
+        d.ham = h;
+        d.nsteps = kSteps;
+        d.dt = 8.0 / (0.5 * (emax - emin));   // R dt = 8 < 16 terms
+        d.order = 16;
+        d.prec = 0.0;
+        d.psi0 = psi.data();
+        K = (int)std::max<long>(1, std::min<long>(512, (2L << 20) / n));
+        int64_t fp = 0;
+        rc_check(qdx_batch_create(tc, precision, 1 << 20, &tb));
+        rc_check(qdx_batch_footprint(tb, &d, &fp));
+        rc_check(qdx_batch_destroy(tb));
+        tb = nullptr;
+        rc_check(qdx_batch_create(tc, precision, (int64_t)K * (fp + 4096), &tb));
+        for (int k = 0; k < K; ++k) {
+            int64_t id = 0;
+            rc_check(qdx_batch_submit(tb, &d, &id));
+        }
+    }
+    ~TuneBench() {
+        if (tb) qdx_batch_destroy(tb);
+        if (h) qdx_ham_destroy(h);
+        if (tc) qdx_ctx_destroy(tc);
+    }
+    void set(const std::vector<int> &r0, const std::vector<int> &r1, const JobTiling &t) {
+        for (auto &kv : tb->jobs) {
+            kv.second.rad0 = r0;
+            kv.second.rad1 = r1;
+            kv.second.tile = t;
+        }
+        tb->dirty = true;
+    }
+    // row and column kernel time per tick (us) of the current schedule
+    std::pair<double, double> measure() {
+        rc_check(qdx_batch_run(tb, 1));   // tables, kernel LDS limits, caches
+        rc_check(qdx_ctx_synchronize(tc));
+        rc_check(qdx_ctx_profile(tc, 1));
+        rc_check(qdx_batch_run(tb, kTicks));
+        int64_t n = 0;
+        double row = 0, col = 0;
+        rc_check(qdx_ctx_kernel_time(tc, QDX_K_BATCH_ROW, &n, &row));
+        rc_check(qdx_ctx_kernel_time(tc, QDX_K_BATCH_COL, &n, &col));
+        rc_check(qdx_ctx_profile(tc, 0));
+        return {row * 1e3 / kTicks, col * 1e3 / kTicks};
+    }
+};
+
+// radix plans worth timing for length L: every plan with the fewest stages
+// (at most 12) and four with one stage more
+std::vector<std::vector<int>> plan_choices(int L) {
+    std::vector<std::vector<int>> out, more;
+    const size_t nmin = radix_plan(L).size();
+    for (auto &r : radix_candidates(L, 1)) {
+        if (r.size() == nmin) {
+            if (out.size() < 12) out.push_back(r);
+        } else if (more.size() < 4) {
+            more.push_back(r);
+        }
+    }
+    out.insert(out.end(), more.begin(), more.end());
+    return out;
+}
+
+void tune_shape(qdx_batch *b, int n0, int n1, qdx_tune_result *res) {
+    const Grid g = shape_grid(n0, n1);
+    const size_t esz = 2 * b->rsz();
+    TuneBench tb(b->ctx->device, b->precision, g);
+    tb.tb->lbr = b->lbr;   // the batch's LDS schedule
+    tb.tb->lbc = b->lbc;
+    // a candidate replaces the best only when it is 2 % faster (timing noise)
+    auto better = [](double t, double best) { return t < 0.98 * best; };
+    std::vector<int> r0 = radix_plan(n0), r1 = radix_plan(n1);
+    JobTiling bt = job_tiling(g);
+    int ncand = 1;
+    tb.set(r0, r1, bt);
+    const auto base = tb.measure();
+    double brow = base.first, bcol = base.second;
+    // radix plans (a square grid shares one plan between rows and columns)
+    if (n0 == n1) {
+        for (auto &c : plan_choices(n0)) {
+            if (c == r0) continue;
+            tb.set(c, c, bt);
+            const auto m = tb.measure();
+            ++ncand;
+            if (better(m.first + m.second, brow + bcol)) {
+                r0 = r1 = c;
+                brow = m.first;
+                bcol = m.second;
+            }
+        }
+    } else {
+        for (auto &c : plan_choices(n0)) {
+            if (c == r0) continue;
+            tb.set(c, r1, bt);
+            const auto m = tb.measure();
+            ++ncand;
+            if (better(m.first, brow)) {
+                r0 = c;
+                brow = m.first;
+            }
+        }
+        for (auto &c : plan_choices(n1)) {
+            if (c == r1) continue;
+            tb.set(r0, c, bt);
+            const auto m = tb.measure();
+            ++ncand;
+            if (better(m.second, bcol)) {
+                r1 = c;
+                bcol = m.second;
+            }
+        }
+    }
+    // row tiles: full blocks of 256 or 512 threads
+    const int TL = n0 / 8, TL1 = n1 / 8;
+    const JobTiling row0 = bt;
+    for (int NT : {256, 512}) {
+        if (TL > NT) continue;
+        JobTiling t = row0;
+        t.NT = NT;
+        t.rb = std::min(n1, NT / TL);
+        t.nrt = (n1 + t.rb - 1) / t.rb;
+        if (t.NT == row0.NT && t.rb == row0.rb) continue;
+        if (row_lds(g, t, esz, false) > kLds) continue;
+        tb.set(r0, r1, t);
+        const auto m = tb.measure();
+        ++ncand;
+        if (better(m.first, brow)) {
+            bt.NT = t.NT;
+            bt.rb = t.rb;
+            bt.nrt = t.nrt;
+            brow = m.first;
+        }
+    }
+    // column tiles: width 1 .. 16
+    const JobTiling col0 = bt;
+    for (int W = 1; W <= 16; W *= 2) {
+        if (W * TL1 > 256 || n0 % W != 0) continue;
+        JobTiling t = col0;
+        t.W = W;
+        t.nct = n0 / W;
+        if (t.W == col0.W) continue;
+        if (col_lds(g, t, esz, false) > kLds) continue;
+        tb.set(r0, r1, t);
+        const auto m = tb.measure();
+        ++ncand;
+        if (better(m.second, bcol)) {
+            bt.W = t.W;
+            bt.nct = t.nct;
+            bcol = m.second;
+        }
+    }
+    const bool fp32 = b->precision == QDX_FP32_BULK;
+    b->ctx->radix[{n0, fp32}] = r0;
+    b->ctx->radix[{n1, fp32}] = r1;
+    b->tilings[{n0, n1}] = bt;
+    if (res) {
+        *res = qdx_tune_result{};
+        res->tiling = tiling_to(bt);
+        res->nst0 = (int32_t)r0.size();
+        res->nst1 = (int32_t)r1.size();
+        for (size_t i = 0; i < r0.size(); ++i) res->radix0[i] = r0[i];
+        for (size_t i = 0; i < r1.size(); ++i) res->radix1[i] = r1[i];
+        res->row_us_default = base.first;
+        res->col_us_default = base.second;
+        res->row_us = brow;
+        res->col_us = bcol;
+        res->candidates = ncand;
+    }
+}
+
+// LDS schedule of the current mix: every (row, column) buffer choice timed on
+// the batch's own jobs (the choice does not change results, so the ticks are
+// ordinary steps of the jobs).  Returns the ms per tick of (2,2), (1,2),
+// (2,1), (1,1) in ms[4].
+void tune_lds(qdx_batch *b, double ms[4]) {
+    // every active job runs through all 12 ticks, so each schedule is timed
+    // on the same work
+    if (!any_steps_left(b)) fail(QDX_ERR_STATE, "no active jobs to time the LDS schedule on");
+    for (auto &kv : b->jobs) {
+        const BatchJob &J = kv.second;
+        if (J.active_at(b->ticks) && J.start_tick + J.nsteps - b->ticks < 12)
+            fail(QDX_ERR_STATE, "an active job has fewer than 12 steps left");
+    }
+    qdx_ctx *ctx = b->ctx;
+    if (ctx->profiling) fail(QDX_ERR_STATE, "LDS schedule tuning needs the context's profiling off");
+    const int modes[4][2] = {{2, 2}, {1, 2}, {2, 1}, {1, 1}};
+    double best = 0;
+    int bi = -1;
+    // kernel by kernel: a graph capture would fall into the timed ticks
+    struct GraphsOff {
+        qdx_batch *b;
+        bool was;
+        ~GraphsOff() { b->use_graphs = was; }
+    } off{b, b->use_graphs};
+    b->use_graphs = false;
+    for (int m = 0; m < 4; ++m) {
+        b->lbr = modes[m][0];
+        b->lbc = modes[m][1];
+        b->dirty = true;
+        tick(b);   // tables of this schedule
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < 2; ++k) tick(b);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        ms[m] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / 2;
+        if (bi < 0 || ms[m] < best) {
+            best = ms[m];
+            bi = m;
+        }
+    }
+    b->lbr = modes[bi][0];
+    b->lbc = modes[bi][1];
+    b->dirty = true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -799,6 +1123,8 @@ int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_ba
         auto b = std::make_unique<qdx_batch>();
         b->ctx = ctx;
         b->precision = precision;
+        b->lbr = precision == QDX_FP32_BULK ? 1 : 2;
+        b->lbc = precision == QDX_FP32_BULK ? 2 : 1;
         b->arena.init((size_t)hbm_budget);
         b->dtick.alloc(sizeof(int));
         HIPCHK(hipMemset(b->dtick.p, 0, sizeof(int)));
@@ -989,6 +1315,148 @@ int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed) {
         if (!b) fail(QDX_ERR_PARAM, "NULL argument");
         if (captured) *captured = b->n_captured;
         if (replayed) *replayed = b->n_replayed;
+    });
+}
+
+int qdx_batch_get_tiling(qdx_batch *b, int32_t n0, int32_t n1, qdx_tiling *t) {
+    return guarded([&] {
+        if (!b || !t) fail(QDX_ERR_PARAM, "NULL argument");
+        const Grid g = shape_grid(n0, n1);
+        auto it = b->tilings.find({n0, n1});
+        *t = tiling_to(it != b->tilings.end() ? it->second : job_tiling(g));
+    });
+}
+
+int qdx_batch_set_tiling(qdx_batch *b, int32_t n0, int32_t n1, const qdx_tiling *t) {
+    return guarded([&] {
+        if (!b || !t) fail(QDX_ERR_PARAM, "NULL argument");
+        const Grid g = shape_grid(n0, n1);
+        b->tilings[{n0, n1}] = tiling_from(g, *t, 2 * b->rsz());
+    });
+}
+
+int qdx_batch_autotune(qdx_batch *b, int32_t n0, int32_t n1, qdx_tune_result *out) {
+    return guarded([&] {
+        if (!b) fail(QDX_ERR_PARAM, "NULL argument");
+        HIPCHK(hipSetDevice(b->ctx->device));
+        tune_shape(b, n0, n1, out);
+    });
+}
+
+int qdx_batch_set_lds_schedule(qdx_batch *b, int32_t row_bufs, int32_t col_bufs) {
+    return guarded([&] {
+        if (!b) fail(QDX_ERR_PARAM, "NULL argument");
+        if ((row_bufs != 1 && row_bufs != 2) || (col_bufs != 1 && col_bufs != 2))
+            fail(QDX_ERR_PARAM, "LDS buffers per line are 1 or 2");
+        b->lbr = row_bufs;
+        b->lbc = col_bufs;
+        b->dirty = true;
+    });
+}
+
+int qdx_batch_get_lds_schedule(qdx_batch *b, int32_t *row_bufs, int32_t *col_bufs) {
+    return guarded([&] {
+        if (!b || !row_bufs || !col_bufs) fail(QDX_ERR_PARAM, "NULL argument");
+        *row_bufs = b->lbr;
+        *col_bufs = b->lbc;
+    });
+}
+
+int qdx_batch_autotune_lds(qdx_batch *b, double *ms_per_tick) {
+    return guarded([&] {
+        if (!b) fail(QDX_ERR_PARAM, "NULL argument");
+        HIPCHK(hipSetDevice(b->ctx->device));
+        double ms[4] = {0, 0, 0, 0};
+        tune_lds(b, ms);
+        if (ms_per_tick)
+            for (int i = 0; i < 4; ++i) ms_per_tick[i] = ms[i];
+    });
+}
+
+int qdx_batch_set_autotune(qdx_batch *b, int32_t enable) {
+    return guarded([&] {
+        if (!b) fail(QDX_ERR_PARAM, "NULL argument");
+        b->autotune = enable != 0;
+    });
+}
+
+int qdx_batch_tuning_save(qdx_batch *b, const char *path) {
+    return guarded([&] {
+        if (!b || !path) fail(QDX_ERR_PARAM, "NULL argument");
+        FILE *f = std::fopen(path, "w");
+        if (!f) fail(QDX_ERR_PARAM, std::string("cannot write ") + path);
+        const bool fp32 = b->precision == QDX_FP32_BULK;
+        for (auto &kv : b->ctx->radix) {
+            if (kv.first.second != fp32) continue;
+            std::fprintf(f, "plan %d", kv.first.first);
+            for (int r : kv.second) std::fprintf(f, " %d", r);
+            std::fprintf(f, "\n");
+        }
+        for (auto &kv : b->tilings) {
+            const JobTiling &t = kv.second;
+            std::fprintf(f, "tile %d %d %d %d %d\n", kv.first.first, kv.first.second, t.NT, t.rb, t.W);
+        }
+        std::fprintf(f, "lds %d %d\n", b->lbr, b->lbc);
+        const bool ok = std::fclose(f) == 0;
+        if (!ok) fail(QDX_ERR_PARAM, std::string("cannot write ") + path);
+    });
+}
+
+int qdx_batch_tuning_load(qdx_batch *b, const char *path) {
+    return guarded([&] {
+        if (!b || !path) fail(QDX_ERR_PARAM, "NULL argument");
+        FILE *f = std::fopen(path, "r");
+        if (!f) fail(QDX_ERR_PARAM, std::string("cannot read ") + path);
+        std::map<int, std::vector<int>> plans;
+        std::map<std::pair<int, int>, JobTiling> tiles;
+        int lds_r = 0, lds_c = 0;
+        char line[512];
+        std::string bad;
+        while (bad.empty() && std::fgets(line, sizeof line, f)) {
+            char kind[8] = {};
+            int used = 0;
+            if (std::sscanf(line, "%7s%n", kind, &used) != 1) continue;   // blank line
+            const char *rest = line + used;
+            if (std::strcmp(kind, "plan") == 0) {
+                int L = 0, r = 0, n = 0;
+                std::vector<int> seq;
+                if (std::sscanf(rest, "%d%n", &L, &n) != 1) bad = line;
+                rest += n;
+                while (bad.empty() && std::sscanf(rest, "%d%n", &r, &n) == 1) {
+                    seq.push_back(r);
+                    rest += n;
+                }
+                if (bad.empty() && !radix_ok(L, seq)) bad = line;
+                plans[L] = seq;
+            } else if (std::strcmp(kind, "lds") == 0) {
+                if (std::sscanf(rest, "%d %d", &lds_r, &lds_c) != 2 || (lds_r != 1 && lds_r != 2) ||
+                    (lds_c != 1 && lds_c != 2))
+                    bad = line;
+            } else if (std::strcmp(kind, "tile") == 0) {
+                int n0, n1;
+                qdx_tiling q;
+                if (std::sscanf(rest, "%d %d %d %d %d", &n0, &n1, &q.row_threads, &q.row_lines, &q.col_width) != 5) {
+                    bad = line;
+                } else {
+                    try {
+                        tiles[{n0, n1}] = tiling_from(shape_grid(n0, n1), q, 2 * b->rsz());
+                    } catch (const QdxError &) {
+                        bad = line;
+                    }
+                }
+            } else {
+                bad = line;
+            }
+        }
+        std::fclose(f);
+        if (!bad.empty()) fail(QDX_ERR_PARAM, std::string("tuning file: bad line: ") + bad);
+        for (auto &kv : plans) b->ctx->radix[{kv.first, b->precision == QDX_FP32_BULK}] = kv.second;
+        for (auto &kv : tiles) b->tilings[kv.first] = kv.second;
+        if (lds_r) {
+            b->lbr = lds_r;
+            b->lbc = lds_c;
+            b->dirty = true;
+        }
     });
 }
 

@@ -21,8 +21,9 @@
 
 namespace qdx {
 
-// geometry of one row tile
-template <class R>
+// geometry of one row tile; LB LDS buffers per line (2: ping-pong, 1: the
+// FFT's middle stages in place)
+template <class R, int LB>
 struct BRowTile {
     int L, TL, line, lt;
     long row;
@@ -39,23 +40,24 @@ struct BRowTile {
         const int lp = L + (L >> 3);
         const int l = active ? line : 0;
         ln.a = smem + (long)l * lp;
-        ln.b = smem + (long)(J.rb + l) * lp;
+        ln.b = LB == 2 ? smem + (long)(J.rb + l) * lp : ln.a;
         ln.es = 1;
         ln.pad = true;
-        tw = smem + (long)2 * J.rb * lp;
+        tw = smem + (long)LB * J.rb * lp;
     }
     __device__ long gidx(int q) const { return row * L + lt + q * TL; }
 };
 
-template <class R>
-__device__ __forceinline__ void bfft(cx<R> (&v)[8], const BRowTile<R> &B, const LinePlan &p, bool twl, int sign) {
-    if (twl) fft8<R>(v, B.ln, p, B.tw, sign, B.lt, B.active, kTwLds);
-    else fft8<R>(v, B.ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, B.lt, B.active, kTwMem);
+template <class R, int LB>
+__device__ __forceinline__ void bfft(cx<R> (&v)[8], const BRowTile<R, LB> &B, const LinePlan &p, bool twl,
+                                     int sign) {
+    if (twl) fft8<R, LB == 1>(v, B.ln, p, B.tw, sign, B.lt, B.active, kTwLds);
+    else fft8<R, LB == 1>(v, B.ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, B.lt, B.active, kTwMem);
 }
 
 // PHASE 0: init (phi_0 = psi, sum |psi|^2 of the step start, X = FFT_0 phi_0)
 // PHASE 1: prologue (order 1); PHASE 2: order k >= 2
-template <class R, int NT, int PHASE>
+template <class R, int NT, int PHASE, int LB>
 __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
                                              const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -66,7 +68,7 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
     if (PHASE == 2 && (k >= J.order || J.ctl->done)) return;
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ double red[16];
-    BRowTile<R> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
+    BRowTile<R, LB> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
     const LinePlan &p0 = plans[J.plan0];
     const int local = b - J.rt0;   // tile index within the job
     cx<double> *psi = reinterpret_cast<cx<double> *>(J.psi);
@@ -219,8 +221,8 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
 }
 
 // dimension-1 part of the kinetic operator of a 2D job:
-// Y = IFFT_1(T_1/N_1 FFT_1 phi_{k-1}), held in X
-template <class R>
+// Y = IFFT_1(T_1/N_1 FFT_1 phi_{k-1}), held in X; LB LDS buffers per line
+template <class R, int LB>
 __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
                                               const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -242,10 +244,10 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     const cx<R> *in = reinterpret_cast<const cx<R> *>((k - 1) % 2 == 0 ? J.phiA : J.phiB) + col;
     LdsLine<R> ln;
     ln.a = smem + c;
-    ln.b = smem + (long)W * tw_lds_len(L) + c;
+    ln.b = LB == 2 ? smem + (long)W * tw_lds_len(L) + c : ln.a;
     ln.es = W;
     ln.pad = true;
-    cx<R> *twt = smem + (long)2 * W * tw_lds_len(L);
+    cx<R> *twt = smem + (long)LB * W * tw_lds_len(L);
     cx<R> v[8];
     if (active) {
 #pragma unroll
@@ -270,12 +272,12 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     }
     const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
     const int tsh = twl ? kTwLds : kTwMem;
-    fft8<R>(v, ln, p, tws, -1, lt, active, tsh);
+    fft8<R, LB == 1>(v, ln, p, tws, -1, lt, active, tsh);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
-    fft8<R>(v, ln, p, tws, +1, lt, active, tsh);
+    fft8<R, LB == 1>(v, ln, p, tws, +1, lt, active, tsh);
     if (active) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * N0] = v[q];
@@ -482,7 +484,7 @@ __global__ void __launch_bounds__(NT) k_bgspo_row(const BJob<R> *__restrict__ jo
     if (step < 0 || step >= J.nsteps) return;
     if (PHASE == 1 && J.nd == 1) return;
     extern __shared__ __align__(16) char smem_raw[];
-    BRowTile<R> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
+    BRowTile<R, 2> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
     const LinePlan &p0 = plans[J.plan0];
     if (twl) stage_twiddles(B.tw, p0);
     const bool two = J.S == 2;
@@ -617,39 +619,52 @@ static void allow_lds(K kernel, size_t bytes) {
     allowed = bytes;
 }
 
-template <class R, int NT, int PHASE>
+template <class R, int NT, int PHASE, int LB>
 static void row_launch(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
                        const LinePlan *plans, const int *tick, size_t lds, bool twl) {
-    allow_lds(k_brow<R, NT, PHASE>, lds);
-    QDX_LAUNCH((k_brow<R, NT, PHASE>), dim3(ntiles), dim3(NT), lds, s, jobs, tiles, plans, k, tick, twl ? 1 : 0);
+    allow_lds(k_brow<R, NT, PHASE, LB>, lds);
+    QDX_LAUNCH((k_brow<R, NT, PHASE, LB>), dim3(ntiles), dim3(NT), lds, s, jobs, tiles, plans, k, tick,
+               twl ? 1 : 0);
     QDX_HIP(hipGetLastError());
 }
 
-template <class R>
-void BatchLaunch<R>::row(hipStream_t s, int NT, int phase, int k, const BJob<R> *jobs, const BTile *tiles,
-                         int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
-    if (ntiles <= 0) return;
-    if (lds > 160 * 1024) throw std::runtime_error("row tile exceeds the 160 KiB LDS of a gfx950 CU");
-    if (NT == 256) {
-        if (phase == 0) row_launch<R, 256, 0>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-        else if (phase == 1) row_launch<R, 256, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-        else row_launch<R, 256, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-    } else if (NT == 512) {
-        if (phase == 0) row_launch<R, 512, 0>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-        else if (phase == 1) row_launch<R, 512, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-        else row_launch<R, 512, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-    } else {
-        throw std::runtime_error("row tiles use 256 or 512 threads");
-    }
+template <class R, int NT>
+static void row_phase(hipStream_t s, int lbuf, int phase, int k, const BJob<R> *jobs, const BTile *tiles,
+                      int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool twl) {
+    // the init phase runs no transform: one instance serves both layouts
+    if (phase == 0) row_launch<R, NT, 0, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else if (phase == 1 && lbuf == 2) row_launch<R, NT, 1, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else if (phase == 1) row_launch<R, NT, 1, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else if (lbuf == 2) row_launch<R, NT, 2, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else row_launch<R, NT, 2, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
 }
 
 template <class R>
-void BatchLaunch<R>::col(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+void BatchLaunch<R>::row(hipStream_t s, int NT, int lbuf, int phase, int k, const BJob<R> *jobs,
+                         const BTile *tiles, int ntiles, const LinePlan *plans, const int *tick, size_t lds,
+                         bool tw_lds) {
+    if (ntiles <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("row tile exceeds the 160 KiB LDS of a gfx950 CU");
+    if (lbuf != 1 && lbuf != 2) throw std::runtime_error("row tiles hold 1 or 2 LDS buffers per line");
+    if (NT == 256) row_phase<R, 256>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+    else if (NT == 512) row_phase<R, 512>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+    else throw std::runtime_error("row tiles use 256 or 512 threads");
+}
+
+template <class R>
+void BatchLaunch<R>::col(hipStream_t s, int lbuf, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
                          const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
     if (ntiles <= 0) return;
     if (lds > 160 * 1024) throw std::runtime_error("column tile exceeds the 160 KiB LDS of a gfx950 CU");
-    allow_lds(k_bcol<R>, lds);
-    QDX_LAUNCH((k_bcol<R>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
+    if (lbuf == 2) {
+        allow_lds(k_bcol<R, 2>, lds);
+        QDX_LAUNCH((k_bcol<R, 2>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
+    } else if (lbuf == 1) {
+        allow_lds(k_bcol<R, 1>, lds);
+        QDX_LAUNCH((k_bcol<R, 1>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
+    } else {
+        throw std::runtime_error("column tiles hold 1 or 2 LDS buffers per line");
+    }
     QDX_HIP(hipGetLastError());
 }
 

@@ -189,6 +189,47 @@ __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const Lds
     }
 }
 
+// The same stage on one buffer (in place): every thread reads its inputs,
+// the block synchronises, then the outputs are written.  Called by all
+// threads of the block (it synchronises); threads without a line pass
+// active = false.
+template <int RR, class R>
+__device__ __forceinline__ void mid_stage_ip(cx<R> *buf, const LdsLine<R> &ln, int L, int Ns, int TL, int lt,
+                                             int sign, const cx<R> *tw, int tsh, bool active) {
+    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
+    const int m = L / RR;
+    cx<R> x[Q][RR];
+    if (active) {
+#pragma unroll
+        for (int p = 0; p < Q; ++p) {
+            const int j = lt + p * TL;
+            if (j < m) {
+#pragma unroll
+                for (int t = 0; t < RR; ++t) x[p][t] = buf[ln.idx(j + t * m)];
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int tstep = L / (Ns * RR);
+    const bool pow2 = (Ns & (Ns - 1)) == 0;
+#pragma unroll
+    for (int p = 0; p < Q; ++p) {
+        const int j = lt + p * TL;
+        if (j < m) {
+            const int k = pow2 ? (j & (Ns - 1)) : (j % Ns);
+            if (k != 0) {
+#pragma unroll
+                for (int t = 1; t < RR; ++t) x[p][t] = cmul(x[p][t], twiddle(tw, k * t * tstep, sign, tsh));
+            }
+            Bfly<RR, R>::run(x[p], sign);
+            const int ob = (j - k) * RR + k;
+#pragma unroll
+            for (int t = 0; t < RR; ++t) buf[ln.idx(ob + t * Ns)] = x[p][t];
+        }
+    }
+}
+
 // copy the twiddle table of plan p into LDS (all threads of the block),
 // entry q at q + q/8: the strided reads of a stage (q = k t L/(Ns r)) then
 // fall on different banks.  The copy takes L + L/8 entries.
@@ -203,10 +244,12 @@ __host__ __device__ constexpr int tw_lds_len(int L) { return L + (L >> 3); }
 
 // Full unnormalised transform (sign -1 forward).  Every thread of the block
 // must call it (it synchronises); threads without a line pass active=false.
+// IP: the middle stages run in place on buffer a (half the LDS, one more
+// barrier per middle stage); otherwise they alternate between a and b.
 // tw: the table exp(-2 pi i q / L), q in [0, L): the padded LDS copy made by
 // stage_twiddles (tsh = kTwLds; the barrier at the start of fft8 publishes
 // it) or the table in memory (tsh = kTwMem).
-template <class R>
+template <class R, bool IP = false>
 __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const LinePlan &p, const cx<R> *tw,
                                      int sign, int lt, bool active, int tsh) {
     const int L = p.L, TL = L >> 3;
@@ -246,7 +289,20 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
     int Ns = r0;
     const cx<R> *in = ln.a;
     cx<R> *out = ln.b;
-    for (int st = 1; st < p.nst - 1; ++st) {
+    if constexpr (IP) {
+        for (int st = 1; st < p.nst - 1; ++st) {
+            const int r = p.rad[st];
+            switch (r) {
+            case 8: mid_stage_ip<8, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
+            case 4: mid_stage_ip<4, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
+            case 2: mid_stage_ip<2, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
+            default: mid_stage_ip<3, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
+            }
+            __syncthreads();
+            Ns *= r;
+        }
+    }
+    for (int st = 1; !IP && st < p.nst - 1; ++st) {
         const int r = p.rad[st];
         if (active) {
             switch (r) {

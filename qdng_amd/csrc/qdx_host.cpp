@@ -325,6 +325,28 @@ int qdx_ctx_kernel_time(qdx_ctx *ctx, int32_t kernel_class, int64_t *launches, d
     });
 }
 
+int qdx_ctx_set_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, const int32_t *radix, int32_t nst) {
+    return guarded([&] {
+        if (!ctx || !radix) fail(QDX_ERR_PARAM, "NULL argument");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        if (nst < 1 || nst > kMaxStages) fail(QDX_ERR_PARAM, "1 to 16 radix stages");
+        radix_plan(L);   // L = 8 * 2^a * 3^b
+        std::vector<int> r(radix, radix + nst);
+        if (!radix_ok(L, r)) fail(QDX_ERR_PARAM, "radix plan does not fit the line length");
+        ctx->radix[{L, precision == QDX_FP32_BULK}] = r;
+    });
+}
+
+int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *radix, int32_t *nst) {
+    return guarded([&] {
+        if (!ctx || !radix || !nst) fail(QDX_ERR_PARAM, "NULL argument");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        const std::vector<int> r = ctx->radix_of(L, precision == QDX_FP32_BULK);
+        for (size_t i = 0; i < r.size(); ++i) radix[i] = r[i];
+        *nst = (int32_t)r.size();
+    });
+}
+
 int qdx_ctx_synchronize(qdx_ctx *ctx) {
     return guarded([&] {
         if (!ctx) fail(QDX_ERR_PARAM, "ctx is NULL");

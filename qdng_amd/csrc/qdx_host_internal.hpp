@@ -12,6 +12,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -175,6 +176,47 @@ inline std::vector<int> radix_plan(int L) {
     return r;
 }
 
+// A radix sequence fft8 can run for length L: product L, first and last
+// radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 4, 2 or 3.
+inline bool radix_ok(int L, const std::vector<int> &r) {
+    if (r.empty() || (int)r.size() > kMaxStages) return false;
+    long prod = 1;
+    for (size_t i = 0; i < r.size(); ++i) {
+        const int x = r[i];
+        const bool edge = i == 0 || i + 1 == r.size();
+        if (!(x == 8 || x == 4 || x == 2 || (!edge && x == 3))) return false;
+        prod *= x;
+    }
+    if (L == 8) return r.size() == 1 && r[0] == 8;
+    return r.size() >= 2 && prod == L;
+}
+
+// Radix sequences for L with at most `extra` stages more than the fewest
+// possible: every ordering of the middle radices, edges from {8, 4, 2}.
+inline std::vector<std::vector<int>> radix_candidates(int L, int extra) {
+    std::vector<std::vector<int>> out;
+    if (L == 8) return {{8}};
+    const int nmin = (int)radix_plan(L).size();
+    std::vector<int> cur;
+    // depth-first over sequences whose prefix product divides L
+    std::function<void(long)> rec = [&](long prod) {
+        const int n = (int)cur.size();
+        if (prod == L) {
+            if (radix_ok(L, cur)) out.push_back(cur);
+            return;
+        }
+        if (n >= nmin + extra) return;
+        for (int x : {8, 4, 2, 3}) {
+            if ((L / prod) % x != 0) continue;
+            cur.push_back(x);
+            rec(prod * x);
+            cur.pop_back();
+        }
+    };
+    rec(1);
+    return out;
+}
+
 }  // namespace
 
 struct qdx_ctx {
@@ -183,6 +225,8 @@ struct qdx_ctx {
     int refs = 0;          // live wave functions, Hamiltonians and propagators
     bool closing = false;  // qdx_ctx_destroy called while children were alive
     std::map<int, std::unique_ptr<DevBuf>> tw64, tw32;
+    // radix plans set (or tuned) per line length and precision (fp32 flag)
+    std::map<std::pair<int, bool>, std::vector<int>> radix;
     DevBuf partials;   // reduction scratch
     DevBuf scal;       // small device result area
     double *pinned = nullptr;
@@ -218,7 +262,13 @@ struct qdx_ctx {
         return next_event++;
     }
 
-    LinePlan plan(int L, bool fp32) {
+    // radix sequence in use for length L: set / tuned, else the default
+    std::vector<int> radix_of(int L, bool fp32) const {
+        auto rt = radix.find({L, fp32});
+        return rt != radix.end() ? rt->second : radix_plan(L);
+    }
+    LinePlan plan(int L, bool fp32) { return plan(L, fp32, radix_of(L, fp32)); }
+    LinePlan plan(int L, bool fp32, const std::vector<int> &r) {
         auto &tab = fp32 ? tw32 : tw64;
         auto it = tab.find(L);
         if (it == tab.end()) {
@@ -237,9 +287,9 @@ struct qdx_ctx {
             }
             it = tab.emplace(L, std::move(buf)).first;
         }
+        if (!radix_ok(L, r)) fail(QDX_ERR_PARAM, "radix plan does not fit the line length");
         LinePlan p;
         p.L = L;
-        auto r = radix_plan(L);
         p.nst = (int)r.size();
         for (int i = 0; i < p.nst; ++i) p.rad[i] = r[i];
         p.tw = it->second->p;
