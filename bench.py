@@ -294,19 +294,26 @@ def serving_extra(ctx, lams, n_jobs):
 
 
 def cpu_baseline(p, steps):
-    """FP64 oracle on the host cores for `steps` Chebyshev steps from psi0."""
+    """FP64 oracle on the host cores for `steps` Chebyshev steps from psi0.
+    Only the steps are timed; the Reporter observables before each step
+    (norm^2, <psi(t)|psi0>, Reporter.cpp:101-137) are taken outside."""
     from oracle import oracle as O
     threads = min(os.cpu_count() or 1, 16)
     O.set_threads(threads)
     O.reset()
     U = O.Cheby(p.oracle_ham(), 5.0, prec=1e-10)
+    g = p.oracle_grid()
     psi = p.psi0.copy()
-    t0 = time.perf_counter()
+    dt = 0.0
     nh = 0
+    norm2, ac = [], []
     for i in range(steps):
+        norm2.append(O.norm2(g, psi, p.S))
+        ac.append(O.overlap(g, psi, p.psi0, p.S))
+        t0 = time.perf_counter()
         nh += U.step(psi, i)
-    dt = time.perf_counter() - t0
-    return psi, nh, dt, threads
+        dt += time.perf_counter() - t0
+    return psi, nh, dt, threads, np.array(norm2), np.array(ac)
 
 
 def main():
@@ -383,7 +390,7 @@ def main():
             "traffic": measured_traffic(),
             "traffic_source": "profiles/cheby_row_2d_512x512_2state_fp64_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.000 "
                               "+ WRITE_SIZE x 1.000, calibrated on a 256 MiB device copy)",
-            "kernel": "qdx::k_cheby_row<double, 2, false, false>",
+            "kernel": "qdx::k_cheby_row<double, 2, false, false, 256>",
             "launches": launches,
             "avg_launch_us": row_ms / launches * 1e3,
             "algorithmic_bytes_per_launch": row_bytes / launches,
@@ -392,12 +399,16 @@ def main():
             "measured": "HIP events attached to every launch (hipExtLaunchKernelGGL) over a second "
                         "timed pass of the same steps",
             "ms_per_step_profiled_pass": rp["seconds"] * 1e3 / args.steps,
+            "column_pass": {"kernel": "qdx::k_col<double, 4>", "launches": rp["line"][0],
+                            "avg_launch_us": rp["line"][1] / max(rp["line"][0], 1) * 1e3,
+                            "algorithmic_bytes_per_launch": 32.0 * p.size * p.S,
+                            "note": "phi_{k-1} read + Y written, 16 B each per point and state"},
         },
     }
 
     # ---- CPU baseline: FP64 oracle on a bounded sample of the same workload --
     if args.cpu_steps > 0:
-        psi_cpu, nh_cpu, dt_cpu, threads = cpu_baseline(p, args.cpu_steps)
+        psi_cpu, nh_cpu, dt_cpu, threads, n2_cpu, ac_cpu = cpu_baseline(p, args.cpu_steps)
         out["cpu_baseline"] = {
             "value": npts * nh_cpu / dt_cpu,
             "unit": UNIT,
@@ -406,25 +417,35 @@ def main():
             "sample": f"first {args.cpu_steps} Chebyshev steps of the same workload on the FP64 oracle "
                       f"(oracle/qdx_oracle.cpp, OpenMP {threads} threads), {dt_cpu:.1f} s",
         }
-        # parity of the same steps on the GPU
-        U = Q.Cheby(p.gpu_ham(ctx), 5.0, prec=1e-10)
-        w = p.gpu_wf(ctx)
-        U.run(w, 0, args.cpu_steps, observables=False)
-        g = w.download()
-        out.setdefault("extra", {})["l2_fp64_vs_oracle"] = float(np.linalg.norm(g - psi_cpu) / np.linalg.norm(psi_cpu))
+        # parity of the same steps on the GPU: final psi, norm drift and the
+        # autocorrelation series (Reporter observables before each step)
+        def parity(precision):
+            U = Q.Cheby(p.gpu_ham(ctx), 5.0, prec=1e-10, precision=precision)
+            w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
+            n2, ac, _, _ = U.run(w, 0, args.cpu_steps, w0)
+            g = w.download()
+            return {"l2_vs_oracle": float(np.linalg.norm(g - psi_cpu) / np.linalg.norm(psi_cpu)),
+                    "norm_drift_max": float(np.abs(n2 - n2[0]).max()),
+                    "norm_drift_max_oracle": float(np.abs(n2_cpu - n2_cpu[0]).max()),
+                    "autocorr_max_abs_err": float(np.abs(ac - ac_cpu).max()),
+                    "steps": args.cpu_steps}
+        ex = out.setdefault("extra", {})
+        r64 = parity("fp64")
+        ex["l2_fp64_vs_oracle"] = r64["l2_vs_oracle"]
+        ex["parity_fp64"] = r64
         if not args.no_extra:
-            U32 = Q.Cheby(p.gpu_ham(ctx), 5.0, prec=1e-10, precision="fp32")
-            w32 = p.gpu_wf(ctx)
-            U32.run(w32, 0, args.cpu_steps, observables=False)
-            g32 = w32.download()
-            out["extra"]["l2_fp32_bulk_vs_oracle"] = float(np.linalg.norm(g32 - psi_cpu) / np.linalg.norm(psi_cpu))
+            r32p = parity("fp32")
+            ex["l2_fp32_bulk_vs_oracle"] = r32p["l2_vs_oracle"]
+            ex["parity_fp32_bulk"] = r32p
 
     # ---- variants: FP32-bulk Chebyshev, GSPO -------------------------------
     if not args.no_extra:
         ex = out.setdefault("extra", {})
-        r32 = run_cheby(ctx, p, "fp32", args.steps, args.warmup, profile=True)
+        # timed without events (as the FP64 headline), kernels from a second pass
+        r32 = run_cheby(ctx, p, "fp32", args.steps, args.warmup, profile=False)
         h32 = int(r32["n_hpsi"].sum())
-        l32, ms32 = r32["row"]
+        rp32 = run_cheby(ctx, p, "fp32", args.steps, args.warmup, profile=True)
+        l32, ms32 = rp32["row"]
         b32 = p.size * p.S * (B_ROW["fp32"] * (h32 - args.steps) + B_ROW_PRO["fp32"] * args.steps)
         ex["fp32_bulk"] = {
             "value": npts * h32 / r32["seconds"], "unit": UNIT,
@@ -432,6 +453,7 @@ def main():
             "hpsi_per_step": [int(r32["n_hpsi"].min()), int(r32["n_hpsi"].max())],
             "row_kernel_achieved_GBs": b32 / (ms32 * 1e-3) / 1e9,
             "row_kernel_avg_launch_us": ms32 / l32 * 1e3,
+            "col_kernel_avg_launch_us": rp32["line"][1] / max(rp32["line"][0], 1) * 1e3,
             "vs_fp64": (npts * h32 / r32["seconds"]) / value,
         }
         log(f"fp32-bulk: {ex['fp32_bulk']['value']:.4g} grid-pt*Hpsi/s")
