@@ -328,6 +328,9 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer 4 times (reference for FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--serve-rates", default="50,500,2000", help="Poisson arrival rates (jobs/s), comma separated")
+    ap.add_argument("--batched-only", choices=["fp64", "fp32"],
+                    help="only run the batched-jobs workload in this precision (default schedules, "
+                         "--batch-steps ticks) and print its tick time: a short program for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -342,6 +345,11 @@ def main():
         for _ in range(4):
             b.copy_from(a)
         log("copied 4 x 256 MiB device to device")
+        return
+    if args.batched_only:
+        r = run_batched(ctx, args.batched_only, args.batch_steps, 1)
+        log(f"batched {args.batched_only}: {r['seconds'] * 1e3 / args.batch_steps:.3f} ms/tick "
+            f"over {args.batch_steps} ticks")
         return
     p = build_problem(512)
     npts = p.size * p.S
