@@ -256,7 +256,7 @@ int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed);
 
 /* Radix plan of the line transforms of length L in one precision on a
  * context: the sequence of radices (product L; first and last 8, 4 or 2;
- * middle 8, 4, 2 or 3; at most 16).  A plan set here is used by the
+ * middle 8, 6, 4, 3 or 2; at most 16).  A plan set here is used by the
  * propagators and Hamiltonians created and the batch jobs submitted
  * afterwards on this context (a batch job keeps the plans it was submitted
  * with).  Different plans give the same transform up to rounding. */
@@ -284,8 +284,10 @@ int qdx_batch_set_tiling(qdx_batch *b, int32_t n0, int32_t n1, const qdx_tiling 
  * synthetic fixed-length jobs of that shape (about 2 M grid points) in a
  * context and batch of its own on the same device, and times the row and
  * column kernels with events on their dispatch packets: first the radix
- * plans of N0 and N1 (each candidate with at most one stage more than the
- * fewest), then the row tilings, then the column tilings.  The winners are
+ * plans of N0 and N1 (up to 12 with no more stages than the default plan,
+ * fewest first, and 4 with one stage more), then the column widths (row
+ * tiles keep the default block size: another one would give the shape a
+ * launch of its own).  The winners are
  * installed: the plans on the batch's context (qdx_ctx_set_radix_plan), the
  * tiling on the batch (qdx_batch_set_tiling), for jobs submitted afterwards.
  * Times are per tick of the synthetic jobs, in microseconds. */

@@ -51,6 +51,39 @@ def test_apply_1d_all_lengths(ctx, N):
     assert rel(dst.download(), p.oracle_ham().apply(psi)) < 1e-12
 
 
+# radix plans other than the default, including radix-6 middle stages, on a
+# context of their own (the plan belongs to the context)
+RADIX_PLANS = [(48, [4, 6, 2]), (96, [8, 6, 2]), (192, [4, 6, 8]), (384, [8, 6, 8]), (384, [2, 8, 6, 4]),
+               (768, [8, 6, 4, 4]), (1536, [8, 8, 6, 4]), (3072, [8, 8, 6, 8]), (4096, [2, 8, 8, 4, 8]),
+               (512, [4, 2, 8, 2, 4])]
+
+
+@pytest.mark.parametrize("N,plan", RADIX_PLANS)
+def test_apply_1d_radix_plans(N, plan):
+    import qdng_amd as Q
+    note(f"apply 1D N={N} plan {plan}")
+    c = Q.Context(0)
+    for prec in ("fp64", "fp32"):
+        c.set_radix_plan(N, plan, prec)
+    p = cases.harmonic_1d(N, L=6.0)
+    rng = np.random.default_rng(N + len(plan))
+    psi = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    ref = p.oracle_ham().apply(psi)
+    H = p.gpu_ham(c)
+    src, dst = p.gpu_wf(c, psi), p.gpu_wf(c)
+    H.apply(dst, src)
+    assert rel(dst.download(), ref) < 1e-12
+    # the FP32-bulk Chebyshev path runs the FP32 transforms with this plan;
+    # the random psi carries every wave number, so the FP32 rounding of the
+    # transforms reaches ~2e-5 after 3 steps at N = 3072 (any plan)
+    U32 = Q.Cheby(p.gpu_ham(c), 1.0, prec=1e-10, precision="fp32")
+    U64 = Q.Cheby(p.gpu_ham(c), 1.0, prec=1e-10, precision="fp64")
+    w32, w64 = p.gpu_wf(c, psi), p.gpu_wf(c, psi)
+    U32.run(w32, 0, 3, observables=False)
+    U64.run(w64, 0, 3, observables=False)
+    assert rel(w32.download(), w64.download()) < 5e-5
+
+
 @pytest.mark.parametrize("dims,S", [([64, 48], 1), ([32, 96], 2), ([128, 128], 2), ([24, 16, 32], 1),
                                     ([32, 32, 32], 2), ([16, 64, 8], 1)])
 def test_apply_multi_d(ctx, dims, S):
