@@ -256,7 +256,7 @@ int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed);
 
 /* Radix plan of the line transforms of length L in one precision on a
  * context: the sequence of radices (product L; first and last 8, 4 or 2;
- * middle 8, 6, 4, 3 or 2; at most 16).  A plan set here is used by the
+ * middle 8, 4, 3 or 2; at most 16).  A plan set here is used by the
  * propagators and Hamiltonians created and the batch jobs submitted
  * afterwards on this context (a batch job keeps the plans it was submitted
  * with).  Different plans give the same transform up to rounding. */

@@ -112,28 +112,6 @@ struct Bfly<3, R> {
     }
 };
 
-// 6 = 2 x 3: y[u] = A[u mod 3] + w6^u B[u mod 3], A / B the 3-point DFTs of
-// the even / odd inputs, w6 = exp(sign i pi / 3)
-template <class R>
-struct Bfly<6, R> {
-    static __device__ __forceinline__ void run(cx<R> *x, int sign) {
-        const R h = (R)0.86602540378443864676372317075293618;  // sqrt(3)/2
-        cx<R> a[3] = {x[0], x[2], x[4]};
-        cx<R> b[3] = {x[1], x[3], x[5]};
-        Bfly<3, R>::run(a, sign);
-        Bfly<3, R>::run(b, sign);
-        // w6 b1 = (1/2 + sign i h) b1, w6^2 b2 = (-1/2 + sign i h) b2
-        const cx<R> t1 = scale(b[1], (R)0.5) + mul_si(scale(b[1], h), sign);
-        const cx<R> t2 = mul_si(scale(b[2], h), sign) - scale(b[2], (R)0.5);
-        x[0] = a[0] + b[0];
-        x[3] = a[0] - b[0];
-        x[1] = a[1] + t1;
-        x[4] = a[1] - t1;
-        x[2] = a[2] + t2;
-        x[5] = a[2] - t2;
-    }
-};
-
 template <class R>
 struct Bfly<8, R> {
     static __device__ __forceinline__ void run(cx<R> *x, int sign) {
@@ -162,7 +140,7 @@ struct Bfly<8, R> {
 // those elements and the last stage (span L/r) writes exactly those elements,
 // so both run on registers; only the middle stages go through LDS, using two
 // buffers alternately (one barrier per stage).  The plan's first and last
-// radices are 8, 4 or 2; middle stages may also be radix 6 or 3.
+// radices are 8, 4 or 2; middle stages may also be radix 3.
 //
 // LDS addressing of one line: element i of buffer a / b lives at
 // a[idx(i)] / b[idx(i)], idx(i) = (pad ? i + i/8 : i) * es.
@@ -184,17 +162,10 @@ __device__ __forceinline__ cx<R> twiddle(const cx<R> *tw, int q, int sign, int t
     return w;
 }
 
-// butterflies per thread in a middle stage of radix RR: m = L / RR
-// butterflies on TL = L / 8 threads
-template <int RR>
-constexpr int mid_q() {
-    return RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : RR == 6 ? 2 : 1;
-}
-
 template <int RR, class R>
 __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const LdsLine<R> &ln, int L, int Ns,
                                           int TL, int lt, int sign, const cx<R> *tw, int tsh) {
-    constexpr int Q = mid_q<RR>();
+    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
     const int m = L / RR;
     const int tstep = L / (Ns * RR);
     const bool pow2 = (Ns & (Ns - 1)) == 0;
@@ -225,7 +196,7 @@ __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const Lds
 template <int RR, class R>
 __device__ __forceinline__ void mid_stage_ip(cx<R> *buf, const LdsLine<R> &ln, int L, int Ns, int TL, int lt,
                                              int sign, const cx<R> *tw, int tsh, bool active) {
-    constexpr int Q = mid_q<RR>();
+    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
     const int m = L / RR;
     cx<R> x[Q][RR];
     if (active) {
@@ -325,7 +296,6 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             case 8: mid_stage_ip<8, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
             case 4: mid_stage_ip<4, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
             case 2: mid_stage_ip<2, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
-            case 6: mid_stage_ip<6, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
             default: mid_stage_ip<3, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
             }
             __syncthreads();
@@ -339,7 +309,6 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
             case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
             case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
-            case 6: mid_stage<6, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
             default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
             }
         }

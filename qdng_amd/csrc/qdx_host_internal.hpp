@@ -177,14 +177,14 @@ inline std::vector<int> radix_plan(int L) {
 }
 
 // A radix sequence fft8 can run for length L: product L, first and last
-// radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 6, 4, 3 or 2.
+// radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 4, 3 or 2.
 inline bool radix_ok(int L, const std::vector<int> &r) {
     if (r.empty() || (int)r.size() > kMaxStages) return false;
     long prod = 1;
     for (size_t i = 0; i < r.size(); ++i) {
         const int x = r[i];
         const bool edge = i == 0 || i + 1 == r.size();
-        if (!(x == 8 || x == 4 || x == 2 || (!edge && (x == 3 || x == 6)))) return false;
+        if (!(x == 8 || x == 4 || x == 2 || (!edge && x == 3))) return false;
         prod *= x;
     }
     if (L == 8) return r.size() == 1 && r[0] == 8;
@@ -206,7 +206,7 @@ inline std::vector<std::vector<int>> radix_candidates(int L, int extra) {
             return;
         }
         if (n >= nmin + extra) return;
-        for (int x : {8, 6, 4, 3, 2}) {
+        for (int x : {8, 4, 3, 2}) {
             if ((L / prod) % x != 0) continue;
             cur.push_back(x);
             rec(prod * x);

@@ -51,11 +51,11 @@ def test_apply_1d_all_lengths(ctx, N):
     assert rel(dst.download(), p.oracle_ham().apply(psi)) < 1e-12
 
 
-# radix plans other than the default, including radix-6 middle stages, on a
-# context of their own (the plan belongs to the context)
-RADIX_PLANS = [(48, [4, 6, 2]), (96, [8, 6, 2]), (192, [4, 6, 8]), (384, [8, 6, 8]), (384, [2, 8, 6, 4]),
-               (768, [8, 6, 4, 4]), (1536, [8, 8, 6, 4]), (3072, [8, 8, 6, 8]), (4096, [2, 8, 8, 4, 8]),
-               (512, [4, 2, 8, 2, 4])]
+# radix plans other than the default on a context of their own (the plan
+# belongs to the context)
+RADIX_PLANS = [(48, [4, 3, 4]), (96, [2, 8, 3, 2]), (192, [4, 3, 2, 8]), (384, [8, 3, 8, 2]),
+               (384, [2, 8, 3, 8]), (768, [8, 3, 8, 4]), (1536, [8, 8, 3, 8]), (3072, [8, 4, 4, 3, 8]),
+               (4096, [2, 8, 8, 4, 8]), (512, [4, 2, 8, 2, 4])]
 
 
 @pytest.mark.parametrize("N,plan", RADIX_PLANS)
