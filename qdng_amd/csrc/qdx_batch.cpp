@@ -334,8 +334,10 @@ struct qdx_batch {
     bool autotune = false;
     // LDS buffers per line of the Chebyshev row / column tiles of all 2D jobs
     // (2: ping-pong, 1: in place; bit-identical results).  Default: in place
-    // where it raises the kernel's register-limited occupancy (FP32 rows
-    // 3 -> 4 waves per SIMD, FP64 columns 2 -> 3).
+    // where the LDS, not the registers, limits the waves per SIMD: FP64 rows
+    // (2 -> 3) and columns (2 -> 3), FP32 columns; FP32 rows are held to 4
+    // waves by their registers either way and keep the barrier-saving
+    // ping-pong.
     int lbr = 2, lbc = 2;
     size_t rsz() const { return precision == QDX_FP32_BULK ? sizeof(float) : sizeof(double); }
 };
@@ -1113,8 +1115,8 @@ int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_ba
         auto b = std::make_unique<qdx_batch>();
         b->ctx = ctx;
         b->precision = precision;
-        b->lbr = precision == QDX_FP32_BULK ? 1 : 2;
-        b->lbc = precision == QDX_FP32_BULK ? 2 : 1;
+        b->lbr = precision == QDX_FP32_BULK ? 2 : 1;
+        b->lbc = 1;
         b->arena.init((size_t)hbm_budget);
         b->dtick.alloc(sizeof(int));
         HIPCHK(hipMemset(b->dtick.p, 0, sizeof(int)));

@@ -292,11 +292,16 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
     if constexpr (IP) {
         for (int st = 1; st < p.nst - 1; ++st) {
             const int r = p.rad[st];
+            // the lane index is made opaque per stage: otherwise every radix
+            // case's lane-dependent addresses are hoisted out of the loop and
+            // stay live together (tens of VGPRs, a lower occupancy)
+            int lv = lt;
+            asm volatile("" : "+v"(lv));
             switch (r) {
-            case 8: mid_stage_ip<8, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
-            case 4: mid_stage_ip<4, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
-            case 2: mid_stage_ip<2, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
-            default: mid_stage_ip<3, R>(ln.a, ln, L, Ns, TL, lt, sign, tw, tsh, active); break;
+            case 8: mid_stage_ip<8, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            case 4: mid_stage_ip<4, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            case 2: mid_stage_ip<2, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            default: mid_stage_ip<3, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
             }
             __syncthreads();
             Ns *= r;
@@ -304,12 +309,14 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
     }
     for (int st = 1; !IP && st < p.nst - 1; ++st) {
         const int r = p.rad[st];
+        int lv = lt;   // opaque per stage, as above
+        asm volatile("" : "+v"(lv));
         if (active) {
             switch (r) {
-            case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
-            case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
-            case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
-            default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lt, sign, tw, tsh); break;
+            case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
             }
         }
         __syncthreads();
