@@ -306,7 +306,7 @@ int qdx_batch_set_autotune(qdx_batch *b, int32_t enable);
  * buffers per line, 2 (the FFT's middle stages alternate between two
  * buffers, one barrier per stage) or 1 (in place: half the LDS, one more
  * barrier per middle stage).  Results are bit-identical either way.
- * Default: rows 2 / columns 1 in FP32-bulk, rows 1 / columns 1 in FP64. */
+ * Default: 1 for rows and columns. */
 int qdx_batch_set_lds_schedule(qdx_batch *b, int32_t row_bufs, int32_t col_bufs);
 int qdx_batch_get_lds_schedule(qdx_batch *b, int32_t *row_bufs, int32_t *col_bufs);
 /* Time the four LDS schedules on the batch's current jobs (three ticks each;

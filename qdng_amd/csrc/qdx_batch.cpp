@@ -333,12 +333,11 @@ struct qdx_batch {
     std::map<std::pair<int, int>, JobTiling> tilings;
     bool autotune = false;
     // LDS buffers per line of the Chebyshev row / column tiles of all 2D jobs
-    // (2: ping-pong, 1: in place; bit-identical results).  Default: in place
-    // where the LDS, not the registers, limits the waves per SIMD: FP64 rows
-    // (2 -> 3) and columns (2 -> 3), FP32 columns; FP32 rows are held to 4
-    // waves by their registers either way and keep the barrier-saving
-    // ping-pong.
-    int lbr = 2, lbc = 2;
+    // (2: ping-pong, 1: in place; bit-identical results).  Default: in place,
+    // since the LDS of two buffers, not the registers, bounds the waves per
+    // SIMD of both kernels in both precisions (FP64 rows and columns 4
+    // waves in place, 2 with two buffers; FP32 6 against 4).
+    int lbr = 1, lbc = 1;
     size_t rsz() const { return precision == QDX_FP32_BULK ? sizeof(float) : sizeof(double); }
 };
 
@@ -1115,7 +1114,7 @@ int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_ba
         auto b = std::make_unique<qdx_batch>();
         b->ctx = ctx;
         b->precision = precision;
-        b->lbr = precision == QDX_FP32_BULK ? 2 : 1;
+        b->lbr = 1;
         b->lbc = 1;
         b->arena.init((size_t)hbm_budget);
         b->dtick.alloc(sizeof(int));

@@ -108,11 +108,6 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
         for (int q = 0; q < 8; ++q) v[q] = phim1[B.gidx(q)];
     }
     const bool has_t = J.tk0 != nullptr;
-    double tq[8];
-    if (B.active) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + J.tk0[B.lt + q * B.TL]) * J.inv_n0 : 0.0;
-    }
     if (twl) stage_twiddles(B.tw, p0);
     if (PHASE == 1) {
         if (t.first == 0) {
@@ -141,9 +136,14 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
     }
     const double Ed = J.nt > 0 ? -J.laser[step] * J.lscale : 0.0;
     bfft(v, B, p0, twl, -1);
+    // T_0/N_0 read after the forward transform (not held across it: the
+    // registers bound the waves per SIMD; other waves hide the load)
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+        for (int q = 0; q < 8; ++q) {
+            const double t = has_t ? (0.0 + J.tk0[B.lt + q * B.TL]) * J.inv_n0 : 0.0;
+            v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
+        }
     }
     bfft(v, B, p0, twl, +1);
 
@@ -167,13 +167,15 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
         cx<double> cm1 = mk<double>(0, 0), cm2 = mk<double>(0, 0);
         if (PHASE == 2 && nterm >= 2) cm1 = mk<double>(cf[2 * (k - 1)], cf[2 * (k - 1) + 1]);
         if (PHASE == 2 && nterm >= 3) cm2 = mk<double>(cf[2 * (k - 2)], cf[2 * (k - 2) + 1]);
+        // two points at a time: fewer operands live, more waves per SIMD
+        constexpr int kEp = 2;
 #pragma unroll
-        for (int h0 = 0; h0 < 8; h0 += 4) {
-            cx<double> ps[4];
-            cx<R> m2[4], yv[4], pm[4];
-            double wv[4];
+        for (int h0 = 0; h0 < 8; h0 += kEp) {
+            cx<double> ps[kEp];
+            cx<R> m2[kEp], yv[kEp], pm[kEp];
+            double wv[kEp];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kEp; ++u) {
                 const long gi = B.gidx(h0 + u);
                 if (upd) ps[u] = psi[gi];
                 pm[u] = phim1[gi];
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
                 wv[u] = w;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kEp; ++u) {
                 const int q = h0 + u;
                 const long gi = B.gidx(q);
                 cx<double> hp = mk<double>(0, 0);
@@ -253,12 +255,6 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = in[(long)(lt + q * TL) * N0];
     }
-    double tq[8];
-    if (active) {
-        const bool has_t = J.tk1 != nullptr;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + J.tk1[lt + q * TL]) * J.inv_n1 : 0.0;
-    }
     if (twl) stage_twiddles(twt, p);
     if (k >= 3) {
         ConvCheck cc;
@@ -273,9 +269,13 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     const cx<R> *tws = twl ? twt : reinterpret_cast<const cx<R> *>(p.tw);
     const int tsh = twl ? kTwLds : kTwMem;
     fft8<R, LB == 1>(v, ln, p, tws, -1, lt, active, tsh);
-    if (active) {
+    if (active) {   // T_1/N_1 read after the forward transform, as in k_brow
+        const bool has_t = J.tk1 != nullptr;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+        for (int q = 0; q < 8; ++q) {
+            const double t = has_t ? (0.0 + J.tk1[lt + q * TL]) * J.inv_n1 : 0.0;
+            v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
+        }
     }
     fft8<R, LB == 1>(v, ln, p, tws, +1, lt, active, tsh);
     if (active) {
