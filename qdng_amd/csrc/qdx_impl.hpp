@@ -141,12 +141,15 @@ struct Launch {
     // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator)
     static void col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode,
                          const LinePlan &p, const HamDev<R> &h, double tau, R *X,
-                         const ChebyCtl *ctl, const ConvCheck &cc);
+                         ChebyCtl *ctl, const ConvCheck &cc);
     // kinetic part along dimension `dim` of the separable T = sum_d T_d:
-    // Y = IFFT_d(T_d/N_d FFT_d(src)), or Y += that when `accumulate`
+    // Y = IFFT_d(T_d/N_d FFT_d(src)), or Y += that when `accumulate`.  The
+    // first pass of an order also evaluates the break test of the previous
+    // order and records its outcome in ctl (done, accnorm), which the row
+    // kernel of the order then only reads
     static void kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate,
                          const LinePlan &p, const HamDev<R> &h, const R *src, R *Y,
-                         const ChebyCtl *ctl, const ConvCheck &cc);
+                         ChebyCtl *ctl, const ConvCheck &cc);
     // dst = H src (both complex128) through the same kernels; X scratch
     static void ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h,
                           const LinePlan *plans, const double *src, double *dst, R *X);

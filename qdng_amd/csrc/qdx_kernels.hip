@@ -184,14 +184,18 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
             }
         }
     } else {
-        double acc_prev;
-        if (prev_converged(cc, red, &acc_prev)) {
-            // the previous order ended the series (OCheby.cpp:179-182)
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                ctl->done = 1;
-                ctl->accnorm = acc_prev;
+        // break test of the previous order (OCheby.cpp:179-182).  On grids
+        // with nd >= 2 the order's first column pass has evaluated it and
+        // recorded a break in ctl->done, read at the kernel's start.
+        if (ONE_D) {
+            double acc_prev;
+            if (prev_converged(cc, red, &acc_prev)) {
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    ctl->done = 1;
+                    ctl->accnorm = acc_prev;
+                }
+                return;
             }
-            return;
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) ctl->last_order = ord.k;
     }
@@ -307,7 +311,7 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
 template <class R, int MODE>
 __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
                                               cx<R> *__restrict__ X, const cx<R> *__restrict__ src,
-                                              const ChebyCtl *ctl, ConvCheck cc) {
+                                              ChebyCtl *ctl, ConvCheck cc) {
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
@@ -365,7 +369,16 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     if (tw_in_lds) stage_twiddles(twl, p);
     {
         double a;
-        if (prev_converged(cc, &bc, &a)) return;
+        if (prev_converged(cc, &bc, &a)) {
+            // the previous order ended the series (OCheby.cpp:179-182); the
+            // row kernel of this order reads the outcome from ctl.  Every
+            // block reaches the same decision, block 0 records it.
+            if (MODE == 4 && ctl && b == 0 && threadIdx.x == 0) {
+                ctl->done = 1;
+                ctl->accnorm = a;
+            }
+            return;
+        }
     }
     FFT(MODE == 1 ? +1 : -1);
     if (MODE >= 2) {
@@ -826,7 +839,7 @@ int Launch<R>::cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, co
 
 template <class R>
 void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
-                         const HamDev<R> &h, double tau, R *X, const ChebyCtl *ctl, const ConvCheck &cc) {
+                         const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc) {
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
@@ -860,7 +873,7 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
 
 template <class R>
 void Launch<R>::kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate, const LinePlan &plan,
-                         const HamDev<R> &h, const R *src, R *Y, const ChebyCtl *ctl, const ConvCheck &cc) {
+                         const HamDev<R> &h, const R *src, R *Y, ChebyCtl *ctl, const ConvCheck &cc) {
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
