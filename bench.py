@@ -316,6 +316,63 @@ def cpu_baseline(p, steps):
     return psi, nh, dt, threads, np.array(norm2), np.array(ac)
 
 
+def cpu_one_thread(p, steps):
+    """The FP64 oracle on one host thread, first `steps` steps of the same
+    workload (beside the multi-threaded cpu_baseline)."""
+    from oracle import oracle as O
+    O.set_threads(1)
+    U = O.Cheby(p.oracle_ham(), 5.0, prec=1e-10)
+    psi = p.psi0.copy()
+    t0 = time.perf_counter()
+    nh = sum(U.step(psi, i) for i in range(steps))
+    dt = time.perf_counter() - t0
+    O.set_threads(min(os.cpu_count() or 1, 16))
+    return {"value": p.size * p.S * nh / dt, "unit": UNIT, "cores": 1, "kind": "port",
+            "sample": f"first {steps} steps, {dt:.1f} s"}
+
+
+def cheby_1d_small(ctx, steps=1000):
+    """The small 1D case (256 points on [-8, 8], m = 1836, V = 0.05 x^2,
+    Gaussian psi0 at x = 1, dt = 10, prec = 1e-12, 1000 steps): a latency-bound
+    workload (a few kernels of 256 points per order), reported but not judged
+    on the roofline; parity of the final psi, the norm and the
+    autocorrelation series against the oracle."""
+    import cases
+    import qdng_amd as Q
+    from oracle import oracle as O
+    p = cases.harmonic_1d(256, L=8.0, mass=1836.0, k=0.1, x0=1.0, sigma=0.5)
+    U = Q.Cheby(p.gpu_ham(ctx), 10.0, prec=1e-12)
+    w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
+    sync(ctx)
+    t0 = time.perf_counter()
+    n2, ac, nh, _ = U.run(w, 0, steps, w0)
+    sync(ctx)
+    gpu_s = time.perf_counter() - t0
+    O.set_threads(1)   # 256 points: threads would only add overhead
+    Uo = O.Cheby(p.oracle_ham(), 10.0, prec=1e-12)
+    g = p.oracle_grid()
+    psi = p.psi0.copy()
+    n2o, aco = np.zeros(steps), np.zeros(steps, dtype=complex)
+    cpu_s = 0.0
+    nho = 0
+    for i in range(steps):
+        n2o[i] = O.norm2(g, psi)
+        aco[i] = O.overlap(g, psi, p.psi0)
+        t1 = time.perf_counter()
+        nho += Uo.step(psi, i)
+        cpu_s += time.perf_counter() - t1
+    O.set_threads(min(os.cpu_count() or 1, 16))
+    gv = w.download()
+    return {"value": 256 * int(nh.sum()) / gpu_s, "unit": UNIT, "ms_per_step": gpu_s * 1e3 / steps,
+            "hpsi_per_step": [int(nh.min()), int(nh.max())], "hpsi_oracle": int(nho), "hpsi_gpu": int(nh.sum()),
+            "cpu_value": 256 * nho / cpu_s, "cpu_threads": 1,
+            "l2_vs_oracle": float(np.linalg.norm(gv - psi) / np.linalg.norm(psi)),
+            "norm_drift_max": float(np.abs(n2 - n2[0]).max()),
+            "autocorr_max_abs_err": float(np.abs(ac - aco).max()),
+            "note": "latency bound (one 32-thread line; the whole step runs in one block): not judged on "
+                    "the roofline"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,6 +522,8 @@ def main():
             "vs_fp64": (npts * h32 / r32["seconds"]) / value,
         }
         log(f"fp32-bulk: {ex['fp32_bulk']['value']:.4g} grid-pt*Hpsi/s")
+        ex["cpu_1_thread"] = cpu_one_thread(p, 2)
+        ex["cheby_1d_256_latency"] = cheby_1d_small(ctx)
         gsteps = max(args.steps // 4, 1)
         for prec in ("fp64", "fp32"):
             t = run_gspo(ctx, p, prec, gsteps, min(args.warmup, 10))

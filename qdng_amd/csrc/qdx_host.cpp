@@ -30,6 +30,10 @@ struct qdx_prop {
     ChebyParams cp;
     int k_launch = 0;   // orders launched per step (launch_window)
     DevBuf phiA, phiB, X, ctl, partials, lastorders;
+    // single-state 1D lines: the whole step in one block (k_cheby_line) with
+    // the phased coefficients coeff[k] exp(-i offset dt) on the device
+    bool line_step = false;
+    DevBuf coefd;
     int last_nb = 0, nb_init = 0;   // partial counts of the last Chebyshev step
     // GSPO
     std::vector<qdx_ham *> ops;
@@ -67,6 +71,14 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     double *part = p->partials.as<double>();
     double *slot[2] = {part, part + kMaxPartials};
     double *init = part + 2 * kMaxPartials;
+    if (p->line_step) {
+        KTimer t(ctx, QDX_K_CHEBY_ROW);
+        Launch<R>::cheby_line(s, g, h, plans[0], p->coefd.as<double>(), p->k_launch, p->cp.offset,
+                              1.0 / p->cp.scaling, p->cp.prec, psi->ptr(), ctl, p->partials.as<double>());
+        p->last_nb = 1;
+        p->nb_init = 1;
+        return;
+    }
     const int S = p->S;
     int nb_init;
     {
@@ -666,6 +678,22 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         p->ctl.alloc(sizeof(ChebyCtl));
         HIPCHK(hipMemset(p->ctl.p, 0, sizeof(ChebyCtl)));
         p->partials.alloc(3 * kMaxPartials * sizeof(double));
+        // (FP64 lines above 2048 points need more registers than a wave of a
+        // 512-thread block holds; they keep the per-order kernels)
+        p->line_step = p->grid.nd == 1 && p->S == 1 &&
+                       p->grid.N[0] <= (precision == QDX_FP32_BULK ? kMaxLine : 2048);
+        if (p->line_step) {
+            // the same products as the per-order scalars of cheby_step
+            const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * dt * p->cp.offset);
+            std::vector<double> c(2 * (size_t)std::max(p->k_launch, 1));
+            for (int k = 0; k < p->k_launch; ++k) {
+                const std::complex<double> v = p->cp.coeff[k] * ph;
+                c[2 * k] = v.real();
+                c[2 * k + 1] = v.imag();
+            }
+            p->coefd.alloc(c.size() * sizeof(double));
+            HIPCHK(hipMemcpy(p->coefd.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
         HIPCHK(hipDeviceSynchronize());
         ctx_acquire(p->ctx);
         *out = p.release();

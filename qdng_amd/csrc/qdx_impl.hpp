@@ -136,6 +136,11 @@ struct Launch {
                          const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X,
                          double *psi, ChebyCtl *ctl, double *partials, const double *init_part,
                          int nb_init);
+    // a whole Chebyshev step of a single-state 1D line in one block (see
+    // k_cheby_line); coef: (re, im) of coeff[k] exp(-i offset dt), k < order
+    static void cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
+                           const double *coef, int order, double offset, double inv_scaling, double prec,
+                           double *psi, ChebyCtl *ctl, double *partials);
     // strided line pass along dimension `dim` (1 or 2) on X, in place.
     // mode 0: forward, 1: inverse, 2: forward * T/N inverse (kinetic),
     // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator)

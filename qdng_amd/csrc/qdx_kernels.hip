@@ -300,6 +300,138 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
 }
 
 // ---------------------------------------------------------------------------
+// A whole Chebyshev step of a single-state 1D line (L <= 4096) in one block:
+// the order-by-order arithmetic of k_cheby_init and k_cheby_row (same
+// operands in the same order: prologue, recurrence, FP64 psi accumulation,
+// the fixed-length FP32 series' update every third order, one block sum of
+// |psi|^2 per order and the break test on it, OCheby.cpp:92-197), with psi
+// and the recurrence terms in registers and only the FFT stages in LDS: one
+// launch per step instead of one per order.  coef[2k], coef[2k+1]: coeff[k]
+// exp(-i offset dt).  partials: the per-order slots of the multi-kernel path
+// ([k % 2] * kMaxPartials, step start at 2 * kMaxPartials), one sum each.
+template <class R, int NT>
+__global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LinePlan p0, const double *coef,
+                                                   int order, double offset, double inv_scaling, double prec,
+                                                   cx<double> *__restrict__ psi, ChebyCtl *ctl,
+                                                   double *__restrict__ partials) {
+    extern __shared__ __align__(16) char smem_raw[];
+    __shared__ double red[16];
+    __shared__ double bc;
+    RowBlock<R> B(g, 1, 1, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    const double *tk0 = h.tk[0][0];
+    const bool has_t = tk0 != nullptr;
+    const R *Wp = h.W[0][0];
+    const R *Mp = h.mu[0][0];
+    const double Ed = h.Edip[0][0];
+    const double uni = h.unity[0];
+    cx<double> ps[8];
+    cx<R> pm1[8], pm2[8], v[8];
+    double wv[8], tq[8];
+    double acc = 0;
+    if (B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long gi = B.gidx(q);
+            ps[q] = psi[gi];
+            acc += ps[q].r * ps[q].r + ps[q].i * ps[q].i;
+            pm1[q] = from_d<R>(ps[q]);
+            double w = uni;
+            if (Wp) w += (double)Wp[gi];
+            if (Mp) w += Ed * (double)Mp[gi];
+            wv[q] = w;
+            tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_nd[0] : 0.0;
+        }
+    }
+    // sum |psi|^2 of the step start (the init kernel's single partial)
+    double norm0 = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        partials[2 * kMaxPartials] = norm0;
+        bc = norm0;
+        ctl->norm0 = norm0;
+        ctl->done = 0;
+        ctl->last_order = 1;
+    }
+    __syncthreads();
+    norm0 = bc;
+    B.stage(p0);
+    const cx<double> c0 = mk<double>(coef[0], coef[1]);
+    for (int k = 1; k < order; ++k) {
+        if (k >= 3 && prec > 0) {
+            // break test of order k - 1 on its sum (stored in bc below)
+            const double a = bc;
+            if (fabs(norm0 - a) / norm0 < prec) {   // uniform over the block
+                if (threadIdx.x == 0) {
+                    ctl->done = 1;
+                    ctl->accnorm = a;
+                }
+                break;   // psi holds the series through order k - 1
+            }
+        }
+        if (threadIdx.x == 0) ctl->last_order = k;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = pm1[q];
+        B.fft(v, p0, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+        }
+        B.fft(v, p0, +1);
+        // psi update at this order: every order with the break test, the
+        // last three terms every third order of the fixed-length series
+        int nterm = 1;
+        if (k >= 2 && !(prec > 0)) {
+            const int m = (k - 1) % 3;
+            nterm = m == 0 ? 3 : (k == order - 1 ? m : 0);
+        }
+        const cx<double> ck = mk<double>(coef[2 * k], coef[2 * k + 1]);
+        const cx<double> cm1 = k >= 2 ? mk<double>(coef[2 * (k - 1)], coef[2 * (k - 1) + 1]) : mk<double>(0, 0);
+        const cx<double> cm2 = k >= 3 ? mk<double>(coef[2 * (k - 2)], coef[2 * (k - 2) + 1]) : mk<double>(0, 0);
+        acc = 0;
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                cx<double> hp = mk<double>(0, 0);
+                if (has_t) hp = to_d(v[q]);
+                const cx<double> p = to_d(pm1[q]);
+                hp.r += wv[q] * p.r;
+                hp.i += wv[q] * p.i;
+                const cx<double> b = mk<double>((hp.r - offset * p.r) * inv_scaling, (hp.i - offset * p.i) * inv_scaling);
+                cx<double> pk;
+                if (k == 1) {
+                    pk = mk<double>(b.i, -b.r);
+                    ps[q] = cmul(c0, ps[q]) + cmul(ck, pk);
+                } else {
+                    const cx<double> mm = to_d(pm2[q]);
+                    pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);
+                    if (nterm > 0) {
+                        cx<double> sv = ps[q];
+                        if (nterm >= 3) sv = sv + cmul(cm2, mm);
+                        if (nterm >= 2) sv = sv + cmul(cm1, p);
+                        sv = sv + cmul(ck, pk);
+                        acc += sv.r * sv.r + sv.i * sv.i;
+                        ps[q] = sv;
+                    }
+                }
+                pm2[q] = pm1[q];
+                pm1[q] = from_d<R>(pk);
+            }
+        }
+        if (k >= 2) {
+            const double a = block_sum(acc, red);
+            if (threadIdx.x == 0) {
+                partials[(k % 2) * kMaxPartials] = a;
+                bc = a;
+            }
+            __syncthreads();
+        }
+    }
+    if (B.active) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) psi[B.gidx(q)] = ps[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Strided line pass along dimension `dim` (1 or 2) on X, in place.  A block
 // owns a tile of W adjacent dimension-0 positions for one state and one index
 // of the remaining dimension; the W lanes of a butterfly touch W adjacent
@@ -835,6 +967,30 @@ int Launch<R>::cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, co
     QDX_DISPATCH_S(h.S, (nb = cheby_row_S<R, SS>(s, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials,
                                                  init_part, nb_init)));
     return nb;
+}
+
+template <class R>
+void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
+                           const double *coef, int order, double offset, double inv_scaling, double prec,
+                           double *psi, ChebyCtl *ctl, double *partials) {
+    const int L = g.N[0], TL = L / 8;
+    if (g.nd != 1 || h.S != 1 || L > kMaxLine) throw std::runtime_error("whole-step line kernel: one 1D line");
+    LinePlan p = p0;
+    size_t lds = row_lds_bytes<R>(L, 1, true);
+    p.tw_lds = lds <= kLdsBytes ? 1 : 0;
+    if (!p.tw_lds) lds = row_lds_bytes<R>(L, 1, false);
+    const int threads = std::max(64, round64(TL));
+    auto cpsi = reinterpret_cast<cx<double> *>(psi);
+    if (threads <= 256) {
+        set_lds(k_cheby_line<R, 256>, lds);
+        QDX_LAUNCH((k_cheby_line<R, 256>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, inv_scaling,
+                   prec, cpsi, ctl, partials);
+    } else {
+        set_lds(k_cheby_line<R, 512>, lds);
+        QDX_LAUNCH((k_cheby_line<R, 512>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, inv_scaling,
+                   prec, cpsi, ctl, partials);
+    }
+    QDX_HIP(hipGetLastError());
 }
 
 template <class R>
