@@ -264,6 +264,18 @@ int qdx_ctx_set_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, const int
 /* the plan in use for length L (radix: room for 16) */
 int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *radix, int32_t *nst);
 
+/* Points per thread of the Chebyshev propagators' line kernels on grids with
+ * two or more dimensions (the row kernel and the kinetic passes along
+ * dimensions >= 1), for steps taken afterwards: 8, 4, or 0 (the default) to
+ * choose 4 when the grid gives fewer than two waves per SIMD at 8 and its
+ * lines fit (states * N0 / 4 <= 512 threads, N_d <= 2048).  At 4 a line
+ * uses the context's plan for its length when that plan's first and last
+ * radix are 4 or 2, else first and last radix 4.  Both give the same
+ * transform up to rounding.  No QDng counterpart: FFTW lays out its own
+ * threads (src/fft/fft.cpp:95-110). */
+int qdx_ctx_set_line_points(qdx_ctx *ctx, int32_t points);
+int qdx_ctx_get_line_points(qdx_ctx *ctx, int32_t *points);
+
 /* Tile shape of the batched kernels for 2D Chebyshev jobs of one grid
  * shape.  Row tiles: row_lines lines of dimension 0 (N0/8 threads each) in a
  * block of row_threads (256 or 512) threads.  Column tiles: col_width

@@ -60,6 +60,7 @@ EXPORTS = [
     "qdx_ctx_set_radix_plan", "qdx_ctx_get_radix_plan", "qdx_batch_get_tiling", "qdx_batch_set_tiling",
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
+    "qdx_ctx_set_line_points", "qdx_ctx_get_line_points",
 ]
 
 K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
@@ -152,6 +153,8 @@ def lib():
         "qdx_batch_set_graphs": [vp, i32],
         "qdx_ctx_set_radix_plan": [vp, i32, i32, C.POINTER(i32), i32],
         "qdx_ctx_get_radix_plan": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)],
+        "qdx_ctx_set_line_points": [vp, i32],
+        "qdx_ctx_get_line_points": [vp, C.POINTER(i32)],
         "qdx_batch_get_tiling": [vp, i32, i32, C.POINTER(Tiling)],
         "qdx_batch_set_tiling": [vp, i32, i32, C.POINTER(Tiling)],
         "qdx_batch_autotune": [vp, i32, i32, C.POINTER(TuneResult)],
@@ -221,6 +224,17 @@ class Context:
         n = C.c_int32()
         _chk(lib().qdx_ctx_get_radix_plan(self.h, _PREC[precision], int(L), r, C.byref(n)))
         return list(r[:n.value])
+
+    def set_line_points(self, points: int) -> None:
+        """points per thread of the Chebyshev line kernels on grids with nd >= 2:
+        8, 4 or 0 to choose (qdx_ctx_set_line_points)"""
+        _chk(lib().qdx_ctx_set_line_points(self.h, int(points)))
+
+    @property
+    def line_points(self) -> int:
+        n = C.c_int32()
+        _chk(lib().qdx_ctx_get_line_points(self.h, C.byref(n)))
+        return n.value
 
     def kernel_time(self, kernel_class: int):
         """(launches, total milliseconds) of one kernel class since profile(True)."""

@@ -133,14 +133,15 @@ struct Bfly<8, R> {
 };
 
 // ---------------------------------------------------------------------------
-// Register-resident Stockham transform of one line of length L = 8*TL.
+// Register-resident Stockham transform of one line of length L = P*TL
+// (P = 8 or 4 points per thread).
 //
-// Thread lt (0 <= lt < TL) of a line owns elements lt + q*TL, q = 0..7, in
+// Thread lt (0 <= lt < TL) of a line owns elements lt + q*TL, q = 0..P-1, in
 // v[q].  In the Stockham formulation the first stage (span 1) reads exactly
 // those elements and the last stage (span L/r) writes exactly those elements,
 // so both run on registers; only the middle stages go through LDS, using two
 // buffers alternately (one barrier per stage).  The plan's first and last
-// radices are 8, 4 or 2; middle stages may also be radix 3.
+// radices are 8, 4 or 2 and divide P; middle stages may also be radix 3.
 //
 // LDS addressing of one line: element i of buffer a / b lives at
 // a[idx(i)] / b[idx(i)], idx(i) = (pad ? i + i/8 : i) * es.
@@ -162,10 +163,15 @@ __device__ __forceinline__ cx<R> twiddle(const cx<R> *tw, int q, int sign, int t
     return w;
 }
 
-template <int RR, class R>
+// butterflies of one middle stage per thread: ceil(P / RR), the last one
+// guarded by j < L / RR
+template <int RR, int P>
+constexpr int stage_bflies() { return (P + RR - 1) / RR; }
+
+template <int RR, class R, int P = 8>
 __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const LdsLine<R> &ln, int L, int Ns,
                                           int TL, int lt, int sign, const cx<R> *tw, int tsh) {
-    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
+    constexpr int Q = stage_bflies<RR, P>();
     const int m = L / RR;
     const int tstep = L / (Ns * RR);
     const bool pow2 = (Ns & (Ns - 1)) == 0;
@@ -193,10 +199,10 @@ __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const Lds
 // the block synchronises, then the outputs are written.  Called by all
 // threads of the block (it synchronises); threads without a line pass
 // active = false.
-template <int RR, class R>
+template <int RR, class R, int P = 8>
 __device__ __forceinline__ void mid_stage_ip(cx<R> *buf, const LdsLine<R> &ln, int L, int Ns, int TL, int lt,
                                              int sign, const cx<R> *tw, int tsh, bool active) {
-    constexpr int Q = RR == 2 ? 4 : RR == 3 ? 3 : RR == 4 ? 2 : 1;
+    constexpr int Q = stage_bflies<RR, P>();
     const int m = L / RR;
     cx<R> x[Q][RR];
     if (active) {
@@ -247,28 +253,32 @@ __host__ __device__ constexpr int tw_lds_len(int L) { return L + (L >> 3); }
 // IP: the middle stages run in place on buffer a (half the LDS, one more
 // barrier per middle stage); otherwise they alternate between a and b.
 // tw: the table exp(-2 pi i q / L), q in [0, L): the padded LDS copy made by
-// stage_twiddles (tsh = kTwLds; the barrier at the start of fft8 publishes
-// it) or the table in memory (tsh = kTwMem).
-template <class R, bool IP = false>
-__device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const LinePlan &p, const cx<R> *tw,
-                                     int sign, int lt, bool active, int tsh) {
-    const int L = p.L, TL = L >> 3;
-    if (p.nst == 1) {   // L == 8
-        if (active) Bfly<8, R>::run(v, sign);
+// stage_twiddles (tsh = kTwLds; the barrier at the start of fft_line
+// publishes it) or the table in memory (tsh = kTwMem).
+template <class R, int P, bool IP = false>
+__device__ __forceinline__ void fft_line(cx<R> (&v)[P], const LdsLine<R> &ln, const LinePlan &p, const cx<R> *tw,
+                                         int sign, int lt, bool active, int tsh) {
+    static_assert(P == 8 || P == 4, "8 or 4 points per thread");
+    const int L = p.L, TL = L / P;
+    if (p.nst == 1) {   // L == P
+        if (active) Bfly<P, R>::run(v, sign);
         return;
     }
     __syncthreads();   // previous readers of the LDS buffers are done
-    // first stage: span 1, no twiddles, registers -> buffer a
+    // first stage: span 1, no twiddles, registers -> buffer a.  Butterfly j =
+    // lt + q*TL (q < P/r) takes the thread's elements q + t*(P/r).
     const int r0 = p.rad[0];
     if (active) {
-        if (r0 == 8) {
-            Bfly<8, R>::run(v, sign);
+        if (P == 8 && r0 == 8) {
+            Bfly<P, R>::run(v, sign);
 #pragma unroll
-            for (int t = 0; t < 8; ++t) ln.a[ln.idx(lt * 8 + t)] = v[t];
+            for (int t = 0; t < P; ++t) ln.a[ln.idx(lt * P + t)] = v[t];
         } else if (r0 == 4) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                cx<R> x[4] = {v[q], v[q + 2], v[q + 4], v[q + 6]};
+            for (int q = 0; q < P / 4; ++q) {
+                cx<R> x[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x[t] = v[q + t * (P / 4)];
                 Bfly<4, R>::run(x, sign);
                 const int j = lt + q * TL;
 #pragma unroll
@@ -276,8 +286,8 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                cx<R> x[2] = {v[q], v[q + 4]};
+            for (int q = 0; q < P / 2; ++q) {
+                cx<R> x[2] = {v[q], v[q + P / 2]};
                 Bfly<2, R>::run(x, sign);
                 const int j = lt + q * TL;
 #pragma unroll
@@ -298,10 +308,10 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             int lv = lt;
             asm volatile("" : "+v"(lv));
             switch (r) {
-            case 8: mid_stage_ip<8, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
-            case 4: mid_stage_ip<4, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
-            case 2: mid_stage_ip<2, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
-            default: mid_stage_ip<3, R>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            case 8: mid_stage_ip<8, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            case 4: mid_stage_ip<4, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            case 2: mid_stage_ip<2, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            default: mid_stage_ip<3, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
             }
             __syncthreads();
             Ns *= r;
@@ -313,10 +323,10 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
         asm volatile("" : "+v"(lv));
         if (active) {
             switch (r) {
-            case 8: mid_stage<8, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
-            case 4: mid_stage<4, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
-            case 2: mid_stage<2, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
-            default: mid_stage<3, R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            case 8: mid_stage<8, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            case 4: mid_stage<4, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            case 2: mid_stage<2, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            default: mid_stage<3, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
             }
         }
         __syncthreads();
@@ -325,10 +335,11 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
         out = const_cast<cx<R> *>(tmp);
         Ns *= r;
     }
-    // last stage: span L/r, buffer -> registers in the owned layout
+    // last stage: span L/r, buffer -> registers in the owned layout.
+    // Butterfly j = lt + q*TL (q < P/r) yields the elements q + t*(P/r).
     const int rl = p.rad[p.nst - 1];
     if (!active) return;
-    if (rl == 8) {
+    if (P == 8 && rl == 8) {
         const int j = lt;
         cx<R> x[8];
 #pragma unroll
@@ -342,7 +353,7 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
         for (int t = 0; t < 8; ++t) v[t] = x[t];
     } else if (rl == 4) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < P / 4; ++q) {
             const int j = lt + q * TL;
             cx<R> x[4];
 #pragma unroll
@@ -353,11 +364,11 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             }
             Bfly<4, R>::run(x, sign);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) v[q + 2 * t] = x[t];
+            for (int t = 0; t < 4; ++t) v[q + (P / 4) * t] = x[t];
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < P / 2; ++q) {
             const int j = lt + q * TL;
             cx<R> x[2];
 #pragma unroll
@@ -365,9 +376,16 @@ __device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const 
             if (j != 0) x[1] = cmul(x[1], twiddle(tw, j, sign, tsh));
             Bfly<2, R>::run(x, sign);
             v[q] = x[0];
-            v[q + 4] = x[1];
+            v[q + P / 2] = x[1];
         }
     }
+}
+
+// the 8-points-per-thread transform of the batched kernels
+template <class R, bool IP = false>
+__device__ __forceinline__ void fft8(cx<R> (&v)[8], const LdsLine<R> &ln, const LinePlan &p, const cx<R> *tw,
+                                     int sign, int lt, bool active, int tsh) {
+    fft_line<R, 8, IP>(v, ln, p, tw, sign, lt, active, tsh);
 }
 
 // ---------------------------------------------------------------------------

@@ -50,6 +50,18 @@ qdx_wf *check_wf(qdx_wf *w) {
     return w;
 }
 
+// points per thread of the Chebyshev line kernels on this grid (see
+// qdx_ctx_set_line_points)
+int cheby_line_points(const qdx_ctx *ctx, const GridDev &g, int S) {
+    if (g.nd < 2 || ctx->line_ppt == 8) return 8;
+    if (S * (g.N[0] / 4) > 512) return 8;
+    for (int d = 1; d < g.nd; ++d)
+        if (g.N[d] > 2048) return 8;
+    if (ctx->line_ppt == 4) return 4;
+    const long waves8 = (long)S * g.size / (8 * 64);
+    return waves8 < 2 * 1024 ? 4 : 8;   // 256 CUs x 4 SIMDs
+}
+
 template <class R>
 void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     qdx_ctx *ctx = p->ctx;
@@ -62,7 +74,8 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     else h = H->dev.h64;
     H->set_fields(h, step);
     LinePlan plans[3];
-    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan(p->grid.N[d], fp32);
+    const int ppt = cheby_line_points(ctx, g, p->S);
+    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan_ppt(p->grid.N[d], fp32, ppt);
     R *A = p->phiA.as<R>(), *B = p->phiB.as<R>();
     R *X = p->grid.nd > 1 ? p->X.as<R>() : nullptr;
     ChebyCtl *ctl = p->ctl.as<ChebyCtl>();
@@ -356,6 +369,21 @@ int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *
         const std::vector<int> r = ctx->radix_of(L, precision == QDX_FP32_BULK);
         for (size_t i = 0; i < r.size(); ++i) radix[i] = r[i];
         *nst = (int32_t)r.size();
+    });
+}
+
+int qdx_ctx_set_line_points(qdx_ctx *ctx, int32_t points) {
+    return guarded([&] {
+        if (!ctx) fail(QDX_ERR_PARAM, "ctx is NULL");
+        if (points != 0 && points != 4 && points != 8) fail(QDX_ERR_PARAM, "points per thread: 0, 4 or 8");
+        ctx->line_ppt = points;
+    });
+}
+
+int qdx_ctx_get_line_points(qdx_ctx *ctx, int32_t *points) {
+    return guarded([&] {
+        if (!ctx || !points) fail(QDX_ERR_PARAM, "NULL argument");
+        *points = ctx->line_ppt;
     });
 }
 

@@ -176,6 +176,27 @@ inline std::vector<int> radix_plan(int L) {
     return r;
 }
 
+// Default plan of the 4-points-per-thread kernels: first and last radix 4
+// (2 where L has too few factors 2), radix-8, -4, -2 and -3 stages between.
+inline std::vector<int> radix_plan4(int L) {
+    int a = 0, b = 0, m = L;
+    while (m % 2 == 0) { ++a; m /= 2; }
+    while (m % 3 == 0) { ++b; m /= 3; }
+    if (m != 1 || a < 3) fail(QDX_ERR_PARAM, "line length must be 8 * 2^a * 3^b");
+    const int first = a >= 5 ? 4 : 2;
+    a -= first == 4 ? 2 : 1;
+    const int last = a >= 2 ? 4 : 2;
+    a -= last == 4 ? 2 : 1;
+    std::vector<int> r{first};
+    while (a >= 3) { r.push_back(8); a -= 3; }
+    if (a == 2) r.push_back(4);
+    if (a == 1) r.push_back(2);
+    for (int i = 0; i < b; ++i) r.push_back(3);
+    r.push_back(last);
+    if ((int)r.size() > kMaxStages) fail(QDX_ERR_PARAM, "too many radix stages");
+    return r;
+}
+
 // A radix sequence fft8 can run for length L: product L, first and last
 // radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 4, 3 or 2.
 inline bool radix_ok(int L, const std::vector<int> &r) {
@@ -268,6 +289,20 @@ struct qdx_ctx {
         return rt != radix.end() ? rt->second : radix_plan(L);
     }
     LinePlan plan(int L, bool fp32) { return plan(L, fp32, radix_of(L, fp32)); }
+    // plan of the kernels with ppt points per thread (8 or 4).  At 4 the
+    // first and last radix must be 4 or 2: the plan set for L when it has
+    // such edges, else radix_plan4.
+    LinePlan plan_ppt(int L, bool fp32, int ppt) {
+        if (ppt == 8) return plan(L, fp32);
+        std::vector<int> r = radix_of(L, fp32);
+        if (r.size() < 2 || r.front() > 4 || r.back() > 4) r = radix_plan4(L);
+        LinePlan p = plan(L, fp32, r);
+        p.ppt = 4;
+        return p;
+    }
+    // points per thread of the single propagator's Chebyshev line kernels:
+    // 0 chooses (4 on grids that give fewer than two waves per SIMD at 8)
+    int line_ppt = 0;
     LinePlan plan(int L, bool fp32, const std::vector<int> &r) {
         auto &tab = fp32 ? tw32 : tw64;
         auto it = tab.find(L);

@@ -24,6 +24,8 @@ struct LinePlan {
     int rad[kMaxStages] = {};
     const void *tw = nullptr;
     int tw_lds = 1;    // kernels stage the table in LDS (set by the launcher)
+    int ppt = 8;       // points per thread of the line kernels (8, or 4: first
+                       // and last radix 4 or 2)
 };
 
 // Grid as the kernels see it: N[d] points in dimension d, dimension 0

@@ -11,8 +11,10 @@
 // GSPO (OGSPO.cpp:114-139) uses the same line machinery with the x-space
 // split operators exp(-i tau W(x)) evaluated per point in registers.
 //
-// Thread layout of every line kernel: a line of L = 8*TL points is handled by
-// TL threads, thread lt owning points lt + q*TL, q = 0..7 (see fft8).
+// Thread layout of every line kernel: a line of L = P*TL points is handled by
+// TL threads, thread lt owning points lt + q*TL, q = 0..P-1 (see fft_line);
+// P = 8, or 4 for the Chebyshev row kernel and kinetic passes of grids too
+// small to give every SIMD two waves at 8 points per thread.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -47,8 +49,8 @@ __device__ __forceinline__ double kin_sum(const HamDev<R> &h, int st, const Grid
 }
 
 // Row-block geometry: rb rows of dimension 0 for each of S states; line
-// l = st * rb + rr.  Two padded LDS buffers per line.
-template <class R>
+// l = st * rb + rr; P points per thread.  Two padded LDS buffers per line.
+template <class R, int P = 8>
 struct RowBlock {
     int L, TL, rb, nl, line, lt, st, rr;
     long nrows, row;
@@ -59,7 +61,7 @@ struct RowBlock {
     bool twl;        // twiddle table staged in LDS (else read from the plan)
     __device__ RowBlock(const GridDev &g, int rb_, int S, cx<R> *smem, const LinePlan &p) {
         L = g.N[0];
-        TL = L >> 3;
+        TL = L / P;
         rb = rb_;
         nl = S * rb;
         nrows = g.size / L;
@@ -88,9 +90,9 @@ struct RowBlock {
         if (twl) stage_twiddles(tw, p);
     }
     // transform with the LDS or the global twiddle table; both calls inline
-    __device__ void fft(cx<R> (&v)[8], const LinePlan &p, int sign) {
-        if (twl) fft8<R>(v, ln, p, tw, sign, lt, active, kTwLds);
-        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active, kTwMem);
+    __device__ void fft(cx<R> (&v)[P], const LinePlan &p, int sign) {
+        if (twl) fft_line<R, P>(v, ln, p, tw, sign, lt, active, kTwLds);
+        else fft_line<R, P>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active, kTwMem);
     }
     __device__ long gidx(int q) const { return row * L + lt + q * TL; }
 };
@@ -142,8 +144,8 @@ __global__ void __launch_bounds__(512) k_cheby_init(GridDev g, LinePlan p0, int 
 // ---------------------------------------------------------------------------
 // Chebyshev row kernel (see file header).  PRO: prologue (order 1).
 // NT: the largest block the instance is launched with (256 leaves twice the
-// registers per thread of 512).
-template <class R, int S, bool ONE_D, bool PRO, int NT>
+// registers per thread of 512).  P: points per thread.
+template <class R, int S, bool ONE_D, bool PRO, int NT, int P = 8>
 __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord,
                                                     ConvCheck cc, int rb, const cx<R> *__restrict__ phim1,
                                                     cx<R> *__restrict__ phim2, cx<R> *__restrict__ X,
@@ -153,25 +155,25 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
     if (ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ double red[16];
-    RowBlock<R> B(g, rb, S, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    RowBlock<R, P> B(g, rb, S, reinterpret_cast<cx<R> *>(smem_raw), p0);
     const int st = B.st;
     const long base = (long)st * g.size;
 
     // the line of phi_{k-1} this thread transforms; the epilogue operands
     // (psi, phi_{k-2}, Y, phi_{k-1} of every state, W) are read after the
     // transforms, so that only the line stays live across them
-    cx<R> v[8];
+    cx<R> v[P];
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = phim1[base + B.gidx(q)];
+        for (int q = 0; q < P; ++q) v[q] = phim1[base + B.gidx(q)];
     }
     // this thread's row of the operator matrix (runtime state index, read once)
     const double *tk0 = h.tk[st][0];
     const bool has_t = tk0 != nullptr;
-    double tq[8];
+    double tq[P];
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_nd[0] : 0.0;
+        for (int q = 0; q < P; ++q) tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_nd[0] : 0.0;
     }
     B.stage(p0);
     if (PRO) {
@@ -210,9 +212,9 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
     }
     const double uni = h.unity[st];
     // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j,
-    // in two halves of four points.  The first half's operands are requested
+    // in chunks of four points.  The first chunk's operands are requested
     // before the transforms (their latency hides behind them), the second
-    // half's right after.
+    // (P = 8) right after.
     const int nterm = PRO ? 1 : ord.nterm;
     const bool upd = nterm > 0;
     struct Half {
@@ -285,13 +287,13 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
     B.fft(v, p0, -1);
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+        for (int q = 0; q < P; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
     }
     B.fft(v, p0, +1);
     if (B.active) {
-        load_half(H1, 4);
+        if constexpr (P == 8) load_half(H1, 4);
         finish_half(H0, 0);
-        finish_half(H1, 4);
+        if constexpr (P == 8) finish_half(H1, 4);
     }
     if (PRO) return;
     // partial sum |psi|^2 of this block; the next kernels test convergence
@@ -440,7 +442,7 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
 //      3: forward * exp(-i tau T)/N inverse,
 //      4: X = IFFT(T_dim/N_dim FFT(src)), the dimension's part of the
 //         separable kinetic operator, 5: X += the same.
-template <class R, int MODE>
+template <class R, int MODE, int P = 8>
 __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
                                               cx<R> *__restrict__ X, const cx<R> *__restrict__ src,
                                               ChebyCtl *ctl, ConvCheck cc) {
@@ -450,7 +452,7 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     __shared__ double bc;
     const int N0 = g.N[0];
     const int L = g.N[dim];
-    const int TL = L >> 3;
+    const int TL = L / P;
     const long estride = dim == 1 ? N0 : (long)N0 * g.N[1];
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     const long ostride = dim == 1 ? (long)N0 * g.N[1] : N0;
@@ -470,29 +472,29 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     ln.es = W;
     ln.pad = true;
     cx<R> *twl = smem + (long)2 * W * tw_lds_len(L);
-    cx<R> v[8], yv[8];
+    cx<R> v[P], yv[P];
     const bool tw_in_lds = p.tw_lds;
     auto FFT = [&](int sign) {
-        if (tw_in_lds) fft8<R>(v, ln, p, twl, sign, lt, active, kTwLds);
-        else fft8<R>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active, kTwMem);
+        if (tw_in_lds) fft_line<R, P>(v, ln, p, twl, sign, lt, active, kTwLds);
+        else fft_line<R, P>(v, ln, p, reinterpret_cast<const cx<R> *>(p.tw), sign, lt, active, kTwMem);
     };
 
     if (active) {
         const cx<R> *in = MODE >= 4 ? src + (base - X) : base;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = in[(long)(lt + q * TL) * estride];
+        for (int q = 0; q < P; ++q) v[q] = in[(long)(lt + q * TL) * estride];
         if (MODE == 5) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) yv[q] = base[(long)(lt + q * TL) * estride];
+            for (int q = 0; q < P; ++q) yv[q] = base[(long)(lt + q * TL) * estride];
         }
     }
     // kinetic table values of this thread's elements, requested early
-    double tq[8];
+    double tq[P];
     if (MODE >= 2 && active) {
         const bool has_t = h.tk[st][0] != nullptr;
         const int k0 = c0 + c;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < P; ++q) {
             const int kd = lt + q * TL;
             if (MODE >= 4) tq[q] = has_t ? (0.0 + h.tk[st][dim][kd]) * h.inv_nd[dim] : 0.0;
             else tq[q] = has_t ? kin_sum<R>(h, st, g, k0, dim == 1 ? kd : oth, dim == 2 ? kd : oth) : 0.0;
@@ -516,7 +518,7 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     if (MODE >= 2) {
         if (active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < P; ++q) {
                 const double tt = tq[q];
                 if (MODE >= 4) {
                     v[q] = mk<R>((R)((double)v[q].r * tt), (R)((double)v[q].i * tt));
@@ -536,10 +538,10 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     if (active) {
         if (MODE == 5) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = v[q] + yv[q];
+            for (int q = 0; q < P; ++q) v[q] = v[q] + yv[q];
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * estride] = v[q];
+        for (int q = 0; q < P; ++q) base[(long)(lt + q * TL) * estride] = v[q];
     }
 }
 
@@ -826,10 +828,10 @@ static int round64(int n) {
 }
 
 // rows per block: grow the block towards 256 threads while the grid keeps at
-// least 512 blocks
-static int rows_per_block(const GridDev &g, int S, int L) {
+// least 512 blocks (P points per thread)
+static int rows_per_block(const GridDev &g, int S, int L, int P = 8) {
     const long nrows = g.size / L;
-    const int TL = L / 8;
+    const int TL = L / P;
     int rb = 1;
     while (rb * 2 <= nrows && S * rb * 2 * TL <= 256 && nrows / (rb * 2) >= 512) rb *= 2;
     return rb;
@@ -852,11 +854,11 @@ static size_t col_lds_bytes(int W, int L, size_t elem) {
 
 // tile width of the strided passes: adjacent dimension-0 positions per block.
 // Prefer >= 64-byte segments, keep >= 256 blocks when possible.
-static int col_tile(const GridDev &g, int S, int dim, int L, size_t elem) {
+static int col_tile(const GridDev &g, int S, int dim, int L, size_t elem, int P = 8) {
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     int W = 16;
     auto fits = [&](int w) {
-        return g.N[0] % w == 0 && col_lds_bytes(w, L, elem) <= 96 * 1024 && w * (L / 8) <= 512;
+        return g.N[0] % w == 0 && col_lds_bytes(w, L, elem) <= 96 * 1024 && w * (L / P) <= 512;
     };
     while (W > 1 && !fits(W)) W /= 2;
     const int wmin = std::max(1, (int)(64 / elem));
@@ -873,10 +875,10 @@ template <class R>
 static RowLaunch row_launch(const GridDev &g, int S, LinePlan &p0, bool extra_exchange = false) {
     const int L = g.N[0];
     RowLaunch r;
-    r.rb = rows_per_block(g, S, L);
+    r.rb = rows_per_block(g, S, L, p0.ppt);
     const long nrows = g.size / L;
     r.nb = (int)((nrows + r.rb - 1) / r.rb);
-    r.threads = std::max(64, round64(S * r.rb * (L / 8)));
+    r.threads = std::max(64, round64(S * r.rb * (L / p0.ppt)));
     const size_t extra = extra_exchange ? (size_t)S * r.rb * L * sizeof(cx<R>) : 0;
     // keep the twiddle table in LDS when it fits next to the line buffers
     p0.tw_lds = row_lds_bytes<R>(L, S * r.rb, true) + extra <= kLdsBytes ? 1 : 0;
@@ -888,6 +890,7 @@ template <class R, int S>
 static int cheby_init_S(hipStream_t st, const GridDev &g, const LinePlan &plan, const double *psi, R *phi0,
                         R *X, ChebyCtl *ctl, double *partials) {
     LinePlan p0 = plan;
+    p0.ppt = 8;   // the init kernel holds 8 points per thread
     RowLaunch r = row_launch<R>(g, S, p0);
     if (r.nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
     auto cpsi = reinterpret_cast<const cx<double> *>(psi);
@@ -922,10 +925,10 @@ int Launch<R>::cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan
     return nb;
 }
 
-template <class R, int S, bool ONE_D, bool PRO>
-static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &plan,
-                        const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
-                        ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
+template <class R, int S, bool ONE_D, bool PRO, int P>
+static int cheby_row_SKP(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &plan,
+                         const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                         ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
     LinePlan p0 = plan;
     RowLaunch r = row_launch<R>(g, S, p0);
     if (r.nb > kMaxPartials) throw std::runtime_error("grid too large for the partial-sum buffer");
@@ -934,16 +937,32 @@ static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, co
     auto cX = reinterpret_cast<cx<R> *>(X);
     auto cpsi = reinterpret_cast<cx<double> *>(psi);
     if (r.threads <= 256) {
-        set_lds(k_cheby_row<R, S, ONE_D, PRO, 256>, r.lds);
-        QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 256>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
-                   r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
+        set_lds(k_cheby_row<R, S, ONE_D, PRO, 256, P>, r.lds);
+        QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 256, P>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord,
+                   cc, r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
     } else {
-        set_lds(k_cheby_row<R, S, ONE_D, PRO, 512>, r.lds);
-        QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 512>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord, cc,
-                   r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
+        set_lds(k_cheby_row<R, S, ONE_D, PRO, 512, P>, r.lds);
+        QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 512, P>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord,
+                   cc, r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
     }
     QDX_HIP(hipGetLastError());
     return r.nb;
+}
+
+// 4 points per thread only on grids with nd >= 2 (single-state 1D lines run
+// the whole-step kernel)
+template <class R, int S, bool ONE_D, bool PRO>
+static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &plan,
+                        const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                        ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
+    if (plan.ppt == 4) {
+        if constexpr (!ONE_D)
+            return cheby_row_SKP<R, S, ONE_D, PRO, 4>(st, g, h, plan, ord, cc, phim1, phim2, X, psi, ctl, partials,
+                                                      init_part, nb_init);
+        throw std::runtime_error("4 points per thread: grids with nd >= 2 only");
+    }
+    return cheby_row_SKP<R, S, ONE_D, PRO, 8>(st, g, h, plan, ord, cc, phim1, phim2, X, psi, ctl, partials, init_part,
+                                              nb_init);
 }
 
 template <class R, int S>
@@ -1027,27 +1046,34 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     QDX_HIP(hipGetLastError());
 }
 
-template <class R>
-void Launch<R>::kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate, const LinePlan &plan,
-                         const HamDev<R> &h, const R *src, R *Y, ChebyCtl *ctl, const ConvCheck &cc) {
+template <class R, int P>
+static void kin_pass_P(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate, const LinePlan &plan,
+                       const HamDev<R> &h, const R *src, R *Y, ChebyCtl *ctl, const ConvCheck &cc) {
     const int L = g.N[dim];
-    const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
+    const int W = col_tile(g, S, dim, L, sizeof(cx<R>), P);
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
     const int nb = (g.N[0] / W) * nother * S;
-    const int threads = std::max(64, round64(W * (L / 8)));
+    const int threads = std::max(64, round64(W * (L / P)));
     const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *y = reinterpret_cast<cx<R> *>(Y);
     const cx<R> *in = reinterpret_cast<const cx<R> *>(src);
     LinePlan pl = plan;
     pl.tw_lds = 1;
     if (accumulate) {
-        set_lds(k_col<R, 5>, lds);
-        QDX_LAUNCH((k_col<R, 5>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+        set_lds(k_col<R, 5, P>, lds);
+        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
     } else {
-        set_lds(k_col<R, 4>, lds);
-        QDX_LAUNCH((k_col<R, 4>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+        set_lds(k_col<R, 4, P>, lds);
+        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
     }
     QDX_HIP(hipGetLastError());
+}
+
+template <class R>
+void Launch<R>::kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate, const LinePlan &plan,
+                         const HamDev<R> &h, const R *src, R *Y, ChebyCtl *ctl, const ConvCheck &cc) {
+    if (plan.ppt == 4) kin_pass_P<R, 4>(s, g, S, dim, accumulate, plan, h, src, Y, ctl, cc);
+    else kin_pass_P<R, 8>(s, g, S, dim, accumulate, plan, h, src, Y, ctl, cc);
 }
 
 template <class R, int S>

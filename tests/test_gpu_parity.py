@@ -190,6 +190,41 @@ def test_cheby_fp32_bulk_accuracy(ctx, name, builder, dt):
     assert rel(got, ref) < 1e-6
 
 
+# 8 and 4 points per thread of the Chebyshev line kernels on grids with
+# nd >= 2 (the automatic choice takes 4 on grids this small).  Plans at 4:
+# 96 -> [4, 2, 3, 4] by default, a set plan with edges 4/2 is used as set,
+# one with an edge 8 gives way to the default; 24 -> [2, 3, 4].
+@pytest.mark.parametrize("points", [4, 8])
+@pytest.mark.parametrize("name,builder,dt,precision,plan", [
+    ("morse_2state_2d", lambda: cases.morse_2state_2d(96), 5.0, "fp64", None),
+    ("morse_2state_2d_plan42", lambda: cases.morse_2state_2d(96), 5.0, "fp64", [4, 3, 4, 2]),
+    ("morse_2state_2d_plan8", lambda: cases.morse_2state_2d(96), 5.0, "fp64", [8, 3, 4]),
+    ("anharmonic_3d", lambda: cases.anharmonic_3d(24), 3.5, "fp64", None),
+    ("morse_2state_2d_fp32", lambda: cases.morse_2state_2d(128), 5.0, "fp32", None),
+])
+def test_cheby_line_points(name, builder, dt, precision, plan, points):
+    note(f"cheby {name} points {points}")
+    import qdng_amd as Q
+    c = Q.Context(0)
+    try:
+        assert c.line_points == 0
+        c.set_line_points(points)
+        assert c.line_points == points
+        p = builder()
+        if plan:
+            c.set_radix_plan(p.N[0], plan, precision)
+        got, ref, counts = cheby_pair(c, p, dt, 1e-10, precision, 3)
+        if precision == "fp64":
+            assert rel(got, ref) < 1e-10
+            assert all(a == b for a, b in counts), counts
+        else:
+            assert rel(got, ref) < 1e-6
+        with pytest.raises(Q.QdxError):
+            c.set_line_points(5)
+    finally:
+        c.close()
+
+
 def test_run_matches_stepwise_and_oracle_observables(ctx):
     note("prop_run observables")
     import qdng_amd as Q
