@@ -34,7 +34,8 @@ typedef enum qdx_status {
     QDX_ERR_OVERFLOW = -3,    /* QDng EOverflow (tools/Exception.h:148-154)      */
     QDX_ERR_MEMORY = -4,      /* QDng EMemError (tools/Exception.h:76-84)        */
     QDX_ERR_DEVICE = -5,      /* HIP runtime failure                            */
-    QDX_ERR_STATE = -6        /* call order violated (Operator.h:48-58)          */
+    QDX_ERR_STATE = -6,       /* call order violated (Operator.h:48-58)          */
+    QDX_ERR_IO = -7           /* QDng EIOError (tools/Exception.h:86-92)         */
 } qdx_status;
 
 typedef enum qdx_precision {

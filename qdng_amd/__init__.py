@@ -1,6 +1,6 @@
 """qdng_amd: MI355X grid-propagation path of QDng behind a C ABI (libqdx.so).
 
-This module is a thin ctypes binding of include/qdx.h.  Everything numeric
+This module is a thin ctypes binding of include/qdx.h and include/qdx_io.h.  Everything numeric
 runs in libqdx.so on the GPU; there is no CPU fallback: if the library is
 missing or no HIP device is present, the calls raise.
 
@@ -61,6 +61,12 @@ EXPORTS = [
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
     "qdx_ctx_set_line_points", "qdx_ctx_get_line_points",
+    # include/qdx_io.h
+    "qdx_meta_create", "qdx_meta_read", "qdx_meta_write", "qdx_meta_size", "qdx_meta_key", "qdx_meta_get",
+    "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
+    "qdx_file_read", "qdx_file_write", "qdx_gridop_load", "qdx_gridop_save", "qdx_laser_load",
+    "qdx_laser_save", "qdx_propagation_meta", "qdx_wf_load", "qdx_wf_save", "qdx_ham_add_potential_file",
+    "qdx_ham_add_dipole_file",
 ]
 
 K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
@@ -166,6 +172,29 @@ def lib():
         "qdx_batch_tuning_load": [vp, C.c_char_p],
         "qdx_batch_graph_stats": [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "qdx_batch_destroy": [vp],
+        "qdx_meta_create": [C.POINTER(vp)],
+        "qdx_meta_read": [C.c_char_p, C.POINTER(vp)],
+        "qdx_meta_write": [vp, C.c_char_p],
+        "qdx_meta_size": [vp, C.POINTER(i32)],
+        "qdx_meta_key": [vp, i32, C.c_char_p, i32],
+        "qdx_meta_get": [vp, C.c_char_p, C.c_char_p, i32],
+        "qdx_meta_set": [vp, C.c_char_p, C.c_char_p],
+        "qdx_meta_set_int": [vp, C.c_char_p, C.c_int64],
+        "qdx_meta_set_double": [vp, C.c_char_p, C.c_double],
+        "qdx_meta_destroy": [vp],
+        "qdx_file_meta": [C.c_char_p, C.POINTER(vp)],
+        "qdx_file_read": [C.c_char_p, C.c_char_p, i32, vp, C.c_int64],
+        "qdx_file_write": [C.c_char_p, C.c_char_p, i32, vp, vp, C.c_int64],
+        "qdx_gridop_load": [C.c_char_p, i32, C.c_double, C.c_double, C.POINTER(Grid), dp, C.c_int64,
+                            C.POINTER(C.c_int64)],
+        "qdx_gridop_save": [C.c_char_p, C.POINTER(Grid), dp],
+        "qdx_laser_load": [C.c_char_p, dp, C.POINTER(C.c_int64), dp, C.c_int64],
+        "qdx_laser_save": [C.c_char_p, C.c_double, C.c_int64, dp],
+        "qdx_propagation_meta": [C.c_char_p, C.c_int64, C.c_double, C.c_char_p, i32],
+        "qdx_wf_load": [vp, C.c_char_p, i32, C.POINTER(vp)],
+        "qdx_wf_save": [vp, C.c_char_p, i32],
+        "qdx_ham_add_potential_file": [vp, i32, i32, C.c_char_p, i32, C.c_double, C.c_double],
+        "qdx_ham_add_dipole_file": [vp, i32, i32, C.c_char_p, C.c_char_p, C.c_double],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -182,6 +211,132 @@ def _chk(rc: int) -> None:
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _enc(s) -> bytes:
+    return os.fsencode(s)
+
+
+# ---------------------------------------------------------------------------
+# QDng files (include/qdx_io.h): host-only, no device needed
+
+def _meta_to_dict(m) -> dict:
+    n = C.c_int32()
+    _chk(lib().qdx_meta_size(m, C.byref(n)))
+    out = {}
+    buf = C.create_string_buffer(4096)
+    for i in range(n.value):
+        _chk(lib().qdx_meta_key(m, i, buf, len(buf)))
+        k = buf.value
+        _chk(lib().qdx_meta_get(m, k, buf, len(buf)))
+        out[k.decode()] = buf.value.decode()
+    return out
+
+
+def read_meta(path: str) -> dict:
+    """key -> value of a QDng .meta file, in key order (qdx_meta_read)"""
+    m = C.c_void_p()
+    _chk(lib().qdx_meta_read(_enc(path), C.byref(m)))
+    try:
+        return _meta_to_dict(m)
+    finally:
+        lib().qdx_meta_destroy(m)
+
+
+def file_meta(name: str) -> dict:
+    """meta data of the file name (name.meta, else the name cut at its last '_'; qdx_file_meta)"""
+    m = C.c_void_p()
+    _chk(lib().qdx_file_meta(_enc(name), C.byref(m)))
+    try:
+        return _meta_to_dict(m)
+    finally:
+        lib().qdx_meta_destroy(m)
+
+
+def _dict_to_meta(items: dict):
+    m = C.c_void_p()
+    _chk(lib().qdx_meta_create(C.byref(m)))
+    try:
+        for k, v in items.items():
+            if isinstance(v, bool) or isinstance(v, str):
+                _chk(lib().qdx_meta_set(m, k.encode(), str(v).encode()))
+            elif isinstance(v, (int, np.integer)):
+                _chk(lib().qdx_meta_set_int(m, k.encode(), int(v)))
+            else:
+                _chk(lib().qdx_meta_set_double(m, k.encode(), float(v)))
+    except Exception:
+        lib().qdx_meta_destroy(m)
+        raise
+    return m
+
+
+def write_meta(path: str, items: dict) -> None:
+    """write key = value lines in key order (qdx_meta_write); floats with 6 significant digits"""
+    m = _dict_to_meta(items)
+    try:
+        _chk(lib().qdx_meta_write(m, _enc(path)))
+    finally:
+        lib().qdx_meta_destroy(m)
+
+
+def read_file(name: str, suffix: str, count: int, dtype=np.complex128, counter: int = -1) -> np.ndarray:
+    """count values of a raw data file name[_counter]suffix (qdx_file_read)"""
+    out = np.zeros(int(count), dtype=dtype)
+    _chk(lib().qdx_file_read(_enc(name), suffix.encode(), int(counter), out.ctypes.data_as(C.c_void_p),
+                             out.nbytes))
+    return out
+
+
+def write_file(name: str, suffix: str, data: np.ndarray, meta: dict | None = None, counter: int = -1) -> None:
+    """raw data file name[_counter]suffix and, with meta, name.meta (qdx_file_write)"""
+    a = np.ascontiguousarray(data)
+    m = _dict_to_meta(meta) if meta is not None else None
+    try:
+        _chk(lib().qdx_file_write(_enc(name), suffix.encode(), int(counter), m, a.ctypes.data_as(C.c_void_p),
+                                  a.nbytes))
+    finally:
+        if m is not None:
+            lib().qdx_meta_destroy(m)
+
+
+def load_gridop(name: str, setzero: bool = False, offset: float = 0.0, scale: float = 1.0):
+    """(N, xmin, xmax, values) of a QDng grid operator file (qdx_gridop_load)"""
+    g = Grid()
+    n = C.c_int64()
+    _chk(lib().qdx_gridop_load(_enc(name), 0, 0.0, 1.0, C.byref(g), None, 0, C.byref(n)))
+    v = np.zeros(n.value)
+    _chk(lib().qdx_gridop_load(_enc(name), int(bool(setzero)), float(offset), float(scale), C.byref(g), _dp(v),
+                               v.size, C.byref(n)))
+    nd = g.ndims
+    return list(g.N[:nd]), list(g.xmin[:nd]), list(g.xmax[:nd]), v
+
+
+def save_gridop(name: str, N, xmin, xmax, values) -> None:
+    g = make_grid(N, xmin, xmax)
+    v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    if v.size != int(np.prod(N)):
+        raise ValueError("values do not match the grid")
+    _chk(lib().qdx_gridop_save(_enc(name), C.byref(g), _dp(v)))
+
+
+def load_laser(name: str):
+    """(dt, field) of a QDng laser file (qdx_laser_load)"""
+    dt = C.c_double()
+    nt = C.c_int64()
+    _chk(lib().qdx_laser_load(_enc(name), C.byref(dt), C.byref(nt), None, 0))
+    v = np.zeros(nt.value)
+    _chk(lib().qdx_laser_load(_enc(name), C.byref(dt), C.byref(nt), _dp(v), v.size))
+    return dt.value, v
+
+
+def save_laser(name: str, dt: float, field) -> None:
+    v = np.ascontiguousarray(field, dtype=np.float64).ravel()
+    _chk(lib().qdx_laser_save(_enc(name), float(dt), v.size, _dp(v)))
+
+
+def write_propagation_meta(directory: str, nt: int, dt: float, wf_base: str = "WF", wcycle: int = 1) -> None:
+    """Propagation.meta as ProgPropa writes it (qdx_propagation_meta); directory '' or ending in '/'"""
+    _chk(lib().qdx_propagation_meta(_enc(directory), int(nt), float(dt), wf_base.encode(), int(wcycle)))
 
 
 def make_grid(N, xmin, xmax) -> Grid:
@@ -301,6 +456,30 @@ class WaveFunction:
     def normalize(self) -> None:
         _chk(lib().qdx_wf_normalize(self.h))
 
+    @classmethod
+    def load(cls, ctx: Context, name: str, counter: int = -1) -> "WaveFunction":
+        """Wave function from QDng files name[_counter].wf + name.meta
+        (WFGridCartesian) or name-<i>... (WFMultistate); qdx_wf_load."""
+        h = C.c_void_p()
+        _chk(lib().qdx_wf_load(ctx.h, _enc(name), int(counter), C.byref(h)))
+        m = file_meta(name)
+        S = int(m["states"]) if m.get("CLASS") == "WFMultistate" else 1
+        g = file_meta(name + "-0") if S > 1 or m.get("CLASS") == "WFMultistate" else m
+        nd = int(g["dims"])
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.N = [int(g[f"N{d}"]) for d in range(nd)]
+        self.grid = make_grid(self.N, [float(g[f"xmin{d}"]) for d in range(nd)],
+                              [float(g[f"xmax{d}"]) for d in range(nd)])
+        self.S = S
+        self.h = h
+        self._destroy = lib().qdx_wf_destroy
+        return self
+
+    def save(self, name: str, counter: int = -1) -> None:
+        """Write in QDng's binary format (qdx_wf_save)."""
+        _chk(lib().qdx_wf_save(self.h, _enc(name), int(counter)))
+
     def __del__(self):
         try:
             if self.h:
@@ -346,6 +525,20 @@ class Hamiltonian:
         if m.size != self.size:
             raise ValueError("dipole size does not match the grid")
         _chk(lib().qdx_ham_add_dipole(self.h, int(row), int(col), _dp(m), _dp(e), e.size, float(scale)))
+        return self
+
+    def add_potential_file(self, row: int, col: int, name: str, setzero: bool = False, offset: float = 0.0,
+                           scale: float = 1.0) -> "Hamiltonian":
+        """OGridPotential from QDng files name.op + name.meta (qdx_ham_add_potential_file)"""
+        _chk(lib().qdx_ham_add_potential_file(self.h, int(row), int(col), _enc(name), int(bool(setzero)),
+                                              float(offset), float(scale)))
+        return self
+
+    def add_dipole_file(self, row: int, col: int, mu_name: str, laser_name: str,
+                        scale: float = 1.0) -> "Hamiltonian":
+        """OGridDipole + laser from QDng files (qdx_ham_add_dipole_file)"""
+        _chk(lib().qdx_ham_add_dipole_file(self.h, int(row), int(col), _enc(mu_name), _enc(laser_name),
+                                           float(scale)))
         return self
 
     def bounds(self):

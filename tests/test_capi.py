@@ -1,5 +1,5 @@
 """CPU checks of libqdx.so: it loads, exports every entry point of
-include/qdx.h, and its host-side rules (QDng bounds, Chebyshev order and
+include/qdx.h and include/qdx_io.h, and its host-side rules (QDng bounds, Chebyshev order and
 coefficients, Bessel values, argument checks) agree with the oracle.  No
 device call is made here."""
 import ctypes as C
@@ -17,8 +17,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_symbols():
-    text = open(os.path.join(ROOT, "include", "qdx.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(qdx_\w+)\s*\(", text, re.M)))
+    syms = set()
+    for h in ("qdx.h", "qdx_io.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        syms |= set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(qdx_\w+)\s*\(", text, re.M))
+    return sorted(syms)
 
 
 def test_library_exports_every_declared_symbol():
