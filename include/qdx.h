@@ -99,6 +99,8 @@ int qdx_wf_norm(qdx_wf *wf, double *norm, double *per_state_pop);
 int qdx_wf_overlap(qdx_wf *a, qdx_wf *b, double out[2]);
 /* psi *= 1/Norm() (WaveFunction.h:103-106) */
 int qdx_wf_normalize(qdx_wf *wf);
+/* y += (re + i im) x (cVec MultElementsAdd; same grid and states) */
+int qdx_wf_axpy(qdx_wf *y, double re, double im, const qdx_wf *x);
 int qdx_wf_destroy(qdx_wf *wf);
 
 /* ---- Hamiltonian ----------------------------------------------------------
@@ -132,6 +134,8 @@ int qdx_ham_destroy(qdx_ham *h);
 /* BesselJ0 (linalg/Bessel.cpp:22-58): J_k(x), k = 0..n-1, and the number of
  * trailing values zeroed by underflow. */
 int qdx_bessel_j0(int32_t n, double x, double *out, int32_t *nz);
+/* BesselI0 (linalg/Bessel.cpp:61-98, ZBESI kode 2): I_k(x) exp(-x), x >= 0 */
+int qdx_bessel_i0(int32_t n, double x, double *out, int32_t *nz);
 
 /* ---- propagators --------------------------------------------------------- */
 typedef struct qdx_cheby_info {
@@ -149,7 +153,28 @@ typedef struct qdx_cheby_info {
  * propagator. */
 int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double scaling,
                      int32_t precision, qdx_prop **out);
+/* OCheby in imaginary time, backward (exponent -dt), as ProgEigen sets it up
+ * (ImaginaryTime() + Backward(), ProgEigen.cpp:691-694; OPropagator.h:82-108):
+ * offset Emin - 0.05 |Emin|, scaling 1.05 |Emax - Emin|, coefficients from
+ * I_k(R dt) exp(-R dt) (OCheby.cpp:406-446), the recurrence and relative
+ * break test of imag_time_step (OCheby.cpp:200-297).  A step maps psi to
+ * exp(-R dt) exp(-H dt) psi (the series' constant factor; normalise).  FP64
+ * only: the series cancels by exp(-R dt), and rounding grows accordingly. */
+int qdx_cheby_create_imag(qdx_ham *h, double dt, int32_t order, double prec, double scaling, qdx_prop **out);
 int qdx_cheby_info_get(qdx_prop *p, qdx_cheby_info *info);
+/* Eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF,
+ * ProgEigen.cpp:318-412): for i < nef, psi = psi0; repeat at most max_steps
+ * times while |1 - <psi_old|psi>| > conv: one step of U (from
+ * qdx_cheby_create_imag), remove the states found so far (OProjection,
+ * psi -= sum_j ef_j <ef_j|psi>), normalise; the raw energy is
+ * <ef_i|H|ef_i>.  diag != 0 (QDng's default, DiagBasis, ProgEigen.cpp:93-138):
+ * H is diagonalised in the found basis and the returned states and energies
+ * (ascending) are those of the diagonalised basis.  QDng's defaults: conv
+ * 1e-5 with diag, 1e-8 without; max_steps 512 (ProgEigen.h:15-18).
+ * efs receives nef new wave functions (the caller destroys them); energies
+ * nef values; energies_raw and steps (nullable) the relaxation's own. */
+int qdx_eigen_imag(qdx_prop *U, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv, int32_t diag,
+                   qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps);
 /* The scalars and coefficients OCheby::Init would choose for this Hamiltonian,
  * without allocating device memory (h may have been created with ctx NULL).
  * coeff (nullable) receives min(coeff_len, order) values. */

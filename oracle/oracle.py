@@ -52,6 +52,9 @@ def lib():
             "orc_set_threads": [C.c_int],
             "orc_fft": [C.c_int, i32p, dp, C.c_int],
             "orc_bessel_j": [C.c_int, C.c_double, dp, ip],
+            "orc_bessel_i_scaled": [C.c_int, C.c_double, dp, ip],
+            "orc_cheby_create_imag": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                      C.c_double, C.POINTER(ChebyInfo)],
             "orc_grid": [C.c_int, i32p, dp, dp],
             "orc_op_kinetic": [C.c_int, dp],
             "orc_op_potential": [C.c_int, dp],
@@ -118,6 +121,14 @@ def bessel_j(n: int, x: float):
     out = np.zeros(n)
     nz = C.c_int(0)
     _chk(lib().orc_bessel_j(int(n), float(x), _dp(out), C.byref(nz)))
+    return out, nz.value
+
+
+def bessel_i_scaled(n: int, x: float):
+    """I_k(x) exp(-x), k < n, and the emulated underflow count"""
+    out = np.zeros(n)
+    nz = C.c_int(0)
+    _chk(lib().orc_bessel_i_scaled(int(n), float(x), _dp(out), C.byref(nz)))
     return out, nz.value
 
 
@@ -223,12 +234,15 @@ def expec_momentum(grid: Grid, psi: np.ndarray, dim: int = 0) -> float:
 
 
 class Cheby:
-    """Real-time Chebyshev propagator (QDng OCheby, forward)."""
+    """Chebyshev propagator (QDng OCheby): real time forward, or imaginary
+    time backward (imag=True, exponent -dt)."""
 
-    def __init__(self, H: Op, dt: float, order: int = 0, prec: float = -1.0, scaling: float = 0.0):
+    def __init__(self, H: Op, dt: float, order: int = 0, prec: float = -1.0, scaling: float = 0.0,
+                 imag: bool = False):
         info = ChebyInfo()
-        self.id = _chk(lib().orc_cheby_create(H.id, H.nstates, float(dt), int(order), float(prec),
-                                              float(scaling), C.byref(info)))
+        create = lib().orc_cheby_create_imag if imag else lib().orc_cheby_create
+        self.id = _chk(create(H.id, H.nstates, float(dt), int(order), float(prec), float(scaling),
+                              C.byref(info)))
         self.info = info
         self.H = H
         self.order = info.order

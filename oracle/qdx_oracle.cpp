@@ -181,6 +181,27 @@ int bessel_j(int n, double x, std::vector<double> &out) {
 }
 
 /* ======================================================================
+ * Modified Bessel I_k(x) exp(-x), k = 0..n-1 (QDng src/linalg/Bessel.cpp:
+ * 61-98 calls SLATEC ZBESI with kode = 2, scaled by exp(-x)).  Values come
+ * from libstdc++'s std::cyl_bessel_i; the underflow count NZ is emulated as
+ * for bessel_j.
+ * ==================================================================== */
+int bessel_i_scaled(int n, double x, std::vector<double> &out) {
+    out.assign(n, 0.0);
+    if (x > 700) throw OracleError("bessel_i_scaled: argument beyond the oracle's range");
+    const double ex = std::exp(-x);
+    for (int k = 0; k < n; ++k) out[k] = std::cyl_bessel_i((double)k, x) * ex;
+    const double elim = 2.303 * (1021.0 * 0.30102999566398120 - 3.0);
+    const double ulim = std::exp(-elim);
+    int nz = 0;
+    for (int k = n - 1; k >= 0 && std::fabs(out[k]) < ulim; --k) {
+        out[k] = 0.0;
+        ++nz;
+    }
+    return nz;
+}
+
+/* ======================================================================
  * Grid (src/qdlib/GridSystem.cpp:207-211: dx = (xmax-xmin)/(N-1))
  * ==================================================================== */
 struct Grid {
@@ -472,6 +493,7 @@ struct Cheby {
     double scaling = 1;
     std::vector<double> coeff;
     double emin = 0, emax = 0;
+    bool imag = false;          /* imaginary time, backward: exponent -dt */
 };
 std::vector<Cheby> g_cheby;
 
@@ -482,23 +504,31 @@ double raw_norm(const cplx *psi, long n) { /* OCheby.cpp:64-90 */
 }
 
 int cheby_create(int ham, int S, double dt, int order_param, double prec, double scaling_param,
-                 orc_cheby_info *info) {
+                 orc_cheby_info *info, bool imag = false) {
     Cheby c;
     c.ham = ham;
     c.S = S;
     c.dt = dt;
+    c.imag = imag;
     const Op &H = op_at(ham);
     if (op_states(H) != S) throw OracleError("Hamiltonian and wave function differ in number of states");
     if (dt == 0) throw OracleError("No time step defined");
     cplx Emax = op_emax(H), Emin = op_emin(H);
     c.emax = Emax.real();
     c.emin = Emin.real();
-    /* OCheby.cpp:406-416 (real time, forward: exponent -i dt) */
-    c.offset = cplx((Emax + Emin).real() / 2, (Emax + Emin).imag());
-    if (c.offset.imag() != 0)
+    if (!imag) {
+        /* OCheby.cpp:406-416 (real time, forward: exponent -i dt) */
+        c.offset = cplx((Emax + Emin).real() / 2, (Emax + Emin).imag());
+        if (c.offset.imag() != 0)
+            c.scaling = std::abs(Emax - Emin) * 1.05;
+        else
+            c.scaling = std::abs(Emax - Emin) / 2;
+    } else {
+        /* OCheby.cpp:410-416 (imaginary time: exponent real, so the
+         * 1.05 widening applies) */
+        c.offset = Emin - std::abs(Emin) * 0.05;
         c.scaling = std::abs(Emax - Emin) * 1.05;
-    else
-        c.scaling = std::abs(Emax - Emin) / 2;
+    }
     int order = order_param > 0 ? order_param : 0;
     if (order <= 0) order = std::abs(int(20 * c.scaling * dt));   /* OCheby.cpp:419-420 */
     if (order < 10) order = 10;                                    /* OCheby.cpp:422-423 */
@@ -506,7 +536,8 @@ int cheby_create(int ham, int S, double dt, int order_param, double prec, double
     if (order_param > 0) order = order_param;                      /* OCheby.cpp:433-434 */
     if (order < 3) throw OracleError("Chebychev recursion must be at least 3");
     std::vector<double> bessel;
-    int zeroes = bessel_j(order, c.scaling * dt, bessel);          /* OCheby.cpp:440-442 */
+    int zeroes = imag ? bessel_i_scaled(order, c.scaling * dt, bessel)   /* OCheby.cpp:443-446 */
+                      : bessel_j(order, c.scaling * dt, bessel);         /* OCheby.cpp:440-442 */
     order -= zeroes;                                               /* OCheby.cpp:451 */
     if (order_param <= 0) {                                        /* OCheby.cpp:454-464 */
         const double precission = 1e-32;                           /* OCheby.h:10 BESSEL_DELTA */
@@ -514,7 +545,7 @@ int cheby_create(int ham, int S, double dt, int order_param, double prec, double
         while (i < order && std::fabs(bessel[i - 1] - bessel[i - 2]) > precission) i++;
         order = i;
     }
-    if (order > 0 && order <= int(c.scaling * dt))                 /* OCheby.cpp:467-468 */
+    if (!imag && order > 0 && order <= int(c.scaling * dt))        /* OCheby.cpp:467-468 */
         throw OracleError("Chebychev recursion order is to small");
     c.order = order;
     c.nz = zeroes;
@@ -575,6 +606,50 @@ int cheby_step(Cheby &c, int step, cplx *psi) {
         const double nconv = std::fabs(norm - accnorm) / norm;          /* OCheby.cpp:179-182 */
         if (nconv < c.prec && c.offset.imag() == 0) break;
         std::swap(k1, k0);                                              /* OCheby.cpp:191-193 */
+    }
+    return napply;
+}
+
+/* OCheby.cpp:200-297, imaginary time, backward (ProgEigen.cpp:691-694:
+ * ImaginaryTime() then Backward(), exponent -dt, OPropagator.h:82-94) */
+int cheby_imag_step(Cheby &c, int step, cplx *psi) {
+    const Op &H = op_at(c.ham);
+    const long n = op_size(H) * c.S;
+    const cplx exponent = -c.dt;
+    const cplx exp_ = exponent / c.dt;
+    std::vector<cplx> ket0(psi, psi + n), ket1(n), ket2(n), buf(n);
+    int napply = 0;
+    double norm_last = 0;
+
+    op_apply(H, step, psi, ket1.data());
+    ++napply;
+    for (long j = 0; j < n; ++j) ket1[j] += -1.0 * c.offset * psi[j];   /* AddElements */
+    const cplx f1 = exp_ / c.scaling;
+    for (long j = 0; j < n; ++j) ket1[j] *= f1;                       /* MultElements */
+    const cplx ph = std::exp(exponent * c.offset);
+    const cplx c0 = c.coeff[0] * ph, c1 = c.coeff[1] * ph;
+    for (long j = 0; j < n; ++j) psi[j] *= c0;
+    for (long j = 0; j < n; ++j) buf[j] = ket1[j] * c1;
+    for (long j = 0; j < n; ++j) psi[j] += buf[j];
+
+    cplx *k0 = ket0.data(), *k1 = ket1.data(), *k2 = ket2.data(), *bf = buf.data();
+    const cplx exp2 = 2.0 * exp_;
+    for (int i = 2; i < c.order; ++i) {
+        double accnorm = 0;
+        const cplx coeff = c.coeff[i] * ph;
+        op_apply(H, step, k1, bf);
+        ++napply;
+        for (long j = 0; j < n; ++j) {                                  /* OCheby.cpp:270-277 */
+            bf[j] = (bf[j] - c.offset * k1[j]) / c.scaling;
+            bf[j] *= exp2;
+            k2[j] = bf[j] - k0[j];
+            k0[j] = k2[j];
+            psi[j] += k2[j] * coeff;
+            accnorm = accnorm + (std::conj(psi[j]) * psi[j]).real();
+        }
+        if (std::fabs(accnorm - norm_last) / norm_last < c.prec) break;  /* OCheby.cpp:283-285 */
+        norm_last = accnorm;
+        std::swap(k1, k0);
     }
     return napply;
 }
@@ -927,6 +1002,21 @@ int orc_cheby_create(int ham, int nstates, double dt, int order, double prec, do
     return guarded([&] { return cheby_create(ham, nstates, dt, order, prec, scaling, info); });
 }
 
+int orc_cheby_create_imag(int ham, int nstates, double dt, int order, double prec, double scaling,
+                          orc_cheby_info *info) {
+    return guarded([&] { return cheby_create(ham, nstates, dt, order, prec, scaling, info, true); });
+}
+
+int orc_bessel_i_scaled(int n, double x, double *out, int *nz) {
+    return guarded([&] {
+        std::vector<double> v;
+        const int z = bessel_i_scaled(n, x, v);
+        std::copy(v.begin(), v.end(), out);
+        if (nz) *nz = z;
+        return 0;
+    });
+}
+
 int orc_cheby_coeffs(int prop, double *coeff) {
     return guarded([&] {
         const Cheby &c = g_cheby.at(prop);
@@ -937,7 +1027,9 @@ int orc_cheby_coeffs(int prop, double *coeff) {
 
 int orc_cheby_step(int prop, int step, double *psi, int *n_hpsi) {
     return guarded([&] {
-        int n = cheby_step(g_cheby.at(prop), step, reinterpret_cast<cplx *>(psi));
+        Cheby &c = g_cheby.at(prop);
+        int n = c.imag ? cheby_imag_step(c, step, reinterpret_cast<cplx *>(psi))
+                       : cheby_step(c, step, reinterpret_cast<cplx *>(psi));
         if (n_hpsi) *n_hpsi = n;
         return 0;
     });

@@ -43,6 +43,8 @@ int orc_fft(int ndims, const int32_t *N, double *data, int sign);
 
 /* ---- Bessel J_k(x), k = 0..n-1, plus the emulated underflow count ---- */
 int orc_bessel_j(int n, double x, double *out, int *nz);
+/* I_k(x) exp(-x), k = 0..n-1 (Bessel.cpp:61-98, ZBESI kode = 2) */
+int orc_bessel_i_scaled(int n, double x, double *out, int *nz);
 
 /* ---- grid ------------------------------------------------------------ */
 int orc_grid(int ndims, const int32_t *N, const double *xmin, const double *xmax);
@@ -85,8 +87,13 @@ typedef struct orc_cheby_info {
 /* order <= 0: automatic; prec < 0: QDng default 1e-10; scaling <= 0: automatic */
 int orc_cheby_create(int ham, int nstates, double dt, int order, double prec,
                      double scaling, orc_cheby_info *info);
+/* imaginary time, backward (exponent -dt): OCheby with ImaginaryTime() +
+ * Backward() as ProgEigen sets it up (ProgEigen.cpp:691-694) */
+int orc_cheby_create_imag(int ham, int nstates, double dt, int order, double prec,
+                          double scaling, orc_cheby_info *info);
 int orc_cheby_coeffs(int prop, double *coeff /* order doubles */);
-/* one real-time step at clock step `step`; returns number of H applications */
+/* one step at clock step `step` (real or imaginary time, as created);
+ * returns number of H applications */
 int orc_cheby_step(int prop, int step, double *psi, int *n_hpsi);
 
 /* ---- GSPO ------------------------------------------------------------ */

@@ -60,7 +60,8 @@ EXPORTS = [
     "qdx_ctx_set_radix_plan", "qdx_ctx_get_radix_plan", "qdx_batch_get_tiling", "qdx_batch_set_tiling",
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
-    "qdx_ctx_set_line_points", "qdx_ctx_get_line_points",
+    "qdx_ctx_set_line_points", "qdx_ctx_get_line_points", "qdx_cheby_create_imag", "qdx_bessel_i0",
+    "qdx_wf_axpy", "qdx_eigen_imag",
     # include/qdx_io.h
     "qdx_meta_create", "qdx_meta_read", "qdx_meta_write", "qdx_meta_size", "qdx_meta_key", "qdx_meta_get",
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
@@ -160,6 +161,10 @@ def lib():
         "qdx_ctx_set_radix_plan": [vp, i32, i32, C.POINTER(i32), i32],
         "qdx_ctx_get_radix_plan": [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)],
         "qdx_ctx_set_line_points": [vp, i32],
+        "qdx_cheby_create_imag": [vp, C.c_double, i32, C.c_double, C.c_double, C.POINTER(vp)],
+        "qdx_bessel_i0": [i32, C.c_double, dp, C.POINTER(i32)],
+        "qdx_wf_axpy": [vp, C.c_double, C.c_double, vp],
+        "qdx_eigen_imag": [vp, vp, i32, i32, C.c_double, i32, C.POINTER(vp), dp, dp, C.POINTER(i32)],
         "qdx_ctx_get_line_points": [vp, C.POINTER(i32)],
         "qdx_batch_get_tiling": [vp, i32, i32, C.POINTER(Tiling)],
         "qdx_batch_set_tiling": [vp, i32, i32, C.POINTER(Tiling)],
@@ -356,6 +361,14 @@ def bessel_j0(n: int, x: float):
     return out, nz.value
 
 
+def bessel_i0(n: int, x: float):
+    """I_k(x) exp(-x), k < n, and the underflow count (qdx_bessel_i0)"""
+    out = np.zeros(n)
+    nz = C.c_int32(0)
+    _chk(lib().qdx_bessel_i0(int(n), float(x), _dp(out), C.byref(nz)))
+    return out, nz.value
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
@@ -455,6 +468,11 @@ class WaveFunction:
 
     def normalize(self) -> None:
         _chk(lib().qdx_wf_normalize(self.h))
+
+    def axpy(self, a: complex, x: "WaveFunction") -> "WaveFunction":
+        """self += a x (qdx_wf_axpy)"""
+        _chk(lib().qdx_wf_axpy(self.h, float(complex(a).real), float(complex(a).imag), x.h))
+        return self
 
     @classmethod
     def load(cls, ctx: Context, name: str, counter: int = -1) -> "WaveFunction":
@@ -603,14 +621,21 @@ class _Propagator:
 
 
 class Cheby(_Propagator):
-    """OCheby (real time, forward)."""
+    """OCheby: real time, forward; or imag=True: imaginary time, backward
+    (exponent -dt, qdx_cheby_create_imag; FP64)."""
 
     def __init__(self, H: Hamiltonian, dt: float, order: int = 0, prec: float = -1.0,
-                 scaling: float = 0.0, precision="fp64"):
+                 scaling: float = 0.0, precision="fp64", imag: bool = False):
         self.H = H
         self.h = C.c_void_p()
-        _chk(lib().qdx_cheby_create(H.h, float(dt), int(order), float(prec), float(scaling),
-                                    _PREC[precision], C.byref(self.h)))
+        if imag:
+            if _PREC[precision] != FP64:
+                raise QdxError(-1, "imaginary time needs FP64")
+            _chk(lib().qdx_cheby_create_imag(H.h, float(dt), int(order), float(prec), float(scaling),
+                                             C.byref(self.h)))
+        else:
+            _chk(lib().qdx_cheby_create(H.h, float(dt), int(order), float(prec), float(scaling),
+                                        _PREC[precision], C.byref(self.h)))
         self._destroy = lib().qdx_prop_destroy
         self.info = ChebyInfo()
         _chk(lib().qdx_cheby_info_get(self.h, C.byref(self.info)))
@@ -620,6 +645,29 @@ class Cheby(_Propagator):
         out = np.zeros(self.order)
         _chk(lib().qdx_cheby_coeffs(self.h, _dp(out)))
         return out
+
+
+def eigen_imag(U: "Cheby", psi0: WaveFunction, nef: int, max_steps: int = 512, conv: float | None = None,
+               diag: bool = True):
+    """Eigenstates by imaginary-time relaxation (qdx_eigen_imag, ProgEigen
+    method "imag"); conv defaults to QDng's 1e-5 with diag, 1e-8 without.
+    Returns (wave functions, energies, raw energies, steps per state)."""
+    if conv is None:
+        conv = 1e-5 if diag else 1e-8
+    hs = (C.c_void_p * nef)()
+    e = np.zeros(nef)
+    er = np.zeros(nef)
+    st = np.zeros(nef, dtype=np.int32)
+    _chk(lib().qdx_eigen_imag(U.h, psi0.h, int(nef), int(max_steps), float(conv), int(bool(diag)), hs, _dp(e),
+                              _dp(er), st.ctypes.data_as(C.POINTER(C.c_int32))))
+    out = []
+    for h in hs:
+        w = WaveFunction.__new__(WaveFunction)
+        w.ctx, w.N, w.grid, w.S = psi0.ctx, list(psi0.N), psi0.grid, psi0.S
+        w.h = C.c_void_p(h)
+        w._destroy = lib().qdx_wf_destroy
+        out.append(w)
+    return out, e, er, st
 
 
 class GSPO(_Propagator):

@@ -29,6 +29,7 @@ struct qdx_prop {
     qdx_ham *ham = nullptr;
     ChebyParams cp;
     int k_launch = 0;   // orders launched per step (launch_window)
+    bool imag = false;  // imaginary time, backward (exponent -dt)
     DevBuf phiA, phiB, X, ctl, partials, lastorders;
     // single-state 1D lines: the whole step in one block (k_cheby_line) with
     // the phased coefficients coeff[k] exp(-i offset dt) on the device
@@ -81,8 +82,10 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     ChebyCtl *ctl = p->ctl.as<ChebyCtl>();
     // partial-sum slots: [0],[1] alternate between orders, [2] holds the
     // sum |psi|^2 of the step start
+    // (imaginary time compares two previous orders: three slots rotate)
     double *part = p->partials.as<double>();
-    double *slot[2] = {part, part + kMaxPartials};
+    const int nslot = p->imag ? 3 : 2;
+    double *slot[3] = {part, part + kMaxPartials, part + 3 * kMaxPartials};
     double *init = part + 2 * kMaxPartials;
     if (p->line_step) {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
@@ -102,6 +105,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     ConvCheck cc;
     cc.norm0 = &ctl->norm0;
     cc.prec = p->cp.prec;
+    cc.rel = p->imag ? 1 : 0;
     // T = sum_d T_d: the kinetic parts along dimensions >= 1 of phi_{k-1} go
     // to X; the row kernel adds the dimension-0 part
     auto cols = [&](const R *pm1, const ConvCheck &c) {
@@ -110,8 +114,11 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
             Launch<R>::kin_pass(s, g, S, d, d > 1, plans[d], h, pm1, X, ctl, c);
         }
     };
-    const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
+    // exp(exponent * offset), exponent -i dt (real time) or -dt (imaginary)
+    const std::complex<double> ph = p->imag ? std::complex<double>(std::exp(-p->dt * p->cp.offset), 0.0)
+                                            : std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
     ChebyOrder ord;
+    ord.imag = p->imag ? 1 : 0;
     ord.offset = p->cp.offset;
     ord.inv_scaling = 1.0 / p->cp.scaling;
     ord.prec = p->cp.prec;
@@ -124,7 +131,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     cols(A, cc);   // k_prev = 0: no test
     {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, A, B, X, psi->ptr(), ctl, slot[1], init, nb_init);
+        Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, A, B, X, psi->ptr(), ctl, slot[1 % nslot], init, nb_init);
     }
     int nb_prev = 0;
     for (int k = 2; k < p->k_launch; ++k) {
@@ -145,14 +152,15 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
             ord.cm2i = c2.imag();
         }
         cc.k_prev = k - 1;
-        cc.part = slot[(k - 1) % 2];
+        cc.part = slot[(k - 1) % nslot];
+        cc.part2 = slot[(k - 2) % nslot];
         cc.nb = nb_prev;
         R *pm1 = (k - 1) % 2 == 0 ? A : B;
         R *pm2 = k % 2 == 0 ? A : B;
         cols(pm1, cc);
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, pm1, pm2, X, psi->ptr(), ctl, slot[k % 2], init,
-                                       nb_init);
+        nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, pm1, pm2, X, psi->ptr(), ctl, slot[k % nslot],
+                                       init, nb_init);
     }
     p->last_nb = nb_prev;
     p->nb_init = nb_init;
@@ -266,21 +274,29 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
     if (pending.n > 0) Launch<R>::gspo_xonly(s, g, p->S, pending, psi);
 }
 
-// |norm0 - accnorm| / norm0 of the last executed order of the last step,
+// |norm0 - accnorm| / norm0 of the last executed order of the last step
+// (imaginary time: |accnorm - accnorm of the order before| / the latter),
 // from the partial sums the kernels stored
 double cheby_last_nconv(qdx_prop *p) {
     ChebyCtl c;
     HIPCHK(hipMemcpy(&c, p->ctl.p, sizeof(c), hipMemcpyDeviceToHost));
-    if (c.last_order < 2) return 0.0;
-    std::vector<double> a(p->last_nb), b(p->nb_init);
+    if (c.last_order < (p->imag ? 3 : 2)) return 0.0;
     const double *part = p->partials.as<double>();
-    HIPCHK(hipMemcpy(a.data(), part + (c.last_order % 2) * kMaxPartials, a.size() * sizeof(double),
-                     hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(b.data(), part + 2 * kMaxPartials, b.size() * sizeof(double), hipMemcpyDeviceToHost));
-    double acc = 0, n0 = 0;
-    for (double v : a) acc += v;
-    for (double v : b) n0 += v;
-    return std::fabs(n0 - acc) / n0;
+    // slot of order k, as cheby_step lays them out
+    auto slot = [&](int k) {
+        const int i = k % (p->imag ? 3 : 2);
+        return part + (i == 2 ? 3 : i) * kMaxPartials;
+    };
+    auto sum = [&](const double *src, int n) {
+        std::vector<double> a(n);
+        HIPCHK(hipMemcpy(a.data(), src, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double t = 0;
+        for (double v : a) t += v;
+        return t;
+    };
+    const double acc = sum(slot(c.last_order), p->last_nb);
+    const double ref = p->imag ? sum(slot(c.last_order - 1), p->last_nb) : sum(part + 2 * kMaxPartials, p->nb_init);
+    return std::fabs(ref - acc) / ref;
 }
 
 void prop_step(qdx_prop *p, qdx_wf *psi, long step) {
@@ -504,11 +520,17 @@ int qdx_wf_normalize(qdx_wf *wf) {
         double tot = 0;
         for (int s = 0; s < wf->S; ++s) tot += per[2 * s] * wf->grid.weight();
         const double f = 1.0 / std::sqrt(tot);
-        // host round trip is fine here: normalisation happens once per job
-        std::vector<double> h((size_t)wf->S * wf->n() * 2);
-        HIPCHK(hipMemcpy(h.data(), wf->ptr(), h.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (auto &v : h) v *= f;
-        HIPCHK(hipMemcpy(wf->ptr(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        launch_scale_axpy(wf->ctx->stream, wf->ptr(), nullptr, (long)wf->S * wf->n(), f, 0.0, 0.0);
+    });
+}
+
+int qdx_wf_axpy(qdx_wf *y, double re, double im, const qdx_wf *x) {
+    return guarded([&] {
+        check_wf(y);
+        check_wf(const_cast<qdx_wf *>(x));
+        if (y->ctx != x->ctx) fail(QDX_ERR_INCOMPAT, "wave functions on different contexts");
+        if (y->S != x->S || y->n() != x->n()) fail(QDX_ERR_INCOMPAT, "wave functions differ in size");
+        launch_scale_axpy(y->ctx->stream, y->ptr(), const_cast<qdx_wf *>(x)->ptr(), (long)y->S * y->n(), 1.0, re, im);
     });
 }
 
@@ -680,12 +702,14 @@ int qdx_cheby_params(qdx_ham *h, double dt, int32_t order, double prec, double s
     });
 }
 
-int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double scaling, int32_t precision,
-                     qdx_prop **out) {
-    return guarded([&] {
+static void cheby_create_impl(qdx_ham *h, double dt, int32_t order, double prec, double scaling, int32_t precision,
+                              bool imag, qdx_prop **out) {
         if (!h || !out) fail(QDX_ERR_PARAM, "NULL argument");
         if (!h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
         if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        // the imaginary-time series cancels down to exp(-R dt) of its terms:
+        // FP32 terms would leave nothing of the result
+        if (imag && precision != QDX_FP64) fail(QDX_ERR_PARAM, "imaginary time needs FP64");
         auto p = std::make_unique<qdx_prop>();
         p->kind = qdx_prop::CHEBY;
         p->ctx = h->ctx;
@@ -694,9 +718,12 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         p->grid = h->grid;
         p->S = h->S;
         p->ham = h;
-        p->cp = cheby_params(h, dt, order, prec, scaling);
+        p->imag = imag;
+        p->cp = cheby_params(h, dt, order, prec, scaling, imag);
         if (precision == QDX_FP32_BULK) fp32_series(p->cp, dt);
-        p->k_launch = launch_window(p->cp, dt);
+        // imaginary time: every order up to the series length (its break
+        // test compares consecutive orders, not the coefficients)
+        p->k_launch = imag ? p->cp.order : launch_window(p->cp, dt);
         if (precision == QDX_FP32_BULK) h->ensure32();
         const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
         const size_t nbytes = (size_t)p->S * p->grid.size() * es;
@@ -705,10 +732,11 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         if (p->grid.nd > 1) p->X.alloc(nbytes);
         p->ctl.alloc(sizeof(ChebyCtl));
         HIPCHK(hipMemset(p->ctl.p, 0, sizeof(ChebyCtl)));
-        p->partials.alloc(3 * kMaxPartials * sizeof(double));
+        p->partials.alloc(4 * kMaxPartials * sizeof(double));
         // (FP64 lines above 2048 points need more registers than a wave of a
-        // 512-thread block holds; they keep the per-order kernels)
-        p->line_step = p->grid.nd == 1 && p->S == 1 &&
+        // 512-thread block holds; they keep the per-order kernels, and so
+        // does imaginary time)
+        p->line_step = !imag && p->grid.nd == 1 && p->S == 1 &&
                        p->grid.N[0] <= (precision == QDX_FP32_BULK ? kMaxLine : 2048);
         if (p->line_step) {
             // the same products as the per-order scalars of cheby_step
@@ -725,6 +753,24 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
         HIPCHK(hipDeviceSynchronize());
         ctx_acquire(p->ctx);
         *out = p.release();
+}
+
+int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double scaling, int32_t precision,
+                     qdx_prop **out) {
+    return guarded([&] { cheby_create_impl(h, dt, order, prec, scaling, precision, false, out); });
+}
+
+int qdx_cheby_create_imag(qdx_ham *h, double dt, int32_t order, double prec, double scaling, qdx_prop **out) {
+    return guarded([&] { cheby_create_impl(h, dt, order, prec, scaling, QDX_FP64, true, out); });
+}
+
+int qdx_bessel_i0(int32_t n, double x, double *out, int32_t *nz) {
+    return guarded([&] {
+        if (!out) fail(QDX_ERR_PARAM, "out is NULL");
+        std::vector<double> b;
+        int z = bessel_i0_scaled(n, x, b);
+        std::copy(b.begin(), b.end(), out);
+        if (nz) *nz = z;
     });
 }
 
@@ -910,6 +956,174 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
             for (int k = 0; k < S; ++k) t += per[2 * k];
             stats->norm2 = t * w;
         }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF + DiagBasis)
+
+namespace {
+
+// cyclic Jacobi rotations with accumulated eigenvectors; returns eigenvalues
+// ascending and V (column i = eigenvector i, row-major n x n)
+void sym_eigen_vectors(int n, std::vector<double> a, std::vector<double> &ev, std::vector<double> &V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    double scale = 0;
+    for (double x : a) scale += x * x;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+        if (off <= 1e-32 * scale) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[(size_t)p * n + q];
+                if (apq == 0) continue;
+                const double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - sn * akq;
+                    a[(size_t)k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - sn * aqk;
+                    a[(size_t)q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = c * vkp - sn * vkq;
+                    V[(size_t)k * n + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+    ev.resize(n);
+    std::vector<double> W((size_t)n * n);
+    for (int i = 0; i < n; ++i) {
+        ev[i] = a[(size_t)idx[i] * n + idx[i]];
+        for (int k = 0; k < n; ++k) W[(size_t)k * n + i] = V[(size_t)k * n + idx[i]];
+    }
+    V.swap(W);
+}
+
+void chk(int rc) {
+    if (rc != QDX_OK) fail(rc, g_err);
+}
+
+// <a|b> (qdx_wf_overlap); an out-parameter, since this file's C-linkage block
+// holds it
+void ovl(qdx_wf *a, qdx_wf *b, std::complex<double> &z) {
+    double o[2];
+    chk(qdx_wf_overlap(a, b, o));
+    z = {o[0], o[1]};
+}
+
+struct WfHold {   // owns wave functions until released to the caller
+    std::vector<qdx_wf *> w;
+    ~WfHold() {
+        for (qdx_wf *x : w)
+            if (x) qdx_wf_destroy(x);
+    }
+    qdx_wf *make(const qdx_wf *like) {
+        qdx_grid g{};
+        g.ndims = like->grid.nd;
+        for (int d = 0; d < like->grid.nd; ++d) {
+            g.N[d] = like->grid.N[d];
+            g.xmin[d] = like->grid.xmin[d];
+            g.xmax[d] = like->grid.xmax[d];
+        }
+        qdx_wf *x = nullptr;
+        chk(qdx_wf_create(like->ctx, &g, like->S, &x));
+        w.push_back(x);
+        return x;
+    }
+};
+
+}  // namespace
+
+int qdx_eigen_imag(qdx_prop *U, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv, int32_t diag,
+                   qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps) {
+    return guarded([&] {
+        if (!U || !psi0 || !efs || !energies) fail(QDX_ERR_PARAM, "NULL argument");
+        if (U->kind != qdx_prop::CHEBY || !U->imag)
+            fail(QDX_ERR_PARAM, "an imaginary-time Chebyshev propagator is required");
+        if (nef < 1) fail(QDX_ERR_PARAM, "Nef must be >= 1");
+        if (max_steps < 1) fail(QDX_ERR_PARAM, "steps must be >= 1");
+        qdx_wf *p0 = const_cast<qdx_wf *>(psi0);
+        check_prop_wf(U, p0);
+        qdx_ham *H = U->ham;
+        WfHold keep, tmp;
+        std::vector<qdx_wf *> ef;
+        std::vector<double> e_raw(nef);
+        qdx_wf *psi_old = tmp.make(p0), *hbuf = tmp.make(p0);
+        for (int i = 0; i < nef; ++i) {   // ProgEigen.cpp:365-404
+            qdx_wf *psi = keep.make(p0);
+            chk(qdx_wf_copy(psi, p0));
+            int s = 0;
+            double dff = 1;
+            while (s < max_steps && dff > conv) {
+                chk(qdx_wf_copy(psi_old, psi));
+                prop_step(U, psi, s);
+                if (i > 0) {   // OProjection::Apply (OProjection.cpp:133-149), psi -= P psi
+                    std::vector<std::complex<double>> c(i);
+                    for (int j = 0; j < i; ++j) ovl(ef[j], psi, c[j]);
+                    for (int j = 0; j < i; ++j) chk(qdx_wf_axpy(psi, -c[j].real(), -c[j].imag(), ef[j]));
+                }
+                chk(qdx_wf_normalize(psi));
+                std::complex<double> z;
+                ovl(psi_old, psi, z);
+                dff = std::abs(1.0 - z);
+                ++s;
+            }
+            ef.push_back(psi);
+            chk(qdx_ham_expec(H, psi, s, &e_raw[i]));
+            if (steps) steps[i] = s;
+        }
+        if (energies_raw) std::copy(e_raw.begin(), e_raw.end(), energies_raw);
+        if (diag) {   // DiagBasis (ProgEigen.cpp:93-138)
+            std::vector<double> S((size_t)nef * nef, 0.0);
+            for (int i = 0; i < nef; ++i) {
+                S[(size_t)i * nef + i] = e_raw[i];
+                for (int j = 0; j < i; ++j) {   // MatrixElement(bra = ef_i, ket = ef_j)
+                    chk(qdx_ham_apply(H, hbuf, ef[j], 0));
+                    std::complex<double> z;
+                    ovl(ef[i], hbuf, z);
+                    const double m = z.real();
+                    S[(size_t)j * nef + i] = S[(size_t)i * nef + j] = m;
+                }
+            }
+            std::vector<double> ev, V;
+            sym_eigen_vectors(nef, S, ev, V);
+            std::vector<qdx_wf *> mixed;
+            for (int i = 0; i < nef; ++i) {   // psi_i = sum_j S(j, i) ef_j
+                qdx_wf *m = keep.make(p0);
+                chk(qdx_wf_copy(m, ef[0]));
+                launch_scale_axpy(m->ctx->stream, m->ptr(), nullptr, (long)m->S * m->n(), V[(size_t)0 * nef + i], 0, 0);
+                for (int j = 1; j < nef; ++j) chk(qdx_wf_axpy(m, V[(size_t)j * nef + i], 0.0, ef[j]));
+                mixed.push_back(m);
+                energies[i] = ev[i];
+            }
+            for (qdx_wf *x : ef)   // the raw set is not returned
+                for (auto &w : keep.w)
+                    if (w == x) {
+                        qdx_wf_destroy(w);
+                        w = nullptr;
+                    }
+            ef = mixed;
+        } else {
+            std::copy(e_raw.begin(), e_raw.end(), energies);
+        }
+        chk(qdx_ctx_synchronize(U->ctx));
+        for (int i = 0; i < nef; ++i) efs[i] = ef[i];
+        for (auto &w : keep.w)   // ownership passes to the caller
+            for (qdx_wf *x : ef)
+                if (w == x) w = nullptr;
     });
 }
 

@@ -728,6 +728,42 @@ inline int bessel_j0(int n, double x, std::vector<double> &out) {
     return nz;
 }
 
+// Modified Bessel I_k(x) exp(-x), k = 0..n-1, x >= 0, by Miller's backward
+// recurrence I_{k-1} = (2k/x) I_k + I_{k+1} in extended precision,
+// normalised with I_0 + 2 sum_{k>=1} I_k = e^x.  QDng obtains them from
+// SLATEC ZBESI with kode = 2 (Bessel.cpp:61-98); the underflow count is
+// reproduced as for bessel_j0.
+inline int bessel_i0_scaled(int n, double x, std::vector<double> &out) {
+    out.assign(n, 0.0);
+    if (n < 1) fail(QDX_ERR_PARAM, "Bessel: n must be >= 1");
+    if (!(x >= 0)) fail(QDX_ERR_PARAM, "Bessel I: argument must be >= 0");
+    if (x == 0) {
+        out[0] = 1.0;
+        return 0;
+    }
+    const long double ax = x;
+    // I_m / I_0 ~ exp(-m^2 / (2x)) for m < x and faster beyond: start where
+    // both m^2 / (2x) and m log(2m / (e x)) exceed 100
+    int m = std::max(n, (int)ax) + 16;
+    while ((long double)m * m / (2 * ax) < 100.0L && (long double)m * std::log(2.0L * m / (2.718281828459045L * ax)) < 100.0L)
+        m += 8;
+    std::vector<long double> v(m + 2, 0.0L);
+    v[m + 1] = 0.0L;
+    v[m] = 1e-300L;
+    for (int k = m; k >= 1; --k) v[k - 1] = (2.0L * k / ax) * v[k] + v[k + 1];
+    long double norm = v[0];
+    for (int k = 1; k <= m; ++k) norm += 2.0L * v[k];
+    for (int k = 0; k < n; ++k) out[k] = (double)(v[k] / norm);
+    const double elim = 2.303 * (1021.0 * 0.30102999566398120 - 3.0);
+    const double ulim = std::exp(-elim);
+    int nz = 0;
+    for (int k = n - 1; k >= 0 && std::fabs(out[k]) < ulim; --k) {
+        out[k] = 0.0;
+        ++nz;
+    }
+    return nz;
+}
+
 inline constexpr double kFp32CoeffFloor = 1e-10;
 
 struct ChebyParams {
@@ -736,13 +772,22 @@ struct ChebyParams {
     std::vector<double> coeff;
 };
 
-// OCheby::Init (OCheby.cpp:382-493), real time, forward
-inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double prec, double scaling_param) {
+// OCheby::Init (OCheby.cpp:382-493), real time, forward; imag: imaginary
+// time (offset Emin - 5 % |Emin|, R = 1.05 |Emax - Emin|, coefficients from
+// the scaled modified Bessel functions, OCheby.cpp:406-446)
+inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double prec, double scaling_param,
+                                bool imag = false) {
     if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
     ChebyParams c;
     h->bounds(c.emin, c.emax);
-    c.offset = (c.emax + c.emin) / 2;
-    c.scaling = std::fabs(c.emax - c.emin) / 2;
+    if (imag) {
+        if (!(dt > 0)) fail(QDX_ERR_PARAM, "imaginary time needs dt > 0");
+        c.offset = c.emin - std::fabs(c.emin) * 0.05;
+        c.scaling = std::fabs(c.emax - c.emin) * 1.05;
+    } else {
+        c.offset = (c.emax + c.emin) / 2;
+        c.scaling = std::fabs(c.emax - c.emin) / 2;
+    }
     int order = order_param > 0 ? order_param : 0;
     if (order <= 0) order = std::abs(int(20 * c.scaling * dt));
     if (order < 10) order = 10;
@@ -750,14 +795,15 @@ inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double p
     if (order_param > 0) order = order_param;
     if (order < 3) fail(QDX_ERR_PARAM, "Chebychev recursion must be at least 3");
     std::vector<double> b;
-    int zeroes = bessel_j0(order, c.scaling * dt, b);
+    int zeroes = imag ? bessel_i0_scaled(order, c.scaling * dt, b) : bessel_j0(order, c.scaling * dt, b);
     order -= zeroes;
     if (order_param <= 0) {
         int i = 2;
         while (i < order && std::fabs(b[i - 1] - b[i - 2]) > 1e-32) i++;
         order = i;
     }
-    if (order > 0 && order <= int(c.scaling * dt)) fail(QDX_ERR_PARAM, "Chebychev recursion order is to small");
+    if (!imag && order > 0 && order <= int(c.scaling * dt))
+        fail(QDX_ERR_PARAM, "Chebychev recursion order is to small");
     c.order = order;
     c.nz = zeroes;
     c.coeff.assign(order, 0.0);

@@ -74,8 +74,12 @@ struct ConvCheck {
     const double *part = nullptr;   // partials of order k_prev
     const double *norm0 = nullptr;  // sum |psi|^2 at step start (ChebyCtl::norm0)
     int nb = 0;
-    int k_prev = 0;                 // < 2: nothing to test
+    int k_prev = 0;                 // < 2 (< 3 with rel): nothing to test
     double prec = 0;
+    // imaginary time (OCheby.cpp:283-285): the relative change against the
+    // order before, |acc(k_prev) - acc(k_prev-1)| / acc(k_prev-1) < prec
+    int rel = 0;
+    const double *part2 = nullptr;  // partials of order k_prev - 1 (same count nb)
 };
 constexpr int kMaxPartials = 8192;
 
@@ -92,6 +96,10 @@ struct ChebyOrder {
     int nterm = 1;
     double cm1r = 0, cm1i = 0;   // coeff[k-1] * exp(-i offset dt)
     double cm2r = 0, cm2i = 0;   // coeff[k-2] * exp(-i offset dt)
+    // imaginary time, backward (exponent -dt, OCheby.cpp:200-297): phi_1 =
+    // -b, phi_k = -2 b - phi_{k-2} with b = (H - offset) phi_{k-1} / R, and
+    // the coefficients above carry exp(-dt offset) instead
+    int imag = 0;
 };
 
 // One x-space split operator of GSPO: exp(c * W(x)) with c = -i tau, W the
@@ -174,6 +182,8 @@ struct Launch {
 
 // precision-independent helpers
 void launch_convert_c128_to_c64(hipStream_t s, const double *src, float *dst, long n);
+// y = sa * y + (br + i bi) * x over n complex128 values (x may be null)
+void launch_scale_axpy(hipStream_t s, double *y, const double *x, long n, double sa, double br, double bi);
 void launch_convert_c64_to_c128(hipStream_t s, const float *src, double *dst, long n);
 // Deterministic FP64 reductions over a complex128 vector set [S][n]:
 // out[s] = sum_j conj(a_sj) b_sj  (re, im pairs), S <= kMaxStates

@@ -264,11 +264,13 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
                                             (hp.i - ord.offset * p.i) * ord.inv_scaling);
             cx<double> pk;
             if (PRO) {
-                pk = mk<double>(b.i, -b.r);                               // * (-i)
+                pk = ord.imag ? mk<double>(-b.r, -b.i)                    // * (-1)
+                              : mk<double>(b.i, -b.r);                    // * (-i)
                 psi[base + gi] = cmul(c0, H.ps[u]) + cmul(ck, pk);
             } else {
                 const cx<double> mm = to_d(H.m2[u]);
-                pk = mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);    // * (-2i) + phi_{k-2}
+                pk = ord.imag ? mk<double>(-2.0 * b.r - mm.r, -2.0 * b.i - mm.i)   // * (-2) - phi_{k-2}
+                              : mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);   // * (-2i) + phi_{k-2}
                 if (upd) {
                     cx<double> sv = H.ps[u];
                     if (nterm >= 3) sv = sv + cmul(cm2, mm);
@@ -813,6 +815,26 @@ __global__ void k_c128_to_c64(const double *__restrict__ s, float *__restrict__ 
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
         d[i] = (float)s[i];
 }
+// y = sa * y + b * x over n complex128 values (x may be null): scaling and
+// linear combinations of wave functions (cVec MultElements / MultElementsAdd)
+__global__ void k_scale_axpy(double *__restrict__ y, const double *__restrict__ x, long n, double sa, double br,
+                             double bi) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        double yr = y[2 * i], yi = y[2 * i + 1];
+        if (sa != 1.0) {
+            yr *= sa;
+            yi *= sa;
+        }
+        if (x) {
+            const double xr = x[2 * i], xi = x[2 * i + 1];
+            yr += br * xr - bi * xi;
+            yi += br * xi + bi * xr;
+        }
+        y[2 * i] = yr;
+        y[2 * i + 1] = yi;
+    }
+}
+
 __global__ void k_c64_to_c128(const float *__restrict__ s, double *__restrict__ d, long n2) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
         d[i] = (double)s[i];
@@ -1166,6 +1188,12 @@ void launch_convert_c128_to_c64(hipStream_t s, const double *src, float *dst, lo
     const long n2 = 2 * n;
     const int nb = (int)std::min<long>((n2 + 255) / 256, 8192);
     QDX_LAUNCH(k_c128_to_c64, dim3(nb), dim3(256), 0, s, src, dst, n2);
+    QDX_HIP(hipGetLastError());
+}
+
+void launch_scale_axpy(hipStream_t s, double *y, const double *x, long n, double sa, double br, double bi) {
+    const int nb = (int)std::min<long>((n + 255) / 256, 8192);
+    QDX_LAUNCH(k_scale_axpy, dim3(std::max(nb, 1)), dim3(256), 0, s, y, x, n, sa, br, bi);
     QDX_HIP(hipGetLastError());
 }
 
