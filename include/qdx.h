@@ -162,11 +162,19 @@ int qdx_cheby_create(qdx_ham *h, double dt, int32_t order, double prec, double s
  * only: the series cancels by exp(-R dt), and rounding grows accordingly. */
 int qdx_cheby_create_imag(qdx_ham *h, double dt, int32_t order, double prec, double scaling, qdx_prop **out);
 int qdx_cheby_info_get(qdx_prop *p, qdx_cheby_info *info);
+/* OSIL (OSIL.cpp:79-190): short iterative Lanczos with at most `order`
+ * Lanczos vectors (<= 0: 20) and error bound err (<= 0: 1e-12), OSIL.h:11-12;
+ * imag != 0: imaginary time, backward (exponent -dt), as ProgEigen sets it
+ * up.  The Lanczos recursion runs on the device (H applications, scalar
+ * products, vector updates), the tridiagonal matrix is diagonalised on the
+ * host.  FP64. */
+int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t imag, qdx_prop **out);
 /* Eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF,
  * ProgEigen.cpp:318-412): for i < nef, psi = psi0; repeat at most max_steps
  * times while |1 - <psi_old|psi>| > conv: one step of U (from
- * qdx_cheby_create_imag), remove the states found so far (OProjection,
- * psi -= sum_j ef_j <ef_j|psi>), normalise; the raw energy is
+ * qdx_cheby_create_imag, or qdx_sil_create with imag), remove the states
+ * found so far (OProjection, psi -= sum_j ef_j <ef_j|psi>), normalise; the
+ * raw energy is
  * <ef_i|H|ef_i>.  diag != 0 (QDng's default, DiagBasis, ProgEigen.cpp:93-138):
  * H is diagonalised in the found basis and the returned states and energies
  * (ascending) are those of the diagonalised basis.  QDng's defaults: conv

@@ -71,6 +71,8 @@ def lib():
                                  C.c_double, C.POINTER(ChebyInfo)],
             "orc_cheby_coeffs": [C.c_int, dp],
             "orc_cheby_step": [C.c_int, C.c_int, dp, ip],
+            "orc_sil_create": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int],
+            "orc_sil_step": [C.c_int, C.c_int, dp, ip],
             "orc_gspo_create": [ip, C.c_int, C.c_int, C.c_double],
             "orc_gspo_step": [C.c_int, C.c_int, dp],
         }
@@ -260,6 +262,20 @@ class Cheby:
         assert psi.dtype == np.complex128 and psi.flags.c_contiguous
         n = C.c_int(0)
         _chk(lib().orc_cheby_step(self.id, int(step), _dp(psi), C.byref(n)))
+        return n.value
+
+
+class SIL:
+    """Short iterative Lanczos propagator (QDng OSIL): real time forward, or
+    imaginary time backward (imag=True)."""
+
+    def __init__(self, H: Op, dt: float, order: int = 0, err: float = 0.0, imag: bool = False):
+        self.id = _chk(lib().orc_sil_create(H.id, H.nstates, float(dt), int(order), float(err), int(bool(imag))))
+        self.H = H
+
+    def step(self, psi: np.ndarray, step: int = 0) -> int:
+        n = C.c_int(0)
+        _chk(lib().orc_sil_step(self.id, int(step), _dp(psi), C.byref(n)))
         return n.value
 
 

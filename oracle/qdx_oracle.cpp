@@ -655,6 +655,80 @@ int cheby_imag_step(Cheby &c, int step, cplx *psi) {
 }
 
 /* ======================================================================
+ * Short iterative Lanczos (OSIL.cpp:79-190): real time forward (exponent
+ * -i dt) or imaginary time backward (-dt)
+ * ==================================================================== */
+struct SIL {
+    int ham = -1;
+    int S = 1;
+    double dt = 0;
+    int order = 20;     /* OSIL.h:11 SIL_DEF_ORDER */
+    double err = 1e-12; /* OSIL.h:12 SIL_DEF_ERR */
+    bool imag = false;
+};
+std::vector<SIL> g_sil;
+
+int sil_step(const SIL &c, int step, cplx *psi) {
+    const Op &H = op_at(c.ham);
+    const Grid &g = g_grids.at(op_grid(H));
+    const long n = op_size(H) * c.S;
+    const int order = c.order;
+    auto norm = [&](const cplx *v) { return std::sqrt(overlap(g, c.S, v, v).real()); };
+    std::vector<std::vector<cplx>> q(order, std::vector<cplx>(n));
+    std::vector<cplx> buf0(n);
+    std::vector<double> alpha(order, 0.0), beta(order, 0.0);
+    int napply = 0;
+    const double psinorm = norm(psi);
+    q[0].assign(psi, psi + n);
+    op_apply(H, step, q[0].data(), buf0.data());                      /* H q_0 */
+    ++napply;
+    alpha[0] = overlap(g, c.S, q[0].data(), buf0.data()).real();
+    q[1] = buf0;
+    for (long j = 0; j < n; ++j) q[1][j] -= q[0][j] * alpha[0];
+    beta[0] = norm(q[1].data());
+    for (long j = 0; j < n; ++j) q[1][j] *= 1 / beta[0];
+    int convorder = 2, it;
+    for (it = 2; it < order; it++) {                                  /* OSIL.cpp:110-160 */
+        double err = std::pow(c.dt, it - 2) * beta[0];
+        for (int i = 1; i < it - 2; i++) err *= beta[i] / double(i);
+        err = std::fabs(err);
+        convorder++;
+        op_apply(H, step, q[it - 1].data(), q[it].data());
+        ++napply;
+        alpha[it - 1] = overlap(g, c.S, q[it - 1].data(), q[it].data()).real();
+        for (long j = 0; j < n; ++j) q[it][j] -= q[it - 1][j] * alpha[it - 1];
+        for (long j = 0; j < n; ++j) q[it][j] -= q[it - 2][j] * beta[it - 2];
+        beta[it - 1] = norm(q[it].data());
+        for (long j = 0; j < n; ++j) q[it][j] *= 1 / beta[it - 1];
+        if (err < c.err * psinorm) break;
+    }
+    {                                                                  /* H->Expec(q_{it-1}) */
+        std::vector<cplx> hq(n);
+        op_apply(H, step, q[it - 1].data(), hq.data());
+        ++napply;
+        alpha[it - 1] = overlap(g, c.S, q[it - 1].data(), hq.data()).real();
+    }
+    /* DiagLZB (OSIL.cpp:168-190): the order x order tridiagonal matrix
+     * (entries not reached stay 0), Z e^{c Lambda} Z^T e_0 */
+    std::vector<double> T((size_t)order * order, 0.0), ev, Z;
+    for (int i = 0; i < order; ++i) {
+        T[(size_t)i * order + i] = alpha[i];
+        if (i + 1 < order) T[(size_t)i * order + i + 1] = T[(size_t)(i + 1) * order + i] = beta[i];
+    }
+    sym_eigen(order, T, ev, &Z);
+    const cplx expo = c.imag ? cplx(-c.dt, 0) : cplx(0, -c.dt);
+    std::vector<cplx> vect(order, 0.0);
+    for (int k = 0; k < order; ++k) {
+        const cplx w = std::exp(expo * ev[k]) * Z[(size_t)0 * order + k];
+        for (int i = 0; i < order; ++i) vect[i] += Z[(size_t)i * order + k] * w;
+    }
+    for (long j = 0; j < n; ++j) psi[j] *= vect[0];
+    for (int i = 1; i < convorder; ++i)
+        for (long j = 0; j < n; ++j) psi[j] += q[i][j] * vect[i];
+    return napply;
+}
+
+/* ======================================================================
  * GSPO (OGSPO.cpp)
  * ==================================================================== */
 struct GSPO {
@@ -1013,6 +1087,30 @@ int orc_bessel_i_scaled(int n, double x, double *out, int *nz) {
         const int z = bessel_i_scaled(n, x, v);
         std::copy(v.begin(), v.end(), out);
         if (nz) *nz = z;
+        return 0;
+    });
+}
+
+int orc_sil_create(int ham, int nstates, double dt, int order, double err, int imag) {
+    return guarded([&] {
+        SIL c;
+        c.ham = ham;
+        c.S = nstates;
+        c.dt = dt;
+        if (order > 0) c.order = order;
+        if (c.order < 3) throw OracleError("SIL: order doesn't make sense");
+        if (err > 0) c.err = err;
+        c.imag = imag != 0;
+        if (op_states(op_at(ham)) != nstates) throw OracleError("Hamiltonian and wave function differ in number of states");
+        g_sil.push_back(c);
+        return (int)g_sil.size() - 1;
+    });
+}
+
+int orc_sil_step(int prop, int step, double *psi, int *n_hpsi) {
+    return guarded([&] {
+        int n = sil_step(g_sil.at(prop), step, reinterpret_cast<cplx *>(psi));
+        if (n_hpsi) *n_hpsi = n;
         return 0;
     });
 }

@@ -96,6 +96,12 @@ int orc_cheby_coeffs(int prop, double *coeff /* order doubles */);
  * returns number of H applications */
 int orc_cheby_step(int prop, int step, double *psi, int *n_hpsi);
 
+/* ---- short iterative Lanczos (OSIL.cpp) ------------------------------- */
+/* order <= 0: 20; err <= 0: 1e-12 (OSIL.h:11-12); imag: imaginary time,
+ * backward (exponent -dt) */
+int orc_sil_create(int ham, int nstates, double dt, int order, double err, int imag);
+int orc_sil_step(int prop, int step, double *psi, int *n_hpsi);
+
 /* ---- GSPO ------------------------------------------------------------ */
 int orc_gspo_create(const int *ops, int nops, int nstates, double dt);
 int orc_gspo_step(int prop, int step, double *psi);

@@ -19,7 +19,7 @@
 // ---------------------------------------------------------------------------
 // propagators
 struct qdx_prop {
-    enum Kind { CHEBY, GSPO } kind;
+    enum Kind { CHEBY, GSPO, SIL } kind;
     qdx_ctx *ctx = nullptr;
     int precision = QDX_FP64;
     double dt = 0;
@@ -36,6 +36,13 @@ struct qdx_prop {
     bool line_step = false;
     DevBuf coefd;
     int last_nb = 0, nb_init = 0;   // partial counts of the last Chebyshev step
+    // SIL (Lanczos vectors and a buffer as device wave functions; `imag`
+    // above selects imaginary time)
+    int sil_order = 20;
+    double sil_err = 1e-12;
+    std::vector<qdx_wf *> lzb;
+    qdx_wf *sil_buf = nullptr;
+    int sil_napply = 0;   // H applications of the last step
     // GSPO
     std::vector<qdx_ham *> ops;
     DevBuf psi32, psi0_32;
@@ -299,7 +306,13 @@ double cheby_last_nconv(qdx_prop *p) {
     return std::fabs(ref - acc) / ref;
 }
 
+void sil_step(qdx_prop *p, qdx_wf *psi, long step);
+
 void prop_step(qdx_prop *p, qdx_wf *psi, long step) {
+    if (p->kind == qdx_prop::SIL) {
+        sil_step(p, psi, step);
+        return;
+    }
     if (p->kind == qdx_prop::CHEBY) {
         if (p->precision == QDX_FP32_BULK) cheby_step<float>(p, psi, step);
         else cheby_step<double>(p, psi, step);
@@ -859,6 +872,9 @@ int qdx_prop_apply(qdx_prop *p, qdx_wf *psi, int64_t step, qdx_step_stats *stats
                 HIPCHK(hipStreamSynchronize(s));
                 stats->n_hpsi = c.last_order;
                 stats->nconv = cheby_last_nconv(p);
+            } else if (p->kind == qdx_prop::SIL) {
+                stats->n_hpsi = p->sil_napply;
+                stats->nconv = 0;
             } else {
                 stats->n_hpsi = 1;
                 stats->nconv = 0;
@@ -902,6 +918,7 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
             }
         }
         double *part = p->partials.as<double>();
+        std::vector<int> sil_counts;
         for (int64_t i = 0; i < nsteps; ++i) {
             double *o = obs + i * per_step;
             if (norm2) {
@@ -916,6 +933,7 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
             if (p->kind == qdx_prop::CHEBY)
                 HIPCHK(hipMemcpyAsync(orders + i, &p->ctl.as<ChebyCtl>()->last_order, sizeof(int),
                                       hipMemcpyDeviceToDevice, s));
+            if (p->kind == qdx_prop::SIL) sil_counts.push_back(p->sil_napply);
         }
         if (g32) launch_convert_c64_to_c128(s, p->psi32.as<float>(), psi->ptr(), (long)S * n);
         std::vector<double> host((size_t)nsteps * per_step + nsteps);
@@ -942,7 +960,7 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
                 autocorr[2 * i] = re * w;
                 autocorr[2 * i + 1] = im * w;
             }
-            const int cnt = p->kind == qdx_prop::CHEBY ? hord[i] : 1;
+            const int cnt = p->kind == qdx_prop::CHEBY ? hord[i] : p->kind == qdx_prop::SIL ? sil_counts[i] : 1;
             if (n_hpsi_per_step) n_hpsi_per_step[i] = cnt;
             total += cnt;
         }
@@ -961,6 +979,7 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
 
 // ---------------------------------------------------------------------------
 // eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF + DiagBasis)
+// and the short iterative Lanczos propagator (OSIL)
 
 namespace {
 
@@ -1051,8 +1070,8 @@ int qdx_eigen_imag(qdx_prop *U, const qdx_wf *psi0, int32_t nef, int32_t max_ste
                    qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps) {
     return guarded([&] {
         if (!U || !psi0 || !efs || !energies) fail(QDX_ERR_PARAM, "NULL argument");
-        if (U->kind != qdx_prop::CHEBY || !U->imag)
-            fail(QDX_ERR_PARAM, "an imaginary-time Chebyshev propagator is required");
+        if ((U->kind != qdx_prop::CHEBY && U->kind != qdx_prop::SIL) || !U->imag)
+            fail(QDX_ERR_PARAM, "an imaginary-time Chebyshev or SIL propagator is required");
         if (nef < 1) fail(QDX_ERR_PARAM, "Nef must be >= 1");
         if (max_steps < 1) fail(QDX_ERR_PARAM, "steps must be >= 1");
         qdx_wf *p0 = const_cast<qdx_wf *>(psi0);
@@ -1127,14 +1146,136 @@ int qdx_eigen_imag(qdx_prop *U, const qdx_wf *psi0, int32_t nef, int32_t max_ste
     });
 }
 
+int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t imag, qdx_prop **out) {
+    return guarded([&] {
+        if (!h || !out) fail(QDX_ERR_PARAM, "NULL argument");
+        if (!h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
+        if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
+        if (order <= 0) order = 20;                        // OSIL.h:11
+        if (order < 3) fail(QDX_ERR_PARAM, "SIL: order doesn't make sense");
+        auto p = std::make_unique<qdx_prop>();
+        p->kind = qdx_prop::SIL;
+        p->ctx = h->ctx;
+        p->precision = QDX_FP64;
+        p->dt = dt;
+        p->grid = h->grid;
+        p->S = h->S;
+        p->ham = h;
+        p->imag = imag != 0;
+        p->sil_order = order;
+        p->sil_err = err > 0 ? err : 1e-12;                // OSIL.h:12
+        qdx_grid g{};
+        g.ndims = h->grid.nd;
+        for (int d = 0; d < h->grid.nd; ++d) {
+            g.N[d] = h->grid.N[d];
+            g.xmin[d] = h->grid.xmin[d];
+            g.xmax[d] = h->grid.xmax[d];
+        }
+        auto cleanup = [&] {
+            for (qdx_wf *w : p->lzb) qdx_wf_destroy(w);
+            p->lzb.clear();
+            if (p->sil_buf) qdx_wf_destroy(p->sil_buf);
+            p->sil_buf = nullptr;
+        };
+        for (int i = 0; i <= order; ++i) {
+            qdx_wf *w = nullptr;
+            const int rc = qdx_wf_create(h->ctx, &g, h->S, &w);
+            if (rc != QDX_OK) {
+                const std::string e = g_err;
+                cleanup();
+                fail(rc, e);
+            }
+            if (i < order) p->lzb.push_back(w);
+            else p->sil_buf = w;
+        }
+        ctx_acquire(p->ctx);
+        *out = p.release();
+    });
+}
+
 int qdx_prop_destroy(qdx_prop *p) {
     return guarded([&] {
         if (!p) return;
         qdx_ctx *c = p->ctx;
         if (c) (void)hipStreamSynchronize(c->stream);
+        for (qdx_wf *w : p->lzb) qdx_wf_destroy(w);
+        if (p->sil_buf) qdx_wf_destroy(p->sil_buf);
         delete p;
         ctx_release(c);
     });
 }
 
 }  // extern "C"
+
+namespace {
+
+// OSIL::BuildLZB + DiagLZB (OSIL.cpp:79-190) over device operations: the
+// Lanczos vectors live on the device, the tridiagonal matrix is diagonalised
+// on the host.  QDng's own details are kept: q_0 is psi unnormalised, the
+// error estimate dt^(it-2) b_0 prod_{i=1}^{it-3} b_i / i ends the iteration,
+// the last diagonal element is recomputed as <q|H|q>, and the tridiagonal
+// matrix keeps its full order (entries not reached stay 0).
+void sil_step(qdx_prop *p, qdx_wf *psi, long step) {
+    const int order = p->sil_order;
+    std::vector<qdx_wf *> &q = p->lzb;
+    qdx_wf *buf = p->sil_buf;
+    qdx_ham *H = p->ham;
+    hipStream_t s = p->ctx->stream;
+    const long n = (long)p->S * psi->n();
+    std::vector<double> alpha(order, 0.0), beta(order, 0.0);
+    int napply = 0;
+    auto norm_of = [&](qdx_wf *w) {
+        double nn = 0;
+        chk(qdx_wf_norm(w, &nn, nullptr));
+        return nn;
+    };
+    auto scale = [&](qdx_wf *w, double f) { launch_scale_axpy(s, w->ptr(), nullptr, n, f, 0.0, 0.0); };
+    std::complex<double> z;
+    const double psinorm = norm_of(psi);
+    chk(qdx_wf_copy(q[0], psi));
+    chk(qdx_ham_apply(H, buf, q[0], step));
+    ++napply;
+    ovl(q[0], buf, z);
+    alpha[0] = z.real();
+    chk(qdx_wf_copy(q[1], buf));
+    chk(qdx_wf_axpy(q[1], -alpha[0], 0.0, q[0]));
+    beta[0] = norm_of(q[1]);
+    scale(q[1], 1 / beta[0]);
+    int convorder = 2, it;
+    for (it = 2; it < order; it++) {
+        double err = std::pow(p->dt, it - 2) * beta[0];
+        for (int i = 1; i < it - 2; i++) err *= beta[i] / double(i);
+        err = std::fabs(err);
+        convorder++;
+        chk(qdx_ham_apply(H, q[it], q[it - 1], step));
+        ++napply;
+        ovl(q[it - 1], q[it], z);
+        alpha[it - 1] = z.real();
+        chk(qdx_wf_axpy(q[it], -alpha[it - 1], 0.0, q[it - 1]));
+        chk(qdx_wf_axpy(q[it], -beta[it - 2], 0.0, q[it - 2]));
+        beta[it - 1] = norm_of(q[it]);
+        scale(q[it], 1 / beta[it - 1]);
+        if (err < p->sil_err * psinorm) break;
+    }
+    chk(qdx_ham_expec(H, q[it - 1], step, &alpha[it - 1]));
+    ++napply;
+    // Z e^{c Lambda} Z^T e_0, c = -i dt (real time) or -dt (imaginary)
+    std::vector<double> T((size_t)order * order, 0.0), ev, Z;
+    for (int i = 0; i < order; ++i) {
+        T[(size_t)i * order + i] = alpha[i];
+        if (i + 1 < order) T[(size_t)i * order + i + 1] = T[(size_t)(i + 1) * order + i] = beta[i];
+    }
+    sym_eigen_vectors(order, T, ev, Z);
+    const std::complex<double> expo = p->imag ? std::complex<double>(-p->dt, 0) : std::complex<double>(0, -p->dt);
+    std::vector<std::complex<double>> vect(order, 0.0);
+    for (int k = 0; k < order; ++k) {
+        const std::complex<double> w = std::exp(expo * ev[k]) * Z[(size_t)0 * order + k];
+        for (int i = 0; i < order; ++i) vect[i] += Z[(size_t)i * order + k] * w;
+    }
+    // psi = vect_0 psi + sum_{i < convorder} vect_i q_i (q_0 holds psi)
+    launch_scale_axpy(s, psi->ptr(), q[0]->ptr(), n, 0.0, vect[0].real(), vect[0].imag());
+    for (int i = 1; i < convorder; ++i) chk(qdx_wf_axpy(psi, vect[i].real(), vect[i].imag(), q[i]));
+    p->sil_napply = napply;
+}
+
+}  // namespace

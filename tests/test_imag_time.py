@@ -275,3 +275,90 @@ def test_device_eigenstates_match_qdng_tge(ctx):
         lambda bra, ket: O.overlap(g, bra, Ho.apply(ket)), lambda: p.psi0.copy(), 3, 1000, 1e-8)
     assert list(steps2) == list(steps_o)
     assert np.allclose(e2, e_o, rtol=1e-9, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# short iterative Lanczos (OSIL.cpp:79-190), real and imaginary time
+
+@pytest.mark.parametrize("imag,dt,order,tol", [(False, 0.5, 20, 1e-12), (False, 5.0, 25, 1e-9),
+                                               (True, 5.0, 25, 1e-10)])
+def test_oracle_sil_step_is_matrix_exponential(imag, dt, order, tol):
+    p = tge_problem()
+    H, M = dense_h(p)
+    w, V = np.linalg.eigh(0.5 * (M + M.conj().T))
+    f = np.exp(-w * dt) if imag else np.exp(-1j * w * dt)
+    ref = V @ (f * (V.conj().T @ p.psi0))
+    U = O.SIL(H, dt, order=order, imag=imag)
+    psi = p.psi0.copy()
+    n = U.step(psi, 0)
+    assert 3 <= n <= order
+    assert np.linalg.norm(psi - ref) / np.linalg.norm(ref) < tol
+    # the wrong exponent sign is far outside the bar
+    fbad = np.exp(+w * dt) if imag else np.conj(f)
+    bad = V @ (fbad * (V.conj().T @ p.psi0))
+    assert np.linalg.norm(bad / np.linalg.norm(bad) - ref / np.linalg.norm(ref)) > 1e3 * tol
+
+
+def test_oracle_sil_eigenstates_match_qdng_tge():
+    # TGE-SIL (tests/TGE-SIL.qdi.in: SIL order 25, dt 5) checks the same
+    # energies as TGE-Cheby (checktests.sh:124-127)
+    p = tge_problem()
+    H = p.oracle_ham()
+    g = p.oracle_grid()
+    U = O.SIL(H, 5.0, order=25, imag=True)
+
+    def project_out(psi, efs):
+        buf = np.zeros_like(psi)
+        for ef in efs:
+            buf += ef * O.overlap(g, ef, psi)
+        psi -= buf
+
+    def normalize(psi):
+        psi /= np.sqrt(O.norm2(g, psi))
+
+    e_raw, e_diag, steps = imag_time_ef(
+        lambda psi, s: U.step(psi, s), project_out, normalize, lambda a, b: O.overlap(g, a, b),
+        lambda w: H.expec(w), lambda bra, ket: O.overlap(g, bra, H.apply(ket)), lambda: p.psi0.copy(),
+        10, 1000, 1e-5)
+    assert_tge_energies(e_diag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,builder,dt,order,imag,steps", [
+    ("tge_1d_real", tge_problem, 0.5, 20, False, 5),
+    ("tge_1d_real_dt5", tge_problem, 5.0, 25, False, 4),
+    ("tge_1d_imag", tge_problem, 5.0, 25, True, 4),
+    ("morse_2state_2d_real", lambda: cases.morse_2state_2d(64), 1.0, 20, False, 3),
+    ("anharmonic_3d_imag", lambda: cases.anharmonic_3d(16), 1.0, 20, True, 2),
+])
+def test_device_sil_matches_oracle(ctx, name, builder, dt, order, imag, steps):
+    import qdng_amd as Q
+    p = builder()
+    U = Q.SIL(p.gpu_ham(ctx), dt, order=order, imag=imag)
+    Uo = O.SIL(p.oracle_ham(), dt, order=order, imag=imag)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    counts = []
+    for s in range(steps):
+        st = U.apply(w, s)
+        n = Uo.step(psi, s)
+        counts.append(n)
+        assert st.n_hpsi == n, (s, st.n_hpsi, n)
+    got = w.download()
+    assert np.linalg.norm(got - psi) / np.linalg.norm(psi) < 1e-10
+    # the run loop takes the same steps and reports the same H counts
+    w2 = p.gpu_wf(ctx)
+    _, _, nh, _ = U.run(w2, 0, steps, observables=False)
+    assert list(nh) == counts
+    assert np.array_equal(w2.download(), got)
+
+
+@pytest.mark.gpu
+def test_device_sil_eigenstates_match_qdng_tge(ctx):
+    import qdng_amd as Q
+    p = tge_problem()
+    U = Q.SIL(p.gpu_ham(ctx), 5.0, order=25, imag=True)
+    efs, e, e_raw, steps = Q.eigen_imag(U, p.gpu_wf(ctx), 10, max_steps=1000, conv=1e-5, diag=True)
+    assert_tge_energies(e)
+    with pytest.raises(Q.QdxError):
+        Q.eigen_imag(Q.SIL(p.gpu_ham(ctx), 5.0, order=25), p.gpu_wf(ctx), 2)   # real time: rejected
