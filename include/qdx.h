@@ -172,17 +172,19 @@ int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t ima
 /* Eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF,
  * ProgEigen.cpp:318-412): for i < nef, psi = psi0; repeat at most max_steps
  * times while |1 - <psi_old|psi>| > conv: one step of U (from
- * qdx_cheby_create_imag, or qdx_sil_create with imag), remove the states
- * found so far (OProjection, psi -= sum_j ef_j <ef_j|psi>), normalise; the
- * raw energy is
- * <ef_i|H|ef_i>.  diag != 0 (QDng's default, DiagBasis, ProgEigen.cpp:93-138):
- * H is diagonalised in the found basis and the returned states and energies
- * (ascending) are those of the diagonalised basis.  QDng's defaults: conv
- * 1e-5 with diag, 1e-8 without; max_steps 512 (ProgEigen.h:15-18).
+ * qdx_cheby_create_imag, qdx_gspo_create_imag, or qdx_sil_create with
+ * imag), remove the states found so far (OProjection, psi -= sum_j ef_j
+ * <ef_j|psi>), normalise; the raw energy is <ef_i|H|ef_i> with H the given
+ * Hamiltonian, or U's own when H is NULL (required for GSPO, whose split
+ * operators form no Hamiltonian).  diag != 0 (QDng's default, DiagBasis,
+ * ProgEigen.cpp:93-138): H is diagonalised in the found basis and the
+ * returned states and energies (ascending) are those of the diagonalised
+ * basis.  QDng's defaults: conv 1e-5 with diag, 1e-8 without; max_steps 512
+ * (ProgEigen.h:15-18).
  * efs receives nef new wave functions (the caller destroys them); energies
  * nef values; energies_raw and steps (nullable) the relaxation's own. */
-int qdx_eigen_imag(qdx_prop *U, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv, int32_t diag,
-                   qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps);
+int qdx_eigen_imag(qdx_prop *U, qdx_ham *H, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv,
+                   int32_t diag, qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps);
 /* The scalars and coefficients OCheby::Init would choose for this Hamiltonian,
  * without allocating device memory (h may have been created with ctx NULL).
  * coeff (nullable) receives min(coeff_len, order) values. */
@@ -196,6 +198,11 @@ int qdx_cheby_coeffs(qdx_prop *p, double *coeff);
  * QDng's GridNablaSq / GridPotential / GridDipole / DMultistate. */
 int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precision,
                     qdx_prop **out);
+/* OGSPO in imaginary time, backward (exponent -dt, as ProgEigen sets it up):
+ * every split operator becomes exp(-dt sf D) (OGSPO.cpp:30-47 with
+ * ODSpace::InitExponential, ODSpace.h:65-69) -- real decays, the coupled
+ * 2 x 2 potential through cosh / sinh.  FP64. */
+int qdx_gspo_create_imag(qdx_ham *const *ops, int32_t nops, double dt, qdx_prop **out);
 
 typedef struct qdx_step_stats {
     int64_t n_hpsi;        /* H applications executed (Chebyshev: 1 + recurrence

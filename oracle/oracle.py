@@ -74,6 +74,7 @@ def lib():
             "orc_sil_create": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int],
             "orc_sil_step": [C.c_int, C.c_int, dp, ip],
             "orc_gspo_create": [ip, C.c_int, C.c_int, C.c_double],
+            "orc_gspo_create_ex": [ip, C.c_int, C.c_int, C.c_double, C.c_int],
             "orc_gspo_step": [C.c_int, C.c_int, dp],
         }
         for name, args in sig.items():
@@ -280,11 +281,12 @@ class SIL:
 
 
 class GSPO:
-    """General split-operator propagator (QDng OGSPO)."""
+    """General split-operator propagator (QDng OGSPO); imag=True: imaginary
+    time, backward (exponent -dt)."""
 
-    def __init__(self, ops, dt: float, nstates: int = 1):
+    def __init__(self, ops, dt: float, nstates: int = 1, imag: bool = False):
         ids = (C.c_int * len(ops))(*[o.id for o in ops])
-        self.id = _chk(lib().orc_gspo_create(ids, len(ops), int(nstates), float(dt)))
+        self.id = _chk(lib().orc_gspo_create_ex(ids, len(ops), int(nstates), float(dt), int(bool(imag))))
 
     def step(self, psi: np.ndarray, step: int = 0) -> None:
         assert psi.dtype == np.complex128 and psi.flags.c_contiguous

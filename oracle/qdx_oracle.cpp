@@ -735,6 +735,7 @@ struct GSPO {
     std::vector<int> ops;
     int S = 1;
     double dt = 0;
+    bool imag = false;   /* imaginary time, backward: exponent -dt */
 };
 std::vector<GSPO> g_gspo;
 
@@ -823,7 +824,7 @@ void split_apply(const Op &o, int step, cplx c, cplx *psi, int S) {
 
 void gspo_step(const GSPO &p, int step, cplx *psi) {
     const cplx I(0, 1);
-    const cplx exponent = -I * p.dt;
+    const cplx exponent = p.imag ? cplx(-p.dt, 0) : -I * p.dt;   /* OPropagator.h:62-108 */
     const int L = (int)p.ops.size();
     /* OGSPO.cpp:119-138: ops n-1..1 with c/2, then 0 (full) .. n-1 (c/2) */
     for (int i = L - 1; i > 0; --i) split_apply(op_at(p.ops[i]), step, exponent * 0.5, psi, p.S);
@@ -1133,7 +1134,7 @@ int orc_cheby_step(int prop, int step, double *psi, int *n_hpsi) {
     });
 }
 
-int orc_gspo_create(const int *ops, int nops, int nstates, double dt) {
+int orc_gspo_create_ex(const int *ops, int nops, int nstates, double dt, int imag) {
     return guarded([&] {
         if (nops < 1) throw OracleError("GSPO is empty!");
         if (dt == 0) throw OracleError("No time step defined");
@@ -1146,9 +1147,14 @@ int orc_gspo_create(const int *ops, int nops, int nstates, double dt) {
         }
         p.S = nstates;
         p.dt = dt;
+        p.imag = imag != 0;
         g_gspo.push_back(p);
         return (int)g_gspo.size() - 1;
     });
+}
+
+int orc_gspo_create(const int *ops, int nops, int nstates, double dt) {
+    return orc_gspo_create_ex(ops, nops, nstates, dt, 0);
 }
 
 int orc_gspo_step(int prop, int step, double *psi) {

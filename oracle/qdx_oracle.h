@@ -104,6 +104,8 @@ int orc_sil_step(int prop, int step, double *psi, int *n_hpsi);
 
 /* ---- GSPO ------------------------------------------------------------ */
 int orc_gspo_create(const int *ops, int nops, int nstates, double dt);
+/* imag: imaginary time, backward (exponent -dt) */
+int orc_gspo_create_ex(const int *ops, int nops, int nstates, double dt, int imag);
 int orc_gspo_step(int prop, int step, double *psi);
 
 #ifdef __cplusplus

@@ -61,7 +61,7 @@ EXPORTS = [
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
     "qdx_ctx_set_line_points", "qdx_ctx_get_line_points", "qdx_cheby_create_imag", "qdx_bessel_i0",
-    "qdx_wf_axpy", "qdx_eigen_imag", "qdx_sil_create",
+    "qdx_wf_axpy", "qdx_eigen_imag", "qdx_sil_create", "qdx_gspo_create_imag",
     # include/qdx_io.h
     "qdx_meta_create", "qdx_meta_read", "qdx_meta_write", "qdx_meta_size", "qdx_meta_key", "qdx_meta_get",
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
@@ -165,7 +165,8 @@ def lib():
         "qdx_bessel_i0": [i32, C.c_double, dp, C.POINTER(i32)],
         "qdx_wf_axpy": [vp, C.c_double, C.c_double, vp],
         "qdx_sil_create": [vp, C.c_double, i32, C.c_double, i32, C.POINTER(vp)],
-        "qdx_eigen_imag": [vp, vp, i32, i32, C.c_double, i32, C.POINTER(vp), dp, dp, C.POINTER(i32)],
+        "qdx_gspo_create_imag": [C.POINTER(vp), i32, C.c_double, C.POINTER(vp)],
+        "qdx_eigen_imag": [vp, vp, vp, i32, i32, C.c_double, i32, C.POINTER(vp), dp, dp, C.POINTER(i32)],
         "qdx_ctx_get_line_points": [vp, C.POINTER(i32)],
         "qdx_batch_get_tiling": [vp, i32, i32, C.POINTER(Tiling)],
         "qdx_batch_set_tiling": [vp, i32, i32, C.POINTER(Tiling)],
@@ -660,9 +661,10 @@ class SIL(_Propagator):
 
 
 def eigen_imag(U, psi0: WaveFunction, nef: int, max_steps: int = 512, conv: float | None = None,
-               diag: bool = True):
+               diag: bool = True, H: Hamiltonian | None = None):
     """Eigenstates by imaginary-time relaxation (qdx_eigen_imag, ProgEigen
-    method "imag"); conv defaults to QDng's 1e-5 with diag, 1e-8 without.
+    method "imag"); conv defaults to QDng's 1e-5 with diag, 1e-8 without; H:
+    the Hamiltonian of the energies (default U's own; GSPO needs it).
     Returns (wave functions, energies, raw energies, steps per state)."""
     if conv is None:
         conv = 1e-5 if diag else 1e-8
@@ -670,8 +672,8 @@ def eigen_imag(U, psi0: WaveFunction, nef: int, max_steps: int = 512, conv: floa
     e = np.zeros(nef)
     er = np.zeros(nef)
     st = np.zeros(nef, dtype=np.int32)
-    _chk(lib().qdx_eigen_imag(U.h, psi0.h, int(nef), int(max_steps), float(conv), int(bool(diag)), hs, _dp(e),
-                              _dp(er), st.ctypes.data_as(C.POINTER(C.c_int32))))
+    _chk(lib().qdx_eigen_imag(U.h, H.h if H is not None else None, psi0.h, int(nef), int(max_steps), float(conv),
+                              int(bool(diag)), hs, _dp(e), _dp(er), st.ctypes.data_as(C.POINTER(C.c_int32))))
     out = []
     for h in hs:
         w = WaveFunction.__new__(WaveFunction)
@@ -685,11 +687,16 @@ def eigen_imag(U, psi0: WaveFunction, nef: int, max_steps: int = 512, conv: floa
 class GSPO(_Propagator):
     """OGSPO with split operators A, B, C, ... (each a Hamiltonian of one term kind)."""
 
-    def __init__(self, ops, dt: float, precision="fp64"):
+    def __init__(self, ops, dt: float, precision="fp64", imag: bool = False):
         self.ops = list(ops)
         arr = (C.c_void_p * len(self.ops))(*[o.h.value for o in self.ops])
         self.h = C.c_void_p()
-        _chk(lib().qdx_gspo_create(arr, len(self.ops), float(dt), _PREC[precision], C.byref(self.h)))
+        if imag:   # imaginary time, backward (qdx_gspo_create_imag; FP64)
+            if _PREC[precision] != FP64:
+                raise QdxError(-1, "imaginary time needs FP64")
+            _chk(lib().qdx_gspo_create_imag(arr, len(self.ops), float(dt), C.byref(self.h)))
+        else:
+            _chk(lib().qdx_gspo_create(arr, len(self.ops), float(dt), _PREC[precision], C.byref(self.h)))
         self._destroy = lib().qdx_prop_destroy
 
 

@@ -197,10 +197,11 @@ std::vector<SplitOp> gspo_sequence(qdx_prop *p) {
 }
 
 template <class R>
-XSplitDev<R> xsplit(qdx_ham *op, double tau, long step) {
+XSplitDev<R> xsplit(qdx_ham *op, double tau, long step, bool imag) {
     XSplitDev<R> x;
     x.S = op->S;
     x.tau = tau;
+    x.imag = imag ? 1 : 0;
     x.coupled = 0;
     HamDev<R> h;
     if constexpr (std::is_same<R, float>::value) h = op->dev.h32;
@@ -235,7 +236,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
                 Launch<R>::gspo_xonly(s, g, p->S, pending, psi);
                 pending.n = 0;
             }
-            pending.op[pending.n++] = xsplit<R>(o.op, o.tau, step);
+            pending.op[pending.n++] = xsplit<R>(o.op, o.tau, step, p->imag);
             ++i;
             continue;
         }
@@ -244,7 +245,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
         post.n = 0;
         size_t j = i + 1;
         while (j < seq.size() && !seq[j].kinetic && post.n < kMaxXOps) {
-            post.op[post.n++] = xsplit<R>(seq[j].op, seq[j].tau, step);
+            post.op[post.n++] = xsplit<R>(seq[j].op, seq[j].tau, step, p->imag);
             ++j;
         }
         HamDev<R> kin;
@@ -252,7 +253,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
         else kin = o.op->dev.h64;
         if (p->grid.nd == 1) {
             KTimer t(ctx, QDX_K_GSPO_ROW);
-            Launch<R>::gspo_1d(s, g, p->S, plans[0], kin, o.tau, pending, post, psi);
+            Launch<R>::gspo_1d(s, g, p->S, plans[0], kin, o.tau, pending, post, psi, p->imag);
         } else {
             R *X = p->X.as<R>();
             {
@@ -261,7 +262,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
             }
             auto col = [&](int dim, int mode) {
                 KTimer t(ctx, QDX_K_LINE_PASS);
-                Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, o.tau, X, nullptr, ConvCheck());
+                Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, o.tau, X, nullptr, ConvCheck(), p->imag);
             };
             if (p->grid.nd == 2) {
                 col(1, 3);
@@ -807,8 +808,9 @@ int qdx_cheby_coeffs(qdx_prop *p, double *coeff) {
     });
 }
 
-int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precision, qdx_prop **out) {
-    return guarded([&] {
+static void gspo_create_impl(qdx_ham *const *ops, int32_t nops, double dt, int32_t precision, bool imag,
+                             qdx_prop **out) {
+    {
         if (!ops || !out) fail(QDX_ERR_PARAM, "NULL argument");
         if (nops < 1) fail(QDX_ERR_PARAM, "GSPO is empty!");
         if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
@@ -817,6 +819,7 @@ int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precis
         p->kind = qdx_prop::GSPO;
         p->precision = precision;
         p->dt = dt;
+        p->imag = imag;
         for (int i = 0; i < nops; ++i) {
             qdx_ham *o = ops[i];
             if (!o || !o->ctx) fail(QDX_ERR_PARAM, "GSPO misses an operator. Sequence must be A, B, C, ...");
@@ -852,7 +855,15 @@ int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precis
         p->partials.alloc(kMaxPartials * sizeof(double));
         ctx_acquire(p->ctx);
         *out = p.release();
-    });
+    }
+}
+
+int qdx_gspo_create(qdx_ham *const *ops, int32_t nops, double dt, int32_t precision, qdx_prop **out) {
+    return guarded([&] { gspo_create_impl(ops, nops, dt, precision, false, out); });
+}
+
+int qdx_gspo_create_imag(qdx_ham *const *ops, int32_t nops, double dt, qdx_prop **out) {
+    return guarded([&] { gspo_create_impl(ops, nops, dt, QDX_FP64, true, out); });
 }
 
 int qdx_prop_apply(qdx_prop *p, qdx_wf *psi, int64_t step, qdx_step_stats *stats) {
@@ -1066,17 +1077,21 @@ struct WfHold {   // owns wave functions until released to the caller
 
 }  // namespace
 
-int qdx_eigen_imag(qdx_prop *U, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv, int32_t diag,
-                   qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps) {
+int qdx_eigen_imag(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv,
+                   int32_t diag, qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps) {
     return guarded([&] {
         if (!U || !psi0 || !efs || !energies) fail(QDX_ERR_PARAM, "NULL argument");
-        if ((U->kind != qdx_prop::CHEBY && U->kind != qdx_prop::SIL) || !U->imag)
-            fail(QDX_ERR_PARAM, "an imaginary-time Chebyshev or SIL propagator is required");
+        if (!U->imag) fail(QDX_ERR_PARAM, "an imaginary-time propagator is required");
         if (nef < 1) fail(QDX_ERR_PARAM, "Nef must be >= 1");
         if (max_steps < 1) fail(QDX_ERR_PARAM, "steps must be >= 1");
         qdx_wf *p0 = const_cast<qdx_wf *>(psi0);
         check_prop_wf(U, p0);
-        qdx_ham *H = U->ham;
+        // the energies' Hamiltonian: given, else the propagator's own
+        // (OPropagator::Hamiltonian, ProgEigen.cpp:697); GSPO has none
+        qdx_ham *H = Hin ? Hin : U->ham;
+        if (!H) fail(QDX_ERR_PARAM, "a Hamiltonian is required for the energies");
+        if (!(H->grid == p0->grid) || H->S != p0->S || H->ctx != p0->ctx)
+            fail(QDX_ERR_INCOMPAT, "Hamiltonian and wave function differ");
         WfHold keep, tmp;
         std::vector<qdx_wf *> ef;
         std::vector<double> e_raw(nef);

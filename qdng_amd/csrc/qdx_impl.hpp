@@ -109,6 +109,7 @@ struct XSplitDev {
     int S = 1;
     int coupled = 0;     // off-diagonal elements present
     double tau = 0;      // dt * sf
+    int imag = 0;        // imaginary time: exp(-tau W(x)) instead of exp(-i tau W(x))
     const R *V[kMaxStates][kMaxStates] = {};   // potential arrays
     const R *mu[kMaxStates][kMaxStates] = {};  // dipole arrays
     double Edip[kMaxStates][kMaxStates] = {};  // -laser[step]*scale
@@ -153,10 +154,11 @@ struct Launch {
                            double *psi, ChebyCtl *ctl, double *partials);
     // strided line pass along dimension `dim` (1 or 2) on X, in place.
     // mode 0: forward, 1: inverse, 2: forward * T/N inverse (kinetic),
-    // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator)
+    // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator),
+    // or exp(-tau T)/N with imag (imaginary time)
     static void col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode,
                          const LinePlan &p, const HamDev<R> &h, double tau, R *X,
-                         ChebyCtl *ctl, const ConvCheck &cc);
+                         ChebyCtl *ctl, const ConvCheck &cc, int imag = 0);
     // kinetic part along dimension `dim` of the separable T = sum_d T_d:
     // Y = IFFT_d(T_d/N_d FFT_d(src)), or Y += that when `accumulate`.  The
     // first pass of an order also evaluates the break test of the previous
@@ -175,7 +177,7 @@ struct Launch {
                               const XGroupDev<R> &post, const R *X, R *psi_out);
     static void gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
                         const HamDev<R> &kin, double tau_kin, const XGroupDev<R> &pre,
-                        const XGroupDev<R> &post, R *psi);
+                        const XGroupDev<R> &post, R *psi, int imag = 0);
     static void gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp,
                            R *psi);
 };

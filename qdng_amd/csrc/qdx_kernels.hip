@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
 template <class R, int MODE, int P = 8>
 __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
                                               cx<R> *__restrict__ X, const cx<R> *__restrict__ src,
-                                              ChebyCtl *ctl, ConvCheck cc) {
+                                              ChebyCtl *ctl, ConvCheck cc, int kimag) {
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
@@ -527,6 +527,10 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
                 } else if (MODE == 2) {
                     const double t = tt * h.inv_n;
                     v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
+                } else if (kimag) {
+                    // exp(-tau T) / N, imaginary time
+                    const double f = exp(-tau * tt) * h.inv_n;
+                    v[q] = mk<R>((R)((double)v[q].r * f), (R)((double)v[q].i * f));
                 } else {
                     // exp(-i tau T) / N (OGSPO.cpp:37-41 folds 1/N into the exponential)
                     double sn, cs;
@@ -632,6 +636,11 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
     if (!op.coupled || S == 1) {
 #pragma unroll
         for (int i = 0; i < S; ++i) {
+            if (op.imag) {   // exp(-tau w), imaginary time
+                const double f = exp(-op.tau * Wm[i][i]);
+                v[i] = mk<double>(v[i].r * f, v[i].i * f);
+                continue;
+            }
             double sn, cs;
             sincos(-op.tau * Wm[i][i], &sn, &cs);
             v[i] = cmul(v[i], mk<double>(cs, sn));
@@ -639,6 +648,22 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
         return;
     }
     if constexpr (S == 2) {
+        if (op.imag) {
+            // exp(-tau W) = e^{-tau m} [cosh(tau O) - sinh(tau O)/O (W - m)]
+            const double m = 0.5 * (Wm[0][0] + Wm[1][1]);
+            const double d = 0.5 * (Wm[0][0] - Wm[1][1]);
+            const double b = Wm[0][1];
+            const double O = sqrt(d * d + b * b);
+            const double ch = cosh(op.tau * O);
+            const double sh = O > 0 ? sinh(op.tau * O) / O : op.tau;
+            const double e = exp(-op.tau * m);
+            const double u00 = e * (ch - sh * d), u11 = e * (ch + sh * d), u01 = -e * sh * b;
+            const cx<double> a0 = mk<double>(u00 * v[0].r + u01 * v[1].r, u00 * v[0].i + u01 * v[1].i);
+            const cx<double> a1 = mk<double>(u01 * v[0].r + u11 * v[1].r, u01 * v[0].i + u11 * v[1].i);
+            v[0] = a0;
+            v[1] = a1;
+            return;
+        }
         // exp(-i tau W) = e^{-i tau m} [cos(tau O) - i sin(tau O)/O (W - m)]
         const double m = 0.5 * (Wm[0][0] + Wm[1][1]);
         const double d = 0.5 * (Wm[0][0] - Wm[1][1]);
@@ -669,7 +694,7 @@ __device__ __forceinline__ void xgroup_point(const XGroupDev<R> &grp, long gi, c
 // on all states of a point at once, so the S lines of a row are exchanged
 // through LDS around them.
 template <class R, int S, bool ONE_D, int PHASE>
-__global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, LinePlan p0, int rb,
+__global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, int kimag, LinePlan p0, int rb,
                                                    XGroupDev<R> pre, XGroupDev<R> post, cx<R> *__restrict__ psi,
                                                    cx<R> *__restrict__ X) {
     extern __shared__ __align__(16) char smem_raw[];
@@ -740,6 +765,11 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const double tt = has_t ? 0.0 + kin.tk[st][0][B.lt + q * B.TL] : 0.0;
+                if (kimag) {   // exp(-tau T)/N, imaginary time
+                    const double f = exp(-tau * tt) * kin.inv_n;
+                    v[q] = mk<R>((R)((double)v[q].r * f), (R)((double)v[q].i * f));
+                    continue;
+                }
                 double sn, cs;
                 sincos(-tau * tt, &sn, &cs);
                 v[q] = from_d<R>(cmul(to_d(v[q]), mk<double>(cs * kin.inv_n, sn * kin.inv_n)));
@@ -1036,7 +1066,7 @@ void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, 
 
 template <class R>
 void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
-                         const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc) {
+                         const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag) {
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
@@ -1050,19 +1080,19 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     switch (mode) {
     case 0:
         set_lds(k_col<R, 0>, lds);
-        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
         break;
     case 1:
         set_lds(k_col<R, 1>, lds);
-        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
         break;
     case 2:
         set_lds(k_col<R, 2>, lds);
-        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
         break;
     default:
         set_lds(k_col<R, 3>, lds);
-        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
         break;
     }
     QDX_HIP(hipGetLastError());
@@ -1083,10 +1113,10 @@ static void kin_pass_P(hipStream_t s, const GridDev &g, int S, int dim, bool acc
     pl.tw_lds = 1;
     if (accumulate) {
         set_lds(k_col<R, 5, P>, lds);
-        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc, 0);
     } else {
         set_lds(k_col<R, 4, P>, lds);
-        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc, 0);
     }
     QDX_HIP(hipGetLastError());
 }
@@ -1137,12 +1167,13 @@ void Launch<R>::ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h, c
 
 template <class R, bool ONE_D, int PHASE>
 static void gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
-                            const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X) {
+                            const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X,
+                            int kimag = 0) {
     QDX_DISPATCH_S(S, {
         LinePlan p0 = plan;
         RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
         set_lds(k_gspo_row<R, SS, ONE_D, PHASE>, r.lds);
-        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, p0,
+        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, kimag, p0,
                            r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
     });
     QDX_HIP(hipGetLastError());
@@ -1166,8 +1197,8 @@ void Launch<R>::gspo_row_post(hipStream_t s, const GridDev &g, int S, const Line
 
 template <class R>
 void Launch<R>::gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const HamDev<R> &kin,
-                        double tau, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi) {
-    gspo_row_launch<R, true, 0>(s, g, S, kin, tau, p0, pre, post, psi, nullptr);
+                        double tau, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, int imag) {
+    gspo_row_launch<R, true, 0>(s, g, S, kin, tau, p0, pre, post, psi, nullptr, imag);
 }
 
 template <class R>

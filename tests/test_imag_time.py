@@ -362,3 +362,88 @@ def test_device_sil_eigenstates_match_qdng_tge(ctx):
     assert_tge_energies(e)
     with pytest.raises(Q.QdxError):
         Q.eigen_imag(Q.SIL(p.gpu_ham(ctx), 5.0, order=25), p.gpu_wf(ctx), 2)   # real time: rejected
+
+
+# ---------------------------------------------------------------------------
+# GSPO in imaginary time (exponent -dt): every split operator a real decay
+
+def test_oracle_gspo_imag_is_strang_product():
+    # ops (kin, pot): exp(-V dt/2) exp(-T dt) exp(-V dt/2) (OGSPO.cpp:114-139)
+    p = tge_problem()
+    kin = p.oracle_split_op(["kin"])
+    pot = p.oracle_split_op(["pot"])
+    n = p.size
+    Tm = np.zeros((n, n), dtype=complex)
+    for j in range(n):
+        e = np.zeros(n, dtype=complex)
+        e[j] = 1
+        Tm[:, j] = kin.apply(e)
+    v = pot.apply(np.ones(n, dtype=complex)).real
+    wt, Vt = np.linalg.eigh(0.5 * (Tm + Tm.conj().T))
+    dt = 5.0
+    half = np.exp(-0.5 * dt * v)
+    ref = half * (Vt @ (np.exp(-dt * wt) * (Vt.conj().T @ (half * p.psi0))))
+    G = O.GSPO([kin, pot], dt, 1, imag=True)
+    psi = p.psi0.copy()
+    G.step(psi, 0)
+    assert np.linalg.norm(psi - ref) / np.linalg.norm(ref) < 1e-12
+    # real time differs completely
+    Gr = O.GSPO([kin, pot], dt, 1)
+    psr = p.psi0.copy()
+    Gr.step(psr, 0)
+    assert np.linalg.norm(psr - ref) > 0.1 * np.linalg.norm(ref)
+
+
+def test_oracle_gspo_eigenstates_match_qdng_tge():
+    # TGE-GSPO (tests/TGE-GSPO.qdi.in: A kinetic, B potential, dt 5) checks
+    # the same energies as TGE-Cheby (checktests.sh:124-127)
+    p = tge_problem()
+    H = p.oracle_ham()
+    g = p.oracle_grid()
+    G = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], 5.0, 1, imag=True)
+
+    def project_out(psi, efs):
+        buf = np.zeros_like(psi)
+        for ef in efs:
+            buf += ef * O.overlap(g, ef, psi)
+        psi -= buf
+
+    def normalize(psi):
+        psi /= np.sqrt(O.norm2(g, psi))
+
+    e_raw, e_diag, steps = imag_time_ef(
+        lambda psi, s: G.step(psi, s), project_out, normalize, lambda a, b: O.overlap(g, a, b),
+        lambda w: H.expec(w), lambda bra, ket: O.overlap(g, bra, H.apply(ket)), lambda: p.psi0.copy(),
+        10, 1000, 1e-5)
+    assert_tge_energies(e_diag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,builder,dt,steps", [
+    ("tge_1d", tge_problem, 5.0, 4),
+    ("morse_2state_2d_coupled", lambda: cases.morse_2state_2d(64), 5.0, 3),
+    ("anharmonic_3d", lambda: cases.anharmonic_3d(16), 3.5, 2),
+])
+def test_device_gspo_imag_matches_oracle(ctx, name, builder, dt, steps):
+    import qdng_amd as Q
+    p = builder()
+    G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], dt, imag=True)
+    Go = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], dt, p.S, imag=True)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for s in range(steps):
+        G.apply(w, s)
+        Go.step(psi, s)
+    assert np.linalg.norm(w.download() - psi) / np.linalg.norm(psi) < 1e-12
+
+
+@pytest.mark.gpu
+def test_device_gspo_eigenstates_match_qdng_tge(ctx):
+    import qdng_amd as Q
+    p = tge_problem()
+    G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], 5.0, imag=True)
+    with pytest.raises(Q.QdxError):   # split operators form no Hamiltonian: H is required
+        Q.eigen_imag(G, p.gpu_wf(ctx), 2)
+    efs, e, e_raw, steps = Q.eigen_imag(G, p.gpu_wf(ctx), 10, max_steps=1000, conv=1e-5, diag=True,
+                                        H=p.gpu_ham(ctx))
+    assert_tge_energies(e)
