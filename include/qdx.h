@@ -169,11 +169,16 @@ int qdx_cheby_info_get(qdx_prop *p, qdx_cheby_info *info);
  * products, vector updates), the tridiagonal matrix is diagonalised on the
  * host.  FP64. */
 int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t imag, qdx_prop **out);
+/* OArnoldi (OArnoldi.cpp:64-190): Arnoldi with at most `order` Krylov vectors
+ * (<= 0: 20), convergence bound conv (<= 0: 1e-12), OArnoldi.h:11-13;
+ * imag as for qdx_sil_create.  Modified Gram-Schmidt on the device; the
+ * Hessenberg matrix exponential exp(c H_A) e_0 on the host.  FP64. */
+int qdx_arnoldi_create(qdx_ham *h, double dt, int32_t order, double conv, int32_t imag, qdx_prop **out);
 /* Eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF,
  * ProgEigen.cpp:318-412): for i < nef, psi = psi0; repeat at most max_steps
  * times while |1 - <psi_old|psi>| > conv: one step of U (from
- * qdx_cheby_create_imag, qdx_gspo_create_imag, or qdx_sil_create with
- * imag), remove the states found so far (OProjection, psi -= sum_j ef_j
+ * qdx_cheby_create_imag, qdx_gspo_create_imag, or qdx_sil_create /
+ * qdx_arnoldi_create with imag), remove the states found so far (OProjection, psi -= sum_j ef_j
  * <ef_j|psi>), normalise; the raw energy is <ef_i|H|ef_i> with H the given
  * Hamiltonian, or U's own when H is NULL (required for GSPO, whose split
  * operators form no Hamiltonian).  diag != 0 (QDng's default, DiagBasis,

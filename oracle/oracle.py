@@ -73,6 +73,8 @@ def lib():
             "orc_cheby_step": [C.c_int, C.c_int, dp, ip],
             "orc_sil_create": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int],
             "orc_sil_step": [C.c_int, C.c_int, dp, ip],
+            "orc_arnoldi_create": [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int],
+            "orc_arnoldi_step": [C.c_int, C.c_int, dp, ip],
             "orc_gspo_create": [ip, C.c_int, C.c_int, C.c_double],
             "orc_gspo_create_ex": [ip, C.c_int, C.c_int, C.c_double, C.c_int],
             "orc_gspo_step": [C.c_int, C.c_int, dp],
@@ -277,6 +279,20 @@ class SIL:
     def step(self, psi: np.ndarray, step: int = 0) -> int:
         n = C.c_int(0)
         _chk(lib().orc_sil_step(self.id, int(step), _dp(psi), C.byref(n)))
+        return n.value
+
+
+class Arnoldi:
+    """Arnoldi propagator (QDng OArnoldi): real time forward, or imaginary
+    time backward (imag=True)."""
+
+    def __init__(self, H: Op, dt: float, order: int = 0, conv: float = 0.0, imag: bool = False):
+        self.id = _chk(lib().orc_arnoldi_create(H.id, H.nstates, float(dt), int(order), float(conv),
+                                                int(bool(imag))))
+
+    def step(self, psi: np.ndarray, step: int = 0) -> int:
+        n = C.c_int(0)
+        _chk(lib().orc_arnoldi_step(self.id, int(step), _dp(psi), C.byref(n)))
         return n.value
 
 

@@ -729,6 +729,100 @@ int sil_step(const SIL &c, int step, cplx *psi) {
 }
 
 /* ======================================================================
+ * Arnoldi (OArnoldi.cpp:64-190)
+ * ==================================================================== */
+struct Arnoldi {
+    int ham = -1;
+    int S = 1;
+    double dt = 0;
+    int maxorder = 20;   /* OArnoldi.h:11 ARNOLDI_DEF_ORDER */
+    double conv = 1e-12; /* OArnoldi.h:13 ARNOLDI_CONV */
+    bool imag = false;
+    int order = 20;      /* _order persists between steps (OArnoldi.cpp:44) */
+};
+std::vector<Arnoldi> g_arnoldi;
+
+/* exp(A) e_0 for a small complex matrix: scaling and squaring with a Taylor
+ * series summed to machine precision (independent of the library's Pade) */
+std::vector<cplx> expm_col0(int n, std::vector<cplx> A) {
+    double nrm = 0;
+    for (int j = 0; j < n; ++j) {
+        double c = 0;
+        for (int i = 0; i < n; ++i) c += std::abs(A[(size_t)i * n + j]);
+        nrm = std::max(nrm, c);
+    }
+    int sq = 0;
+    while (nrm > 0.5) {
+        nrm /= 2;
+        ++sq;
+    }
+    const double f = std::ldexp(1.0, -sq);
+    for (auto &a : A) a *= f;
+    auto mul = [&](const std::vector<cplx> &X, const std::vector<cplx> &Y) {
+        std::vector<cplx> Z((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k)
+                for (int j = 0; j < n; ++j) Z[(size_t)i * n + j] += X[(size_t)i * n + k] * Y[(size_t)k * n + j];
+        return Z;
+    };
+    std::vector<cplx> E((size_t)n * n, 0.0), term((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) E[(size_t)i * n + i] = term[(size_t)i * n + i] = 1.0;
+    for (int k = 1; k < 40; ++k) {
+        term = mul(term, A);
+        for (auto &t : term) t /= double(k);
+        for (size_t i = 0; i < E.size(); ++i) E[i] += term[i];
+    }
+    for (int i = 0; i < sq; ++i) E = mul(E, E);
+    std::vector<cplx> c0(n);
+    for (int i = 0; i < n; ++i) c0[i] = E[(size_t)i * n];
+    return c0;
+}
+
+int arnoldi_step(Arnoldi &c, int step, cplx *psi) {
+    const Op &H = op_at(c.ham);
+    const Grid &g = g_grids.at(op_grid(H));
+    const long n = op_size(H) * c.S;
+    const int M = c.maxorder;
+    auto norm = [&](const cplx *v) { return std::sqrt(overlap(g, c.S, v, v).real()); };
+    std::vector<std::vector<cplx>> q(M, std::vector<cplx>(n));
+    std::vector<cplx> buf0(n), tHA((size_t)M * M, 0.0);
+    int napply = 0;
+    const double psinorm = norm(psi);
+    q[0].assign(psi, psi + n);
+    for (int it = 1; it < M; it++) {
+        op_apply(H, step, q[it - 1].data(), buf0.data());
+        ++napply;
+        for (int j = 0; j <= it - 1; j++) {                           /* modified Gram-Schmidt */
+            const cplx h = overlap(g, c.S, q[j].data(), buf0.data());
+            tHA[(size_t)j * M + it - 1] = h;
+            for (long k = 0; k < n; ++k) buf0[k] += q[j][k] * (-1.0 * h);
+        }
+        const double nb = norm(buf0.data());
+        if (nb < 1e-11) break;                                        /* ARNOLDI_UNDERRUN */
+        tHA[(size_t)it * M + it - 1] = nb;
+        for (long k = 0; k < n; ++k) buf0[k] *= 1. / nb;
+        q[it] = buf0;
+        c.order = it + 1;
+        cplx err = std::pow(c.dt, it - 2) * tHA[(size_t)1 * M + 0] * psinorm;
+        for (int i = 1; i < it - 2; i++) err *= tHA[(size_t)(i + 1) * M + i] / double(i);
+        if (std::abs(err) < c.conv) break;
+    }
+    const int order = c.order;
+    op_apply(H, step, q[order - 1].data(), buf0.data());               /* last column */
+    ++napply;
+    for (int j = 0; j < order; j++) tHA[(size_t)j * M + order - 1] = overlap(g, c.S, q[j].data(), buf0.data());
+    const cplx expo = c.imag ? cplx(-c.dt, 0) : cplx(0, -c.dt);
+    std::vector<cplx> A((size_t)order * order);
+    for (int i = 0; i < order; ++i)
+        for (int j = 0; j < order; ++j) A[(size_t)i * order + j] = expo * tHA[(size_t)i * M + j];
+    const std::vector<cplx> vect = expm_col0(order, A);               /* ZR e^{c Lambda} ZR^-1 e_0 */
+    for (long k = 0; k < n; ++k) psi[k] *= vect[0];
+    for (int i = 1; i < order; ++i)
+        for (long k = 0; k < n; ++k) psi[k] += q[i][k] * vect[i];
+    return napply;
+}
+
+/* ======================================================================
  * GSPO (OGSPO.cpp)
  * ==================================================================== */
 struct GSPO {
@@ -1111,6 +1205,30 @@ int orc_sil_create(int ham, int nstates, double dt, int order, double err, int i
 int orc_sil_step(int prop, int step, double *psi, int *n_hpsi) {
     return guarded([&] {
         int n = sil_step(g_sil.at(prop), step, reinterpret_cast<cplx *>(psi));
+        if (n_hpsi) *n_hpsi = n;
+        return 0;
+    });
+}
+
+int orc_arnoldi_create(int ham, int nstates, double dt, int order, double conv, int imag) {
+    return guarded([&] {
+        Arnoldi c;
+        c.ham = ham;
+        c.S = nstates;
+        c.dt = dt;
+        if (order > 0) c.maxorder = order;
+        if (c.maxorder < 3) throw OracleError("Arnoldi: order doesn't make sense");
+        c.order = c.maxorder;
+        if (conv > 0) c.conv = conv;
+        c.imag = imag != 0;
+        g_arnoldi.push_back(c);
+        return (int)g_arnoldi.size() - 1;
+    });
+}
+
+int orc_arnoldi_step(int prop, int step, double *psi, int *n_hpsi) {
+    return guarded([&] {
+        int n = arnoldi_step(g_arnoldi.at(prop), step, reinterpret_cast<cplx *>(psi));
         if (n_hpsi) *n_hpsi = n;
         return 0;
     });

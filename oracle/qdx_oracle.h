@@ -102,6 +102,11 @@ int orc_cheby_step(int prop, int step, double *psi, int *n_hpsi);
 int orc_sil_create(int ham, int nstates, double dt, int order, double err, int imag);
 int orc_sil_step(int prop, int step, double *psi, int *n_hpsi);
 
+/* ---- Arnoldi (OArnoldi.cpp) -------------------------------------------- */
+/* order <= 0: 20; conv <= 0: 1e-12 (OArnoldi.h:11-13) */
+int orc_arnoldi_create(int ham, int nstates, double dt, int order, double conv, int imag);
+int orc_arnoldi_step(int prop, int step, double *psi, int *n_hpsi);
+
 /* ---- GSPO ------------------------------------------------------------ */
 int orc_gspo_create(const int *ops, int nops, int nstates, double dt);
 /* imag: imaginary time, backward (exponent -dt) */

@@ -61,7 +61,7 @@ EXPORTS = [
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
     "qdx_ctx_set_line_points", "qdx_ctx_get_line_points", "qdx_cheby_create_imag", "qdx_bessel_i0",
-    "qdx_wf_axpy", "qdx_eigen_imag", "qdx_sil_create", "qdx_gspo_create_imag",
+    "qdx_wf_axpy", "qdx_eigen_imag", "qdx_sil_create", "qdx_gspo_create_imag", "qdx_arnoldi_create",
     # include/qdx_io.h
     "qdx_meta_create", "qdx_meta_read", "qdx_meta_write", "qdx_meta_size", "qdx_meta_key", "qdx_meta_get",
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
@@ -166,6 +166,7 @@ def lib():
         "qdx_wf_axpy": [vp, C.c_double, C.c_double, vp],
         "qdx_sil_create": [vp, C.c_double, i32, C.c_double, i32, C.POINTER(vp)],
         "qdx_gspo_create_imag": [C.POINTER(vp), i32, C.c_double, C.POINTER(vp)],
+        "qdx_arnoldi_create": [vp, C.c_double, i32, C.c_double, i32, C.POINTER(vp)],
         "qdx_eigen_imag": [vp, vp, vp, i32, i32, C.c_double, i32, C.POINTER(vp), dp, dp, C.POINTER(i32)],
         "qdx_ctx_get_line_points": [vp, C.POINTER(i32)],
         "qdx_batch_get_tiling": [vp, i32, i32, C.POINTER(Tiling)],
@@ -657,6 +658,17 @@ class SIL(_Propagator):
         self.H = H
         self.h = C.c_void_p()
         _chk(lib().qdx_sil_create(H.h, float(dt), int(order), float(err), int(bool(imag)), C.byref(self.h)))
+        self._destroy = lib().qdx_prop_destroy
+
+
+class Arnoldi(_Propagator):
+    """OArnoldi (qdx_arnoldi_create): real time forward, or imaginary time
+    backward (imag=True); FP64."""
+
+    def __init__(self, H: Hamiltonian, dt: float, order: int = 0, conv: float = 0.0, imag: bool = False):
+        self.H = H
+        self.h = C.c_void_p()
+        _chk(lib().qdx_arnoldi_create(H.h, float(dt), int(order), float(conv), int(bool(imag)), C.byref(self.h)))
         self._destroy = lib().qdx_prop_destroy
 
 

@@ -19,7 +19,7 @@
 // ---------------------------------------------------------------------------
 // propagators
 struct qdx_prop {
-    enum Kind { CHEBY, GSPO, SIL } kind;
+    enum Kind { CHEBY, GSPO, SIL, ARNOLDI } kind;
     qdx_ctx *ctx = nullptr;
     int precision = QDX_FP64;
     double dt = 0;
@@ -42,7 +42,10 @@ struct qdx_prop {
     double sil_err = 1e-12;
     std::vector<qdx_wf *> lzb;
     qdx_wf *sil_buf = nullptr;
-    int sil_napply = 0;   // H applications of the last step
+    int sil_napply = 0;   // H applications of the last step (SIL, Arnoldi)
+    // Arnoldi (the Krylov vectors in lzb, the buffer in sil_buf)
+    double arn_conv = 1e-12;
+    int arn_order = 0;    // converged order; kept between steps as OArnoldi does
     // GSPO
     std::vector<qdx_ham *> ops;
     DevBuf psi32, psi0_32;
@@ -308,10 +311,15 @@ double cheby_last_nconv(qdx_prop *p) {
 }
 
 void sil_step(qdx_prop *p, qdx_wf *psi, long step);
+void arnoldi_step(qdx_prop *p, qdx_wf *psi, long step);
 
 void prop_step(qdx_prop *p, qdx_wf *psi, long step) {
     if (p->kind == qdx_prop::SIL) {
         sil_step(p, psi, step);
+        return;
+    }
+    if (p->kind == qdx_prop::ARNOLDI) {
+        arnoldi_step(p, psi, step);
         return;
     }
     if (p->kind == qdx_prop::CHEBY) {
@@ -883,7 +891,7 @@ int qdx_prop_apply(qdx_prop *p, qdx_wf *psi, int64_t step, qdx_step_stats *stats
                 HIPCHK(hipStreamSynchronize(s));
                 stats->n_hpsi = c.last_order;
                 stats->nconv = cheby_last_nconv(p);
-            } else if (p->kind == qdx_prop::SIL) {
+            } else if (p->kind == qdx_prop::SIL || p->kind == qdx_prop::ARNOLDI) {
                 stats->n_hpsi = p->sil_napply;
                 stats->nconv = 0;
             } else {
@@ -944,7 +952,7 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
             if (p->kind == qdx_prop::CHEBY)
                 HIPCHK(hipMemcpyAsync(orders + i, &p->ctl.as<ChebyCtl>()->last_order, sizeof(int),
                                       hipMemcpyDeviceToDevice, s));
-            if (p->kind == qdx_prop::SIL) sil_counts.push_back(p->sil_napply);
+            if (p->kind == qdx_prop::SIL || p->kind == qdx_prop::ARNOLDI) sil_counts.push_back(p->sil_napply);
         }
         if (g32) launch_convert_c64_to_c128(s, p->psi32.as<float>(), psi->ptr(), (long)S * n);
         std::vector<double> host((size_t)nsteps * per_step + nsteps);
@@ -971,7 +979,9 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
                 autocorr[2 * i] = re * w;
                 autocorr[2 * i + 1] = im * w;
             }
-            const int cnt = p->kind == qdx_prop::CHEBY ? hord[i] : p->kind == qdx_prop::SIL ? sil_counts[i] : 1;
+            const int cnt = p->kind == qdx_prop::CHEBY                                   ? hord[i]
+                            : (p->kind == qdx_prop::SIL || p->kind == qdx_prop::ARNOLDI) ? sil_counts[i]
+                                                                                         : 1;
             if (n_hpsi_per_step) n_hpsi_per_step[i] = cnt;
             total += cnt;
         }
@@ -1052,6 +1062,92 @@ void ovl(qdx_wf *a, qdx_wf *b, std::complex<double> &z) {
     double o[2];
     chk(qdx_wf_overlap(a, b, o));
     z = {o[0], o[1]};
+}
+
+// exp(A) e_0 for a small complex matrix A (n x n, row-major): scaling and
+// squaring with the [13/13] Pade approximant (N. J. Higham, "The scaling and
+// squaring method for the matrix exponential revisited", SIAM J. Matrix
+// Anal. Appl. 26 (2005) 1179-1193), the linear system by LU with partial
+// pivoting
+void expm_col0(int n, const std::vector<std::complex<double>> &Ain, std::vector<std::complex<double>> &col0) {
+    using cd = std::complex<double>;
+    static const double b[14] = {64764752532480000.0, 32382376266240000.0, 7771770303897600.0,
+                                 1187353796428800.0,  129060195264000.0,   10559470521600.0,
+                                 670442572800.0,      33522128640.0,       1323241920.0,
+                                 40840800.0,          960960.0,            16380.0,
+                                 182.0,               1.0};
+    const double theta13 = 5.371920351148152;
+    auto idx = [n](int i, int j) { return (size_t)i * n + j; };
+    auto mul = [&](const std::vector<cd> &X, const std::vector<cd> &Y) {
+        std::vector<cd> Z((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < n; ++k) {
+                const cd x = X[idx(i, k)];
+                if (x == cd(0)) continue;
+                for (int j = 0; j < n; ++j) Z[idx(i, j)] += x * Y[idx(k, j)];
+            }
+        return Z;
+    };
+    double nrm = 0;   // 1-norm
+    for (int j = 0; j < n; ++j) {
+        double c = 0;
+        for (int i = 0; i < n; ++i) c += std::abs(Ain[idx(i, j)]);
+        nrm = std::max(nrm, c);
+    }
+    int sq = 0;
+    if (nrm > theta13) sq = (int)std::ceil(std::log2(nrm / theta13));
+    std::vector<cd> A = Ain;
+    const double f = std::ldexp(1.0, -sq);
+    for (auto &a : A) a *= f;
+    const std::vector<cd> A2 = mul(A, A), A4 = mul(A2, A2), A6 = mul(A4, A2);
+    std::vector<cd> U1((size_t)n * n), V((size_t)n * n), W1((size_t)n * n), W2((size_t)n * n);
+    for (size_t k = 0; k < A.size(); ++k) {
+        W1[k] = b[13] * A6[k] + b[11] * A4[k] + b[9] * A2[k];
+        W2[k] = b[12] * A6[k] + b[10] * A4[k] + b[8] * A2[k];
+    }
+    std::vector<cd> T1 = mul(A6, W1), T2 = mul(A6, W2);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const size_t k = idx(i, j);
+            const double I = i == j ? 1.0 : 0.0;
+            U1[k] = T1[k] + b[7] * A6[k] + b[5] * A4[k] + b[3] * A2[k] + b[1] * I;
+            V[k] = T2[k] + b[6] * A6[k] + b[4] * A4[k] + b[2] * A2[k] + b[0] * I;
+        }
+    const std::vector<cd> U = mul(A, U1);
+    // (V - U) X = (V + U)
+    std::vector<cd> P((size_t)n * n), Q((size_t)n * n);
+    for (size_t k = 0; k < P.size(); ++k) {
+        P[k] = V[k] - U[k];
+        Q[k] = V[k] + U[k];
+    }
+    std::vector<int> piv(n);
+    for (int c = 0; c < n; ++c) {
+        int pr = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::abs(P[idx(r, c)]) > std::abs(P[idx(pr, c)])) pr = r;
+        if (P[idx(pr, c)] == cd(0)) fail(QDX_ERR_OVERFLOW, "matrix exponential: singular Pade denominator");
+        if (pr != c)
+            for (int j = 0; j < n; ++j) {
+                std::swap(P[idx(c, j)], P[idx(pr, j)]);
+                std::swap(Q[idx(c, j)], Q[idx(pr, j)]);
+            }
+        for (int r = c + 1; r < n; ++r) {
+            const cd m = P[idx(r, c)] / P[idx(c, c)];
+            if (m == cd(0)) continue;
+            for (int j = c; j < n; ++j) P[idx(r, j)] -= m * P[idx(c, j)];
+            for (int j = 0; j < n; ++j) Q[idx(r, j)] -= m * Q[idx(c, j)];
+        }
+    }
+    std::vector<cd> X((size_t)n * n);
+    for (int j = 0; j < n; ++j)
+        for (int r = n - 1; r >= 0; --r) {
+            cd t = Q[idx(r, j)];
+            for (int k = r + 1; k < n; ++k) t -= P[idx(r, k)] * X[idx(k, j)];
+            X[idx(r, j)] = t / P[idx(r, r)];
+        }
+    for (int i = 0; i < sq; ++i) X = mul(X, X);
+    col0.resize(n);
+    for (int i = 0; i < n; ++i) col0[i] = X[idx(i, 0)];
 }
 
 struct WfHold {   // owns wave functions until released to the caller
@@ -1161,15 +1257,18 @@ int qdx_eigen_imag(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nef, i
     });
 }
 
-int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t imag, qdx_prop **out) {
-    return guarded([&] {
+// SIL and Arnoldi: `order` Krylov vectors and one buffer as device wave
+// functions
+static void krylov_create(qdx_ham *h, double dt, int32_t order, double err, int32_t imag, bool arnoldi,
+                          qdx_prop **out) {
+    {
         if (!h || !out) fail(QDX_ERR_PARAM, "NULL argument");
         if (!h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
         if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
-        if (order <= 0) order = 20;                        // OSIL.h:11
-        if (order < 3) fail(QDX_ERR_PARAM, "SIL: order doesn't make sense");
+        if (order <= 0) order = 20;                        // OSIL.h:11, OArnoldi.h:11
+        if (order < 3) fail(QDX_ERR_PARAM, arnoldi ? "Arnoldi: order doesn't make sense" : "SIL: order doesn't make sense");
         auto p = std::make_unique<qdx_prop>();
-        p->kind = qdx_prop::SIL;
+        p->kind = arnoldi ? qdx_prop::ARNOLDI : qdx_prop::SIL;
         p->ctx = h->ctx;
         p->precision = QDX_FP64;
         p->dt = dt;
@@ -1179,6 +1278,8 @@ int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t ima
         p->imag = imag != 0;
         p->sil_order = order;
         p->sil_err = err > 0 ? err : 1e-12;                // OSIL.h:12
+        p->arn_conv = err > 0 ? err : 1e-12;               // OArnoldi.h:13
+        p->arn_order = order;
         qdx_grid g{};
         g.ndims = h->grid.nd;
         for (int d = 0; d < h->grid.nd; ++d) {
@@ -1205,7 +1306,15 @@ int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t ima
         }
         ctx_acquire(p->ctx);
         *out = p.release();
-    });
+    }
+}
+
+int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t imag, qdx_prop **out) {
+    return guarded([&] { krylov_create(h, dt, order, err, imag, false, out); });
+}
+
+int qdx_arnoldi_create(qdx_ham *h, double dt, int32_t order, double conv, int32_t imag, qdx_prop **out) {
+    return guarded([&] { krylov_create(h, dt, order, conv, imag, true, out); });
 }
 
 int qdx_prop_destroy(qdx_prop *p) {
@@ -1290,6 +1399,61 @@ void sil_step(qdx_prop *p, qdx_wf *psi, long step) {
     // psi = vect_0 psi + sum_{i < convorder} vect_i q_i (q_0 holds psi)
     launch_scale_axpy(s, psi->ptr(), q[0]->ptr(), n, 0.0, vect[0].real(), vect[0].imag());
     for (int i = 1; i < convorder; ++i) chk(qdx_wf_axpy(psi, vect[i].real(), vect[i].imag(), q[i]));
+    p->sil_napply = napply;
+}
+
+// OArnoldi::BuildLZB + DiagLZB + Propagate (OArnoldi.cpp:64-190) over device
+// operations: modified Gram-Schmidt against all previous vectors, the
+// Hessenberg matrix's last column from plain products with H q, and the
+// combination exp(c H_A) e_0 (here by Pade instead of QDng's
+// eigendecomposition ZR e^{c Lambda} ZR^-1: the same matrix function).
+// The converged order carries over to the next step, as in OArnoldi.
+void arnoldi_step(qdx_prop *p, qdx_wf *psi, long step) {
+    const int M = p->sil_order;
+    std::vector<qdx_wf *> &q = p->lzb;
+    qdx_wf *buf = p->sil_buf;
+    qdx_ham *H = p->ham;
+    hipStream_t s = p->ctx->stream;
+    const long n = (long)p->S * psi->n();
+    std::vector<std::complex<double>> tHA((size_t)M * M, 0.0);
+    int napply = 0;
+    std::complex<double> z;
+    double psinorm = 0;
+    chk(qdx_wf_norm(psi, &psinorm, nullptr));
+    chk(qdx_wf_copy(q[0], psi));
+    for (int it = 1; it < M; it++) {
+        chk(qdx_ham_apply(H, buf, q[it - 1], step));
+        ++napply;
+        for (int j = 0; j <= it - 1; j++) {
+            ovl(q[j], buf, z);
+            tHA[(size_t)j * M + it - 1] = z;
+            chk(qdx_wf_axpy(buf, -z.real(), -z.imag(), q[j]));
+        }
+        double nb = 0;
+        chk(qdx_wf_norm(buf, &nb, nullptr));
+        if (nb < 1e-11) break;   // ARNOLDI_UNDERRUN (OArnoldi.h:12)
+        tHA[(size_t)it * M + it - 1] = nb;
+        chk(qdx_wf_copy(q[it], buf));
+        launch_scale_axpy(s, q[it]->ptr(), nullptr, n, 1. / nb, 0.0, 0.0);
+        p->arn_order = it + 1;
+        std::complex<double> err = std::pow(p->dt, it - 2) * tHA[(size_t)1 * M + 0] * psinorm;
+        for (int i = 1; i < it - 2; i++) err *= tHA[(size_t)(i + 1) * M + i] / double(i);
+        if (std::abs(err) < p->arn_conv) break;
+    }
+    const int order = p->arn_order;
+    chk(qdx_ham_apply(H, buf, q[order - 1], step));
+    ++napply;
+    for (int j = 0; j < order; j++) {
+        ovl(q[j], buf, z);
+        tHA[(size_t)j * M + order - 1] = z;
+    }
+    const std::complex<double> expo = p->imag ? std::complex<double>(-p->dt, 0) : std::complex<double>(0, -p->dt);
+    std::vector<std::complex<double>> A((size_t)order * order), vect;
+    for (int i = 0; i < order; ++i)
+        for (int j = 0; j < order; ++j) A[(size_t)i * order + j] = expo * tHA[(size_t)i * M + j];
+    expm_col0(order, A, vect);
+    launch_scale_axpy(s, psi->ptr(), q[0]->ptr(), n, 0.0, vect[0].real(), vect[0].imag());
+    for (int i = 1; i < order; ++i) chk(qdx_wf_axpy(psi, vect[i].real(), vect[i].imag(), q[i]));
     p->sil_napply = napply;
 }
 

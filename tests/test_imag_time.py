@@ -447,3 +447,78 @@ def test_device_gspo_eigenstates_match_qdng_tge(ctx):
     efs, e, e_raw, steps = Q.eigen_imag(G, p.gpu_wf(ctx), 10, max_steps=1000, conv=1e-5, diag=True,
                                         H=p.gpu_ham(ctx))
     assert_tge_energies(e)
+
+
+# ---------------------------------------------------------------------------
+# Arnoldi (OArnoldi.cpp:64-190), real and imaginary time
+
+@pytest.mark.parametrize("imag,dt,order,tol", [(False, 0.5, 20, 1e-12), (False, 5.0, 20, 1e-8),
+                                               (True, 5.0, 15, 1e-9)])
+def test_oracle_arnoldi_step_is_matrix_exponential(imag, dt, order, tol):
+    p = tge_problem()
+    H, M = dense_h(p)
+    w, V = np.linalg.eigh(0.5 * (M + M.conj().T))
+    f = np.exp(-w * dt) if imag else np.exp(-1j * w * dt)
+    ref = V @ (f * (V.conj().T @ p.psi0))
+    U = O.Arnoldi(H, dt, order=order, imag=imag)
+    psi = p.psi0.copy()
+    n = U.step(psi, 0)
+    assert 2 <= n <= order
+    assert np.linalg.norm(psi - ref) / np.linalg.norm(ref) < tol
+    fbad = np.exp(+w * dt) if imag else np.conj(f)
+    bad = V @ (fbad * (V.conj().T @ p.psi0))
+    assert np.linalg.norm(bad / np.linalg.norm(bad) - ref / np.linalg.norm(ref)) > 1e3 * tol
+
+
+def test_oracle_arnoldi_eigenstates_match_qdng_tge():
+    # TGE-Arnoldi (tests/TGE-Arnoldi.qdi.in: order 15, dt 5) checks the same
+    # energies as TGE-Cheby (checktests.sh:124-127)
+    p = tge_problem()
+    H = p.oracle_ham()
+    g = p.oracle_grid()
+    U = O.Arnoldi(H, 5.0, order=15, imag=True)
+
+    def project_out(psi, efs):
+        buf = np.zeros_like(psi)
+        for ef in efs:
+            buf += ef * O.overlap(g, ef, psi)
+        psi -= buf
+
+    def normalize(psi):
+        psi /= np.sqrt(O.norm2(g, psi))
+
+    e_raw, e_diag, steps = imag_time_ef(
+        lambda psi, s: U.step(psi, s), project_out, normalize, lambda a, b: O.overlap(g, a, b),
+        lambda w: H.expec(w), lambda bra, ket: O.overlap(g, bra, H.apply(ket)), lambda: p.psi0.copy(),
+        10, 1000, 1e-5)
+    assert_tge_energies(e_diag)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,builder,dt,order,imag,steps", [
+    ("tge_1d_real", tge_problem, 0.5, 20, False, 4),
+    ("tge_1d_imag", tge_problem, 5.0, 15, True, 4),
+    ("morse_2state_2d_real", lambda: cases.morse_2state_2d(64), 1.0, 20, False, 3),
+    ("anharmonic_3d_imag", lambda: cases.anharmonic_3d(16), 1.0, 20, True, 2),
+])
+def test_device_arnoldi_matches_oracle(ctx, name, builder, dt, order, imag, steps):
+    import qdng_amd as Q
+    p = builder()
+    U = Q.Arnoldi(p.gpu_ham(ctx), dt, order=order, imag=imag)
+    Uo = O.Arnoldi(p.oracle_ham(), dt, order=order, imag=imag)
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for s in range(steps):
+        st = U.apply(w, s)
+        n = Uo.step(psi, s)
+        assert st.n_hpsi == n, (s, st.n_hpsi, n)
+    assert np.linalg.norm(w.download() - psi) / np.linalg.norm(psi) < 1e-10
+
+
+@pytest.mark.gpu
+def test_device_arnoldi_eigenstates_match_qdng_tge(ctx):
+    import qdng_amd as Q
+    p = tge_problem()
+    U = Q.Arnoldi(p.gpu_ham(ctx), 5.0, order=15, imag=True)
+    efs, e, e_raw, steps = Q.eigen_imag(U, p.gpu_wf(ctx), 10, max_steps=1000, conv=1e-5, diag=True)
+    assert_tge_energies(e)
