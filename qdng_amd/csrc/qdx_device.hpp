@@ -428,16 +428,17 @@ __device__ __forceinline__ double fixed_sum(const double *p, int n, double *bcas
 }
 
 // true when the previous order met |norm0 - accnorm| / norm0 < prec (real
-// time, OCheby.cpp:179-182) or, with c.rel, |accnorm - accnorm of the order
-// before| / the latter < prec (imaginary time, OCheby.cpp:283-285; QDng's
-// first comparison, against 0, never passes)
+// time, OCheby.cpp:179-182) or, with prec < 0, |accnorm - accnorm of the
+// order before| / the latter < -prec (imaginary time, OCheby.cpp:283-285;
+// QDng's first comparison, against 0, never passes)
 __device__ __forceinline__ bool prev_converged(const ConvCheck &c, double *bcast, double *acc_out) {
-    if (c.k_prev < (c.rel ? 3 : 2) || !(c.prec > 0)) return false;   // prec 0: fixed-length series
+    const bool rel = c.prec < 0;   // imaginary time (see ConvCheck)
+    if (c.k_prev < (rel ? 3 : 2) || c.prec == 0) return false;   // prec 0: fixed-length series
     const double acc = fixed_sum(c.part, c.nb, bcast);
     *acc_out = acc;
-    if (c.rel) {
-        const double last = fixed_sum(c.part2, c.nb, bcast);
-        return fabs(acc - last) / last < c.prec;
+    if (rel) {
+        const double last = fixed_sum(c.norm0, c.nb, bcast);
+        return fabs(acc - last) / last < -c.prec;
     }
     const double n0 = *c.norm0;
     return fabs(n0 - acc) / n0 < c.prec;

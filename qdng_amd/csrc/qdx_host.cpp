@@ -114,8 +114,9 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
 
     ConvCheck cc;
     cc.norm0 = &ctl->norm0;
-    cc.prec = p->cp.prec;
-    cc.rel = p->imag ? 1 : 0;
+    // imaginary time: the relative test, marked by prec < 0, with norm0
+    // pointing at the partials of the order before (set per order below)
+    cc.prec = p->imag ? -p->cp.prec : p->cp.prec;
     // T = sum_d T_d: the kinetic parts along dimensions >= 1 of phi_{k-1} go
     // to X; the row kernel adds the dimension-0 part
     auto cols = [&](const R *pm1, const ConvCheck &c) {
@@ -128,7 +129,6 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     const std::complex<double> ph = p->imag ? std::complex<double>(std::exp(-p->dt * p->cp.offset), 0.0)
                                             : std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
     ChebyOrder ord;
-    ord.imag = p->imag ? 1 : 0;
     ord.offset = p->cp.offset;
     ord.inv_scaling = 1.0 / p->cp.scaling;
     ord.prec = p->cp.prec;
@@ -141,7 +141,8 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     cols(A, cc);   // k_prev = 0: no test
     {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, A, B, X, psi->ptr(), ctl, slot[1 % nslot], init, nb_init);
+        Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, p->imag, A, B, X, psi->ptr(), ctl, slot[1 % nslot], init,
+                             nb_init);
     }
     int nb_prev = 0;
     for (int k = 2; k < p->k_launch; ++k) {
@@ -163,14 +164,14 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
         }
         cc.k_prev = k - 1;
         cc.part = slot[(k - 1) % nslot];
-        cc.part2 = slot[(k - 2) % nslot];
+        if (p->imag) cc.norm0 = slot[(k - 2) % nslot];
         cc.nb = nb_prev;
         R *pm1 = (k - 1) % 2 == 0 ? A : B;
         R *pm2 = k % 2 == 0 ? A : B;
         cols(pm1, cc);
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, pm1, pm2, X, psi->ptr(), ctl, slot[k % nslot],
-                                       init, nb_init);
+        nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, p->imag, pm1, pm2, X, psi->ptr(), ctl,
+                                       slot[k % nslot], init, nb_init);
     }
     p->last_nb = nb_prev;
     p->nb_init = nb_init;
@@ -200,11 +201,10 @@ std::vector<SplitOp> gspo_sequence(qdx_prop *p) {
 }
 
 template <class R>
-XSplitDev<R> xsplit(qdx_ham *op, double tau, long step, bool imag) {
+XSplitDev<R> xsplit(qdx_ham *op, double tau, long step) {
     XSplitDev<R> x;
     x.S = op->S;
     x.tau = tau;
-    x.imag = imag ? 1 : 0;
     x.coupled = 0;
     HamDev<R> h;
     if constexpr (std::is_same<R, float>::value) h = op->dev.h32;
@@ -236,10 +236,10 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
         const SplitOp &o = seq[i];
         if (!o.kinetic) {
             if (pending.n == kMaxXOps) {
-                Launch<R>::gspo_xonly(s, g, p->S, pending, psi);
+                Launch<R>::gspo_xonly(s, g, p->S, pending, psi, p->imag);
                 pending.n = 0;
             }
-            pending.op[pending.n++] = xsplit<R>(o.op, o.tau, step, p->imag);
+            pending.op[pending.n++] = xsplit<R>(o.op, o.tau, step);
             ++i;
             continue;
         }
@@ -248,7 +248,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
         post.n = 0;
         size_t j = i + 1;
         while (j < seq.size() && !seq[j].kinetic && post.n < kMaxXOps) {
-            post.op[post.n++] = xsplit<R>(seq[j].op, seq[j].tau, step, p->imag);
+            post.op[post.n++] = xsplit<R>(seq[j].op, seq[j].tau, step);
             ++j;
         }
         HamDev<R> kin;
@@ -261,7 +261,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
             R *X = p->X.as<R>();
             {
                 KTimer t(ctx, QDX_K_GSPO_ROW);
-                Launch<R>::gspo_row_pre(s, g, p->S, plans[0], pending, psi, X);
+                Launch<R>::gspo_row_pre(s, g, p->S, plans[0], pending, psi, X, p->imag);
             }
             auto col = [&](int dim, int mode) {
                 KTimer t(ctx, QDX_K_LINE_PASS);
@@ -276,13 +276,13 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
             }
             {
                 KTimer t(ctx, QDX_K_GSPO_ROW);
-                Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi);
+                Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi, p->imag);
             }
         }
         pending.n = 0;
         i = j;
     }
-    if (pending.n > 0) Launch<R>::gspo_xonly(s, g, p->S, pending, psi);
+    if (pending.n > 0) Launch<R>::gspo_xonly(s, g, p->S, pending, psi, p->imag);
 }
 
 // |norm0 - accnorm| / norm0 of the last executed order of the last step

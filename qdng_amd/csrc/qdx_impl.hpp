@@ -70,16 +70,18 @@ struct ChebyCtl {
 // the per-block partial sums that the previous row kernel stored.  Every
 // kernel sums the partials in the same fixed order (lane-strided in wave 0,
 // then a fixed shuffle tree), so all kernels reach the same decision.
+//
+// Imaginary time (OCheby.cpp:283-285) tests the relative change against the
+// order before, |acc(k_prev) - acc(k_prev-1)| / acc(k_prev-1) < |prec|; it
+// is marked by prec < 0, and norm0 then points at the partials of order
+// k_prev - 1 (same count nb).  (The struct keeps its size: kernel arguments
+// that grew by a few bytes slowed the hot kernels measurably.)
 struct ConvCheck {
     const double *part = nullptr;   // partials of order k_prev
     const double *norm0 = nullptr;  // sum |psi|^2 at step start (ChebyCtl::norm0)
     int nb = 0;
-    int k_prev = 0;                 // < 2 (< 3 with rel): nothing to test
+    int k_prev = 0;                 // < 2 (< 3 relative): nothing to test
     double prec = 0;
-    // imaginary time (OCheby.cpp:283-285): the relative change against the
-    // order before, |acc(k_prev) - acc(k_prev-1)| / acc(k_prev-1) < prec
-    int rel = 0;
-    const double *part2 = nullptr;  // partials of order k_prev - 1 (same count nb)
 };
 constexpr int kMaxPartials = 8192;
 
@@ -96,10 +98,9 @@ struct ChebyOrder {
     int nterm = 1;
     double cm1r = 0, cm1i = 0;   // coeff[k-1] * exp(-i offset dt)
     double cm2r = 0, cm2i = 0;   // coeff[k-2] * exp(-i offset dt)
-    // imaginary time, backward (exponent -dt, OCheby.cpp:200-297): phi_1 =
-    // -b, phi_k = -2 b - phi_{k-2} with b = (H - offset) phi_{k-1} / R, and
-    // the coefficients above carry exp(-dt offset) instead
-    int imag = 0;
+    // (imaginary time, backward, exponent -dt, OCheby.cpp:200-297, is a
+    // separate kernel instance: phi_1 = -b, phi_k = -2 b - phi_{k-2} with
+    // b = (H - offset) phi_{k-1} / R; the coefficients carry exp(-dt offset))
 };
 
 // One x-space split operator of GSPO: exp(c * W(x)) with c = -i tau, W the
@@ -109,7 +110,6 @@ struct XSplitDev {
     int S = 1;
     int coupled = 0;     // off-diagonal elements present
     double tau = 0;      // dt * sf
-    int imag = 0;        // imaginary time: exp(-tau W(x)) instead of exp(-i tau W(x))
     const R *V[kMaxStates][kMaxStates] = {};   // potential arrays
     const R *mu[kMaxStates][kMaxStates] = {};  // dipole arrays
     double Edip[kMaxStates][kMaxStates] = {};  // -laser[step]*scale
@@ -144,7 +144,7 @@ struct Launch {
     // init_part/nb_init: partials of the init kernel, summed into
     // ctl->norm0 by the prologue (ord.k == 1)
     static int cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                         const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X,
+                         const ChebyOrder &ord, const ConvCheck &cc, bool imag, const R *phim1, R *phim2, R *X,
                          double *psi, ChebyCtl *ctl, double *partials, const double *init_part,
                          int nb_init);
     // a whole Chebyshev step of a single-state 1D line in one block (see
@@ -171,15 +171,16 @@ struct Launch {
     static void ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h,
                           const LinePlan *plans, const double *src, double *dst, R *X);
     // GSPO: one step; psi complex R (state-major)
+    // imag (here and below): imaginary time, exp(-tau D) (FP64 only)
     static void gspo_row_pre(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
-                             const XGroupDev<R> &pre, const R *psi_in, R *X);
+                             const XGroupDev<R> &pre, const R *psi_in, R *X, int imag = 0);
     static void gspo_row_post(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
-                              const XGroupDev<R> &post, const R *X, R *psi_out);
+                              const XGroupDev<R> &post, const R *X, R *psi_out, int imag = 0);
     static void gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
                         const HamDev<R> &kin, double tau_kin, const XGroupDev<R> &pre,
                         const XGroupDev<R> &post, R *psi, int imag = 0);
     static void gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp,
-                           R *psi);
+                           R *psi, int imag = 0);
 };
 
 // precision-independent helpers

@@ -144,8 +144,9 @@ __global__ void __launch_bounds__(512) k_cheby_init(GridDev g, LinePlan p0, int 
 // ---------------------------------------------------------------------------
 // Chebyshev row kernel (see file header).  PRO: prologue (order 1).
 // NT: the largest block the instance is launched with (256 leaves twice the
-// registers per thread of 512).  P: points per thread.
-template <class R, int S, bool ONE_D, bool PRO, int NT, int P = 8>
+// registers per thread of 512).  P: points per thread.  IMAG: imaginary
+// time (the recurrence signs of OCheby::imag_time_step).
+template <class R, int S, bool ONE_D, bool PRO, int NT, int P = 8, bool IMAG = false>
 __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord,
                                                     ConvCheck cc, int rb, const cx<R> *__restrict__ phim1,
                                                     cx<R> *__restrict__ phim2, cx<R> *__restrict__ X,
@@ -264,12 +265,12 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
                                             (hp.i - ord.offset * p.i) * ord.inv_scaling);
             cx<double> pk;
             if (PRO) {
-                pk = ord.imag ? mk<double>(-b.r, -b.i)                    // * (-1)
+                pk = IMAG ? mk<double>(-b.r, -b.i)                        // * (-1)
                               : mk<double>(b.i, -b.r);                    // * (-i)
                 psi[base + gi] = cmul(c0, H.ps[u]) + cmul(ck, pk);
             } else {
                 const cx<double> mm = to_d(H.m2[u]);
-                pk = ord.imag ? mk<double>(-2.0 * b.r - mm.r, -2.0 * b.i - mm.i)   // * (-2) - phi_{k-2}
+                pk = IMAG ? mk<double>(-2.0 * b.r - mm.r, -2.0 * b.i - mm.i)       // * (-2) - phi_{k-2}
                               : mk<double>(2.0 * b.i + mm.r, -2.0 * b.r + mm.i);   // * (-2i) + phi_{k-2}
                 if (upd) {
                     cx<double> sv = H.ps[u];
@@ -444,10 +445,10 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
 //      3: forward * exp(-i tau T)/N inverse,
 //      4: X = IFFT(T_dim/N_dim FFT(src)), the dimension's part of the
 //         separable kinetic operator, 5: X += the same.
-template <class R, int MODE, int P = 8>
+template <class R, int MODE, int P = 8, bool IMAG = false>
 __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
                                               cx<R> *__restrict__ X, const cx<R> *__restrict__ src,
-                                              ChebyCtl *ctl, ConvCheck cc, int kimag) {
+                                              ChebyCtl *ctl, ConvCheck cc) {
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
@@ -527,7 +528,7 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
                 } else if (MODE == 2) {
                     const double t = tt * h.inv_n;
                     v[q] = mk<R>((R)((double)v[q].r * t), (R)((double)v[q].i * t));
-                } else if (kimag) {
+                } else if (IMAG) {
                     // exp(-tau T) / N, imaginary time
                     const double f = exp(-tau * tt) * h.inv_n;
                     v[q] = mk<R>((R)((double)v[q].r * f), (R)((double)v[q].i * f));
@@ -621,7 +622,7 @@ __global__ void __launch_bounds__(512) k_apply_row(GridDev g, HamDev<R> h, LineP
 // GSPO x-space split operators on one grid point (all states):
 //   psi <- exp(-i tau W(x)) psi, W real symmetric (ODSpace.h:65-69,
 //   OGridDipole.cpp:83-87, ODMultistate.cpp:109-159 / TransformMSD.cpp:31-101)
-template <class R, int S>
+template <class R, int S, bool IMAG = false>
 __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx<double> (&v)[S]) {
     double Wm[S][S];
 #pragma unroll
@@ -636,7 +637,7 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
     if (!op.coupled || S == 1) {
 #pragma unroll
         for (int i = 0; i < S; ++i) {
-            if (op.imag) {   // exp(-tau w), imaginary time
+            if (IMAG) {   // exp(-tau w), imaginary time
                 const double f = exp(-op.tau * Wm[i][i]);
                 v[i] = mk<double>(v[i].r * f, v[i].i * f);
                 continue;
@@ -648,7 +649,7 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
         return;
     }
     if constexpr (S == 2) {
-        if (op.imag) {
+        if (IMAG) {
             // exp(-tau W) = e^{-tau m} [cosh(tau O) - sinh(tau O)/O (W - m)]
             const double m = 0.5 * (Wm[0][0] + Wm[1][1]);
             const double d = 0.5 * (Wm[0][0] - Wm[1][1]);
@@ -683,9 +684,9 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
     }
 }
 
-template <class R, int S>
+template <class R, int S, bool IMAG = false>
 __device__ __forceinline__ void xgroup_point(const XGroupDev<R> &grp, long gi, cx<double> (&v)[S]) {
-    for (int o = 0; o < grp.n; ++o) xsplit_point<R, S>(grp.op[o], gi, v);
+    for (int o = 0; o < grp.n; ++o) xsplit_point<R, S, IMAG>(grp.op[o], gi, v);
 }
 
 // GSPO kinetic split step over rows.  ONE_D: psi -> pre -> FFT -> exp(-i tau
@@ -693,8 +694,8 @@ __device__ __forceinline__ void xgroup_point(const XGroupDev<R> &grp, long gi, c
 // FFT_0 -> X; PHASE 1: X -> IFFT_0 -> post -> psi.  The x-space operators act
 // on all states of a point at once, so the S lines of a row are exchanged
 // through LDS around them.
-template <class R, int S, bool ONE_D, int PHASE>
-__global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, int kimag, LinePlan p0, int rb,
+template <class R, int S, bool ONE_D, int PHASE, bool IMAG = false>
+__global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, LinePlan p0, int rb,
                                                    XGroupDev<R> pre, XGroupDev<R> post, cx<R> *__restrict__ psi,
                                                    cx<R> *__restrict__ X) {
     extern __shared__ __align__(16) char smem_raw[];
@@ -714,7 +715,7 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
                 for (int q = 0; q < 8; ++q) {
                     cx<double> w[S];
                     w[0] = to_d(v[q]);
-                    xgroup_point<R, S>(grp, B.gidx(q), w);
+                    xgroup_point<R, S, IMAG>(grp, B.gidx(q), w);
                     v[q] = from_d<R>(w[0]);
                 }
             }
@@ -735,7 +736,7 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
                 cx<double> w[S];
 #pragma unroll
                 for (int s2 = 0; s2 < S; ++s2) w[s2] = to_d(xa[((long)s2 * B.rb + B.rr) * B.L + x]);
-                xgroup_point<R, S>(grp, B.row * B.L + x, w);
+                xgroup_point<R, S, IMAG>(grp, B.row * B.L + x, w);
 #pragma unroll
                 for (int s2 = 0; s2 < S; ++s2) xa[((long)s2 * B.rb + B.rr) * B.L + x] = from_d<R>(w[s2]);
             }
@@ -765,7 +766,7 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const double tt = has_t ? 0.0 + kin.tk[st][0][B.lt + q * B.TL] : 0.0;
-                if (kimag) {   // exp(-tau T)/N, imaginary time
+                if (IMAG) {   // exp(-tau T)/N, imaginary time
                     const double f = exp(-tau * tt) * kin.inv_n;
                     v[q] = mk<R>((R)((double)v[q].r * f), (R)((double)v[q].i * f));
                     continue;
@@ -789,14 +790,14 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
     }
 }
 
-template <class R, int S>
+template <class R, int S, bool IMAG = false>
 __global__ void k_gspo_xonly(GridDev g, XGroupDev<R> grp, cx<R> *__restrict__ psi) {
     const long gi = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= g.size) return;
     cx<double> v[S];
 #pragma unroll
     for (int st = 0; st < S; ++st) v[st] = to_d(psi[(long)st * g.size + gi]);
-    xgroup_point<R, S>(grp, gi, v);
+    xgroup_point<R, S, IMAG>(grp, gi, v);
 #pragma unroll
     for (int st = 0; st < S; ++st) psi[(long)st * g.size + gi] = from_d<R>(v[st]);
 }
@@ -979,7 +980,7 @@ int Launch<R>::cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan
 
 template <class R, int S, bool ONE_D, bool PRO, int P>
 static int cheby_row_SKP(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &plan,
-                         const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                         const ChebyOrder &ord, const ConvCheck &cc, bool imag, const R *phim1, R *phim2, R *X, double *psi,
                          ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
     LinePlan p0 = plan;
     RowLaunch r = row_launch<R>(g, S, p0);
@@ -988,7 +989,15 @@ static int cheby_row_SKP(hipStream_t st, const GridDev &g, const HamDev<R> &h, c
     auto cphim2 = reinterpret_cast<cx<R> *>(phim2);
     auto cX = reinterpret_cast<cx<R> *>(X);
     auto cpsi = reinterpret_cast<cx<double> *>(psi);
-    if (r.threads <= 256) {
+    if (imag) {   // imaginary time: FP64 only, the 512-thread instance
+        if constexpr (std::is_same<R, double>::value) {
+            set_lds(k_cheby_row<R, S, ONE_D, PRO, 512, P, true>, r.lds);
+            QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 512, P, true>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h,
+                       p0, ord, cc, r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
+        } else {
+            throw std::runtime_error("imaginary time: FP64 only");
+        }
+    } else if (r.threads <= 256) {
         set_lds(k_cheby_row<R, S, ONE_D, PRO, 256, P>, r.lds);
         QDX_LAUNCH((k_cheby_row<R, S, ONE_D, PRO, 256, P>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, p0, ord,
                    cc, r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part, nb_init);
@@ -1005,37 +1014,37 @@ static int cheby_row_SKP(hipStream_t st, const GridDev &g, const HamDev<R> &h, c
 // the whole-step kernel)
 template <class R, int S, bool ONE_D, bool PRO>
 static int cheby_row_SK(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &plan,
-                        const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                        const ChebyOrder &ord, const ConvCheck &cc, bool imag, const R *phim1, R *phim2, R *X, double *psi,
                         ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
     if (plan.ppt == 4) {
         if constexpr (!ONE_D)
-            return cheby_row_SKP<R, S, ONE_D, PRO, 4>(st, g, h, plan, ord, cc, phim1, phim2, X, psi, ctl, partials,
+            return cheby_row_SKP<R, S, ONE_D, PRO, 4>(st, g, h, plan, ord, cc, imag, phim1, phim2, X, psi, ctl, partials,
                                                       init_part, nb_init);
         throw std::runtime_error("4 points per thread: grids with nd >= 2 only");
     }
-    return cheby_row_SKP<R, S, ONE_D, PRO, 8>(st, g, h, plan, ord, cc, phim1, phim2, X, psi, ctl, partials, init_part,
+    return cheby_row_SKP<R, S, ONE_D, PRO, 8>(st, g, h, plan, ord, cc, imag, phim1, phim2, X, psi, ctl, partials, init_part,
                                               nb_init);
 }
 
 template <class R, int S>
 static int cheby_row_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                       const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                       const ChebyOrder &ord, const ConvCheck &cc, bool imag, const R *phim1, R *phim2, R *X, double *psi,
                        ChebyCtl *ctl, double *partials, const double *ip, int ni) {
     const bool one = g.nd == 1;
     const bool pro = ord.k == 1;
     if (one && pro)
-        return cheby_row_SK<R, S, true, true>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
-    if (one) return cheby_row_SK<R, S, true, false>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
-    if (pro) return cheby_row_SK<R, S, false, true>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
-    return cheby_row_SK<R, S, false, false>(st, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials, ip, ni);
+        return cheby_row_SK<R, S, true, true>(st, g, h, p0, ord, cc, imag, phim1, phim2, X, psi, ctl, partials, ip, ni);
+    if (one) return cheby_row_SK<R, S, true, false>(st, g, h, p0, ord, cc, imag, phim1, phim2, X, psi, ctl, partials, ip, ni);
+    if (pro) return cheby_row_SK<R, S, false, true>(st, g, h, p0, ord, cc, imag, phim1, phim2, X, psi, ctl, partials, ip, ni);
+    return cheby_row_SK<R, S, false, false>(st, g, h, p0, ord, cc, imag, phim1, phim2, X, psi, ctl, partials, ip, ni);
 }
 
 template <class R>
 int Launch<R>::cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                         const ChebyOrder &ord, const ConvCheck &cc, const R *phim1, R *phim2, R *X, double *psi,
+                         const ChebyOrder &ord, const ConvCheck &cc, bool imag, const R *phim1, R *phim2, R *X, double *psi,
                          ChebyCtl *ctl, double *partials, const double *init_part, int nb_init) {
     int nb = 0;
-    QDX_DISPATCH_S(h.S, (nb = cheby_row_S<R, SS>(s, g, h, p0, ord, cc, phim1, phim2, X, psi, ctl, partials,
+    QDX_DISPATCH_S(h.S, (nb = cheby_row_S<R, SS>(s, g, h, p0, ord, cc, imag, phim1, phim2, X, psi, ctl, partials,
                                                  init_part, nb_init)));
     return nb;
 }
@@ -1080,19 +1089,29 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     switch (mode) {
     case 0:
         set_lds(k_col<R, 0>, lds);
-        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
+        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     case 1:
         set_lds(k_col<R, 1>, lds);
-        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
+        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     case 2:
         set_lds(k_col<R, 2>, lds);
-        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
+        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     default:
+        if (imag) {   // imaginary time: FP64 only
+            if constexpr (std::is_same<R, double>::value) {
+                set_lds(k_col<R, 3, 8, true>, lds);
+                QDX_LAUNCH((k_col<R, 3, 8, true>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none,
+                           ctl, cc);
+            } else {
+                throw std::runtime_error("imaginary time: FP64 only");
+            }
+            break;
+        }
         set_lds(k_col<R, 3>, lds);
-        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, imag);
+        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
         break;
     }
     QDX_HIP(hipGetLastError());
@@ -1113,10 +1132,10 @@ static void kin_pass_P(hipStream_t s, const GridDev &g, int S, int dim, bool acc
     pl.tw_lds = 1;
     if (accumulate) {
         set_lds(k_col<R, 5, P>, lds);
-        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc, 0);
+        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
     } else {
         set_lds(k_col<R, 4, P>, lds);
-        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc, 0);
+        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
     }
     QDX_HIP(hipGetLastError());
 }
@@ -1165,34 +1184,45 @@ void Launch<R>::ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h, c
     QDX_DISPATCH_S(h.S, (ham_apply_S<R, SS>(s, g, h, plans, src, dst, X)));
 }
 
-template <class R, bool ONE_D, int PHASE>
-static void gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
-                            const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X,
-                            int kimag = 0) {
+template <class R, bool ONE_D, int PHASE, bool IMAG>
+static void gspo_row_launch_I(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                              const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
+                              R *X) {
     QDX_DISPATCH_S(S, {
         LinePlan p0 = plan;
         RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
-        set_lds(k_gspo_row<R, SS, ONE_D, PHASE>, r.lds);
-        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, kimag, p0,
+        set_lds(k_gspo_row<R, SS, ONE_D, PHASE, IMAG>, r.lds);
+        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE, IMAG>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, p0,
                            r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
     });
     QDX_HIP(hipGetLastError());
 }
 
+// imag: imaginary time (FP64 only)
+template <class R, bool ONE_D, int PHASE>
+static void gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                            const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X,
+                            int imag = 0) {
+    if (!imag) return gspo_row_launch_I<R, ONE_D, PHASE, false>(s, g, S, kin, tau, plan, pre, post, psi, X);
+    if constexpr (std::is_same<R, double>::value)
+        return gspo_row_launch_I<R, ONE_D, PHASE, true>(s, g, S, kin, tau, plan, pre, post, psi, X);
+    throw std::runtime_error("imaginary time: FP64 only");
+}
+
 template <class R>
 void Launch<R>::gspo_row_pre(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const XGroupDev<R> &pre,
-                             const R *psi, R *X) {
+                             const R *psi, R *X, int imag) {
     HamDev<R> none;
     XGroupDev<R> empty;
-    gspo_row_launch<R, false, 0>(s, g, S, none, 0.0, p0, pre, empty, const_cast<R *>(psi), X);
+    gspo_row_launch<R, false, 0>(s, g, S, none, 0.0, p0, pre, empty, const_cast<R *>(psi), X, imag);
 }
 
 template <class R>
 void Launch<R>::gspo_row_post(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const XGroupDev<R> &post,
-                              const R *X, R *psi) {
+                              const R *X, R *psi, int imag) {
     HamDev<R> none;
     XGroupDev<R> empty;
-    gspo_row_launch<R, false, 1>(s, g, S, none, 0.0, p0, empty, post, psi, const_cast<R *>(X));
+    gspo_row_launch<R, false, 1>(s, g, S, none, 0.0, p0, empty, post, psi, const_cast<R *>(X), imag);
 }
 
 template <class R>
@@ -1202,9 +1232,20 @@ void Launch<R>::gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &
 }
 
 template <class R>
-void Launch<R>::gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp, R *psi) {
+void Launch<R>::gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp, R *psi, int imag) {
     const int threads = 256;
     const int nb = (int)((g.size + threads - 1) / threads);
+    if (imag) {
+        if constexpr (std::is_same<R, double>::value) {
+            QDX_DISPATCH_S(S, {
+                QDX_LAUNCH((k_gspo_xonly<R, SS, true>), dim3(nb), dim3(threads), 0, s, g, grp,
+                           reinterpret_cast<cx<R> *>(psi));
+            });
+            QDX_HIP(hipGetLastError());
+            return;
+        }
+        throw std::runtime_error("imaginary time: FP64 only");
+    }
     QDX_DISPATCH_S(S, {
         QDX_LAUNCH((k_gspo_xonly<R, SS>), dim3(nb), dim3(threads), 0, s, g, grp,
                            reinterpret_cast<cx<R> *>(psi));
