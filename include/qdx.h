@@ -99,6 +99,14 @@ int qdx_wf_norm(qdx_wf *wf, double *norm, double *per_state_pop);
 int qdx_wf_overlap(qdx_wf *a, qdx_wf *b, double out[2]);
 /* psi *= 1/Norm() (WaveFunction.h:103-106) */
 int qdx_wf_normalize(qdx_wf *wf);
+/* Post-step expectation filters of FiltersList (src/FiltersList.cpp):
+ * per state s, <x_d> = sum_x x_d |psi_s(x)|^2 prod(dx) (OGridPosition,
+ * x = xmin + n dx) and <p_d> = <psi_s| -i d/dx_d |psi_s> (OGridNabla as
+ * momentum with Kspace::Init1Dddx, OGridNabla.cpp:76-95), the latter through
+ * the transform along d on the device (sum_k k |psi~(k)|^2 / N_d).
+ * per_state: S values. */
+int qdx_wf_expec_position(qdx_wf *wf, int32_t dim, double *per_state);
+int qdx_wf_expec_momentum(qdx_wf *wf, int32_t dim, double *per_state);
 /* y += (re + i im) x (cVec MultElementsAdd; same grid and states) */
 int qdx_wf_axpy(qdx_wf *y, double re, double im, const qdx_wf *x);
 int qdx_wf_destroy(qdx_wf *wf);

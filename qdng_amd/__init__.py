@@ -61,7 +61,7 @@ EXPORTS = [
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
     "qdx_ctx_set_line_points", "qdx_ctx_get_line_points", "qdx_cheby_create_imag", "qdx_bessel_i0",
-    "qdx_wf_axpy", "qdx_eigen_imag", "qdx_sil_create", "qdx_gspo_create_imag", "qdx_arnoldi_create",
+    "qdx_wf_axpy", "qdx_wf_expec_position", "qdx_wf_expec_momentum", "qdx_eigen_imag", "qdx_sil_create", "qdx_gspo_create_imag", "qdx_arnoldi_create",
     # include/qdx_io.h
     "qdx_meta_create", "qdx_meta_read", "qdx_meta_write", "qdx_meta_size", "qdx_meta_key", "qdx_meta_get",
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
@@ -164,6 +164,8 @@ def lib():
         "qdx_cheby_create_imag": [vp, C.c_double, i32, C.c_double, C.c_double, C.POINTER(vp)],
         "qdx_bessel_i0": [i32, C.c_double, dp, C.POINTER(i32)],
         "qdx_wf_axpy": [vp, C.c_double, C.c_double, vp],
+        "qdx_wf_expec_position": [vp, i32, dp],
+        "qdx_wf_expec_momentum": [vp, i32, dp],
         "qdx_sil_create": [vp, C.c_double, i32, C.c_double, i32, C.POINTER(vp)],
         "qdx_gspo_create_imag": [C.POINTER(vp), i32, C.c_double, C.POINTER(vp)],
         "qdx_arnoldi_create": [vp, C.c_double, i32, C.c_double, i32, C.POINTER(vp)],
@@ -471,6 +473,18 @@ class WaveFunction:
 
     def normalize(self) -> None:
         _chk(lib().qdx_wf_normalize(self.h))
+
+    def expec_position(self, dim: int = 0) -> np.ndarray:
+        """<x_dim> of each state (qdx_wf_expec_position, GridPosition filter)"""
+        out = np.zeros(self.S)
+        _chk(lib().qdx_wf_expec_position(self.h, int(dim), _dp(out)))
+        return out
+
+    def expec_momentum(self, dim: int = 0) -> np.ndarray:
+        """<p_dim> of each state (qdx_wf_expec_momentum, GridMomentum filter)"""
+        out = np.zeros(self.S)
+        _chk(lib().qdx_wf_expec_momentum(self.h, int(dim), _dp(out)))
+        return out
 
     def axpy(self, a: complex, x: "WaveFunction") -> "WaveFunction":
         """self += a x (qdx_wf_axpy)"""

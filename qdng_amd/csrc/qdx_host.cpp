@@ -546,6 +546,47 @@ int qdx_wf_normalize(qdx_wf *wf) {
     });
 }
 
+// per-state results of launch_wnorm (in ctx->scal) to the host
+static void wnorm_fetch(qdx_wf *wf, double factor, double *per_state) {
+    qdx_ctx *c = wf->ctx;
+    HIPCHK(hipMemcpyAsync(c->pinned, c->scal.p, 2 * wf->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < wf->S; ++s) per_state[s] = c->pinned[2 * s] * factor;
+}
+
+int qdx_wf_expec_position(qdx_wf *wf, int32_t dim, double *per_state) {
+    return guarded([&] {
+        check_wf(wf);
+        if (!per_state) fail(QDX_ERR_PARAM, "NULL array");
+        if (dim < 0 || dim >= wf->grid.nd) fail(QDX_ERR_PARAM, "dimension out of range");
+        qdx_ctx *c = wf->ctx;
+        launch_wnorm(c->stream, wf->ptr(), wf->n(), wf->S, wf->grid.dev(), dim, wf->grid.xmin[dim], wf->grid.dx(dim),
+                     nullptr, c->partials.as<double>(), c->scal.as<double>());
+        wnorm_fetch(wf, wf->grid.weight(), per_state);
+    });
+}
+
+int qdx_wf_expec_momentum(qdx_wf *wf, int32_t dim, double *per_state) {
+    return guarded([&] {
+        check_wf(wf);
+        if (!per_state) fail(QDX_ERR_PARAM, "NULL array");
+        if (dim < 0 || dim >= wf->grid.nd) fail(QDX_ERR_PARAM, "dimension out of range");
+        qdx_ctx *c = wf->ctx;
+        const size_t bytes = (size_t)wf->S * wf->n() * 2 * sizeof(double);
+        if (c->wfx.bytes < bytes) c->wfx.alloc(bytes);
+        const std::vector<double> k = kspace_d1(wf->grid, dim);
+        if (c->ktab.bytes < k.size() * sizeof(double)) c->ktab.alloc(k.size() * sizeof(double));
+        HIPCHK(hipMemcpyAsync(c->ktab.p, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const GridDev g = wf->grid.dev();
+        Launch<double>::line_fft(c->stream, g, wf->S, dim, c->plan(wf->grid.N[dim], false), wf->ptr(),
+                                 c->wfx.as<double>());
+        launch_wnorm(c->stream, c->wfx.as<double>(), wf->n(), wf->S, g, dim, 0.0, 0.0, c->ktab.as<double>(),
+                     c->partials.as<double>(), c->scal.as<double>());
+        // (the host vector k stays alive until the copy has run: synchronised here)
+        wnorm_fetch(wf, wf->grid.weight() / wf->grid.N[dim], per_state);
+    });
+}
+
 int qdx_wf_axpy(qdx_wf *y, double re, double im, const qdx_wf *x) {
     return guarded([&] {
         check_wf(y);

@@ -250,6 +250,7 @@ struct qdx_ctx {
     std::map<std::pair<int, bool>, std::vector<int>> radix;
     DevBuf partials;   // reduction scratch
     DevBuf scal;       // small device result area
+    DevBuf wfx, ktab;  // scratch of the momentum expectation (a transformed copy, k table)
     double *pinned = nullptr;
 
     // per-launch HIP-event timing (qdx_ctx_profile): a fixed pool of events;
@@ -451,6 +452,28 @@ inline std::vector<double> kspace_d2(const Grid &grid, int d, double mass) {
         for (int q = 1; q < (Nx + 1) / 2; ++q) {
             k1[q] = (double(q) * double(q)) * dp;
             k1[Nx - q] = (double(q) * double(q)) * dp;
+        }
+    }
+    return k1;
+}
+
+// k-space factors of the momentum -i d/dx along dimension d in FFT order
+// (Kspace::Init1Dddx, Kspace.cpp:64-88; the Nyquist point of an even N is
+// taken as -N/2 dk)
+inline std::vector<double> kspace_d1(const Grid &grid, int d) {
+    const int Nx = grid.N[d];
+    const double dp = grid.dk(d);
+    std::vector<double> k1(Nx);
+    if (Nx % 2 == 0) {
+        for (int q = 0; q < Nx / 2; ++q) {
+            k1[q] = double(q) * dp;
+            k1[Nx - q - 1] = -1 * double(q + 1) * dp;
+        }
+    } else {
+        k1[0] = 0;
+        for (int q = 1; q < (Nx + 1) / 2; ++q) {
+            k1[q] = double(q) * dp;
+            k1[Nx - q] = -1 * double(q) * dp;
         }
     }
     return k1;

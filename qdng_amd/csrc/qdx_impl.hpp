@@ -181,10 +181,18 @@ struct Launch {
                         const XGroupDev<R> &post, R *psi, int imag = 0);
     static void gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp,
                            R *psi, int imag = 0);
+    // X = the unnormalised forward transform of every state's lines along
+    // `dim` (src complex128)
+    static void line_fft(hipStream_t s, const GridDev &g, int S, int dim, const LinePlan &p, const double *src,
+                         R *X);
 };
 
 // precision-independent helpers
 void launch_convert_c128_to_c64(hipStream_t s, const double *src, float *dst, long n);
+// out[2 s] = sum_i w(i) |a_s(i)|^2 (out[2 s + 1] = 0), w the coordinate along
+// dim: ktab[i_dim] if given, else base + step i_dim
+void launch_wnorm(hipStream_t s, const double *a, long n, int S, const GridDev &g, int dim, double base, double step,
+                  const double *ktab, double *partials, double *out);
 // y = sa * y + (br + i bi) * x over n complex128 values (x may be null)
 void launch_scale_axpy(hipStream_t s, double *y, const double *x, long n, double sa, double br, double bi);
 void launch_convert_c64_to_c128(hipStream_t s, const float *src, double *dst, long n);

@@ -824,6 +824,27 @@ __global__ void k_overlap_stage1(const cx<R> *__restrict__ a, const cx<R> *__res
     }
 }
 
+// sum_i w(i) |a_s(i)|^2 per state s (blockIdx.y), w the coordinate of i
+// along `dim`: ktab[i_dim] when ktab is given (k-space), else base + step
+// i_dim (x-space).  Partials as (sum, 0) pairs for k_overlap_stage2.
+__global__ void k_wnorm_stage1(const cx<double> *__restrict__ a, long n, int stride, int Nd, double base,
+                               double step, const double *__restrict__ ktab, double *partials) {
+    __shared__ double red[16];
+    const int st = blockIdx.y;
+    double sum = 0;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const int id = (int)((j / stride) % Nd);
+        const double w = ktab ? ktab[id] : base + step * (double)id;
+        const cx<double> x = a[(long)st * n + j];
+        sum += w * (x.r * x.r + x.i * x.i);
+    }
+    const double ps = block_sum(sum, red);
+    if (threadIdx.x == 0) {
+        partials[2 * (st * gridDim.x + blockIdx.x)] = ps;
+        partials[2 * (st * gridDim.x + blockIdx.x) + 1] = 0.0;
+    }
+}
+
 __global__ void k_overlap_stage2(const double *partials, int nbx, int S, double *out) {
     __shared__ double red[16];
     for (int st = 0; st < S; ++st) {
@@ -1253,6 +1274,25 @@ void Launch<R>::gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupD
     QDX_HIP(hipGetLastError());
 }
 
+template <class R>
+void Launch<R>::line_fft(hipStream_t s, const GridDev &g, int S, int dim, const LinePlan &plan, const double *src,
+                         R *X) {
+    if (dim == 0) {
+        QDX_DISPATCH_S(S, {
+            LinePlan p0 = plan;
+            RowLaunch r = row_launch<R>(g, SS, p0);
+            set_lds(k_row_fwd<R, SS>, r.lds);
+            QDX_LAUNCH((k_row_fwd<R, SS>), dim3(r.nb), dim3(r.threads), r.lds, s, g, p0, r.rb,
+                       reinterpret_cast<const cx<double> *>(src), reinterpret_cast<cx<R> *>(X));
+        });
+        QDX_HIP(hipGetLastError());
+        return;
+    }
+    if constexpr (!std::is_same<R, double>::value) throw std::runtime_error("line_fft: FP64 only");
+    QDX_HIP(hipMemcpyAsync(X, src, (size_t)S * g.size * 2 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    col_pass(s, g, S, dim, 0, plan, HamDev<R>(), 0.0, X, nullptr, ConvCheck());
+}
+
 template struct Launch<double>;
 template struct Launch<float>;
 
@@ -1283,6 +1323,21 @@ static void overlap_impl(hipStream_t s, const R *a, const R *b, long n, int S, d
     if (nbx < 1) nbx = 1;
     QDX_LAUNCH((k_overlap_stage1<R>), dim3(nbx, S), dim3(threads), 0, s,
                        reinterpret_cast<const cx<R> *>(a), reinterpret_cast<const cx<R> *>(b), n, partials);
+    QDX_HIP(hipGetLastError());
+    QDX_LAUNCH(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
+    QDX_HIP(hipGetLastError());
+}
+
+void launch_wnorm(hipStream_t s, const double *a, long n, int S, const GridDev &g, int dim, double base, double step,
+                  const double *ktab, double *partials, double *out) {
+    if (dim < 0 || dim >= g.nd) throw std::runtime_error("dimension out of range");
+    int stride = 1;
+    for (int d = 0; d < dim; ++d) stride *= g.N[d];
+    const int threads = 256;
+    int nbx = (int)std::min<long>((n + threads - 1) / threads, 1024);
+    if (nbx < 1) nbx = 1;
+    QDX_LAUNCH(k_wnorm_stage1, dim3(nbx, S), dim3(threads), 0, s, reinterpret_cast<const cx<double> *>(a), n, stride,
+               g.N[dim], base, step, ktab, partials);
     QDX_HIP(hipGetLastError());
     QDX_LAUNCH(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
     QDX_HIP(hipGetLastError());

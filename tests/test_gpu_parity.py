@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import cases
+from test_oracle import _digits_ok
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -345,6 +346,56 @@ def test_qdng_regression_on_device(ctx, name):
     assert abs(energy - gold["energy"]) < 6e-9, (energy, gold["energy"])
     assert abs(autoc.real - gold["autocorr"][0]) < 6e-9
     assert abs(autoc.imag - gold["autocorr"][1]) < 6e-9
+    # expec_post.dat: the post-step filters on the device (FiltersList),
+    # written after the last of the `steps` steps (time steps * dt)
+    post = cases.kat_data()[1]["golden"][name]["expec_post"]
+    assert post["time"] == pytest.approx(steps * dt)
+    U.apply(w, steps - 1)
+    vals = device_post_expectations(ctx, name, p, w)
+    assert len(vals) == len(post["values"])
+    for v, want, pf in zip(vals, post["values"], post["prefix"]):
+        assert _digits_ok(v, want, pf), (name, v, want)
+
+
+def device_post_expectations(ctx, name, p, w):
+    """The expec_post.dat values of the KATs (the oracle's
+    tests/test_oracle.py::_post_expectations) from device filters: momentum,
+    position, projections on the eigenfunctions, <T>, <V>."""
+    import qdng_amd as Q
+    a, _ = cases.kat_data()
+    n = p.size
+    N, lo, hi = p.N, p.xmin, p.xmax
+    mom, pos = w.expec_momentum(0), w.expec_position(0)
+    Tk = Q.Hamiltonian(ctx, N, lo, hi).add_kinetic(0, [1763.2])
+    if p.S == 1:
+        vals = [mom[0], pos[0]]
+        efs = [Q.WaveFunction(ctx, N, lo, hi).upload(a["GridPot0_ef_%d" % i]) for i in range(4)]
+        proj = [abs(w.overlap(ef)) ** 2 for ef in efs]
+        vals += [sum(proj)] if name == "TGP-Expec" else proj[:3]
+        V = Q.Hamiltonian(ctx, N, lo, hi).add_potential(0, 0, a["GridPot0"])
+        return vals + [Tk.expec(w), V.expec(w)]
+    st = [Q.WaveFunction(ctx, N, lo, hi).upload(w.download(s), 0) for s in range(p.S)]
+    Vs = [Q.Hamiltonian(ctx, N, lo, hi).add_potential(0, 0, a[k]) for k in ("GridPot0", "GridPot1")]
+    return (list(mom) + list(pos) + [Tk.expec(x) for x in st] + [Vs[s].expec(st[s]) for s in range(2)])
+
+
+@pytest.mark.parametrize("dims,S", [([128], 1), ([64, 48], 2), ([24, 16, 32], 1)])
+def test_expec_position_momentum_against_oracle(ctx, dims, S):
+    note(f"expec x/p {dims} S={S}")
+    import qdng_amd as Q
+    rng = np.random.default_rng(17)
+    lo = [-3.0 - 0.5 * d for d in range(len(dims))]
+    hi = [4.0 + 0.25 * d for d in range(len(dims))]
+    n = int(np.prod(dims))
+    psi = rng.standard_normal(S * n) + 1j * rng.standard_normal(S * n)
+    w = Q.WaveFunction(ctx, dims, lo, hi, S).upload(psi)
+    g = O.Grid(dims, lo, hi)
+    for d in range(len(dims)):
+        x, pm = w.expec_position(d), w.expec_momentum(d)
+        for s in range(S):
+            part = psi[s * n:(s + 1) * n].copy()
+            assert x[s] == pytest.approx(O.expec_position(g, part, d), rel=1e-12, abs=1e-12)
+            assert pm[s] == pytest.approx(O.expec_momentum(g, part, d), rel=1e-11, abs=1e-11)
 
 
 # ---------------------------------------------------------------------------
