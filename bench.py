@@ -455,7 +455,7 @@ def main():
             "traffic": measured_traffic(),
             "traffic_source": "profiles/cheby_row_2d_512x512_2state_fp64_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.000 "
                               "+ WRITE_SIZE x 1.000, calibrated on a 256 MiB device copy)",
-            "kernel": "qdx::k_cheby_row<double, 2, false, false, 256, 4>",
+            "kernel": "qdx::k_cheby_row<double, 2, false, false, 256, 4, false>",
             "points_per_thread": 4,
             "launches": launches,
             "avg_launch_us": row_ms / launches * 1e3,
@@ -465,7 +465,7 @@ def main():
             "measured": "HIP events attached to every launch (hipExtLaunchKernelGGL) over a second "
                         "timed pass of the same steps",
             "ms_per_step_profiled_pass": rp["seconds"] * 1e3 / args.steps,
-            "column_pass": {"kernel": "qdx::k_col<double, 4, 4>", "launches": rp["line"][0],
+            "column_pass": {"kernel": "qdx::k_col<double, 4, 4, false>", "launches": rp["line"][0],
                             "avg_launch_us": rp["line"][1] / max(rp["line"][0], 1) * 1e3,
                             "algorithmic_bytes_per_launch": 32.0 * p.size * p.S,
                             "note": "phi_{k-1} read + Y written, 16 B each per point and state"},
