@@ -227,7 +227,10 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
     const bool fp32 = std::is_same<R, float>::value;
     const GridDev g = p->grid.dev();
     LinePlan plans[3];
-    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan(p->grid.N[d], fp32);
+    // 4 points per thread where the grid is too small for two waves per
+    // SIMD at 8 (the rule of the Chebyshev line kernels)
+    const int ppt = cheby_line_points(ctx, g, p->S);
+    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan_ppt(p->grid.N[d], fp32, ppt);
     auto seq = gspo_sequence(p);
     XGroupDev<R> pending;
     pending.n = 0;

@@ -689,21 +689,22 @@ __device__ __forceinline__ void xgroup_point(const XGroupDev<R> &grp, long gi, c
     for (int o = 0; o < grp.n; ++o) xsplit_point<R, S, IMAG>(grp.op[o], gi, v);
 }
 
-// GSPO kinetic split step over rows.  ONE_D: psi -> pre -> FFT -> exp(-i tau
+// GSPO kinetic split step over rows (P points per thread).  ONE_D: psi ->
+// pre -> FFT -> exp(-i tau
 // T)/N -> IFFT -> post -> psi in one pass.  Otherwise PHASE 0: psi -> pre ->
 // FFT_0 -> X; PHASE 1: X -> IFFT_0 -> post -> psi.  The x-space operators act
 // on all states of a point at once, so the S lines of a row are exchanged
 // through LDS around them.
-template <class R, int S, bool ONE_D, int PHASE, bool IMAG = false>
+template <class R, int S, bool ONE_D, int PHASE, bool IMAG = false, int P = 8>
 __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, LinePlan p0, int rb,
                                                    XGroupDev<R> pre, XGroupDev<R> post, cx<R> *__restrict__ psi,
                                                    cx<R> *__restrict__ X) {
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
-    RowBlock<R> B(g, rb, S, smem, p0);
+    RowBlock<R, P> B(g, rb, S, smem, p0);
     const int st = B.st;
     const long base = (long)st * g.size;
-    cx<R> v[8];
+    cx<R> v[P];
     B.stage(p0);
     // exchange area after the transform buffers and the twiddle table: [S][rb][L]
     cx<R> *xa = B.after;
@@ -712,7 +713,7 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
         if (S == 1) {
             if (B.active) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
+                for (int q = 0; q < P; ++q) {
                     cx<double> w[S];
                     w[0] = to_d(v[q]);
                     xgroup_point<R, S, IMAG>(grp, B.gidx(q), w);
@@ -724,13 +725,13 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
         __syncthreads();
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL] = v[q];
+            for (int q = 0; q < P; ++q) xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL] = v[q];
         }
         __syncthreads();
         // the points of row rr are shared out among the S lines' threads
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < P; ++q) {
                 const int x = B.lt + q * B.TL;
                 if (x % S != st) continue;
                 cx<double> w[S];
@@ -744,27 +745,27 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
         __syncthreads();
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL];
+            for (int q = 0; q < P; ++q) v[q] = xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL];
         }
     };
     if (PHASE == 0 || ONE_D) {
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = psi[base + B.gidx(q)];
+            for (int q = 0; q < P; ++q) v[q] = psi[base + B.gidx(q)];
         }
         xspace(pre);
         B.fft(v, p0, -1);
         if (!ONE_D) {
             if (B.active) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) X[base + B.gidx(q)] = v[q];
+                for (int q = 0; q < P; ++q) X[base + B.gidx(q)] = v[q];
             }
             return;
         }
         if (B.active) {
             const bool has_t = kin.tk[st][0] != nullptr;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < P; ++q) {
                 const double tt = has_t ? 0.0 + kin.tk[st][0][B.lt + q * B.TL] : 0.0;
                 if (IMAG) {   // exp(-tau T)/N, imaginary time
                     const double f = exp(-tau * tt) * kin.inv_n;
@@ -779,14 +780,14 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
     } else {
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = X[base + B.gidx(q)];
+            for (int q = 0; q < P; ++q) v[q] = X[base + B.gidx(q)];
         }
     }
     B.fft(v, p0, +1);
     xspace(post);
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) psi[base + B.gidx(q)] = v[q];
+        for (int q = 0; q < P; ++q) psi[base + B.gidx(q)] = v[q];
     }
 }
 
@@ -1094,9 +1095,52 @@ void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, 
     QDX_HIP(hipGetLastError());
 }
 
+template <class R, int P>
+static void col_pass_P4(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
+                        const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag) {
+    const int L = g.N[dim];
+    const int W = col_tile(g, S, dim, L, sizeof(cx<R>), P);
+    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
+    const int nb = (g.N[0] / W) * nother * S;
+    const int threads = std::max(64, round64(W * (L / P)));
+    const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
+    cx<R> *x = reinterpret_cast<cx<R> *>(X);
+    LinePlan pl = plan;
+    pl.tw_lds = 1;
+    const cx<R> *none = nullptr;
+    switch (mode) {
+    case 0:
+        set_lds(k_col<R, 0, P>, lds);
+        QDX_LAUNCH((k_col<R, 0, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        break;
+    case 1:
+        set_lds(k_col<R, 1, P>, lds);
+        QDX_LAUNCH((k_col<R, 1, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        break;
+    case 3:
+        if (imag) {
+            if constexpr (std::is_same<R, double>::value) {
+                set_lds(k_col<R, 3, P, true>, lds);
+                QDX_LAUNCH((k_col<R, 3, P, true>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x,
+                           none, ctl, cc);
+            } else {
+                throw std::runtime_error("imaginary time: FP64 only");
+            }
+            break;
+        }
+        set_lds(k_col<R, 3, P>, lds);
+        QDX_LAUNCH((k_col<R, 3, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        break;
+    default:
+        throw std::runtime_error("line pass: mode 0, 1 or 3 at 4 points per thread");
+    }
+    QDX_HIP(hipGetLastError());
+}
+
 template <class R>
 void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
                          const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag) {
+    if (plan.ppt == 4) return col_pass_P4<R, 4>(s, g, S, dim, mode, plan, h, tau, X, ctl, cc, imag);
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
     const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
@@ -1205,18 +1249,30 @@ void Launch<R>::ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h, c
     QDX_DISPATCH_S(h.S, (ham_apply_S<R, SS>(s, g, h, plans, src, dst, X)));
 }
 
+template <class R, bool ONE_D, int PHASE, bool IMAG, int P>
+static void gspo_row_launch_IP(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                               const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
+                               R *X) {
+    QDX_DISPATCH_S(S, {
+        LinePlan p0 = plan;
+        RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
+        set_lds(k_gspo_row<R, SS, ONE_D, PHASE, IMAG, P>, r.lds);
+        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE, IMAG, P>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau,
+                           p0, r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
+    });
+    QDX_HIP(hipGetLastError());
+}
+
+// 4 points per thread on grids with nd >= 2 when the plan asks for it
 template <class R, bool ONE_D, int PHASE, bool IMAG>
 static void gspo_row_launch_I(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
                               const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
                               R *X) {
-    QDX_DISPATCH_S(S, {
-        LinePlan p0 = plan;
-        RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
-        set_lds(k_gspo_row<R, SS, ONE_D, PHASE, IMAG>, r.lds);
-        QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE, IMAG>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau, p0,
-                           r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
-    });
-    QDX_HIP(hipGetLastError());
+    if constexpr (!ONE_D) {
+        if (plan.ppt == 4) return gspo_row_launch_IP<R, ONE_D, PHASE, IMAG, 4>(s, g, S, kin, tau, plan, pre, post, psi, X);
+    }
+    if (plan.ppt != 8) throw std::runtime_error("GSPO rows: 8 or 4 points per thread (4 for nd >= 2)");
+    gspo_row_launch_IP<R, ONE_D, PHASE, IMAG, 8>(s, g, S, kin, tau, plan, pre, post, psi, X);
 }
 
 // imag: imaginary time (FP64 only)

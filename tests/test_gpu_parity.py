@@ -286,6 +286,38 @@ def test_gspo_fp64_laser_1d_and_3d(ctx):
     assert rel(w.download(), psi) < 1e-12
 
 
+# the GSPO kinetic kernels at 8 and 4 points per thread (grids with nd >= 2)
+@pytest.mark.parametrize("points", [4, 8])
+@pytest.mark.parametrize("name,builder,dt", [
+    ("morse_2state_2d", lambda: cases.morse_2state_2d(96), 5.0),
+    ("anharmonic_3d", lambda: cases.anharmonic_3d(24), 3.5),
+])
+def test_gspo_line_points(name, builder, dt, points):
+    note(f"gspo {name} points {points}")
+    import qdng_amd as Q
+    c = Q.Context(0)
+    try:
+        c.set_line_points(points)
+        p = builder()
+        ops = [p.gpu_split_op(c, ["kin"]), p.gpu_split_op(c, ["pot"])]
+        ops_o = [p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])]
+        G, Go = Q.GSPO(ops, dt), O.GSPO(ops_o, dt, p.S)
+        w, psi = p.gpu_wf(c), p.psi0.copy()
+        for i in range(5):
+            G.apply(w, i)
+            Go.step(psi, i)
+        assert rel(w.download(), psi) < 1e-12
+        # imaginary time takes the same kernels with the real exponent
+        G, Go = Q.GSPO(ops, dt, imag=True), O.GSPO(ops_o, dt, p.S, imag=True)
+        w, psi = p.gpu_wf(c), p.psi0.copy()
+        for i in range(3):
+            G.apply(w, i)
+            Go.step(psi, i)
+        assert rel(w.download(), psi) < 1e-12
+    finally:
+        c.close()
+
+
 def test_gspo_potential_first_ordering(ctx):
     # A = V (full step), B = T (half steps on both sides): OGSPO.cpp:119-138
     note("gspo A=V B=T")
