@@ -1,0 +1,532 @@
+"""QDng program input files (XML, `.qdi`) run on the device path.
+
+The loader follows QDng's operator-graph construction (src/ChainLoader.cpp:
+34-457): an element's `name` attribute names the class, its tag is free
+(`A`, `Tkin`, `m1.0`, ...), `key` registers the object and `ref` reuses it;
+`Sum` adds its children, `Multistate`/`DMultistate` place theirs at the matrix
+element named by the tag `m<i>.<j>`.  Supported classes: GridNablaSq
+(`mass`, comma separated per dimension), GridPotential (`file`, `setzero`,
+`offset`, `scale`; OGridPotential.cpp:36-76), GridDipole (`file`, `laser`,
+`scale`), Sum, Multistate (`unity`, `states`), DMultistate; as filters also
+GridMomentum and GridPosition (`dim`) and Projection (`files`, `start`, `num`,
+`step`; OProjection.cpp:42-68).  Propagators: Cheby (`prec`, `order`,
+`scaling`), GSPO (split operators as children), SIL (`order`, `err`), Arnoldi
+(`order`, `conv`).  Wave functions: a single `file`, or `name="Multistate"`
+with children `wf<i>` (missing states are zero), `normalize`.
+
+Programs:
+  propa  ProgPropa (src/ProgPropa.cpp:40-160, 270-340): `dt`, `steps`,
+         `wcycle` (10), `dir`, `fname` ("WF"), `nfile`; the report table of
+         Reporter::Analyze (src/Reporter.cpp:63-158, taken before each step:
+         step, time, population, populations per state, energy, <psi0|psi>),
+         post-step `expeconly` filters written to `expec_post.dat`
+         (src/FiltersList.cpp:135-330, values renormalised by <psi|psi>), the
+         wave function written every `wcycle` steps with Propagation.meta.
+  eigen  ProgEigen method "imag" (src/ProgEigen.cpp:50-80, 150-260): `dt`,
+         `steps` (512), `Nef` (20), `conv`, `diag`, `fname` ("ef"), `ename`
+         ("energy.dat"), `dir`; eigenfunctions written as a sequence and the
+         energy table in QDng's format.
+
+Not supported (QdxError with QDX_ERR_INCOMPAT): other operator classes, the
+"ac" eigen method and `LC` wave functions, pre-step filters and filter actions
+other than `expeconly`, and a propagation `start` other than 0.  The spectrum
+file (`spectrum=`) is not written.
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import os
+import re
+import sys
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from . import (Arnoldi, Cheby, GSPO, Hamiltonian, QdxError, SIL, WaveFunction, eigen_imag,
+               write_propagation_meta)
+
+ERR_PARAM, ERR_INCOMPAT = -1, -2
+AU2WAVENUMBERS = 219475.0   # src/qdlib/Conversion.h:9
+
+
+def _err(msg, code=ERR_PARAM):
+    return QdxError(code, msg)
+
+
+def _bool(v, default=False):
+    if v is None:
+        return default
+    s = v.strip().lower()
+    if s in ("true", "1", "yes", "on"):
+        return True
+    if s in ("false", "0", "no", "off"):
+        return False
+    raise _err(f"not a boolean: '{v}'")
+
+
+def _attrs(el, srcdir):
+    a = dict(el.attrib)
+    if srcdir is not None:
+        a = {k: v.replace("@srcdir@", srcdir) for k, v in a.items()}
+    return a
+
+
+# ---------------------------------------------------------------------------
+# operator graph
+
+class Term:
+    """One grid operator: kind 'kin', 'pot', 'dip' (Hamiltonian terms) or
+    'mom', 'pos', 'proj' (filters only)."""
+
+    def __init__(self, kind, cls, **params):
+        self.kind, self.cls, self.params = kind, cls, params
+
+
+class Sum:
+    def __init__(self, items):
+        self.items, self.cls = items, "Sum"
+
+
+class Multi:
+    def __init__(self, elems, states, unity, dmulti):
+        self.elems, self.states, self.unity, self.dmulti = elems, states, unity, dmulti
+        self.cls = "DMultistate" if dmulti else "Multistate"
+
+    @property
+    def S(self):
+        n = max(max(i, j) for i, j in self.elems) + 1 if self.elems else 1
+        return max(n, self.states)
+
+
+class Loader:
+    """ChainLoader: operators from XML elements with key/ref bookkeeping."""
+
+    def __init__(self, srcdir=None):
+        self.srcdir = srcdir
+        self.keys = {}
+
+    def op(self, el):
+        a = _attrs(el, self.srcdir)
+        if "ref" in a:
+            if a["ref"] not in self.keys:
+                raise _err(f"operator reference '{a['ref']}' is not defined")
+            return self.keys[a["ref"]]
+        name = a.get("name")
+        if not name:
+            raise _err(f"element <{el.tag}> names no operator class")
+        if name == "Sum":
+            o = Sum([self.op(c) for c in el])
+            if not o.items:
+                raise _err("empty Sum")
+        elif name in ("Multistate", "DMultistate"):
+            elems = {}
+            for c in el:
+                m = re.fullmatch(r"m(\d+)\.(\d+)", c.tag)
+                if not m:
+                    raise _err(f"matrix element tag <{c.tag}> is not m<i>.<j>")
+                elems[(int(m.group(1)), int(m.group(2)))] = self.op(c)
+            o = Multi(elems, int(a.get("states", 0)), _bool(a.get("unity"), True), name == "DMultistate")
+        elif name == "GridNablaSq":
+            if "mass" not in a:
+                raise _err("GridNablaSq needs a mass")
+            o = Term("kin", name, mass=[float(v) for v in a["mass"].split(",")])
+        elif name == "GridPotential":
+            o = Term("pot", name, file=self._file(a, name), setzero=_bool(a.get("setzero")),
+                     offset=float(a.get("offset", 0.0)), scale=float(a.get("scale", 1.0)))
+        elif name == "GridDipole":
+            if "laser" not in a:
+                raise _err("GridDipole needs a laser file")
+            o = Term("dip", name, file=self._file(a, name), laser=a["laser"], scale=float(a.get("scale", 1.0)))
+        elif name in ("GridMomentum", "GridPosition"):
+            o = Term("mom" if name == "GridMomentum" else "pos", name, dim=int(a.get("dim", 0)))
+        elif name == "Projection":
+            o = Term("proj", name, files=self._file(a, name, "files"), start=int(a.get("start", 0)),
+                     num=int(a.get("num", 1)), step=int(a.get("step", 1)))
+        else:
+            raise _err(f"operator class '{name}' is not supported", ERR_INCOMPAT)
+        if "key" in a:
+            self.keys[a["key"]] = o
+        return o
+
+    @staticmethod
+    def _file(a, name, attr="file"):
+        if attr not in a:
+            raise _err(f"{name} needs a '{attr}' attribute")
+        return a[attr]
+
+    def propagator(self, el):
+        a = _attrs(el, self.srcdir)
+        name = a.get("name")
+        if name in ("Cheby", "SIL", "Arnoldi"):
+            hs = [c for c in el if c.tag == "hamiltonian"]
+            if len(hs) != 1:
+                raise _err(f"{name} needs one <hamiltonian>")
+            return name, a, [self.op(hs[0])]
+        if name == "GSPO":
+            ops = [self.op(c) for c in el]
+            if len(ops) < 2:
+                raise _err("GSPO needs at least two split operators")
+            return name, a, ops
+        raise _err(f"propagator '{name}' is not supported", ERR_INCOMPAT)
+
+
+def _terms(op):
+    """Flatten a Sum (or a single term) into Hamiltonian terms."""
+    if isinstance(op, Term):
+        if op.kind not in ("kin", "pot", "dip"):
+            raise _err(f"{op.cls} is a filter, not a Hamiltonian term", ERR_INCOMPAT)
+        return [op]
+    if isinstance(op, Sum):
+        return [t for o in op.items for t in _terms(o)]
+    raise _err(f"{op.cls} inside a matrix element is not supported", ERR_INCOMPAT)
+
+
+def _add_term(H, i, j, t):
+    p = t.params
+    if t.kind == "kin":
+        if i != j:
+            raise _err("a kinetic operator belongs on the diagonal", ERR_INCOMPAT)
+        H.add_kinetic(i, p["mass"])
+    elif t.kind == "pot":
+        H.add_potential_file(i, j, p["file"], p["setzero"], p["offset"], p["scale"])
+    else:
+        H.add_dipole_file(i, j, p["file"], p["laser"], p["scale"])
+
+
+def _grid_lists(w):
+    g = w.grid
+    return list(w.N), [g.xmin[d] for d in range(len(w.N))], [g.xmax[d] for d in range(len(w.N))]
+
+
+def build_hamiltonian(ctx, op, psi):
+    """A device Hamiltonian (OSum or OMultistate) on the grid of psi."""
+    N, lo, hi = _grid_lists(psi)
+    if isinstance(op, Multi):
+        if op.S != psi.S:
+            raise _err(f"{op.cls} has {op.S} states, the wave function {psi.S}", ERR_INCOMPAT)
+        H = Hamiltonian(ctx, N, lo, hi, op.S, True, op.unity)
+        for (i, j), e in sorted(op.elems.items()):
+            for t in _terms(e):
+                _add_term(H, i, j, t)
+        return H
+    if psi.S != 1:
+        raise _err("a multistate wave function needs a Multistate operator", ERR_INCOMPAT)
+    H = Hamiltonian(ctx, N, lo, hi)
+    for t in _terms(op):
+        _add_term(H, 0, 0, t)
+    return H
+
+
+def _merged(ops):
+    """The Hamiltonian of a GSPO propagator: the sum of its split operators."""
+    if all(isinstance(o, Multi) for o in ops):
+        elems = {}
+        for o in ops:
+            for k, e in o.elems.items():
+                elems.setdefault(k, []).append(e)
+        return Multi({k: Sum(v) for k, v in elems.items()}, max(o.S for o in ops), True, False)
+    if any(isinstance(o, Multi) for o in ops):
+        raise _err("GSPO split operators mix Multistate and single-state operators", ERR_INCOMPAT)
+    return Sum(list(ops))
+
+
+def build_propagator(ctx, spec, psi, dt, imag=False):
+    """(propagator, Hamiltonian of the energies) from Loader.propagator()."""
+    name, a, ops = spec
+    if name == "GSPO":
+        U = GSPO([build_hamiltonian(ctx, o, psi) for o in ops], dt, imag=imag)
+        return U, build_hamiltonian(ctx, _merged(ops), psi)
+    H = build_hamiltonian(ctx, ops[0], psi)
+    if name == "Cheby":
+        U = Cheby(H, dt, order=int(a.get("order", 0)), prec=float(a.get("prec", -1.0)),
+                  scaling=float(a.get("scaling", 0.0)), imag=imag)
+    elif name == "SIL":
+        U = SIL(H, dt, order=int(a.get("order", 0)), err=float(a.get("err", 0.0)), imag=imag)
+    else:
+        U = Arnoldi(H, dt, order=int(a.get("order", 0)), conv=float(a.get("conv", 0.0)), imag=imag)
+    return U, H
+
+
+def load_wave_function(ctx, el, srcdir=None):
+    a = _attrs(el, srcdir)
+    name = a.get("name")
+    if name in (None, "WFGridCartesian"):
+        if "file" not in a:
+            raise _err("<wf> needs a file")
+        w = WaveFunction.load(ctx, a["file"])
+    elif name == "Multistate":
+        parts = {}
+        for c in el:
+            m = re.fullmatch(r"wf(\d+)", c.tag)
+            if not m:
+                raise _err(f"state tag <{c.tag}> is not wf<i>")
+            parts[int(m.group(1))] = load_wave_function(ctx, c, srcdir)
+        if not parts:
+            raise _err("empty Multistate wave function")
+        S = max(int(a.get("states", 0)), max(parts) + 1)
+        first = next(iter(parts.values()))
+        N, lo, hi = _grid_lists(first)
+        w = WaveFunction(ctx, N, lo, hi, S)
+        w.upload(np.zeros(S * w.size, dtype=np.complex128))
+        for s, p in parts.items():
+            if p.S != 1 or _grid_lists(p) != (N, lo, hi):
+                raise _err("the states of a Multistate wave function need one grid", ERR_INCOMPAT)
+            w.upload(p.download(), s)
+    else:
+        raise _err(f"wave function type '{name}' is not supported", ERR_INCOMPAT)
+    if _bool(a.get("normalize")):
+        w.normalize()
+    return w
+
+
+# ---------------------------------------------------------------------------
+# post-step filters (expeconly)
+
+class _Filter:
+    def __init__(self, ctx, op, psi, label):
+        self.label = label
+        self.kind = None
+        N, lo, hi = _grid_lists(psi)
+        if isinstance(op, Multi):
+            # expectation of an S-state operator with (mostly) one element
+            kinds = [e.kind for e in op.elems.values() if isinstance(e, Term)]
+            if len(op.elems) == 1 and kinds and kinds[0] in ("mom", "pos"):
+                (s, s2), e = next(iter(op.elems.items()))
+                if s != s2:
+                    raise _err("position/momentum filters sit on the diagonal", ERR_INCOMPAT)
+                self.kind, self.state, self.dim = e.kind, s, e.params["dim"]
+            else:
+                if op.S != psi.S:
+                    raise _err(f"filter has {op.S} states, the wave function {psi.S}", ERR_INCOMPAT)
+                self.kind, self.H = "ham", build_hamiltonian(ctx, op, psi)
+        elif isinstance(op, Term) and op.kind in ("mom", "pos"):
+            if psi.S != 1:
+                raise _err("position/momentum of a multistate function: use a Multistate filter", ERR_INCOMPAT)
+            self.kind, self.state, self.dim = op.kind, 0, op.params["dim"]
+        elif isinstance(op, Term) and op.kind == "proj":
+            if psi.S != 1:
+                raise _err("Projection filters on one state only", ERR_INCOMPAT)
+            p = op.params
+            self.kind = "proj"
+            self.efs = [WaveFunction.load(ctx, p["files"], counter=p["start"] + k * p["step"])
+                        for k in range(p["num"])]
+        else:
+            self.kind, self.H = "ham", build_hamiltonian(ctx, op, psi)
+
+    def value(self, psi, step):
+        if self.kind == "mom":
+            return float(psi.expec_momentum(self.dim)[self.state])
+        if self.kind == "pos":
+            return float(psi.expec_position(self.dim)[self.state])
+        if self.kind == "proj":
+            return float(sum(abs(psi.overlap(e)) ** 2 for e in self.efs))
+        return self.H.expec(psi, step)
+
+
+def _g10(v):
+    return "%.10g" % v
+
+
+# ---------------------------------------------------------------------------
+# programs
+
+class Result:
+    """What a program produced: report rows (propa), energies (eigen), and
+    the output folder."""
+
+    def __init__(self, kind, directory):
+        self.kind, self.dir = kind, directory
+        self.report = []       # (step, time, norm2, pops, energy, autocorr)
+        self.report_text = []
+        self.expec_post = []   # (time, values)
+        self.energies = None
+        self.energies_raw = None
+        self.steps = None
+
+
+def _outdir(a, outdir):
+    d = a.get("dir", "")
+    if d and not d.endswith("/"):
+        d += "/"
+    if outdir:
+        d = os.path.join(outdir, d)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    return d
+
+
+def _report_line(step, time, n2, pops, energy, ac):
+    s = "%d\t%.1f\t" % (step, time)
+    s += "\t%.6f" % n2
+    for p in pops:
+        s += "\t%.6f" % p
+    s += "\t%.8f" % energy
+    s += "\t%.8f,%.8fi" % (ac.real, ac.imag)
+    return s
+
+
+def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
+    a = _attrs(el, srcdir)
+    dt, steps = float(a.get("dt", 0)), int(a.get("steps", 0))
+    if dt == 0:
+        raise _err("Zero time step defined")
+    if steps == 0:
+        raise _err("Zero number of time step defined")
+    wcycle = int(a.get("wcycle", 10))
+    if wcycle < 1:
+        raise _err("Write cycle less than one defined")
+    if int(a.get("start", 0)) != 0:
+        raise _err("propagation restart (start != 0) is not supported", ERR_INCOMPAT)
+    d = _outdir(a, outdir)
+    fname = a.get("fname", "WF")
+    ld = Loader(srcdir)
+    props = [c for c in el if c.tag == "propagator"]
+    wfs = [c for c in el if c.tag == "wf"]
+    if len(props) != 1 or len(wfs) != 1:
+        raise _err("<propa> needs one <propagator> and one <wf>")
+    if any(c.tag == "filterpre" for c in el):
+        raise _err("pre-step filters are not supported", ERR_INCOMPAT)
+    spec = ld.propagator(props[0])
+    psi = load_wave_function(ctx, wfs[0], srcdir)
+    U, H = build_propagator(ctx, spec, psi, dt)
+    N, lo, hi = _grid_lists(psi)
+    psi0 = WaveFunction(ctx, N, lo, hi, psi.S)
+    psi0.copy_from(psi)
+    filters = []
+    for sec in (c for c in el if c.tag == "filterpost"):
+        for f in sec:
+            if f.tag != "expeconly":
+                raise _err(f"filter action <{f.tag}> is not supported", ERR_INCOMPAT)
+            op = ld.op(f)
+            fa = _attrs(f, srcdir)
+            filters.append(_Filter(ctx, op, psi, fa.get("header", "O" + op.cls)))
+    res = Result("propa", d)
+    out = open(d + a["nfile"], "w") if a.get("nfile") else None
+    header = "Step\tTime[au]\tPopulation" + "".join("\t\tPop_%d" % s for s in range(psi.S) if psi.S > 1) + \
+        "\t\tEnergy[au]\t<Psi0|PsiT>"
+    emit = (lambda s: out.write(s + "\n")) if out else (log or (lambda s: None))
+    emit(header)
+    pf = open(d + "expec_post.dat", "w") if filters else None
+    if pf:
+        pf.write("time \t" + "".join(f.label + "\t" for f in filters) + "\n")
+    psi.save(d + fname, counter=0)
+    nwritten = 1
+    try:
+        for i in range(1, steps + 1):
+            # Reporter::Analyze on the state of clock step i - 1
+            n, per = psi.norm()
+            pops = [float(x) for x in per] if psi.S > 1 else []   # squared norms per state
+            e = H.expec(psi, i - 1)
+            ac = psi.overlap(psi0)
+            row = (i - 1, (i - 1) * dt, n * n, pops, e, ac)
+            if not math.isfinite(n):
+                raise _err("Norm is not a number")
+            res.report.append(row)
+            line = _report_line(*row)
+            res.report_text.append(line)
+            emit(line)
+            U.apply(psi, i - 1)
+            if filters:
+                n2 = psi.norm()[0] ** 2
+                vals = [f.value(psi, i) / n2 for f in filters]
+                res.expec_post.append((i * dt, vals))
+                pf.write(_g10(i * dt) + "\t" + "".join(_g10(v) + "\t" for v in vals) + "\n")
+            if i % wcycle == 0:
+                psi.save(d + fname, counter=nwritten)
+                nwritten += 1
+                write_propagation_meta(d, i, dt, fname, wcycle)
+    finally:
+        if out:
+            out.close()
+        if pf:
+            pf.close()
+    res.psi = psi
+    return res
+
+
+def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
+    a = _attrs(el, srcdir)
+    method = a.get("method", "imag")
+    if method != "imag":
+        raise _err(f"eigen method '{method}' is not supported", ERR_INCOMPAT)
+    if "dt" not in a:
+        raise _err("No time step defined")
+    dt = float(a["dt"])
+    nef = int(a.get("Nef", 20))
+    diag = _bool(a.get("diag"), True)
+    conv = float(a["conv"]) if "conv" in a else None
+    max_steps = int(a.get("steps", 512))
+    d = _outdir(a, outdir)
+    ld = Loader(srcdir)
+    props = [c for c in el if c.tag == "propagator"]
+    wfs = [c for c in el if c.tag == "wf"]
+    if len(props) != 1 or len(wfs) != 1:
+        raise _err("<eigen> needs one <propagator> and one <wf>")
+    spec = ld.propagator(props[0])
+    psi = load_wave_function(ctx, wfs[0], srcdir)
+    U, H = build_propagator(ctx, spec, psi, dt, imag=True)
+    efs, e, er, st = eigen_imag(U, psi, nef, max_steps, conv, diag, H)
+    fname = a.get("fname", "ef")
+    for k, w in enumerate(efs):
+        w.save(d + fname, counter=k)
+    en = e if diag else er
+    with open(d + a.get("ename", "energy.dat"), "w") as f:
+        for i in range(nef):
+            f.write("%d\t%.8f" % (i, en[i]))
+            f.write("\t%.2f" % (en[i] * AU2WAVENUMBERS))
+            f.write("\t%.2f" % ((en[i] - en[0]) * AU2WAVENUMBERS))
+            if i > 0:
+                f.write("\t%.2f" % ((en[i] - en[i - 1]) * AU2WAVENUMBERS))
+            f.write("\n")
+    res = Result("eigen", d)
+    res.energies, res.energies_raw, res.steps, res.efs = e, er, st, efs
+    if log:
+        for i in range(nef):
+            log("%d\t%.8f" % (i, en[i]))
+    return res
+
+
+PROGRAMS = {"propa": run_propa, "eigen": run_eigen}
+
+
+def programs(path):
+    """The program elements of a .qdi file, in order."""
+    root = ET.parse(path).getroot()
+    if root.tag != "qdng":
+        raise _err(f"{path}: the root element is <{root.tag}>, not <qdng>")
+    return list(root)
+
+
+def run(path, ctx, srcdir=None, outdir=None, which=None, log=None):
+    """Run the programs of a QDng input file (all, or the indices in `which`)
+    on the device.  srcdir replaces @srcdir@ in attribute values; outdir
+    prefixes the programs' `dir`.  Returns one Result per program run."""
+    out = []
+    for k, el in enumerate(programs(path)):
+        if which is not None and k not in which:
+            continue
+        if el.tag not in PROGRAMS:
+            raise _err(f"program <{el.tag}> is not supported", ERR_INCOMPAT)
+        out.append(PROGRAMS[el.tag](ctx, el, srcdir, outdir, log))
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="Run a QDng program input file on the GPU")
+    ap.add_argument("input")
+    ap.add_argument("-d", "--dir", default=None, help="output folder prefix")
+    ap.add_argument("--srcdir", default=None, help="value of @srcdir@")
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args(argv)
+    from . import Context
+    ctx = Context(args.device)
+    try:
+        run(args.input, ctx, args.srcdir, args.dir, log=lambda s: print(s, flush=True))
+    finally:
+        ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,180 @@
+"""QDng program inputs (XML .qdi) through qdng_amd.qdi.
+
+The inputs are QDng's own grid regression tests (tests/*.qdi.in, copied
+unchanged with their data files by tests/golden/make_file_fixtures.py).  CPU
+tests check the operator graphs the loader builds (key/ref sharing, matrix
+elements, @srcdir@) and its error paths; the `gpu` tests run the programs end
+to end and compare the last report line and the last expec_post.dat line with
+the values tests/checktests.sh greps for, and the eigen programs' energy.dat
+with CheckTGE-im-Energy (checktests.sh:4-29)."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from test_oracle import _digits_ok
+import qdng_amd as Q
+from qdng_amd import qdi
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qdng_files")
+PROPA = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby", "TGPL-IR-Pulse-GSPO",
+         "TGPLMS-Cheby", "TGPLMS-SIL"]
+EIGEN = ["TGE-Cheby", "TGE-GSPO", "TGE-SIL", "TGE-Arnoldi"]
+# checktests.sh: CheckT-Expec TGPLMS-SIL (the report line equals TGPLMS-Cheby's)
+SIL_POST = ["-0.3059539159", "3.364408943", "1.661276151", "1.739725823", "0.001196532131", "0.007276466639",
+            "0.001079240961", "0.07679400487"]
+TGE_ENERGIES = ["0.0037100", "0.0110668", "0.0181411", "0.0249017", "0.0313281", "0.0374320",
+                "0.0431786", "0.0485", "0.0535", "0.058"]
+
+
+def qdi_path(name):
+    return os.path.join(FIX, name + ".qdi.in")
+
+
+def prog(name, k=0):
+    return qdi.programs(qdi_path(name))[k]
+
+
+def test_every_input_parses():
+    for name in PROPA + EIGEN:
+        for el in qdi.programs(qdi_path(name)):
+            assert el.tag in ("propa", "eigen")
+            ld = qdi.Loader(FIX)
+            props = [c for c in el if c.tag == "propagator"]
+            assert len(props) == 1
+            pname, attrs, ops = ld.propagator(props[0])
+            assert pname in ("Cheby", "GSPO", "SIL", "Arnoldi")
+            for sec in (c for c in el if c.tag == "filterpost"):
+                for f in sec:
+                    ld.op(f)
+
+
+def test_multistate_graph_with_refs():
+    ld = qdi.Loader(FIX)
+    el = prog("TGPLMS-Cheby")
+    name, a, (H,) = ld.propagator([c for c in el if c.tag == "propagator"][0])
+    assert name == "Cheby" and float(a["prec"]) == 1e-12
+    assert isinstance(H, qdi.Multi) and not H.dmulti and H.unity and H.S == 2
+    assert sorted(H.elems) == [(0, 0), (1, 0), (1, 1)]
+    t0, v0 = H.elems[(0, 0)].items
+    t1, v1 = H.elems[(1, 1)].items
+    assert t0 is t1 and t0.kind == "kin" and t0.params["mass"] == [1763.2]   # <Tkin ref="T"/>
+    assert v0.params["file"] == os.path.join(FIX, "GridPot0") and v1.params["file"].endswith("GridPot1")
+    dip = H.elems[(1, 0)]
+    assert dip.kind == "dip" and dip.params["laser"] == os.path.join(FIX, "GridLaser_w0.176")
+    # the filters refer to the Hamiltonian's terms by key
+    filt = [ld.op(f) for f in [c for c in el if c.tag == "filterpost"][0]]
+    assert [type(f) for f in filt] == [qdi.Multi] * 8
+    assert filt[1].elems[(1, 1)] is filt[0].elems[(0, 0)]               # <m1.1 ref="p"/>
+    assert filt[6].elems[(0, 0)] is v0 and filt[7].elems[(1, 1)] is v1
+    assert not filt[0].unity and filt[0].S == 2
+
+
+def test_gspo_split_operators_and_their_sum():
+    ld = qdi.Loader(FIX)
+    el = prog("TGPMS-GSPO")
+    name, a, ops = ld.propagator([c for c in el if c.tag == "propagator"][0])
+    assert name == "GSPO" and len(ops) == 2 and all(o.dmulti for o in ops)
+    m = qdi._merged(ops)
+    assert sorted(m.elems) == [(0, 0), (1, 1)]
+    assert [t.kind for t in qdi._terms(m.elems[(1, 1)])] == ["kin", "pot"]
+    ld = qdi.Loader(FIX)
+    el = prog("TGPL-IR-Pulse-GSPO")
+    name, a, ops = ld.propagator([c for c in el if c.tag == "propagator"][0])
+    assert [o.kind for o in ops] == ["kin", "pot", "dip"]
+    assert [t.kind for t in qdi._terms(qdi._merged(ops))] == ["kin", "pot", "dip"]
+
+
+def test_loader_errors(tmp_path):
+    import xml.etree.ElementTree as ET
+    ld = qdi.Loader()
+    for xml, code in [('<A ref="nope"/>', -1), ('<A name="GridKick"/>', -2), ('<A/>', -1),
+                      ('<A name="GridPotential"/>', -1), ('<A name="GridDipole" file="x"/>', -1),
+                      ('<A name="Multistate"><x0 name="GridNablaSq" mass="1"/></A>', -1),
+                      ('<A name="Sum"/>', -1), ('<A name="GridPotential" file="f" setzero="maybe"/>', -1)]:
+        with pytest.raises(Q.QdxError) as e:
+            ld.op(ET.fromstring(xml))
+        assert e.value.code == code, xml
+    with pytest.raises(Q.QdxError) as e:
+        ld.propagator(ET.fromstring('<propagator name="SPO"/>'))
+    assert e.value.code == -2
+    with pytest.raises(Q.QdxError):
+        ld.propagator(ET.fromstring('<propagator name="GSPO"><A name="GridNablaSq" mass="1"/></propagator>'))
+    # a kinetic operator off the diagonal, a filter inside a Hamiltonian
+    m = ld.op(ET.fromstring('<H name="Multistate"><m1.0 name="GridNablaSq" mass="1"/></H>'))
+    with pytest.raises(Q.QdxError):
+        qdi._add_term(None, 1, 0, m.elems[(1, 0)])
+    with pytest.raises(Q.QdxError) as e:
+        qdi._terms(ld.op(ET.fromstring('<F name="GridMomentum"/>')))
+    assert e.value.code == -2
+    # the eigen method "ac" (second program of the TGE inputs) is rejected
+    # before anything touches a device
+    with pytest.raises(Q.QdxError) as e:
+        qdi.run(qdi_path("TGE-Cheby"), None, FIX, str(tmp_path), which=[1])
+    assert e.value.code == -2
+    bad = tmp_path / "bad.qdi"
+    bad.write_text("<root><propa/></root>")
+    with pytest.raises(Q.QdxError):
+        qdi.programs(str(bad))
+
+
+# ---------------------------------------------------------------------------
+# end to end on the device
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", PROPA)
+def test_propagation_inputs(ctx, name, tmp_path):
+    print("[gpu case] qdi", name, flush=True)
+    (res,) = qdi.run(qdi_path(name), ctx, FIX, str(tmp_path))
+    gold = cases.kat_data()[1]["golden"]["TGPLMS-Cheby" if name == "TGPLMS-SIL" else name]
+    rep = gold["report"]
+    step, time, n2, pops, energy, ac = res.report[-1]
+    assert (step, time) == (rep["step"], rep["time"])
+    assert abs(n2 - rep["population"]) < 5e-7
+    for got, want in zip(pops, rep["pops"]):
+        assert abs(got - want) < 5e-7
+    assert abs(energy - rep["energy"]) < 6e-9, (energy, rep["energy"])
+    assert abs(ac.real - rep["autocorr"][0]) < 6e-9 and abs(ac.imag - rep["autocorr"][1]) < 6e-9
+    # expec_post.dat: header plus one line per step, the last at steps * dt
+    lines = open(os.path.join(res.dir, "expec_post.dat")).read().splitlines()
+    assert lines[0].startswith("time \t") and len(lines) == 1 + len(res.report)
+    last = lines[-1].split("\t")
+    post = gold["expec_post"]
+    assert float(last[0]) == pytest.approx(post["time"])
+    want = SIL_POST if name == "TGPLMS-SIL" else post["values"]
+    prefix = [False] * len(SIL_POST) if name == "TGPLMS-SIL" else post["prefix"]
+    vals = [float(v) for v in last[1:] if v]
+    assert len(vals) == len(want)
+    for v, w, pf in zip(vals, want, prefix):
+        assert _digits_ok(v, w, pf), (name, v, w)
+    # wave functions: the initial one and one per write cycle, Propagation.meta
+    a = open(qdi_path(name)).read()
+    steps = len(res.report)
+    wcycle = int(a.split('wcycle="')[1].split('"')[0])
+    nfiles = 1 + steps // wcycle
+    base = os.path.join(res.dir, "WF")
+    S = res.psi.S
+    for k in range(nfiles):
+        f = base + ("-0" if S > 1 else "") + "_%d.wf" % k
+        assert os.path.exists(f), f
+    if steps >= wcycle:
+        assert Q.read_meta(os.path.join(res.dir, "Propagation.meta"))["Nt"] == str(wcycle * (steps // wcycle))
+        last_wf = Q.WaveFunction.load(ctx, base, counter=nfiles - 1)
+        assert np.array_equal(last_wf.download(), res.psi.download())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", EIGEN)
+def test_eigen_inputs(ctx, name, tmp_path):
+    print("[gpu case] qdi", name, flush=True)
+    (res,) = qdi.run(qdi_path(name), ctx, FIX, str(tmp_path), which=[0])
+    rows = [l.split("\t") for l in open(os.path.join(res.dir, "energy.dat")).read().splitlines()]
+    assert [r[0] for r in rows] == [str(i) for i in range(10)]
+    for r, want in zip(rows, TGE_ENERGIES):
+        assert r[1].startswith(want), (name, r[1], want)
+    assert float(rows[0][2]) == pytest.approx(float(rows[0][1]) * 219475, abs=0.01)
+    for k in range(10):
+        assert os.path.exists(os.path.join(res.dir, "ef_%d.wf" % k))
+    ef = Q.WaveFunction.load(ctx, os.path.join(res.dir, "ef"), counter=3)
+    assert ef.norm()[0] == pytest.approx(1.0, abs=1e-9)
