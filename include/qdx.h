@@ -198,6 +198,22 @@ int qdx_arnoldi_create(qdx_ham *h, double dt, int32_t order, double conv, int32_
  * nef values; energies_raw and steps (nullable) the relaxation's own. */
 int qdx_eigen_imag(qdx_prop *U, qdx_ham *H, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv,
                    int32_t diag, qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps);
+/* ProgEigen method "ac" (ProgEigen::AutoCorrEF, src/ProgEigen.cpp:414-600):
+ * nsteps - 1 real-time steps of U from psi0, every state kept on the device
+ * and c(t) = <psi(t)|psi0>; the Hann window (win != 0, math_functions.h:
+ * 108-113), the forward FFT and PeakFinder on |FFT c| (tolerance tol, QDng
+ * default 1; width > 0: the windowed search, else the region search,
+ * tools/PeakFinder.cpp:83-170) give one energy E = index * 2 pi/(dt nsteps)
+ * (index - nsteps above nsteps/2) per peak; each eigenfunction is
+ * psi0 + sum_s psi(s dt) e^{i E s dt}, normalised, with the raw energy
+ * <psi|H|psi>; diag != 0 then diagonalises H in that basis (DiagBasis).
+ * *nef receives the number of peaks (QDX_ERR_PARAM when above max_ef; efs,
+ * energies and energies_raw hold max_ef entries); autocorr (nullable,
+ * 2*nsteps doubles) receives c(t) before the window.  Device memory: nsteps
+ * wave functions. */
+int qdx_eigen_ac(qdx_prop *U, qdx_ham *H, const qdx_wf *psi0, int32_t nsteps, int32_t win, double tol,
+                 int32_t width, int32_t diag, int32_t max_ef, qdx_wf **efs, double *energies, double *energies_raw,
+                 int32_t *nef, double *autocorr);
 /* The scalars and coefficients OCheby::Init would choose for this Hamiltonian,
  * without allocating device memory (h may have been created with ctx NULL).
  * coeff (nullable) receives min(coeff_len, order) values. */

@@ -307,3 +307,74 @@ class GSPO:
     def step(self, psi: np.ndarray, step: int = 0) -> None:
         assert psi.dtype == np.complex128 and psi.flags.c_contiguous
         _chk(lib().orc_gspo_step(self.id, int(step), _dp(psi)))
+
+
+# ---------------------------------------------------------------------------
+# ProgEigen method "ac" (test infrastructure: numpy restatement)
+
+def peaks_regions(d, tol=1.0):
+    """PeakFinder::Find(data) (tools/PeakFinder.cpp:83-126): the maximum of
+    every run of values above mean + tol*sigma (population sigma); a run still
+    open at the end counts when the last value is not its maximum."""
+    d = np.asarray(d, dtype=float)
+    lim = d.mean() + tol * d.std()
+    pk, open_, dmax, imax = [], False, 0.0, -1
+    for i, v in enumerate(d):
+        if not open_ and v > lim:
+            open_, dmax, imax = True, 0.0, -1
+        if open_ and v < lim:
+            open_ = False
+            pk.append(imax)
+        if open_ and v > dmax:
+            dmax, imax = v, i
+    if open_ and imax != -1 and d[-1] < dmax:
+        pk.append(imax)
+    return pk
+
+
+def peaks_window(d, tol, w):
+    """PeakFinder::Find(data, winsize) (tools/PeakFinder.cpp:131-170): the
+    strict local maxima (Maximas, 177-190: d falls after rising), kept when
+    above mean + tol*sigma of d[max(0, i-w) : min(n-1, i+w)]."""
+    d = np.asarray(d, dtype=float)
+    n = d.size
+    cand = [i - 1 for i in range(2, n) if d[i] < d[i - 1] and d[i - 1] - d[i - 2] > 0]
+    pk = []
+    for m in cand:
+        lo, hi = max(0, m - w), min(n - 1, m + w)
+        seg = d[lo:hi]
+        if d[m] > seg.mean() + tol * seg.std():
+            pk.append(m)
+    return pk
+
+
+def autocorr_ef(step, overlap, expec, psi0, nsteps, dt, win=True, tol=1.0, width=-1):
+    """ProgEigen::AutoCorrEF (ProgEigen.cpp:414-600) over callables:
+    step(psi, k) propagates in place at clock step k, overlap(a, b) = <a|b>,
+    expec(psi) = <psi|H|psi>.  Returns (raw energies, peak indices,
+    autocorrelation before the window, eigenfunctions)."""
+    buf = [psi0.copy()]
+    ac = np.zeros(nsteps, dtype=complex)
+    ac[0] = 1.0
+    psi = psi0.copy()
+    for i in range(1, nsteps):
+        step(psi, i - 1)
+        buf.append(psi.copy())
+        ac[i] = overlap(psi, psi0)
+    raw = ac.copy()
+    if win:   # Hann window over the whole record (math_functions.h:108-113)
+        n = np.arange(nsteps)
+        ac = ac * 0.5 * (1.0 - np.cos(2 * np.pi * n / (nsteps - 1)))
+    power = np.abs(np.fft.fft(ac))   # FFTW_FORWARD, unnormalised
+    pk = peaks_window(power, tol, width) if width > 0 else peaks_regions(power, tol)
+    dw = 2 * np.pi / (dt * nsteps)
+    e_raw, efs = [], []
+    for idx in pk:
+        e = idx * dw if idx < nsteps // 2 else (idx - nsteps) * dw
+        f = psi0.copy()
+        for s in range(1, nsteps):
+            f += buf[s] * np.exp(1j * e * s * dt)
+        f /= np.sqrt(overlap(f, f).real)
+        efs.append(f)
+        e_raw.append(expec(f))
+    return np.array(e_raw), pk, raw, efs

@@ -61,7 +61,7 @@ EXPORTS = [
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
     "qdx_batch_set_lds_schedule", "qdx_batch_get_lds_schedule", "qdx_batch_autotune_lds",
     "qdx_ctx_set_line_points", "qdx_ctx_get_line_points", "qdx_cheby_create_imag", "qdx_bessel_i0",
-    "qdx_wf_axpy", "qdx_wf_expec_position", "qdx_wf_expec_momentum", "qdx_eigen_imag", "qdx_sil_create", "qdx_gspo_create_imag", "qdx_arnoldi_create",
+    "qdx_wf_axpy", "qdx_wf_expec_position", "qdx_wf_expec_momentum", "qdx_eigen_imag", "qdx_eigen_ac", "qdx_sil_create", "qdx_gspo_create_imag", "qdx_arnoldi_create",
     # include/qdx_io.h
     "qdx_meta_create", "qdx_meta_read", "qdx_meta_write", "qdx_meta_size", "qdx_meta_key", "qdx_meta_get",
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
@@ -170,6 +170,7 @@ def lib():
         "qdx_gspo_create_imag": [C.POINTER(vp), i32, C.c_double, C.POINTER(vp)],
         "qdx_arnoldi_create": [vp, C.c_double, i32, C.c_double, i32, C.POINTER(vp)],
         "qdx_eigen_imag": [vp, vp, vp, i32, i32, C.c_double, i32, C.POINTER(vp), dp, dp, C.POINTER(i32)],
+        "qdx_eigen_ac": [vp, vp, vp, i32, i32, C.c_double, i32, i32, i32, C.POINTER(vp), dp, dp, C.POINTER(i32), dp],
         "qdx_ctx_get_line_points": [vp, C.POINTER(i32)],
         "qdx_batch_get_tiling": [vp, i32, i32, C.POINTER(Tiling)],
         "qdx_batch_set_tiling": [vp, i32, i32, C.POINTER(Tiling)],
@@ -708,6 +709,29 @@ def eigen_imag(U, psi0: WaveFunction, nef: int, max_steps: int = 512, conv: floa
         w._destroy = lib().qdx_wf_destroy
         out.append(w)
     return out, e, er, st
+
+
+def eigen_ac(U, psi0: WaveFunction, nsteps: int = 512, win: bool = True, tol: float = 1.0, width: int = -1,
+             diag: bool = False, H: Hamiltonian | None = None, max_ef: int = 256):
+    """Eigenstates from the autocorrelation (qdx_eigen_ac, ProgEigen method
+    "ac"): nsteps - 1 real-time steps of U, peaks of |FFT <psi(t)|psi0>|,
+    one function integrated out per peak.  Returns (wave functions, energies,
+    raw energies, autocorrelation)."""
+    hs = (C.c_void_p * max_ef)()
+    e = np.zeros(max_ef)
+    er = np.zeros(max_ef)
+    n = C.c_int32()
+    ac = np.zeros(2 * nsteps)
+    _chk(lib().qdx_eigen_ac(U.h, H.h if H is not None else None, psi0.h, int(nsteps), int(bool(win)), float(tol),
+                            int(width), int(bool(diag)), int(max_ef), hs, _dp(e), _dp(er), C.byref(n), _dp(ac)))
+    out = []
+    for h in hs[:n.value]:
+        w = WaveFunction.__new__(WaveFunction)
+        w.ctx, w.N, w.grid, w.S = psi0.ctx, list(psi0.N), psi0.grid, psi0.S
+        w.h = C.c_void_p(h)
+        w._destroy = lib().qdx_wf_destroy
+        out.append(w)
+    return out, e[:n.value].copy(), er[:n.value].copy(), ac[0::2] + 1j * ac[1::2]
 
 
 class GSPO(_Propagator):

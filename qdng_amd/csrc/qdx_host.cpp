@@ -1046,6 +1046,7 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
 // eigenstates by imaginary-time relaxation (ProgEigen::ImagTimeEF + DiagBasis)
 // and the short iterative Lanczos propagator (OSIL)
 
+extern "C++" {   // helpers with C++ types inside the extern "C" block
 namespace {
 
 // cyclic Jacobi rotations with accumulated eigenvectors; returns eigenvalues
@@ -1213,9 +1214,150 @@ struct WfHold {   // owns wave functions until released to the caller
         w.push_back(x);
         return x;
     }
+    void drop(qdx_wf *x) {   // destroy one held function now
+        for (auto &y : w)
+            if (y == x) {
+                qdx_wf_destroy(y);
+                y = nullptr;
+            }
+    }
+    void release(const std::vector<qdx_wf *> &out) {   // ownership passes to the caller
+        for (auto &y : w)
+            for (qdx_wf *x : out)
+                if (y == x) y = nullptr;
+    }
 };
 
+// ProgEigen::DiagBasis (ProgEigen.cpp:93-138): diagonalise H in the basis ef
+// (raw energies on the diagonal) and return the mixed functions
+// psi_i = sum_j V(j, i) ef_j with the eigenvalues in `energies`; the raw set
+// is destroyed.
+std::vector<qdx_wf *> diag_basis(qdx_ham *H, const qdx_wf *like, const std::vector<qdx_wf *> &ef,
+                                 const std::vector<double> &e_raw, WfHold &keep, WfHold &tmp, double *energies) {
+    const int nef = (int)ef.size();
+    qdx_wf *hbuf = tmp.make(like);
+    std::vector<double> S((size_t)nef * nef, 0.0);
+    for (int i = 0; i < nef; ++i) {
+        S[(size_t)i * nef + i] = e_raw[i];
+        for (int j = 0; j < i; ++j) {   // MatrixElement(bra = ef_i, ket = ef_j)
+            chk(qdx_ham_apply(H, hbuf, ef[j], 0));
+            std::complex<double> z;
+            ovl(ef[i], hbuf, z);
+            const double m = z.real();
+            S[(size_t)j * nef + i] = S[(size_t)i * nef + j] = m;
+        }
+    }
+    std::vector<double> ev, V;
+    sym_eigen_vectors(nef, S, ev, V);
+    std::vector<qdx_wf *> mixed;
+    for (int i = 0; i < nef; ++i) {
+        qdx_wf *m = keep.make(like);
+        chk(qdx_wf_copy(m, ef[0]));
+        launch_scale_axpy(m->ctx->stream, m->ptr(), nullptr, (long)m->S * m->n(), V[(size_t)0 * nef + i], 0, 0);
+        for (int j = 1; j < nef; ++j) chk(qdx_wf_axpy(m, V[(size_t)j * nef + i], 0.0, ef[j]));
+        mixed.push_back(m);
+        energies[i] = ev[i];
+    }
+    for (qdx_wf *x : ef) keep.drop(x);
+    return mixed;
+}
+
+// PeakFinder::Find(data) (tools/PeakFinder.cpp:83-126): the maximum of every
+// region above mean + tol * sigma; a region still open at the end counts
+// when the last value is not its maximum
+std::vector<int> peaks_regions(const std::vector<double> &d, double tol) {
+    const size_t n = d.size();
+    double mean = 0, sig = 0;
+    for (double v : d) mean += v;
+    mean /= double(n);
+    for (double v : d) sig += (v - mean) * (v - mean);
+    sig = std::sqrt(sig / double(n));
+    const double lim = mean + tol * sig;
+    std::vector<int> pk;
+    bool open = false;
+    double dmax = 0;
+    int imax = -1;
+    for (size_t i = 0; i < n; ++i) {
+        if (!open && d[i] > lim) {
+            open = true;
+            dmax = 0;
+            imax = -1;
+        }
+        if (open && d[i] < lim) {
+            open = false;
+            pk.push_back(imax);
+        }
+        if (open && d[i] > dmax) {
+            dmax = d[i];
+            imax = (int)i;
+        }
+    }
+    if (open && imax != -1 && d[n - 1] < dmax) pk.push_back(imax);
+    return pk;
+}
+
+// PeakFinder::Find(data, winsize) (tools/PeakFinder.cpp:131-170, Maximas
+// 177-190): every strict local maximum (d falls after rising) above
+// mean + tol * sigma of the window [i - w, i + w) clipped to the data
+std::vector<int> peaks_window(const std::vector<double> &d, double tol, int w) {
+    const int n = (int)d.size();
+    std::vector<int> maxi;
+    double delta = 0;
+    for (int i = 1; i < n; ++i) {
+        if (d[i] < d[i - 1] && delta > 0) maxi.push_back(i - 1);
+        delta = d[i] - d[i - 1];
+    }
+    std::vector<int> pk;
+    for (int m : maxi) {
+        const int lo = m < w ? 0 : m - w;
+        const int hi = m + w > n - 1 ? n - 1 : m + w;
+        double mean = 0, sig = 0;
+        for (int i = lo; i < hi; ++i) mean += d[i];
+        mean /= double(hi - lo);
+        for (int i = lo; i < hi; ++i) sig += (d[i] - mean) * (d[i] - mean);
+        sig = std::sqrt(sig / double(hi - lo));
+        if (d[m] > mean + tol * sig) pk.push_back(m);
+    }
+    return pk;
+}
+
+// forward DFT, unnormalised, e^{-2 pi i k t / n} (FFTW_FORWARD): radix 2 when
+// n is a power of two, else the direct sum
+void dft_forward(std::vector<std::complex<double>> &a) {
+    const size_t n = a.size();
+    if (n && (n & (n - 1)) == 0) {
+        for (size_t i = 1, j = 0; i < n; ++i) {   // bit reversal
+            size_t bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) std::swap(a[i], a[j]);
+        }
+        for (size_t len = 2; len <= n; len <<= 1) {
+            for (size_t i = 0; i < n; i += len)
+                for (size_t k = 0; k < len / 2; ++k) {
+                    const double ang = -2.0 * M_PI * double(k) / double(len);
+                    const std::complex<double> w(std::cos(ang), std::sin(ang));
+                    const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+                    a[i + k] = u + v;
+                    a[i + k + len / 2] = u - v;
+                }
+        }
+        return;
+    }
+    std::vector<std::complex<double>> out(n);
+    for (size_t k = 0; k < n; ++k) {
+        std::complex<double> acc = 0;
+        for (size_t t = 0; t < n; ++t) {
+            const double ang = -2.0 * M_PI * double((k * t) % n) / double(n);
+            acc += a[t] * std::complex<double>(std::cos(ang), std::sin(ang));
+        }
+        out[k] = acc;
+    }
+    a.swap(out);
+}
+
 }  // namespace
+}  // extern "C++"
 
 int qdx_eigen_imag(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv,
                    int32_t diag, qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps) {
@@ -1235,7 +1377,7 @@ int qdx_eigen_imag(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nef, i
         WfHold keep, tmp;
         std::vector<qdx_wf *> ef;
         std::vector<double> e_raw(nef);
-        qdx_wf *psi_old = tmp.make(p0), *hbuf = tmp.make(p0);
+        qdx_wf *psi_old = tmp.make(p0);
         for (int i = 0; i < nef; ++i) {   // ProgEigen.cpp:365-404
             qdx_wf *psi = keep.make(p0);
             chk(qdx_wf_copy(psi, p0));
@@ -1260,44 +1402,84 @@ int qdx_eigen_imag(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nef, i
             if (steps) steps[i] = s;
         }
         if (energies_raw) std::copy(e_raw.begin(), e_raw.end(), energies_raw);
-        if (diag) {   // DiagBasis (ProgEigen.cpp:93-138)
-            std::vector<double> S((size_t)nef * nef, 0.0);
-            for (int i = 0; i < nef; ++i) {
-                S[(size_t)i * nef + i] = e_raw[i];
-                for (int j = 0; j < i; ++j) {   // MatrixElement(bra = ef_i, ket = ef_j)
-                    chk(qdx_ham_apply(H, hbuf, ef[j], 0));
-                    std::complex<double> z;
-                    ovl(ef[i], hbuf, z);
-                    const double m = z.real();
-                    S[(size_t)j * nef + i] = S[(size_t)i * nef + j] = m;
-                }
-            }
-            std::vector<double> ev, V;
-            sym_eigen_vectors(nef, S, ev, V);
-            std::vector<qdx_wf *> mixed;
-            for (int i = 0; i < nef; ++i) {   // psi_i = sum_j S(j, i) ef_j
-                qdx_wf *m = keep.make(p0);
-                chk(qdx_wf_copy(m, ef[0]));
-                launch_scale_axpy(m->ctx->stream, m->ptr(), nullptr, (long)m->S * m->n(), V[(size_t)0 * nef + i], 0, 0);
-                for (int j = 1; j < nef; ++j) chk(qdx_wf_axpy(m, V[(size_t)j * nef + i], 0.0, ef[j]));
-                mixed.push_back(m);
-                energies[i] = ev[i];
-            }
-            for (qdx_wf *x : ef)   // the raw set is not returned
-                for (auto &w : keep.w)
-                    if (w == x) {
-                        qdx_wf_destroy(w);
-                        w = nullptr;
-                    }
-            ef = mixed;
-        } else {
-            std::copy(e_raw.begin(), e_raw.end(), energies);
-        }
+        if (diag) ef = diag_basis(H, p0, ef, e_raw, keep, tmp, energies);   // DiagBasis
+        else std::copy(e_raw.begin(), e_raw.end(), energies);
         chk(qdx_ctx_synchronize(U->ctx));
         for (int i = 0; i < nef; ++i) efs[i] = ef[i];
-        for (auto &w : keep.w)   // ownership passes to the caller
-            for (qdx_wf *x : ef)
-                if (w == x) w = nullptr;
+        keep.release(ef);
+    });
+}
+
+int qdx_eigen_ac(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nsteps, int32_t win, double tol,
+                 int32_t width, int32_t diag, int32_t max_ef, qdx_wf **efs, double *energies, double *energies_raw,
+                 int32_t *nef, double *autocorr) {
+    return guarded([&] {
+        if (!U || !psi0 || !efs || !energies || !nef) fail(QDX_ERR_PARAM, "NULL argument");
+        if (U->imag) fail(QDX_ERR_PARAM, "the autocorrelation method needs a real-time propagator");
+        if (nsteps < 2) fail(QDX_ERR_PARAM, "Not enough steps for autocorrelation");
+        if (max_ef < 1) fail(QDX_ERR_PARAM, "max_ef must be >= 1");
+        qdx_wf *p0 = const_cast<qdx_wf *>(psi0);
+        check_prop_wf(U, p0);
+        qdx_ham *H = Hin ? Hin : U->ham;
+        if (!H) fail(QDX_ERR_PARAM, "a Hamiltonian is required for the energies");
+        if (!(H->grid == p0->grid) || H->S != p0->S || H->ctx != p0->ctx)
+            fail(QDX_ERR_INCOMPAT, "Hamiltonian and wave function differ");
+        WfHold keep, tmp, tbuf;
+        // ProgEigen.cpp:436-470: propagate, keep every step, <psi(t)|psi0>
+        const int N = nsteps;
+        std::vector<std::complex<double>> ac(N);
+        ac[0] = 1.0;
+        std::vector<qdx_wf *> buf(N);
+        buf[0] = p0;
+        qdx_wf *psi = tmp.make(p0);
+        chk(qdx_wf_copy(psi, p0));
+        for (int i = 1; i < N; ++i) {
+            prop_step(U, psi, i - 1);
+            buf[i] = tbuf.make(p0);
+            chk(qdx_wf_copy(buf[i], psi));
+            ovl(psi, p0, ac[i]);
+            if (std::isnan(ac[i].real())) fail(QDX_ERR_OVERFLOW, "Correlation is not a number");
+        }
+        if (autocorr)
+            for (int i = 0; i < N; ++i) {
+                autocorr[2 * i] = ac[i].real();
+                autocorr[2 * i + 1] = ac[i].imag();
+            }
+        // Hann window (math_functions.h:108-113), forward FFT, |.|, peaks
+        if (win)
+            for (int n = 0; n < N; ++n) ac[n] *= 0.5 * (1. - std::cos(2 * M_PI * double(n) / double(N - 1)));
+        dft_forward(ac);
+        std::vector<double> power(N);
+        for (int i = 0; i < N; ++i) power[i] = std::abs(ac[i]);
+        const std::vector<int> pk = width > 0 ? peaks_window(power, tol, width) : peaks_regions(power, tol);
+        const int n_ef = (int)pk.size();
+        if (n_ef > max_ef) fail(QDX_ERR_PARAM, "more peaks than max_ef");
+        if (n_ef == 0) fail(QDX_ERR_PARAM, "no peaks found in the spectrum");
+        // ProgEigen.cpp:562-596: integrate every peak's function out
+        const double dt = U->dt;
+        const double dw = 2 * M_PI / (dt * double(N));
+        std::vector<qdx_wf *> ef;
+        std::vector<double> e_raw(n_ef);
+        for (int k = 0; k < n_ef; ++k) {
+            const int idx = pk[k];
+            const double e = idx < N / 2 ? double(idx) * dw : double(idx - N) * dw;
+            qdx_wf *f = keep.make(p0);
+            chk(qdx_wf_copy(f, p0));
+            for (int s = 1; s < N; ++s) {
+                const double ph = e * double(s) * dt;
+                chk(qdx_wf_axpy(f, std::cos(ph), std::sin(ph), buf[s]));
+            }
+            chk(qdx_wf_normalize(f));
+            chk(qdx_ham_expec(H, f, 0, &e_raw[k]));
+            ef.push_back(f);
+        }
+        if (energies_raw) std::copy(e_raw.begin(), e_raw.end(), energies_raw);
+        if (diag) ef = diag_basis(H, p0, ef, e_raw, keep, tmp, energies);
+        else std::copy(e_raw.begin(), e_raw.end(), energies);
+        chk(qdx_ctx_synchronize(U->ctx));
+        for (int i = 0; i < n_ef; ++i) efs[i] = ef[i];
+        *nef = n_ef;
+        keep.release(ef);
     });
 }
 

@@ -22,14 +22,19 @@ Programs:
          post-step `expeconly` filters written to `expec_post.dat`
          (src/FiltersList.cpp:135-330, values renormalised by <psi|psi>), the
          wave function written every `wcycle` steps with Propagation.meta.
-  eigen  ProgEigen method "imag" (src/ProgEigen.cpp:50-80, 150-260): `dt`,
-         `steps` (512), `Nef` (20), `conv`, `diag`, `fname` ("ef"), `ename`
-         ("energy.dat"), `dir`; eigenfunctions written as a sequence and the
-         energy table in QDng's format.
+  eigen  ProgEigen (src/ProgEigen.cpp:50-80, 150-260) method "imag" (`Nef`
+         (20), `conv`, `diag` on) or "ac" (`win` on, `tol` 1, `width`, `diag`
+         off; src/ProgEigen.cpp:414-600): `dt`, `steps` (512), `fname`
+         ("ef"), `ename` ("energy.dat"), `dir`; eigenfunctions written as a
+         sequence and the energy table in QDng's format.  Relative wave
+         function files are looked up in the output folder first (a later
+         program reads what an earlier one wrote).
 
-Not supported (QdxError with QDX_ERR_INCOMPAT): other operator classes, the
-"ac" eigen method and `LC` wave functions, pre-step filters and filter actions
-other than `expeconly`, and a propagation `start` other than 0.  The spectrum
+`LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
+twice, no coefficient).  Not supported (QdxError with QDX_ERR_INCOMPAT): other
+operator classes, reading a stored propagation (`read=`), pre-step filters
+and filter actions other than `expeconly`, and a propagation `start` other
+than 0.  The spectrum
 file (`spectrum=`) is not written.
 """
 from __future__ import annotations
@@ -43,7 +48,7 @@ import xml.etree.ElementTree as ET
 
 import numpy as np
 
-from . import (Arnoldi, Cheby, GSPO, Hamiltonian, QdxError, SIL, WaveFunction, eigen_imag,
+from . import (Arnoldi, Cheby, GSPO, Hamiltonian, QdxError, SIL, WaveFunction, eigen_ac, eigen_imag,
                write_propagation_meta)
 
 ERR_PARAM, ERR_INCOMPAT = -1, -2
@@ -248,20 +253,32 @@ def build_propagator(ctx, spec, psi, dt, imag=False):
     return U, H
 
 
-def load_wave_function(ctx, el, srcdir=None):
+def _resolve(path, base):
+    """A relative file name is looked up under `base` (the output folder)
+    first: programs read what earlier programs of the same input wrote."""
+    if not base or os.path.isabs(path):
+        return path
+    cand = os.path.join(base, path)
+    metas = [cand + ".meta", cand + ".wf"]
+    if "_" in os.path.basename(cand):
+        metas.append(cand.rsplit("_", 1)[0] + ".meta")
+    return cand if any(os.path.exists(m) for m in metas) else path
+
+
+def load_wave_function(ctx, el, srcdir=None, base=None):
     a = _attrs(el, srcdir)
     name = a.get("name")
     if name in (None, "WFGridCartesian"):
         if "file" not in a:
             raise _err("<wf> needs a file")
-        w = WaveFunction.load(ctx, a["file"])
+        w = WaveFunction.load(ctx, _resolve(a["file"], base))
     elif name == "Multistate":
         parts = {}
         for c in el:
             m = re.fullmatch(r"wf(\d+)", c.tag)
             if not m:
                 raise _err(f"state tag <{c.tag}> is not wf<i>")
-            parts[int(m.group(1))] = load_wave_function(ctx, c, srcdir)
+            parts[int(m.group(1))] = load_wave_function(ctx, c, srcdir, base)
         if not parts:
             raise _err("empty Multistate wave function")
         S = max(int(a.get("states", 0)), max(parts) + 1)
@@ -273,6 +290,21 @@ def load_wave_function(ctx, el, srcdir=None):
             if p.S != 1 or _grid_lists(p) != (N, lo, hi):
                 raise _err("the states of a Multistate wave function need one grid", ERR_INCOMPAT)
             w.upload(p.download(), s)
+    elif name == "LC":
+        # ChainLoader::LoadWFLC_ (src/ChainLoader.cpp:257-283) adds every
+        # member twice and applies no coefficient
+        w = None
+        for c in el:
+            x = load_wave_function(ctx, c, srcdir, base)
+            if w is None:
+                N, lo, hi = _grid_lists(x)
+                w = WaveFunction(ctx, N, lo, hi, x.S)
+                w.copy_from(x)
+            else:
+                w.axpy(1.0, x)
+            w.axpy(1.0, x)
+        if w is None:
+            raise _err("empty LC wave function")
     else:
         raise _err(f"wave function type '{name}' is not supported", ERR_INCOMPAT)
     if _bool(a.get("normalize")):
@@ -448,13 +480,16 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
 def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
     a = _attrs(el, srcdir)
     method = a.get("method", "imag")
-    if method != "imag":
+    if method not in ("imag", "ac"):
         raise _err(f"eigen method '{method}' is not supported", ERR_INCOMPAT)
+    if "read" in a:
+        raise _err("reading a stored propagation (read=) is not supported", ERR_INCOMPAT)
     if "dt" not in a:
         raise _err("No time step defined")
     dt = float(a["dt"])
     nef = int(a.get("Nef", 20))
-    diag = _bool(a.get("diag"), True)
+    # ProgEigen.cpp:187, 206: diag defaults on for imag, off for ac
+    diag = _bool(a.get("diag"), method == "imag")
     conv = float(a["conv"]) if "conv" in a else None
     max_steps = int(a.get("steps", 512))
     d = _outdir(a, outdir)
@@ -464,9 +499,16 @@ def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
     if len(props) != 1 or len(wfs) != 1:
         raise _err("<eigen> needs one <propagator> and one <wf>")
     spec = ld.propagator(props[0])
-    psi = load_wave_function(ctx, wfs[0], srcdir)
-    U, H = build_propagator(ctx, spec, psi, dt, imag=True)
-    efs, e, er, st = eigen_imag(U, psi, nef, max_steps, conv, diag, H)
+    psi = load_wave_function(ctx, wfs[0], srcdir, outdir)
+    st = ac = None
+    if method == "imag":
+        U, H = build_propagator(ctx, spec, psi, dt, imag=True)
+        efs, e, er, st = eigen_imag(U, psi, nef, max_steps, conv, diag, H)
+    else:   # ProgEigen::AutoCorrEF: real time, one function per spectral peak
+        U, H = build_propagator(ctx, spec, psi, dt)
+        efs, e, er, ac = eigen_ac(U, psi, max_steps, _bool(a.get("win"), True), float(a.get("tol", 1.0)),
+                                  int(a.get("width", -1)), diag, H)
+        nef = len(efs)
     fname = a.get("fname", "ef")
     for k, w in enumerate(efs):
         w.save(d + fname, counter=k)
@@ -480,7 +522,7 @@ def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
                 f.write("\t%.2f" % ((en[i] - en[i - 1]) * AU2WAVENUMBERS))
             f.write("\n")
     res = Result("eigen", d)
-    res.energies, res.energies_raw, res.steps, res.efs = e, er, st, efs
+    res.energies, res.energies_raw, res.steps, res.efs, res.autocorr = e, er, st, efs, ac
     if log:
         for i in range(nef):
             log("%d\t%.8f" % (i, en[i]))

@@ -21,6 +21,8 @@ FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qdng_f
 PROPA = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby", "TGPL-IR-Pulse-GSPO",
          "TGPLMS-Cheby", "TGPLMS-SIL"]
 EIGEN = ["TGE-Cheby", "TGE-GSPO", "TGE-SIL", "TGE-Arnoldi"]
+# checktests.sh:31-45, CheckTGE-ac-Energy
+TGE_AC_ENERGIES = ["0.003714", "0.011074", "0.018152", "0.024892", "0.031319", "0.037427"]
 # checktests.sh: CheckT-Expec TGPLMS-SIL (the report line equals TGPLMS-Cheby's)
 SIL_POST = ["-0.3059539159", "3.364408943", "1.661276151", "1.739725823", "0.001196532131", "0.007276466639",
             "0.001079240961", "0.07679400487"]
@@ -108,11 +110,14 @@ def test_loader_errors(tmp_path):
     with pytest.raises(Q.QdxError) as e:
         qdi._terms(ld.op(ET.fromstring('<F name="GridMomentum"/>')))
     assert e.value.code == -2
-    # the eigen method "ac" (second program of the TGE inputs) is rejected
-    # before anything touches a device
-    with pytest.raises(Q.QdxError) as e:
-        qdi.run(qdi_path("TGE-Cheby"), None, FIX, str(tmp_path), which=[1])
-    assert e.value.code == -2
+    # unknown eigen methods and stored propagations are rejected before
+    # anything touches a device
+    for attrs in ('method="lanczos"', 'method="ac" read="TGE-run"'):
+        f = tmp_path / "e.qdi"
+        f.write_text('<qdng><eigen dt="5" %s/></qdng>' % attrs)
+        with pytest.raises(Q.QdxError) as e:
+            qdi.run(str(f), None, FIX, str(tmp_path))
+        assert e.value.code == -2
     bad = tmp_path / "bad.qdi"
     bad.write_text("<root><propa/></root>")
     with pytest.raises(Q.QdxError):
@@ -168,7 +173,13 @@ def test_propagation_inputs(ctx, name, tmp_path):
 @pytest.mark.parametrize("name", EIGEN)
 def test_eigen_inputs(ctx, name, tmp_path):
     print("[gpu case] qdi", name, flush=True)
-    (res,) = qdi.run(qdi_path(name), ctx, FIX, str(tmp_path), which=[0])
+    res, res_ac = qdi.run(qdi_path(name), ctx, FIX, str(tmp_path))
+    # method "ac" on the LC of the first six functions the imag program wrote
+    rows = [l.split("\t") for l in open(os.path.join(res_ac.dir, "energy.dat")).read().splitlines()]
+    assert [r[0] for r in rows] == [str(i) for i in range(len(rows))] and len(rows) >= 6
+    for r, want in zip(rows, TGE_AC_ENERGIES):
+        assert r[1].startswith(want), (name, "ac", r[1], want)
+    assert os.path.exists(os.path.join(res_ac.dir, "ef_5.wf"))
     rows = [l.split("\t") for l in open(os.path.join(res.dir, "energy.dat")).read().splitlines()]
     assert [r[0] for r in rows] == [str(i) for i in range(10)]
     for r, want in zip(rows, TGE_ENERGIES):
