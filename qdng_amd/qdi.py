@@ -30,12 +30,13 @@ Programs:
          function files are looked up in the output folder first (a later
          program reads what an earlier one wrote).
 
-`LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
+With `spectrum=` both programs write the autocorrelation spectrum
+(Reporter::WriteSpectrum, src/Reporter.cpp:160-190; Hann window unless
+`win="false"`).  `LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
 twice, no coefficient).  Not supported (QdxError with QDX_ERR_INCOMPAT): other
 operator classes, reading a stored propagation (`read=`), pre-step filters
 and filter actions other than `expeconly`, and a propagation `start` other
-than 0.  The spectrum
-file (`spectrum=`) is not written.
+than 0.
 """
 from __future__ import annotations
 
@@ -360,6 +361,29 @@ def _g10(v):
     return "%.10g" % v
 
 
+def hann(c):
+    """FunctionGenerator::Hann (src/math/math_functions.h:108-113): the
+    window over the whole record, w(0) = w(n-1) = 0."""
+    c = np.asarray(c, dtype=complex)
+    n = np.arange(c.size)
+    return c * (0.5 * (1.0 - np.cos(2 * np.pi * n / (c.size - 1))))
+
+
+def write_spectrum(corr, dt, path):
+    """Reporter::WriteSpectrum (src/Reporter.cpp:160-190): angular frequency
+    (i dw below n/2, -(n-i) dw above), |F|, Re F, Im F of the unnormalised
+    forward transform divided by n/2 (integer), default stream format."""
+    c = np.asarray(corr, dtype=complex)
+    N = c.size
+    F = np.fft.fft(c)
+    dw = 2 * np.pi / (dt * N)
+    h = N // 2
+    with open(path, "w") as f:
+        for i in range(N):
+            w = i * dw if i < h else -(N - i) * dw
+            f.write("%g\t%g\t%g\t%g\n" % (w, abs(F[i]) / h, F[i].real / h, F[i].imag / h))
+
+
 # ---------------------------------------------------------------------------
 # programs
 
@@ -444,6 +468,7 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
         pf.write("time \t" + "".join(f.label + "\t" for f in filters) + "\n")
     psi.save(d + fname, counter=0)
     nwritten = 1
+    spec = np.zeros(steps, dtype=complex) if "spectrum" in a else None
     try:
         for i in range(1, steps + 1):
             # Reporter::Analyze on the state of clock step i - 1
@@ -455,6 +480,8 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
             if not math.isfinite(n):
                 raise _err("Norm is not a number")
             res.report.append(row)
+            if spec is not None:
+                spec[i - 1] = ac
             line = _report_line(*row)
             res.report_text.append(line)
             emit(line)
@@ -473,6 +500,8 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
             out.close()
         if pf:
             pf.close()
+    if spec is not None:   # Reporter::Finalize (src/Reporter.cpp:199-215)
+        write_spectrum(hann(spec) if _bool(a.get("win"), True) else spec, dt, d + a["spectrum"])
     res.psi = psi
     return res
 
@@ -506,9 +535,11 @@ def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
         efs, e, er, st = eigen_imag(U, psi, nef, max_steps, conv, diag, H)
     else:   # ProgEigen::AutoCorrEF: real time, one function per spectral peak
         U, H = build_propagator(ctx, spec, psi, dt)
-        efs, e, er, ac = eigen_ac(U, psi, max_steps, _bool(a.get("win"), True), float(a.get("tol", 1.0)),
-                                  int(a.get("width", -1)), diag, H)
+        win = _bool(a.get("win"), True)
+        efs, e, er, ac = eigen_ac(U, psi, max_steps, win, float(a.get("tol", 1.0)), int(a.get("width", -1)),
+                                  diag, H)
         nef = len(efs)
+        write_spectrum(hann(ac) if win else ac, dt, d + a.get("spectrum", "spectrum.dat"))
     fname = a.get("fname", "ef")
     for k, w in enumerate(efs):
         w.save(d + fname, counter=k)

@@ -153,6 +153,13 @@ def test_propagation_inputs(ctx, name, tmp_path):
     assert len(vals) == len(want)
     for v, w, pf in zip(vals, want, prefix):
         assert _digits_ok(v, w, pf), (name, v, w)
+    # spectrum.dat: one line per report line; the ground-state inputs put the
+    # largest peak on the energy (<psi(t)|psi0> ~ e^{+iEt})
+    spec = [[float(v) for v in l.split("\t")] for l in open(os.path.join(res.dir, "spectrum.dat"))]
+    assert len(spec) == len(res.report)
+    if name == "TGP-Expec":
+        top = max(spec, key=lambda r: r[1])
+        assert abs(top[0] - energy) < 2 * spec[1][0]   # spec[1][0] = dw
     # wave functions: the initial one and one per write cycle, Propagation.meta
     a = open(qdi_path(name)).read()
     steps = len(res.report)
@@ -180,6 +187,7 @@ def test_eigen_inputs(ctx, name, tmp_path):
     for r, want in zip(rows, TGE_AC_ENERGIES):
         assert r[1].startswith(want), (name, "ac", r[1], want)
     assert os.path.exists(os.path.join(res_ac.dir, "ef_5.wf"))
+    assert len(open(os.path.join(res_ac.dir, "spectrum.dat")).read().splitlines()) == 2048
     rows = [l.split("\t") for l in open(os.path.join(res.dir, "energy.dat")).read().splitlines()]
     assert [r[0] for r in rows] == [str(i) for i in range(10)]
     for r, want in zip(rows, TGE_ENERGIES):
@@ -189,3 +197,21 @@ def test_eigen_inputs(ctx, name, tmp_path):
         assert os.path.exists(os.path.join(res.dir, "ef_%d.wf" % k))
     ef = Q.WaveFunction.load(ctx, os.path.join(res.dir, "ef"), counter=3)
     assert ef.norm()[0] == pytest.approx(1.0, abs=1e-9)
+
+
+def test_spectrum_file(tmp_path):
+    # a pure tone on bin 3 of 16: amplitude N/(N/2) = 2 at w = 3 dw, zero elsewhere
+    N, dt = 16, 0.5
+    dw = 2 * np.pi / (dt * N)
+    c = np.exp(1j * 3 * dw * dt * np.arange(N))
+    qdi.write_spectrum(c, dt, str(tmp_path / "s.dat"))
+    rows = [[float(v) for v in l.split("\t")] for l in open(tmp_path / "s.dat").read().splitlines()]
+    assert len(rows) == N
+    assert [r[0] for r in rows] == pytest.approx([i * dw if i < N // 2 else -(N - i) * dw for i in range(N)],
+                                                  rel=1e-5)
+    assert rows[3][1] == pytest.approx(2.0, rel=1e-6) and rows[3][2] == pytest.approx(2.0, rel=1e-6)
+    assert max(abs(r[1]) for i, r in enumerate(rows) if i != 3) < 1e-9
+    # the Hann window zeroes both ends and is symmetric
+    w = qdi.hann(np.ones(9))
+    assert w[0] == 0 and abs(w[-1]) < 1e-15 and w[4] == pytest.approx(1.0)
+    assert np.allclose(w, w[::-1])
