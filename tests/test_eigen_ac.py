@@ -103,3 +103,24 @@ def test_device_ac_matches_oracle(ctx):
         Q.eigen_ac(U, w0, 1)
     with pytest.raises(Q.QdxError):
         Q.eigen_ac(U, w0, 2048, max_ef=3)
+
+
+# the other branches of the device driver: a record length that is not a
+# power of two (direct transform), no window, the windowed peak search
+@pytest.mark.gpu
+@pytest.mark.parametrize("nsteps,win,width,tol", [(1500, True, -1, 1.0), (1024, False, -1, 1.0),
+                                                  (2048, True, 6, 2.0), (1536, False, 10, 1.5)])
+def test_device_ac_variants_match_oracle(ctx, nsteps, win, width, tol):
+    import qdng_amd as Q
+    print("[gpu case] eigen ac", nsteps, win, width, tol, flush=True)
+    p, psi0 = tge_ac_input()
+    H, g = p.oracle_ham(), p.oracle_grid()
+    U = O.Cheby(H, 5.0)
+    e_o, pk, ac_o, _ = O.autocorr_ef(lambda ps, k: U.step(ps, k), lambda a, b: O.overlap(g, a, b),
+                                     lambda f: H.expec(f), psi0, nsteps, 5.0, win=win, tol=tol, width=width)
+    assert len(pk) >= 1
+    Ud = Q.Cheby(p.gpu_ham(ctx), 5.0)
+    efs, e, er, ac = Q.eigen_ac(Ud, p.gpu_wf(ctx, psi0), nsteps, win=win, tol=tol, width=width)
+    assert len(efs) == len(pk), (len(efs), pk)
+    assert np.allclose(ac, ac_o, rtol=0, atol=1e-10)
+    assert np.allclose(er, e_o, rtol=1e-10, atol=0)
