@@ -215,3 +215,14 @@ def test_spectrum_file(tmp_path):
     w = qdi.hann(np.ones(9))
     assert w[0] == 0 and abs(w[-1]) < 1e-15 and w[4] == pytest.approx(1.0)
     assert np.allclose(w, w[::-1])
+
+
+def test_report_line_format():
+    # Reporter::Analyze's line (src/Reporter.cpp:92-150) for the values of
+    # checktests.sh's TGP-Expec and TGPMS-Cheby lines, compared token by token
+    got = qdi._report_line(999, 19980.0, 1.0, [], 0.00371015, complex(0.29455816, -0.95562855))
+    assert got.split() == "999    19980.0         1.000000        0.00371015      0.29455816,-0.95562855i".split()
+    got = qdi._report_line(999, 9990.0, 1.0, [0.5, 0.5], 0.08955451, complex(0.43841526, -0.36254206))
+    want = "999    9990.0          1.000000        0.500000        0.500000        0.08955451      0.43841526,-0.36254206i"
+    assert got.split() == want.split()
+    assert got.startswith("999\t9990.0\t\t1.000000\t0.500000")
