@@ -19,7 +19,8 @@ Programs:
          `wcycle` (10), `dir`, `fname` ("WF"), `nfile`; the report table of
          Reporter::Analyze (src/Reporter.cpp:63-158, taken before each step:
          step, time, population, populations per state, energy, <psi0|psi>),
-         post-step `expeconly` filters written to `expec_post.dat`
+         pre- and post-step filters written to `expec_pre.dat` and
+         `expec_post.dat`
          (src/FiltersList.cpp:135-330, values renormalised by <psi|psi>), the
          wave function written every `wcycle` steps with Propagation.meta.
   eigen  ProgEigen (src/ProgEigen.cpp:50-80, 150-260) method "imag" (`Nef`
@@ -34,9 +35,9 @@ With `spectrum=` both programs write the autocorrelation spectrum
 (Reporter::WriteSpectrum, src/Reporter.cpp:160-190; Hann window unless
 `win="false"`).  `LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
 twice, no coefficient).  Not supported (QdxError with QDX_ERR_INCOMPAT): other
-operator classes, reading a stored propagation (`read=`), pre-step filters
-and filter actions other than `expeconly`, and a propagation `start` other
-than 0.
+operator classes, reading a stored propagation (`read=`), filter actions
+other than `expeconly` and `normalize`, and a propagation `start` other than
+0.
 """
 from __future__ import annotations
 
@@ -361,6 +362,59 @@ def _g10(v):
     return "%.10g" % v
 
 
+class FilterList:
+    """FiltersList (src/FiltersList.cpp:135-330) for one <filterpre> or
+    <filterpost> section: actions `expeconly` (write Re<psi|O|psi>/<psi|psi>)
+    and `normalize`, in order; one line per call into the section's file
+    (`fname`, default expec_pre.dat / expec_post.dat): the time, then the
+    values, tab-terminated, %.10g.  Values are divided by <psi|psi> unless
+    the section sets renorm="false"."""
+
+    def __init__(self, ctx, ld, sections, psi, d, default_name, srcdir):
+        self.items = []
+        self.f = None
+        if not sections:
+            return
+        if len(sections) > 1:
+            raise _err("one filter section of each kind")
+        sa = _attrs(sections[0], srcdir)
+        # renormalised unless the section says renorm="false" (FiltersList.cpp:181-182)
+        self.renorm = _bool(sa.get("renorm"), True)
+        for f in sections[0]:
+            if f.tag == "normalize":
+                self.items.append(None)
+                continue
+            if f.tag != "expeconly":
+                raise _err(f"filter action <{f.tag}> is not supported", ERR_INCOMPAT)
+            op = ld.op(f)
+            fa = _attrs(f, srcdir)
+            self.items.append(_Filter(ctx, op, psi, fa.get("header", "O" + op.cls)))
+        if not self.items:
+            raise _err("Empty filter list provided")
+        if any(x is not None for x in self.items):
+            self.f = open(d + sa.get("fname", default_name), "w")
+            self.f.write("time \t" + "".join(x.label + "\t" for x in self.items if x is not None) + "\n")
+
+    def apply(self, psi, step, time):
+        if not self.items:
+            return None
+        vals = []
+        for x in self.items:
+            if x is None:
+                psi.normalize()
+            else:
+                v = x.value(psi, step)
+                vals.append(v / psi.norm()[0] ** 2 if self.renorm else v)
+        if self.f:
+            self.f.write(_g10(time) + "\t" + "".join(_g10(v) + "\t" for v in vals) + "\n")
+        return vals
+
+    def close(self):
+        if self.f:
+            self.f.close()
+            self.f = None
+
+
 def hann(c):
     """FunctionGenerator::Hann (src/math/math_functions.h:108-113): the
     window over the whole record, w(0) = w(n-1) = 0."""
@@ -441,36 +495,26 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
     wfs = [c for c in el if c.tag == "wf"]
     if len(props) != 1 or len(wfs) != 1:
         raise _err("<propa> needs one <propagator> and one <wf>")
-    if any(c.tag == "filterpre" for c in el):
-        raise _err("pre-step filters are not supported", ERR_INCOMPAT)
     spec = ld.propagator(props[0])
     psi = load_wave_function(ctx, wfs[0], srcdir)
     U, H = build_propagator(ctx, spec, psi, dt)
     N, lo, hi = _grid_lists(psi)
     psi0 = WaveFunction(ctx, N, lo, hi, psi.S)
     psi0.copy_from(psi)
-    filters = []
-    for sec in (c for c in el if c.tag == "filterpost"):
-        for f in sec:
-            if f.tag != "expeconly":
-                raise _err(f"filter action <{f.tag}> is not supported", ERR_INCOMPAT)
-            op = ld.op(f)
-            fa = _attrs(f, srcdir)
-            filters.append(_Filter(ctx, op, psi, fa.get("header", "O" + op.cls)))
+    pre = FilterList(ctx, ld, [c for c in el if c.tag == "filterpre"], psi, d, "expec_pre.dat", srcdir)
+    post = FilterList(ctx, ld, [c for c in el if c.tag == "filterpost"], psi, d, "expec_post.dat", srcdir)
     res = Result("propa", d)
     out = open(d + a["nfile"], "w") if a.get("nfile") else None
     header = "Step\tTime[au]\tPopulation" + "".join("\t\tPop_%d" % s for s in range(psi.S) if psi.S > 1) + \
         "\t\tEnergy[au]\t<Psi0|PsiT>"
     emit = (lambda s: out.write(s + "\n")) if out else (log or (lambda s: None))
     emit(header)
-    pf = open(d + "expec_post.dat", "w") if filters else None
-    if pf:
-        pf.write("time \t" + "".join(f.label + "\t" for f in filters) + "\n")
     psi.save(d + fname, counter=0)
     nwritten = 1
     spec = np.zeros(steps, dtype=complex) if "spectrum" in a else None
     try:
         for i in range(1, steps + 1):
+            pre.apply(psi, i - 1, (i - 1) * dt)
             # Reporter::Analyze on the state of clock step i - 1
             n, per = psi.norm()
             pops = [float(x) for x in per] if psi.S > 1 else []   # squared norms per state
@@ -486,11 +530,9 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
             res.report_text.append(line)
             emit(line)
             U.apply(psi, i - 1)
-            if filters:
-                n2 = psi.norm()[0] ** 2
-                vals = [f.value(psi, i) / n2 for f in filters]
+            vals = post.apply(psi, i, i * dt)
+            if vals is not None:
                 res.expec_post.append((i * dt, vals))
-                pf.write(_g10(i * dt) + "\t" + "".join(_g10(v) + "\t" for v in vals) + "\n")
             if i % wcycle == 0:
                 psi.save(d + fname, counter=nwritten)
                 nwritten += 1
@@ -498,8 +540,8 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
     finally:
         if out:
             out.close()
-        if pf:
-            pf.close()
+        pre.close()
+        post.close()
     if spec is not None:   # Reporter::Finalize (src/Reporter.cpp:199-215)
         write_spectrum(hann(spec) if _bool(a.get("win"), True) else spec, dt, d + a["spectrum"])
     res.psi = psi

@@ -226,3 +226,31 @@ def test_report_line_format():
     want = "999    9990.0          1.000000        0.500000        0.500000        0.08955451      0.43841526,-0.36254206i"
     assert got.split() == want.split()
     assert got.startswith("999\t9990.0\t\t1.000000\t0.500000")
+
+
+@pytest.mark.gpu
+def test_pre_filters_normalize_and_renorm(ctx, tmp_path):
+    # TGP-Expec shortened to 20 steps, with a pre-step section (position,
+    # then normalize) and a post section without renormalisation
+    from oracle import oracle as O
+    print("[gpu case] qdi filters", flush=True)
+    src = open(qdi_path("TGP-Expec")).read()
+    src = src.replace('steps="1000" wcycle="1000"', 'steps="20" wcycle="10"')
+    src = src.replace("<filterpost>", '<filterpre fname="pre.dat"><expeconly name="GridPosition"/><normalize/>'
+                                      '</filterpre><filterpost renorm="false">')
+    f = tmp_path / "in.qdi"
+    f.write_text(src)
+    (res,) = qdi.run(str(f), ctx, FIX, str(tmp_path))
+    pre = [l.split("\t") for l in open(os.path.join(res.dir, "pre.dat")).read().splitlines()]
+    assert pre[0] == ["time ", "OGridPosition", ""] and len(pre) == 21
+    assert [float(r[0]) for r in pre[1:3]] == [0.0, 20.0]
+    # the first pre-step value is <x> of the normalised initial state
+    g = O.Grid([128], [1.8897], [6.1104])
+    psi0 = np.fromfile(os.path.join(FIX, "GridPot0_ef_0.wf"), dtype="<c16")
+    psi0 = psi0 / np.sqrt(O.norm2(g, psi0))
+    assert float(pre[1][1]) == pytest.approx(O.expec_position(g, psi0), rel=1e-9)
+    post = open(os.path.join(res.dir, "expec_post.dat")).read().splitlines()
+    assert len(post) == 21 and len(res.expec_post) == 20
+    # files at steps 10 and 20 besides the initial one
+    assert os.path.exists(os.path.join(res.dir, "WF_2.wf"))
+    assert Q.read_meta(os.path.join(res.dir, "Propagation.meta"))["Nt"] == "20"
