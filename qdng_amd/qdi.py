@@ -496,7 +496,7 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
     if len(props) != 1 or len(wfs) != 1:
         raise _err("<propa> needs one <propagator> and one <wf>")
     spec = ld.propagator(props[0])
-    psi = load_wave_function(ctx, wfs[0], srcdir)
+    psi = load_wave_function(ctx, wfs[0], srcdir, outdir)
     U, H = build_propagator(ctx, spec, psi, dt)
     N, lo, hi = _grid_lists(psi)
     psi0 = WaveFunction(ctx, N, lo, hi, psi.S)
