@@ -36,8 +36,9 @@ With `spectrum=` both programs write the autocorrelation spectrum
 `win="false"`).  `LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
 twice, no coefficient).  Not supported (QdxError with QDX_ERR_INCOMPAT): other
 operator classes, reading a stored propagation (`read=`), filter actions
-other than `expeconly` and `normalize`, and a propagation `start` other than
-0.
+other than `expeconly`, `normalize` and `apply` of a GridKick, and a
+propagation `start` other than 0.  Init filters (`filterinit`) run once
+before the initial state is kept for the autocorrelation.
 """
 from __future__ import annotations
 
@@ -147,6 +148,10 @@ class Loader:
             o = Term("dip", name, file=self._file(a, name), laser=a["laser"], scale=float(a.get("scale", 1.0)))
         elif name in ("GridMomentum", "GridPosition"):
             o = Term("mom" if name == "GridMomentum" else "pos", name, dim=int(a.get("dim", 0)))
+        elif name == "GridKick":
+            if "k" not in a:
+                raise _err("GridKick needs a momentum vector k")
+            o = Term("kick", name, k=[float(v) for v in a["k"].split(",")], scale=float(a.get("scale", 1.0)))
         elif name == "Projection":
             o = Term("proj", name, files=self._file(a, name, "files"), start=int(a.get("start", 0)),
                      num=int(a.get("num", 1)), step=int(a.get("step", 1)))
@@ -362,6 +367,33 @@ def _g10(v):
     return "%.10g" % v
 
 
+class _Kick:
+    """OGridKick (src/qdlib/OGridKick.cpp): psi(x) *= exp(i sum_n k_n scale
+    x_n), x_n = xmin_n + i dx_n; applied on the host (an initialisation
+    filter, not a per-step operator)."""
+
+    def __init__(self, op, psi):
+        N, lo, hi = _grid_lists(psi)
+        k = op.params["k"]
+        if len(k) > len(N):
+            raise _err("OGridKick: Number of dims doesn't match WF")
+        if psi.S != 1:
+            raise _err("GridKick acts on one state", ERR_INCOMPAT)
+        ph = np.zeros(N[::-1])
+        for n in range(len(N)):
+            kn = (k[n] if n < len(k) else 0.0) * op.params["scale"]
+            if kn != 0:
+                dx = (hi[n] - lo[n]) / (N[n] - 1)
+                x = (lo[n] + np.arange(N[n]) * dx) * kn
+                shape = [1] * len(N)
+                shape[len(N) - 1 - n] = N[n]   # dimension 0 fastest
+                ph = ph + x.reshape(shape)
+        self.f = np.exp(1j * ph).ravel()
+
+    def apply(self, psi):
+        psi.upload(psi.download() * self.f)
+
+
 class FilterList:
     """FiltersList (src/FiltersList.cpp:135-330) for one <filterpre> or
     <filterpost> section: actions `expeconly` (write Re<psi|O|psi>/<psi|psi>)
@@ -384,6 +416,12 @@ class FilterList:
             if f.tag == "normalize":
                 self.items.append(None)
                 continue
+            if f.tag == "apply":
+                op = ld.op(f)
+                if not (isinstance(op, Term) and op.kind == "kick"):
+                    raise _err(f"apply: {op.cls} is not supported (GridKick is)", ERR_INCOMPAT)
+                self.items.append(_Kick(op, psi))
+                continue
             if f.tag != "expeconly":
                 raise _err(f"filter action <{f.tag}> is not supported", ERR_INCOMPAT)
             op = ld.op(f)
@@ -391,9 +429,10 @@ class FilterList:
             self.items.append(_Filter(ctx, op, psi, fa.get("header", "O" + op.cls)))
         if not self.items:
             raise _err("Empty filter list provided")
-        if any(x is not None for x in self.items):
+        shown = [x for x in self.items if isinstance(x, _Filter)]
+        if shown:
             self.f = open(d + sa.get("fname", default_name), "w")
-            self.f.write("time \t" + "".join(x.label + "\t" for x in self.items if x is not None) + "\n")
+            self.f.write("time \t" + "".join(x.label + "\t" for x in shown) + "\n")
 
     def apply(self, psi, step, time):
         if not self.items:
@@ -402,6 +441,8 @@ class FilterList:
         for x in self.items:
             if x is None:
                 psi.normalize()
+            elif isinstance(x, _Kick):
+                x.apply(psi)
             else:
                 v = x.value(psi, step)
                 vals.append(v / psi.norm()[0] ** 2 if self.renorm else v)
@@ -497,6 +538,10 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
         raise _err("<propa> needs one <propagator> and one <wf>")
     spec = ld.propagator(props[0])
     psi = load_wave_function(ctx, wfs[0], srcdir, outdir)
+    # initialisation filters, once, before psi0 is taken (ProgPropa.cpp:216-229)
+    init = FilterList(ctx, ld, [c for c in el if c.tag == "filterinit"], psi, d, "expec_init.dat", srcdir)
+    init.apply(psi, 0, 0.0)
+    init.close()
     U, H = build_propagator(ctx, spec, psi, dt)
     N, lo, hi = _grid_lists(psi)
     psi0 = WaveFunction(ctx, N, lo, hi, psi.S)

@@ -20,7 +20,7 @@ FILES = ["GridPot0.meta", "GridPot0.op", "GridPot1.meta", "GridPot1.op", "GridDi
 # the program inputs of the grid regression tests (XML, @srcdir@ names the
 # folder of the data files), run end to end by tests/test_qdi.py
 INPUTS = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby", "TGPL-IR-Pulse-GSPO",
-          "TGPLMS-Cheby", "TGPLMS-SIL", "TGE-Cheby", "TGE-GSPO", "TGE-SIL", "TGE-Arnoldi"]
+          "TGPLMS-Cheby", "TGPLMS-SIL", "TGP-Kick", "TGE-Cheby", "TGE-GSPO", "TGE-SIL", "TGE-Arnoldi"]
 
 
 def main(root):

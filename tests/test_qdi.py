@@ -19,7 +19,12 @@ from qdng_amd import qdi
 
 FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qdng_files")
 PROPA = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby", "TGPL-IR-Pulse-GSPO",
-         "TGPLMS-Cheby", "TGPLMS-SIL"]
+         "TGPLMS-Cheby", "TGPLMS-SIL", "TGP-Kick"]
+# checktests.sh: CheckT TGP-Kick and CheckT-Expec TGP-Kick (grep prefixes)
+KICK = {"report": {"step": 999, "time": 19980.0, "population": 1.0, "pops": [], "energy": 0.00399372,
+                   "autocorr": [0.29628786, -0.88631099]},
+        "expec_post": {"time": 20000.0, "prefix": [True] * 5,
+                       "values": ["-0.81968278", "3.23866603", "0.99999811", "0.00195512887", "0.00203859515"]}}
 EIGEN = ["TGE-Cheby", "TGE-GSPO", "TGE-SIL", "TGE-Arnoldi"]
 # checktests.sh:31-45, CheckTGE-ac-Energy
 TGE_AC_ENERGIES = ["0.003714", "0.011074", "0.018152", "0.024892", "0.031319", "0.037427"]
@@ -91,7 +96,7 @@ def test_gspo_split_operators_and_their_sum():
 def test_loader_errors(tmp_path):
     import xml.etree.ElementTree as ET
     ld = qdi.Loader()
-    for xml, code in [('<A ref="nope"/>', -1), ('<A name="GridKick"/>', -2), ('<A/>', -1),
+    for xml, code in [('<A ref="nope"/>', -1), ('<A name="Gobbler"/>', -2), ('<A name="GridKick"/>', -1), ('<A/>', -1),
                       ('<A name="GridPotential"/>', -1), ('<A name="GridDipole" file="x"/>', -1),
                       ('<A name="Multistate"><x0 name="GridNablaSq" mass="1"/></A>', -1),
                       ('<A name="Sum"/>', -1), ('<A name="GridPotential" file="f" setzero="maybe"/>', -1)]:
@@ -132,7 +137,8 @@ def test_loader_errors(tmp_path):
 def test_propagation_inputs(ctx, name, tmp_path):
     print("[gpu case] qdi", name, flush=True)
     (res,) = qdi.run(qdi_path(name), ctx, FIX, str(tmp_path))
-    gold = cases.kat_data()[1]["golden"]["TGPLMS-Cheby" if name == "TGPLMS-SIL" else name]
+    gold = KICK if name == "TGP-Kick" else \
+        cases.kat_data()[1]["golden"]["TGPLMS-Cheby" if name == "TGPLMS-SIL" else name]
     rep = gold["report"]
     step, time, n2, pops, energy, ac = res.report[-1]
     assert (step, time) == (rep["step"], rep["time"])
