@@ -50,6 +50,7 @@ def lib():
         L.orc_last_error.restype = C.c_char_p
         sig = {
             "orc_set_threads": [C.c_int],
+            "orc_plant_mistake": [C.c_int],
             "orc_fft": [C.c_int, i32p, dp, C.c_int],
             "orc_bessel_j": [C.c_int, C.c_double, dp, ip],
             "orc_bessel_i_scaled": [C.c_int, C.c_double, dp, ip],
@@ -108,6 +109,12 @@ def _c128(a) -> np.ndarray:
 
 def set_threads(n: int) -> int:
     return lib().orc_set_threads(int(n))
+
+
+def plant_mistake(code: int) -> None:
+    """Test hook: make the oracle commit a known mistake (1: transposed
+    per-point eigenvectors in the coupled GSPO split); 0 restores it."""
+    lib().orc_plant_mistake(int(code))
 
 
 def reset() -> None:

@@ -332,6 +332,9 @@ void sym_eigen(int n, std::vector<double> a, std::vector<double> &ev, std::vecto
     if (vec) *vec = V;   /* columns are eigenvectors */
 }
 
+/* planted-mistake switch of the tests (0: the restatement as written) */
+int g_plant = 0;
+
 /* laser value at clock step (Laser.cpp:125-128) */
 double laser_at(const Op &o, int step) {
     if (step < 0 || step >= (int)o.laser.size()) throw OracleError("clock step outside the laser field");
@@ -890,7 +893,9 @@ void split_apply(const Op &o, int step, cplx c, cplx *psi, int S) {
         }
         return;
     }
-    /* per grid point diagonalisation */
+    /* per grid point diagonalisation; g_plant == 1 swaps X and X^T (a
+     * planted mistake for the tests, never set otherwise) */
+    const bool swapx = g_plant == 1;
     const int M = o.S;
     std::vector<double> W(M * M), ev, X;
     std::vector<cplx> a(M), b(M);
@@ -904,12 +909,12 @@ void split_apply(const Op &o, int step, cplx c, cplx *psi, int S) {
         for (int s = 0; s < M; ++s) a[s] = psi[s * n + i];
         for (int r = 0; r < M; ++r) { /* forward: X^T psi */
             cplx t = 0;
-            for (int s = 0; s < M; ++s) t += X[s * M + r] * a[s];
+            for (int s = 0; s < M; ++s) t += (swapx ? X[r * M + s] : X[s * M + r]) * a[s];
             b[r] = t * std::exp(c * ev[r]);
         }
         for (int r = 0; r < M; ++r) { /* backward: X psi */
             cplx t = 0;
-            for (int s = 0; s < M; ++s) t += X[r * M + s] * b[s];
+            for (int s = 0; s < M; ++s) t += (swapx ? X[s * M + r] : X[r * M + s]) * b[s];
             psi[r * n + i] = t;
         }
     }
@@ -933,6 +938,14 @@ void gspo_step(const GSPO &p, int step, cplx *psi) {
 extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
+
+/* Test hook: plant a known mistake in the restatement so that a test can
+ * show that its check catches it.  1: transposed per-point eigenvectors in
+ * the coupled GSPO split.  0 restores the restatement. */
+int orc_plant_mistake(int code) {
+    g_plant = code;
+    return 0;
+}
 
 int orc_set_threads(int n) {
 #ifdef _OPENMP

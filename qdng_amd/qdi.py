@@ -400,9 +400,12 @@ class FilterList:
     and `normalize`, in order; one line per call into the section's file
     (`fname`, default expec_pre.dat / expec_post.dat): the time, then the
     values, tab-terminated, %.10g.  Values are divided by <psi|psi> unless
-    the section sets renorm="false"."""
+    the section sets renorm="false".  Init sections (`filterinit`) start
+    unrenormalised: QDng's FiltersList defaults to false
+    (src/FiltersList.cpp:11) and only the pre and post sections switch it on
+    (src/ProgPropa.cpp:260, 276); a `renorm` attribute overrides either."""
 
-    def __init__(self, ctx, ld, sections, psi, d, default_name, srcdir):
+    def __init__(self, ctx, ld, sections, psi, d, default_name, srcdir, renorm_default=True):
         self.items = []
         self.f = None
         if not sections:
@@ -410,8 +413,8 @@ class FilterList:
         if len(sections) > 1:
             raise _err("one filter section of each kind")
         sa = _attrs(sections[0], srcdir)
-        # renormalised unless the section says renorm="false" (FiltersList.cpp:181-182)
-        self.renorm = _bool(sa.get("renorm"), True)
+        # the section's renorm attribute overrides the default (FiltersList.cpp:181-182)
+        self.renorm = _bool(sa.get("renorm"), renorm_default)
         for f in sections[0]:
             if f.tag == "normalize":
                 self.items.append(None)
@@ -539,7 +542,8 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
     spec = ld.propagator(props[0])
     psi = load_wave_function(ctx, wfs[0], srcdir, outdir)
     # initialisation filters, once, before psi0 is taken (ProgPropa.cpp:216-229)
-    init = FilterList(ctx, ld, [c for c in el if c.tag == "filterinit"], psi, d, "expec_init.dat", srcdir)
+    init = FilterList(ctx, ld, [c for c in el if c.tag == "filterinit"], psi, d, "expec_init.dat", srcdir,
+                      renorm_default=False)
     init.apply(psi, 0, 0.0)
     init.close()
     U, H = build_propagator(ctx, spec, psi, dt)

@@ -295,3 +295,69 @@ def test_planted_mistakes():
     assert abs(run(pot_scale=1.01) - want) > 1e-4
     dx_wrong_xmax = p.xmin[0] + (p.xmax[0] - p.xmin[0]) * p.N[0] / (p.N[0] - 1)
     assert abs(run(grid_xmax=dx_wrong_xmax) - want) > 1e-4
+
+
+# ---------------------------------------------------------------------------
+# coupled-state GSPO split (ODMultistate.cpp:109-159, TransformMSD.cpp:31-101)
+# against the dense Strang product of the full 2N x 2N matrices
+
+def _dense_kinetic(p, mass):
+    # F^-1 diag(k^2 / 2m) F on a periodic box of N dx (numpy's DFT matrices)
+    N = p.N[0]
+    dx = p.dx(0)
+    k = 2 * np.pi * np.fft.fftfreq(N, d=dx)
+    F = np.fft.fft(np.eye(N), axis=0)
+    return np.linalg.inv(F) @ np.diag(k ** 2 / (2 * mass)) @ F
+
+
+def _coupled_problem(N=24):
+    p = cases.Problem([N], [-3.0], [3.0], nstates=2)
+    x = p.axes()[0]
+    p.kinetic(0, 700.0).kinetic(1, 700.0)
+    p.potential(0, 0, 0.05 * x ** 2)
+    p.potential(1, 1, 0.1 + 0.03 * (x - 0.5) ** 2)
+    p.potential(0, 1, 0.02 * np.exp(-x ** 2) + 0.01)
+    rng = np.random.default_rng(4)
+    psi = rng.standard_normal(2 * N) + 1j * rng.standard_normal(2 * N)
+    return p, x, psi / np.linalg.norm(psi)
+
+
+def test_gspo_coupled_split_vs_dense_strang():
+    N, dt = 24, 3.0
+    p, x, psi0 = _coupled_problem(N)
+    K = _dense_kinetic(p, 700.0)
+    T = np.block([[K, np.zeros((N, N))], [np.zeros((N, N)), K]])
+    V = np.block([[np.diag(0.05 * x ** 2), np.diag(0.02 * np.exp(-x ** 2) + 0.01)],
+                  [np.diag(0.02 * np.exp(-x ** 2) + 0.01), np.diag(0.1 + 0.03 * (x - 0.5) ** 2)]])
+    half = scipy.linalg.expm(-0.5j * dt * V)
+    ref = half @ scipy.linalg.expm(-1j * dt * T) @ half @ psi0
+    # the Strang product differs from the exact propagator at O(dt^3):
+    # the check below separates it from exp(-i H dt) as well
+    exact = scipy.linalg.expm(-1j * dt * (T + V)) @ psi0
+    assert np.linalg.norm(exact - ref) > 1e-6
+
+    def run(plant):
+        O.plant_mistake(plant)
+        try:
+            A = p.oracle_split_op(["kin"])
+            B = p.oracle_split_op(["pot"])
+            G = O.GSPO([A, B], dt, nstates=2)
+            psi = psi0.copy()
+            G.step(psi)
+            return psi
+        finally:
+            O.plant_mistake(0)
+
+    got = run(0)
+    assert np.linalg.norm(got - ref) < 1e-12, np.linalg.norm(got - ref)
+    # the planted mistake (X^T e^{c Lambda} X instead of X e^{c Lambda} X^T)
+    # fails the same check: with distinct diagonals X is not symmetric
+    bad = run(1)
+    assert np.linalg.norm(bad - ref) > 1e-3
+    # and in imaginary time (exponent -dt): exp(-V dt/2) exp(-T dt) exp(-V dt/2)
+    G = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], dt, nstates=2, imag=True)
+    psi = psi0.copy()
+    G.step(psi)
+    h2 = scipy.linalg.expm(-0.5 * dt * V)
+    ref_i = h2 @ scipy.linalg.expm(-dt * T) @ h2 @ psi0
+    assert np.linalg.norm(psi - ref_i) < 1e-12 * np.linalg.norm(ref_i)

@@ -1,13 +1,16 @@
 """QDng program inputs (XML .qdi) through qdng_amd.qdi.
 
-The inputs are QDng's own grid regression tests (tests/*.qdi.in, copied
-unchanged with their data files by tests/golden/make_file_fixtures.py).  CPU
+The inputs are the programs of QDng's grid regression cases, generated from
+their parameters by tests/golden/qdi_programs.py; the data files they read
+(potentials, dipoles, lasers, wave functions) are in tests/golden/qdng_files.  CPU
 tests check the operator graphs the loader builds (key/ref sharing, matrix
 elements, @srcdir@) and its error paths; the `gpu` tests run the programs end
 to end and compare the last report line and the last expec_post.dat line with
 the values tests/checktests.sh greps for, and the eigen programs' energy.dat
 with CheckTGE-im-Energy (checktests.sh:4-29)."""
 import os
+import sys
+import tempfile
 
 import numpy as np
 import pytest
@@ -17,7 +20,12 @@ from test_oracle import _digits_ok
 import qdng_amd as Q
 from qdng_amd import qdi
 
-FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "qdng_files")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIX = os.path.join(GOLDEN, "qdng_files")
+sys.path.insert(0, GOLDEN)
+import qdi_programs  # noqa: E402
+
+_PROGRAMS = qdi_programs.write_all(tempfile.mkdtemp(prefix="qdi_programs_"))
 PROPA = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby", "TGPL-IR-Pulse-GSPO",
          "TGPLMS-Cheby", "TGPLMS-SIL", "TGP-Kick"]
 # checktests.sh: CheckT TGP-Kick and CheckT-Expec TGP-Kick (grep prefixes)
@@ -36,7 +44,7 @@ TGE_ENERGIES = ["0.0037100", "0.0110668", "0.0181411", "0.0249017", "0.0313281",
 
 
 def qdi_path(name):
-    return os.path.join(FIX, name + ".qdi.in")
+    return _PROGRAMS[name]
 
 
 def prog(name, k=0):
@@ -260,3 +268,38 @@ def test_pre_filters_normalize_and_renorm(ctx, tmp_path):
     # files at steps 10 and 20 besides the initial one
     assert os.path.exists(os.path.join(res.dir, "WF_2.wf"))
     assert Q.read_meta(os.path.join(res.dir, "Propagation.meta"))["Nt"] == "20"
+
+
+def test_filter_renorm_defaults(tmp_path, monkeypatch):
+    # FiltersList starts with renorm off (src/FiltersList.cpp:11); ProgPropa
+    # switches it on for the pre and post sections only (ProgPropa.cpp:260,
+    # 276), so an expeconly in <filterinit> reports <psi|O|psi> undivided
+    # for an unnormalised psi; the section's renorm attribute overrides both
+    import xml.etree.ElementTree as ET
+
+    class FakeFilter:
+        def __init__(self, ctx, op, psi, label):
+            self.label = label
+
+        def value(self, psi, step):
+            return 3.0
+
+    class FakePsi:
+        def norm(self):
+            return (2.0, [2.0])          # <psi|psi> = 4
+
+    monkeypatch.setattr(qdi, "_Filter", FakeFilter)
+    ld = qdi.Loader(FIX)
+    d = str(tmp_path) + "/"
+
+    def values(xml, **kw):
+        fl = qdi.FilterList(None, ld, [ET.fromstring(xml)], FakePsi(), d, "expec.dat", FIX, **kw)
+        v = fl.apply(FakePsi(), 0, 0.0)
+        fl.close()
+        return v
+
+    sec = '<filterinit%s><expeconly name="GridPosition"/></filterinit>'
+    assert values(sec % "", renorm_default=False) == [3.0]
+    assert values(sec % ' renorm="true"', renorm_default=False) == [0.75]
+    assert values(sec.replace("init", "post") % "") == [0.75]
+    assert values(sec.replace("init", "post") % ' renorm="false"') == [3.0]
