@@ -40,9 +40,14 @@ typedef enum qdx_status {
 
 typedef enum qdx_precision {
     QDX_FP64 = 0,        /* plain path: every array complex128 / float64        */
-    QDX_FP32_BULK = 1    /* FFT, recurrence and operator tables in FP32; Bessel
+    QDX_FP32_BULK = 1,   /* FFT, recurrence and operator tables in FP32; Bessel
                             coefficients, reductions and the accumulated psi in
                             FP64                                                 */
+    QDX_COMPACT = 2      /* FP64 arithmetic throughout; the Chebyshev terms
+                            phi_k and the kinetic pass buffer of grids with
+                            nd >= 2 (S <= 2) stored in 40 bits per component
+                            (FP32 + an 8-bit residual, relative error 2^-32),
+                            fixed-length series; everything else as QDX_FP64 */
 } qdx_precision;
 
 typedef struct qdx_ctx qdx_ctx;

@@ -20,7 +20,9 @@ LIB_PATH = os.path.join(_HERE, "libqdx.so")
 
 FP64 = 0
 FP32_BULK = 1
-_PREC = {"fp64": FP64, "fp32": FP32_BULK, "fp32_bulk": FP32_BULK, FP64: FP64, FP32_BULK: FP32_BULK}
+COMPACT = 2      # FP64 arithmetic, Chebyshev terms stored in 40 bits per component
+_PREC = {"fp64": FP64, "fp32": FP32_BULK, "fp32_bulk": FP32_BULK, "compact": COMPACT, FP64: FP64,
+         FP32_BULK: FP32_BULK, COMPACT: COMPACT}
 
 _lib = None
 

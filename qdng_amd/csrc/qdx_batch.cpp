@@ -197,7 +197,9 @@ JobHost gspo_host(qdx_ham *T, qdx_ham *W, int nsteps) {
     return j;
 }
 
-JobLayout job_layout(const BatchJob &J, size_t rsz) {
+// rsz: bytes of a real operator value; tsz: bytes of one complex value of the
+// Chebyshev term buffers (A, B, X; 10 in the compact 40-bit storage)
+JobLayout job_layout(const BatchJob &J, size_t rsz, size_t tsz) {
     JobLayout l{};
     size_t o = 0;
     auto take = [&](size_t n) {
@@ -209,15 +211,15 @@ JobLayout job_layout(const BatchJob &J, size_t rsz) {
     const size_t n = (size_t)g.size(), S = (size_t)J.S;
     if (J.kind == K_CHEBY) {
         l.psi = take(16 * n);
-        l.A = take(2 * rsz * n);
-        l.B = take(2 * rsz * n);
+        l.A = take(tsz * n);
+        l.B = take(tsz * n);
         l.coef = take(16 * (size_t)J.cp.order);
         // room for the partials of one-line row tiles, so any tiling fits
         l.part = take(8 * 3 * (size_t)std::max(J.tile.nrt, J.grid.nd == 2 ? J.grid.N[1] : 1));
     } else {
         l.psi = take(2 * rsz * n * S);
     }
-    if (g.nd == 2) l.X = take(2 * rsz * n * S);
+    if (g.nd == 2) l.X = take((J.kind == K_CHEBY ? tsz : 2 * rsz) * n * S);
     if (J.has_V) l.V = take(rsz * n);
     if (J.has_mu) l.mu = take(rsz * n);
     if (J.has_V11) l.V11 = take(rsz * n);
@@ -339,6 +341,8 @@ struct qdx_batch {
     // waves in place, 2 with two buffers; FP32 6 against 4).
     int lbr = 1, lbc = 1;
     size_t rsz() const { return precision == QDX_FP32_BULK ? sizeof(float) : sizeof(double); }
+    // QDX_COMPACT: the terms of 2D Chebyshev jobs in the 40-bit storage
+    bool cmp() const { return precision == QDX_COMPACT; }
 };
 
 namespace {
@@ -578,14 +582,15 @@ void enqueue_tick(qdx_batch *b) {
                 if (n == 0) continue;
                 KTimer t(ctx, QDX_K_BATCH_ROW);
                 BatchLaunch<R>::row(s, i == L_ROW256 ? 256 : 512, b->lbr, phase, k, jobs, tiles(i), n, plans, tick,
-                                    b->lds[i], b->twl[i]);
+                                    b->lds[i], b->twl[i], b->cmp());
             }
         };
         auto cols = [&](int k) {
             const int n = tiles_for(b->pre[L_COL], k);
             if (n == 0) return;
             KTimer t(ctx, QDX_K_BATCH_COL);
-            BatchLaunch<R>::col(s, b->lbc, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL]);
+            BatchLaunch<R>::col(s, b->lbc, k, jobs, tiles(L_COL), n, plans, tick, b->lds[L_COL], b->twl[L_COL],
+                                b->cmp());
         };
         if (cheb2d) {
             rows(0, 0);
@@ -671,7 +676,10 @@ BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
         check_grid(J.grid);
         host = cheby_host(h, d->nsteps);
         J.cp = cheby_params(h, d->dt, d->order, d->prec, 0.0);
-        if (b->precision == QDX_FP32_BULK) fp32_series(J.cp, d->dt);
+        // fixed-length series for FP32 terms and for the compact terms of 2D
+        // jobs (1D jobs of a compact batch keep their terms in registers and
+        // run as FP64 jobs)
+        if (b->precision == QDX_FP32_BULK || (b->cmp() && J.grid.nd == 2)) fp32_series(J.cp, d->dt);
         J.cp.order = launch_window(J.cp, d->dt);   // orders launched (and coefficients uploaded)
     } else {
         if (d->nops != 2 || !d->ops || !d->ops[0] || !d->ops[1])
@@ -701,7 +709,8 @@ BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
     J.has_T1 = host.has_T1;
     J.lscale = host.lscale;
     J.nt = J.has_mu ? (int)host.laser.size() : 0;
-    J.lay = job_layout(J, b->rsz());
+    const bool cmp_terms = b->cmp() && J.kind == K_CHEBY && J.grid.nd == 2;
+    J.lay = job_layout(J, b->rsz(), cmp_terms ? 10 : 2 * b->rsz());
     return J;
 }
 
@@ -783,7 +792,7 @@ bool try_submit(qdx_batch *b, const qdx_job_desc *d, int64_t &id) {
 void copy_psi(qdx_batch *b, const BatchJob &J, double *out) {
     const char *src = b->arena.base() + J.off + J.lay.psi;
     const size_t n = (size_t)J.grid.size() * J.S;
-    if (J.kind == K_CHEBY || b->precision == QDX_FP64) {
+    if (J.kind == K_CHEBY || b->precision != QDX_FP32_BULK) {
         HIPCHK(hipMemcpy(out, src, 16 * n, hipMemcpyDeviceToHost));
         return;
     }
@@ -1108,7 +1117,8 @@ extern "C" {
 int qdx_batch_create(qdx_ctx *ctx, int32_t precision, int64_t hbm_budget, qdx_batch **out) {
     return guarded([&] {
         if (!ctx || !out) fail(QDX_ERR_PARAM, "NULL argument");
-        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK && precision != QDX_COMPACT)
+            fail(QDX_ERR_PARAM, "unknown precision");
         if (hbm_budget <= 0) fail(QDX_ERR_PARAM, "HBM budget must be positive");
         HIPCHK(hipSetDevice(ctx->device));
         auto b = std::make_unique<qdx_batch>();

@@ -57,7 +57,8 @@ __device__ __forceinline__ void bfft(cx<R> (&v)[8], const BRowTile<R, LB> &B, co
 
 // PHASE 0: init (phi_0 = psi, sum |psi|^2 of the step start, X = FFT_0 phi_0)
 // PHASE 1: prologue (order 1); PHASE 2: order k >= 2
-template <class R, int NT, int PHASE, int LB>
+// CMP: phi and Y in the compact 40-bit term storage (R double)
+template <class R, int NT, int PHASE, int LB, bool CMP = false>
 __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
                                              const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
             for (int q = 0; q < 8; ++q) {
                 const cx<double> x = psi[B.gidx(q)];
                 acc += x.r * x.r + x.i * x.i;
-                A[B.gidx(q)] = from_d<R>(x);
+                tst<CMP>(A, B.gidx(q), J.size, from_d<R>(x));
             }
         }
         const double part = block_sum(acc, red);
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
     cx<R> v[8];
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = phim1[B.gidx(q)];
+        for (int q = 0; q < 8; ++q) v[q] = tld<CMP>(phim1, B.gidx(q), J.size);
     }
     const bool has_t = J.tk0 != nullptr;
     if (twl) stage_twiddles(B.tw, p0);
@@ -178,9 +179,9 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
             for (int u = 0; u < kEp; ++u) {
                 const long gi = B.gidx(h0 + u);
                 if (upd) ps[u] = psi[gi];
-                pm[u] = phim1[gi];
-                if (PHASE == 2) m2[u] = phik[gi];
-                if (two_d) yv[u] = X[gi];
+                pm[u] = tld<CMP>(phim1, gi, J.size);
+                if (PHASE == 2) m2[u] = tld<CMP>(phik, gi, J.size);
+                if (two_d) yv[u] = tld<CMP>(X, gi, J.size);
                 double w = 0.0;
                 if (J.V) w += (double)J.V[gi];
                 if (J.mu) w += Ed * (double)J.mu[gi];
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
                         psi[gi] = sv;
                     }
                 }
-                phik[gi] = from_d<R>(pk);
+                tst<CMP>(phik, gi, J.size, from_d<R>(pk));
             }
         }
     }
@@ -223,8 +224,9 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
 }
 
 // dimension-1 part of the kinetic operator of a 2D job:
-// Y = IFFT_1(T_1/N_1 FFT_1 phi_{k-1}), held in X; LB LDS buffers per line
-template <class R, int LB>
+// Y = IFFT_1(T_1/N_1 FFT_1 phi_{k-1}), held in X; LB LDS buffers per line;
+// CMP: phi and Y in the compact 40-bit term storage (R double)
+template <class R, int LB, bool CMP = false>
 __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, const BTile *__restrict__ tiles,
                                               const LinePlan *__restrict__ plans, int k, const int *tick_p, int twl) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
@@ -242,8 +244,8 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     const int lt = threadIdx.x / W;
     const int col = t.first + c;
     const bool active = lt < TL && col < N0;
-    cx<R> *base = reinterpret_cast<cx<R> *>(J.X) + col;
-    const cx<R> *in = reinterpret_cast<const cx<R> *>((k - 1) % 2 == 0 ? J.phiA : J.phiB) + col;
+    cx<R> *Xb = reinterpret_cast<cx<R> *>(J.X);
+    const cx<R> *in = reinterpret_cast<const cx<R> *>((k - 1) % 2 == 0 ? J.phiA : J.phiB);
     LdsLine<R> ln;
     ln.a = smem + c;
     ln.b = LB == 2 ? smem + (long)W * tw_lds_len(L) + c : ln.a;
@@ -253,7 +255,7 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     cx<R> v[8];
     if (active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = in[(long)(lt + q * TL) * N0];
+        for (int q = 0; q < 8; ++q) v[q] = tld<CMP>(in, col + (long)(lt + q * TL) * N0, J.size);
     }
     if (twl) stage_twiddles(twt, p);
     if (k >= 3) {
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     fft8<R, LB == 1>(v, ln, p, tws, +1, lt, active, tsh);
     if (active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) base[(long)(lt + q * TL) * N0] = v[q];
+        for (int q = 0; q < 8; ++q) tst<CMP>(Xb, col + (long)(lt + q * TL) * N0, J.size, v[q]);
     }
 }
 
@@ -619,53 +621,72 @@ static void allow_lds(K kernel, size_t bytes) {
     allowed = bytes;
 }
 
-template <class R, int NT, int PHASE, int LB>
+template <class R, int NT, int PHASE, int LB, bool CMP>
 static void row_launch(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
                        const LinePlan *plans, const int *tick, size_t lds, bool twl) {
-    allow_lds(k_brow<R, NT, PHASE, LB>, lds);
-    QDX_LAUNCH((k_brow<R, NT, PHASE, LB>), dim3(ntiles), dim3(NT), lds, s, jobs, tiles, plans, k, tick,
+    allow_lds(k_brow<R, NT, PHASE, LB, CMP>, lds);
+    QDX_LAUNCH((k_brow<R, NT, PHASE, LB, CMP>), dim3(ntiles), dim3(NT), lds, s, jobs, tiles, plans, k, tick,
                twl ? 1 : 0);
     QDX_HIP(hipGetLastError());
 }
 
-template <class R, int NT>
+template <class R, int NT, bool CMP>
 static void row_phase(hipStream_t s, int lbuf, int phase, int k, const BJob<R> *jobs, const BTile *tiles,
                       int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool twl) {
     // the init phase runs no transform: one instance serves both layouts
-    if (phase == 0) row_launch<R, NT, 0, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
-    else if (phase == 1 && lbuf == 2) row_launch<R, NT, 1, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
-    else if (phase == 1) row_launch<R, NT, 1, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
-    else if (lbuf == 2) row_launch<R, NT, 2, 2>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
-    else row_launch<R, NT, 2, 1>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    if (phase == 0) row_launch<R, NT, 0, 2, CMP>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else if (phase == 1 && lbuf == 2) row_launch<R, NT, 1, 2, CMP>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else if (phase == 1) row_launch<R, NT, 1, 1, CMP>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else if (lbuf == 2) row_launch<R, NT, 2, 2, CMP>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
+    else row_launch<R, NT, 2, 1, CMP>(s, k, jobs, tiles, ntiles, plans, tick, lds, twl);
 }
 
 template <class R>
 void BatchLaunch<R>::row(hipStream_t s, int NT, int lbuf, int phase, int k, const BJob<R> *jobs,
                          const BTile *tiles, int ntiles, const LinePlan *plans, const int *tick, size_t lds,
-                         bool tw_lds) {
+                         bool tw_lds, bool cmp) {
     if (ntiles <= 0) return;
     if (lds > 160 * 1024) throw std::runtime_error("row tile exceeds the 160 KiB LDS of a gfx950 CU");
     if (lbuf != 1 && lbuf != 2) throw std::runtime_error("row tiles hold 1 or 2 LDS buffers per line");
-    if (NT == 256) row_phase<R, 256>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-    else if (NT == 512) row_phase<R, 512>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
-    else throw std::runtime_error("row tiles use 256 or 512 threads");
+    if (NT != 256 && NT != 512) throw std::runtime_error("row tiles use 256 or 512 threads");
+    if (cmp) {
+        if constexpr (std::is_same<R, double>::value) {
+            if (NT == 256) row_phase<R, 256, true>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+            else row_phase<R, 512, true>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+        } else {
+            throw std::runtime_error("compact terms: FP64 batches only");
+        }
+        return;
+    }
+    if (NT == 256) row_phase<R, 256, false>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+    else row_phase<R, 512, false>(s, lbuf, phase, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+}
+
+template <class R, int LB, bool CMP>
+static void col_launch(hipStream_t s, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
+                       const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
+    allow_lds(k_bcol<R, LB, CMP>, lds);
+    QDX_LAUNCH((k_bcol<R, LB, CMP>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
+    QDX_HIP(hipGetLastError());
 }
 
 template <class R>
 void BatchLaunch<R>::col(hipStream_t s, int lbuf, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
-                         const LinePlan *plans, const int *tick, size_t lds, bool tw_lds) {
+                         const LinePlan *plans, const int *tick, size_t lds, bool tw_lds, bool cmp) {
     if (ntiles <= 0) return;
     if (lds > 160 * 1024) throw std::runtime_error("column tile exceeds the 160 KiB LDS of a gfx950 CU");
-    if (lbuf == 2) {
-        allow_lds(k_bcol<R, 2>, lds);
-        QDX_LAUNCH((k_bcol<R, 2>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
-    } else if (lbuf == 1) {
-        allow_lds(k_bcol<R, 1>, lds);
-        QDX_LAUNCH((k_bcol<R, 1>), dim3(ntiles), dim3(256), lds, s, jobs, tiles, plans, k, tick, tw_lds ? 1 : 0);
-    } else {
-        throw std::runtime_error("column tiles hold 1 or 2 LDS buffers per line");
+    if (lbuf != 1 && lbuf != 2) throw std::runtime_error("column tiles hold 1 or 2 LDS buffers per line");
+    if (cmp) {
+        if constexpr (std::is_same<R, double>::value) {
+            if (lbuf == 2) col_launch<R, 2, true>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+            else col_launch<R, 1, true>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+        } else {
+            throw std::runtime_error("compact terms: FP64 batches only");
+        }
+        return;
     }
-    QDX_HIP(hipGetLastError());
+    if (lbuf == 2) col_launch<R, 2, false>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
+    else col_launch<R, 1, false>(s, k, jobs, tiles, ntiles, plans, tick, lds, tw_lds);
 }
 
 template <class R>

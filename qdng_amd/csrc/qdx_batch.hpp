@@ -74,11 +74,13 @@ struct BatchLaunch {
     // init (phase 0), prologue (phase 1) or order k >= 2 (phase 2) over the
     // row tiles [0, ntiles); NT = 256 or 512 threads per block; lbuf LDS
     // buffers per line (2: ping-pong, 1: in place)
+    // (cmp: phi and X of the 2D jobs in the compact 40-bit term storage,
+    // R double)
     static void row(hipStream_t s, int NT, int lbuf, int phase, int k, const BJob<R> *jobs, const BTile *tiles,
-                    int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool tw_lds);
+                    int ntiles, const LinePlan *plans, const int *tick, size_t lds, bool tw_lds, bool cmp = false);
     // dimension-1 part of the kinetic operator of the 2D jobs (into X)
     static void col(hipStream_t s, int lbuf, int k, const BJob<R> *jobs, const BTile *tiles, int ntiles,
-                    const LinePlan *plans, const int *tick, size_t lds, bool tw_lds);
+                    const LinePlan *plans, const int *tick, size_t lds, bool tw_lds, bool cmp = false);
     // one whole time step of each 1D job in `list` (one block per job)
     static void line(hipStream_t s, int NT, const BJob<R> *jobs, const int *list, int n, const LinePlan *plans,
                      const int *tick, size_t lds, bool tw_lds);

@@ -31,6 +31,7 @@ struct qdx_prop {
     int k_launch = 0;   // orders launched per step (launch_window)
     bool imag = false;  // imaginary time, backward (exponent -dt)
     DevBuf phiA, phiB, X, ctl, partials, lastorders;
+    bool cmp = false;   // QDX_COMPACT on a grid with nd >= 2: compact terms
     // single-state 1D lines: the whole step in one block (k_cheby_line) with
     // the phased coefficients coeff[k] exp(-i offset dt) on the device
     bool line_step = false;
@@ -109,7 +110,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     int nb_init;
     {
         KTimer t(ctx, QDX_K_CHEBY_INIT);
-        nb_init = Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, nullptr, ctl, init);
+        nb_init = Launch<R>::cheby_init(s, g, S, plans[0], psi->ptr(), A, nullptr, ctl, init, p->cmp);
     }
 
     ConvCheck cc;
@@ -122,7 +123,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     auto cols = [&](const R *pm1, const ConvCheck &c) {
         for (int d = 1; d < p->grid.nd; ++d) {
             KTimer t(ctx, QDX_K_LINE_PASS);
-            Launch<R>::kin_pass(s, g, S, d, d > 1, plans[d], h, pm1, X, ctl, c);
+            Launch<R>::kin_pass(s, g, S, d, d > 1, plans[d], h, pm1, X, ctl, c, p->cmp);
         }
     };
     // exp(exponent * offset), exponent -i dt (real time) or -dt (imaginary)
@@ -142,7 +143,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
         Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, p->imag, A, B, X, psi->ptr(), ctl, slot[1 % nslot], init,
-                             nb_init);
+                             nb_init, p->cmp);
     }
     int nb_prev = 0;
     for (int k = 2; k < p->k_launch; ++k) {
@@ -171,7 +172,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
         cols(pm1, cc);
         KTimer t(ctx, QDX_K_CHEBY_ROW);
         nb_prev = Launch<R>::cheby_row(s, g, h, plans[0], ord, cc, p->imag, pm1, pm2, X, psi->ptr(), ctl,
-                                       slot[k % nslot], init, nb_init);
+                                       slot[k % nslot], init, nb_init, p->cmp);
     }
     p->last_nb = nb_prev;
     p->nb_init = nb_init;
@@ -326,6 +327,7 @@ void prop_step(qdx_prop *p, qdx_wf *psi, long step) {
         return;
     }
     if (p->kind == qdx_prop::CHEBY) {
+        // QDX_COMPACT: FP64 arithmetic (compact terms when p->cmp)
         if (p->precision == QDX_FP32_BULK) cheby_step<float>(p, psi, step);
         else cheby_step<double>(p, psi, step);
         // record executed order count of this step
@@ -406,7 +408,8 @@ int qdx_ctx_set_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, const int
 int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *radix, int32_t *nst) {
     return guarded([&] {
         if (!ctx || !radix || !nst) fail(QDX_ERR_PARAM, "NULL argument");
-        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK && precision != QDX_COMPACT)
+            fail(QDX_ERR_PARAM, "unknown precision");
         const std::vector<int> r = ctx->radix_of(L, precision == QDX_FP32_BULK);
         for (size_t i = 0; i < r.size(); ++i) radix[i] = r[i];
         *nst = (int32_t)r.size();
@@ -772,7 +775,8 @@ static void cheby_create_impl(qdx_ham *h, double dt, int32_t order, double prec,
                               bool imag, qdx_prop **out) {
         if (!h || !out) fail(QDX_ERR_PARAM, "NULL argument");
         if (!h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
-        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK && precision != QDX_COMPACT)
+            fail(QDX_ERR_PARAM, "unknown precision");
         // the imaginary-time series cancels down to exp(-R dt) of its terms:
         // FP32 terms would leave nothing of the result
         if (imag && precision != QDX_FP64) fail(QDX_ERR_PARAM, "imaginary time needs FP64");
@@ -786,12 +790,16 @@ static void cheby_create_impl(qdx_ham *h, double dt, int32_t order, double prec,
         p->ham = h;
         p->imag = imag;
         p->cp = cheby_params(h, dt, order, prec, scaling, imag);
-        if (precision == QDX_FP32_BULK) fp32_series(p->cp, dt);
+        // compact terms on grids with nd >= 2 and one or two states; other
+        // grids keep their terms in registers or take the FP64 kernels
+        p->cmp = precision == QDX_COMPACT && h->grid.nd >= 2 && h->S <= 2;
+        if (precision == QDX_FP32_BULK || p->cmp) fp32_series(p->cp, dt);
         // imaginary time: every order up to the series length (its break
         // test compares consecutive orders, not the coefficients)
         p->k_launch = imag ? p->cp.order : launch_window(p->cp, dt);
         if (precision == QDX_FP32_BULK) h->ensure32();
-        const size_t es = precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
+        // element bytes of the term buffers (compact: 8 hi + 2 lo)
+        const size_t es = p->cmp ? 10 : precision == QDX_FP32_BULK ? 2 * sizeof(float) : 2 * sizeof(double);
         const size_t nbytes = (size_t)p->S * p->grid.size() * es;
         p->phiA.alloc(nbytes);
         p->phiB.alloc(nbytes);
@@ -866,7 +874,10 @@ static void gspo_create_impl(qdx_ham *const *ops, int32_t nops, double dt, int32
         if (!ops || !out) fail(QDX_ERR_PARAM, "NULL argument");
         if (nops < 1) fail(QDX_ERR_PARAM, "GSPO is empty!");
         if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
-        if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
+        if (precision != QDX_FP64 && precision != QDX_FP32_BULK && precision != QDX_COMPACT)
+            fail(QDX_ERR_PARAM, "unknown precision");
+        // QDX_COMPACT has no GSPO terms to store: psi stays complex128 (FP64)
+        if (precision == QDX_COMPACT) precision = QDX_FP64;
         auto p = std::make_unique<qdx_prop>();
         p->kind = qdx_prop::GSPO;
         p->precision = precision;

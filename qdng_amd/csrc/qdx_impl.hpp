@@ -136,8 +136,10 @@ struct Launch {
     // psi (complex128) -> phi (complex R); norm0 into ctl; resets ctl.
     // For nd >= 2 also X = FFT_0(phi).
     // returns the number of partial sums written (one per block)
+    // cmp (below): phi, X in the compact 40-bit term storage (R double,
+    // nd >= 2, S <= 2)
     static int cheby_init(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
-                          const double *psi, R *phi0, R *X, ChebyCtl *ctl, double *partials);
+                          const double *psi, R *phi0, R *X, ChebyCtl *ctl, double *partials, bool cmp = false);
     // One order of the series; prologue when ord.k == 1.  T phi_{k-1} is
     // the dimension-0 filter IFFT_0(T_0/N_0 FFT_0 phi_{k-1}) in registers,
     // plus (nd >= 2) the other dimensions' part Y that kin_pass left in X.
@@ -146,7 +148,7 @@ struct Launch {
     static int cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
                          const ChebyOrder &ord, const ConvCheck &cc, bool imag, const R *phim1, R *phim2, R *X,
                          double *psi, ChebyCtl *ctl, double *partials, const double *init_part,
-                         int nb_init);
+                         int nb_init, bool cmp = false);
     // a whole Chebyshev step of a single-state 1D line in one block (see
     // k_cheby_line); coef: (re, im) of coeff[k] exp(-i offset dt), k < order
     static void cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
@@ -166,7 +168,7 @@ struct Launch {
     // kernel of the order then only reads
     static void kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool accumulate,
                          const LinePlan &p, const HamDev<R> &h, const R *src, R *Y,
-                         ChebyCtl *ctl, const ConvCheck &cc);
+                         ChebyCtl *ctl, const ConvCheck &cc, bool cmp = false);
     // dst = H src (both complex128) through the same kernels; X scratch
     static void ham_apply(hipStream_t s, const GridDev &g, const HamDev<R> &h,
                           const LinePlan *plans, const double *src, double *dst, R *X);

@@ -191,6 +191,41 @@ def test_cheby_fp32_bulk_accuracy(ctx, name, builder, dt):
     assert rel(got, ref) < 1e-6
 
 
+# compact terms (FP64 arithmetic, 40-bit storage of phi_k and of the kinetic
+# pass buffer): every grid shape with nd >= 2 and one or two states, against
+# the oracle at the 1e-6 bar (the storage rounding alone is ~1e-9 here)
+@pytest.mark.parametrize("name,builder,dt,steps", [
+    ("morse_2state_2d", lambda: cases.morse_2state_2d(96), 5.0, 6),
+    ("anharmonic_3d", lambda: cases.anharmonic_3d(32), 3.5, 4),
+    ("anharmonic_3d_long", lambda: cases.anharmonic_3d(24), 37.0, 2),
+    ("two_state_3d", lambda: _two_state_3d(), 2.0, 3),
+    ("laser_2d", lambda: cases.driven_job([64, 96], np.random.default_rng(8), 10), 1.0, 8),
+    ("harmonic_1d", lambda: cases.harmonic_1d(256), 10.0, 5),
+])
+def test_cheby_compact_accuracy(ctx, name, builder, dt, steps):
+    note(f"cheby compact {name}")
+    p = builder()
+    got, ref, counts = cheby_pair(ctx, p, dt, 1e-10, "compact", steps)
+    e = rel(got, ref)
+    note(f"  rel L2 {e:.2e}")
+    assert e < 1e-6
+    if p.nd == 1:   # 1D lines keep their terms in registers: the FP64 path
+        assert e < 1e-10 and all(a == b for a, b in counts)
+
+
+def _two_state_3d():
+    p = cases.Problem([16, 32, 16], [-4.0] * 3, [4.0] * 3, nstates=2)
+    ax = p.axes()
+    r2 = sum(a ** 2 for a in ax)
+    for s in range(2):
+        p.kinetic(s, [1500.0, 1800.0, 2100.0])
+        p.potential(s, s, 0.04 * r2 + 0.02 * s)
+    p.potential(0, 1, 0.01 * np.exp(-r2))
+    g = cases.gaussian(p, [0.3, -0.2, 0.1], 0.6)
+    p.psi0 = cases.normalized(p, np.concatenate([g.ravel(), 0.5 * g.ravel()]))
+    return p
+
+
 # 8 and 4 points per thread of the Chebyshev line kernels on grids with
 # nd >= 2 (the automatic choice takes 4 on grids this small).  Plans at 4:
 # 96 -> [4, 2, 3, 4] by default, a set plan with edges 4/2 is used as set,
