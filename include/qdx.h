@@ -142,6 +142,17 @@ int qdx_ham_apply(qdx_ham *h, qdx_wf *dst, qdx_wf *src, int64_t step);
 /* Re <psi|H(step)|psi> (Operator::Expec, Operator.h:249-252) */
 int qdx_ham_expec(qdx_ham *h, qdx_wf *psi, int64_t step, double *out);
 int qdx_ham_destroy(qdx_ham *h);
+/* <psi|H|psi> as qdx_ham_expec, without allocating: no counterpart in QDng.
+ * It is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  H psi, the kinetic pass buffer and
+ * the reduction partials live in one fixed device array per GPU (about
+ * 16 MiB, looked up by qdx_ctx_create), which limits S * N to 524288 grid
+ * points (QDX_ERR_MEMORY beyond).  Set-up first: the Hamiltonian must have
+ * been applied once (qdx_ham_apply or qdx_ham_expec), which uploads its
+ * arrays and builds the transform tables (QDX_ERR_STATE otherwise).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * host threads, other contexts) race. */
+int qdx_ham_expec_noalloc(qdx_ham *h, qdx_wf *psi, int64_t step, double *out);
 
 /* ---- host-side parameter rules (no device needed) ------------------------ */
 /* BesselJ0 (linalg/Bessel.cpp:22-58): J_k(x), k = 0..n-1, and the number of

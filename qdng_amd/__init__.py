@@ -53,7 +53,7 @@ EXPORTS = [
     "qdx_wf_create", "qdx_wf_upload", "qdx_wf_download", "qdx_wf_copy", "qdx_wf_norm",
     "qdx_wf_overlap", "qdx_wf_normalize", "qdx_wf_destroy", "qdx_ham_create",
     "qdx_ham_add_kinetic", "qdx_ham_add_potential", "qdx_ham_add_dipole", "qdx_ham_bounds",
-    "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
+    "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_expec_noalloc", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
     "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
     "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
     "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
@@ -137,6 +137,7 @@ def lib():
         "qdx_ham_bounds": [vp, dp, dp],
         "qdx_ham_apply": [vp, vp, vp, C.c_int64],
         "qdx_ham_expec": [vp, vp, C.c_int64, dp],
+        "qdx_ham_expec_noalloc": [vp, vp, C.c_int64, dp],
         "qdx_ham_destroy": [vp],
         "qdx_bessel_j0": [i32, C.c_double, dp, C.POINTER(i32)],
         "qdx_cheby_create": [vp, C.c_double, i32, C.c_double, C.c_double, i32, C.POINTER(vp)],

@@ -362,6 +362,8 @@ int qdx_ctx_create(int device, qdx_ctx **out) {
         HIPCHK(hipSetDevice(device));
         auto c = std::make_unique<qdx_ctx>();
         c->device = device;
+        // device scratch of qdx_ham_expec_noalloc, looked up once per device
+        qdx::noalloc_scratch_setup(device);
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->partials.alloc(kMaxPartials * sizeof(double));
         c->scal.alloc(4096 * sizeof(double));
@@ -731,6 +733,47 @@ int qdx_ham_expec(qdx_ham *h, qdx_wf *psi, int64_t step, double *out) {
         double re = 0;
         for (int s = 0; s < psi->S; ++s) re += per[2 * s];
         *out = re * psi->grid.weight();   // Operator.h:226-252
+    });
+}
+
+int qdx_ham_expec_noalloc(qdx_ham *h, qdx_wf *psi, int64_t step, double *out) {
+    return guarded([&] {
+        if (!out) fail(QDX_ERR_PARAM, "out is NULL");
+        if (!h || !h->ctx) fail(QDX_ERR_PARAM, "Hamiltonian has no device context");
+        check_wf(psi);
+        if (!(psi->grid == h->grid) || psi->S != h->S) fail(QDX_ERR_INCOMPAT, "wave function does not match the Hamiltonian");
+        if (psi->ctx != h->ctx) fail(QDX_ERR_INCOMPAT, "wave function and Hamiltonian on different contexts");
+        qdx_ctx *c = h->ctx;
+        const long n = psi->n();
+        const long pts = (long)psi->S * n;
+        if (pts > kNoAllocPoints)
+            fail(QDX_ERR_MEMORY, "qdx_ham_expec_noalloc: " + std::to_string(pts) + " grid points times states exceed its "
+                                 "scratch of " + std::to_string(kNoAllocPoints));
+        // set-up happens elsewhere: the operator arrays are uploaded and the
+        // transform tables built by the first qdx_ham_apply / qdx_ham_expec
+        if (!h->final_) fail(QDX_ERR_STATE, "qdx_ham_expec_noalloc: apply the Hamiltonian once first (set-up)");
+        LinePlan plans[3];
+        for (int d = 0; d < h->grid.nd; ++d) {
+            if (!c->tw64.count(h->grid.N[d]))
+                fail(QDX_ERR_STATE, "qdx_ham_expec_noalloc: apply the Hamiltonian once first (set-up)");
+            plans[d] = c->plan(h->grid.N[d], false);   // cached table: no allocation
+        }
+        unsigned char *base = static_cast<unsigned char *>(qdx::noalloc_scratch(c->device));
+        if (!base) fail(QDX_ERR_STATE, "device scratch was not set up for this device");
+        // layout: H psi | X | partials | per-state sums
+        double *hpsi = reinterpret_cast<double *>(base);
+        double *X = reinterpret_cast<double *>(base + (size_t)kNoAllocPoints * 16);
+        double *partials = reinterpret_cast<double *>(base + (size_t)2 * kNoAllocPoints * 16);
+        double *scal = partials + kMaxPartials;
+        HamDev<double> hd = h->dev.h64;
+        h->set_fields(hd, step);
+        Launch<double>::ham_apply(c->stream, h->grid.dev(), hd, plans, psi->ptr(), hpsi, X);
+        launch_overlap(c->stream, psi->ptr(), hpsi, n, psi->S, partials, scal);
+        HIPCHK(hipMemcpyAsync(c->pinned, scal, 2 * psi->S * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double re = 0;
+        for (int s = 0; s < psi->S; ++s) re += c->pinned[2 * s];
+        *out = re * psi->grid.weight();
     });
 }
 

@@ -85,6 +85,18 @@ struct ConvCheck {
 };
 constexpr int kMaxPartials = 8192;
 
+// Scratch of the allocation-free expectation value (qdx_ham_expec_noalloc):
+// a file-scope device array of kNoAllocBytes per GPU holding H psi, the
+// kinetic pass buffer X (grids with nd >= 2), reduction partials and the
+// per-state results, for at most kNoAllocPoints = S * N grid points.
+constexpr long kNoAllocPoints = 524288;
+constexpr size_t kNoAllocBytes = (size_t)2 * kNoAllocPoints * 16 + (size_t)kMaxPartials * 8 + 4096;
+// Looks up the array's device address on `device` (which must be current)
+// once and keeps it; throws std::runtime_error if the lookup fails.
+void *noalloc_scratch_setup(int device);
+// The address found by noalloc_scratch_setup, or nullptr before it ran.
+void *noalloc_scratch(int device);
+
 // Chebyshev scalars for one order (OCheby.cpp:97-122).
 struct ChebyOrder {
     int k;               // order index (1 for the prologue)

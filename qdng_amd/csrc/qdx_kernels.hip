@@ -18,8 +18,8 @@
 //
 // Translation units: qdx_cheby.hip (Chebyshev init, row and line kernels),
 // qdx_lines.hip (strided line passes, H applied to a vector), qdx_gspo.hip
-// (GSPO), this file (reductions, conversions); shared pieces in
-// qdx_kernels_common.hpp.
+// (GSPO), this file (reductions, conversions, the allocation-free entry
+// point's device scratch); shared pieces in qdx_kernels_common.hpp.
 #include "qdx_kernels_common.hpp"
 
 namespace qdx {
@@ -161,6 +161,37 @@ void launch_wnorm(hipStream_t s, const double *a, long n, int S, const GridDev &
     QDX_HIP(hipGetLastError());
     QDX_LAUNCH(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
     QDX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Device scratch of the allocation-free entry point.  Host code never passes
+// the array's name to a kernel or a copy: it uses the address that
+// hipGetSymbolAddress returns for the current device, kept per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void *noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+void *noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 void launch_overlap(hipStream_t s, const double *a, const double *b, long n, int S, double *partials, double *out) {
