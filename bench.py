@@ -113,8 +113,8 @@ def run_gspo(ctx, p, precision, steps, warmup):
 
 # batched jobs, single state with a laser dipole: phi_{k-1}, phi_{k-2}, phi_k,
 # psi read/write, V and mu per grid point and executed order
-B_BATCH = {"fp64": 16 * 3 + 32 + 8 + 8, "fp32": 8 * 3 + 32 + 4 + 4}
-B_BATCH_M2 = {"fp64": 16, "fp32": 8}   # the prologue reads no phi_{k-2}
+B_BATCH = {"fp64": 16 * 3 + 32 + 8 + 8, "fp32": 8 * 3 + 32 + 4 + 4, "compact": 10 * 3 + 32 + 8 + 8}
+B_BATCH_M2 = {"fp64": 16, "fp32": 8, "compact": 10}   # the prologue reads no phi_{k-2}
 
 
 def run_batched(ctx, precision, steps, prof_ticks, seed=2026, tune=False):

@@ -46,8 +46,9 @@ typedef enum qdx_precision {
     QDX_COMPACT = 2      /* FP64 arithmetic throughout; the Chebyshev terms
                             phi_k and the kinetic pass buffer of grids with
                             nd >= 2 (S <= 2) stored in 40 bits per component
-                            (FP32 + an 8-bit residual, relative error 2^-32),
-                            fixed-length series; everything else as QDX_FP64 */
+                            (the upper 40 bits of the FP64 pattern, rounded:
+                            relative error 2^-29), fixed-length series;
+                            everything else as QDX_FP64                      */
 } qdx_precision;
 
 typedef struct qdx_ctx qdx_ctx;

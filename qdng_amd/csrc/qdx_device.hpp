@@ -74,32 +74,24 @@ template <class R>
 __device__ __forceinline__ cx<R> from_d(cx<double> a) { return mk<R>((R)a.r, (R)a.i); }
 
 // ---------------------------------------------------------------------------
-// Compact (40-bit) storage of complex128 terms.  Each component is kept as
-// its FP32 rounding h and the rest x - h in units of ulp(h)/256 as a signed
-// byte l: x ~ h + l * 2^(e - 158), e the biased FP32 exponent of h (at least
-// 1), so the relative error is at most 2^-32 (FP32 storage: 2^-24).  A
-// buffer of n complex values holds the n hi pairs (8 n bytes) followed by the
-// n lo pairs (2 n bytes).
-struct alignas(2) i8x2 {
-    signed char x, y;
+// Compact (40-bit) storage of complex128 terms.  Each component keeps the
+// upper 40 bits of its FP64 pattern (sign, exponent and the 28 leading
+// mantissa bits), rounded to nearest on the dropped bits: relative error at
+// most 2^-29 (FP32 storage: 2^-24); encoding and decoding are integer moves.
+// A buffer of n complex values holds the n pairs of upper words (8 n bytes:
+// bits 63..32 of re and im) followed by the n pairs of the next bytes (2 n
+// bytes: bits 31..24).
+struct alignas(2) u8x2 {
+    unsigned char x, y;
 };
 
-__device__ __forceinline__ int c40_exp(float h) {
-    const int e = (int)((__float_as_uint(h) >> 23) & 0xffu);
-    return e > 0 ? e : 1;
+__device__ __forceinline__ void t40_split(double x, unsigned &hi, unsigned char &lo) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x) + 0x800000ull;   // round at bit 23
+    hi = (unsigned)(b >> 32);
+    lo = (unsigned char)(b >> 24);
 }
-// 2^k for -1022 <= k <= 1023, from the exponent bits
-__device__ __forceinline__ double pow2d(int k) { return __longlong_as_double((long long)(1023 + k) << 52); }
-
-__device__ __forceinline__ void c40_split(double x, float &h, signed char &l) {
-    h = __double2float_rn(x);
-    const double r = (x - (double)h) * pow2d(158 - c40_exp(h));   // |r| <= 128
-    int q = __double2int_rn(r);
-    q = q < -128 ? -128 : (q > 127 ? 127 : q);
-    l = (signed char)q;
-}
-__device__ __forceinline__ double c40_join(float h, signed char l) {
-    return (double)h + (double)l * pow2d(c40_exp(h) - 158);
+__device__ __forceinline__ double t40_join(unsigned hi, unsigned char lo) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | ((unsigned long long)lo << 24)));
 }
 
 // element i of a term buffer of n complex values: plain cx<R>, or (CMP, R
@@ -110,9 +102,9 @@ __device__ __forceinline__ cx<R> tld(const cx<R> *p, long i, long n) {
         return p[i];
     } else {
         static_assert(sizeof(R) == 8, "compact storage holds complex128 terms");
-        const float2 h = reinterpret_cast<const float2 *>(p)[i];
-        const i8x2 l = reinterpret_cast<const i8x2 *>(reinterpret_cast<const float2 *>(p) + n)[i];
-        return mk<R>(c40_join(h.x, l.x), c40_join(h.y, l.y));
+        const uint2 h = reinterpret_cast<const uint2 *>(p)[i];
+        const u8x2 l = reinterpret_cast<const u8x2 *>(reinterpret_cast<const uint2 *>(p) + n)[i];
+        return mk<R>(t40_join(h.x, l.x), t40_join(h.y, l.y));
     }
 }
 template <bool CMP, class R>
@@ -121,12 +113,12 @@ __device__ __forceinline__ void tst(cx<R> *p, long i, long n, cx<R> v) {
         p[i] = v;
     } else {
         static_assert(sizeof(R) == 8, "compact storage holds complex128 terms");
-        float2 h;
-        i8x2 l;
-        c40_split(v.r, h.x, l.x);
-        c40_split(v.i, h.y, l.y);
-        reinterpret_cast<float2 *>(p)[i] = h;
-        reinterpret_cast<i8x2 *>(reinterpret_cast<float2 *>(p) + n)[i] = l;
+        uint2 h;
+        u8x2 l;
+        t40_split(v.r, h.x, l.x);
+        t40_split(v.i, h.y, l.y);
+        reinterpret_cast<uint2 *>(p)[i] = h;
+        reinterpret_cast<u8x2 *>(reinterpret_cast<uint2 *>(p) + n)[i] = l;
     }
 }
 

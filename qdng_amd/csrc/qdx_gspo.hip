@@ -79,13 +79,19 @@ __device__ __forceinline__ void xgroup_point(const XGroupDev<R> &grp, long gi, c
 // GSPO kinetic split step over rows (P points per thread).  ONE_D: psi ->
 // pre -> FFT -> exp(-i tau
 // T)/N -> IFFT -> post -> psi in one pass.  Otherwise PHASE 0: psi -> pre ->
-// FFT_0 -> X; PHASE 1: X -> IFFT_0 -> post -> psi.  The x-space operators act
-// on all states of a point at once, so the S lines of a row are exchanged
-// through LDS around them.
+// FFT_0 -> X; PHASE 1: X -> IFFT_0 -> post -> psi; PHASE 2 (between two
+// steps of a run): X -> IFFT_0 -> post of step n -> pre of step n + 1 ->
+// FFT_0 -> X, so that psi stays in X between the steps.  With opart, PHASE 2
+// also stores this block's sums of the state between the steps, per state s:
+// opart[2 (s nb + b)] = sum |psi|^2 and, after 2 S nb entries, the
+// (re, im) of sum conj(psi) psi0 (Reporter observables, Reporter.cpp:101-137).
+// The x-space operators act on all states of a point at once, so the S lines
+// of a row are exchanged through LDS around them.
 template <class R, int S, bool ONE_D, int PHASE, bool IMAG = false, int P = 8>
 __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, double tau, LinePlan p0, int rb,
                                                    XGroupDev<R> pre, XGroupDev<R> post, cx<R> *__restrict__ psi,
-                                                   cx<R> *__restrict__ X) {
+                                                   cx<R> *__restrict__ X, const cx<R> *__restrict__ psi0,
+                                                   double *__restrict__ opart) {
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
     RowBlock<R, P> B(g, rb, S, smem, p0);
@@ -172,6 +178,42 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
     }
     B.fft(v, p0, +1);
     xspace(post);
+    if (PHASE == 2) {
+        if (opart) {
+            __shared__ double red[16];
+            double n2 = 0, ar = 0, ai = 0;
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const cx<double> a = to_d(v[q]), b = to_d(psi0[base + B.gidx(q)]);
+                    n2 += a.r * a.r + a.i * a.i;
+                    ar += a.r * b.r + a.i * b.i;
+                    ai += a.r * b.i - a.i * b.r;
+                }
+            }
+            const long nb = gridDim.x;
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2) {
+                const bool mine = B.active && st == s2;
+                const double t0 = block_sum(mine ? n2 : 0.0, red);
+                const double t1 = block_sum(mine ? ar : 0.0, red);
+                const double t2 = block_sum(mine ? ai : 0.0, red);
+                if (threadIdx.x == 0) {
+                    opart[2 * (s2 * nb + blockIdx.x)] = t0;
+                    opart[2 * (s2 * nb + blockIdx.x) + 1] = 0.0;
+                    opart[2 * S * nb + 2 * (s2 * nb + blockIdx.x)] = t1;
+                    opart[2 * S * nb + 2 * (s2 * nb + blockIdx.x) + 1] = t2;
+                }
+            }
+        }
+        xspace(pre);
+        B.fft(v, p0, -1);
+        if (B.active) {
+#pragma unroll
+            for (int q = 0; q < P; ++q) X[base + B.gidx(q)] = v[q];
+        }
+        return;
+    }
     if (B.active) {
 #pragma unroll
         for (int q = 0; q < P; ++q) psi[base + B.gidx(q)] = v[q];
@@ -190,40 +232,49 @@ __global__ void k_gspo_xonly(GridDev g, XGroupDev<R> grp, cx<R> *__restrict__ ps
     for (int st = 0; st < S; ++st) psi[(long)st * g.size + gi] = from_d<R>(v[st]);
 }
 
+// returns the number of row blocks launched
 template <class R, bool ONE_D, int PHASE, bool IMAG, int P>
-static void gspo_row_launch_IP(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
-                               const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
-                               R *X) {
+static int gspo_row_launch_IP(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                              const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
+                              R *X, const R *psi0, double *opart) {
+    int nb = 0;
     QDX_DISPATCH_S(S, {
         LinePlan p0 = plan;
         RowLaunch r = row_launch<R>(g, SS, p0, SS > 1);
+        if (opart && (long)4 * SS * r.nb > kMaxPartials)
+            throw std::runtime_error("grid too large for the observables' partial sums");
         set_lds(k_gspo_row<R, SS, ONE_D, PHASE, IMAG, P>, r.lds);
         QDX_LAUNCH((k_gspo_row<R, SS, ONE_D, PHASE, IMAG, P>), dim3(r.nb), dim3(r.threads), r.lds, s, g, kin, tau,
-                           p0, r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X));
+                   p0, r.rb, pre, post, reinterpret_cast<cx<R> *>(psi), reinterpret_cast<cx<R> *>(X),
+                   reinterpret_cast<const cx<R> *>(psi0), opart);
+        nb = r.nb;
     });
     QDX_HIP(hipGetLastError());
+    return nb;
 }
 
 // 4 points per thread on grids with nd >= 2 when the plan asks for it
 template <class R, bool ONE_D, int PHASE, bool IMAG>
-static void gspo_row_launch_I(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
-                              const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
-                              R *X) {
+static int gspo_row_launch_I(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                             const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi,
+                             R *X, const R *psi0, double *opart) {
     if constexpr (!ONE_D) {
-        if (plan.ppt == 4) return gspo_row_launch_IP<R, ONE_D, PHASE, IMAG, 4>(s, g, S, kin, tau, plan, pre, post, psi, X);
+        if (plan.ppt == 4)
+            return gspo_row_launch_IP<R, ONE_D, PHASE, IMAG, 4>(s, g, S, kin, tau, plan, pre, post, psi, X, psi0,
+                                                                 opart);
     }
     if (plan.ppt != 8) throw std::runtime_error("GSPO rows: 8 or 4 points per thread (4 for nd >= 2)");
-    gspo_row_launch_IP<R, ONE_D, PHASE, IMAG, 8>(s, g, S, kin, tau, plan, pre, post, psi, X);
+    return gspo_row_launch_IP<R, ONE_D, PHASE, IMAG, 8>(s, g, S, kin, tau, plan, pre, post, psi, X, psi0, opart);
 }
 
 // imag: imaginary time (FP64 only)
 template <class R, bool ONE_D, int PHASE>
-static void gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
-                            const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X,
-                            int imag = 0) {
-    if (!imag) return gspo_row_launch_I<R, ONE_D, PHASE, false>(s, g, S, kin, tau, plan, pre, post, psi, X);
+static int gspo_row_launch(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
+                           const LinePlan &plan, const XGroupDev<R> &pre, const XGroupDev<R> &post, R *psi, R *X,
+                           int imag = 0, const R *psi0 = nullptr, double *opart = nullptr) {
+    if (!imag) return gspo_row_launch_I<R, ONE_D, PHASE, false>(s, g, S, kin, tau, plan, pre, post, psi, X, psi0, opart);
     if constexpr (std::is_same<R, double>::value)
-        return gspo_row_launch_I<R, ONE_D, PHASE, true>(s, g, S, kin, tau, plan, pre, post, psi, X);
+        return gspo_row_launch_I<R, ONE_D, PHASE, true>(s, g, S, kin, tau, plan, pre, post, psi, X, psi0, opart);
     throw std::runtime_error("imaginary time: FP64 only");
 }
 
@@ -241,6 +292,13 @@ void Launch<R>::gspo_row_post(hipStream_t s, const GridDev &g, int S, const Line
     HamDev<R> none;
     XGroupDev<R> empty;
     gspo_row_launch<R, false, 1>(s, g, S, none, 0.0, p0, empty, post, psi, const_cast<R *>(X), imag);
+}
+
+template <class R>
+int Launch<R>::gspo_row_bridge(hipStream_t s, const GridDev &g, int S, const LinePlan &p0, const XGroupDev<R> &post,
+                               const XGroupDev<R> &pre, R *X, const R *psi0, double *opart, int imag) {
+    HamDev<R> none;
+    return gspo_row_launch<R, false, 2>(s, g, S, none, 0.0, p0, pre, post, nullptr, X, imag, psi0, opart);
 }
 
 template <class R>
@@ -278,7 +336,10 @@ void Launch<R>::gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupD
                                            const XGroupDev<R> &, const R *, R *, int);                              \
     template void Launch<R>::gspo_1d(hipStream_t, const GridDev &, int, const LinePlan &, const HamDev<R> &,       \
                                      double, const XGroupDev<R> &, const XGroupDev<R> &, R *, int);                 \
-    template void Launch<R>::gspo_xonly(hipStream_t, const GridDev &, int, const XGroupDev<R> &, R *, int);
+    template void Launch<R>::gspo_xonly(hipStream_t, const GridDev &, int, const XGroupDev<R> &, R *, int);       \
+    template int Launch<R>::gspo_row_bridge(hipStream_t, const GridDev &, int, const LinePlan &,                   \
+                                            const XGroupDev<R> &, const XGroupDev<R> &, R *, const R *, double *,  \
+                                            int);
 QDX_INST_GSPO(double)
 QDX_INST_GSPO(float)
 

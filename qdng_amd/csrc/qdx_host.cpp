@@ -289,6 +289,93 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
     if (pending.n > 0) Launch<R>::gspo_xonly(s, g, p->S, pending, psi, p->imag);
 }
 
+// GSPO runs of several steps on grids with nd >= 2 whose sequence holds one
+// kinetic split operator (x-space groups before and after it): psi stays
+// transformed along dimension 0 in X between the steps, and one row pass
+// ends step n and starts step n + 1 (post of n, pre of n + 1).  Per step: the
+// strided passes and one row pass, plus one row pass at the start.
+struct GspoFused {
+    bool ok = false;
+    size_t kpos = 0;   // index of the kinetic operator in the sequence
+};
+
+GspoFused gspo_fusable(qdx_prop *p) {
+    GspoFused f;
+    if (p->kind != qdx_prop::GSPO || p->grid.nd < 2) return f;
+    const auto seq = gspo_sequence(p);
+    int nkin = 0;
+    for (size_t i = 0; i < seq.size(); ++i)
+        if (seq[i].kinetic) {
+            ++nkin;
+            f.kpos = i;
+        }
+    f.ok = nkin == 1 && f.kpos <= (size_t)kMaxXOps && seq.size() - f.kpos - 1 <= (size_t)kMaxXOps;
+    return f;
+}
+
+// steps first .. first + nsteps - 1 of a fusable GSPO run on psi (complex R);
+// obs_part/obs: when given, the Reporter sums of the state before steps
+// 1 .. nsteps - 1 go to obs + i * per_step (as qdx_prop_run lays them out)
+template <class R>
+void gspo_run_fused(qdx_prop *p, R *psi, const R *psi0, long first, long nsteps, double *obs, size_t per_step) {
+    qdx_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const bool fp32 = std::is_same<R, float>::value;
+    const GridDev g = p->grid.dev();
+    LinePlan plans[3];
+    const int ppt = cheby_line_points(ctx, g, p->S);
+    for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan_ppt(p->grid.N[d], fp32, ppt);
+    const auto seq = gspo_sequence(p);
+    const size_t kpos = gspo_fusable(p).kpos;
+    auto group = [&](size_t a, size_t b, long step) {
+        XGroupDev<R> grp;
+        grp.n = 0;
+        for (size_t i = a; i < b; ++i) grp.op[grp.n++] = xsplit<R>(seq[i].op, seq[i].tau, step);
+        return grp;
+    };
+    HamDev<R> kin;
+    if constexpr (std::is_same<R, float>::value) kin = seq[kpos].op->dev.h32;
+    else kin = seq[kpos].op->dev.h64;
+    const double tau = seq[kpos].tau;
+    R *X = p->X.as<R>();
+    double *opart = obs ? p->partials.as<double>() : nullptr;
+    {
+        KTimer t(ctx, QDX_K_GSPO_ROW);
+        Launch<R>::gspo_row_pre(s, g, p->S, plans[0], group(0, kpos, first), psi, X, p->imag);
+    }
+    for (long i = 0; i < nsteps; ++i) {
+        const long step = first + i;
+        auto col = [&](int dim, int mode) {
+            KTimer t(ctx, QDX_K_LINE_PASS);
+            Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, tau, X, nullptr, ConvCheck(), p->imag);
+        };
+        if (p->grid.nd == 2) {
+            col(1, 3);
+        } else {
+            col(1, 0);
+            col(2, 3);
+            col(1, 1);
+        }
+        const XGroupDev<R> post = group(kpos + 1, seq.size(), step);
+        if (i == nsteps - 1) {
+            KTimer t(ctx, QDX_K_GSPO_ROW);
+            Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi, p->imag);
+            break;
+        }
+        int nb;
+        {
+            KTimer t(ctx, QDX_K_GSPO_ROW);
+            nb = Launch<R>::gspo_row_bridge(s, g, p->S, plans[0], post, group(0, kpos, step + 1), X, psi0, opart,
+                                            p->imag);
+        }
+        if (opart) {
+            double *o = obs + (i + 1) * per_step;
+            launch_sum_partials(s, opart, nb, p->S, o);                         // <psi|psi>
+            launch_sum_partials(s, opart + 2 * p->S * nb, nb, p->S, o + 2 * p->S);   // <psi|psi0>
+        }
+    }
+}
+
 // |norm0 - accnorm| / norm0 of the last executed order of the last step
 // (imaginary time: |accnorm - accnorm of the order before| / the latter),
 // from the partial sums the kernels stored
@@ -1036,7 +1123,27 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
         }
         double *part = p->partials.as<double>();
         std::vector<int> sil_counts;
-        for (int64_t i = 0; i < nsteps; ++i) {
+        const bool fused = nsteps > 0 && gspo_fusable(p).ok;
+        if (fused) {
+            // observables before step 0 here; those before the later steps
+            // from the row passes between the steps (psi0: autocorrelation)
+            double *o = obs;
+            const bool want = norm2 || autocorr;
+            if (g32) {
+                float *ps = p->psi32.as<float>();
+                const float *p0 = autocorr ? p->psi0_32.as<float>() : ps;
+                if (norm2) launch_overlap_c64(s, ps, ps, n, S, part, o);
+                if (autocorr) launch_overlap_c64(s, ps, p0, n, S, part, o + 2 * S);
+                gspo_run_fused<float>(p, ps, p0, first_step, nsteps, want ? obs : nullptr, per_step);
+            } else {
+                double *ps = psi->ptr();
+                const double *p0 = autocorr ? psi0->ptr() : ps;
+                if (norm2) launch_overlap(s, ps, ps, n, S, part, o);
+                if (autocorr) launch_overlap(s, ps, p0, n, S, part, o + 2 * S);
+                gspo_run_fused<double>(p, ps, p0, first_step, nsteps, want ? obs : nullptr, per_step);
+            }
+        }
+        for (int64_t i = 0; i < (fused ? 0 : nsteps); ++i) {
             double *o = obs + i * per_step;
             if (norm2) {
                 if (g32) launch_overlap_c64(s, p->psi32.as<float>(), p->psi32.as<float>(), n, S, part, o);

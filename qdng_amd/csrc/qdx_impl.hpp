@@ -190,6 +190,13 @@ struct Launch {
                              const XGroupDev<R> &pre, const R *psi_in, R *X, int imag = 0);
     static void gspo_row_post(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
                               const XGroupDev<R> &post, const R *X, R *psi_out, int imag = 0);
+    // between two steps of a run (psi held in X, FFT_0 applied): IFFT_0,
+    // post of step n, pre of step n + 1, FFT_0.  opart (nullable): per block
+    // and state sum |psi|^2 and sum conj(psi) psi0 of the state between the
+    // steps (2 S nb (re, im) pairs, see k_gspo_row); returns nb
+    static int gspo_row_bridge(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
+                               const XGroupDev<R> &post, const XGroupDev<R> &pre, R *X, const R *psi0,
+                               double *opart, int imag = 0);
     static void gspo_1d(hipStream_t s, const GridDev &g, int S, const LinePlan &p0,
                         const HamDev<R> &kin, double tau_kin, const XGroupDev<R> &pre,
                         const XGroupDev<R> &post, R *psi, int imag = 0);
@@ -214,6 +221,9 @@ void launch_convert_c64_to_c128(hipStream_t s, const float *src, double *dst, lo
 // out[s] = sum_j conj(a_sj) b_sj  (re, im pairs), S <= kMaxStates
 void launch_overlap(hipStream_t s, const double *a, const double *b, long n, int S, double *partials,
                     double *out);
+// out[2 s], out[2 s + 1] = the sums over b < nbx of partials[2 (s nbx + b)],
+// partials[2 (s nbx + b) + 1], in the fixed order of launch_overlap
+void launch_sum_partials(hipStream_t s, const double *partials, int nbx, int S, double *out);
 void launch_overlap_c64(hipStream_t s, const float *a, const float *b, long n, int S,
                         double *partials, double *out);
 

@@ -148,6 +148,11 @@ static void overlap_impl(hipStream_t s, const R *a, const R *b, long n, int S, d
     QDX_HIP(hipGetLastError());
 }
 
+void launch_sum_partials(hipStream_t s, const double *partials, int nbx, int S, double *out) {
+    QDX_LAUNCH(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
+    QDX_HIP(hipGetLastError());
+}
+
 void launch_wnorm(hipStream_t s, const double *a, long n, int S, const GridDev &g, int dim, double base, double step,
                   const double *ktab, double *partials, double *out) {
     if (dim < 0 || dim >= g.nd) throw std::runtime_error("dimension out of range");

@@ -368,6 +368,46 @@ def test_gspo_potential_first_ordering(ctx):
     assert rel(w.download(), psi) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["coupled_2d", "laser_2d", "anharmonic_3d"])
+def test_gspo_run_fused_matches_single_steps(ctx, name, precision):
+    # qdx_prop_run keeps psi transformed along dimension 0 between the steps
+    # and ends step n and starts step n + 1 in one row pass: the same
+    # arithmetic in the same order as one qdx_prop_apply per step (three
+    # passes), so the states agree bit for bit; the observables before each
+    # step come from that row pass and agree with the separate reductions
+    import qdng_amd as Q
+    note(f"gspo fused run vs single steps {name} {precision}")
+    if name == "coupled_2d":
+        p, ops = cases.morse_2state_2d(64), (["kin"], ["pot"])
+    elif name == "laser_2d":
+        p, ops = cases.driven_job([64, 96], np.random.default_rng(5), 12), (["kin"], ["pot", "dip"])
+    else:
+        p, ops = cases.anharmonic_3d(16), (["kin"], ["pot"])
+    steps = 7
+    G = Q.GSPO([p.gpu_split_op(ctx, k) for k in ops], 2.0, precision=precision)
+    w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
+    n2, ac, _, _ = G.run(w, 0, steps, w0)
+    v = p.gpu_wf(ctx)
+    n2_ref, ac_ref = [], []
+    for i in range(steps):
+        n2_ref.append(v.norm()[0] ** 2)
+        ac_ref.append(v.overlap(w0))
+        G.apply(v, i)
+    assert np.array_equal(w.download(), v.download())
+    # FP32: the run measures the state before step 0 (and psi0) rounded to
+    # complex64, the single steps psi0 itself; from step 1 on both see the
+    # same complex64 states
+    lo = 0 if precision == "fp64" else 1
+    n2, n2_ref, ac, ac_ref = (np.asarray(a) for a in (n2, n2_ref, ac, ac_ref))
+    dn = np.max(np.abs(n2[lo:] - n2_ref[lo:]) / np.abs(n2_ref[lo:]))
+    da = np.max(np.abs(ac[lo:] - ac_ref[lo:]))
+    note(f"  norm^2 rel diff {dn:.2e}, autocorrelation diff {da:.2e}")
+    assert dn < 1e-13
+    assert da < (1e-13 if precision == "fp64" else 1e-7)   # FP32: psi0 rounded in the run
+    assert np.allclose(n2[:lo], n2_ref[:lo], rtol=1e-7) and np.allclose(ac[:lo], ac_ref[:lo], atol=1e-7)
+
+
 def test_gspo_fp32_bulk(ctx):
     note("gspo fp32-bulk")
     import qdng_amd as Q

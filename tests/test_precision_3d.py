@@ -3,7 +3,7 @@ Chebyshev at order ~64 (dt 3.5) and ~256 (dt 37), 20 steps, the bulk paths
 against the plain FP64 path on the same complex64 initial state.
 
 - "compact": FP64 arithmetic with the recurrence terms and the kinetic pass
-  buffer stored in 40 bits per component (relative rounding 2^-32); it meets
+  buffer stored in 40 bits per component (relative rounding 2^-29); it meets
   the 1e-6 relative-L2 bar at both orders.
 - "fp32": FP32 transforms and terms.  Its error is dominated by the FP32
   arithmetic of the transforms (a CPU model with FP64 transforms and FP32
@@ -72,7 +72,7 @@ def test_bulk_vs_fp64_3d_128(ctx, dt, order_lo, order_hi, precision):
     if precision == "compact":
         # the difference is mostly the FP64 series' truncation: it stops at
         # QDng's norm test (prec 1e-10), the compact series runs its fixed
-        # length (a few more terms); the 40-bit storage adds ~1e-9
+        # length (a few more terms); the 40-bit storage adds ~1e-8
         assert err < BAR
         assert driftb < 1e-8
         assert dac < 1e-7
