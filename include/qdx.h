@@ -271,6 +271,14 @@ int qdx_prop_apply(qdx_prop *p, qdx_wf *psi, int64_t step, qdx_step_stats *stats
 int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, qdx_wf *psi0,
                  double *norm2, double *autocorr, int32_t *n_hpsi_per_step,
                  qdx_step_stats *stats);
+/* OPropagator::Forward() / Backward() (src/qdlib/OPropagator.h:62-100) of a
+ * real-time propagator: backward = 1 gives the exponent +i dt (the operators
+ * conjugated, as QDng's Backward sets them); imaginary-time propagators run
+ * backward (exponent -dt) only and reject forward with QDX_ERR_INCOMPAT. */
+int qdx_prop_set_direction(qdx_prop *p, int32_t backward);
+/* OPropagator::Exponent() (src/qdlib/OPropagator.h:148-155): c of
+ * exp(c H) for one step, as (re, im). */
+int qdx_prop_exponent(qdx_prop *p, double out[2]);
 int qdx_prop_destroy(qdx_prop *p);
 
 /* ---- batched jobs ---------------------------------------------------------

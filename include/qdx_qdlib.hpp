@@ -18,7 +18,10 @@
 //   OGridDipole (qdlib/OGridDipole.h)            OGridDipole      + laser samples
 //   OSum (qdlib/OSum.h)                          OSum             Add(Operator*)
 //   OMultistate (qdlib/OMultistate.h)            OMultistate      Add(Operator*, i, j), "unity"
+//   Operator (qdlib/Operator.h)                  Operator         virtual interface
+//   OPropagator (qdlib/OPropagator.h)            OPropagator      Exponent, Forward, Backward, ...
 //   OCheby (qdlib/OCheby.h)                      OCheby           Add("hamiltonian", H)
+//   OSIL, OArnoldi (qdlib/OSIL.h, OArnoldi.h)    OSIL, OArnoldi   Add("hamiltonian", H)
 //   OGSPO (qdlib/OGSPO.h)                        OGSPO            Add("A"/"B"/..., op)
 //   EParamProblem ... (tools/Exception.h)        same names
 #ifndef QDX_QDLIB_HPP
@@ -219,9 +222,14 @@ class WaveFunction {
     }
 
   public:
+    WaveFunction() = default;
+    WaveFunction(const WaveFunction &) = delete;
     virtual ~WaveFunction() {
         if (wf_) qdx_wf_destroy(wf_);
     }
+    // a new vector of the same kind on the same grid, values zero
+    // (WaveFunction::NewInstance, qdlib/WaveFunction.h)
+    virtual WaveFunction *NewInstance() const = 0;
     long size() const { return n_ * S_; }
     int States() const { return S_; }
     // the device vector, current (host changes uploaded)
@@ -291,7 +299,7 @@ class WFGridCartesian : public WaveFunction, public GridSystem {
         alloc(GridSystem::g_, 1);
     }
     // a new vector on the same grid, values zero
-    WFGridCartesian *NewInstance() const {
+    WFGridCartesian *NewInstance() const override {
         auto *w = new WFGridCartesian();
         w->GridSystem::g_ = GridSystem::g_;
         w->alloc(GridSystem::g_, 1);
@@ -303,7 +311,7 @@ class WFGridCartesian : public WaveFunction, public GridSystem {
 class WFMultistate : public WaveFunction {
   public:
     void Init(const qdx_grid &g, int states) { alloc(g, states); }
-    WFMultistate *NewInstance() const {
+    WFMultistate *NewInstance() const override {
         auto *w = new WFMultistate();
         w->alloc(g_, S_);
         return w;
@@ -313,31 +321,120 @@ class WFMultistate : public WaveFunction {
 };
 
 // ---------------------------------------------------------------------------
-// operators
+// operators: QDng's polymorphic interface (qdlib/Operator.h:60-374).  Code
+// that holds an Operator* or an OPropagator* drives any of the classes below
+// through these virtual methods.
 class Operator {
   protected:
     ParamContainer params_;
     QDClock *clock_ = nullptr;
+    bool conj_ = false;
+    bool hermitian_ = true;
+    long step() const { return clock_ ? clock_->TimeStep() : 0; }
 
   public:
     virtual ~Operator() = default;
+    virtual Operator *NewInstance() const = 0;                      // Operator.h:142
     virtual const string &Name() const = 0;
-    virtual void Init(ParamContainer &pm) { params_ = pm; }
+    virtual void Init(ParamContainer &pm) { params_ = pm; }         // Operator.h:149
+    virtual void Init(WaveFunction *psi) = 0;                        // Operator.h:161
     ParamContainer &Params() { return params_; }
-    void Clock(QDClock *c) { clock_ = c; }
+    virtual void Clock(QDClock *c) { clock_ = c; }                   // Operator.h:190
     QDClock *Clock() const { return clock_; }
+    virtual void UpdateTime() {}                                     // Operator.h:183
+    bool Hermitian() const { return hermitian_; }                   // Operator.h:206
+    bool Conj() const { return conj_; }
+    virtual void Conj(bool c) { conj_ = c; }                         // Operator.h:219
+    virtual dcomplex Emax() = 0;                                     // Operator.h:259
+    virtual dcomplex Emin() = 0;                                     // Operator.h:266
+    virtual void Apply(WaveFunction *dst, WaveFunction *src) = 0;    // Operator.h:276
+    // in place (Operator.h:281): through a copy of psi
+    virtual void Apply(WaveFunction *psi) {
+        std::unique_ptr<WaveFunction> buf(psi->NewInstance());
+        *buf = *psi;
+        Apply(psi, buf.get());
+    }
+    // dst += O src (Operator.h:307)
+    virtual void ApplyAdd(WaveFunction *dst, WaveFunction *src) {
+        std::unique_ptr<WaveFunction> buf(src->NewInstance());
+        Apply(buf.get(), src);
+        check(qdx_wf_axpy(dst->device(), 1.0, 0.0, buf->device()));
+        dst->DeviceChanged();
+    }
+    // <bra|O|ket> (Operator.h:226)
+    virtual dcomplex MatrixElement(WaveFunction *bra, WaveFunction *ket) {
+        std::unique_ptr<WaveFunction> buf(ket->NewInstance());
+        Apply(buf.get(), ket);
+        return *bra * *buf;
+    }
+    // Re <psi|O|psi> (Operator.h:249)
+    virtual double Expec(WaveFunction *psi) { return MatrixElement(psi, psi).real(); }
+    virtual bool Valid(WaveFunction *psi) { return psi != nullptr; }   // Operator.h:361
 };
 
-// grid terms are leaves: they add themselves to a qdx Hamiltonian element
-class GridOperator : public Operator {
+// operators held on the device as one qdx Hamiltonian, built on the first
+// Init(WaveFunction*) from the terms the class contributes
+class DeviceOperator : public Operator {
+  protected:
+    qdx_ham *ham_ = nullptr;
+    qdx_grid g_{};
+    virtual bool multistate() const { return false; }
+    virtual bool unity() const { return true; }
+    virtual int states() const { return 1; }
+    void need() const {
+        if (!ham_) throw EParamProblem(Name() + " used before Init(WaveFunction*)");
+    }
+
   public:
-    virtual void AddTo(qdx_ham *h, int row, int col) const = 0;
+    ~DeviceOperator() override {
+        if (ham_) qdx_ham_destroy(ham_);
+    }
+    // the operator's terms at matrix element (row, col) of h (a single
+    // operator at (0, 0); composites place their children)
+    virtual void AddTerms(qdx_ham *h, int row, int col) const = 0;
+    using Operator::Init;
+    void Init(WaveFunction *psi) override {
+        if (!psi) throw EParamProblem("Invalid WaveFunction");
+        if (ham_) return;
+        g_ = psi->grid();
+        if (psi->States() != states()) throw EIncompatible("operator and wave function differ in the number of states");
+        check(qdx_ham_create(context(), &g_, states(), multistate() ? 1 : 0, unity() ? 1 : 0, &ham_));
+        AddTerms(ham_, 0, 0);
+    }
+    qdx_ham *ham() const { return ham_; }
+    dcomplex Emax() override {
+        need();
+        double lo, hi;
+        check(qdx_ham_bounds(ham_, &lo, &hi));
+        return hi;
+    }
+    dcomplex Emin() override {
+        need();
+        double lo, hi;
+        check(qdx_ham_bounds(ham_, &lo, &hi));
+        return lo;
+    }
+    void Apply(WaveFunction *dst, WaveFunction *src) override {
+        Init(src);
+        check(qdx_ham_apply(ham_, dst->device(), src->device(), step()));
+        dst->DeviceChanged();
+    }
+    using Operator::Apply;
+    double Expec(WaveFunction *psi) override {
+        Init(psi);
+        double e = 0;
+        check(qdx_ham_expec(ham_, psi->device(), step(), &e));
+        return e;
+    }
 };
 
-class OGridNablaSq : public GridOperator {
+// grid operators (single-state leaves); in a composite they add their term at
+// the element they are placed on
+class OGridNablaSq : public DeviceOperator {
     std::vector<double> mass_;
 
   public:
+    OGridNablaSq *NewInstance() const override { return new OGridNablaSq(); }
     const string &Name() const override {
         static const string n = "OGridNablaSq";
         return n;
@@ -350,26 +447,29 @@ class OGridNablaSq : public GridOperator {
         for (double m : mass_)
             if (m == 0) throw EParamProblem("Zero mass defined");
     }
-    void AddTo(qdx_ham *h, int row, int col) const override {
+    using DeviceOperator::Init;
+    void AddTerms(qdx_ham *h, int row, int col) const override {
         if (row != col) throw EIncompatible("kinetic operator on an off-diagonal element");
+        if (mass_.empty()) throw EParamProblem("No mass defined");
         std::vector<double> m(mass_);
         m.resize(4, -1.0);
         check(qdx_ham_add_kinetic(h, row, m.data()));
     }
 };
 
-class OGridPotential : public GridOperator {
+class OGridPotential : public DeviceOperator {
   protected:
     std::vector<double> v_;
 
   public:
+    OGridPotential *NewInstance() const override { return new OGridPotential(); }
     const string &Name() const override {
         static const string n = "OGridPotential";
         return n;
     }
     void SetData(const std::vector<double> &v) { v_ = v; }
     double &operator[](long i) { return v_[i]; }
-    void AddTo(qdx_ham *h, int row, int col) const override {
+    void AddTerms(qdx_ham *h, int row, int col) const override {
         if (v_.empty()) throw EParamProblem("GridPotential without data");
         check(qdx_ham_add_potential(h, row, col, v_.data()));
     }
@@ -381,95 +481,49 @@ class OGridDipole : public OGridPotential {
     double scale_ = 1.0;
 
   public:
+    OGridDipole *NewInstance() const override { return new OGridDipole(); }
     const string &Name() const override {
         static const string n = "OGridDipole";
         return n;
     }
     std::vector<double> &GetLaser() { return laser_; }
     void Scale(double s) { scale_ = s; }
-    void AddTo(qdx_ham *h, int row, int col) const override {
+    void AddTerms(qdx_ham *h, int row, int col) const override {
         if (v_.empty() || laser_.empty()) throw EParamProblem("GridDipole without dipole or laser");
         check(qdx_ham_add_dipole(h, row, col, v_.data(), laser_.data(), (int32_t)laser_.size(), scale_));
     }
 };
 
-// composite operators build a qdx Hamiltonian on first use
-class OComposite : public Operator {
-  protected:
-    qdx_ham *ham_ = nullptr;
-    qdx_grid g_{};
-    int S_ = 1;
-    virtual void build(qdx_ham *h) const = 0;
-    virtual bool multistate() const = 0;
-    virtual bool unity() const { return true; }
-    virtual int states() const { return 1; }
+// OSum (qdlib/OSum.cpp): the sum of its operators on one state
+class OSum : public DeviceOperator {
+    std::vector<const DeviceOperator *> ops_;
 
   public:
-    ~OComposite() override {
-        if (ham_) qdx_ham_destroy(ham_);
-    }
-    // Operator::Init(WaveFunction*): bind to the wave function's grid
-    void Init(WaveFunction *psi) {
-        if (ham_) return;
-        g_ = psi->grid();
-        S_ = states();
-        if (psi->States() != S_) throw EIncompatible("operator and wave function differ in the number of states");
-        check(qdx_ham_create(context(), &g_, S_, multistate() ? 1 : 0, unity() ? 1 : 0, &ham_));
-        build(ham_);
-    }
-    using Operator::Init;
-    qdx_ham *ham() const { return ham_; }
-    dcomplex Emax() {
-        double lo, hi;
-        check(qdx_ham_bounds(ham_, &lo, &hi));
-        return hi;
-    }
-    dcomplex Emin() {
-        double lo, hi;
-        check(qdx_ham_bounds(ham_, &lo, &hi));
-        return lo;
-    }
-    void Apply(WaveFunction *dst, WaveFunction *src) {
-        Init(src);
-        check(qdx_ham_apply(ham_, dst->device(), src->device(), clock_ ? clock_->TimeStep() : 0));
-        dst->DeviceChanged();
-    }
-    // <psi|H|psi>
-    dcomplex Expec(WaveFunction *psi) {
-        Init(psi);
-        double e = 0;
-        check(qdx_ham_expec(ham_, psi->device(), clock_ ? clock_->TimeStep() : 0, &e));
-        return e;
-    }
-};
-
-class OSum : public OComposite {
-    std::vector<const GridOperator *> ops_;
-    void build(qdx_ham *h) const override {
-        if (ops_.empty()) throw EParamProblem("Sum Operator is empty");
-        for (auto *o : ops_) o->AddTo(h, 0, 0);
-    }
-    bool multistate() const override { return false; }
-
-  public:
+    OSum *NewInstance() const override { return new OSum(); }
     const string &Name() const override {
         static const string n = "OSum";
         return n;
     }
-    void Add(const GridOperator *o) { ops_.push_back(o); }
+    void Add(Operator *o) {
+        auto *d = dynamic_cast<const DeviceOperator *>(o);
+        if (!d) throw EIncompatible("OSum takes grid operators");
+        ops_.push_back(d);
+    }
     int Size() const { return (int)ops_.size(); }
+    void AddTerms(qdx_ham *h, int row, int col) const override {
+        if (ops_.empty()) throw EParamProblem("Sum Operator is empty");
+        for (auto *o : ops_) o->AddTerms(h, row, col);
+    }
 };
 
-class OMultistate : public OComposite {
+// OMultistate (qdlib/OMultistate.cpp): operators at matrix elements (i, j);
+// "unity" (default true) puts the identity on empty diagonal elements
+class OMultistate : public DeviceOperator {
     struct E {
-        const GridOperator *op;
+        const DeviceOperator *op;
         int i, j;
     };
     std::vector<E> elems_;
-    void build(qdx_ham *h) const override {
-        if (elems_.empty()) throw EParamProblem("Multistate operator is empty");
-        for (auto &e : elems_) e.op->AddTo(h, e.i, e.j);
-    }
     bool multistate() const override { return true; }
     int states() const override {
         int n = 0;
@@ -483,49 +537,117 @@ class OMultistate : public OComposite {
     }
 
   public:
+    OMultistate *NewInstance() const override { return new OMultistate(); }
     const string &Name() const override {
         static const string n = "OMultistate";
         return n;
     }
-    void Add(const GridOperator *o, int i, int j) { elems_.push_back({o, i, j}); }
+    void Add(Operator *o, int i, int j) {
+        auto *d = dynamic_cast<const DeviceOperator *>(o);
+        if (!d) throw EIncompatible("OMultistate takes grid operators");
+        elems_.push_back({d, i, j});
+    }
+    void AddTerms(qdx_ham *h, int row, int col) const override {
+        if (elems_.empty()) throw EParamProblem("Multistate operator is empty");
+        if (row != 0 || col != 0) throw EIncompatible("a Multistate operator inside another one");
+        for (auto &e : elems_) e.op->AddTerms(h, e.i, e.j);
+    }
 };
 
 // ---------------------------------------------------------------------------
-// propagators (OPropagator, OCheby, OGSPO)
+// propagators (qdlib/OPropagator.h:23-186): exponent c of exp(c H) per step,
+// -i dt forward and +i dt backward in real time; imaginary time runs
+// backward (exponent -dt, as ProgEigen sets it up)
 class OPropagator : public Operator {
   protected:
     qdx_prop *prop_ = nullptr;
+    bool forward_ = true, imaginary_ = false;
+    Operator *H_ = nullptr;
+    void need_clock() const {
+        if (!clock_) throw EIncompatible("Propagator has no clock");
+    }
+    void sync_direction() {
+        if (prop_ && !imaginary_) check(qdx_prop_set_direction(prop_, forward_ ? 0 : 1));
+    }
+    // QDng's init guards (OPropagator.h:171-184)
+    void check_init(WaveFunction *psi) const {
+        if (!clock_) throw EParamProblem("Progagator is missing a clock");
+        if (clock_->Dt() == 0) throw EParamProblem("No time step defined");
+        if (!psi) throw EParamProblem("Invalid WaveFunction");
+        if (imaginary_ && forward_) throw EIncompatible("imaginary time runs backward (exponent -dt) only");
+    }
 
   public:
     ~OPropagator() override {
         if (prop_) qdx_prop_destroy(prop_);
     }
+    void Forward() {
+        need_clock();
+        forward_ = true;
+        conj_ = false;
+        sync_direction();
+    }
+    void Backward() {
+        need_clock();
+        forward_ = false;
+        conj_ = true;
+        sync_direction();
+    }
+    void ImaginaryTime() {
+        if (prop_) throw EIncompatible("set imaginary time before Init(WaveFunction*)");
+        imaginary_ = true;
+    }
+    void RealTime() {
+        if (prop_ && imaginary_) throw EIncompatible("set real time before Init(WaveFunction*)");
+        imaginary_ = false;
+    }
+    void Conj(bool c) override { c ? Backward() : Forward(); }
+    using Operator::Conj;
+    // OPropagator::Exponent (OPropagator.h:148-155)
+    dcomplex Exponent() const {
+        need_clock();
+        const double dt = clock_->Dt();
+        if (imaginary_) return forward_ ? dt : -dt;
+        return forward_ ? dcomplex(0, -dt) : dcomplex(0, dt);
+    }
+    bool ImaginaryTimeSet() const { return imaginary_; }
+    Operator *Hamiltonian() { return H_; }
+    // propagators have no spectral bounds (OPropagator.h:157-159)
+    dcomplex Emax() override { return 0; }
+    dcomplex Emin() override { return 0; }
     // one time step at the clock's step
-    void Apply(WaveFunction *psi) {
+    void Apply(WaveFunction *psi) override {
         if (!prop_) Init(psi);
         qdx_step_stats st;
-        check(qdx_prop_apply(prop_, psi->device(), clock_ ? clock_->TimeStep() : 0, &st));
+        check(qdx_prop_apply(prop_, psi->device(), step(), &st));
         psi->DeviceChanged();
     }
-    void Apply(WaveFunction *dst, WaveFunction *src) {
+    void Apply(WaveFunction *dst, WaveFunction *src) override {
         *dst = *src;
         Apply(dst);
     }
-    virtual void Init(WaveFunction *psi) = 0;
     using Operator::Init;
+    qdx_prop *prop() const { return prop_; }
 };
 
+// the Hamiltonian of a propagator given as one device operator
+inline DeviceOperator *as_device(Operator *o, const char *what) {
+    auto *d = dynamic_cast<DeviceOperator *>(o);
+    if (!d) throw EIncompatible(string(what) + ": the operator must be a grid operator or a Sum/Multistate of them");
+    return d;
+}
+
 class OCheby : public OPropagator {
-    OComposite *H_ = nullptr;
     int precision_ = QDX_FP64;
 
   public:
+    OCheby *NewInstance() const override { return new OCheby(); }
     const string &Name() const override {
         static const string n = "OCheby";
         return n;
     }
     // Add("hamiltonian", H)
-    void Add(const string &key, OComposite *H) {
+    void Add(const string &key, Operator *H) {
         if (key != "hamiltonian") throw EParamProblem("OCheby takes a hamiltonian");
         H_ = H;
     }
@@ -534,8 +656,10 @@ class OCheby : public OPropagator {
     // params "order", "prec", "scaling" as OCheby::Init(ParamContainer&)
     void Init(WaveFunction *psi) override {
         if (!H_) throw EParamProblem("Chebychev propagator has no hamiltonian");
-        if (!clock_) throw EParamProblem("No clock");
-        H_->Init(psi);
+        check_init(psi);
+        DeviceOperator *H = as_device(H_, "OCheby");
+        H->Clock(clock_);
+        H->Init(psi);
         int order = 0;
         double prec = -1, scaling = 0;
         params_.GetValue("order", order);
@@ -543,7 +667,9 @@ class OCheby : public OPropagator {
         params_.GetValue("scaling", scaling);
         if (prop_) qdx_prop_destroy(prop_);
         prop_ = nullptr;
-        check(qdx_cheby_create(H_->ham(), clock_->Dt(), order, prec, scaling, precision_, &prop_));
+        if (imaginary_) check(qdx_cheby_create_imag(H->ham(), clock_->Dt(), order, prec, scaling, &prop_));
+        else check(qdx_cheby_create(H->ham(), clock_->Dt(), order, prec, scaling, precision_, &prop_));
+        sync_direction();
         qdx_cheby_info info;
         check(qdx_cheby_info_get(prop_, &info));
         // the values OCheby::Init publishes (OCheby.cpp:443-493)
@@ -555,30 +681,130 @@ class OCheby : public OPropagator {
     }
 };
 
+// Lanczos and Arnoldi: params "order" and "err" / "conv" (OSIL.cpp:79-190,
+// OArnoldi.cpp:64-190)
+class OSIL : public OPropagator {
+  public:
+    OSIL *NewInstance() const override { return new OSIL(); }
+    const string &Name() const override {
+        static const string n = "OSIL";
+        return n;
+    }
+    void Add(const string &key, Operator *H) {
+        if (key != "hamiltonian") throw EParamProblem("OSIL takes a hamiltonian");
+        H_ = H;
+    }
+    using Operator::Init;
+    void Init(WaveFunction *psi) override {
+        if (!H_) throw EParamProblem("SIL propagator has no hamiltonian");
+        check_init(psi);
+        DeviceOperator *H = as_device(H_, "OSIL");
+        H->Clock(clock_);
+        H->Init(psi);
+        int order = 0;
+        double err = 0;
+        params_.GetValue("order", order);
+        params_.GetValue("err", err);
+        if (prop_) qdx_prop_destroy(prop_);
+        prop_ = nullptr;
+        check(qdx_sil_create(H->ham(), clock_->Dt(), order, err, imaginary_ ? 1 : 0, &prop_));
+        sync_direction();
+    }
+};
+
+class OArnoldi : public OPropagator {
+  public:
+    OArnoldi *NewInstance() const override { return new OArnoldi(); }
+    const string &Name() const override {
+        static const string n = "OArnoldi";
+        return n;
+    }
+    void Add(const string &key, Operator *H) {
+        if (key != "hamiltonian") throw EParamProblem("OArnoldi takes a hamiltonian");
+        H_ = H;
+    }
+    using Operator::Init;
+    void Init(WaveFunction *psi) override {
+        if (!H_) throw EParamProblem("Arnoldi propagator has no hamiltonian");
+        check_init(psi);
+        DeviceOperator *H = as_device(H_, "OArnoldi");
+        H->Clock(clock_);
+        H->Init(psi);
+        int order = 0;
+        double conv = 0;
+        params_.GetValue("order", order);
+        params_.GetValue("conv", conv);
+        if (prop_) qdx_prop_destroy(prop_);
+        prop_ = nullptr;
+        check(qdx_arnoldi_create(H->ham(), clock_->Dt(), order, conv, imaginary_ ? 1 : 0, &prop_));
+        sync_direction();
+    }
+};
+
 class OGSPO : public OPropagator {
-    std::map<string, OComposite *> ops_;
+    std::map<string, Operator *> ops_;
     int precision_ = QDX_FP64;
+    // H = A + B + ... of the split operators (OGSPO.cpp:61-105)
+    class SplitSum : public DeviceOperator {
+        std::vector<const DeviceOperator *> ops_;
+        int S_ = 1;
+        bool ms_ = false;
+        bool multistate() const override { return ms_; }
+        bool unity() const override { return false; }
+        int states() const override { return S_; }
+
+      public:
+        SplitSum *NewInstance() const override { return new SplitSum(); }
+        const string &Name() const override {
+            static const string n = "OSum";
+            return n;
+        }
+        void Set(const std::vector<const DeviceOperator *> &ops, int S) {
+            ops_ = ops;
+            S_ = S;
+            ms_ = S > 1;
+        }
+        void AddTerms(qdx_ham *h, int row, int col) const override {
+            for (auto *o : ops_) {
+                if (auto *m = dynamic_cast<const OMultistate *>(o)) m->AddTerms(h, 0, 0);
+                else o->AddTerms(h, row, col);
+            }
+        }
+    };
+    std::unique_ptr<SplitSum> sum_;
 
   public:
+    OGSPO *NewInstance() const override { return new OGSPO(); }
     const string &Name() const override {
         static const string n = "OGSPO";
         return n;
     }
     // operators "A", "B", "C", ... in this order (OGSPO.cpp:49-61)
-    void Add(const string &key, OComposite *op) { ops_[key] = op; }
+    void Add(const string &key, Operator *op) { ops_[key] = op; }
     void Precision(int p) { precision_ = p; }
     using Operator::Init;
     void Init(WaveFunction *psi) override {
         if (ops_.empty()) throw EParamProblem("GSPO needs at least one operator");
-        if (!clock_) throw EParamProblem("No clock");
+        check_init(psi);
         std::vector<qdx_ham *> h;
+        std::vector<const DeviceOperator *> dev;
         for (auto &kv : ops_) {
-            kv.second->Init(psi);
-            h.push_back(kv.second->ham());
+            DeviceOperator *d = as_device(kv.second, "OGSPO");
+            d->Clock(clock_);
+            d->Init(psi);
+            h.push_back(d->ham());
+            dev.push_back(d);
         }
         if (prop_) qdx_prop_destroy(prop_);
         prop_ = nullptr;
-        check(qdx_gspo_create(h.data(), (int32_t)h.size(), clock_->Dt(), precision_, &prop_));
+        if (imaginary_) check(qdx_gspo_create_imag(h.data(), (int32_t)h.size(), clock_->Dt(), &prop_));
+        else check(qdx_gspo_create(h.data(), (int32_t)h.size(), clock_->Dt(), precision_, &prop_));
+        sync_direction();
+        sum_.reset(new SplitSum());
+        sum_->Set(dev, psi->States());
+        sum_->Clock(clock_);
+        sum_->Init(psi);
+        H_ = sum_.get();
     }
 };
 

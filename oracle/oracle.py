@@ -51,6 +51,7 @@ def lib():
         sig = {
             "orc_set_threads": [C.c_int],
             "orc_plant_mistake": [C.c_int],
+            "orc_set_backward": [C.c_int, C.c_int, C.c_int],
             "orc_fft": [C.c_int, i32p, dp, C.c_int],
             "orc_bessel_j": [C.c_int, C.c_double, dp, ip],
             "orc_bessel_i_scaled": [C.c_int, C.c_double, dp, ip],
@@ -267,6 +268,10 @@ class Cheby:
         _chk(lib().orc_cheby_coeffs(self.id, _dp(out)))
         return out
 
+    def backward(self, on: bool = True) -> None:
+        """OPropagator::Backward(): real time with exponent +i dt."""
+        _chk(lib().orc_set_backward(0, self.id, int(bool(on))))
+
     def step(self, psi: np.ndarray, step: int = 0):
         """Advance psi in place by one step; returns the number of H applications."""
         assert psi.dtype == np.complex128 and psi.flags.c_contiguous
@@ -283,6 +288,10 @@ class SIL:
         self.id = _chk(lib().orc_sil_create(H.id, H.nstates, float(dt), int(order), float(err), int(bool(imag))))
         self.H = H
 
+    def backward(self, on: bool = True) -> None:
+        """OPropagator::Backward(): real time with exponent +i dt."""
+        _chk(lib().orc_set_backward(1, self.id, int(bool(on))))
+
     def step(self, psi: np.ndarray, step: int = 0) -> int:
         n = C.c_int(0)
         _chk(lib().orc_sil_step(self.id, int(step), _dp(psi), C.byref(n)))
@@ -297,6 +306,10 @@ class Arnoldi:
         self.id = _chk(lib().orc_arnoldi_create(H.id, H.nstates, float(dt), int(order), float(conv),
                                                 int(bool(imag))))
 
+    def backward(self, on: bool = True) -> None:
+        """OPropagator::Backward(): real time with exponent +i dt."""
+        _chk(lib().orc_set_backward(2, self.id, int(bool(on))))
+
     def step(self, psi: np.ndarray, step: int = 0) -> int:
         n = C.c_int(0)
         _chk(lib().orc_arnoldi_step(self.id, int(step), _dp(psi), C.byref(n)))
@@ -310,6 +323,10 @@ class GSPO:
     def __init__(self, ops, dt: float, nstates: int = 1, imag: bool = False):
         ids = (C.c_int * len(ops))(*[o.id for o in ops])
         self.id = _chk(lib().orc_gspo_create_ex(ids, len(ops), int(nstates), float(dt), int(bool(imag))))
+
+    def backward(self, on: bool = True) -> None:
+        """OPropagator::Backward(): real time with exponent +i dt."""
+        _chk(lib().orc_set_backward(3, self.id, int(bool(on))))
 
     def step(self, psi: np.ndarray, step: int = 0) -> None:
         assert psi.dtype == np.complex128 and psi.flags.c_contiguous

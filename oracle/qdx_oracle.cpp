@@ -497,6 +497,7 @@ struct Cheby {
     std::vector<double> coeff;
     double emin = 0, emax = 0;
     bool imag = false;          /* imaginary time, backward: exponent -dt */
+    bool backward = false;      /* real time backward: exponent +i dt (OPropagator.h:82-100) */
 };
 std::vector<Cheby> g_cheby;
 
@@ -574,7 +575,8 @@ int cheby_step(Cheby &c, int step, cplx *psi) {
     const Op &H = op_at(c.ham);
     const long n = op_size(H) * c.S;
     const cplx I(0, 1);
-    const cplx exponent = -I * c.dt;               /* OPropagator.h:44-55 */
+    /* OPropagator.h:44-100: -i dt forward, +i dt backward */
+    const cplx exponent = c.backward ? I * c.dt : -I * c.dt;
     const cplx exp_ = exponent / c.dt;
     std::vector<cplx> ket0(psi, psi + n), ket1(n), ket2(n), buf(n);
     int napply = 0;
@@ -668,6 +670,7 @@ struct SIL {
     int order = 20;     /* OSIL.h:11 SIL_DEF_ORDER */
     double err = 1e-12; /* OSIL.h:12 SIL_DEF_ERR */
     bool imag = false;
+    bool backward = false;   /* real time backward: exponent +i dt */
 };
 std::vector<SIL> g_sil;
 
@@ -719,7 +722,7 @@ int sil_step(const SIL &c, int step, cplx *psi) {
         if (i + 1 < order) T[(size_t)i * order + i + 1] = T[(size_t)(i + 1) * order + i] = beta[i];
     }
     sym_eigen(order, T, ev, &Z);
-    const cplx expo = c.imag ? cplx(-c.dt, 0) : cplx(0, -c.dt);
+    const cplx expo = c.imag ? cplx(-c.dt, 0) : cplx(0, c.backward ? c.dt : -c.dt);
     std::vector<cplx> vect(order, 0.0);
     for (int k = 0; k < order; ++k) {
         const cplx w = std::exp(expo * ev[k]) * Z[(size_t)0 * order + k];
@@ -741,6 +744,7 @@ struct Arnoldi {
     int maxorder = 20;   /* OArnoldi.h:11 ARNOLDI_DEF_ORDER */
     double conv = 1e-12; /* OArnoldi.h:13 ARNOLDI_CONV */
     bool imag = false;
+    bool backward = false;   /* real time backward: exponent +i dt */
     int order = 20;      /* _order persists between steps (OArnoldi.cpp:44) */
 };
 std::vector<Arnoldi> g_arnoldi;
@@ -814,7 +818,7 @@ int arnoldi_step(Arnoldi &c, int step, cplx *psi) {
     op_apply(H, step, q[order - 1].data(), buf0.data());               /* last column */
     ++napply;
     for (int j = 0; j < order; j++) tHA[(size_t)j * M + order - 1] = overlap(g, c.S, q[j].data(), buf0.data());
-    const cplx expo = c.imag ? cplx(-c.dt, 0) : cplx(0, -c.dt);
+    const cplx expo = c.imag ? cplx(-c.dt, 0) : cplx(0, c.backward ? c.dt : -c.dt);
     std::vector<cplx> A((size_t)order * order);
     for (int i = 0; i < order; ++i)
         for (int j = 0; j < order; ++j) A[(size_t)i * order + j] = expo * tHA[(size_t)i * M + j];
@@ -833,6 +837,7 @@ struct GSPO {
     int S = 1;
     double dt = 0;
     bool imag = false;   /* imaginary time, backward: exponent -dt */
+    bool backward = false;   /* real time backward: exponent +i dt */
 };
 std::vector<GSPO> g_gspo;
 
@@ -923,7 +928,7 @@ void split_apply(const Op &o, int step, cplx c, cplx *psi, int S) {
 
 void gspo_step(const GSPO &p, int step, cplx *psi) {
     const cplx I(0, 1);
-    const cplx exponent = p.imag ? cplx(-p.dt, 0) : -I * p.dt;   /* OPropagator.h:62-108 */
+    const cplx exponent = p.imag ? cplx(-p.dt, 0) : (p.backward ? I : -I) * p.dt;   /* OPropagator.h:62-108 */
     const int L = (int)p.ops.size();
     /* OGSPO.cpp:119-138: ops n-1..1 with c/2, then 0 (full) .. n-1 (c/2) */
     for (int i = L - 1; i > 0; --i) split_apply(op_at(p.ops[i]), step, exponent * 0.5, psi, p.S);
@@ -945,6 +950,26 @@ const char *orc_last_error(void) { return g_err.c_str(); }
 int orc_plant_mistake(int code) {
     g_plant = code;
     return 0;
+}
+
+/* OPropagator::Backward() / Forward() (OPropagator.h:62-100) on a real-time
+ * propagator: kind 0 Chebyshev, 1 SIL, 2 Arnoldi, 3 GSPO.  Backward real time
+ * has the exponent +i dt; the operators here are real, so their complex
+ * conjugates (Operator::Conj, set by Backward) are themselves. */
+int orc_set_backward(int kind, int id, int backward) {
+    return guarded([&] {
+        auto chk = [&](bool imag) {
+            if (imag) throw OracleError("imaginary-time propagators keep their direction");
+        };
+        switch (kind) {
+        case 0: chk(g_cheby.at(id).imag); g_cheby.at(id).backward = backward != 0; break;
+        case 1: chk(g_sil.at(id).imag); g_sil.at(id).backward = backward != 0; break;
+        case 2: chk(g_arnoldi.at(id).imag); g_arnoldi.at(id).backward = backward != 0; break;
+        case 3: chk(g_gspo.at(id).imag); g_gspo.at(id).backward = backward != 0; break;
+        default: throw OracleError("unknown propagator kind");
+        }
+        return 0;
+    });
 }
 
 int orc_set_threads(int n) {

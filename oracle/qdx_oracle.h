@@ -37,7 +37,9 @@ extern "C" {
 /* ---- plumbing -------------------------------------------------------- */
 const char *orc_last_error(void);
 int orc_set_threads(int n);
-int orc_plant_mistake(int code);   /* test hook: 1 = transposed GSPO eigenvectors */
+int orc_plant_mistake(int code);
+/* real-time propagator direction: kind 0 Cheby, 1 SIL, 2 Arnoldi, 3 GSPO */
+int orc_set_backward(int kind, int id, int backward);   /* test hook: 1 = transposed GSPO eigenvectors */
 
 /* ---- FFT (dimension 0 fastest, unnormalised, sign -1 forward) -------- */
 int orc_fft(int ndims, const int32_t *N, double *data, int sign);

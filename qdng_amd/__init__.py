@@ -55,7 +55,7 @@ EXPORTS = [
     "qdx_ham_add_kinetic", "qdx_ham_add_potential", "qdx_ham_add_dipole", "qdx_ham_bounds",
     "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_expec_noalloc", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
     "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
-    "qdx_prop_apply", "qdx_prop_run", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
+    "qdx_prop_apply", "qdx_prop_run", "qdx_prop_set_direction", "qdx_prop_exponent", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
     "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
     "qdx_batch_job_state", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
     "qdx_batch_serve", "qdx_batch_set_graphs", "qdx_batch_graph_stats", "qdx_batch_destroy",
@@ -146,6 +146,8 @@ def lib():
         "qdx_cheby_coeffs": [vp, dp],
         "qdx_gspo_create": [C.POINTER(vp), i32, C.c_double, i32, C.POINTER(vp)],
         "qdx_prop_apply": [vp, vp, C.c_int64, C.POINTER(StepStats)],
+        "qdx_prop_set_direction": [vp, i32],
+        "qdx_prop_exponent": [vp, dp],
         "qdx_prop_run": [vp, vp, C.c_int64, C.c_int64, vp, dp, dp, C.POINTER(i32), C.POINTER(StepStats)],
         "qdx_prop_destroy": [vp],
         "qdx_ctx_profile": [vp, i32],
@@ -616,6 +618,16 @@ class _Propagator:
         st = StepStats()
         _chk(lib().qdx_prop_apply(self.h, psi.h, int(step), C.byref(st)))
         return st
+
+    def backward(self, on: bool = True) -> None:
+        """OPropagator::Backward() / Forward(): real time with exponent +i dt."""
+        _chk(lib().qdx_prop_set_direction(self.h, int(bool(on))))
+
+    def exponent(self) -> complex:
+        """OPropagator::Exponent(): c of exp(c H) for one step."""
+        out = np.zeros(2)
+        _chk(lib().qdx_prop_exponent(self.h, _dp(out)))
+        return complex(out[0], out[1])
 
     def run(self, psi: WaveFunction, first_step: int, nsteps: int, psi0: WaveFunction | None = None,
             observables: bool = True):

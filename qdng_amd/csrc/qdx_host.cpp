@@ -30,6 +30,7 @@ struct qdx_prop {
     ChebyParams cp;
     int k_launch = 0;   // orders launched per step (launch_window)
     bool imag = false;  // imaginary time, backward (exponent -dt)
+    bool backward = false;  // real time backward (exponent +i dt), any real-time kind
     DevBuf phiA, phiB, X, ctl, partials, lastorders;
     bool cmp = false;   // QDX_COMPACT on a grid with nd >= 2: compact terms
     // single-state 1D lines: the whole step in one block (k_cheby_line) with
@@ -404,7 +405,23 @@ double cheby_last_nconv(qdx_prop *p) {
 void sil_step(qdx_prop *p, qdx_wf *psi, long step);
 void arnoldi_step(qdx_prop *p, qdx_wf *psi, long step);
 
+void prop_step_forward(qdx_prop *p, qdx_wf *psi, long step);
+
+// OPropagator::Backward() in real time (exponent +i dt, with the operators
+// conjugated, OPropagator.h:82-100): exp(+i H* dt) psi = conj(exp(-i H dt)
+// conj(psi)), the forward step between two conjugations (exact: conjugation
+// commutes with every rounding of the forward step)
 void prop_step(qdx_prop *p, qdx_wf *psi, long step) {
+    if (!p->backward) return prop_step_forward(p, psi, step);
+    const bool g32 = p->kind == qdx_prop::GSPO && p->precision == QDX_FP32_BULK;
+    void *d = g32 ? static_cast<void *>(p->psi32.as<float>()) : static_cast<void *>(psi->ptr());
+    const long n = (long)p->S * psi->n();
+    launch_conj(p->ctx->stream, d, n, g32);
+    prop_step_forward(p, psi, step);
+    launch_conj(p->ctx->stream, d, n, g32);
+}
+
+void prop_step_forward(qdx_prop *p, qdx_wf *psi, long step) {
     if (p->kind == qdx_prop::SIL) {
         sil_step(p, psi, step);
         return;
@@ -1122,8 +1139,10 @@ int qdx_prop_run(qdx_prop *p, qdx_wf *psi, int64_t first_step, int64_t nsteps, q
             }
         }
         double *part = p->partials.as<double>();
+        if (!part || p->partials.bytes < kMaxPartials * sizeof(double))
+            fail(QDX_ERR_STATE, "propagator has no reduction buffer for the observables");
         std::vector<int> sil_counts;
-        const bool fused = nsteps > 0 && gspo_fusable(p).ok;
+        const bool fused = nsteps > 0 && !p->backward && gspo_fusable(p).ok;
         if (fused) {
             // observables before step 0 here; those before the later steps
             // from the row passes between the steps (psi0: autocorrelation)
@@ -1691,6 +1710,8 @@ static void krylov_create(qdx_ham *h, double dt, int32_t order, double err, int3
             if (i < order) p->lzb.push_back(w);
             else p->sil_buf = w;
         }
+        // reduction partials of qdx_prop_run's observables
+        p->partials.alloc(kMaxPartials * sizeof(double));
         ctx_acquire(p->ctx);
         *out = p.release();
     }
@@ -1702,6 +1723,27 @@ int qdx_sil_create(qdx_ham *h, double dt, int32_t order, double err, int32_t ima
 
 int qdx_arnoldi_create(qdx_ham *h, double dt, int32_t order, double conv, int32_t imag, qdx_prop **out) {
     return guarded([&] { krylov_create(h, dt, order, conv, imag, true, out); });
+}
+
+int qdx_prop_set_direction(qdx_prop *p, int32_t backward) {
+    return guarded([&] {
+        if (!p) fail(QDX_ERR_PARAM, "propagator is NULL");
+        if (p->imag) {
+            if (!backward) fail(QDX_ERR_INCOMPAT, "imaginary-time propagators run backward (exponent -dt) only");
+            return;
+        }
+        p->backward = backward != 0;
+    });
+}
+
+int qdx_prop_exponent(qdx_prop *p, double out[2]) {
+    return guarded([&] {
+        if (!p || !out) fail(QDX_ERR_PARAM, "NULL argument");
+        // OPropagator::Exponent (OPropagator.h:44-155): -i dt forward, +i dt
+        // backward; imaginary time backward -dt
+        out[0] = p->imag ? -p->dt : 0.0;
+        out[1] = p->imag ? 0.0 : (p->backward ? p->dt : -p->dt);
+    });
 }
 
 int qdx_prop_destroy(qdx_prop *p) {

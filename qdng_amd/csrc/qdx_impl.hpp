@@ -221,6 +221,8 @@ void launch_convert_c64_to_c128(hipStream_t s, const float *src, double *dst, lo
 // out[s] = sum_j conj(a_sj) b_sj  (re, im pairs), S <= kMaxStates
 void launch_overlap(hipStream_t s, const double *a, const double *b, long n, int S, double *partials,
                     double *out);
+// data[i] <- conj(data[i]) for n complex values (complex64 when fp32)
+void launch_conj(hipStream_t s, void *data, long n, bool fp32);
 // out[2 s], out[2 s + 1] = the sums over b < nbx of partials[2 (s nbx + b)],
 // partials[2 (s nbx + b) + 1], in the fixed order of launch_overlap
 void launch_sum_partials(hipStream_t s, const double *partials, int nbx, int S, double *out);

@@ -148,6 +148,21 @@ static void overlap_impl(hipStream_t s, const R *a, const R *b, long n, int S, d
     QDX_HIP(hipGetLastError());
 }
 
+template <class T>
+__global__ void k_conj(T *__restrict__ d, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[2 * i + 1] = -d[2 * i + 1];
+}
+
+void launch_conj(hipStream_t s, void *data, long n, bool fp32) {
+    const int threads = 256;
+    const int nb = (int)std::min<long>((n + threads - 1) / threads, 4096);
+    if (nb < 1) return;
+    if (fp32) QDX_LAUNCH((k_conj<float>), dim3(nb), dim3(threads), 0, s, static_cast<float *>(data), n);
+    else QDX_LAUNCH((k_conj<double>), dim3(nb), dim3(threads), 0, s, static_cast<double *>(data), n);
+    QDX_HIP(hipGetLastError());
+}
+
 void launch_sum_partials(hipStream_t s, const double *partials, int nbx, int S, double *out) {
     QDX_LAUNCH(k_overlap_stage2, dim3(1), dim3(256), 0, s, partials, nbx, S, out);
     QDX_HIP(hipGetLastError());

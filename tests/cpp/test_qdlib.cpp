@@ -11,7 +11,9 @@
 // QDng's exceptions for bad parameters.
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -255,7 +257,7 @@ int main() {
         G.Clock(&clock);
         G.Add("A", &Tk);
         G.Add("B", &Vp);
-        const double e0 = H.Expec(&a).real();
+        const double e0 = H.Expec(&a);
         for (int s = 0; s < 50; ++s) {
             clock.TimeStep(s);
             U.Apply(&a);
@@ -263,9 +265,135 @@ int main() {
         }
         // the series stops at a norm change below prec = 1e-10 per step
         ASSERT_NEAR(1.0, a.Norm(), 1e-8);
-        ASSERT_NEAR(e0, H.Expec(&a).real(), 1e-8);
+        ASSERT_NEAR(e0, H.Expec(&a), 1e-8);
         const double ov = std::abs(a * b);
         ASSERT_NEAR(1.0, ov, 1e-4);   // second-order splitting error of GSPO at dt = 2
+    });
+
+    // QDng code holds operators and propagators as Operator* / OPropagator*
+    // (ProgPropa, Reporter, ProgEigen): the same calls through the virtual
+    // interface (Operator.h:142-369, OPropagator.h:44-169)
+    run("Operator interface through Operator*", [&] {
+        const int N = 64;
+        const qdx_grid g = line_grid(N, -6, 6);
+        const double dx = 12.0 / (N - 1);
+        WFMultistate psi;
+        psi.Init(g, 1);
+        std::vector<double> V(N);
+        for (int i = 0; i < N; ++i) {
+            const double x = -6 + i * dx;
+            V[i] = 0.05 * x * x + 0.01;
+            psi.at(0, i) = dcomplex(std::exp(-(x - 0.5) * (x - 0.5)), 0.3 * std::exp(-x * x));
+        }
+        OGridNablaSq T;
+        ParamContainer pm;
+        pm.SetValue("mass", 900.0);
+        T.Init(pm);
+        OGridPotential Vp;
+        Vp.SetData(V);
+        OSum H;
+        H.Add(&T);
+        H.Add(&Vp);
+        std::vector<Operator *> ops = {&T, &Vp, &H};
+        std::vector<std::unique_ptr<WaveFunction>> out;
+        for (Operator *o : ops) {
+            o->Init(&psi);
+            std::unique_ptr<Operator> fresh(o->NewInstance());
+            ASSERT_TRUE(fresh->Name() == o->Name());
+            out.emplace_back(psi.NewInstance());
+            o->Apply(out.back().get(), &psi);
+            // in place equals out of place
+            std::unique_ptr<WaveFunction> w(psi.NewInstance());
+            *w = psi;
+            o->Apply(w.get());
+            for (int i = 0; i < N; ++i) ASSERT_NEAR(0, std::abs((*w)[i] - (*out.back())[i]), 1e-14);
+            // Expec is Re <psi|O|psi> = Re MatrixElement(psi, psi)
+            ASSERT_NEAR(o->MatrixElement(&psi, &psi).real(), o->Expec(&psi), 1e-12);
+            ASSERT_TRUE(o->Hermitian());
+        }
+        // H psi = T psi + V psi through ApplyAdd (Operator.h:307)
+        std::unique_ptr<WaveFunction> tv(psi.NewInstance());
+        T.Apply(tv.get(), &psi);
+        Vp.ApplyAdd(tv.get(), &psi);
+        for (int i = 0; i < N; ++i) ASSERT_NEAR(0, std::abs((*tv)[i] - (*out[2])[i]), 1e-13);
+        // bounds: kinetic (pi^2/2)/(m dx^2) and 0, potential max/min, sum of both
+        ASSERT_NEAR(M_PI * M_PI / 2 / (900.0 * dx * dx), T.Emax().real(), 1e-12);
+        ASSERT_NEAR(0.0, T.Emin().real(), 1e-15);
+        ASSERT_NEAR(*std::max_element(V.begin(), V.end()), Vp.Emax().real(), 1e-15);
+        ASSERT_NEAR(*std::min_element(V.begin(), V.end()), Vp.Emin().real(), 1e-15);
+        ASSERT_NEAR(T.Emax().real() + Vp.Emax().real(), H.Emax().real(), 1e-12);
+    });
+
+    run("OPropagator interface: exponent, backward, imaginary time", [&] {
+        const int N = 128;
+        const qdx_grid g = line_grid(N, -8, 8);
+        const double dx = 16.0 / (N - 1);
+        std::vector<double> V(N);
+        WFMultistate psi;
+        psi.Init(g, 1);
+        for (int i = 0; i < N; ++i) {
+            const double x = -8 + i * dx;
+            V[i] = 0.05 * x * x;
+            psi.at(0, i) = std::exp(-(x - 1) * (x - 1));
+        }
+        psi.Normalize();
+        OGridNablaSq T;
+        ParamContainer pm;
+        pm.SetValue("mass", 1836.0);
+        T.Init(pm);
+        OGridPotential Vp;
+        Vp.SetData(V);
+        OSum H, Tk, Vk;
+        H.Add(&T);
+        H.Add(&Vp);
+        Tk.Add(&T);
+        Vk.Add(&Vp);
+        QDClock clock;
+        clock.Dt(4.0);
+        OCheby cheby;
+        cheby.Add("hamiltonian", &H);
+        OSIL sil;
+        sil.Add("hamiltonian", &H);
+        OArnoldi arn;
+        arn.Add("hamiltonian", &H);
+        OGSPO gspo;
+        gspo.Add("A", &Tk);
+        gspo.Add("B", &Vk);
+        std::vector<OPropagator *> props = {&cheby, &sil, &arn, &gspo};
+        for (OPropagator *U : props) {
+            ASSERT_THROW(EIncompatible, U->Backward());     // no clock yet
+            U->Clock(&clock);
+            ASSERT_NEAR(-4.0, U->Exponent().imag(), 0);    // forward: -i dt
+            ASSERT_NEAR(0.0, U->Exponent().real(), 0);
+            std::unique_ptr<WaveFunction> w(psi.NewInstance());
+            *w = psi;
+            U->Apply(w.get());                              // Init on first use
+            ASSERT_TRUE(U->Hamiltonian() != nullptr);
+            const double e0 = U->Hamiltonian()->Expec(&psi), e1 = U->Hamiltonian()->Expec(w.get());
+            ASSERT_NEAR(e0, e1, U == &gspo ? 1e-5 : 1e-9);  // GSPO: splitting error
+            U->Backward();                                   // exponent +i dt
+            ASSERT_NEAR(4.0, U->Exponent().imag(), 0);
+            ASSERT_TRUE(U->Conj());
+            U->Apply(w.get());                               // back to psi
+            for (int i = 0; i < N; ++i) ASSERT_NEAR(0, std::abs((*w)[i] - psi.at(0, i)), 1e-9);
+            U->Conj(false);                                  // Conj(false) is Forward()
+            ASSERT_NEAR(-4.0, U->Exponent().imag(), 0);
+            ASSERT_NEAR(0.0, U->Emax().real(), 0);           // propagators have no bounds
+        }
+        // imaginary time runs backward: exponent -dt, the energy decreases
+        OCheby im;
+        im.Add("hamiltonian", &H);
+        im.Clock(&clock);
+        im.ImaginaryTime();
+        ASSERT_THROW(EIncompatible, im.Init(&psi));        // forward imaginary time: exp(+H dt)
+        im.Backward();
+        ASSERT_NEAR(-4.0, im.Exponent().real(), 0);
+        std::unique_ptr<WaveFunction> w(psi.NewInstance());
+        *w = psi;
+        OPropagator *Ui = &im;
+        Ui->Apply(w.get());
+        w->Normalize();
+        ASSERT_TRUE(H.Expec(w.get()) < H.Expec(&psi));
     });
 
     run("exceptions", [&] {

@@ -521,3 +521,60 @@ def test_incompatible_wave_function_is_rejected(ctx):
     w = lp.gpu_wf(ctx)
     with pytest.raises(Q.QdxError):
         U2.apply(w, 10)           # clock step beyond the laser field (Laser.cpp:110-112)
+
+
+# ---------------------------------------------------------------------------
+# backward real time (OPropagator::Backward, exponent +i dt): device against
+# the oracle's direct restatement, one propagator kind at a time
+
+@pytest.mark.parametrize("kind,name", [("cheby", "harmonic_1d"), ("cheby", "morse_2state_2d"),
+                                       ("cheby", "anharmonic_3d"), ("gspo", "morse_2state_2d"),
+                                       ("gspo32", "morse_2state_2d"), ("sil", "laser_1d"),
+                                       ("arnoldi", "morse_2state_2d")])
+def test_backward_real_time_vs_oracle(ctx, kind, name):
+    import qdng_amd as Q
+    note(f"backward {kind} {name}")
+    p = {"harmonic_1d": lambda: cases.harmonic_1d(256), "morse_2state_2d": lambda: cases.morse_2state_2d(64),
+         "anharmonic_3d": lambda: cases.anharmonic_3d(24), "laser_1d": lambda: cases.laser_1d(256)}[name]()
+    dt = 2.0
+    if kind == "cheby":
+        U, Uo = Q.Cheby(p.gpu_ham(ctx), dt, prec=1e-12), O.Cheby(p.oracle_ham(), dt, prec=1e-12)
+    elif kind.startswith("gspo"):
+        U = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], dt,
+                   precision="fp32" if kind == "gspo32" else "fp64")
+        Uo = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], dt, p.S)
+    elif kind == "sil":
+        U, Uo = Q.SIL(p.gpu_ham(ctx), dt, order=20, err=1e-12), O.SIL(p.oracle_ham(), dt, order=20, err=1e-12)
+    else:
+        U, Uo = Q.Arnoldi(p.gpu_ham(ctx), dt, order=20, conv=1e-12), O.Arnoldi(p.oracle_ham(), dt, order=20,
+                                                                              conv=1e-12)
+    assert U.exponent() == complex(0, -dt)
+    U.backward(True)
+    Uo.backward(True)
+    assert U.exponent() == complex(0, dt)
+    steps = 4
+    w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
+    n2, ac, _, _ = U.run(w, 0, steps, w0)
+    psi = p.psi0.copy()
+    for i in range(steps):
+        Uo.step(psi, i)
+    e = rel(w.download(), psi)
+    note(f"  rel L2 {e:.2e}")
+    assert e < (1e-5 if kind == "gspo32" else 1e-10)
+    # and forward again: the first step undoes the last backward one
+    U.backward(False)
+    v = p.gpu_wf(ctx, w.download())
+    U.apply(v, steps - 1)
+    # (FP32 GSPO correlates the complex64 rounding of psi0 with itself)
+    assert np.isfinite(n2).all() and abs(ac[0] - 1.0) < (1e-7 if kind == "gspo32" else 1e-10)
+
+
+def test_imag_time_propagator_keeps_its_direction(ctx):
+    import qdng_amd as Q
+    p = cases.harmonic_1d(64)
+    U = Q.Cheby(p.gpu_ham(ctx), 5.0, imag=True)
+    assert U.exponent() == complex(-5.0, 0)
+    U.backward(True)
+    with pytest.raises(Q.QdxError) as e:
+        U.backward(False)
+    assert e.value.code == -2
