@@ -14,9 +14,14 @@ metric: grid-pt * H*psi / s = sum over steps of N_points * states * executed
 H applications (1 + recurrence iterations actually run), divided by the wall
 time of the timed region.
 
-Also reported (object "extra"): the FP32-bulk path on the same workload, the
-GSPO path (grid-pt * step / s), and the relative L2 difference to the FP64
-CPU oracle after a few steps.
+Also reported (object "extra"), one record per benchmark configuration
+("configs": the 2D two-state Chebyshev and GSPO workload, the 3D 128^3
+precision variant, the 512-job batch and the serving stream), each with the
+FP64, compact (FP64 arithmetic, 40-bit stored terms) and FP32 paths, the
+relative L2 error against FP64, a roofline figure and a CPU baseline: the
+FP64 oracle with QDng's FFT library interface (FFTW3 API, served by MKL's
+libmkl_rt.so when it loads; else the oracle's own transform, labelled) on the
+host cores and on one thread, medians over repeats.
 """
 from __future__ import annotations
 
@@ -39,8 +44,12 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per grid point per state per executed order of the row
 # kernel (streaming model): phi_{k-1} + phi_{k-2} + phi_k + psi read/write +
 # V_ii + V_ij (coupling); the prologue (order 1) reads no phi_{k-2}
-B_ROW = {"fp64": (16 + 16 + 16 + 32 + 8 + 8), "fp32": (8 + 8 + 8 + 32 + 4 + 4)}
-B_ROW_PRO = {"fp64": B_ROW["fp64"] - 16, "fp32": B_ROW["fp32"] - 8}
+B_ROW = {"fp64": (16 + 16 + 16 + 32 + 8 + 8), "fp32": (8 + 8 + 8 + 32 + 4 + 4),
+         "compact": (10 + 10 + 10 + 32 + 8 + 8)}
+B_ROW_PRO = {"fp64": B_ROW["fp64"] - 16, "fp32": B_ROW["fp32"] - 8, "compact": B_ROW["compact"] - 10}
+# GSPO per step and grid point of one state: psi read and write + V
+B_GSPO = {"fp64": 32 + 8, "fp32": 16 + 4}
+PRECISIONS = ("fp64", "compact", "fp32")
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cheby_row_2d_512x512_2state_fp64_hbm_traffic.md")
@@ -98,7 +107,9 @@ def run_cheby(ctx, p, precision, steps, warmup, profile):
     return {"seconds": t1 - t0, "n_hpsi": nh, "order": U.order, "row": rowk, "line": linek}
 
 
-def run_gspo(ctx, p, precision, steps, warmup):
+def run_gspo(ctx, p, precision, steps, warmup, profile=False):
+    """GSPO steps of the two-state workload through qdx_prop_run (one row pass
+    between steps); with profile, the per-kernel times of a second pass."""
     import qdng_amd as Q
     G = Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["pot"])], 5.0, precision=precision)
     w = p.gpu_wf(ctx)
@@ -108,7 +119,15 @@ def run_gspo(ctx, p, precision, steps, warmup):
     t0 = time.perf_counter()
     G.run(w, warmup, steps, observables=False)
     sync(ctx)
-    return time.perf_counter() - t0
+    t = time.perf_counter() - t0
+    if not profile:
+        return t
+    ctx.profile(True)
+    G.run(w, warmup + steps, steps, observables=False)
+    sync(ctx)
+    rowk, colk = ctx.kernel_time(Q.K_GSPO_ROW), ctx.kernel_time(Q.K_LINE_PASS)
+    ctx.profile(False)
+    return t, rowk, colk
 
 
 # batched jobs, single state with a laser dipole: phi_{k-1}, phi_{k-2}, phi_k,
@@ -173,18 +192,26 @@ def run_batched(ctx, precision, steps, prof_ticks, seed=2026, tune=False):
     return {"seconds": t1 - t0, "work": work, "row": rowk, "col": colk, "line": linek,
             "row_bytes": float(alg[two_d].sum()), "line_bytes": float(alg[~two_d].sum()),
             "work_2d": float((sizes * (h1 - h0))[two_d].sum()),
-            "jobs": jobs, "ids": ids, "batch": B, "points": int(sizes.sum()),
+            "jobs": jobs, "ids": ids, "batch": B, "points": int(sizes.sum()), "ticks_total": int(B.state(ids[0])[2]),
             "graphs": graphs, "eager_ms_per_tick": eager_ms, "tuning": tuning, "tune_s": tune_s, "lds": lds}
 
 
 TUNING_FILES = {}   # precision -> tuning file written by the batched run (read by the serving run)
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def batched_extra(ctx, steps, prof_ticks):
+    """The 512-job batch (seed 2026) in every precision: throughput, the row
+    kernel's roofline, FP64-relative error per job, and the oracle on the host
+    over one step of a sample of the jobs as CPU baseline."""
     import tempfile
-    out = {}
+    out = {"metric": "grid-pt*Hpsi/s, 512 laser-driven jobs (256 1D, 256 2D) per tick", "unit": UNIT,
+           "precisions": {}}
     res = {}
-    for prec in ("fp64", "fp32"):
+    for prec in PRECISIONS:
         r = run_batched(ctx, prec, steps, prof_ticks, tune=True)
         res[prec] = r
         fd, path = tempfile.mkstemp(prefix=f"qdx_tuning_{prec}_", suffix=".txt")
@@ -196,7 +223,7 @@ def batched_extra(ctx, steps, prof_ticks):
         nl, msl = r["line"]
         ach = r["row_bytes"] / (ms * 1e-3) / 1e9
         value = r["work"] / r["seconds"]
-        out[prec] = {
+        out["precisions"][prec] = {
             "value": value, "unit": UNIT, "ms_per_tick": r["seconds"] * 1e3 / steps,
             "effective_GBs": B_BATCH[prec] * value / 1e9,
             "effective_frac": B_BATCH[prec] * value / 1e9 / HBM_PEAK_GBS,
@@ -204,33 +231,35 @@ def batched_extra(ctx, steps, prof_ticks):
             "tick_graphs": {"captured": r["graphs"][0], "replays": r["graphs"][1],
                             "ms_per_tick_kernel_by_kernel": r["eager_ms_per_tick"]},
             "autotune": {"seconds": r["tune_s"], "lds_schedule": r["lds"],
-                         "shapes": {k: {"radix": v["radix0"], "tiling": v["tiling"],
-                                        "row_us_default": v["row_us_default"], "row_us": v["row_us"],
-                                        "col_us_default": v["col_us_default"], "col_us": v["col_us"],
-                                        "candidates": v["candidates"]} for k, v in r["tuning"].items()}},
-            "row_kernel": {"achieved_GBs": ach, "frac": ach / HBM_PEAK_GBS, "launches": n,
-                           "avg_launch_us": ms / max(n, 1) * 1e3, "jobs": "2D"},
+                         "shapes": {k: {"radix": v["radix0"], "tiling": v["tiling"], "row_us": v["row_us"],
+                                        "col_us": v["col_us"]} for k, v in r["tuning"].items()}},
+            "roofline": {"bound": "hbm", "kernel": "k_brow (2D jobs)", "achieved": ach, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "launches": n,
+                         "avg_launch_us": ms / max(n, 1) * 1e3,
+                         "bytes_model": f"{B_BATCH[prec]} B per grid point per executed order"},
             "col_kernel": {"launches": nc, "avg_launch_us": msc / max(nc, 1) * 1e3},
             "line_kernel": {"launches": nl, "avg_launch_us": msl / max(nl, 1) * 1e3, "jobs": "1D",
-                            "algorithmic_GBs": r["line_bytes"] / max(msl * 1e-3, 1e-12) / 1e9,
-                            "note": "whole step per block with the line in registers: the algorithmic "
-                                    "bytes are not HBM traffic"},
+                            "note": "whole step per block with the line in registers"},
         }
         log(f"batched {prec}: {value:.4g} grid-pt*Hpsi/s, {r['seconds'] * 1e3 / steps:.2f} ms/tick, "
             f"row kernel {ach:.0f} GB/s")
-    out["fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
-    b64, b32 = res["fp64"]["batch"], res["fp32"]["batch"]
-    errs = []
-    for j64, j32 in zip(res["fp64"]["ids"], res["fp32"]["ids"]):
-        a, b = b64.fetch(j64), b32.fetch(j32)
-        errs.append(np.linalg.norm(b - a) / np.linalg.norm(a))
-    w = int(np.argmax(errs))
-    out["l2_worst_job"] = {"N": res["fp64"]["jobs"][w].N, "l2": float(errs[w])}
-    out["l2_fp32_vs_fp64_max"] = float(max(errs))
-    out["l2_fp32_vs_fp64_median"] = float(np.median(errs))
-    out["workload"] = {"jobs": 512, "points": res["fp64"]["points"], "dt": 1.0,
-                       "timed_ticks": steps, "profiled_ticks": prof_ticks,
-                       "bytes_model": "96 B (fp64) / 64 B (fp32) per grid point per executed order"}
+    ref = [res["fp64"]["batch"].fetch(j) for j in res["fp64"]["ids"]]
+    for prec in ("compact", "fp32"):
+        b = res[prec]["batch"]
+        errs = [rel_l2(b.fetch(j), a) for j, a in zip(res[prec]["ids"], ref)]
+        w = int(np.argmax(errs))
+        out["precisions"][prec].update({
+            "vs_fp64": out["precisions"][prec]["value"] / out["precisions"]["fp64"]["value"],
+            "l2_vs_fp64_worst": float(errs[w]), "l2_worst_job": res["fp64"]["jobs"][w].N,
+            "l2_vs_fp64_median": float(np.median(errs))})
+    jobs = res["fp64"]["jobs"]
+    out["value"] = out["precisions"]["fp64"]["value"]
+    out["roofline"] = out["precisions"]["fp64"]["roofline"]
+    out["workload"] = {"jobs": 512, "points": res["fp64"]["points"], "dt": 1.0, "timed_ticks": steps,
+                       "profiled_ticks": prof_ticks, "steps_compared": res["fp64"]["ticks_total"]}
+    out["cpu_baseline"] = cpu_batched(jobs[::4])
+    for r in res.values():
+        r["batch"] = None
     return out
 
 
@@ -262,14 +291,15 @@ def serve_requests(lam, n_jobs, seed=7):
 
 def serving_extra(ctx, lams, n_jobs):
     import qdng_amd as Q
-    out = {"mix": "90% batched-distribution Chebyshev jobs, 10% two-state 256x256 GSPO jobs, "
+    out = {"metric": "grid-pt*Hpsi/s (GSPO: grid-pt*step) and p50/p99 job latency of a Poisson stream",
+           "mix": "90% batched-distribution Chebyshev jobs, 10% two-state 256x256 GSPO jobs, "
                   "steps U[200, 5000], dt 1, Poisson arrivals (seed 7)", "jobs_per_rate": n_jobs, "rates": {}}
     for lam in lams:
         arrivals, reqs, meta = serve_requests(lam, n_jobs)
         pts = np.array([m[1] for m in meta], dtype=float)
         r = {}
         finals = {}
-        for prec in ("fp32", "fp64"):
+        for prec in PRECISIONS:
             B = Q.Batch(ctx, prec, 32 << 30)
             if prec in TUNING_FILES:
                 B.load_tuning(TUNING_FILES[prec])   # the batched run's schedules
@@ -285,50 +315,180 @@ def serving_extra(ctx, lams, n_jobs):
             finals[prec] = psis
             log(f"serve lam={lam} {prec}: {r[prec]['throughput']:.3g}, p50 {r[prec]['p50_latency_s']:.3f} s, "
                 f"p99 {r[prec]['p99_latency_s']:.3f} s")
-        errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(finals["fp32"], finals["fp64"])]
-        r["l2_fp32_vs_fp64_median"] = float(np.median(errs))
-        r["l2_fp32_vs_fp64_max"] = float(np.max(errs))
-        r["fp32_vs_fp64_throughput"] = r["fp32"]["throughput"] / r["fp64"]["throughput"]
+        for prec in ("compact", "fp32"):
+            errs = [rel_l2(a, b) for a, b in zip(finals[prec], finals["fp64"])]
+            r[prec]["l2_vs_fp64_median"] = float(np.median(errs))
+            r[prec]["l2_vs_fp64_worst"] = float(np.max(errs))
+            r[prec]["vs_fp64"] = r[prec]["throughput"] / r["fp64"]["throughput"]
         out["rates"][str(lam)] = r
     return out
 
 
-def cpu_baseline(p, steps):
-    """FP64 oracle on the host cores for `steps` Chebyshev steps from psi0.
-    Only the steps are timed; the Reporter observables before each step
-    (norm^2, <psi(t)|psi0>, Reporter.cpp:101-137) are taken outside."""
+# ---------------------------------------------------------------------------
+# CPU baselines: the FP64 oracle (QDng's algorithm) with the FFTW3 API through
+# MKL where it loads (QDng calls FFTW), on the host cores and on one thread
+
+def host_threads():
+    return min(os.cpu_count() or 1, 16)
+
+
+def cpu_fft_backend():
     from oracle import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+    return O.set_fft_backend("fftw")
+
+
+def timed_median(run, repeats):
+    """run() -> (seconds, work); median of the per-repeat rates after one
+    untimed warm-up (plans, first touch)."""
+    run()
+    rates, secs = [], 0.0
+    for _ in range(repeats):
+        t, work = run()
+        rates.append(work / t)
+        secs += t
+    return float(np.median(rates)), secs
+
+
+def cpu_cheby(p, dt, steps, repeats, threads, prec=1e-10):
+    """Chebyshev steps of problem p on the oracle: (rate, seconds, H applications)."""
+    from oracle import oracle as O
     O.set_threads(threads)
-    O.reset()
-    U = O.Cheby(p.oracle_ham(), 5.0, prec=1e-10)
-    g = p.oracle_grid()
+    U = O.Cheby(p.oracle_ham(), dt, prec=prec)
     psi = p.psi0.copy()
-    dt = 0.0
-    nh = 0
-    norm2, ac = [], []
-    for i in range(steps):
-        norm2.append(O.norm2(g, psi, p.S))
-        ac.append(O.overlap(g, psi, p.psi0, p.S))
+    state = {"step": 0, "nh": 0}
+
+    def run():
         t0 = time.perf_counter()
-        nh += U.step(psi, i)
-        dt += time.perf_counter() - t0
-    return psi, nh, dt, threads, np.array(norm2), np.array(ac)
+        nh = 0
+        for _ in range(steps):
+            nh += U.step(psi, state["step"])
+            state["step"] += 1
+        state["nh"] = nh
+        return time.perf_counter() - t0, p.size * p.S * nh
+    rate, secs = timed_median(run, repeats)
+    return rate, secs, state["nh"]
 
 
-def cpu_one_thread(p, steps):
-    """The FP64 oracle on one host thread, first `steps` steps of the same
-    workload (beside the multi-threaded cpu_baseline)."""
+def cpu_record(rate, rate1, backend, cores, sample, unit=UNIT):
+    return {"value": rate, "unit": unit, "cores": cores, "kind": "port",
+            "fft": "FFTW3 API (MKL libmkl_rt.so)" if backend == "fftw" else "oracle's own FFT (FFTW3/MKL absent)",
+            "value_1_thread": rate1, "sample": sample}
+
+
+def cpu_cfg1(p):
+    backend = cpu_fft_backend()
+    th = host_threads()
+    rate, secs, nh = cpu_cheby(p, 5.0, 2, 5, th)
+    rate1, secs1, _ = cpu_cheby(p, 5.0, 1, 3, 1)
     from oracle import oracle as O
-    O.set_threads(1)
-    U = O.Cheby(p.oracle_ham(), 5.0, prec=1e-10)
-    psi = p.psi0.copy()
-    t0 = time.perf_counter()
-    nh = sum(U.step(psi, i) for i in range(steps))
-    dt = time.perf_counter() - t0
-    O.set_threads(min(os.cpu_count() or 1, 16))
-    return {"value": p.size * p.S * nh / dt, "unit": UNIT, "cores": 1, "kind": "port",
-            "sample": f"first {steps} steps, {dt:.1f} s"}
+    O.set_threads(th)
+    return cpu_record(rate, rate1, backend, th,
+                      f"Chebyshev steps of the 512x512 two-state workload on the FP64 oracle: median of 5 x 2 steps "
+                      f"on {th} threads ({secs:.1f} s), of 3 x 1 step on 1 thread ({secs1:.1f} s), after one warm-up "
+                      f"step each; {nh} H applications per 2 steps")
+
+
+def cpu_gspo(p):
+    from oracle import oracle as O
+    backend = cpu_fft_backend()
+    res = {}
+    for th, steps, rep in ((host_threads(), 10, 5), (1, 4, 3)):
+        O.set_threads(th)
+        G = O.GSPO([p.oracle_split_op(["kin"]), p.oracle_split_op(["pot"])], 5.0, p.S)
+        psi = p.psi0.copy()
+
+        def run():
+            t0 = time.perf_counter()
+            for i in range(steps):
+                G.step(psi, i)
+            return time.perf_counter() - t0, p.size * p.S * steps
+        res[th] = timed_median(run, rep)
+    O.set_threads(host_threads())
+    th = host_threads()
+    return cpu_record(res[th][0], res[1][0], backend, th,
+                      f"GSPO steps of the same workload on the FP64 oracle: median of 5 x 10 steps on {th} threads, "
+                      f"of 3 x 4 steps on 1 thread", unit="grid-pt*step/s")
+
+
+def cpu_batched(jobs):
+    """One Chebyshev step of each job of a sample, one after the other (the
+    batch has no CPU counterpart: QDng runs jobs one at a time)."""
+    from oracle import oracle as O
+    backend = cpu_fft_backend()
+    th = host_threads()
+    O.set_threads(th)
+    t = 0.0
+    work = 0
+    for p in jobs:
+        U = O.Cheby(p.oracle_ham(), 1.0, prec=1e-10)
+        psi = p.psi0.copy()
+        U.step(psi, 0)
+        t0 = time.perf_counter()
+        nh = U.step(psi, 1)
+        t += time.perf_counter() - t0
+        work += p.size * nh
+    return cpu_record(work / t, None, backend, th,
+                      f"second Chebyshev step of every 4th job of the 512-job batch ({len(jobs)} jobs) on the FP64 "
+                      f"oracle, jobs one after the other on {th} threads, {t:.1f} s")
+
+
+def precision_3d(ctx, steps=5):
+    """The 3D 128^3 precision workload (anharmonic oscillator on [-6, 6]^3,
+    complex64 Gaussian) at dt 3.5 (order ~64) and 37 (order ~256): each path's
+    throughput, the row kernel's roofline and the relative L2 to FP64 after
+    `steps` steps from the same state; the oracle on the host at dt 3.5."""
+    import cases
+    import qdng_amd as Q
+    p = cases.anharmonic_3d(128)
+    p.psi0 = p.psi0.astype(np.complex64).astype(np.complex128)
+    npts = p.size
+    out = {"metric": "grid-pt*Hpsi/s and relative L2 to FP64, 3D 128^3 single state", "unit": UNIT, "dt": {}}
+    for dt in (3.5, 37.0):
+        rec = {}
+        ref = None
+        for prec in PRECISIONS:
+            U = Q.Cheby(p.gpu_ham(ctx), dt, prec=1e-10, precision=prec)
+            w = p.gpu_wf(ctx)
+            sync(ctx)
+            t0 = time.perf_counter()
+            _, _, nh, _ = U.run(w, 0, steps, observables=False)
+            sync(ctx)
+            t = time.perf_counter() - t0
+            hpsi = int(nh.sum())
+            v = w.download()
+            ctx.profile(True)
+            w2 = p.gpu_wf(ctx)
+            _, _, nh2, _ = U.run(w2, 0, 1, observables=False)
+            sync(ctx)
+            nrow, msrow = ctx.kernel_time(Q.K_CHEBY_ROW)
+            ctx.profile(False)
+            h2 = int(nh2.sum())
+            cpl = 4 if prec == "fp32" else 8   # one state: no coupling field
+            rb = npts * ((B_ROW[prec] - cpl) * (h2 - 1) + (B_ROW_PRO[prec] - cpl))
+            ach = rb / (msrow * 1e-3) / 1e9 if msrow > 0 else 0.0
+            rec[prec] = {"value": npts * hpsi / t, "unit": UNIT, "ms_per_step": t * 1e3 / steps,
+                         "hpsi_per_step": [int(nh.min()), int(nh.max())], "order": U.order,
+                         "roofline": {"bound": "hbm", "kernel": "k_cheby_row", "achieved": ach,
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                                      "avg_launch_us": msrow / max(nrow, 1) * 1e3}}
+            if prec == "fp64":
+                ref = v
+            else:
+                rec[prec]["l2_vs_fp64"] = rel_l2(v, ref)
+                rec[prec]["vs_fp64"] = rec[prec]["value"] / rec["fp64"]["value"]
+            log(f"3D 128^3 dt={dt} {prec}: {rec[prec]['value']:.4g} grid-pt*Hpsi/s"
+                + (f", L2 vs fp64 {rec[prec]['l2_vs_fp64']:.2e}" if prec != "fp64" else ""))
+        out["dt"][str(dt)] = rec
+    out["value"] = out["dt"]["3.5"]["fp64"]["value"]
+    out["roofline"] = out["dt"]["3.5"]["fp64"]["roofline"]
+    backend = cpu_fft_backend()
+    th = host_threads()
+    rate, secs, nh = cpu_cheby(p, 3.5, 1, 3, th)
+    out["cpu_baseline"] = cpu_record(rate, None, backend, th,
+                                     f"Chebyshev steps at dt 3.5 on the FP64 oracle: median of 3 x 1 step on {th} "
+                                     f"threads after one warm-up step ({secs:.1f} s, {nh} H applications per step)")
+    out["workload"] = {"grid": [128, 128, 128], "box": [-6.0, 6.0], "mass": 1836.0, "steps": steps}
+    return out
 
 
 def cheby_1d_small(ctx, steps=1000):
@@ -385,7 +545,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer 4 times (reference for FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--serve-rates", default="50,500,2000", help="Poisson arrival rates (jobs/s), comma separated")
-    ap.add_argument("--batched-only", choices=["fp64", "fp32"],
+    ap.add_argument("--batched-only", choices=["fp64", "fp32", "compact"],
                     help="only run the batched-jobs workload in this precision (default schedules, "
                          "--batch-steps ticks) and print its tick time: a short program for counter runs")
     args = ap.parse_args()
@@ -455,7 +615,7 @@ def main():
             "traffic": measured_traffic(),
             "traffic_source": "profiles/cheby_row_2d_512x512_2state_fp64_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.000 "
                               "+ WRITE_SIZE x 1.000, calibrated on a 256 MiB device copy)",
-            "kernel": "qdx::k_cheby_row<double, 2, false, false, 256, 4, false>",
+            "kernel": "qdx::k_cheby_row<double, 2, false, false, 256, 4, false, false>",
             "points_per_thread": 4,
             "launches": launches,
             "avg_launch_us": row_ms / launches * 1e3,
@@ -465,6 +625,9 @@ def main():
             "measured": "HIP events attached to every launch (hipExtLaunchKernelGGL) over a second "
                         "timed pass of the same steps",
             "ms_per_step_profiled_pass": rp["seconds"] * 1e3 / args.steps,
+            "step_frac": B_ROW["fp64"] * value / 1e9 / HBM_PEAK_GBS,
+            "step_frac_note": "the whole step (row kernel, column pass, launch gaps) at the same 96 B per grid "
+                              "point and state per executed order",
             "column_pass": {"kernel": "qdx::k_col<double, 4, 4, false>", "launches": rp["line"][0],
                             "avg_launch_us": rp["line"][1] / max(rp["line"][0], 1) * 1e3,
                             "algorithmic_bytes_per_launch": 32.0 * p.size * p.S,
@@ -472,69 +635,91 @@ def main():
         },
     }
 
-    # ---- CPU baseline: FP64 oracle on a bounded sample of the same workload --
+    # ---- parity against the oracle (its own FFT: the checker) over the first
+    # steps, and the CPU baseline (the oracle with the FFTW3 API, timed) ----
+    ex = out.setdefault("extra", {})
     if args.cpu_steps > 0:
-        psi_cpu, nh_cpu, dt_cpu, threads, n2_cpu, ac_cpu = cpu_baseline(p, args.cpu_steps)
-        out["cpu_baseline"] = {
-            "value": npts * nh_cpu / dt_cpu,
-            "unit": UNIT,
-            "cores": threads,
-            "kind": "port",
-            "sample": f"first {args.cpu_steps} Chebyshev steps of the same workload on the FP64 oracle "
-                      f"(oracle/qdx_oracle.cpp, OpenMP {threads} threads), {dt_cpu:.1f} s",
-        }
-        # parity of the same steps on the GPU: final psi, norm drift and the
-        # autocorrelation series (Reporter observables before each step)
+        from oracle import oracle as O
+        O.set_fft_backend("own")
+        O.set_threads(host_threads())
+        Uo = O.Cheby(p.oracle_ham(), 5.0, prec=1e-10)
+        g = p.oracle_grid()
+        psi_cpu = p.psi0.copy()
+        n2_cpu, ac_cpu = [], []
+        for i in range(args.cpu_steps):
+            n2_cpu.append(O.norm2(g, psi_cpu, p.S))
+            ac_cpu.append(O.overlap(g, psi_cpu, p.psi0, p.S))
+            Uo.step(psi_cpu, i)
+        n2_cpu, ac_cpu = np.array(n2_cpu), np.array(ac_cpu)
+
         def parity(precision):
             U = Q.Cheby(p.gpu_ham(ctx), 5.0, prec=1e-10, precision=precision)
             w, w0 = p.gpu_wf(ctx), p.gpu_wf(ctx)
             n2, ac, _, _ = U.run(w, 0, args.cpu_steps, w0)
-            g = w.download()
-            return {"l2_vs_oracle": float(np.linalg.norm(g - psi_cpu) / np.linalg.norm(psi_cpu)),
+            return {"l2_vs_oracle": rel_l2(w.download(), psi_cpu),
                     "norm_drift_max": float(np.abs(n2 - n2[0]).max()),
                     "norm_drift_max_oracle": float(np.abs(n2_cpu - n2_cpu[0]).max()),
                     "autocorr_max_abs_err": float(np.abs(ac - ac_cpu).max()),
                     "steps": args.cpu_steps}
-        ex = out.setdefault("extra", {})
-        r64 = parity("fp64")
-        ex["l2_fp64_vs_oracle"] = r64["l2_vs_oracle"]
-        ex["parity_fp64"] = r64
-        if not args.no_extra:
-            r32p = parity("fp32")
-            ex["l2_fp32_bulk_vs_oracle"] = r32p["l2_vs_oracle"]
-            ex["parity_fp32_bulk"] = r32p
+        ex["parity"] = {prec: parity(prec) for prec in (PRECISIONS if not args.no_extra else ("fp64",))}
+        ex["l2_fp64_vs_oracle"] = ex["parity"]["fp64"]["l2_vs_oracle"]
+        out["cpu_baseline"] = cpu_cfg1(p)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.4g} grid-pt*Hpsi/s on {out['cpu_baseline']['cores']} "
+            f"threads ({out['cpu_baseline']['fft']}), {out['cpu_baseline']['value_1_thread']:.4g} on 1")
 
-    # ---- variants: FP32-bulk Chebyshev, GSPO -------------------------------
+    # ---- the benchmark configurations, every path --------------------------
     if not args.no_extra:
-        ex = out.setdefault("extra", {})
-        # timed without events (as the FP64 headline), kernels from a second pass
-        r32 = run_cheby(ctx, p, "fp32", args.steps, args.warmup, profile=False)
-        h32 = int(r32["n_hpsi"].sum())
-        rp32 = run_cheby(ctx, p, "fp32", args.steps, args.warmup, profile=True)
-        l32, ms32 = rp32["row"]
-        b32 = p.size * p.S * (B_ROW["fp32"] * (h32 - args.steps) + B_ROW_PRO["fp32"] * args.steps)
-        ex["fp32_bulk"] = {
-            "value": npts * h32 / r32["seconds"], "unit": UNIT,
-            "ms_per_step": r32["seconds"] * 1e3 / args.steps,
-            "hpsi_per_step": [int(r32["n_hpsi"].min()), int(r32["n_hpsi"].max())],
-            "row_kernel_achieved_GBs": b32 / (ms32 * 1e-3) / 1e9,
-            "row_kernel_avg_launch_us": ms32 / l32 * 1e3,
-            "col_kernel_avg_launch_us": rp32["line"][1] / max(rp32["line"][0], 1) * 1e3,
-            "vs_fp64": (npts * h32 / r32["seconds"]) / value,
-        }
-        log(f"fp32-bulk: {ex['fp32_bulk']['value']:.4g} grid-pt*Hpsi/s")
-        ex["cpu_1_thread"] = cpu_one_thread(p, 2)
-        ex["cheby_1d_256_latency"] = cheby_1d_small(ctx)
-        gsteps = max(args.steps // 4, 1)
+        configs = {}
+        cfg1 = {"metric": "grid-pt*Hpsi/s, 2D 512x512 two states, Chebyshev dt 5", "unit": UNIT,
+                "value": value, "roofline": out["roofline"], "cpu_baseline": out.get("cpu_baseline"),
+                "precisions": {"fp64": {"value": value, "ms_per_step": out["ms_per_step"],
+                                        "hpsi_per_step": out["config"]["hpsi_per_step"]}}}
+        for prec in ("compact", "fp32"):
+            # timed without events (as the FP64 headline), kernels from a second pass
+            rr = run_cheby(ctx, p, prec, args.steps, args.warmup, profile=False)
+            hh = int(rr["n_hpsi"].sum())
+            rq = run_cheby(ctx, p, prec, args.steps, args.warmup, profile=True)
+            ln, lms = rq["row"]
+            bb = p.size * p.S * (B_ROW[prec] * (hh - args.steps) + B_ROW_PRO[prec] * args.steps)
+            v = npts * hh / rr["seconds"]
+            cfg1["precisions"][prec] = {
+                "value": v, "unit": UNIT, "ms_per_step": rr["seconds"] * 1e3 / args.steps,
+                "hpsi_per_step": [int(rr["n_hpsi"].min()), int(rr["n_hpsi"].max())], "vs_fp64": v / value,
+                "l2_vs_oracle": ex.get("parity", {}).get(prec, {}).get("l2_vs_oracle"),
+                "roofline": {"bound": "hbm", "kernel": "k_cheby_row", "achieved": bb / (lms * 1e-3) / 1e9,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": bb / (lms * 1e-3) / 1e9 / HBM_PEAK_GBS, "avg_launch_us": lms / ln * 1e3,
+                             "bytes_model": f"{B_ROW[prec]} B per grid point and state per executed order"},
+                "col_kernel_avg_launch_us": rq["line"][1] / max(rq["line"][0], 1) * 1e3}
+            log(f"{prec}: {v:.4g} grid-pt*Hpsi/s")
+        configs["cheby_2d_512x512_2state"] = cfg1
+        # GSPO on the same workload: at least 100 timed steps
+        gsteps = max(100, args.steps)
+        gs = {"metric": "grid-pt*step/s, 2D 512x512 two coupled states, GSPO dt 5", "unit": "grid-pt*step/s",
+              "precisions": {}}
         for prec in ("fp64", "fp32"):
-            t = run_gspo(ctx, p, prec, gsteps, min(args.warmup, 10))
-            ex[f"gspo_{prec}"] = {"value": npts * gsteps / t, "unit": "grid-pt*step/s",
-                                  "ms_per_step": t * 1e3 / gsteps, "steps": gsteps}
-            log(f"gspo {prec}: {npts * gsteps / t:.4g} grid-pt*step/s")
+            t, rowk, colk = run_gspo(ctx, p, prec, gsteps, min(args.warmup, 10), profile=True)
+            v = npts * gsteps / t
+            eff = B_GSPO[prec] * v / 1e9
+            gs["precisions"][prec] = {
+                "value": v, "unit": "grid-pt*step/s", "ms_per_step": t * 1e3 / gsteps, "steps": gsteps,
+                "roofline": {"bound": "hbm", "kernel": "whole step (row pass + column pass)", "achieved": eff,
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": eff / HBM_PEAK_GBS,
+                             "bytes_model": f"{B_GSPO[prec]} B per grid point and state per step"},
+                "row_kernel_avg_launch_us": rowk[1] / max(rowk[0], 1) * 1e3, "row_launches": rowk[0],
+                "col_kernel_avg_launch_us": colk[1] / max(colk[0], 1) * 1e3, "col_launches": colk[0]}
+            log(f"gspo {prec}: {v:.4g} grid-pt*step/s, {t * 1e3 / gsteps:.4f} ms/step")
+        gs["value"] = gs["precisions"]["fp64"]["value"]
+        gs["roofline"] = gs["precisions"]["fp64"]["roofline"]
+        gs["cpu_baseline"] = cpu_gspo(p)
+        configs["gspo_2d_512x512_2state"] = gs
+        configs["precision_3d_128"] = precision_3d(ctx)
         if args.batch_steps > 0:
-            ex["batched"] = batched_extra(ctx, args.batch_steps, 5)
+            configs["batched_512_jobs"] = batched_extra(ctx, args.batch_steps, 5)
         if args.serve_jobs > 0:
-            ex["serving"] = serving_extra(ctx, [float(x) for x in args.serve_rates.split(",")], args.serve_jobs)
+            configs["serving"] = serving_extra(ctx, [float(x) for x in args.serve_rates.split(",")], args.serve_jobs)
+        ex["configs"] = configs
+        ex["cheby_1d_256_latency"] = cheby_1d_small(ctx)
     print(json.dumps(out), flush=True)
     ctx.close()
 

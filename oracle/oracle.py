@@ -50,6 +50,7 @@ def lib():
         L.orc_last_error.restype = C.c_char_p
         sig = {
             "orc_set_threads": [C.c_int],
+            "orc_set_fft_backend": [C.c_int],
             "orc_plant_mistake": [C.c_int],
             "orc_set_backward": [C.c_int, C.c_int, C.c_int],
             "orc_fft": [C.c_int, i32p, dp, C.c_int],
@@ -110,6 +111,14 @@ def _c128(a) -> np.ndarray:
 
 def set_threads(n: int) -> int:
     return lib().orc_set_threads(int(n))
+
+
+def set_fft_backend(which: str) -> str:
+    """'own' (the oracle's transform, used by every check) or 'fftw' (the
+    FFTW3 API through MKL's libmkl_rt.so, for the CPU baseline timing);
+    returns the backend in use."""
+    got = _chk(lib().orc_set_fft_backend(1 if which == "fftw" else 0))
+    return "fftw" if got == 1 else "own"
 
 
 def plant_mistake(code: int) -> None:

@@ -19,6 +19,8 @@
 #include <vector>
 #include <algorithm>
 #include <map>
+#include <dlfcn.h>
+#include <cstdlib>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -132,7 +134,87 @@ const Plan1D &plan_for(int n) {
 }
 
 /* ND transform over the QDng layout (dimension 0 contiguous) */
+/* Optional FFT backend for the CPU baseline timing: the public FFTW3 API
+ * (QDng's FFT library, src/fft/fft.cpp) as served by MKL's libmkl_rt.so,
+ * loaded at run time through these prototypes of the documented FFTW3
+ * interface.  Plans are made once per shape and direction (planning is not
+ * part of a step) and executed in place with fftw_execute_dft. */
+namespace fftw3 {
+typedef double fcomplex[2];
+typedef void *plan_t;
+typedef plan_t (*plan_dft_fn)(int rank, const int *n, fcomplex *in, fcomplex *out, int sign, unsigned flags);
+typedef void (*execute_dft_fn)(const plan_t p, fcomplex *in, fcomplex *out);
+typedef void (*destroy_fn)(plan_t p);
+typedef int (*init_threads_fn)(void);
+typedef void (*plan_with_nthreads_fn)(int n);
+typedef void (*set_threads_fn)(int n);
+constexpr unsigned kEstimate = 1u << 6;   /* FFTW_ESTIMATE */
+constexpr unsigned kUnaligned = 1u << 1;  /* FFTW_UNALIGNED */
+struct Api {
+    void *lib = nullptr;
+    plan_dft_fn plan_dft = nullptr;
+    execute_dft_fn execute_dft = nullptr;
+    destroy_fn destroy = nullptr;
+    init_threads_fn init_threads = nullptr;
+    plan_with_nthreads_fn plan_with_nthreads = nullptr;
+    set_threads_fn set_threads = nullptr;
+    bool on = false;
+    int threads = 1;
+    std::map<std::vector<int>, plan_t> plans;   /* key: sign, reversed dims */
+} g_fftw;
+
+bool load() {
+    if (g_fftw.lib) return true;
+    /* MKL's threads through GNU OpenMP, as the rest of this library */
+    setenv("MKL_THREADING_LAYER", "GNU", 0);
+    const char *names[] = {"libmkl_rt.so", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so", "libfftw3.so.3"};
+    for (const char *n : names) {
+        g_fftw.lib = dlopen(n, RTLD_NOW | RTLD_LOCAL);
+        if (g_fftw.lib) break;
+    }
+    if (!g_fftw.lib) return false;
+    g_fftw.plan_dft = (plan_dft_fn)dlsym(g_fftw.lib, "fftw_plan_dft");
+    g_fftw.execute_dft = (execute_dft_fn)dlsym(g_fftw.lib, "fftw_execute_dft");
+    g_fftw.destroy = (destroy_fn)dlsym(g_fftw.lib, "fftw_destroy_plan");
+    g_fftw.init_threads = (init_threads_fn)dlsym(g_fftw.lib, "fftw_init_threads");
+    g_fftw.plan_with_nthreads = (plan_with_nthreads_fn)dlsym(g_fftw.lib, "fftw_plan_with_nthreads");
+    g_fftw.set_threads = (set_threads_fn)dlsym(g_fftw.lib, "MKL_Set_Num_Threads");
+    if (!g_fftw.plan_dft || !g_fftw.execute_dft || !g_fftw.destroy) {
+        dlclose(g_fftw.lib);
+        g_fftw.lib = nullptr;
+        return false;
+    }
+    if (g_fftw.init_threads) g_fftw.init_threads();
+    return true;
+}
+
+void clear_plans() {
+    for (auto &kv : g_fftw.plans) g_fftw.destroy(kv.second);
+    g_fftw.plans.clear();
+}
+
+/* data[] <- the unnormalised transform (sign -1 forward), dimension 0 fastest
+ * (FFTW's rank-nd transform with the dimensions reversed, fft.cpp:91-94) */
+void run(int nd, const int *N, cplx *data, int sign) {
+    std::vector<int> key{sign};
+    for (int d = nd - 1; d >= 0; --d) key.push_back(N[d]);
+    auto it = g_fftw.plans.find(key);
+    if (it == g_fftw.plans.end()) {
+        long total = 1;
+        for (int d = 0; d < nd; ++d) total *= N[d];
+        std::vector<cplx> tmp(total);
+        if (g_fftw.plan_with_nthreads) g_fftw.plan_with_nthreads(g_fftw.threads);
+        plan_t pl = g_fftw.plan_dft(nd, key.data() + 1, reinterpret_cast<fcomplex *>(tmp.data()),
+                                    reinterpret_cast<fcomplex *>(tmp.data()), sign, kEstimate | kUnaligned);
+        if (!pl) throw OracleError("fftw_plan_dft failed");
+        it = g_fftw.plans.emplace(key, pl).first;
+    }
+    g_fftw.execute_dft(it->second, reinterpret_cast<fcomplex *>(data), reinterpret_cast<fcomplex *>(data));
+}
+}  // namespace fftw3
+
 void fftnd(int nd, const int *N, cplx *data, int sign) {
+    if (fftw3::g_fftw.on) return fftw3::run(nd, N, data, sign);
     long total = 1;
     for (int d = 0; d < nd; ++d) total *= N[d];
     long stride = 1;
@@ -972,7 +1054,23 @@ int orc_set_backward(int kind, int id, int backward) {
     });
 }
 
+/* FFT backend of the oracle: 0 its own transform, 1 the FFTW3 API through
+ * MKL's libmkl_rt.so (or libfftw3) when it loads; returns the backend in
+ * use.  For the CPU baseline timing only; the checks use backend 0. */
+int orc_set_fft_backend(int which) {
+    return guarded([&] {
+        fftw3::clear_plans();
+        fftw3::g_fftw.on = which == 1 && fftw3::load();
+        return fftw3::g_fftw.on ? 1 : 0;
+    });
+}
+
 int orc_set_threads(int n) {
+    if (n > 0) {
+        fftw3::g_fftw.threads = n;
+        if (fftw3::g_fftw.set_threads) fftw3::g_fftw.set_threads(n);
+        if (fftw3::g_fftw.lib) fftw3::clear_plans();   /* plans carry their thread count */
+    }
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);
     return omp_get_max_threads();

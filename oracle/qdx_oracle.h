@@ -37,6 +37,7 @@ extern "C" {
 /* ---- plumbing -------------------------------------------------------- */
 const char *orc_last_error(void);
 int orc_set_threads(int n);
+int orc_set_fft_backend(int which);   /* 0 own FFT, 1 FFTW3 API (MKL) if it loads */
 int orc_plant_mistake(int code);
 /* real-time propagator direction: kind 0 Cheby, 1 SIL, 2 Arnoldi, 3 GSPO */
 int orc_set_backward(int kind, int id, int backward);   /* test hook: 1 = transposed GSPO eigenvectors */
