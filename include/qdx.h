@@ -136,8 +136,21 @@ int qdx_ham_add_potential(qdx_ham *h, int32_t row, int32_t col, const double *V)
  * -mu(x) * laser[step] * scale */
 int qdx_ham_add_dipole(qdx_ham *h, int32_t row, int32_t col, const double *mu,
                        const double *laser, int32_t nt, double scale);
+/* OGobbler (OGobbler.cpp:32-204): Butterworth filter edges lp[d] / rp[d] (x
+ * units, one per dimension, NaN = none) of the given order, times gain.
+ * nip = 0: a real potential (bounds gain and 0); nip = 1: the negative
+ * imaginary potential -i v(x) (bounds gain and -i gain), single-state Sum
+ * Hamiltonians only, real-time Chebyshev and Arnoldi propagators (not GSPO,
+ * SIL, imaginary time, backward propagation or the batch). */
+int qdx_ham_add_gobbler(qdx_ham *h, int32_t row, int32_t col, const double *lp, const double *rp,
+                        int32_t order, double gain, int32_t nip);
 /* Emin / Emax with QDng's rules (OSum.cpp:28-51, OMultistate.cpp:163-204) */
 int qdx_ham_bounds(qdx_ham *h, double *emin, double *emax);
+/* the complex Emin / Emax (Operator::Emin / Emax, Operator.h:259-266) as
+ * (re, im) pairs; imaginary parts come from negative imaginary potentials */
+int qdx_ham_bounds_c(qdx_ham *h, double emin[2], double emax[2]);
+/* the grid the Hamiltonian was created on */
+int qdx_ham_grid(const qdx_ham *h, qdx_grid *out);
 /* dst = H(step) src (Operator::Apply, Operator.h:276) */
 int qdx_ham_apply(qdx_ham *h, qdx_wf *dst, qdx_wf *src, int64_t step);
 /* Re <psi|H(step)|psi> (Operator::Expec, Operator.h:249-252) */
@@ -170,6 +183,9 @@ typedef struct qdx_cheby_info {
     double scaling;       /* |Emax-Emin|/2   (OCheby.cpp:413-416)               */
     double prec;          /* break criterion (OCheby.cpp:179-182)               */
     double emin, emax;
+    double offset_i;      /* Im(Emax+Emin) (OCheby.cpp:408): non-zero with a
+                             negative imaginary potential (then R = 1.05
+                             |Emax-Emin| and no break test)                     */
 } qdx_cheby_info;
 
 /* OCheby::Init (OCheby.cpp:358-493), real-time forward propagation.

@@ -404,15 +404,15 @@ class DeviceOperator : public Operator {
     qdx_ham *ham() const { return ham_; }
     dcomplex Emax() override {
         need();
-        double lo, hi;
-        check(qdx_ham_bounds(ham_, &lo, &hi));
-        return hi;
+        double lo[2], hi[2];
+        check(qdx_ham_bounds_c(ham_, lo, hi));
+        return dcomplex(hi[0], hi[1]);
     }
     dcomplex Emin() override {
         need();
-        double lo, hi;
-        check(qdx_ham_bounds(ham_, &lo, &hi));
-        return lo;
+        double lo[2], hi[2];
+        check(qdx_ham_bounds_c(ham_, lo, hi));
+        return dcomplex(lo[0], lo[1]);
     }
     void Apply(WaveFunction *dst, WaveFunction *src) override {
         Init(src);
@@ -491,6 +491,63 @@ class OGridDipole : public OGridPotential {
     void AddTerms(qdx_ham *h, int row, int col) const override {
         if (v_.empty() || laser_.empty()) throw EParamProblem("GridDipole without dipole or laser");
         check(qdx_ham_add_dipole(h, row, col, v_.data(), laser_.data(), (int32_t)laser_.size(), scale_));
+    }
+};
+
+// OGobbler (qdlib/OGobbler.cpp): Butterworth low / high passes with edges
+// "lp<d>" / "rp<d>" (x units), "order" (20), "gain" (1); "nip" turns it into
+// the negative imaginary potential -i v (non-Hermitian; single-state Sum,
+// real-time Chebyshev and Arnoldi only); "residue": Expec = <O psi|O psi>
+class OGobbler : public DeviceOperator {
+    int order_ = 20;
+    double gain_ = 1.0;
+    bool nip_ = false, residue_ = false;
+    int dims_ = 0;
+
+  public:
+    OGobbler *NewInstance() const override { return new OGobbler(); }
+    const string &Name() const override {
+        static const string n = "OGobbler";
+        return n;
+    }
+    void Init(ParamContainer &pm) override {   // OGobbler.cpp:85-122
+        Operator::Init(pm);
+        if (pm.isPresent("order")) {
+            pm.GetValue("order", order_);
+            if (order_ < 1) throw EParamProblem("Filter order to small");
+        }
+        if (pm.isPresent("dims")) pm.GetValue("dims", dims_);
+        pm.GetValue("nip", nip_, false);
+        hermitian_ = !nip_;
+        if (pm.isPresent("gain")) pm.GetValue("gain", gain_);
+        pm.GetValue("residue", residue_, false);
+    }
+    using DeviceOperator::Init;
+    void Conj(bool c) override {
+        if (c && nip_) throw EIncompatible("backward propagation with a negative imaginary potential");
+        Operator::Conj(c);
+    }
+    using Operator::Conj;
+    dcomplex Emax() override { return dcomplex(gain_, 0); }                          // OGobbler.cpp:183-186
+    dcomplex Emin() override { return nip_ ? dcomplex(0, -gain_) : dcomplex(0, 0); }  // OGobbler.cpp:188-192
+    void AddTerms(qdx_ham *h, int row, int col) const override {
+        qdx_grid g{};
+        check(qdx_ham_grid(h, &g));
+        if (dims_ != 0 && dims_ != g.ndims) throw EIncompatible("OGobbler: Number of dims doesn't match WF");
+        double lp[4], rp[4];
+        for (int d = 0; d < 4; ++d) {
+            lp[d] = rp[d] = std::nan("");
+            const string k = std::to_string(d);
+            if (d < g.ndims && params_.isPresent("lp" + k)) params_.GetValue("lp" + k, lp[d]);
+            if (d < g.ndims && params_.isPresent("rp" + k)) params_.GetValue("rp" + k, rp[d]);
+        }
+        check(qdx_ham_add_gobbler(h, row, col, lp, rp, order_, gain_, nip_ ? 1 : 0));
+    }
+    double Expec(WaveFunction *psi) override {   // OGobbler.cpp:166-181
+        if (!residue_) return DeviceOperator::Expec(psi);
+        std::unique_ptr<WaveFunction> ket(psi->NewInstance());
+        Apply(ket.get(), psi);
+        return (*ket * *ket).real();
     }
 };
 

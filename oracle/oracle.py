@@ -65,6 +65,8 @@ def lib():
             "orc_op_sum": [ip, C.c_int],
             "orc_op_multistate": [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int],
             "orc_op_bounds": [C.c_int, dp, dp],
+            "orc_op_bounds_c": [C.c_int, dp, dp],
+            "orc_op_gobbler": [C.c_int, dp, dp, C.c_int, C.c_double, C.c_int],
             "orc_op_apply": [C.c_int, C.c_int, dp, dp],
             "orc_op_expec": [C.c_int, C.c_int, dp, C.c_int, dp],
             "orc_overlap": [C.c_int, C.c_int, dp, dp, dp],
@@ -185,6 +187,13 @@ class Op:
         _chk(lib().orc_op_bounds(self.id, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def bounds_c(self):
+        """(Emin, Emax) as complex numbers."""
+        a = np.zeros(2)
+        b = np.zeros(2)
+        _chk(lib().orc_op_bounds_c(self.id, _dp(a), _dp(b)))
+        return complex(a[0], a[1]), complex(b[0], b[1])
+
     def apply(self, psi: np.ndarray, step: int = 0) -> np.ndarray:
         src = _c128(psi)
         dst = np.zeros_like(src)
@@ -214,6 +223,18 @@ def dipole(grid: Grid, mu, laser, scale: float = 1.0) -> Op:
     e = _f64(laser).ravel()
     assert m.size == grid.size
     return Op(_chk(lib().orc_op_dipole(grid.id, _dp(m), _dp(e), e.size, float(scale))), 1, grid)
+
+
+def gobbler(grid: Grid, lp=None, rp=None, order: int = 20, gain: float = 1.0, nip: bool = False) -> Op:
+    """QDng's Gobbler: lp / rp map a dimension to its low-pass / high-pass
+    edge (x units); nip: the negative imaginary potential -i v."""
+    a = np.full(len(grid.N), np.nan)
+    b = np.full(len(grid.N), np.nan)
+    for d, x in (lp or {}).items():
+        a[d] = x
+    for d, x in (rp or {}).items():
+        b[d] = x
+    return Op(_chk(lib().orc_op_gobbler(grid.id, _dp(a), _dp(b), int(order), float(gain), int(bool(nip)))), 1, grid)
 
 
 def op_sum(*ops: Op) -> Op:

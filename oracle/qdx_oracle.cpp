@@ -310,7 +310,7 @@ std::vector<Grid> g_grids;
 /* ======================================================================
  * Operators
  * ==================================================================== */
-enum Kind { KIN, POT, DIP, SUM, MS };
+enum Kind { KIN, POT, DIP, SUM, MS, NIP };
 
 struct Op {
     Kind kind;
@@ -324,6 +324,8 @@ struct Op {
     int S = 0;                    /* MS */
     std::vector<int> mat;         /* MS: S*S op ids, -1 = empty */
     bool unity = true, hermitian = true, dmulti = false;
+    bool fixed = false;           /* Gobbler: Emax/Emin fixed by the gain */
+    cplx fmax = 0, fmin = 0;
 };
 
 std::vector<Op> g_ops;
@@ -414,7 +416,10 @@ void sym_eigen(int n, std::vector<double> a, std::vector<double> &ev, std::vecto
     if (vec) *vec = V;   /* columns are eigenvectors */
 }
 
-/* planted-mistake switch of the tests (0: the restatement as written) */
+/* planted-mistake switch of the tests (0: the restatement as written; 1:
+ * transposed GSPO eigenvectors, 2: +i v for the negative imaginary
+ * potential, 3: coefficient phase from Re(offset) only, 4: the Gobbler high pass
+ * of a NIP multiplied instead of added) */
 int g_plant = 0;
 
 /* laser value at clock step (Laser.cpp:125-128) */
@@ -427,6 +432,7 @@ cplx op_emax(const Op &o);
 cplx op_emin(const Op &o);
 
 cplx op_emax(const Op &o) {
+    if (o.fixed) return o.fmax;   /* OGobbler.cpp:183-186 */
     switch (o.kind) {
     case KIN: { /* OGridNablaSq.cpp:89-101 */
         const Grid &g = g_grids[o.grid];
@@ -437,6 +443,7 @@ cplx op_emax(const Op &o) {
         return T;
     }
     case POT: return *std::max_element(o.v.begin(), o.v.end());   /* OGridPotential.cpp:124-127 */
+    case NIP: return 0;   /* fixed bounds, returned above */
     case DIP:  /* OGridDipole.cpp:36-39: max(mu) * max(raw laser) */
         return *std::max_element(o.v.begin(), o.v.end()) * *std::max_element(o.laser.begin(), o.laser.end());
     case SUM: { /* OSum.cpp:28-39 */
@@ -457,9 +464,11 @@ cplx op_emax(const Op &o) {
 }
 
 cplx op_emin(const Op &o) {
+    if (o.fixed) return o.fmin;   /* OGobbler.cpp:188-192 */
     switch (o.kind) {
     case KIN: return 0;                                              /* OGridNablaSq.cpp:103-106 */
     case POT: return *std::min_element(o.v.begin(), o.v.end());      /* OGridPotential.cpp:129-132 */
+    case NIP: return 0;
     case DIP: return 0;                                              /* OGridDipole.cpp:41-44 */
     case SUM: {
         cplx d = 0;
@@ -499,6 +508,11 @@ void op_apply_add(const Op &o, int step, const cplx *src, cplx *dst) {
         for (long i = 0; i < n; ++i) dst[i] += src[i] * o.v[i] * e;
         return;
     }
+    case NIP: { /* OGobbler.cpp:197-201 through Operator.h:307-316: += src * (v i) * (-1) */
+        const double sgn = g_plant == 2 ? 1.0 : -1.0;   /* planted: +i v */
+        for (long i = 0; i < n; ++i) dst[i] += src[i] * (o.v[i] * cplx(0, 1)) * sgn;
+        return;
+    }
     default: { /* Operator.h:307-316: temporary + AddElements */
         std::vector<cplx> tmp(n * op_states(o));
         op_apply(o, step, src, tmp.data());
@@ -527,6 +541,11 @@ void op_apply(const Op &o, int step, const cplx *src, cplx *dst) {
     case DIP: {
         const double e = -laser_at(o, step);
         for (long i = 0; i < n; ++i) dst[i] = src[i] * o.v[i] * e;
+        return;
+    }
+    case NIP: { /* OGobbler.cpp:197-201: dest = src * (v i) * sgn, sgn = -1 */
+        const double sgn = g_plant == 2 ? 1.0 : -1.0;   /* planted: +i v */
+        for (long i = 0; i < n; ++i) dst[i] = src[i] * (o.v[i] * cplx(0, 1)) * sgn;
         return;
     }
     case SUM: /* OSum.cpp:53-60 */
@@ -669,7 +688,8 @@ int cheby_step(Cheby &c, int step, cplx *psi) {
     for (long j = 0; j < n; ++j) ket1[j] += -1.0 * c.offset * psi[j];   /* AddElements */
     const cplx f1 = exp_ / c.scaling;
     for (long j = 0; j < n; ++j) ket1[j] *= f1;                       /* MultElements */
-    const cplx ph = std::exp(exponent * c.offset);
+    /* planted 3: the phase of the real part of the offset only */
+    const cplx ph = std::exp(exponent * (g_plant == 3 ? cplx(c.offset.real(), 0) : c.offset));
     const cplx c0 = c.coeff[0] * ph, c1 = c.coeff[1] * ph;
     for (long j = 0; j < n; ++j) psi[j] *= c0;
     for (long j = 0; j < n; ++j) buf[j] = ket1[j] * c1;
@@ -1034,20 +1054,39 @@ int orc_plant_mistake(int code) {
     return 0;
 }
 
+/* the operator holds a negative imaginary potential */
+bool has_nip(const Op &o) {
+    if (o.kind == NIP) return true;
+    for (int k : o.kids)
+        if (has_nip(op_at(k))) return true;
+    for (int k : o.mat)
+        if (k >= 0 && has_nip(op_at(k))) return true;
+    return false;
+}
+
 /* OPropagator::Backward() / Forward() (OPropagator.h:62-100) on a real-time
  * propagator: kind 0 Chebyshev, 1 SIL, 2 Arnoldi, 3 GSPO.  Backward real time
  * has the exponent +i dt; the operators here are real, so their complex
- * conjugates (Operator::Conj, set by Backward) are themselves. */
+ * conjugates (Operator::Conj, set by Backward) are themselves.  Hamiltonians
+ * with a negative imaginary potential (not self-conjugate) are refused. */
 int orc_set_backward(int kind, int id, int backward) {
     return guarded([&] {
         auto chk = [&](bool imag) {
             if (imag) throw OracleError("imaginary-time propagators keep their direction");
         };
+        auto chk_h = [&](int ham) {
+            if (has_nip(op_at(ham)))
+                throw OracleError("backward propagation with a negative imaginary potential is not supported");
+        };
         switch (kind) {
-        case 0: chk(g_cheby.at(id).imag); g_cheby.at(id).backward = backward != 0; break;
-        case 1: chk(g_sil.at(id).imag); g_sil.at(id).backward = backward != 0; break;
-        case 2: chk(g_arnoldi.at(id).imag); g_arnoldi.at(id).backward = backward != 0; break;
-        case 3: chk(g_gspo.at(id).imag); g_gspo.at(id).backward = backward != 0; break;
+        case 0: chk(g_cheby.at(id).imag); chk_h(g_cheby.at(id).ham); g_cheby.at(id).backward = backward != 0; break;
+        case 1: chk(g_sil.at(id).imag); chk_h(g_sil.at(id).ham); g_sil.at(id).backward = backward != 0; break;
+        case 2: chk(g_arnoldi.at(id).imag); chk_h(g_arnoldi.at(id).ham); g_arnoldi.at(id).backward = backward != 0; break;
+        case 3:
+            chk(g_gspo.at(id).imag);
+            for (int o : g_gspo.at(id).ops) chk_h(o);
+            g_gspo.at(id).backward = backward != 0;
+            break;
         default: throw OracleError("unknown propagator kind");
         }
         return 0;
@@ -1156,6 +1195,58 @@ int orc_op_potential(int grid, const double *V) {
     });
 }
 
+/* OGobbler::_Init (OGobbler.cpp:32-82) with its parameters of
+ * OGobbler::Init (OGobbler.cpp:85-164): per dimension d an optional
+ * Butterworth low pass of edge lp[d] and high pass of edge rp[d] (NaN =
+ * absent; edges in grid points x = (edge - xmin) / dx, checked against the
+ * grid as there), filters f(i) = 1 / (1 + (i / x)^(2 order)) and
+ * 1 / (1 + (x / i)^(2 order)) (math_functions.h:40-69) applied along the
+ * dimension: the low pass multiplies, the high pass multiplies (nip = 0,
+ * start value 1) or adds (nip = 1, start value 0); finally times gain.
+ * nip = 1 gives the negative imaginary potential -i v (OGobbler.cpp:195-204);
+ * the bounds are Emax = gain, Emin = -i gain (nip) or 0 (OGobbler.cpp:183-192). */
+int orc_op_gobbler(int grid, const double *lp, const double *rp, int order, double gain, int nip) {
+    return guarded([&] {
+        const Grid &g = g_grids.at(grid);
+        if (order < 1) throw OracleError("Filter order to small");
+        Op o;
+        o.kind = nip ? NIP : POT;
+        o.grid = grid;
+        o.fixed = true;
+        o.fmax = cplx(gain, 0);
+        o.fmin = nip ? cplx(0, -gain) : cplx(0, 0);
+        o.hermitian = !nip;
+        const long n = g.size();
+        o.v.assign(n, nip ? 0.0 : 1.0);
+        long stride = 1;
+        for (int d = 0; d < g.nd; ++d) {
+            const int Nd = g.N[d];
+            for (int side = 0; side < 2; ++side) {
+                const double edge = side == 0 ? lp[d] : rp[d];
+                if (std::isnan(edge)) continue;
+                const double x = (edge - g.xmin[d]) / g.dx(d);
+                if (int(x) > Nd || int(x) < 0) throw OracleError("Center of filter edge out of the grid");
+                std::vector<double> f(Nd);
+                for (int i = 0; i < Nd; ++i)
+                    f[i] = side == 0 ? 1 / (1 + std::pow(double(i) / x, 2 * double(order)))
+                                     : 1 / (1 + std::pow(x / double(i), 2 * double(order)));
+                for (long j = 0; j < n; ++j) {
+                    const double fv = f[(j / stride) % Nd];
+                    if (side == 1 && nip && g_plant != 4)   /* planted: the high pass multiplies */
+                        o.v[j] += fv;
+                    else
+                        o.v[j] *= fv;
+                }
+            }
+            stride *= Nd;
+        }
+        if (gain != 1)
+            for (double &x : o.v) x *= gain;
+        g_ops.push_back(o);
+        return (int)g_ops.size() - 1;
+    });
+}
+
 int orc_op_dipole(int grid, const double *mu, const double *laser, int nt, double scale) {
     return guarded([&] {
         const Grid &g = g_grids.at(grid);
@@ -1220,6 +1311,18 @@ int orc_op_bounds(int op, double *emin, double *emax) {
         const Op &o = op_at(op);
         *emin = op_emin(o).real();
         *emax = op_emax(o).real();
+        return 0;
+    });
+}
+
+int orc_op_bounds_c(int op, double *emin, double *emax) {
+    return guarded([&] {
+        const Op &o = op_at(op);
+        const cplx a = op_emin(o), b = op_emax(o);
+        emin[0] = a.real();
+        emin[1] = a.imag();
+        emax[0] = b.real();
+        emax[1] = b.imag();
         return 0;
     });
 }

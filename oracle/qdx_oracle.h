@@ -38,9 +38,11 @@ extern "C" {
 const char *orc_last_error(void);
 int orc_set_threads(int n);
 int orc_set_fft_backend(int which);   /* 0 own FFT, 1 FFTW3 API (MKL) if it loads */
+/* test hook: 1 transposed GSPO eigenvectors, 2 +i v NIP, 3 coefficient phase
+ * from Re(offset) only, 4 NIP high pass multiplied */
 int orc_plant_mistake(int code);
 /* real-time propagator direction: kind 0 Cheby, 1 SIL, 2 Arnoldi, 3 GSPO */
-int orc_set_backward(int kind, int id, int backward);   /* test hook: 1 = transposed GSPO eigenvectors */
+int orc_set_backward(int kind, int id, int backward);
 
 /* ---- FFT (dimension 0 fastest, unnormalised, sign -1 forward) -------- */
 int orc_fft(int ndims, const int32_t *N, double *data, int sign);
@@ -60,12 +62,16 @@ int orc_op_potential(int grid, const double *V);              /* GridPotential *
 int orc_op_dipole(int grid, const double *mu, const double *laser, int nt,
                   double scale);                               /* GridDipole   */
 int orc_op_sum(const int *ops, int n);                         /* Sum          */
+/* Gobbler: lp, rp hold one edge per dimension, NaN = none; nip = 1: -i v */
+int orc_op_gobbler(int grid, const double *lp, const double *rp, int order, double gain, int nip);
 /* Multistate (dmulti = 0) or DMultistate (dmulti = 1). rows/cols/ops give the
  * matrix elements; unity: missing diagonal acts as identity (QDng default). */
 int orc_op_multistate(int nstates, const int *rows, const int *cols,
                       const int *ops, int n, int unity, int dmulti);
 
 int orc_op_bounds(int op, double *emin, double *emax);
+/* complex Emin / Emax: (re, im) pairs */
+int orc_op_bounds_c(int op, double *emin, double *emax);
 /* dest = O(step) src ; vectors hold nstates*size complex values */
 int orc_op_apply(int op, int step, const double *src, double *dest);
 /* Re <psi|O|psi> with grid weights (QDng Operator::Expec) */

@@ -41,7 +41,7 @@ class Grid(C.Structure):
 class ChebyInfo(C.Structure):
     _fields_ = [("order", C.c_int32), ("nz", C.c_int32), ("offset", C.c_double),
                 ("scaling", C.c_double), ("prec", C.c_double), ("emin", C.c_double),
-                ("emax", C.c_double)]
+                ("emax", C.c_double), ("offset_i", C.c_double)]
 
 
 class StepStats(C.Structure):
@@ -52,7 +52,7 @@ EXPORTS = [
     "qdx_last_error", "qdx_version", "qdx_ctx_create", "qdx_ctx_synchronize", "qdx_ctx_destroy",
     "qdx_wf_create", "qdx_wf_upload", "qdx_wf_download", "qdx_wf_copy", "qdx_wf_norm",
     "qdx_wf_overlap", "qdx_wf_normalize", "qdx_wf_destroy", "qdx_ham_create",
-    "qdx_ham_add_kinetic", "qdx_ham_add_potential", "qdx_ham_add_dipole", "qdx_ham_bounds",
+    "qdx_ham_add_kinetic", "qdx_ham_add_potential", "qdx_ham_add_dipole", "qdx_ham_add_gobbler", "qdx_ham_bounds",
     "qdx_ham_apply", "qdx_ham_expec", "qdx_ham_expec_noalloc", "qdx_ham_destroy", "qdx_bessel_j0", "qdx_cheby_create",
     "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
     "qdx_prop_apply", "qdx_prop_run", "qdx_prop_set_direction", "qdx_prop_exponent", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
@@ -134,6 +134,7 @@ def lib():
         "qdx_ham_add_kinetic": [vp, i32, dp],
         "qdx_ham_add_potential": [vp, i32, i32, dp],
         "qdx_ham_add_dipole": [vp, i32, i32, dp, dp, i32, C.c_double],
+        "qdx_ham_add_gobbler": [vp, i32, i32, dp, dp, i32, C.c_double, i32],
         "qdx_ham_bounds": [vp, dp, dp],
         "qdx_ham_apply": [vp, vp, vp, C.c_int64],
         "qdx_ham_expec": [vp, vp, C.c_int64, dp],
@@ -566,6 +567,20 @@ class Hamiltonian:
         if m.size != self.size:
             raise ValueError("dipole size does not match the grid")
         _chk(lib().qdx_ham_add_dipole(self.h, int(row), int(col), _dp(m), _dp(e), e.size, float(scale)))
+        return self
+
+    def add_gobbler(self, row: int, col: int, lp=None, rp=None, order: int = 20, gain: float = 1.0,
+                    nip: bool = False) -> "Hamiltonian":
+        """OGobbler: lp / rp map a dimension to its low-pass / high-pass edge
+        (x units); nip: the negative imaginary potential -i v (qdx_ham_add_gobbler)"""
+        a = np.full(len(self.N), np.nan)
+        b = np.full(len(self.N), np.nan)
+        for d, x in dict(lp or {}).items():
+            a[int(d)] = float(x)
+        for d, x in dict(rp or {}).items():
+            b[int(d)] = float(x)
+        _chk(lib().qdx_ham_add_gobbler(self.h, int(row), int(col), _dp(a), _dp(b), int(order), float(gain),
+                                       int(bool(nip))))
         return self
 
     def add_potential_file(self, row: int, col: int, name: str, setzero: bool = False, offset: float = 0.0,

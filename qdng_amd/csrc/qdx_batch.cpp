@@ -142,6 +142,7 @@ JobHost cheby_host(qdx_ham *h, int nsteps) {
     const std::vector<double> *las = nullptr;
     for (const Term &t : h->terms) {
         if (t.row != 0 || t.col != 0) fail(QDX_ERR_INCOMPAT, "batched Chebyshev jobs hold one state");
+        if (t.kind == T_NIP) fail(QDX_ERR_INCOMPAT, "batched jobs take no negative imaginary potential");
         if (t.kind == T_POT) {
             add_field(j.V, t.v);
         } else if (t.kind == T_DIP) {
@@ -173,6 +174,7 @@ JobHost gspo_host(qdx_ham *T, qdx_ham *W, int nsteps) {
     bool c01 = false, c10 = false;
     for (const Term &t : W->terms) {
         if (t.kind == T_KIN) fail(QDX_ERR_INCOMPAT, "the second GSPO operator must hold potentials only");
+        if (t.kind == T_NIP) fail(QDX_ERR_INCOMPAT, "batched jobs take no negative imaginary potential");
         if (t.kind == T_DIP) {
             if (S != 1) fail(QDX_ERR_INCOMPAT, "batched GSPO jobs take a dipole on a single state only");
             if (las && (*las != t.laser || j.lscale != t.scale))

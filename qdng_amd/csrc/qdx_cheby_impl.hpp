@@ -56,7 +56,11 @@ __global__ void __launch_bounds__(512) k_cheby_init(GridDev g, LinePlan p0, int 
 // registers per thread of 512).  P: points per thread.  IMAG: imaginary
 // time (the recurrence signs of OCheby::imag_time_step).  CMP: the terms
 // phi and the kinetic part Y in the compact 40-bit storage (R double).
-template <class R, int S, bool ONE_D, bool PRO, int NT, int P = 8, bool IMAG = false, bool CMP = false>
+// NIP: single-state real time with a negative imaginary potential: H adds
+// -i Wi(x) and the offset its imaginary part (b = (H - offset) phi / R with
+// complex offset, OCheby.cpp:104-105, 168).
+template <class R, int S, bool ONE_D, bool PRO, int NT, int P = 8, bool IMAG = false, bool CMP = false,
+          bool NIP = false>
 __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePlan p0, ChebyOrder ord,
                                                     ConvCheck cc, int rb, const cx<R> *__restrict__ phim1,
                                                     cx<R> *__restrict__ phim2, cx<R> *__restrict__ X,
@@ -123,6 +127,7 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
         Ed[j] = h.Edip[st][j];
     }
     const double uni = h.unity[st];
+    const R *Wip = NIP ? h.Wi[st] : nullptr;
     // fused epilogue: H phi = T phi + sum_j (W_ij + unity - E mu_ij) phi_j,
     // in chunks of four points.  The first chunk's operands are requested
     // before the transforms (their latency hides behind them), the second
@@ -133,6 +138,7 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
         cx<double> ps[4];
         cx<R> m2[4], yv[4], pm[S][4];
         double wv[S][4];
+        double wi[4];   // NIP: Wi + Im offset
     };
     auto load_half = [&](Half &H, int h0) {
 #pragma unroll
@@ -149,6 +155,7 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
                 if (Mp[j]) w += Ed[j] * (double)Mp[j][gi];
                 H.wv[j][u] = w;
             }
+            if (NIP) H.wi[u] = ord.offset_i + (Wip ? (double)Wip[gi] : 0.0);
         }
     };
     double acc = 0;
@@ -171,9 +178,15 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
                 hp.r += w * pj.r;
                 hp.i += w * pj.i;
             }
-            // b = (H phi - offset phi) / R
-            const cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
-                                            (hp.i - ord.offset * p.i) * ord.inv_scaling);
+            // b = (H phi - offset phi) / R; NIP: H - offset carries
+            // -i (Wi + Im offset) on the diagonal
+            cx<double> b = mk<double>((hp.r - ord.offset * p.r) * ord.inv_scaling,
+                                      (hp.i - ord.offset * p.i) * ord.inv_scaling);
+            if (NIP) {
+                const double c = H.wi[u] * ord.inv_scaling;
+                b.r += c * p.i;
+                b.i -= c * p.r;
+            }
             cx<double> pk;
             if (PRO) {
                 pk = IMAG ? mk<double>(-b.r, -b.i)                        // * (-1)
@@ -225,10 +238,10 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
 // launch per step instead of one per order.  coef[2k], coef[2k+1]: coeff[k]
 // exp(-i offset dt).  partials: the per-order slots of the multi-kernel path
 // ([k % 2] * kMaxPartials, step start at 2 * kMaxPartials), one sum each.
-template <class R, int NT>
+template <class R, int NT, bool NIP = false>
 __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LinePlan p0, const double *coef,
-                                                   int order, double offset, double inv_scaling, double prec,
-                                                   cx<double> *__restrict__ psi, ChebyCtl *ctl,
+                                                   int order, double offset, double offset_i, double inv_scaling,
+                                                   double prec, cx<double> *__restrict__ psi, ChebyCtl *ctl,
                                                    double *__restrict__ partials) {
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ double red[16];
@@ -238,11 +251,12 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
     const bool has_t = tk0 != nullptr;
     const R *Wp = h.W[0][0];
     const R *Mp = h.mu[0][0];
+    const R *Wip = NIP ? h.Wi[0] : nullptr;
     const double Ed = h.Edip[0][0];
     const double uni = h.unity[0];
     cx<double> ps[8];
     cx<R> pm1[8], pm2[8], v[8];
-    double wv[8], tq[8];
+    double wv[8], tq[8], wi[8];
     double acc = 0;
     if (B.active) {
 #pragma unroll
@@ -255,6 +269,7 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
             if (Wp) w += (double)Wp[gi];
             if (Mp) w += Ed * (double)Mp[gi];
             wv[q] = w;
+            if (NIP) wi[q] = (offset_i + (Wip ? (double)Wip[gi] : 0.0)) * inv_scaling;
             tq[q] = has_t ? (0.0 + tk0[B.lt + q * B.TL]) * h.inv_nd[0] : 0.0;
         }
     }
@@ -311,7 +326,11 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
                 const cx<double> p = to_d(pm1[q]);
                 hp.r += wv[q] * p.r;
                 hp.i += wv[q] * p.i;
-                const cx<double> b = mk<double>((hp.r - offset * p.r) * inv_scaling, (hp.i - offset * p.i) * inv_scaling);
+                cx<double> b = mk<double>((hp.r - offset * p.r) * inv_scaling, (hp.i - offset * p.i) * inv_scaling);
+                if (NIP) {
+                    b.r += wi[q] * p.i;
+                    b.i -= wi[q] * p.r;
+                }
                 cx<double> pk;
                 if (k == 1) {
                     pk = mk<double>(b.i, -b.r);
@@ -396,7 +415,30 @@ static int cheby_row_SKP(hipStream_t st, const GridDev &g, const HamDev<R> &h, c
     auto cphim2 = reinterpret_cast<cx<R> *>(phim2);
     auto cX = reinterpret_cast<cx<R> *>(X);
     auto cpsi = reinterpret_cast<cx<double> *>(psi);
-    if (cmp) {   // compact terms: FP64 arithmetic, real time, nd >= 2, S <= 2
+    if (ord.offset_i != 0 || h.Wi[0] != nullptr) {
+        // negative imaginary potential: one state, real time (512-thread
+        // instances; the block never exceeds 512 threads)
+        if constexpr (S == 1) {
+            if (imag) throw std::runtime_error("negative imaginary potential: real time only");
+            if (cmp) {
+                if constexpr (std::is_same<R, double>::value && !ONE_D) {
+                    set_lds(k_cheby_row<R, 1, ONE_D, PRO, 512, P, false, true, true>, r.lds);
+                    QDX_LAUNCH((k_cheby_row<R, 1, ONE_D, PRO, 512, P, false, true, true>), dim3(r.nb),
+                               dim3(r.threads), r.lds, st, g, h, p0, ord, cc, r.rb, cphim1, cphim2, cX, cpsi, ctl,
+                               partials, init_part, nb_init);
+                } else {
+                    throw std::runtime_error("compact terms: FP64, nd >= 2");
+                }
+            } else {
+                set_lds(k_cheby_row<R, 1, ONE_D, PRO, 512, P, false, false, true>, r.lds);
+                QDX_LAUNCH((k_cheby_row<R, 1, ONE_D, PRO, 512, P, false, false, true>), dim3(r.nb), dim3(r.threads),
+                           r.lds, st, g, h, p0, ord, cc, r.rb, cphim1, cphim2, cX, cpsi, ctl, partials, init_part,
+                           nb_init);
+            }
+        } else {
+            throw std::runtime_error("negative imaginary potential: single-state Hamiltonians only");
+        }
+    } else if (cmp) {   // compact terms: FP64 arithmetic, real time, nd >= 2, S <= 2
         if constexpr (std::is_same<R, double>::value && !ONE_D && S <= 2) {
             if (imag) throw std::runtime_error("compact terms: real time only");
             if (r.threads <= 256) {
@@ -479,8 +521,8 @@ int Launch<R>::cheby_row(hipStream_t s, const GridDev &g, const HamDev<R> &h, co
 
 template <class R>
 void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                           const double *coef, int order, double offset, double inv_scaling, double prec,
-                           double *psi, ChebyCtl *ctl, double *partials) {
+                           const double *coef, int order, double offset, double offset_i, double inv_scaling,
+                           double prec, double *psi, ChebyCtl *ctl, double *partials) {
     const int L = g.N[0], TL = L / 8;
     if (g.nd != 1 || h.S != 1 || L > kMaxLine) throw std::runtime_error("whole-step line kernel: one 1D line");
     LinePlan p = p0;
@@ -489,14 +531,23 @@ void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, 
     if (!p.tw_lds) lds = row_lds_bytes<R>(L, 1, false);
     const int threads = std::max(64, round64(TL));
     auto cpsi = reinterpret_cast<cx<double> *>(psi);
-    if (threads <= 256) {
+    const bool nip = offset_i != 0 || h.Wi[0] != nullptr;   // negative imaginary potential
+    if (nip && threads <= 256) {
+        set_lds(k_cheby_line<R, 256, true>, lds);
+        QDX_LAUNCH((k_cheby_line<R, 256, true>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset,
+                   offset_i, inv_scaling, prec, cpsi, ctl, partials);
+    } else if (nip) {
+        set_lds(k_cheby_line<R, 512, true>, lds);
+        QDX_LAUNCH((k_cheby_line<R, 512, true>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset,
+                   offset_i, inv_scaling, prec, cpsi, ctl, partials);
+    } else if (threads <= 256) {
         set_lds(k_cheby_line<R, 256>, lds);
-        QDX_LAUNCH((k_cheby_line<R, 256>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, inv_scaling,
-                   prec, cpsi, ctl, partials);
+        QDX_LAUNCH((k_cheby_line<R, 256>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, 0.0,
+                   inv_scaling, prec, cpsi, ctl, partials);
     } else {
         set_lds(k_cheby_line<R, 512>, lds);
-        QDX_LAUNCH((k_cheby_line<R, 512>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, inv_scaling,
-                   prec, cpsi, ctl, partials);
+        QDX_LAUNCH((k_cheby_line<R, 512>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, 0.0,
+                   inv_scaling, prec, cpsi, ctl, partials);
     }
     QDX_HIP(hipGetLastError());
 }
@@ -508,6 +559,7 @@ void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, 
                                       const ChebyOrder &, const ConvCheck &, bool, const R *, R *, R *, double *,    \
                                       ChebyCtl *, double *, const double *, int, bool);                             \
     template void Launch<R>::cheby_line(hipStream_t, const GridDev &, const HamDev<R> &, const LinePlan &,           \
-                                        const double *, int, double, double, double, double *, ChebyCtl *, double *);
+                                        const double *, int, double, double, double, double, double *, ChebyCtl *,  \
+                                        double *);
 
 }  // namespace qdx

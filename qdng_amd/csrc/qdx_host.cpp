@@ -101,7 +101,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     double *init = part + 2 * kMaxPartials;
     if (p->line_step) {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        Launch<R>::cheby_line(s, g, h, plans[0], p->coefd.as<double>(), p->k_launch, p->cp.offset,
+        Launch<R>::cheby_line(s, g, h, plans[0], p->coefd.as<double>(), p->k_launch, p->cp.offset, p->cp.offset_i,
                               1.0 / p->cp.scaling, p->cp.prec, psi->ptr(), ctl, p->partials.as<double>());
         p->last_nb = 1;
         p->nb_init = 1;
@@ -127,11 +127,14 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
             Launch<R>::kin_pass(s, g, S, d, d > 1, plans[d], h, pm1, X, ctl, c, p->cmp);
         }
     };
-    // exp(exponent * offset), exponent -i dt (real time) or -dt (imaginary)
-    const std::complex<double> ph = p->imag ? std::complex<double>(std::exp(-p->dt * p->cp.offset), 0.0)
-                                            : std::exp(std::complex<double>(0, -1) * p->dt * p->cp.offset);
+    // exp(exponent * offset), exponent -i dt (real time) or -dt (imaginary);
+    // the offset is complex with a negative imaginary potential
+    const std::complex<double> ph =
+        p->imag ? std::complex<double>(std::exp(-p->dt * p->cp.offset), 0.0)
+                : std::exp(std::complex<double>(0, -1) * p->dt * std::complex<double>(p->cp.offset, p->cp.offset_i));
     ChebyOrder ord;
     ord.offset = p->cp.offset;
+    ord.offset_i = p->cp.offset_i;
     ord.inv_scaling = 1.0 / p->cp.scaling;
     ord.prec = p->cp.prec;
     const std::complex<double> c0 = p->cp.coeff[0] * ph, c1 = p->cp.coeff[1] * ph;
@@ -775,6 +778,62 @@ int qdx_ham_add_potential(qdx_ham *h, int32_t row, int32_t col, const double *V)
     });
 }
 
+// OGobbler (OGobbler.cpp:32-204): Butterworth edges per dimension, the low
+// pass 1 / (1 + (i/x)^(2 order)) and the high pass 1 / (1 + (x/i)^(2 order))
+// over the grid index i with the edge x in grid points
+// (math_functions.h:40-69).  Start value 1, both passes multiply (nip = 0,
+// a real potential); start value 0, the high pass adds (nip = 1, the
+// negative imaginary potential -i v).  Times gain at the end.
+int qdx_ham_add_gobbler(qdx_ham *h, int32_t row, int32_t col, const double *lp, const double *rp, int32_t order,
+                        double gain, int32_t nip) {
+    return guarded([&] {
+        check_elem(h, row, col);
+        if (!lp || !rp) fail(QDX_ERR_PARAM, "lp / rp is NULL");
+        if (order < 1) fail(QDX_ERR_PARAM, "Filter order to small");
+        if (!std::isfinite(gain)) fail(QDX_ERR_PARAM, "gain is not finite");
+        if (nip && (h->multistate || row != col))
+            fail(QDX_ERR_INCOMPAT, "negative imaginary potential: single-state (Sum) Hamiltonians only");
+        const Grid &g = h->grid;
+        const long n = g.size();
+        Term t;
+        t.kind = nip ? T_NIP : T_POT;
+        t.row = row;
+        t.col = col;
+        t.fixed = true;
+        t.fmax = {gain, 0.0};
+        t.fmin = nip ? std::complex<double>(0.0, -gain) : std::complex<double>(0.0, 0.0);
+        t.v.assign(n, nip ? 0.0 : 1.0);
+        long stride = 1;
+        for (int d = 0; d < g.nd; ++d) {
+            const int Nd = g.N[d];
+            for (int pass = 0; pass < 2; ++pass) {
+                const double e = pass == 0 ? lp[d] : rp[d];
+                if (std::isnan(e)) continue;
+                const double x = (e - g.xmin[d]) / g.dx(d);
+                // int(x) > N or int(x) < 0 as in OGobbler::_Init, without the cast
+                if (!std::isfinite(x) || x >= Nd + 1.0 || x <= -1.0)
+                    fail(QDX_ERR_PARAM, "Center of filter edge out of the grid");
+                std::vector<double> f(Nd);
+                for (int i = 0; i < Nd; ++i) {
+                    const double r = pass == 0 ? double(i) / x : x / double(i);
+                    f[i] = 1 / (1 + std::pow(r, 2 * double(order)));
+                }
+                const bool add = pass == 1 && nip;
+                for (long j = 0; j < n; ++j) {
+                    const double fv = f[(j / stride) % Nd];
+                    t.v[j] = add ? t.v[j] + fv : t.v[j] * fv;
+                }
+            }
+            stride *= Nd;
+        }
+        if (gain != 1)
+            for (double &v : t.v) v *= gain;
+        for (double v : t.v)
+            if (std::isnan(v)) fail(QDX_ERR_PARAM, "Gobbler filter holds NaN (an edge at the first grid point)");
+        h->terms.push_back(t);
+    });
+}
+
 int qdx_ham_add_dipole(qdx_ham *h, int32_t row, int32_t col, const double *mu, const double *laser, int32_t nt,
                        double scale) {
     return guarded([&] {
@@ -796,6 +855,26 @@ int qdx_ham_bounds(qdx_ham *h, double *emin, double *emax) {
     return guarded([&] {
         if (!h || !emin || !emax) fail(QDX_ERR_PARAM, "NULL argument");
         h->bounds(*emin, *emax);
+    });
+}
+
+int qdx_ham_bounds_c(qdx_ham *h, double emin[2], double emax[2]) {
+    return guarded([&] {
+        if (!h || !emin || !emax) fail(QDX_ERR_PARAM, "NULL argument");
+        h->bounds(emin[0], emax[0], &emin[1], &emax[1]);
+    });
+}
+
+int qdx_ham_grid(const qdx_ham *h, qdx_grid *out) {
+    return guarded([&] {
+        if (!h || !out) fail(QDX_ERR_PARAM, "NULL argument");
+        *out = qdx_grid{};
+        out->ndims = h->grid.nd;
+        for (int d = 0; d < h->grid.nd; ++d) {
+            out->N[d] = h->grid.N[d];
+            out->xmin[d] = h->grid.xmin[d];
+            out->xmax[d] = h->grid.xmax[d];
+        }
     });
 }
 
@@ -913,6 +992,7 @@ int qdx_cheby_params(qdx_ham *h, double dt, int32_t order, double prec, double s
         info->prec = c.prec;
         info->emin = c.emin;
         info->emax = c.emax;
+        info->offset_i = c.offset_i;
         if (coeff)
             for (int i = 0; i < std::min<int>(coeff_len, c.order); ++i) coeff[i] = c.coeff[i];
     });
@@ -961,7 +1041,8 @@ static void cheby_create_impl(qdx_ham *h, double dt, int32_t order, double prec,
                        p->grid.N[0] <= (precision == QDX_FP32_BULK ? kMaxLine : 2048);
         if (p->line_step) {
             // the same products as the per-order scalars of cheby_step
-            const std::complex<double> ph = std::exp(std::complex<double>(0, -1) * dt * p->cp.offset);
+            const std::complex<double> ph =
+                std::exp(std::complex<double>(0, -1) * dt * std::complex<double>(p->cp.offset, p->cp.offset_i));
             std::vector<double> c(2 * (size_t)std::max(p->k_launch, 1));
             for (int k = 0; k < p->k_launch; ++k) {
                 const std::complex<double> v = p->cp.coeff[k] * ph;
@@ -1005,6 +1086,7 @@ int qdx_cheby_info_get(qdx_prop *p, qdx_cheby_info *info) {
         info->prec = p->cp.prec;
         info->emin = p->cp.emin;
         info->emax = p->cp.emax;
+        info->offset_i = p->cp.offset_i;
     });
 }
 
@@ -1034,6 +1116,7 @@ static void gspo_create_impl(qdx_ham *const *ops, int32_t nops, double dt, int32
             qdx_ham *o = ops[i];
             if (!o || !o->ctx) fail(QDX_ERR_PARAM, "GSPO misses an operator. Sequence must be A, B, C, ...");
             o->finalize();
+            if (o->has_nip()) fail(QDX_ERR_INCOMPAT, "GSPO: a negative imaginary potential has no split operator here");
             if (i == 0) {
                 p->ctx = o->ctx;
                 p->grid = o->grid;
@@ -1673,6 +1756,10 @@ static void krylov_create(qdx_ham *h, double dt, int32_t order, double err, int3
         if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
         if (order <= 0) order = 20;                        // OSIL.h:11, OArnoldi.h:11
         if (order < 3) fail(QDX_ERR_PARAM, arnoldi ? "Arnoldi: order doesn't make sense" : "SIL: order doesn't make sense");
+        h->finalize();
+        if (h->has_nip() && (!arnoldi || imag))
+            fail(QDX_ERR_INCOMPAT, "negative imaginary potential: SIL (Lanczos) needs a Hermitian Hamiltonian; "
+                                   "Arnoldi runs it in real time only");
         auto p = std::make_unique<qdx_prop>();
         p->kind = arnoldi ? qdx_prop::ARNOLDI : qdx_prop::SIL;
         p->ctx = h->ctx;
@@ -1732,6 +1819,9 @@ int qdx_prop_set_direction(qdx_prop *p, int32_t backward) {
             if (!backward) fail(QDX_ERR_INCOMPAT, "imaginary-time propagators run backward (exponent -dt) only");
             return;
         }
+        // backward = conj(forward(conj psi)) holds for self-conjugate H only
+        if (backward && p->ham && p->ham->has_nip())
+            fail(QDX_ERR_INCOMPAT, "backward propagation with a negative imaginary potential is not supported");
         p->backward = backward != 0;
     });
 }

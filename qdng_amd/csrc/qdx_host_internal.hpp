@@ -385,15 +385,20 @@ struct qdx_wf {
 
 namespace qdx {
 
-enum TermKind { T_KIN = 0, T_POT = 1, T_DIP = 2 };
+// T_NIP: negative imaginary potential -i v(x) (OGobbler with nip)
+enum TermKind { T_KIN = 0, T_POT = 1, T_DIP = 2, T_NIP = 3 };
 
 struct Term {
     TermKind kind;
     int row, col;
     std::vector<double> mass;     // kinetic
-    std::vector<double> v;        // potential / dipole mu
+    std::vector<double> v;        // potential / dipole mu / NIP
     std::vector<double> laser;    // dipole field
     double scale = 1.0;
+    // bounds fixed by the operator instead of min / max of v (OGobbler:
+    // Emax = gain, Emin = -i gain with nip, else 0; OGobbler.cpp:183-192)
+    bool fixed = false;
+    std::complex<double> fmax = 0, fmin = 0;
 };
 
 struct Elem {             // applied matrix element after QDng's mirroring
@@ -516,6 +521,8 @@ struct qdx_ham {
         for (int i = 0; i < S; ++i)
             for (int j = 0; j < S; ++j)
                 if (i != j && given[i * S + j] && given[j * S + i]) hermitian = false;
+        if (has_nip() && multistate)
+            fail(QDX_ERR_INCOMPAT, "negative imaginary potential: single-state (Sum) Hamiltonians only");
         elem.assign(S * S, Elem());
         for (int k = 0; k < (int)terms.size(); ++k) {
             const Term &t = terms[k];
@@ -526,8 +533,14 @@ struct qdx_ham {
         upload();
     }
 
+    bool has_nip() const {
+        for (auto &t : terms)
+            if (t.kind == T_NIP) return true;
+        return false;
+    }
     // bounds of one matrix element: the sum of its terms (OSum.cpp:28-51)
     double term_emax(const Term &t) const {
+        if (t.fixed) return t.fmax.real();
         switch (t.kind) {
         case T_KIN: {  // OGridNablaSq.cpp:89-101
             double T = 0;
@@ -538,20 +551,28 @@ struct qdx_ham {
         case T_POT: return *std::max_element(t.v.begin(), t.v.end());
         case T_DIP:   // OGridDipole.cpp:36-39
             return *std::max_element(t.v.begin(), t.v.end()) * *std::max_element(t.laser.begin(), t.laser.end());
+        case T_NIP: return 0;
         }
         return 0;
     }
     double term_emin(const Term &t) const {
+        if (t.fixed) return t.fmin.real();
         if (t.kind == T_POT) return *std::min_element(t.v.begin(), t.v.end());
         return 0;
     }
-    void bounds(double &emin, double &emax) {
+    // Emin / Emax; emin_i / emax_i: imaginary parts (non-zero only for a
+    // Sum with a negative imaginary potential; OMultistate takes |E_ij|)
+    void bounds(double &emin, double &emax, double *emin_i = nullptr, double *emax_i = nullptr) {
         finalize();
+        if (emin_i) *emin_i = 0;
+        if (emax_i) *emax_i = 0;
         if (!multistate) {
             emin = emax = 0;
             for (auto &t : terms) {
                 emax += term_emax(t);
                 emin += term_emin(t);
+                if (t.fixed && emin_i) *emin_i += t.fmin.imag();
+                if (t.fixed && emax_i) *emax_i += t.fmax.imag();
             }
             return;
         }
@@ -577,7 +598,7 @@ struct qdx_ham {
 
     // upload the applied matrix: W_ij = sum of potentials, mu_ij one dipole,
     // kinetic tables per state and dimension (Kspace.cpp:22-50)
-    std::vector<std::vector<double>> hostW, hostMu;
+    std::vector<std::vector<double>> hostW, hostMu, hostWi;
     void upload() {
         if (!ctx) return;
         const long n = grid.size();
@@ -587,6 +608,7 @@ struct qdx_ham {
         for (int d = 0; d < 3; ++d) h.inv_nd[d] = d < grid.nd ? 1.0 / double(grid.N[d]) : 1.0;
         hostW.assign(S * S, {});
         hostMu.assign(S * S, {});
+        hostWi.assign(S, {});
         std::vector<double> tkh;
         std::vector<long> tk_off(S * 3, -1);
         for (int i = 0; i < S; ++i) {
@@ -602,6 +624,10 @@ struct qdx_ham {
                     if (t.kind == T_POT) {
                         if (W.empty()) W.assign(n, 0.0);
                         for (long x = 0; x < n; ++x) W[x] += t.v[x];
+                    } else if (t.kind == T_NIP) {
+                        std::vector<double> &Wi = hostWi[i];
+                        if (Wi.empty()) Wi.assign(n, 0.0);
+                        for (long x = 0; x < n; ++x) Wi[x] += t.v[x];
                     } else if (t.kind == T_DIP) {
                         if (dip && (dip->laser != t.laser || dip->scale != t.scale))
                             fail(QDX_ERR_INCOMPAT, "two dipole terms with different fields on one element");
@@ -636,6 +662,7 @@ struct qdx_ham {
             for (int d = 0; d < 3; ++d)
                 h.tk[i][d] = (d < grid.nd && tk_off[i * 3 + d] >= 0) ? tk.as<double>() + tk_off[i * 3 + d] : nullptr;
         for (int i = 0; i < S; ++i) {
+            h.Wi[i] = hostWi[i].empty() ? nullptr : upload64(hostWi[i]);
             if (h.tk[i][0] == nullptr)
                 for (int d = 0; d < 3; ++d) h.tk[i][d] = nullptr;
             for (int j = 0; j < S; ++j) {
@@ -672,6 +699,7 @@ struct qdx_ham {
         for (int k = 0; k < 3; ++k) f.inv_nd[k] = d.inv_nd[k];
         for (int i = 0; i < S; ++i) {
             f.unity[i] = d.unity[i];
+            f.Wi[i] = hostWi[i].empty() ? nullptr : upload32(hostWi[i]);
             for (int k = 0; k < 3; ++k) f.tk[i][k] = d.tk[i][k];
             for (int j = 0; j < S; ++j) {
                 f.W[i][j] = hostW[i * S + j].empty() ? nullptr : upload32(hostW[i * S + j]);
@@ -792,6 +820,7 @@ inline constexpr double kFp32CoeffFloor = 1e-10;
 struct ChebyParams {
     int order = 0, nz = 0;
     double offset = 0, scaling = 1, prec = 1e-10, emin = 0, emax = 0;
+    double offset_i = 0;   // Im offset (negative imaginary potential)
     std::vector<double> coeff;
 };
 
@@ -802,14 +831,22 @@ inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double p
                                 bool imag = false) {
     if (dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
     ChebyParams c;
-    h->bounds(c.emin, c.emax);
+    double emin_i = 0, emax_i = 0;
+    h->bounds(c.emin, c.emax, &emin_i, &emax_i);
     if (imag) {
         if (!(dt > 0)) fail(QDX_ERR_PARAM, "imaginary time needs dt > 0");
+        if (emin_i != 0 || emax_i != 0)
+            fail(QDX_ERR_INCOMPAT, "negative imaginary potential: real-time Chebyshev only");
         c.offset = c.emin - std::fabs(c.emin) * 0.05;
         c.scaling = std::fabs(c.emax - c.emin) * 1.05;
     } else {
+        // complex bounds (OCheby.cpp:406-416): offset (Re(Emax + Emin) / 2,
+        // Im(Emax + Emin)); with Im offset != 0 R = 1.05 |Emax - Emin| and no
+        // break test (OCheby.cpp:181)
         c.offset = (c.emax + c.emin) / 2;
-        c.scaling = std::fabs(c.emax - c.emin) / 2;
+        c.offset_i = emax_i + emin_i;
+        const double dr = c.emax - c.emin, di = emax_i - emin_i;
+        c.scaling = c.offset_i != 0 ? std::sqrt(dr * dr + di * di) * 1.05 : std::fabs(dr) / 2;
     }
     int order = order_param > 0 ? order_param : 0;
     if (order <= 0) order = std::abs(int(20 * c.scaling * dt));
@@ -833,6 +870,7 @@ inline ChebyParams cheby_params(qdx_ham *h, double dt, int order_param, double p
     c.coeff[0] = b[0];
     for (int i = 1; i < order; ++i) c.coeff[i] = 2.0 * b[i];
     if (prec >= 0) c.prec = prec;
+    if (c.offset_i != 0) c.prec = 0.0;   // the whole series
     return c;
 }
 

@@ -54,6 +54,9 @@ struct HamDev {
     const R *mu[kMaxStates][kMaxStates] = {};
     double Edip[kMaxStates][kMaxStates] = {};
     double unity[kMaxStates] = {};
+    // negative imaginary potential of each diagonal element: H gets -i Wi(x)
+    // (OGobbler with nip, OGobbler.cpp:195-204); null when absent
+    const R *Wi[kMaxStates] = {};
 };
 
 // Per-propagator device control block for the Chebyshev series.
@@ -110,6 +113,10 @@ struct ChebyOrder {
     int nterm = 1;
     double cm1r = 0, cm1i = 0;   // coeff[k-1] * exp(-i offset dt)
     double cm2r = 0, cm2i = 0;   // coeff[k-2] * exp(-i offset dt)
+    // Im offset (OCheby.cpp:407-408, non-zero with a negative imaginary
+    // potential): b = (H - offset_r - i offset_i) phi_{k-1} / R; the kernel
+    // instance with the imaginary diagonal (Wi, offset_i) is chosen on the host
+    double offset_i = 0;
     // (imaginary time, backward, exponent -dt, OCheby.cpp:200-297, is a
     // separate kernel instance: phi_1 = -b, phi_k = -2 b - phi_{k-2} with
     // b = (H - offset) phi_{k-1} / R; the coefficients carry exp(-dt offset))
@@ -163,9 +170,10 @@ struct Launch {
                          int nb_init, bool cmp = false);
     // a whole Chebyshev step of a single-state 1D line in one block (see
     // k_cheby_line); coef: (re, im) of coeff[k] exp(-i offset dt), k < order
+    // offset_i: Im offset (negative imaginary potential)
     static void cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
-                           const double *coef, int order, double offset, double inv_scaling, double prec,
-                           double *psi, ChebyCtl *ctl, double *partials);
+                           const double *coef, int order, double offset, double offset_i, double inv_scaling,
+                           double prec, double *psi, ChebyCtl *ctl, double *partials);
     // strided line pass along dimension `dim` (1 or 2) on X, in place.
     // mode 0: forward, 1: inverse, 2: forward * T/N inverse (kinetic),
     // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator),

@@ -185,6 +185,12 @@ __global__ void __launch_bounds__(512) k_apply_row(GridDev g, HamDev<R> h, LineP
             hp.r += w * pj.r;
             hp.i += w * pj.i;
         }
+        if (h.Wi[st]) {   // negative imaginary potential: -i Wi psi
+            const double c = (double)h.Wi[st][gi];
+            const cx<double> ps = src[base + gi];
+            hp.r += c * ps.i;
+            hp.i -= c * ps.r;
+        }
         dst[base + gi] = hp;
     }
 }

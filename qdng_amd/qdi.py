@@ -7,7 +7,8 @@ The loader follows QDng's operator-graph construction (src/ChainLoader.cpp:
 element named by the tag `m<i>.<j>`.  Supported classes: GridNablaSq
 (`mass`, comma separated per dimension), GridPotential (`file`, `setzero`,
 `offset`, `scale`; OGridPotential.cpp:36-76), GridDipole (`file`, `laser`,
-`scale`), Sum, Multistate (`unity`, `states`), DMultistate; as filters also
+`scale`), Gobbler (`lp<d>`, `rp<d>`, `order` (20), `gain` (1), `nip`, `dims`;
+OGobbler.cpp:85-164), Sum, Multistate (`unity`, `states`), DMultistate; as filters also
 GridMomentum and GridPosition (`dim`) and Projection (`files`, `start`, `num`,
 `step`; OProjection.cpp:42-68).  Propagators: Cheby (`prec`, `order`,
 `scaling`), GSPO (split operators as children), SIL (`order`, `err`), Arnoldi
@@ -146,6 +147,17 @@ class Loader:
             if "laser" not in a:
                 raise _err("GridDipole needs a laser file")
             o = Term("dip", name, file=self._file(a, name), laser=a["laser"], scale=float(a.get("scale", 1.0)))
+        elif name == "Gobbler":
+            order = int(a.get("order", 20))
+            if order < 1:
+                raise _err("Filter order to small")
+            if _bool(a.get("residue")):
+                raise _err("Gobbler: residue expectation values are not supported", ERR_INCOMPAT)
+            edges = {}
+            for side in ("lp", "rp"):
+                edges[side] = {int(k[2:]): float(v) for k, v in a.items() if re.fullmatch(side + r"\d+", k)}
+            o = Term("gob", name, lp=edges["lp"], rp=edges["rp"], order=order, gain=float(a.get("gain", 1.0)),
+                     nip=_bool(a.get("nip")), dims=int(a["dims"]) if "dims" in a else None)
         elif name in ("GridMomentum", "GridPosition"):
             o = Term("mom" if name == "GridMomentum" else "pos", name, dim=int(a.get("dim", 0)))
         elif name == "GridKick":
@@ -186,7 +198,7 @@ class Loader:
 def _terms(op):
     """Flatten a Sum (or a single term) into Hamiltonian terms."""
     if isinstance(op, Term):
-        if op.kind not in ("kin", "pot", "dip"):
+        if op.kind not in ("kin", "pot", "dip", "gob"):
             raise _err(f"{op.cls} is a filter, not a Hamiltonian term", ERR_INCOMPAT)
         return [op]
     if isinstance(op, Sum):
@@ -202,6 +214,13 @@ def _add_term(H, i, j, t):
         H.add_kinetic(i, p["mass"])
     elif t.kind == "pot":
         H.add_potential_file(i, j, p["file"], p["setzero"], p["offset"], p["scale"])
+    elif t.kind == "gob":
+        nd = len(H.N)
+        if p["dims"] is not None and p["dims"] != nd:
+            raise _err("OGobbler: Number of dims doesn't match WF", ERR_INCOMPAT)
+        # lp<d> / rp<d> beyond the wave function's dimensions are not read
+        H.add_gobbler(i, j, {d: x for d, x in p["lp"].items() if d < nd},
+                      {d: x for d, x in p["rp"].items() if d < nd}, p["order"], p["gain"], p["nip"])
     else:
         H.add_dipole_file(i, j, p["file"], p["laser"], p["scale"])
 

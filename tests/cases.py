@@ -65,6 +65,11 @@ class Problem:
         self.terms.append(("pot", row, col, np.ascontiguousarray(V, dtype=float).ravel()))
         return self
 
+    def gobbler(self, row, col, lp=None, rp=None, order=20, gain=1.0, nip=False):
+        """QDng's Gobbler: lp / rp map a dimension to its filter edge."""
+        self.terms.append(("gob", row, col, dict(lp or {}), dict(rp or {}), int(order), float(gain), bool(nip)))
+        return self
+
     def dipole(self, row, col, mu, laser, scale=1.0):
         self.terms.append(("dip", row, col, np.ascontiguousarray(mu, dtype=float).ravel(),
                            np.ascontiguousarray(laser, dtype=float).ravel(), float(scale)))
@@ -82,6 +87,8 @@ class Problem:
             return O.kinetic(g, t[3])
         if t[0] == "pot":
             return O.potential(g, t[3])
+        if t[0] == "gob":
+            return O.gobbler(g, t[3], t[4], t[5], t[6], t[7])
         return O.dipole(g, t[3], t[4], t[5])
 
     def oracle_ham(self):
@@ -108,6 +115,8 @@ class Problem:
             H.add_kinetic(t[1], t[3])
         elif t[0] == "pot":
             H.add_potential(t[1], t[2], t[3])
+        elif t[0] == "gob":
+            H.add_gobbler(t[1], t[2], t[3], t[4], t[5], t[6], t[7])
         else:
             H.add_dipole(t[1], t[2], t[3], t[4], t[5])
 
@@ -193,11 +202,19 @@ def kat_problem(name):
         p.dipole(1, 0, a["GridDip01"], a["GridLaser_w0.176"])
         p.psi0 = normalized(p, np.concatenate([ef0, np.zeros_like(ef0)]))
         return p, "cheby", 1.0, 400, 1e-12
+    if name == "TGP-NIP-Cheby":
+        # T + negative imaginary potential (high pass at x = 5.5, order 5,
+        # gain 0.02) + real Gobbler wall (low pass at x = 2, order 20, gain 0.3)
+        p = Problem(g["N"], g["xmin"], g["xmax"])
+        p.kinetic(0, mass).gobbler(0, 0, rp={0: 5.5}, order=5, gain=0.02, nip=True)
+        p.gobbler(0, 0, lp={0: 2.0}, order=20, gain=0.3, nip=False)
+        p.psi0 = normalized(p, ef0)
+        return p, "cheby", 10.0, 1000, 1e-10
     raise KeyError(name)
 
 
 KAT_NAMES = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby",
-             "TGPL-IR-Pulse-GSPO", "TGPLMS-Cheby"]
+             "TGPL-IR-Pulse-GSPO", "TGPLMS-Cheby", "TGP-NIP-Cheby"]
 
 
 # ---------------------------------------------------------------------------

@@ -396,6 +396,75 @@ int main() {
         ASSERT_TRUE(H.Expec(w.get()) < H.Expec(&psi));
     });
 
+    run("OGobbler: absorbing boundary (negative imaginary potential)", [&] {
+        const int N = 128;
+        const qdx_grid g = line_grid(N, -8, 8);
+        const double dx = 16.0 / (N - 1);
+        WFMultistate psi;
+        psi.Init(g, 1);
+        std::vector<double> V(N);
+        for (int i = 0; i < N; ++i) {
+            const double x = -8 + i * dx;
+            V[i] = 0.002 * x * x;
+            psi.at(0, i) = std::exp(-(x - 2) * (x - 2)) * dcomplex(std::cos(8 * x), std::sin(8 * x));
+        }
+        psi.Normalize();
+        OGridNablaSq T;
+        ParamContainer pm;
+        pm.SetValue("mass", 1836.0);
+        T.Init(pm);
+        OGridPotential Vp;
+        Vp.SetData(V);
+        OGobbler nip, wall;
+        ParamContainer pn, pw;
+        pn.SetValue("rp0", 5.0);
+        pn.SetValue("order", 6);
+        pn.SetValue("gain", 0.05);
+        pn.SetValue("nip", "true");
+        nip.Init(pn);
+        pw.SetValue("lp0", -6.0);
+        pw.SetValue("gain", 0.2);
+        pw.SetValue("residue", "true");
+        wall.Init(pw);
+        ASSERT_TRUE(!nip.Hermitian() && wall.Hermitian());
+        ASSERT_NEAR(-0.05, nip.Emin().imag(), 0);            // OGobbler.cpp:188-192
+        ASSERT_NEAR(0.2, wall.Emax().real(), 0);
+        OSum H;
+        H.Add(&T);
+        H.Add(&Vp);
+        H.Add(&nip);
+        H.Add(&wall);
+        H.Init(&psi);
+        ASSERT_NEAR(-0.05, H.Emin().imag(), 0);              // the sum's complex Emin
+        ASSERT_NEAR(0.0, H.Emax().imag(), 0);
+        // residue: Expec = <O psi|O psi> (OGobbler.cpp:170-176)
+        wall.Init(&psi);
+        std::unique_ptr<WaveFunction> ow(psi.NewInstance());
+        wall.Apply(ow.get(), &psi);
+        ASSERT_NEAR((*ow * *ow).real(), wall.Expec(&psi), 1e-15);
+        QDClock clock;
+        clock.Dt(5.0);
+        OCheby cheby;
+        cheby.Add("hamiltonian", &H);
+        cheby.Clock(&clock);
+        std::unique_ptr<WaveFunction> w(psi.NewInstance());
+        *w = psi;
+        double last = 1.0;
+        for (int k = 0; k < 40; ++k) {   // the packet runs into the absorber
+            cheby.Apply(w.get());
+            const double n2 = (*w * *w).real();
+            ASSERT_TRUE(std::sqrt(n2) <= last + 1e-12);
+            last = std::sqrt(n2);
+        }
+        ASSERT_TRUE(last < 0.9);
+        ASSERT_THROW(EIncompatible, cheby.Backward());      // H is not self-conjugate
+        ASSERT_THROW(EIncompatible, nip.Conj(true));
+        OSIL sil;                                            // Lanczos needs a Hermitian H
+        sil.Add("hamiltonian", &H);
+        sil.Clock(&clock);
+        ASSERT_THROW(EIncompatible, sil.Apply(w.get()));
+    });
+
     run("exceptions", [&] {
         ParamContainer p;
         p.SetValue("mass", 0.0);

@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 # regression tests that exercise only the grid Chebyshev / GSPO path
 IN_SCOPE = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby",
-            "TGPL-IR-Pulse-GSPO", "TGPLMS-Cheby"]
+            "TGPL-IR-Pulse-GSPO", "TGPLMS-Cheby", "TGP-NIP-Cheby"]
 
 
 def read_meta(path):

@@ -42,7 +42,14 @@ PROPAGATIONS = {
                          laser="GridLaser_w0.176", wf="ef0 on state 0 of 2", post=("2state", None)),
     "TGPLMS-SIL": dict(dt=1, steps=400, wcycle=1000, prop=("SIL", {"order": "20"}), ham="2state+dipole",
                        laser="GridLaser_w0.176", wf="ef0 on state 0 of 2", post=("2state", None)),
+    # absorbing boundary: T + a negative imaginary potential (Gobbler, high
+    # pass at x = 5.5) + a real Gobbler wall (low pass at x = 2)
+    "TGP-NIP-Cheby": dict(dt=10, steps=1000, wcycle=1000, prop=("Cheby", {}), ham="1state+nip", wf="ef0",
+                          post=("1state, no projections", None)),
 }
+# the Gobbler operators of 1state+nip: (tag, options)
+NIP_TERMS = [("Vpot", {"dims": "1", "rp0": "5.5", "order": "5", "gain": "0.02", "nip": "true"}),
+             ("VpotIm", {"dims": "1", "lp0": "2", "order": "20", "gain": "0.3", "nip": "false", "key": "V"})]
 
 # eigenstate programs: an imaginary-time relaxation of 10 states from the
 # Gaussian, then the autocorrelation method on the sum of the first six
@@ -96,7 +103,12 @@ def _propagator(parent, spec, laser):
     name, opts = spec["prop"]
     p = _el("propagator", parent, name=name, **opts)
     ham = spec["ham"]
-    if ham.startswith("1state"):
+    if ham == "1state+nip":
+        h = _el("hamiltonian", p, name="Sum")
+        _kinetic(h, "A", "T")
+        for tag, opts in NIP_TERMS:
+            _el(tag, h, name="Gobbler", **opts)
+    elif ham.startswith("1state"):
         # GSPO takes the split operators directly, the others one Hamiltonian
         target = p if name == "GSPO" else _el("hamiltonian", p, name="Sum")
         _one_state_terms(target, True, laser)
@@ -136,6 +148,12 @@ def _wavefunction(parent, kind):
 
 def _post_filters(parent, kind, projections):
     f = _el("filterpost", parent)
+    if kind == "1state, no projections":
+        for cls in ("GridMomentum", "GridPosition"):
+            _el("expeconly", f, name=cls)
+        _el("expeconly", f, ref="T")
+        _el("expeconly", f, ref="V")
+        return f
     if kind == "1state":
         _el("expeconly", f, name="GridMomentum")
         _el("expeconly", f, name="GridPosition")

@@ -53,7 +53,7 @@ def problems():
     out = {}
     for name in cases.KAT_NAMES:
         p, kind, dt, steps, prec = cases.kat_problem(name)
-        if kind == "cheby":
+        if kind == "cheby" and name != "TGP-NIP-Cheby":   # complex bounds: its own test below
             out[name] = (p, dt, prec)
     out["harmonic_1d"] = (cases.harmonic_1d(256), 10.0, 1e-12)
     out["morse_2state_2d"] = (cases.morse_2state_2d(64), 5.0, 1e-10)
@@ -146,3 +146,46 @@ def test_argument_errors():
     H.add_potential(0, 0, np.zeros(16))
     with pytest.raises(Q.QdxError):
         H.cheby_params(0.0)                        # No time step defined
+
+
+def test_gobbler_chebyshev_parameters_match_oracle():
+    # a negative imaginary potential makes Emin complex (OGobbler.cpp:183-192):
+    # offset (Re(Emax + Emin) / 2, Im(Emax + Emin)), R = 1.05 |Emax - Emin|
+    # and the whole series (no break test, OCheby.cpp:181, 406-416)
+    O.reset()
+    p, kind, dt, steps, prec = cases.kat_problem("TGP-NIP-Cheby")
+    H = host_ham(p)
+    Ho = p.oracle_ham()
+    omin, omax = Ho.bounds_c()
+    assert omin.imag == -0.02 and omax.imag == 0
+    emin, emax = H.bounds()
+    assert emin == omin.real and emax == pytest.approx(omax.real, rel=1e-15)
+    info, coeff = H.cheby_params(dt, prec=prec)
+    U = O.Cheby(Ho, dt, prec=prec)
+    assert info.order == U.order
+    assert info.offset == pytest.approx(U.offset, rel=1e-15)
+    assert info.offset_i == -0.02
+    assert info.scaling == pytest.approx(U.scaling, rel=1e-15)
+    assert info.scaling == pytest.approx(1.05 * abs(omax - omin), rel=1e-15)
+    assert info.prec == 0
+    assert np.allclose(coeff, U.coeffs(), rtol=1e-10, atol=1e-13 * np.abs(U.coeffs()).max())
+
+
+def test_gobbler_argument_errors():
+    p = cases.Problem([64], [0.0], [4.0])
+    H = Q.Hamiltonian(None, p.N, p.xmin, p.xmax)
+    with pytest.raises(Q.QdxError) as e:
+        H.add_gobbler(0, 0, rp={0: 9.0})            # edge beyond the grid (OGobbler.cpp:63-64)
+    assert e.value.code == -1
+    with pytest.raises(Q.QdxError):
+        H.add_gobbler(0, 0, lp={0: 2.0}, order=0)  # "Filter order to small"
+    M = Q.Hamiltonian(None, p.N, p.xmin, p.xmax, 2, True)
+    with pytest.raises(Q.QdxError) as e:
+        M.add_gobbler(0, 0, rp={0: 3.0}, nip=True)  # NIP: single-state Sum only
+    assert e.value.code == -2
+    # a real Gobbler is a potential with the bounds gain and 0 anywhere
+    M.add_gobbler(1, 1, rp={0: 3.0}, gain=0.5).add_kinetic(0, [1.0]).add_kinetic(1, [1.0])
+    O.reset()
+    po = cases.Problem(p.N, p.xmin, p.xmax, nstates=2)
+    po.gobbler(1, 1, rp={0: 3.0}, gain=0.5).kinetic(0, [1.0]).kinetic(1, [1.0])
+    assert M.bounds() == pytest.approx(po.oracle_ham().bounds(), rel=1e-14)

@@ -474,6 +474,11 @@ def device_post_expectations(ctx, name, p, w):
     N, lo, hi = p.N, p.xmin, p.xmax
     mom, pos = w.expec_momentum(0), w.expec_position(0)
     Tk = Q.Hamiltonian(ctx, N, lo, hi).add_kinetic(0, [1763.2])
+    if name == "TGP-NIP-Cheby":
+        # momentum, position, T and the real Gobbler, each over <psi|psi>
+        n2 = w.norm()[0] ** 2
+        V = Q.Hamiltonian(ctx, N, lo, hi).add_gobbler(0, 0, lp={0: 2.0}, order=20, gain=0.3)
+        return [v / n2 for v in (mom[0], pos[0], Tk.expec(w), V.expec(w))]
     if p.S == 1:
         vals = [mom[0], pos[0]]
         efs = [Q.WaveFunction(ctx, N, lo, hi).upload(a["GridPot0_ef_%d" % i]) for i in range(4)]
@@ -577,4 +582,92 @@ def test_imag_time_propagator_keeps_its_direction(ctx):
     U.backward(True)
     with pytest.raises(Q.QdxError) as e:
         U.backward(False)
+    assert e.value.code == -2
+
+
+# ---------------------------------------------------------------------------
+# negative imaginary potential (OGobbler with nip): -i v(x) on the diagonal,
+# complex Chebyshev offset, the whole series (no break test)
+
+def _nip_2d(N=96):
+    """2D: T + harmonic V + NIP high passes at x = 5 and y = 5 (v = f0(x) +
+    f1(y), OGobbler.cpp:61-75) + a real Gobbler wall, a Gaussian moving into
+    the absorber."""
+    p = cases.Problem([N, N], [-8.0, -8.0], [8.0, 8.0])
+    x, y = p.axes()
+    p.kinetic(0, [1836.0, 2500.0]).potential(0, 0, 0.01 * (x ** 2 + y ** 2))
+    p.gobbler(0, 0, rp={0: 5.0, 1: 5.0}, order=6, gain=0.05, nip=True)
+    p.gobbler(0, 0, lp={1: -6.0}, order=10, gain=0.2)
+    p.psi0 = cases.normalized(p, cases.gaussian(p, [2.5, 1.5], 0.6, k0=[9.0, 6.0]))
+    return p
+
+
+@pytest.mark.parametrize("name,precision,dt,steps,bar", [
+    ("kat_1d", "fp64", 10.0, 40, 1e-10),
+    ("kat_1d", "fp32", 10.0, 40, 1e-5),
+    ("nip_2d", "fp64", 5.0, 6, 1e-10),
+    ("nip_2d", "compact", 5.0, 6, 1e-6),
+    ("nip_2d", "fp32", 5.0, 6, 1e-5),
+])
+def test_nip_cheby_vs_oracle(ctx, name, precision, dt, steps, bar):
+    import qdng_amd as Q
+    note(f"nip cheby {name} {precision}")
+    p = cases.kat_problem("TGP-NIP-Cheby")[0] if name == "kat_1d" else _nip_2d()
+    U = Q.Cheby(p.gpu_ham(ctx), dt, prec=1e-10, precision=precision)
+    Uo = O.Cheby(p.oracle_ham(), dt, prec=1e-10)
+    assert U.info.offset_i == -(0.02 if name == "kat_1d" else 0.05) and U.info.prec == 0
+    w = p.gpu_wf(ctx)
+    psi = p.psi0.copy()
+    for i in range(steps):
+        st = U.apply(w, i)
+        n = Uo.step(psi, i)
+        if precision == "fp64":
+            assert st.n_hpsi == n   # the whole series in both
+    e = rel(w.download(), psi)
+    n2 = O.norm2(p.oracle_grid(), psi)
+    note(f"  rel L2 {e:.2e}, norm^2 {n2:.4f}")
+    assert n2 < 0.999       # the absorber took part of the packet
+    assert e < bar
+
+
+def test_nip_apply_expec_and_arnoldi(ctx):
+    import qdng_amd as Q
+    note("nip apply / expec / arnoldi")
+    for p in (cases.kat_problem("TGP-NIP-Cheby")[0], _nip_2d(64)):
+        O.reset()
+        H, Ho = p.gpu_ham(ctx), p.oracle_ham()
+        w, d = p.gpu_wf(ctx), p.gpu_wf(ctx)
+        H.apply(d, w, 0)
+        ref = Ho.apply(p.psi0)
+        assert rel(d.download(), ref) < 1e-12
+        # Re <psi|H|psi> (the -i v part is imaginary)
+        assert H.expec(w, 0) == pytest.approx(Ho.expec(p.psi0), rel=1e-12)
+        # Arnoldi handles the non-Hermitian H (real time)
+        U = Q.Arnoldi(H, 5.0, order=20, conv=1e-12)
+        Uo = O.Arnoldi(Ho, 5.0, order=20, conv=1e-12)
+        psi = p.psi0.copy()
+        for i in range(4):
+            U.apply(w, i)
+            Uo.step(psi, i)
+        assert rel(w.download(), psi) < 1e-10
+
+
+def test_nip_unsupported_combinations(ctx):
+    import qdng_amd as Q
+    note("nip rejections")
+    p = cases.kat_problem("TGP-NIP-Cheby")[0]
+    H = p.gpu_ham(ctx)
+    for make in (lambda: Q.SIL(H, 10.0), lambda: Q.Cheby(H, 10.0, imag=True),
+                 lambda: Q.Arnoldi(H, 10.0, imag=True),
+                 lambda: Q.GSPO([p.gpu_split_op(ctx, ["kin"]), p.gpu_split_op(ctx, ["gob"])], 10.0)):
+        with pytest.raises(Q.QdxError) as e:
+            make()
+        assert e.value.code == -2
+    U = Q.Cheby(H, 10.0)
+    with pytest.raises(Q.QdxError) as e:
+        U.backward(True)
+    assert e.value.code == -2
+    B = Q.Batch(ctx, "fp64", 1 << 28)
+    with pytest.raises(Q.QdxError) as e:
+        B.submit(p.gpu_ham(None), 10.0, 2, p.psi0)
     assert e.value.code == -2

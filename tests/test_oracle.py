@@ -227,6 +227,13 @@ def _post_expectations(name, p, g, psi):
     """expec_post.dat values at the final time (FiltersList post filters)."""
     a, _ = cases.kat_data()
     n = p.size
+    if name == "TGP-NIP-Cheby":
+        # GridMomentum, GridPosition, T, V (the real Gobbler), each divided by
+        # <psi|psi> (FiltersList.cpp:135-330)
+        n2 = O.norm2(g, psi)
+        T = O.kinetic(g, [1763.2])
+        V = O.gobbler(g, lp={0: 2.0}, order=20, gain=0.3)
+        return [v / n2 for v in (O.expec_momentum(g, psi), O.expec_position(g, psi), T.expec(psi), V.expec(psi))]
     if p.S == 1:
         vals = [O.expec_momentum(g, psi), O.expec_position(g, psi)]
         if name == "TGP-Expec":
@@ -418,3 +425,73 @@ def test_backward_real_time_is_exp_plus_i_h_dt(kind):
     again = psi0.copy()
     U.step(again)
     assert np.array_equal(again, fwd)
+
+
+# ---------------------------------------------------------------------------
+# Gobbler / negative imaginary potential (OGobbler.cpp): pinned by the
+# TGP-NIP-Cheby regression case above; the checks below pin the pieces.
+
+def _nip_grid():
+    a, meta = cases.kat_data()
+    g = meta["grids"]["GridPot0"]
+    return O.Grid(g["N"], g["xmin"], g["xmax"]), a
+
+
+def test_gobbler_bounds_and_nip_apply():
+    g, a = _nip_grid()
+    psi = cases.normalized(cases.Problem(g.N, g.xmin, g.xmax), a["GridPot0_ef_0"])
+    nip = O.gobbler(g, rp={0: 5.5}, order=5, gain=0.02, nip=True)
+    real = O.gobbler(g, rp={0: 5.5}, order=5, gain=0.02, nip=False)
+    # OGobbler::Emax / Emin: gain and -i gain (nip), gain and 0 (otherwise)
+    assert nip.bounds_c() == (-0.02j, 0.02 + 0j)
+    assert real.bounds_c() == (0j, 0.02 + 0j)
+    # one high-pass edge: 0 + f (nip) and 1 * f (real) are the same filter,
+    # so the NIP applies -i times the real Gobbler
+    assert np.allclose(nip.apply(psi), -1j * real.apply(psi), rtol=0, atol=1e-15)
+    # filter shape: rising from 0 at the first point through gain / 2 at the
+    # edge (x in grid points from xmin) towards gain
+    x = g.axis(0)
+    v = real.apply(np.ones(g.size, complex)).real
+    assert v[0] == 0 and np.all(np.diff(v) > 0) and v[-1] < 0.02
+    assert v[x < 4.0].max() < 0.01 * 0.02 and v[-1] > 0.6 * 0.02
+    xe = (5.5 - g.xmin[0]) / g.dx(0)
+    lo = int(np.floor(xe))
+    assert v[lo] < 0.01 < v[lo + 1]
+    # an edge outside the grid is refused as in OGobbler::Init
+    with pytest.raises(O.OracleError):
+        O.gobbler(g, rp={0: 100.0})
+
+
+def test_nip_planted_mistakes():
+    p, kind, dt, steps, prec = cases.kat_problem("TGP-NIP-Cheby")
+    gold = cases.kat_data()[1]["golden"]["TGP-NIP-Cheby"]["report"]
+
+    def run(plant):
+        O.plant_mistake(plant)
+        try:
+            O.reset()
+            if hasattr(p, "_og"):
+                del p._og
+            U = O.Cheby(p.oracle_ham(), dt, prec=prec)
+            psi = p.psi0.copy()
+            for i in range(steps - 1):
+                U.step(psi, i)
+            return O.norm2(p.oracle_grid(), psi), O.overlap(p.oracle_grid(), psi, p.psi0)
+        finally:
+            O.plant_mistake(0)
+
+    n, ac = run(0)
+    assert abs(n - gold["population"]) < 5e-7 and abs(ac - complex(*gold["autocorr"])) < 1e-8
+    # +i v (growth instead of absorption), the coefficient phase of the real
+    # offset only, and a multiplied NIP high pass (v = 0) each miss the
+    # golden line
+    for plant in (2, 3, 4):
+        n, ac = run(plant)
+        assert abs(n - gold["population"]) > 1e-3 or abs(ac - complex(*gold["autocorr"])) > 1e-4, plant
+
+
+def test_nip_refuses_backward():
+    p, kind, dt, steps, prec = cases.kat_problem("TGP-NIP-Cheby")
+    U = O.Cheby(p.oracle_ham(), dt, prec=prec)
+    with pytest.raises(O.OracleError):
+        U.backward()

@@ -27,7 +27,7 @@ import qdi_programs  # noqa: E402
 
 _PROGRAMS = qdi_programs.write_all(tempfile.mkdtemp(prefix="qdi_programs_"))
 PROPA = ["TGP-Expec", "TGPMS-Cheby", "TGPMS-GSPO", "TGPL-IR-Pulse-Cheby", "TGPL-IR-Pulse-GSPO",
-         "TGPLMS-Cheby", "TGPLMS-SIL", "TGP-Kick"]
+         "TGPLMS-Cheby", "TGPLMS-SIL", "TGP-Kick", "TGP-NIP-Cheby"]
 # checktests.sh: CheckT TGP-Kick and CheckT-Expec TGP-Kick (grep prefixes)
 KICK = {"report": {"step": 999, "time": 19980.0, "population": 1.0, "pops": [], "energy": 0.00399372,
                    "autocorr": [0.29628786, -0.88631099]},
@@ -63,6 +63,26 @@ def test_every_input_parses():
             for sec in (c for c in el if c.tag == "filterpost"):
                 for f in sec:
                     ld.op(f)
+
+
+def test_gobbler_terms():
+    ld = qdi.Loader(FIX)
+    el = prog("TGP-NIP-Cheby")
+    name, a, (H,) = ld.propagator([c for c in el if c.tag == "propagator"][0])
+    assert name == "Cheby" and "prec" not in a
+    kin, nip, wall = qdi._terms(H)
+    assert kin.kind == "kin" and nip.kind == "gob" and wall.kind == "gob"
+    assert nip.params == dict(lp={}, rp={0: 5.5}, order=5, gain=0.02, nip=True, dims=1)
+    assert wall.params == dict(lp={0: 2.0}, rp={}, order=20, gain=0.3, nip=False, dims=1)
+    assert ld.keys["V"] is wall and ld.keys["T"] is kin
+    # defaults (OGobbler::Init): order 20, gain 1, nip off
+    g = ld.op(ET_fromstring('<G name="Gobbler" lp1="0.5"/>'))
+    assert g.params == dict(lp={1: 0.5}, rp={}, order=20, gain=1.0, nip=False, dims=None)
+
+
+def ET_fromstring(x):
+    import xml.etree.ElementTree as ET
+    return ET.fromstring(x)
 
 
 def test_multistate_graph_with_refs():
@@ -104,7 +124,8 @@ def test_gspo_split_operators_and_their_sum():
 def test_loader_errors(tmp_path):
     import xml.etree.ElementTree as ET
     ld = qdi.Loader()
-    for xml, code in [('<A ref="nope"/>', -1), ('<A name="Gobbler"/>', -2), ('<A name="GridKick"/>', -1), ('<A/>', -1),
+    for xml, code in [('<A ref="nope"/>', -1), ('<A name="Flux"/>', -2), ('<A name="GridKick"/>', -1), ('<A/>', -1),
+                      ('<A name="Gobbler" order="0"/>', -1), ('<A name="Gobbler" residue="true"/>', -2),
                       ('<A name="GridPotential"/>', -1), ('<A name="GridDipole" file="x"/>', -1),
                       ('<A name="Multistate"><x0 name="GridNablaSq" mass="1"/></A>', -1),
                       ('<A name="Sum"/>', -1), ('<A name="GridPotential" file="f" setzero="maybe"/>', -1)]:
