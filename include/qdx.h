@@ -343,6 +343,13 @@ int qdx_batch_run(qdx_batch *b, int32_t nticks);
 int qdx_batch_job_state(qdx_batch *b, int64_t id, int64_t *steps_left, int64_t *n_hpsi, int64_t *steps_done);
 /* wait for the enqueued ticks and copy the job's psi (complex128, [S][N]) */
 int qdx_batch_fetch(qdx_batch *b, int64_t id, double *psi);
+/* Per-job failure isolation (Reporter.cpp:112-113, "Norm is not a number"):
+ * *failed_at = -1 for a sound job, else the step whose state had a norm that
+ * is not a number (Chebyshev: the state the step started from; GSPO: the
+ * state it produced).  A failed job stops there: its kernels are skipped,
+ * steps_done and n_hpsi stay at the last sound step, and its batch-mates
+ * are unaffected. */
+int qdx_batch_job_status(qdx_batch *b, int64_t id, int64_t *failed_at);
 /* wait for the enqueued ticks and return the job's memory to the arena */
 int qdx_batch_release(qdx_batch *b, int64_t id);
 /* jobs held, arena bytes in use, ticks enqueued so far */
@@ -460,6 +467,7 @@ typedef struct qdx_serve_result {
     double admit_s;        /* admission, seconds after the start (host clock)  */
     double finish_s;       /* end of the job's last step (device clock)        */
     int64_t n_hpsi;        /* H applications (GSPO: steps)                      */
+    int64_t failed_step;   /* -1, or as qdx_batch_job_status                     */
 } qdx_serve_result;
 int qdx_batch_serve(qdx_batch *b, const qdx_job_desc *jobs, const double *arrival_s, int32_t n,
                     int32_t max_active, double *const *psi_out, qdx_serve_result *out);

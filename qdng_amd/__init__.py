@@ -57,7 +57,7 @@ EXPORTS = [
     "qdx_cheby_info_get", "qdx_cheby_params", "qdx_cheby_coeffs", "qdx_gspo_create",
     "qdx_prop_apply", "qdx_prop_run", "qdx_prop_set_direction", "qdx_prop_exponent", "qdx_prop_destroy", "qdx_ctx_profile", "qdx_ctx_kernel_time",
     "qdx_batch_create", "qdx_batch_footprint", "qdx_batch_submit", "qdx_batch_run",
-    "qdx_batch_job_state", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
+    "qdx_batch_job_state", "qdx_batch_job_status", "qdx_batch_fetch", "qdx_batch_release", "qdx_batch_usage",
     "qdx_batch_serve", "qdx_batch_set_graphs", "qdx_batch_graph_stats", "qdx_batch_destroy",
     "qdx_ctx_set_radix_plan", "qdx_ctx_get_radix_plan", "qdx_batch_get_tiling", "qdx_batch_set_tiling",
     "qdx_batch_autotune", "qdx_batch_set_autotune", "qdx_batch_tuning_save", "qdx_batch_tuning_load",
@@ -82,7 +82,8 @@ class JobDesc(C.Structure):
 
 
 class ServeResult(C.Structure):
-    _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double), ("n_hpsi", C.c_int64)]
+    _fields_ = [("admit_s", C.c_double), ("finish_s", C.c_double), ("n_hpsi", C.c_int64),
+                ("failed_step", C.c_int64)]
 
 
 class Tiling(C.Structure):
@@ -160,6 +161,7 @@ def lib():
         "qdx_batch_job_state": [vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                 C.POINTER(C.c_int64)],
         "qdx_batch_fetch": [vp, C.c_int64, dp],
+        "qdx_batch_job_status": [vp, C.c_int64, C.POINTER(C.c_int64)],
         "qdx_batch_release": [vp, C.c_int64],
         "qdx_batch_usage": [vp, C.POINTER(i32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
         "qdx_batch_serve": [vp, C.POINTER(JobDesc), dp, i32, i32, C.POINTER(dp), C.POINTER(ServeResult)],
@@ -868,6 +870,8 @@ class Batch:
         admit = np.array([res[i].admit_s for i in range(n)])
         finish = np.array([res[i].finish_s for i in range(n)])
         nh = np.array([res[i].n_hpsi for i in range(n)])
+        # -1 per sound request, else the step at which it stopped (failed_step)
+        self.last_failed = np.array([res[i].failed_step for i in range(n)], dtype=np.int64)
         return (admit, finish, nh, psis) if want_psi else (admit, finish, nh)
 
     def run(self, nticks: int = 1) -> None:
@@ -881,6 +885,13 @@ class Batch:
         else:
             _chk(lib().qdx_batch_job_state(self.h, int(jid), C.byref(a), None, None))
         return a.value, b.value, c.value
+
+    def failed_step(self, jid: int) -> int:
+        """-1 for a sound job, else the step whose state's norm was not a
+        number; the job stopped there (qdx_batch_job_status)"""
+        v = C.c_int64()
+        _chk(lib().qdx_batch_job_status(self.h, int(jid), C.byref(v)))
+        return v.value
 
     def fetch(self, jid: int) -> np.ndarray:
         out = np.zeros(self.sizes[jid], dtype=np.complex128)

@@ -807,6 +807,13 @@ void read_stats(qdx_batch *b, const BatchJob &J, long long st[2]) {
     HIPCHK(hipMemcpy(st, b->arena.base() + J.off + J.lay.stats, 2 * sizeof(long long), hipMemcpyDeviceToHost));
 }
 
+// -1, or the step whose state's norm was not a number (the job stopped there)
+int64_t failed_step(qdx_batch *b, const BatchJob &J) {
+    ChebyCtl c;
+    HIPCHK(hipMemcpy(&c, b->arena.base() + J.off + J.lay.ctl, sizeof(c), hipMemcpyDeviceToHost));
+    return c.failed > 0 ? (int64_t)c.failed - 1 : -1;
+}
+
 // return a job's memory; the caller has made sure no enqueued tick still uses it
 void drop_job(qdx_batch *b, int64_t id) {
     BatchJob &J = find_job(b, id);
@@ -1182,6 +1189,15 @@ int qdx_batch_job_state(qdx_batch *b, int64_t id, int64_t *steps_left, int64_t *
     });
 }
 
+int qdx_batch_job_status(qdx_batch *b, int64_t id, int64_t *failed_at) {
+    return guarded([&] {
+        if (!b || !failed_at) fail(QDX_ERR_PARAM, "NULL argument");
+        BatchJob &J = find_job(b, id);
+        HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        *failed_at = failed_step(b, J);
+    });
+}
+
 int qdx_batch_fetch(qdx_batch *b, int64_t id, double *psi) {
     return guarded([&] {
         if (!b || !psi) fail(QDX_ERR_PARAM, "NULL argument");
@@ -1288,6 +1304,7 @@ int qdx_batch_serve(qdx_batch *b, const qdx_job_desc *jobs, const double *arriva
                     read_stats(b, J, st);
                     out[i].finish_s = ms * 1e-3;
                     out[i].n_hpsi = st[0];
+                    out[i].failed_step = failed_step(b, J);
                     if (psi_out && psi_out[i]) copy_psi(b, J, psi_out[i]);
                     drop_job(b, id);
                     active.erase(id);

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(NT) k_brow(const BJob<R> *__restrict__ jobs, c
     const BTile t = tiles[b];
     const BJob<R> J = jobs[t.job];
     const int step = *tick_p - J.start_tick;
-    if (step < 0 || step >= J.nsteps) return;
+    if (step < 0 || step >= J.nsteps || J.ctl->failed) return;
     if (PHASE == 2 && (k >= J.order || J.ctl->done)) return;
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ double red[16];
@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(256) k_bcol(const BJob<R> *__restrict__ jobs, 
     const BTile t = tiles[b];
     const BJob<R> J = jobs[t.job];
     const int step = *tick_p - J.start_tick;
-    if (step < 0 || step >= J.nsteps) return;
+    if (step < 0 || step >= J.nsteps || J.ctl->failed) return;
     if (k >= 2 && (k >= J.order || J.ctl->done)) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
@@ -292,7 +292,14 @@ __global__ void __launch_bounds__(256) k_btick_end(const BJob<R> *__restrict__ j
     for (int j = threadIdx.x; j < njobs; j += blockDim.x) {
         const BJob<R> &J = jobs[j];
         const int step = tk - J.start_tick;
-        if (step >= 0 && step < J.nsteps) {
+        if (step >= 0 && step < J.nsteps && !J.ctl->failed) {
+            // a step that started from (Chebyshev) or produced (GSPO) a state
+            // whose norm is not a number ends the job there; its batch-mates
+            // are untouched
+            if (!isfinite(J.ctl->norm0)) {
+                J.ctl->failed = step + 1;
+                continue;
+            }
             J.stats[0] += J.kind == 1 ? 1 : J.ctl->last_order;
             J.stats[1] += 1;
         }
@@ -311,7 +318,7 @@ __global__ void __launch_bounds__(NT) k_bline(const BJob<R> *__restrict__ jobs, 
                                               const LinePlan *__restrict__ plans, const int *tick_p, int twl) {
     const BJob<R> J = jobs[list[blockIdx.x]];
     const int step = *tick_p - J.start_tick;
-    if (step < 0 || step >= J.nsteps) return;
+    if (step < 0 || step >= J.nsteps || J.ctl->failed) return;
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ double red[16];
     __shared__ double bc;
@@ -483,7 +490,7 @@ __global__ void __launch_bounds__(NT) k_bgspo_row(const BJob<R> *__restrict__ jo
     const BTile t = tiles[b];
     const BJob<R> J = jobs[t.job];
     const int step = *tick_p - J.start_tick;
-    if (step < 0 || step >= J.nsteps) return;
+    if (step < 0 || step >= J.nsteps || J.ctl->failed) return;
     if (PHASE == 1 && J.nd == 1) return;
     extern __shared__ __align__(16) char smem_raw[];
     BRowTile<R, 2> B(J, t.first, reinterpret_cast<cx<R> *>(smem_raw));
@@ -541,12 +548,18 @@ __global__ void __launch_bounds__(NT) k_bgspo_row(const BJob<R> *__restrict__ jo
     bfft(v0, B, p0, twl, +1);
     if (two) bfft(v1, B, p0, twl, +1);
     if (B.active) {
+        bool bad = false;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             wsplit_point(J, B.gidx(q), Ed, th, v0[q], v1[q]);
             psi[B.gidx(q)] = v0[q];
             if (two) psi[J.size + B.gidx(q)] = v1[q];
+            bad |= !isfinite((double)v0[q].r + (double)v0[q].i);
+            if (two) bad |= !isfinite((double)v1[q].r + (double)v1[q].i);
         }
+        // a step that leaves a value that is not a number marks the job
+        // (read at the tick's end); every writer stores the same NaN
+        if (bad) J.ctl->norm0 = __builtin_nan("");
     }
 }
 
@@ -559,7 +572,7 @@ __global__ void __launch_bounds__(256) k_bgspo_col(const BJob<R> *__restrict__ j
     const BTile t = tiles[b];
     const BJob<R> J = jobs[t.job];
     const int step = *tick_p - J.start_tick;
-    if (step < 0 || step >= J.nsteps) return;
+    if (step < 0 || step >= J.nsteps || J.ctl->failed) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
     const LinePlan &p = plans[J.plan1];

@@ -66,7 +66,10 @@ struct ChebyCtl {
     int done;            // series converged (OCheby.cpp:179-182)
     int last_order;      // index i of the last executed recurrence iteration
     unsigned pad0;
-    int pad1;
+    // batched jobs: 1 + the step whose start state had a norm that is not a
+    // number (Reporter.cpp:112-113); 0 while the job is sound.  A failed job
+    // is skipped by every kernel from then on.
+    int failed;
 };
 
 // Break test of the previous order, evaluated at the start of a kernel from

@@ -316,3 +316,80 @@ def test_serve_admission_queues_on_budget(ctx):
         T = Q.Batch(ctx, "fp32", min(sizes))
         T.serve([req_kwargs(*reqs[4])], [0.0])
     assert e.value.code == -4
+
+
+# ---------------------------------------------------------------------------
+# per-job failure isolation: a job whose state stops being a number stops
+# (Reporter.cpp:112-113 "Norm is not a number" ends a QDng propagation); its
+# batch-mates' results do not change by a bit
+
+def _poisoned_laser(p, at):
+    """p with its dipole field not a number at sample `at`"""
+    q = cases.Problem(p.N, p.xmin, p.xmax, p.S, p.multistate, p.unity)
+    q.psi0 = p.psi0.copy()
+    for t in p.terms:
+        if t[0] == "dip":
+            las = t[4].copy()
+            las[at] = np.nan
+            t = t[:4] + (las,) + t[5:]
+        q.terms.append(t)
+    return q
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "compact"])
+def test_failed_job_is_isolated(ctx, precision):
+    import qdng_amd as Q
+    steps = 8
+    jobs = mixed_jobs(steps, seed=21)      # 1D 128, 384, 3072, 4096; 2D 64^2, 96x64, 64x192, 128^2
+    gs = gspo_jobs(steps)[:2]
+    note(f"failure isolation, {precision}")
+    # (job index, how it is poisoned, the step it stops at)
+    bad_psi = {1: 0, 6: 0}                 # a NaN / an inf in psi0: the first step's start state
+    bad_laser = {2: 5, 5: 3}               # NaN field at step s: the state after step s
+    bad_gspo = {0: 0}                      # GSPO: the state its first step produced
+
+    def submit(B, poison):
+        ids = []
+        for i, p in enumerate(jobs):
+            psi0 = p.psi0.copy()
+            if poison and i in bad_psi:
+                psi0[len(psi0) // 3] = np.nan if i == 1 else np.inf
+            q = _poisoned_laser(p, bad_laser[i]) if poison and i in bad_laser else p
+            ids.append(B.submit(q.gpu_ham(None), 1.0, steps, psi0))
+        for i, p in enumerate(gs):
+            psi0 = p.psi0.copy()
+            if poison and i in bad_gspo:
+                psi0[7] = np.nan
+            ids.append(B.submit(dt=1.0, nsteps=steps, psi0=psi0, ops=split_ops(p, None)))
+        return ids
+
+    A = Q.Batch(ctx, precision, 1 << 30)
+    ia = submit(A, False)
+    A.run(steps)
+    M = Q.Batch(ctx, precision, 1 << 30)
+    im = submit(M, True)
+    M.run(steps)
+    nj = len(jobs)
+    for k, (a, m) in enumerate(zip(ia, im)):
+        stop = (bad_psi.get(k, bad_laser.get(k)) if k < nj else bad_gspo.get(k - nj))
+        left, nh, done = M.state(m)
+        if stop is None:
+            assert A.failed_step(a) == -1 and M.failed_step(m) == -1
+            assert np.array_equal(M.fetch(m), A.fetch(a)), k
+            assert (nh, done) == A.state(a)[1:]
+            continue
+        # a laser sample that is not a number spoils step s; the next
+        # step starts from a state that is not a number and stops the job
+        want = stop + 1 if k in bad_laser else stop
+        note(f"  job {k}: failed at step {M.failed_step(m)}, {done} steps done")
+        assert M.failed_step(m) == want and done == want
+        assert (nh > 0) == (want > 0)
+    # the serving loop reports the step as well
+    S = Q.Batch(ctx, precision, 1 << 30)
+    p = jobs[4]
+    psi0 = p.psi0.copy()
+    psi0[0] = np.nan
+    reqs = [dict(H=p.gpu_ham(None), dt=1.0, nsteps=4, psi0=psi0), dict(H=p.gpu_ham(None), dt=1.0, nsteps=4,
+                                                                        psi0=p.psi0)]
+    S.serve(reqs, np.array([0.0, 0.0]))
+    assert list(S.last_failed) == [0, -1]
