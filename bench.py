@@ -424,7 +424,7 @@ def cpu_record(rate, rate1, backend, cores, sample, unit=UNIT):
             "value_1_thread": rate1, "sample": sample}
 
 
-def cpu_cfg1(p):
+def cpu_cheby_2d(p):
     backend = cpu_fft_backend()
     th = host_threads()
     rate, secs, nh = cpu_cheby(p, 5.0, 2, 5, th)
@@ -712,14 +712,14 @@ def main():
                     "steps": args.cpu_steps}
         ex["parity"] = {prec: parity(prec) for prec in (PRECISIONS if not args.no_extra else ("fp64",))}
         ex["l2_fp64_vs_oracle"] = ex["parity"]["fp64"]["l2_vs_oracle"]
-        out["cpu_baseline"] = cpu_cfg1(p)
+        out["cpu_baseline"] = cpu_cheby_2d(p)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.4g} grid-pt*Hpsi/s on {out['cpu_baseline']['cores']} "
             f"threads ({out['cpu_baseline']['fft']}), {out['cpu_baseline']['value_1_thread']:.4g} on 1")
 
     # ---- the benchmark configurations, every path --------------------------
     if not args.no_extra:
         configs = {}
-        cfg1 = {"metric": "grid-pt*Hpsi/s, 2D 512x512 two states, Chebyshev dt 5", "unit": UNIT,
+        rec2d = {"metric": "grid-pt*Hpsi/s, 2D 512x512 two states, Chebyshev dt 5", "unit": UNIT,
                 "value": value, "roofline": out["roofline"], "cpu_baseline": out.get("cpu_baseline"),
                 "precisions": {"fp64": {"value": value, "ms_per_step": out["ms_per_step"],
                                         "hpsi_per_step": out["config"]["hpsi_per_step"]}}}
@@ -731,7 +731,7 @@ def main():
             ln, lms = rq["row"]
             bb = p.size * p.S * (B_ROW[prec] * (hh - args.steps) + B_ROW_PRO[prec] * args.steps)
             v = npts * hh / rr["seconds"]
-            cfg1["precisions"][prec] = {
+            rec2d["precisions"][prec] = {
                 "value": v, "unit": UNIT, "ms_per_step": rr["seconds"] * 1e3 / args.steps,
                 "hpsi_per_step": [int(rr["n_hpsi"].min()), int(rr["n_hpsi"].max())], "vs_fp64": v / value,
                 "l2_vs_oracle": ex.get("parity", {}).get(prec, {}).get("l2_vs_oracle"),
@@ -741,7 +741,7 @@ def main():
                              "bytes_model": f"{B_ROW[prec]} B per grid point and state per executed order"},
                 "col_kernel_avg_launch_us": rq["line"][1] / max(rq["line"][0], 1) * 1e3}
             log(f"{prec}: {v:.4g} grid-pt*Hpsi/s")
-        configs["cheby_2d_512x512_2state"] = cfg1
+        configs["cheby_2d_512x512_2state"] = rec2d
         # GSPO on the same workload: at least 100 timed steps
         gsteps = max(100, args.steps)
         gs = {"metric": "grid-pt*step/s, 2D 512x512 two coupled states, GSPO dt 5", "unit": "grid-pt*step/s",

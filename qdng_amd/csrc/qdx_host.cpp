@@ -1687,19 +1687,22 @@ int qdx_eigen_ac(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nsteps, 
         if (!H) fail(QDX_ERR_PARAM, "a Hamiltonian is required for the energies");
         if (!(H->grid == p0->grid) || H->S != p0->S || H->ctx != p0->ctx)
             fail(QDX_ERR_INCOMPAT, "Hamiltonian and wave function differ");
-        WfHold keep, tmp, tbuf;
-        // ProgEigen.cpp:436-470: propagate, keep every step, <psi(t)|psi0>
+        WfHold keep, tmp;
+        // ProgEigen.cpp:436-470: propagate and record <psi(t)|psi0>.  QDng
+        // keeps every step's wave function (WFBuffer, swapped to disk beyond
+        // its memory cap); here the propagation runs twice instead: this pass
+        // records the autocorrelation, the second one (below, after the peak
+        // search) adds each step into the peak functions.  The propagation is
+        // deterministic, so both passes see the same psi(t) and the device
+        // holds n_ef + 2 wave functions instead of nsteps.
         const int N = nsteps;
         std::vector<std::complex<double>> ac(N);
         ac[0] = 1.0;
-        std::vector<qdx_wf *> buf(N);
-        buf[0] = p0;
+        const int arn_order0 = U->arn_order;   // Arnoldi carries its order between steps
         qdx_wf *psi = tmp.make(p0);
         chk(qdx_wf_copy(psi, p0));
         for (int i = 1; i < N; ++i) {
             prop_step(U, psi, i - 1);
-            buf[i] = tbuf.make(p0);
-            chk(qdx_wf_copy(buf[i], psi));
             ovl(psi, p0, ac[i]);
             if (std::isnan(ac[i].real())) fail(QDX_ERR_OVERFLOW, "Correlation is not a number");
         }
@@ -1722,19 +1725,28 @@ int qdx_eigen_ac(qdx_prop *U, qdx_ham *Hin, const qdx_wf *psi0, int32_t nsteps, 
         const double dt = U->dt;
         const double dw = 2 * M_PI / (dt * double(N));
         std::vector<qdx_wf *> ef;
-        std::vector<double> e_raw(n_ef);
+        std::vector<double> e_raw(n_ef), ek(n_ef);
         for (int k = 0; k < n_ef; ++k) {
             const int idx = pk[k];
-            const double e = idx < N / 2 ? double(idx) * dw : double(idx - N) * dw;
+            ek[k] = idx < N / 2 ? double(idx) * dw : double(idx - N) * dw;
             qdx_wf *f = keep.make(p0);
             chk(qdx_wf_copy(f, p0));
-            for (int s = 1; s < N; ++s) {
-                const double ph = e * double(s) * dt;
-                chk(qdx_wf_axpy(f, std::cos(ph), std::sin(ph), buf[s]));
-            }
-            chk(qdx_wf_normalize(f));
-            chk(qdx_ham_expec(H, f, 0, &e_raw[k]));
             ef.push_back(f);
+        }
+        // second pass: f_k = psi0 + sum_s exp(i E_k s dt) psi(s dt), each
+        // f_k summed in step order
+        U->arn_order = arn_order0;
+        chk(qdx_wf_copy(psi, p0));
+        for (int s = 1; s < N; ++s) {
+            prop_step(U, psi, s - 1);
+            for (int k = 0; k < n_ef; ++k) {
+                const double ph = ek[k] * double(s) * dt;
+                chk(qdx_wf_axpy(ef[k], std::cos(ph), std::sin(ph), psi));
+            }
+        }
+        for (int k = 0; k < n_ef; ++k) {
+            chk(qdx_wf_normalize(ef[k]));
+            chk(qdx_ham_expec(H, ef[k], 0, &e_raw[k]));
         }
         if (energies_raw) std::copy(e_raw.begin(), e_raw.end(), energies_raw);
         if (diag) ef = diag_basis(H, p0, ef, e_raw, keep, tmp, energies);

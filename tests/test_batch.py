@@ -48,23 +48,28 @@ def test_batch_matches_single_path(ctx, precision):
     B = Q.Batch(ctx, precision, 1 << 30)
     ids = [B.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in jobs]
     B.run(steps)
-    # FP32 paths: the 1D jobs run whole steps in registers (psi updated every
+    # FP32: the 1D jobs run whole steps in registers (psi updated every
     # order), the single path updates psi every third order from the stored
-    # FP32 terms; both carry the FP32 transform error, ~5e-6 relative to FP64
-    # for the 3072-point job's 58-term series over 12 steps (FP32 arithmetic
-    # does not meet the 1e-6 bar on such series; the compact storage does,
-    # test_compact_batch_vs_fp64).  Compact: the same FP64 arithmetic with
-    # the terms rounded to 40 bits in both paths, 1D jobs exactly FP64
-    tol = {"fp64": 1e-12, "fp32": 2e-5, "compact": 1e-8}[precision]
+    # FP32 terms, so the two FP32 results differ by their rounding; each is
+    # held to the 1e-6 bar against the FP64 single path instead (the
+    # 3072-point job's 58-term series over 12 steps misses it, ~5e-6: FP32
+    # arithmetic; the compact storage meets it, test_compact_batch_vs_fp64).
+    # Compact: the same FP64 arithmetic with the terms rounded to 40 bits in
+    # both paths, 1D jobs exactly FP64
+    tol = {"fp64": 1e-12, "compact": 1e-8}.get(precision)
+    errs = []
     for p, jid in zip(jobs, ids):
         left, nh, done = B.state(jid)
         got = B.fetch(jid)
-        ref, nh_ref = single_path(ctx, p, 1.0, steps, precision)
+        ref, nh_ref = single_path(ctx, p, 1.0, steps, "fp64" if precision == "fp32" else precision)
         note(f"  job N={p.N} nhpsi {nh}/{nh_ref} rel {rel(got, ref):.2e}")
         assert left == 0 and done == steps
         if precision != "fp32":
             assert nh == nh_ref
-        assert rel(got, ref) < tol
+            assert rel(got, ref) < tol
+        errs.append(rel(got, ref))
+    if precision == "fp32":
+        cases.bulk_bar(errs, "fp32", f" (batch vs FP64, worst of {len(errs)} jobs)")
 
 
 def test_compact_batch_vs_fp64(ctx):

@@ -8,8 +8,8 @@ against the plain FP64 path on the same complex64 initial state.
 - "fp32": FP32 transforms and terms.  Its error is dominated by the FP32
   arithmetic of the transforms (a CPU model with FP64 transforms and FP32
   terms gives a fifth of it) and grows like the square root of the number of
-  steps; at order ~256 it is above the 1e-6 bar (measured 4.5e-6 on MI355X),
-  which the test states rather than hides.
+  steps; at order ~256 it is above the 1e-6 bar (measured 4.5e-6 on MI355X):
+  that case is an expected failure that reports its measured error.
 
 The FP64 device path is the reference at this size; it is itself tied to the
 FP64 oracle (<= 1e-10) on the same Hamiltonian at 24^3 and 32^3 in
@@ -28,7 +28,7 @@ import cases
 
 pytestmark = pytest.mark.gpu
 
-BAR = 1e-6          # relative L2 of the bulk paths against FP64
+BAR = cases.BULK_BAR   # relative L2 of the bulk paths against FP64
 
 
 def _run(ctx, p, dt, precision, steps):
@@ -79,9 +79,7 @@ def test_bulk_vs_fp64_3d_128(ctx, dt, order_lo, order_hi, precision):
         return
     assert driftb < 1e-6
     assert dac < 1e-6
-    if dt < 10:
-        assert err < BAR
-    else:
-        # FP32 arithmetic at order ~256: above the bar, within its measured
-        # law (~1e-6 per sqrt(step) at this series length)
-        assert BAR <= err < 1e-5
+    # FP32 arithmetic at order ~256 misses the bar (measured 4.5e-6 after
+    # 20 steps: ~1e-6 per sqrt(step) at this series length), reported as an
+    # expected failure with its value
+    cases.bulk_bar(err, precision, f" (128^3, dt {dt}, order {order64}, {steps} steps)")

@@ -103,7 +103,7 @@ def test_tile_shapes_agree(ctx):
         assert rel(a, b) < 1e-12
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", cases.BULK_BAR)])
 def test_radix_plans_agree(ctx, precision, tol):
     import qdng_amd as Q
     steps = 4
@@ -115,11 +115,19 @@ def test_radix_plans_agree(ctx, precision, tol):
         c2.set_radix_plan(L, r, precision)
         assert c2.radix_plan(L, precision) == r
     got, nh, _ = run_batch(c2, precision, jobs, steps)
+    if precision == "fp32":
+        # the two FP32 runs differ by their rounding (other transform
+        # orders); each is held to the bar against the FP64 batch
+        ref, _, _ = run_batch(ctx, "fp64", jobs, steps)
+    errs = []
     for p, a, b, h, hr in zip(jobs, got, ref, nh, nh_ref):
         note(f"  job {p.N} rel {rel(a, b):.2e}")
         if precision == "fp64":
             assert h == hr
-        assert rel(a, b) < tol
+            assert rel(a, b) < tol
+        errs.append(rel(a, b))
+    if precision == "fp32":
+        cases.bulk_bar(errs, "fp32", " (other radix plans vs the FP64 batch)")
 
 
 def test_schedule_arguments_are_checked(ctx):
@@ -166,6 +174,7 @@ def test_autotune_installs_a_valid_schedule(ctx, precision, tmp_path):
     jobs = [cases.driven_job([96, 64], rng, steps), cases.driven_job([128, 128], rng, steps)]
     ids = [B.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in jobs]
     B.run(steps)
+    fp32_errs = []
     for p, jid in zip(jobs, ids):
         got = B.fetch(jid)
         if precision == "fp64":
@@ -178,7 +187,7 @@ def test_autotune_installs_a_valid_schedule(ctx, precision, tmp_path):
             U = Q.Cheby(p.gpu_ham(ctx), 1.0, prec=1e-10, precision="fp64")
             w = p.gpu_wf(ctx)
             U.run(w, 0, steps, observables=False)
-            assert rel(got, w.download()) < 2e-5
+            fp32_errs.append(rel(got, w.download()))
     # the tuning file restores the schedule in a fresh context
     path = str(tmp_path / "tuning.txt")
     B.save_tuning(path)
@@ -194,6 +203,8 @@ def test_autotune_installs_a_valid_schedule(ctx, precision, tmp_path):
     bad.write_text("plan 96 8 4 3\n")
     with pytest.raises(Q.QdxError):
         B3.load_tuning(str(bad))
+    if precision == "fp32":
+        cases.bulk_bar(fp32_errs, "fp32", " (tuned schedule vs the FP64 path)")
 
 
 def test_autotune_on_submit_keeps_results_mix_independent(ctx, tmp_path):
