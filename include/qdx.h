@@ -232,8 +232,7 @@ int qdx_arnoldi_create(qdx_ham *h, double dt, int32_t order, double conv, int32_
 int qdx_eigen_imag(qdx_prop *U, qdx_ham *H, const qdx_wf *psi0, int32_t nef, int32_t max_steps, double conv,
                    int32_t diag, qdx_wf **efs, double *energies, double *energies_raw, int32_t *steps);
 /* ProgEigen method "ac" (ProgEigen::AutoCorrEF, src/ProgEigen.cpp:414-600):
- * nsteps - 1 real-time steps of U from psi0, every state kept on the device
- * and c(t) = <psi(t)|psi0>; the Hann window (win != 0, math_functions.h:
+ * nsteps - 1 real-time steps of U from psi0 and c(t) = <psi(t)|psi0>; the Hann window (win != 0, math_functions.h:
  * 108-113), the forward FFT and PeakFinder on |FFT c| (tolerance tol, QDng
  * default 1; width > 0: the windowed search, else the region search,
  * tools/PeakFinder.cpp:83-170) give one energy E = index * 2 pi/(dt nsteps)
@@ -242,8 +241,11 @@ int qdx_eigen_imag(qdx_prop *U, qdx_ham *H, const qdx_wf *psi0, int32_t nef, int
  * <psi|H|psi>; diag != 0 then diagonalises H in that basis (DiagBasis).
  * *nef receives the number of peaks (QDX_ERR_PARAM when above max_ef; efs,
  * energies and energies_raw hold max_ef entries); autocorr (nullable,
- * 2*nsteps doubles) receives c(t) before the window.  Device memory: nsteps
- * wave functions. */
+ * 2*nsteps doubles) receives c(t) before the window.  QDng keeps every
+ * step's function (WFBuffer); here the propagation runs a second time to add
+ * each step into the peak functions, so the device holds n_ef + 2 wave
+ * functions (the propagation is deterministic: both passes see the same
+ * psi(t)). */
 int qdx_eigen_ac(qdx_prop *U, qdx_ham *H, const qdx_wf *psi0, int32_t nsteps, int32_t win, double tol,
                  int32_t width, int32_t diag, int32_t max_ef, qdx_wf **efs, double *energies, double *energies_raw,
                  int32_t *nef, double *autocorr);
@@ -296,6 +298,77 @@ int qdx_prop_set_direction(qdx_prop *p, int32_t backward);
  * exp(c H) for one step, as (re, im). */
 int qdx_prop_exponent(qdx_prop *p, double out[2]);
 int qdx_prop_destroy(qdx_prop *p);
+
+/* ---- optimal control (ProgOCT, src/ProgOCT.cpp) ----------------------------
+ * The laser samples of a Hamiltonian's dipole terms are QDng's Laser objects
+ * (Laser.cpp:125-136): the field applied at clock step s is laser[s] * scale.
+ * qdx_ham_laser_get reads the raw samples of the first dipole term (n values,
+ * n <= its length); qdx_ham_laser_set writes them into every dipole term
+ * (the identical lasers ProgOCT optimises, ProgOCT.cpp:925-950), n at most
+ * the shortest term's length.  A propagator created on the Hamiltonian sees
+ * the new samples from its next step; its Chebyshev scaling stays the one of
+ * its creation (OCheby::Init runs once, as in ProgOCT::Run). */
+int qdx_ham_laser_len(qdx_ham *h, int64_t *n);
+int qdx_ham_laser_get(qdx_ham *h, double *laser, int64_t n);
+int qdx_ham_laser_set(qdx_ham *h, const double *laser, int64_t n);
+
+/* OProjection (OProjection.cpp:93-125): P psi = sign sum_k <wf_k|psi> wf_k
+ * (sign -1 when positive = 0), or psi - that when inv != 0.  The functions
+ * are copied; a projection onto one state of a multistate function is a set
+ * of multistate functions that are zero in the other states (QDng's
+ * OMultistate of projections with unity off, as in OCT operator targets). */
+typedef struct qdx_proj qdx_proj;
+int qdx_proj_create(qdx_wf *const *wfs, int32_t n, int32_t positive, int32_t inv, qdx_proj **out);
+int qdx_proj_apply(qdx_proj *p, qdx_wf *dst, qdx_wf *src);
+/* Re <psi|P|psi> (Operator::Expec, Operator.h:249-252) */
+int qdx_proj_expec(qdx_proj *p, qdx_wf *psi, double *out);
+int qdx_proj_destroy(qdx_proj *p);
+
+typedef enum qdx_oct_method {
+    QDX_OCT_KROTOV = 0,      /* ProgOCT::Iterate, new field = old - correction   */
+    QDX_OCT_RABITZ = 1,      /* ProgOCT::Iterate, new field = -correction        */
+    QDX_OCT_RABITZFB = 2     /* ProgOCT::IterateRabitzFB (Zhu and Rabitz 1998)   */
+} qdx_oct_method;
+
+typedef struct qdx_oct_opts {
+    int32_t method;          /* qdx_oct_method                                  */
+    int32_t operator_target; /* 0: overlap targets psi_t, 1: operator targets   */
+    int32_t phase;           /* phase-sensitive objective                       */
+    int32_t ntargets;        /* 1..16 (MAX_TARGETS, ProgOCT.h:19)               */
+    int32_t iterations;      /* maximum iterations (QDng default 50)            */
+    int32_t nsteps;          /* clock steps of one propagation                   */
+    double conv;             /* convergence (QDng default 1e-5)                  */
+    double alpha;            /* field restriction, != 0                          */
+} qdx_oct_opts;
+
+/* Columns of one report row (ProgOCT::Report, ProgOCT.cpp:363-425): the
+ * iteration, the laser objective (dt * sum of the update terms), the
+ * objective (yield - laser objective), the pulse energy of the applied field
+ * (Laser::PulseEnergy, Laser.cpp:174-184) and the returned yield (|sum of
+ * |overlap|^2|, |sum of overlaps| when phase-sensitive, sum |<O>|), then per
+ * target: norm of the propagated initial state, norm of the target state,
+ * |overlap|^2, phase of the overlap / pi, <O> (NaN where QDng prints
+ * nothing; the iteration-0 row has NaN laser objective and objective). */
+#define QDX_OCT_ROW_FIXED 5
+#define QDX_OCT_ROW_PER_TARGET 5
+
+/* ProgOCT::Run (ProgOCT.cpp:868-1092) over the propagator U, whose
+ * Hamiltonian's dipole terms (or, for GSPO, its split operators' dipole
+ * terms) carry the field being optimised.  The coupling operator is the sum
+ * of those dipole arrays (single state) or the multistate operator of the
+ * off-diagonal ones, with unity off (ChainLoader.h:48-160, ProgOCT.cpp:
+ * 954-961).  psi_i: ntargets initial states; psi_t: ntargets target states
+ * (operator_target = 0); targets: ntargets projections (operator_target =
+ * 1); shape: the field's shape function (at least nsteps samples); gobbler
+ * (nullable): applied to every propagated state after each step, which is
+ * then normalised.  Device memory: nsteps + 1 wave functions per target (the
+ * backward-propagation buffer).  On return the Hamiltonian's lasers hold the
+ * optimised field (QDng writes it as the final laser file); report
+ * (nullable) receives (iterations + 1) rows, rows_done the rows written,
+ * converged whether the change of the yield fell to conv. */
+int qdx_oct_run(qdx_prop *U, const qdx_oct_opts *opts, const double *shape, int64_t nshape,
+                qdx_wf *const *psi_i, qdx_wf *const *psi_t, qdx_proj *const *targets, qdx_ham *gobbler,
+                double *report, int32_t *rows_done, int32_t *converged);
 
 /* ---- batched jobs ---------------------------------------------------------
  * Many independent propagations (1D or 2D grids of different sizes, each

@@ -63,6 +63,7 @@ def lib():
             "orc_op_potential": [C.c_int, dp],
             "orc_op_dipole": [C.c_int, dp, dp, C.c_int, C.c_double],
             "orc_op_sum": [ip, C.c_int],
+            "orc_op_set_laser": [C.c_int, dp, C.c_int],
             "orc_op_multistate": [C.c_int, ip, ip, ip, C.c_int, C.c_int, C.c_int],
             "orc_op_bounds": [C.c_int, dp, dp],
             "orc_op_bounds_c": [C.c_int, dp, dp],
@@ -223,6 +224,12 @@ def dipole(grid: Grid, mu, laser, scale: float = 1.0) -> Op:
     e = _f64(laser).ravel()
     assert m.size == grid.size
     return Op(_chk(lib().orc_op_dipole(grid.id, _dp(m), _dp(e), e.size, float(scale))), 1, grid)
+
+
+def set_laser(op: Op, laser) -> None:
+    """Raw field samples of a GridDipole operator (Laser::Set)."""
+    e = _f64(laser).ravel()
+    _chk(lib().orc_op_set_laser(op.id, _dp(e), e.size))
 
 
 def gobbler(grid: Grid, lp=None, rp=None, order: int = 20, gain: float = 1.0, nip: bool = False) -> Op:
@@ -432,3 +439,202 @@ def autocorr_ef(step, overlap, expec, psi0, nsteps, dt, win=True, tol=1.0, width
         efs.append(f)
         e_raw.append(expec(f))
     return np.array(e_raw), pk, raw, efs
+
+
+# ---------------------------------------------------------------------------
+# Optimal control: ProgOCT (src/ProgOCT.cpp:300-1092) restated over the
+# oracle's propagators.  The field lives in the GridDipole operators `dips`
+# (Laser::Get = raw sample * lscale); `coup` is the coupling operator (the
+# dipole arrays without the field, ProgOCT.cpp:954-961); a target operator
+# is a projection (vectors, positive, inv) (OProjection.cpp:93-125).
+
+def _proj(target, psi, grid, S):
+    vecs, positive, inv = target
+    sign = 1.0 if positive else -1.0
+    acc = np.zeros_like(psi)
+    for v in vecs:
+        acc += overlap(grid, v, psi, S) * sign * v
+    return psi - acc if inv else acc
+
+
+def oct_run(U, dips, laser0, coup, grid, S, shape, psi_i, nsteps, dt, psi_t=None, targets=None,
+            method="krotov", phase=False, iterations=50, conv=1e-5, alpha=1.0, lscale=1.0, gobbler=None):
+    """Returns (report rows, optimised field, converged).  A row holds the
+    iteration, laser objective, objective, pulse energy, yield and per target
+    (norm, target norm, |overlap|^2, phase/pi, <O>), NaN where not printed."""
+    nt = len(psi_i)
+    op_target = targets is not None
+    Ef = np.array(laser0, dtype=float)                   # _laserf
+    Eb = Ef.copy()                                       # _laserb
+
+    def push():
+        for d in dips:
+            set_laser(d, Ef)
+
+    def swap():
+        nonlocal Ef, Eb
+        Ef, Eb = Eb, Ef
+        push()
+
+    def step(w, s):
+        U.step(w, s)
+        if gobbler is not None:                          # Apply, then Normalize
+            w[:] = gobbler.apply(w, s)
+            w /= np.sqrt(overlap(grid, w, w, S).real)
+
+    mem = [[None] * (nsteps + 1) for _ in range(nt)]
+    st = {"laserobj": 0.0}
+
+    def corr(wfi, wft):                                  # CalcCorr
+        res = 0.0
+        for t in range(nt):
+            opwf = coup.apply(wfi[t], 0)
+            if phase:
+                im = overlap(grid, wft[t], opwf, S)
+            else:
+                im = overlap(grid, wfi[t], wft[t], S) * overlap(grid, wft[t], opwf, S)
+            res += im.imag
+        return res
+
+    def field(wfi, wft, s):                              # CalcLaserField
+        im = corr(wfi, wft)
+        sh = shape[s]
+        if method == "krotov":
+            res = Eb[s] * lscale - sh / (alpha * nt) * im
+            st["laserobj"] += (sh / alpha * im) ** 2
+        else:
+            res = -1.0 * sh / (alpha * nt) * im
+            st["laserobj"] += -1.0 / nt * im * res
+        if np.isnan(res):
+            raise OracleError("Laserfield is not a number")
+        return res
+
+    def forward(wf):                                     # PropagateForward
+        for t in range(nt):
+            mem[t][0] = wf[t].copy()
+        for s in range(nsteps):
+            for t in range(nt):
+                step(wf[t], s)
+                mem[t][s + 1] = wf[t].copy()
+
+    def backward(wf):                                    # PropagateBackward
+        swap()
+        U.backward(True)
+        for t in range(nt):
+            mem[t][nsteps] = wf[t].copy()
+        for s in range(nsteps - 1, -1, -1):
+            for t in range(nt):
+                step(wf[t], s)
+                mem[t][s] = wf[t].copy()
+        swap()
+        U.backward(False)
+
+    def iterate(phii, phit):                             # Iterate
+        if not op_target:
+            backward(phit)
+        else:
+            swap()
+            forward(phit)
+            swap()
+            for t in range(nt):
+                phit[t][:] = _proj(targets[t], phit[t], grid, S)
+            backward(phit)
+        for t in range(nt):
+            phii[t][:] = psi_i[t]
+        for s in range(nsteps):
+            bT = [mem[t][s] for t in range(nt)]
+            Ef[s] = field(phii, bT, s)
+            for d in dips:
+                set_laser(d, Ef)
+            for t in range(nt):
+                step(phii[t], s)
+        for t in range(nt):
+            phit[t][:] = mem[t][nsteps]
+
+    def iterate_fb(phii, phit, it):                      # IterateRabitzFB
+        if it == 1:
+            forward(phii)
+        for t in range(nt):
+            phii[t][:] = mem[t][0]
+        if op_target:
+            for t in range(nt):
+                phit[t][:] = _proj(targets[t], phii[t], grid, S)
+        swap()
+        U.backward(True)
+        for s in range(nsteps - 1, -1, -1):
+            bT = [mem[t][s + 1] for t in range(nt)]
+            Eb[s] = field(bT, phit, s)
+            for t in range(nt):
+                mem[t][s + 1] = phit[t].copy()
+            for t in range(nt):
+                step(phit[t], s)
+        for t in range(nt):
+            mem[t][0] = phit[t].copy()
+        swap()
+        U.backward(False)
+        for t in range(nt):
+            phit[t][:] = mem[t][nsteps]
+        for s in range(nsteps):
+            bT = [mem[t][s] for t in range(nt)]
+            Ef[s] = field(phii, bT, s)
+            for d in dips:
+                set_laser(d, Ef)
+            for t in range(nt):
+                mem[t][s] = phii[t].copy()
+            for t in range(nt):
+                step(phii[t], s)
+        for t in range(nt):
+            mem[t][nsteps] = phii[t].copy()
+
+    def report(wfi, wft, it):                            # Report
+        row = [float(it), np.nan, np.nan, 0.0, 0.0] + [np.nan] * (5 * nt)
+        ov_sum = 0j
+        for t in range(nt):
+            b = 5 + 5 * t
+            row[b] = np.sqrt(overlap(grid, wfi[t], wfi[t], S).real)
+            if not op_target:
+                row[b + 1] = np.sqrt(overlap(grid, wft[t], wft[t], S).real)
+                ov = overlap(grid, wfi[t], wft[t], S)
+                row[b + 2] = abs(ov) ** 2
+                if phase:
+                    row[b + 3] = np.angle(ov) / np.pi
+                    ov_sum += ov
+                else:
+                    ov_sum += abs(ov) ** 2
+            else:
+                e = overlap(grid, wfi[t], _proj(targets[t], wfi[t], grid, S), S).real
+                row[b + 4] = e
+                ov_sum += abs(e)
+        if it > 0:
+            row[1] = st["laserobj"]
+            row[2] = ov_sum.real - st["laserobj"]
+        row[3] = 137.0359996 / (4 * np.pi) * float(np.sum(Ef * Ef)) * lscale * dt
+        row[4] = abs(ov_sum)
+        return row
+
+    push()
+    phii = [p.copy() for p in psi_i]
+    phit = [p.copy() for p in (psi_t if not op_target else psi_i)]
+    rows = []
+    forward(phii)
+    rows.append(report(phii, psi_t if not op_target else phii, 0))
+    dT = [1.0, 1.0, 1.0]
+    delta = 1.0
+    it = 1
+    while it <= iterations and delta > conv:
+        if method != "rabitzfb" or it == 1:
+            for t in range(nt):
+                phii[t][:] = psi_i[t]
+                phit[t][:] = (psi_t[t] if not op_target else psi_i[t])
+        st["laserobj"] = 0.0
+        if method == "rabitzfb":
+            iterate_fb(phii, phit, it)
+        else:
+            iterate(phii, phit)
+        st["laserobj"] *= dt
+        rows.append(report(phii, phit, it))
+        dT = [dT[1], dT[2], abs(rows[-1][4])]
+        delta = abs(dT[0] - dT[1]) + abs(dT[1] - dT[2])
+        swap()
+        it += 1
+    return np.array(rows), Eb.copy(), delta <= conv

@@ -1262,6 +1262,18 @@ int orc_op_dipole(int grid, const double *mu, const double *laser, int nt, doubl
     });
 }
 
+/* Laser::Set over the whole field (ProgOCT.cpp:566-567): the raw samples of
+ * a GridDipole operator, nt at most its length */
+int orc_op_set_laser(int op, const double *laser, int nt) {
+    return guarded([&] {
+        Op &o = op_at(op);
+        if (o.kind != DIP) throw OracleError("not a GridDipole operator");
+        if (nt < 0 || nt > (int)o.laser.size()) throw OracleError("laser longer than the operator's field");
+        std::copy(laser, laser + nt, o.laser.begin());
+        return 0;
+    });
+}
+
 int orc_op_sum(const int *ops, int n) {
     return guarded([&] {
         if (n < 1) throw OracleError("Sum Operator is empty");

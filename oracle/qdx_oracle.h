@@ -61,6 +61,8 @@ int orc_op_kinetic(int grid, const double *mass);            /* GridNablaSq  */
 int orc_op_potential(int grid, const double *V);              /* GridPotential */
 int orc_op_dipole(int grid, const double *mu, const double *laser, int nt,
                   double scale);                               /* GridDipole   */
+/* the raw samples of a GridDipole's field (Laser::Set, Laser.cpp:133-136) */
+int orc_op_set_laser(int op, const double *laser, int nt);
 int orc_op_sum(const int *ops, int n);                         /* Sum          */
 /* Gobbler: lp, rp hold one edge per dimension, NaN = none; nip = 1: -i v */
 int orc_op_gobbler(int grid, const double *lp, const double *rp, int order, double gain, int nip);

@@ -75,6 +75,16 @@ EXPORTS = [
 K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
 
 
+class OctOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("operator_target", C.c_int32), ("phase", C.c_int32),
+                ("ntargets", C.c_int32), ("iterations", C.c_int32), ("nsteps", C.c_int32),
+                ("conv", C.c_double), ("alpha", C.c_double)]
+
+
+OCT_METHODS = {"krotov": 0, "rabitz": 1, "rabitzfb": 2}
+OCT_ROW_FIXED, OCT_ROW_PER_TARGET = 5, 5
+
+
 class JobDesc(C.Structure):
     _fields_ = [("ham", C.c_void_p), ("ops", C.POINTER(C.c_void_p)), ("nops", C.c_int32), ("nsteps", C.c_int32),
                 ("dt", C.c_double), ("order", C.c_int32), ("reserved", C.c_int32), ("prec", C.c_double),
@@ -132,6 +142,15 @@ def lib():
         "qdx_wf_normalize": [vp],
         "qdx_wf_destroy": [vp],
         "qdx_ham_create": [vp, C.POINTER(Grid), i32, i32, i32, C.POINTER(vp)],
+        "qdx_ham_laser_len": [vp, C.POINTER(C.c_int64)],
+        "qdx_ham_laser_get": [vp, dp, C.c_int64],
+        "qdx_ham_laser_set": [vp, dp, C.c_int64],
+        "qdx_proj_create": [C.POINTER(vp), i32, i32, i32, C.POINTER(vp)],
+        "qdx_proj_apply": [vp, vp, vp],
+        "qdx_proj_expec": [vp, vp, dp],
+        "qdx_proj_destroy": [vp],
+        "qdx_oct_run": [vp, C.POINTER(OctOpts), dp, C.c_int64, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp,
+                        dp, C.POINTER(i32), C.POINTER(i32)],
         "qdx_ham_add_kinetic": [vp, i32, dp],
         "qdx_ham_add_potential": [vp, i32, i32, dp],
         "qdx_ham_add_dipole": [vp, i32, i32, dp, dp, i32, C.c_double],
@@ -612,6 +631,19 @@ class Hamiltonian:
         _chk(lib().qdx_ham_expec(self.h, psi.h, int(step), C.byref(out)))
         return out.value
 
+    def laser(self) -> np.ndarray:
+        """Raw field samples of the first dipole term (Laser)."""
+        n = C.c_int64()
+        _chk(lib().qdx_ham_laser_len(self.h, C.byref(n)))
+        out = np.zeros(n.value)
+        _chk(lib().qdx_ham_laser_get(self.h, _dp(out), n.value))
+        return out
+
+    def set_laser(self, field) -> None:
+        """Raw field samples of every dipole term (Laser::Set)."""
+        f = np.ascontiguousarray(field, dtype=np.float64).ravel()
+        _chk(lib().qdx_ham_laser_set(self.h, _dp(f), f.size))
+
     def cheby_params(self, dt: float, order: int = 0, prec: float = -1.0, scaling: float = 0.0):
         info = ChebyInfo()
         coeff = np.zeros(4096)
@@ -780,6 +812,49 @@ class GSPO(_Propagator):
         else:
             _chk(lib().qdx_gspo_create(arr, len(self.ops), float(dt), _PREC[precision], C.byref(self.h)))
         self._destroy = lib().qdx_prop_destroy
+
+
+class Projection:
+    """OProjection: P psi = sign sum_k <wf_k|psi> wf_k (or psi - that, inv)."""
+
+    def __init__(self, wfs, positive: bool = True, inv: bool = False):
+        arr = (C.c_void_p * len(wfs))(*[w.h.value for w in wfs])
+        self.h = C.c_void_p()
+        _chk(lib().qdx_proj_create(arr, len(wfs), int(bool(positive)), int(bool(inv)), C.byref(self.h)))
+        self._destroy = lib().qdx_proj_destroy
+
+    def apply(self, dst: WaveFunction, src: WaveFunction) -> None:
+        _chk(lib().qdx_proj_apply(self.h, dst.h, src.h))
+
+    def expec(self, psi: WaveFunction) -> float:
+        out = C.c_double()
+        _chk(lib().qdx_proj_expec(self.h, psi.h, C.byref(out)))
+        return out.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                self._destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def oct_run(U, shape, psi_i, nsteps: int, psi_t=None, targets=None, method="krotov", phase=False,
+            iterations: int = 50, conv: float = 1e-5, alpha: float = 1.0, gobbler=None):
+    """ProgOCT::Run on the device (qdx_oct_run).  Returns (report rows,
+    converged); the propagator's Hamiltonian then holds the optimised field."""
+    o = OctOpts(OCT_METHODS[method], int(targets is not None), int(bool(phase)), len(psi_i), int(iterations),
+                int(nsteps), float(conv), float(alpha))
+    sh = np.ascontiguousarray(shape, dtype=np.float64).ravel()
+    pi = (C.c_void_p * len(psi_i))(*[w.h.value for w in psi_i])
+    pt = (C.c_void_p * len(psi_i))(*[w.h.value for w in psi_t]) if psi_t is not None else None
+    tg = (C.c_void_p * len(psi_i))(*[t.h.value for t in targets]) if targets is not None else None
+    rows = np.full((iterations + 1, OCT_ROW_FIXED + OCT_ROW_PER_TARGET * len(psi_i)), np.nan)
+    nrows, conv_ = C.c_int32(), C.c_int32()
+    _chk(lib().qdx_oct_run(U.h, C.byref(o), _dp(sh), sh.size, pi, pt, tg, gobbler.h if gobbler else None,
+                           _dp(rows), C.byref(nrows), C.byref(conv_)))
+    return rows[:nrows.value], bool(conv_.value)
 
 
 class Batch:

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqdx.so")
 SOURCES = ["qdx_cheby64.hip", "qdx_cheby32.hip", "qdx_lines.hip", "qdx_gspo.hip", "qdx_kernels.hip", "qdx_batch.hip", "qdx_host.cpp",
-           "qdx_batch.cpp", "qdx_io.cpp"]
+           "qdx_batch.cpp", "qdx_io.cpp", "qdx_oct.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed-FP32 instructions are switched off for all device code.

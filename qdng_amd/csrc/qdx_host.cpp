@@ -16,45 +16,6 @@
 
 #include "qdx_host_internal.hpp"
 
-// ---------------------------------------------------------------------------
-// propagators
-struct qdx_prop {
-    enum Kind { CHEBY, GSPO, SIL, ARNOLDI } kind;
-    qdx_ctx *ctx = nullptr;
-    int precision = QDX_FP64;
-    double dt = 0;
-    Grid grid;
-    int S = 1;
-    // Chebyshev
-    qdx_ham *ham = nullptr;
-    ChebyParams cp;
-    int k_launch = 0;   // orders launched per step (launch_window)
-    bool imag = false;  // imaginary time, backward (exponent -dt)
-    bool backward = false;  // real time backward (exponent +i dt), any real-time kind
-    DevBuf phiA, phiB, X, ctl, partials, lastorders;
-    bool cmp = false;   // QDX_COMPACT on a grid with nd >= 2: compact terms
-    // single-state 1D lines: the whole step in one block (k_cheby_line) with
-    // the phased coefficients coeff[k] exp(-i offset dt) on the device
-    bool line_step = false;
-    DevBuf coefd;
-    int last_nb = 0, nb_init = 0;   // partial counts of the last Chebyshev step
-    // SIL (Lanczos vectors and a buffer as device wave functions; `imag`
-    // above selects imaginary time)
-    int sil_order = 20;
-    double sil_err = 1e-12;
-    std::vector<qdx_wf *> lzb;
-    qdx_wf *sil_buf = nullptr;
-    int sil_napply = 0;   // H applications of the last step (SIL, Arnoldi)
-    // Arnoldi (the Krylov vectors in lzb, the buffer in sil_buf)
-    double arn_conv = 1e-12;
-    int arn_order = 0;    // converged order; kept between steps as OArnoldi does
-    // GSPO
-    std::vector<qdx_ham *> ops;
-    DevBuf psi32, psi0_32;
-    DevBuf obs;   // device observables of qdx_prop_run
-
-    virtual ~qdx_prop() = default;
-};
 
 namespace {
 

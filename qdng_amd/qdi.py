@@ -32,6 +32,17 @@ Programs:
          function files are looked up in the output folder first (a later
          program reads what an earlier one wrote).
 
+  oct    ProgOCT (src/ProgOCT.cpp:59-300, 868-1092): `dt`, `steps`, `shape`
+         (laser file of the shape function), `method` (krotov, rabitz,
+         rabitzfb), `ttype` (overlap: `wft<n>` target functions; operator:
+         `target<n>` Projections, or a Multistate of diagonal Projections
+         with unity off), `phase`, `ntargets`, `alpha`, `iterations` (50),
+         `conv` (1e-5), `fname` ("laser"), `writeyield`/`yfile`; `wfi<n>`
+         initial functions, an optional `gobbler` operator; the optimised
+         laser is written as `dir/fname` and the iteration table of
+         ProgOCT::Report is logged (or written to the yield file).  Not
+         supported: method freq, moving targets (mv), writel.
+
 With `spectrum=` both programs write the autocorrelation spectrum
 (Reporter::WriteSpectrum, src/Reporter.cpp:160-190; Hann window unless
 `win="false"`).  `LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
@@ -52,8 +63,8 @@ import xml.etree.ElementTree as ET
 
 import numpy as np
 
-from . import (Arnoldi, Cheby, GSPO, Hamiltonian, QdxError, SIL, WaveFunction, eigen_ac, eigen_imag,
-               write_propagation_meta)
+from . import (Arnoldi, Cheby, GSPO, Hamiltonian, Projection, QdxError, SIL, WaveFunction, eigen_ac, eigen_imag,
+               load_laser, oct_run, save_laser, write_propagation_meta)
 
 ERR_PARAM, ERR_INCOMPAT = -1, -2
 AU2WAVENUMBERS = 219475.0   # src/qdlib/Conversion.h:9
@@ -670,7 +681,176 @@ def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
     return res
 
 
-PROGRAMS = {"propa": run_propa, "eigen": run_eigen}
+def _projection_target(ctx, op, psi, srcdir, base):
+    """An OCT operator target as a device projection: a Projection
+    (single state), or a Multistate of Projections on diagonal elements
+    with unity off (each function placed in its state, zero elsewhere)."""
+    N, lo, hi = _grid_lists(psi)
+
+    def efs(t):
+        p = t.params
+        return [WaveFunction.load(ctx, _resolve(p["files"], base), counter=p["start"] + k * p["step"])
+                for k in range(p["num"])]
+    if isinstance(op, Term) and op.kind == "proj":
+        if psi.S != 1:
+            raise _err("a Projection target of a multistate function: use a Multistate target", ERR_INCOMPAT)
+        return Projection(efs(op))
+    if isinstance(op, Multi):
+        if op.unity:
+            raise _err("a Multistate target needs unity=\"false\"", ERR_INCOMPAT)
+        wfs = []
+        for (i, j), e in sorted(op.elems.items()):
+            if i != j or not (isinstance(e, Term) and e.kind == "proj"):
+                raise _err("Multistate targets hold Projections on the diagonal only", ERR_INCOMPAT)
+            for f in efs(e):
+                w = WaveFunction(ctx, N, lo, hi, psi.S)
+                w.upload(np.zeros(psi.S * w.size, dtype=np.complex128))
+                w.upload(f.download(), i)
+                wfs.append(w)
+        return Projection(wfs)
+    raise _err("OCT operator targets: Projection or a Multistate of Projections", ERR_INCOMPAT)
+
+
+def _oct_table(rows, nt, overlap, phase):
+    """ProgOCT::Report's iteration table (ProgOCT.cpp:363-425): fixed
+    notation with 8 decimals."""
+    head = "iteration\t"
+    for t in range(nt):
+        head += "Norm_%d\t\t" % t
+        if overlap:
+            head += "Norm_t%d\t\tOverlapp_%d\t" % (t, t)
+        else:
+            head += "Operator_%d\t" % t
+        if phase and overlap:
+            head += "Phase_%d\t\t" % t
+    head += "Laser Objective\t\tObjective\t\tPulse Energy"
+    lines = [head]
+    for r in rows:
+        it = int(r[0])
+        s = "%d\t\t" % it
+        for t in range(nt):
+            b = 5 + 5 * t
+            s += "%.8f\t" % r[b]
+            if overlap:
+                s += "%.8f\t%.8f\t" % (r[b + 1], r[b + 2])
+                if phase:
+                    s += "%.8f\t" % r[b + 3]
+            else:
+                s += "%.8f\t" % r[b + 4]
+        s += "-\t\t\t-\t\t\t" if it == 0 else "%.8f\t\t%.8f\t\t" % (r[1], r[2])
+        s += "%.8f" % r[3]
+        lines.append(s)
+    return lines
+
+
+def run_oct(ctx, el, srcdir=None, outdir=None, log=None):
+    """ProgOCT (src/ProgOCT.cpp:59-300, 868-1092) on the device."""
+    a = _attrs(el, srcdir)
+    dt, steps = float(a.get("dt", 0)), int(a.get("steps", 0))
+    if dt == 0:
+        raise _err("Zero time step defined")
+    if steps == 0:
+        raise _err("Zero number of time step defined")
+    method = a.get("method", "krotov")
+    if method == "freq":
+        raise _err("OCT method 'freq' (frequency shaping) is not supported", ERR_INCOMPAT)
+    if method not in ("krotov", "rabitz", "rabitzfb"):
+        raise _err(f"Unknown OCT method {method}")
+    if a.get("coup", "dipole") != "dipole":
+        raise _err("Unsupported coupling specified")
+    ttype = a.get("ttype", "overlap")
+    if ttype not in ("overlap", "operator"):
+        raise _err(f"Unknown OCT target type {ttype}")
+    if _bool(a.get("mv")):
+        raise _err("moving targets (mv) are not supported", ERR_INCOMPAT)
+    if _bool(a.get("writel")):
+        raise _err("writing the laser of every iteration (writel) is not supported", ERR_INCOMPAT)
+    if "shape" not in a:
+        raise _err("No shape file given")
+    iterations = int(a.get("iterations", 50))
+    conv = float(a.get("conv", 1e-5))
+    alpha = float(a.get("alpha", 1.0))
+    nt = int(a.get("ntargets", 1))
+    phase = _bool(a.get("phase"))
+    d = _outdir(a, outdir)
+    ld = Loader(srcdir)
+    props = [c for c in el if c.tag == "propagator"]
+    if len(props) != 1:
+        raise _err("No propagator found")
+    spec = ld.propagator(props[0])
+    sect = {c.tag: c for c in el}
+    psi_i = []
+    for t in range(nt):
+        if "wfi%d" % t not in sect:
+            raise _err("Missing initial wavefunction %d" % t)
+        psi_i.append(load_wave_function(ctx, sect["wfi%d" % t], srcdir, outdir))
+    U, _ = build_propagator(ctx, spec, psi_i[0], dt)
+    kw = {}
+    if ttype == "overlap":
+        psi_t = []
+        for t in range(nt):
+            if "wft%d" % t not in sect:
+                raise _err("Missing target wavefunction %d" % t)
+            psi_t.append(load_wave_function(ctx, sect["wft%d" % t], srcdir, outdir))
+        kw["psi_t"] = psi_t
+    else:
+        tg = []
+        for t in range(nt):
+            if "target%d" % t not in sect:
+                raise _err("Missing target operator %d" % t)
+            tg.append(_projection_target(ctx, ld.op(sect["target%d" % t]), psi_i[0], srcdir, outdir))
+        kw["targets"] = tg
+    G = None
+    if "gobbler" in sect:
+        G = build_hamiltonian(ctx, ld.op(sect["gobbler"]), psi_i[0])
+    _, shape = load_laser(_resolve(a["shape"], outdir))
+    # the laser carried by the propagator's dipole terms and its file's dt
+    H = U.H if hasattr(U, "H") else next(o for o in U.ops if _has_dipole(o))
+    laser_dt = _dipole_laser_dt(spec)
+    rows, converged = oct_run(U, shape, psi_i, steps, method=method, phase=phase, iterations=iterations,
+                              conv=conv, alpha=alpha, gobbler=G, **kw)
+    save_laser(d + a.get("fname", "laser"), laser_dt, H.laser())
+    lines = _oct_table(rows, nt, ttype == "overlap", phase)
+    if _bool(a.get("writeyield")) or "yfile" in a:
+        with open(d + (a.get("yfile") or "Yield"), "w") as f:
+            f.write("\n".join(lines[1:]) + "\n")
+    if log:
+        for ln in lines:
+            log(ln)
+        if converged:
+            log("Convergence reached")
+    res = Result("oct", d)
+    res.rows, res.converged, res.table = rows, converged, lines
+    return res
+
+
+def _has_dipole(H):
+    try:
+        H.laser()
+        return True
+    except QdxError:
+        return False
+
+
+def _dipole_laser_dt(spec):
+    """dt of the first GridDipole's laser file in the propagator's operators."""
+    def walk(o):
+        if isinstance(o, Term):
+            if o.kind == "dip":
+                yield o
+        elif isinstance(o, Sum):
+            for c in o.items:
+                yield from walk(c)
+        elif isinstance(o, Multi):
+            for c in o.elems.values():
+                yield from walk(c)
+    for op in spec[2]:
+        for t in walk(op):
+            return load_laser(t.params["laser"])[0]
+    raise _err("Given type of coupling operator not found in hamiltonian")
+
+
+PROGRAMS = {"propa": run_propa, "eigen": run_eigen, "oct": run_oct}
 
 
 def programs(path):

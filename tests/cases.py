@@ -29,6 +29,7 @@ class Problem:
         self.unity = unity
         self.terms = []
         self.psi0 = None
+        self.oracle_dipoles = []
 
     # geometry -----------------------------------------------------------
     @property
@@ -89,9 +90,12 @@ class Problem:
             return O.potential(g, t[3])
         if t[0] == "gob":
             return O.gobbler(g, t[3], t[4], t[5], t[6], t[7])
-        return O.dipole(g, t[3], t[4], t[5])
+        d = O.dipole(g, t[3], t[4], t[5])
+        self.oracle_dipoles.append(d)
+        return d
 
     def oracle_ham(self):
+        self.oracle_dipoles = []   # the GridDipole operators of this tree
         if not self.multistate:
             return O.op_sum(*[self._oracle_term(t) for t in self.terms])
         elems = {}

@@ -60,6 +60,28 @@ EIGEN = {
     "TGE-SIL": ("SIL", {"order": "25"}),
     "TGE-Arnoldi": ("Arnoldi", {"order": "15"}),
 }
+# optimal control programs: OCT attributes, Hamiltonian kind and laser, the
+# initial state, the target (overlap with an eigenfunction, or a projection
+# onto eigenfunctions: (files, start, num), on state 1 for the two-state
+# case); the one-state cases follow with a GSPO propagation under the
+# optimised laser, as QDng's inputs do
+OCT = {
+    "TGO-IR-Krotov": dict(attrs=dict(alpha=400, dt=10, steps=6750, shape="GridShapeIR", method="krotov"),
+                          ham="1state+dipole", laser="GridLaser_w0.00713", target=("wft", "GridPot0_ef_1")),
+    "TGO-IR-Rabitz": dict(attrs=dict(alpha=50, dt=10, steps=6750, shape="GridShapeIR", iterations=8,
+                                     method="rabitz"),
+                          ham="1state+dipole", laser="GridLaser_w0.00713", target=("wft", "GridPot0_ef_1")),
+    "TGO-IR-Krotov-Projection": dict(attrs=dict(alpha=400, dt=10, steps=6750, shape="GridShapeIR",
+                                                method="krotov", ttype="operator"),
+                                     ham="1state+dipole", laser="GridLaser_w0.00713",
+                                     target=("projection", ("GridPot0_ef", 1, 1))),
+    "TGO-UV-Krotov": dict(attrs=dict(alpha=10, dt=1, steps=400, shape="GridShapeUV", method="krotov",
+                                     ttype="operator", iterations=20),
+                          ham="2state+dipole", laser="GridLaser_w0.176",
+                          target=("projection on state 1", ("GridPot0_ef", 0, 4))),
+}
+OCT_PROPA = dict(dt=10, steps=6750, wcycle=6750)
+
 EIGEN_IMAG = dict(dt=5, steps=1000, Nef=10)
 EIGEN_AC = dict(dt=5, steps=2048, lc=6)
 
@@ -210,8 +232,46 @@ def eigen(name):
     return root
 
 
+def oct_program(name):
+    spec = OCT[name]
+    root = _el("qdng")
+    attrs = dict(spec["attrs"])
+    attrs["shape"] = SRC + attrs["shape"]
+    o = _el("oct", root, dir=name, **attrs)
+    _propagator(o, dict(prop=("Cheby", {"prec": "1e-12"}), ham=spec["ham"]), spec["laser"])
+    if spec["ham"].startswith("1state"):
+        _el("wfi0", o, name="GridCartesian", file=SRC + "GridPot0_ef_0", normalize="true")
+    else:
+        w = _el("wfi0", o, name="Multistate", normalize="true", states="2")
+        _el("wf0", w, file=SRC + "GridPot0_ef_0")
+    kind, what = spec["target"]
+    if kind == "wft":
+        _el("wft0", o, name="GridCartesian", file=SRC + what, normalize="true")
+    elif kind == "projection":
+        files, start, num = what
+        _el("target0", o, name="Projection", files=SRC + files, start=start, num=num)
+    else:
+        files, start, num = what
+        t = _el("target0", o, name="Multistate", unity="false", states="2")
+        _el("m1.1", t, name="Projection", files=SRC + files, start=start, num=num)
+    if spec["ham"].startswith("1state"):
+        pr = _el("propa", root, dir=name, nfile="norm", spectrum="spectrum.dat", **OCT_PROPA)
+        p = _el("propagator", pr, name="GSPO")
+        _kinetic(p, "A", "T")
+        _potential(p, "B", "GridPot0", "V")
+        _el("C", p, name="GridDipole", file=SRC + "GridDip0", laser=name + "/laser")
+        _wavefunction(pr, "ef0")
+        _post_filters(pr, "1state", [{"start": "0"}, {"start": "1"}, {"start": "2"}])
+    return root
+
+
 def text(name):
-    root = propagation(name) if name in PROPAGATIONS else eigen(name)
+    if name in PROPAGATIONS:
+        root = propagation(name)
+    elif name in OCT:
+        root = oct_program(name)
+    else:
+        root = eigen(name)
     ET.indent(root, "  ")
     return ET.tostring(root, encoding="unicode") + "\n"
 
@@ -225,7 +285,7 @@ def write(name, outdir):
 
 def write_all(outdir):
     os.makedirs(outdir, exist_ok=True)
-    return {n: write(n, outdir) for n in list(PROPAGATIONS) + list(EIGEN)}
+    return {n: write(n, outdir) for n in list(PROPAGATIONS) + list(EIGEN) + list(OCT)}
 
 
 if __name__ == "__main__":

@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import cases
+from oracle import oracle as O
 from test_oracle import _digits_ok
 import qdng_amd as Q
 from qdng_amd import qdi
@@ -324,3 +325,96 @@ def test_filter_renorm_defaults(tmp_path, monkeypatch):
     assert values(sec % ' renorm="true"', renorm_default=False) == [0.75]
     assert values(sec.replace("init", "post") % "") == [0.75]
     assert values(sec.replace("init", "post") % ' renorm="false"') == [3.0]
+
+
+# ---------------------------------------------------------------------------
+# optimal control programs (ProgOCT)
+
+OCT = ["TGO-IR-Krotov", "TGO-IR-Rabitz", "TGO-IR-Krotov-Projection", "TGO-UV-Krotov"]
+
+
+def oct_program(name, tmp_path, iterations):
+    """The generated program with fewer iterations."""
+    import xml.etree.ElementTree as ET
+    tree = ET.parse(qdi_path(name))
+    tree.getroot().find("oct").set("iterations", str(iterations))
+    path = str(tmp_path / (name + ".qdi"))
+    tree.write(path)
+    return path
+
+
+def test_oct_inputs_parse_and_reject(tmp_path):
+    for name in OCT:
+        els = qdi.programs(qdi_path(name))
+        assert els[0].tag == "oct"
+        ld = qdi.Loader(FIX)
+        pname, _, _ = ld.propagator(els[0].find("propagator"))
+        assert pname == "Cheby"
+    # unsupported options fail before anything touches a device
+    for attrs, code in (('method="freq"', -2), ('mv="true"', -2), ('writel="true"', -2), ('method="x"', -1),
+                        ('ttype="phase"', -1), ('coup="raman"', -1), ('dt="0"', -1)):
+        f = tmp_path / "o.qdi"
+        base = {"dt": "1", "steps": "10", "shape": "s"}
+        xml = "<qdng><oct " + " ".join('%s="%s"' % kv for kv in base.items() if kv[0] + "=" not in attrs)
+        f.write_text(xml + " " + attrs + "/></qdng>")
+        with pytest.raises(Q.QdxError) as e:
+            qdi.run(str(f), None, FIX, str(tmp_path))
+        assert e.value.code == code, attrs
+
+
+def _oracle_uv_krotov(iterations):
+    """TGO-UV-Krotov restated on the oracle: the TGPLMS Hamiltonian (two
+    states, GridDip01 on m1.0 with GridLaser_w0.176), psi_i the ground state
+    on state 0, the projection onto GridPot0_ef_0..3 in state 1."""
+    a, _ = cases.kat_data()
+    p, _, dt, steps, _ = cases.kat_problem("TGPLMS-Cheby")
+    O.reset()
+    H = p.oracle_ham()
+    U = O.Cheby(H, dt, prec=1e-12)
+    g = p.oracle_grid()
+    coup = O.multistate(2, {(1, 0): O.potential(g, a["GridDip01"])}, unity=False)
+    n = p.size
+    vecs = []
+    for k in range(4):
+        v = np.zeros(2 * n, np.complex128)
+        v[n:] = a["GridPot0_ef_%d" % k]
+        vecs.append(v)
+    _, shape = Q.load_laser(os.path.join(FIX, "GridShapeUV"))
+    return O.oct_run(U, p.oracle_dipoles, a["GridLaser_w0.176"], coup, g, 2, shape, [p.psi0.copy()], steps, dt,
+                     targets=[(vecs, True, False)], method="krotov", iterations=iterations, alpha=10.0,
+                     conv=1e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", OCT)
+def test_oct_inputs(ctx, name, tmp_path):
+    print("[gpu case] qdi oct", name, flush=True)
+    its = 3 if name == "TGO-UV-Krotov" else 2
+    res = qdi.run(oct_program(name, tmp_path, its), ctx, FIX, str(tmp_path))
+    o = res[0]
+    assert o.kind == "oct" and len(o.rows) == its + 1
+    y = o.rows[:, 4]
+    if "Krotov" in name:
+        assert np.all(np.diff(y) > 0), y      # Krotov increases the yield
+    assert len(o.table) == its + 2 and o.table[0].startswith("iteration\t")
+    dt_l, field = Q.load_laser(os.path.join(o.dir, "laser"))
+    assert dt_l == (1.0 if name == "TGO-UV-Krotov" else 10.0)
+    assert np.any(field != Q.load_laser(os.path.join(FIX, "GridLaser_w0.176" if name == "TGO-UV-Krotov"
+                                                     else "GridLaser_w0.00713"))[1])
+    if name == "TGO-UV-Krotov":
+        rows_o, field_o, _ = _oracle_uv_krotov(its)
+        m = ~np.isnan(rows_o)
+        assert np.array_equal(np.isnan(o.rows), np.isnan(rows_o))
+        assert np.max(np.abs(o.rows[m] - rows_o[m]) / np.maximum(np.abs(rows_o[m]), 1e-12)) < 1e-8
+        assert np.max(np.abs(field - field_o)) < 1e-9
+    else:
+        # the propagation that follows reads the optimised laser (GSPO): its
+        # final |<ef_1|psi>|^2 is the last OCT yield (Cheby), to the
+        # difference of the two propagators
+        propa = res[1]
+        assert propa.kind == "propa"
+        last = open(os.path.join(propa.dir, "expec_post.dat")).read().splitlines()[-1].split("\t")
+        vals = [float(v) for v in last[1:] if v]
+        pop1 = vals[3]                        # momentum, position, projections 0, 1, 2, T, V
+        want = o.rows[-1, 5 + 2] if "Projection" not in name else o.rows[-1, 5 + 4]
+        assert abs(pop1 - want) < 0.02 * want + 1e-4, (pop1, want)
