@@ -1,14 +1,28 @@
-/* libqdx file formats: QDng's key/value meta files and raw binary data files,
- * so that QDng inputs (potentials, dipoles, lasers, initial wave functions)
- * load unchanged and wave functions are written the way QDng writes them.
+/* libqdx file formats: QDng's key/value meta files, raw binary data files
+ * and the protobuf stream format, so that QDng inputs (potentials, dipoles,
+ * lasers, initial wave functions) and QDng's outputs load unchanged, and
+ * wave functions and lasers are written the way QDng writes them.
  *
- * Scope: the binary ("raw + .meta") format of FileSingle
- * (src/tools/FileSingle.h:1213-1410), which QDng reads and writes by default
- * (FileWF.h:11: no compression; FileWF.cpp:44-56: wf_format "binary").  Not
- * built: the protobuf stream format (FileSingle.h:950-1210, "QDngPrtoBuf"
- * intro) and compressed payloads (ZLIB/BZIP, FileSingle.h:592-800); a raw
- * file whose last 8 bytes carry a compression tag is rejected with
- * QDX_ERR_INCOMPAT.
+ * Formats (FileSingle, src/tools/FileSingle.h):
+ *   - binary: "raw + .meta" (FileSingle.h:1213-1410);
+ *   - stream: one file opening with the "QDngPrtoBuf" intro, then per object
+ *     a length-prefixed FileSingleHeader message (FileSingleHeader.proto)
+ *     and its payload (FileSingle.h:950-1210); wave functions and grid
+ *     operators carry a length-prefixed GridSystemHeader (GridSystem.proto)
+ *     before their values (WFGridSystem.cpp:140-177, OGridsystem.cpp:63-95),
+ *     lasers their size and dt (Laser.cpp:206-235), a multistate wave
+ *     function its state count followed by one entry per state
+ *     (WFMultistate.cpp:279-293, FileWF.cpp:112-128, 262-276);
+ *   - ZLIB compression (FileSingle.h:595-670): a zlib stream of the payload,
+ *     in the raw format followed by the tag "ZLIBCOM"; compressed wave
+ *     functions hold QDng's lossy k-space reduction (WFGridSystem::Reduce /
+ *     Restore, WFGridSystem.cpp:90-137).  BZIP needs libbz2, which is absent:
+ *     such files are rejected with QDX_ERR_INCOMPAT.
+ * Reading detects the format (FileSingle::DetectFormat, FileSingle.h:
+ * 1110-1140) and the compression.  Writing follows qdx_io_set_format
+ * (QDng's global parameters wf_format, compress and comptol, FileWF.cpp:
+ * 17-56); the library's default is the binary format without compression
+ * (QDng writes streams unless wf_format = "binary"; both programs read both).
  *
  * All functions except qdx_wf_load / qdx_wf_save and the qdx_ham_* ones are
  * host-only and need no device.  Errors as in qdx.h; file errors return
@@ -23,6 +37,19 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* ---- output format ---------------------------------------------------------
+ * wf_format: QDX_FORMAT_BINARY or QDX_FORMAT_STREAM, used by qdx_wf_save,
+ * qdx_laser_save and qdx_gridop_save; compress != 0 writes wave functions
+ * ZLIB-compressed after the k-space reduction with tolerance comptol (QDng
+ * default 1e-6; comptol <= 0 keeps that).  Process-wide. */
+#define QDX_FORMAT_BINARY 0
+#define QDX_FORMAT_STREAM 1
+int qdx_io_set_format(int32_t wf_format, int32_t compress, double comptol);
+int qdx_io_get_format(int32_t *wf_format, int32_t *compress, double *comptol);
+/* the format of an existing data file: QDX_FORMAT_STREAM when it opens with
+ * the stream intro, else QDX_FORMAT_BINARY (QDX_ERR_IO if it cannot be read) */
+int qdx_file_format(const char *path, int32_t *format);
 
 /* ---- key/value meta files (ParamContainer, src/tools/ParamContainer.cpp) ---
  * Reading (Parse, ParamContainer.cpp:731-782): every line holding '=' splits
@@ -57,8 +84,9 @@ int qdx_meta_destroy(qdx_meta *m);
 int qdx_file_meta(const char *name, qdx_meta **out);
 /* Read exactly `bytes` bytes of the data file (ReadDataFromRaw,
  * FileSingle.h:1245-1287: a shorter file is an error, a longer one is read
- * from its start); a compression tag in the file's last 8 bytes
- * (DetectCompressionInPayload, FileSingle.h:769-801) gives QDX_ERR_INCOMPAT. */
+ * from its start); with a ZLIB tag in the file's last 8 bytes
+ * (DetectCompressionInPayload, FileSingle.h:769-801) the inflated data
+ * (BZIP: QDX_ERR_INCOMPAT). */
 int qdx_file_read(const char *name, const char *suffix, int32_t counter, void *dst, int64_t bytes);
 /* Write name.meta (when meta is not NULL) and the data file
  * (WriteSingleFileToRaw, FileSingle.h:1303-1360, uncompressed). */
@@ -90,15 +118,18 @@ int qdx_propagation_meta(const char *dir, int64_t nt, double dt, const char *wf_
 
 /* ---- device objects from files -------------------------------------------- */
 
-/* Wave function from files (FileWF::LoadWaveFunctionByMeta, FileWF.cpp:67-131,
- * binary format): class WFGridCartesian (one state) or WFMultistate (key
+/* Wave function from files (FileWF::LoadWaveFunctionByMeta, FileWF.cpp:67-131):
+ * binary format: class WFGridCartesian (one state) or WFMultistate (key
  * "states"; state i in name-<i>, its own meta file name-<i>.meta, all on one
- * grid).  counter as for qdx_file_read. */
+ * grid); stream format: name[_counter].wf holding one WFGridCartesian or a
+ * WFMultistate with its states; compressed data restored from the k-space
+ * reduction.  counter as for qdx_file_read. */
 int qdx_wf_load(qdx_ctx *ctx, const char *name, int32_t counter, qdx_wf **out);
-/* Write a wave function (FileWF::WriteWaveFunction, FileWF.cpp:231-283,
- * binary format): one state as name[_counter].wf + name.meta (class
- * WFGridCartesian); several as name-<i>[_counter].wf + name-<i>.meta per state
- * and name.meta (class WFMultistate, states). */
+/* Write a wave function (FileWF::WriteWaveFunction, FileWF.cpp:231-283) in
+ * the format of qdx_io_set_format.  Binary: one state as name[_counter].wf +
+ * name.meta (class WFGridCartesian); several as name-<i>[_counter].wf +
+ * name-<i>.meta per state and name.meta (class WFMultistate, states).
+ * Stream: name[_counter].wf holding everything. */
 int qdx_wf_save(qdx_wf *wf, const char *name, int32_t counter);
 
 /* Hamiltonian terms read from QDng files: a potential (qdx_ham_add_potential)

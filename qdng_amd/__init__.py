@@ -69,7 +69,10 @@ EXPORTS = [
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
     "qdx_file_read", "qdx_file_write", "qdx_gridop_load", "qdx_gridop_save", "qdx_laser_load",
     "qdx_laser_save", "qdx_propagation_meta", "qdx_wf_load", "qdx_wf_save", "qdx_ham_add_potential_file",
-    "qdx_ham_add_dipole_file",
+    "qdx_ham_add_dipole_file", "qdx_io_set_format", "qdx_io_get_format", "qdx_file_format",
+    # optimal control (include/qdx.h)
+    "qdx_ham_laser_len", "qdx_ham_laser_get", "qdx_ham_laser_set", "qdx_proj_create", "qdx_proj_apply",
+    "qdx_proj_expec", "qdx_proj_destroy", "qdx_oct_run",
 ]
 
 K_CHEBY_ROW, K_LINE_PASS, K_CHEBY_INIT, K_GSPO_ROW, K_BATCH_ROW, K_BATCH_COL, K_BATCH_LINE = range(7)
@@ -228,6 +231,9 @@ def lib():
         "qdx_gridop_save": [C.c_char_p, C.POINTER(Grid), dp],
         "qdx_laser_load": [C.c_char_p, dp, C.POINTER(C.c_int64), dp, C.c_int64],
         "qdx_laser_save": [C.c_char_p, C.c_double, C.c_int64, dp],
+        "qdx_io_set_format": [i32, i32, C.c_double],
+        "qdx_io_get_format": [C.POINTER(i32), C.POINTER(i32), dp],
+        "qdx_file_format": [C.c_char_p, C.POINTER(i32)],
         "qdx_propagation_meta": [C.c_char_p, C.c_int64, C.c_double, C.c_char_p, i32],
         "qdx_wf_load": [vp, C.c_char_p, i32, C.POINTER(vp)],
         "qdx_wf_save": [vp, C.c_char_p, i32],
@@ -355,6 +361,29 @@ def save_gridop(name: str, N, xmin, xmax, values) -> None:
     if v.size != int(np.prod(N)):
         raise ValueError("values do not match the grid")
     _chk(lib().qdx_gridop_save(_enc(name), C.byref(g), _dp(v)))
+
+
+FORMAT_BINARY, FORMAT_STREAM = 0, 1
+
+
+def set_file_format(wf_format="binary", compress: bool = False, comptol: float = 1e-6) -> None:
+    """Output format of wave functions, lasers and operators (QDng's global
+    parameters wf_format, compress, comptol; qdx_io_set_format)."""
+    fmt = {"binary": FORMAT_BINARY, "stream": FORMAT_STREAM}.get(wf_format, -1)
+    _chk(lib().qdx_io_set_format(fmt, int(bool(compress)), float(comptol)))
+
+
+def get_file_format():
+    f, c, t = C.c_int32(), C.c_int32(), C.c_double()
+    _chk(lib().qdx_io_get_format(C.byref(f), C.byref(c), C.byref(t)))
+    return ("stream" if f.value == FORMAT_STREAM else "binary"), bool(c.value), t.value
+
+
+def file_format(path: str) -> str:
+    """"stream" or "binary": the format of an existing data file (qdx_file_format)."""
+    f = C.c_int32()
+    _chk(lib().qdx_file_format(_enc(path), C.byref(f)))
+    return "stream" if f.value == FORMAT_STREAM else "binary"
 
 
 def load_laser(name: str):

@@ -27,7 +27,7 @@ def build(verbose: bool = False) -> str:
     failed = [src for src, pr in procs if pr.wait() != 0]
     if failed:
         raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
-    cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-shared", "-o", LIB + ".tmp"] + objs
+    cmd = [HIPCC] + COMMON + DEVICE_FLAGS + ["-shared", "-o", LIB + ".tmp"] + objs + ["-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -64,7 +64,7 @@ import xml.etree.ElementTree as ET
 import numpy as np
 
 from . import (Arnoldi, Cheby, GSPO, Hamiltonian, Projection, QdxError, SIL, WaveFunction, eigen_ac, eigen_imag,
-               load_laser, oct_run, save_laser, write_propagation_meta)
+               get_file_format, load_laser, oct_run, save_laser, set_file_format, write_propagation_meta)
 
 ERR_PARAM, ERR_INCOMPAT = -1, -2
 AU2WAVENUMBERS = 219475.0   # src/qdlib/Conversion.h:9
@@ -305,7 +305,7 @@ def _resolve(path, base):
 def load_wave_function(ctx, el, srcdir=None, base=None):
     a = _attrs(el, srcdir)
     name = a.get("name")
-    if name in (None, "WFGridCartesian"):
+    if name in (None, "GridCartesian", "WFGridCartesian"):   # ModuleLoader adds the "WF" prefix
         if "file" not in a:
             raise _err("<wf> needs a file")
         w = WaveFunction.load(ctx, _resolve(a["file"], base))
@@ -864,14 +864,29 @@ def programs(path):
 def run(path, ctx, srcdir=None, outdir=None, which=None, log=None):
     """Run the programs of a QDng input file (all, or the indices in `which`)
     on the device.  srcdir replaces @srcdir@ in attribute values; outdir
-    prefixes the programs' `dir`.  Returns one Result per program run."""
+    prefixes the programs' `dir`.  The root element's attributes are QDng's
+    global parameters (CmdHandler.cpp:97); of them wf_format, compress and
+    comptol (FileWF.cpp:17-56) set the output file format for the run.
+    Returns one Result per program run."""
+    root = ET.parse(path).getroot()
+    if root.tag != "qdng":
+        raise _err(f"{path}: the root element is <{root.tag}>, not <qdng>")
+    old = get_file_format()
+    g = root.attrib
+    fmt = g.get("wf_format", old[0])
+    if fmt not in ("binary", "stream"):
+        raise _err(f"Unrecognized file wave function file format: {fmt}")
     out = []
-    for k, el in enumerate(programs(path)):
-        if which is not None and k not in which:
-            continue
-        if el.tag not in PROGRAMS:
-            raise _err(f"program <{el.tag}> is not supported", ERR_INCOMPAT)
-        out.append(PROGRAMS[el.tag](ctx, el, srcdir, outdir, log))
+    try:
+        set_file_format(fmt, _bool(g.get("compress"), old[1]), float(g.get("comptol", old[2])))
+        for k, el in enumerate(list(root)):
+            if which is not None and k not in which:
+                continue
+            if el.tag not in PROGRAMS:
+                raise _err(f"program <{el.tag}> is not supported", ERR_INCOMPAT)
+            out.append(PROGRAMS[el.tag](ctx, el, srcdir, outdir, log))
+    finally:
+        set_file_format(*old)
     return out
 
 

@@ -143,13 +143,14 @@ def test_read_errors(tmp_path):
     with pytest.raises(Q.QdxError) as e:
         Q.read_file(fx("GridGauss"), ".wf", 129)
     assert e.value.code == -7
-    # compressed payloads carry a tag in their last 8 bytes: rejected
-    for tag in (b"ZLIBCOM\0", b"BZIPCOM\0"):
+    # compressed payloads carry a tag in their last 8 bytes: a ZLIB tag
+    # after data that is no zlib stream is an I/O error, BZIP is rejected
+    for tag, code in ((b"ZLIBCOM\0", -7), (b"BZIPCOM\0", -2)):
         p = tmp_path / "c.wf"
         p.write_bytes(b"\x01" * 64 + tag)
         with pytest.raises(Q.QdxError) as e:
             Q.read_file(str(tmp_path / "c"), ".wf", 4)
-        assert e.value.code == -2
+        assert e.value.code == code
     with pytest.raises(Q.QdxError) as e:
         Q.file_meta(str(tmp_path / "none_3"))
     assert e.value.code == -7

@@ -231,9 +231,15 @@ def test_device_oct_matches_oracle(ctx, method, target, ntargets, phase, system,
     assert rows_d.shape == rows_o.shape
     assert np.array_equal(np.isnan(rows_d), np.isnan(rows_o))
     m = ~np.isnan(rows_o)
-    err = np.abs(rows_d[m] - rows_o[m]) / np.maximum(np.abs(rows_o[m]), 1e-12)
-    print(f"  rows max rel {err.max():.2e}, yields {rows_o[:, 4]}", flush=True)
-    assert err.max() < 1e-8
+    # relative 1e-8; the phase / pi columns absolute 1e-8 (a phase near zero
+    # after several phase-sensitive iterations)
+    tol = 1e-8 * np.abs(rows_o) + 1e-13
+    for t in range(ntargets):
+        tol[:, 5 + 5 * t + 3] = 1e-8
+    diff = np.abs(rows_d - rows_o)
+    print(f"  rows max rel {np.max(diff[m] / (np.abs(rows_o[m]) + 1e-300)):.2e}, yields {rows_o[:, 4]}, "
+          f"phases {rows_o[:, 8]}", flush=True)
+    assert np.all(diff[m] < tol[m]), np.argwhere((diff > tol) & m)
     n = field_o.size
     assert np.max(np.abs(field_d[:n] - field_o)) < 1e-9 * max(1.0, np.max(np.abs(field_o)))
     # the optimisation did something

@@ -418,3 +418,27 @@ def test_oct_inputs(ctx, name, tmp_path):
         pop1 = vals[3]                        # momentum, position, projections 0, 1, 2, T, V
         want = o.rows[-1, 5 + 2] if "Projection" not in name else o.rows[-1, 5 + 4]
         assert abs(pop1 - want) < 0.02 * want + 1e-4, (pop1, want)
+
+
+@pytest.mark.gpu
+def test_global_file_format_parameters(ctx, tmp_path):
+    # <qdng wf_format="stream" compress="true">: the propagation's wave
+    # functions are written as compressed streams, and the format is reset
+    # after the run
+    import xml.etree.ElementTree as ET
+    tree = ET.parse(qdi_path("TGP-Expec"))
+    root = tree.getroot()
+    root.set("wf_format", "stream")
+    root.set("compress", "true")
+    pr = root.find("propa")
+    pr.set("steps", "20")
+    pr.set("wcycle", "10")
+    path = str(tmp_path / "in.qdi")
+    tree.write(path)
+    (res,) = qdi.run(path, ctx, FIX, str(tmp_path))
+    f = os.path.join(res.dir, "WF_2.wf")
+    assert Q.file_format(f) == "stream"
+    assert Q.get_file_format() == ("binary", False, 1e-6)
+    w = Q.WaveFunction.load(ctx, os.path.join(res.dir, "WF"), counter=2)
+    ref = res.psi.download()
+    assert np.max(np.abs(w.download() - ref)) < 1e-4 * np.max(np.abs(ref))
