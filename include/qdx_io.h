@@ -134,7 +134,9 @@ int qdx_wf_save(qdx_wf *wf, const char *name, int32_t counter);
 
 /* Hamiltonian terms read from QDng files: a potential (qdx_ham_add_potential)
  * from name.op with setzero / offset / scale, and a dipole
- * (qdx_ham_add_dipole) from mu_name.op and the laser file laser_name.op.
+ * (qdx_ham_add_dipole) from mu_name.op and the laser file laser_name.op; as in
+ * QDng, a GridDipole's scale multiplies both the dipole array
+ * (OGridPotential::Init) and the field (OLaser::Clock, Laser::Get).
  * The file's grid must equal the Hamiltonian's (QDX_ERR_INCOMPAT). */
 int qdx_ham_add_potential_file(qdx_ham *h, int32_t row, int32_t col, const char *name, int32_t setzero,
                                double offset, double scale);

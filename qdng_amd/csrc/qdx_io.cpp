@@ -1118,7 +1118,11 @@ int qdx_ham_add_dipole_file(qdx_ham *h, int32_t row, int32_t col, const char *mu
         if (!h) fail(QDX_ERR_PARAM, "NULL argument");
         const std::string mn = need_str(mu_name, "mu_name"), ln = need_str(laser_name, "laser_name");
         qdx_grid g;
-        std::vector<double> mu = load_gridop(mn, false, 0.0, 1.0, g);
+        // OGridDipole::Init reads the array with OGridPotential::Init, which
+        // applies `scale` to it (OGridPotential.cpp:71-75), and OLaser::Clock
+        // gives the laser the same scale (OLaser.cpp:124-130): QDng's field
+        // term is -scale^2 mu(x) E(t)
+        std::vector<double> mu = load_gridop(mn, false, 0.0, scale, g);
         check_same_grid(g, h->grid, mn);
         double dt = 0;
         std::vector<double> laser = load_laser(ln, dt);

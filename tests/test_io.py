@@ -199,7 +199,9 @@ def test_hamiltonian_terms_from_files(ctx):
     Ha.add_dipole_file(0, 0, fx("GridDip0"), fx("GridLaser_w0.176"), scale=0.5)
     Hb = Q.Hamiltonian(ctx, N, xmin, xmax).add_kinetic(0, [1836.0])
     Hb.add_potential(0, 0, ((v - v.min()) + 0.1) * 2.0)
-    Hb.add_dipole(0, 0, mu, e, 0.5)
+    # QDng scales a GridDipole's array and its field (OGridPotential::Init,
+    # OLaser::Clock): -scale^2 mu E
+    Hb.add_dipole(0, 0, mu * 0.5, e, 0.5)
     src = Q.WaveFunction(ctx, N, xmin, xmax).upload(kat["GridPot0_ef_1"])
     da, db = Q.WaveFunction(ctx, N, xmin, xmax), Q.WaveFunction(ctx, N, xmin, xmax)
     for step in (0, 37):
