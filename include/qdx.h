@@ -115,6 +115,9 @@ int qdx_wf_expec_position(qdx_wf *wf, int32_t dim, double *per_state);
 int qdx_wf_expec_momentum(qdx_wf *wf, int32_t dim, double *per_state);
 /* y += (re + i im) x (cVec MultElementsAdd; same grid and states) */
 int qdx_wf_axpy(qdx_wf *y, double re, double im, const qdx_wf *x);
+/* the grid and state count of a wave function (WaveFunction::Params,
+ * WFMultistate::States) */
+int qdx_wf_info(const qdx_wf *wf, qdx_grid *grid, int32_t *nstates);
 int qdx_wf_destroy(qdx_wf *wf);
 
 /* ---- Hamiltonian ----------------------------------------------------------

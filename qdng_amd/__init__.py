@@ -69,7 +69,7 @@ EXPORTS = [
     "qdx_meta_set", "qdx_meta_set_int", "qdx_meta_set_double", "qdx_meta_destroy", "qdx_file_meta",
     "qdx_file_read", "qdx_file_write", "qdx_gridop_load", "qdx_gridop_save", "qdx_laser_load",
     "qdx_laser_save", "qdx_propagation_meta", "qdx_wf_load", "qdx_wf_save", "qdx_ham_add_potential_file",
-    "qdx_ham_add_dipole_file", "qdx_io_set_format", "qdx_io_get_format", "qdx_file_format",
+    "qdx_ham_add_dipole_file", "qdx_wf_info", "qdx_io_set_format", "qdx_io_get_format", "qdx_file_format",
     # optimal control (include/qdx.h)
     "qdx_ham_laser_len", "qdx_ham_laser_get", "qdx_ham_laser_set", "qdx_proj_create", "qdx_proj_apply",
     "qdx_proj_expec", "qdx_proj_destroy", "qdx_oct_run",
@@ -232,6 +232,7 @@ def lib():
         "qdx_laser_load": [C.c_char_p, dp, C.POINTER(C.c_int64), dp, C.c_int64],
         "qdx_laser_save": [C.c_char_p, C.c_double, C.c_int64, dp],
         "qdx_io_set_format": [i32, i32, C.c_double],
+        "qdx_wf_info": [vp, C.POINTER(Grid), C.POINTER(i32)],
         "qdx_io_get_format": [C.POINTER(i32), C.POINTER(i32), dp],
         "qdx_file_format": [C.c_char_p, C.POINTER(i32)],
         "qdx_propagation_meta": [C.c_char_p, C.c_int64, C.c_double, C.c_char_p, i32],
@@ -554,22 +555,20 @@ class WaveFunction:
         (WFGridCartesian) or name-<i>... (WFMultistate); qdx_wf_load."""
         h = C.c_void_p()
         _chk(lib().qdx_wf_load(ctx.h, _enc(name), int(counter), C.byref(h)))
-        m = file_meta(name)
-        S = int(m["states"]) if m.get("CLASS") == "WFMultistate" else 1
-        g = file_meta(name + "-0") if S > 1 or m.get("CLASS") == "WFMultistate" else m
-        nd = int(g["dims"])
+        g = Grid()
+        S = C.c_int32()
+        _chk(lib().qdx_wf_info(h, C.byref(g), C.byref(S)))
         self = cls.__new__(cls)
         self.ctx = ctx
-        self.N = [int(g[f"N{d}"]) for d in range(nd)]
-        self.grid = make_grid(self.N, [float(g[f"xmin{d}"]) for d in range(nd)],
-                              [float(g[f"xmax{d}"]) for d in range(nd)])
-        self.S = S
+        self.N = [int(g.N[d]) for d in range(g.ndims)]
+        self.grid = g
+        self.S = S.value
         self.h = h
         self._destroy = lib().qdx_wf_destroy
         return self
 
     def save(self, name: str, counter: int = -1) -> None:
-        """Write in QDng's binary format (qdx_wf_save)."""
+        """Write in QDng's format (qdx_wf_save; binary unless set_file_format chose the stream)."""
         _chk(lib().qdx_wf_save(self.h, _enc(name), int(counter)))
 
     def __del__(self):

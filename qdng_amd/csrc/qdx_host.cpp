@@ -673,6 +673,20 @@ int qdx_wf_axpy(qdx_wf *y, double re, double im, const qdx_wf *x) {
     });
 }
 
+int qdx_wf_info(const qdx_wf *wf, qdx_grid *grid, int32_t *nstates) {
+    return guarded([&] {
+        if (!wf || !wf->ctx || !grid || !nstates) fail(QDX_ERR_PARAM, "NULL argument");
+        *grid = qdx_grid{};
+        grid->ndims = wf->grid.nd;
+        for (int d = 0; d < wf->grid.nd; ++d) {
+            grid->N[d] = wf->grid.N[d];
+            grid->xmin[d] = wf->grid.xmin[d];
+            grid->xmax[d] = wf->grid.xmax[d];
+        }
+        *nstates = wf->S;
+    });
+}
+
 int qdx_wf_destroy(qdx_wf *wf) {
     return guarded([&] {
         if (!wf) return;

@@ -122,8 +122,9 @@ class Multi:
 class Loader:
     """ChainLoader: operators from XML elements with key/ref bookkeeping."""
 
-    def __init__(self, srcdir=None):
+    def __init__(self, srcdir=None, base=None):
         self.srcdir = srcdir
+        self.base = base      # the output folder: relative data files are looked up there first
         self.keys = {}
 
     def op(self, el):
@@ -157,7 +158,8 @@ class Loader:
         elif name == "GridDipole":
             if "laser" not in a:
                 raise _err("GridDipole needs a laser file")
-            o = Term("dip", name, file=self._file(a, name), laser=a["laser"], scale=float(a.get("scale", 1.0)))
+            o = Term("dip", name, file=self._file(a, name), laser=_resolve(a["laser"], self.base),
+                     scale=float(a.get("scale", 1.0)))
         elif name == "Gobbler":
             order = int(a.get("order", 20))
             if order < 1:
@@ -184,11 +186,10 @@ class Loader:
             self.keys[a["key"]] = o
         return o
 
-    @staticmethod
-    def _file(a, name, attr="file"):
+    def _file(self, a, name, attr="file"):
         if attr not in a:
             raise _err(f"{name} needs a '{attr}' attribute")
-        return a[attr]
+        return _resolve(a[attr], self.base)
 
     def propagator(self, el):
         a = _attrs(el, self.srcdir)
@@ -296,7 +297,7 @@ def _resolve(path, base):
     if not base or os.path.isabs(path):
         return path
     cand = os.path.join(base, path)
-    metas = [cand + ".meta", cand + ".wf"]
+    metas = [cand + ".meta", cand + ".wf", cand + ".op"]
     if "_" in os.path.basename(cand):
         metas.append(cand.rsplit("_", 1)[0] + ".meta")
     return cand if any(os.path.exists(m) for m in metas) else path
@@ -564,7 +565,7 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
         raise _err("propagation restart (start != 0) is not supported", ERR_INCOMPAT)
     d = _outdir(a, outdir)
     fname = a.get("fname", "WF")
-    ld = Loader(srcdir)
+    ld = Loader(srcdir, outdir)
     props = [c for c in el if c.tag == "propagator"]
     wfs = [c for c in el if c.tag == "wf"]
     if len(props) != 1 or len(wfs) != 1:
@@ -643,7 +644,7 @@ def run_eigen(ctx, el, srcdir=None, outdir=None, log=None):
     conv = float(a["conv"]) if "conv" in a else None
     max_steps = int(a.get("steps", 512))
     d = _outdir(a, outdir)
-    ld = Loader(srcdir)
+    ld = Loader(srcdir, outdir)
     props = [c for c in el if c.tag == "propagator"]
     wfs = [c for c in el if c.tag == "wf"]
     if len(props) != 1 or len(wfs) != 1:
@@ -773,7 +774,7 @@ def run_oct(ctx, el, srcdir=None, outdir=None, log=None):
     nt = int(a.get("ntargets", 1))
     phase = _bool(a.get("phase"))
     d = _outdir(a, outdir)
-    ld = Loader(srcdir)
+    ld = Loader(srcdir, outdir)
     props = [c for c in el if c.tag == "propagator"]
     if len(props) != 1:
         raise _err("No propagator found")

@@ -198,10 +198,10 @@ def _restore_reduce(psi, shape_np, tol):
 @pytest.mark.parametrize("S", [1, 2])
 def test_wave_function_formats(ctx, tmp_path, fmt, compress, S):
     print(f"[gpu case] wf file {fmt} compress={compress} S={S}", flush=True)
-    N, lo, hi = [24, 10], [-3.0, -1.0], [3.0, 2.0]
+    N, lo, hi = [24, 16], [-3.0, -1.0], [3.0, 2.0]
     rng = np.random.default_rng(5)
     x = np.linspace(-3, 3, 24)[None, :]
-    psi = np.exp(-x ** 2) * (1 + 0.3 * rng.standard_normal((10, 24))) + 0j
+    psi = np.exp(-x ** 2) * (1 + 0.3 * rng.standard_normal((16, 24))) + 0j
     data = np.concatenate([psi.ravel() * (k + 1) for k in range(S)])
     w = Q.WaveFunction(ctx, N, lo, hi, S)
     w.upload(data)
@@ -215,7 +215,7 @@ def test_wave_function_formats(ctx, tmp_path, fmt, compress, S):
     assert Q.file_format(path) == fmt
     got = Q.WaveFunction.load(ctx, str(tmp_path / "WF"), counter=4).download()
     if compress:
-        want = np.concatenate([_restore_reduce(data[k * 240:(k + 1) * 240], (10, 24), 1e-3) for k in range(S)])
+        want = np.concatenate([_restore_reduce(data[k * 384:(k + 1) * 384], (16, 24), 1e-3) for k in range(S)])
         assert np.max(np.abs(got - want)) < 1e-13 * np.max(np.abs(want))
         assert 1e-6 < np.max(np.abs(got - data)) / np.max(np.abs(data)) < 1e-2   # the reduction is lossy
     else:
@@ -231,7 +231,7 @@ def test_wave_function_formats(ctx, tmp_path, fmt, compress, S):
         for k, (h, hb, pl) in enumerate(es[-S:]):
             assert getattr(h, "class") == "WFGridCartesian" and hb == h.SerializeToString()
             assert h.more_files_follow == (k < S - 1) and h.compression == int(compress)
-            assert h.meta_data == "N0 = 24\nN1 = 10\ndims = 2\nxmax0 = 3\nxmax1 = 2\nxmin0 = -3\nxmin1 = -1\n"
+            assert h.meta_data == "N0 = 24\nN1 = 16\ndims = 2\nxmax0 = 3\nxmax1 = 2\nxmin0 = -3\nxmin1 = -1\n"
 
 
 @pytest.mark.gpu
