@@ -236,9 +236,14 @@ __global__ void __launch_bounds__(NT) k_cheby_row(GridDev g, HamDev<R> h, LinePl
 // |psi|^2 per order and the break test on it, OCheby.cpp:92-197), with psi
 // and the recurrence terms in registers and only the FFT stages in LDS: one
 // launch per step instead of one per order.  coef[2k], coef[2k+1]: coeff[k]
-// exp(-i offset dt).  partials: the per-order slots of the multi-kernel path
-// ([k % 2] * kMaxPartials, step start at 2 * kMaxPartials), one sum each.
-template <class R, int NT, bool NIP = false>
+// exp(-i offset dt), staged into LDS (after the line buffers and the twiddle
+// table) before the first order.  Nothing is written to global memory inside
+// the order loop (a global store before a barrier makes the barrier wait for
+// it): the last order, its sum of |psi|^2 (slot [k % 2] * kMaxPartials of the
+// multi-kernel layout) and the break are recorded after the loop; the step
+// start's sum goes to slot 2 * kMaxPartials.  A block of one wave reduces
+// with shuffles alone.
+template <class R, int NT, bool NIP = false, int P = 8>
 __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LinePlan p0, const double *coef,
                                                    int order, double offset, double offset_i, double inv_scaling,
                                                    double prec, cx<double> *__restrict__ psi, ChebyCtl *ctl,
@@ -246,7 +251,9 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
     extern __shared__ __align__(16) char smem_raw[];
     __shared__ double red[16];
     __shared__ double bc;
-    RowBlock<R> B(g, 1, 1, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    RowBlock<R, P> B(g, 1, 1, reinterpret_cast<cx<R> *>(smem_raw), p0);
+    double *cs = reinterpret_cast<double *>(B.after);   // 2 * order coefficients
+    const bool one_wave = blockDim.x <= 64;
     const double *tk0 = h.tk[0][0];
     const bool has_t = tk0 != nullptr;
     const R *Wp = h.W[0][0];
@@ -254,13 +261,14 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
     const R *Wip = NIP ? h.Wi[0] : nullptr;
     const double Ed = h.Edip[0][0];
     const double uni = h.unity[0];
-    cx<double> ps[8];
-    cx<R> pm1[8], pm2[8], v[8];
-    double wv[8], tq[8], wi[8];
+    for (int i = threadIdx.x; i < 2 * order; i += blockDim.x) cs[i] = coef[i];
+    cx<double> ps[P];
+    cx<R> pm1[P], pm2[P], v[P];
+    double wv[P], tq[P], wi[P];
     double acc = 0;
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < P; ++q) {
             const long gi = B.gidx(q);
             ps[q] = psi[gi];
             acc += ps[q].r * ps[q].r + ps[q].i * ps[q].i;
@@ -280,31 +288,29 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
         bc = norm0;
         ctl->norm0 = norm0;
         ctl->done = 0;
-        ctl->last_order = 1;
     }
-    __syncthreads();
+    __syncthreads();   // also publishes the coefficients
     norm0 = bc;
     B.stage(p0);
-    const cx<double> c0 = mk<double>(coef[0], coef[1]);
+    const cx<double> c0 = mk<double>(cs[0], cs[1]);
+    int last = 1;           // the last order whose terms are in psi
+    double a_last = 0;      // its sum of |psi|^2 (orders >= 2)
+    bool broke = false;
     for (int k = 1; k < order; ++k) {
         if (k >= 3 && prec > 0) {
-            // break test of order k - 1 on its sum (stored in bc below)
-            const double a = bc;
-            if (fabs(norm0 - a) / norm0 < prec) {   // uniform over the block
-                if (threadIdx.x == 0) {
-                    ctl->done = 1;
-                    ctl->accnorm = a;
-                }
+            // break test of order k - 1 on its sum (uniform over the block)
+            if (fabs(norm0 - a_last) / norm0 < prec) {
+                broke = true;
                 break;   // psi holds the series through order k - 1
             }
         }
-        if (threadIdx.x == 0) ctl->last_order = k;
+        last = k;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = pm1[q];
+        for (int q = 0; q < P; ++q) v[q] = pm1[q];
         B.fft(v, p0, -1);
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
+            for (int q = 0; q < P; ++q) v[q] = mk<R>((R)((double)v[q].r * tq[q]), (R)((double)v[q].i * tq[q]));
         }
         B.fft(v, p0, +1);
         // psi update at this order: every order with the break test, the
@@ -314,13 +320,13 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
             const int m = (k - 1) % 3;
             nterm = m == 0 ? 3 : (k == order - 1 ? m : 0);
         }
-        const cx<double> ck = mk<double>(coef[2 * k], coef[2 * k + 1]);
-        const cx<double> cm1 = k >= 2 ? mk<double>(coef[2 * (k - 1)], coef[2 * (k - 1) + 1]) : mk<double>(0, 0);
-        const cx<double> cm2 = k >= 3 ? mk<double>(coef[2 * (k - 2)], coef[2 * (k - 2) + 1]) : mk<double>(0, 0);
+        const cx<double> ck = mk<double>(cs[2 * k], cs[2 * k + 1]);
+        const cx<double> cm1 = k >= 2 ? mk<double>(cs[2 * (k - 1)], cs[2 * (k - 1) + 1]) : mk<double>(0, 0);
+        const cx<double> cm2 = k >= 3 ? mk<double>(cs[2 * (k - 2)], cs[2 * (k - 2) + 1]) : mk<double>(0, 0);
         acc = 0;
         if (B.active) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < P; ++q) {
                 cx<double> hp = mk<double>(0, 0);
                 if (has_t) hp = to_d(v[q]);
                 const cx<double> p = to_d(pm1[q]);
@@ -352,17 +358,27 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
             }
         }
         if (k >= 2) {
-            const double a = block_sum(acc, red);
-            if (threadIdx.x == 0) {
-                partials[(k % 2) * kMaxPartials] = a;
-                bc = a;
+            if (one_wave) {
+                a_last = __shfl(wave_sum(acc), 0, 64);   // the block sum, in every lane
+            } else {
+                const double a = block_sum(acc, red);
+                if (threadIdx.x == 0) bc = a;
+                __syncthreads();
+                a_last = bc;
             }
-            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        ctl->last_order = last;
+        if (last >= 2) partials[(last % 2) * kMaxPartials] = a_last;
+        if (broke) {
+            ctl->done = 1;
+            ctl->accnorm = a_last;
         }
     }
     if (B.active) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) psi[B.gidx(q)] = ps[q];
+        for (int q = 0; q < P; ++q) psi[B.gidx(q)] = ps[q];
     }
 }
 
@@ -523,32 +539,37 @@ template <class R>
 void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, const LinePlan &p0,
                            const double *coef, int order, double offset, double offset_i, double inv_scaling,
                            double prec, double *psi, ChebyCtl *ctl, double *partials) {
-    const int L = g.N[0], TL = L / 8;
+    const int L = g.N[0], P = p0.ppt == 4 ? 4 : 8, TL = L / P;
     if (g.nd != 1 || h.S != 1 || L > kMaxLine) throw std::runtime_error("whole-step line kernel: one 1D line");
     LinePlan p = p0;
-    size_t lds = row_lds_bytes<R>(L, 1, true);
+    const size_t lcoef = (size_t)2 * order * sizeof(double);   // coefficients staged in LDS
+    size_t lds = row_lds_bytes<R>(L, 1, true) + lcoef;
     p.tw_lds = lds <= kLdsBytes ? 1 : 0;
-    if (!p.tw_lds) lds = row_lds_bytes<R>(L, 1, false);
+    if (!p.tw_lds) lds = row_lds_bytes<R>(L, 1, false) + lcoef;
     const int threads = std::max(64, round64(TL));
     auto cpsi = reinterpret_cast<cx<double> *>(psi);
     const bool nip = offset_i != 0 || h.Wi[0] != nullptr;   // negative imaginary potential
-    if (nip && threads <= 256) {
-        set_lds(k_cheby_line<R, 256, true>, lds);
-        QDX_LAUNCH((k_cheby_line<R, 256, true>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset,
-                   offset_i, inv_scaling, prec, cpsi, ctl, partials);
+    const double oi = nip ? offset_i : 0.0;
+#define QDX_LINE(NT_, NIP_, P_)                                                                              \
+    do {                                                                                                     \
+        set_lds(k_cheby_line<R, NT_, NIP_, P_>, lds);                                                        \
+        QDX_LAUNCH((k_cheby_line<R, NT_, NIP_, P_>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order,   \
+                   offset, oi, inv_scaling, prec, cpsi, ctl, partials);                                      \
+    } while (0)
+    if (P == 4) {   // short lines: 4 points per thread, at most one 256-thread block
+        if (threads > 256) throw std::runtime_error("whole-step line kernel: 4 points per thread up to 1024 points");
+        if (nip) QDX_LINE(256, true, 4);
+        else QDX_LINE(256, false, 4);
+    } else if (nip && threads <= 256) {
+        QDX_LINE(256, true, 8);
     } else if (nip) {
-        set_lds(k_cheby_line<R, 512, true>, lds);
-        QDX_LAUNCH((k_cheby_line<R, 512, true>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset,
-                   offset_i, inv_scaling, prec, cpsi, ctl, partials);
+        QDX_LINE(512, true, 8);
     } else if (threads <= 256) {
-        set_lds(k_cheby_line<R, 256>, lds);
-        QDX_LAUNCH((k_cheby_line<R, 256>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, 0.0,
-                   inv_scaling, prec, cpsi, ctl, partials);
+        QDX_LINE(256, false, 8);
     } else {
-        set_lds(k_cheby_line<R, 512>, lds);
-        QDX_LAUNCH((k_cheby_line<R, 512>), dim3(1), dim3(threads), lds, s, g, h, p, coef, order, offset, 0.0,
-                   inv_scaling, prec, cpsi, ctl, partials);
+        QDX_LINE(512, false, 8);
     }
+#undef QDX_LINE
     QDX_HIP(hipGetLastError());
 }
 

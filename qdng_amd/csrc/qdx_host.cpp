@@ -62,7 +62,14 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     double *init = part + 2 * kMaxPartials;
     if (p->line_step) {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        Launch<R>::cheby_line(s, g, h, plans[0], p->coefd.as<double>(), p->k_launch, p->cp.offset, p->cp.offset_i,
+        // the whole-step kernel of a single-state line: 4 points per thread
+        // below 512 points (a short line then spreads over more lanes; 256
+        // points fill one wave), 8 above; qdx_ctx_set_line_points 4 or 8
+        // forces the choice (4 up to 1024 points)
+        const int N0 = p->grid.N[0];
+        const int lp = ctx->line_ppt == 8 ? 8 : (ctx->line_ppt == 4 ? (N0 <= 1024 ? 4 : 8) : (N0 < 512 ? 4 : 8));
+        const LinePlan lplan = lp == 8 ? plans[0] : ctx->plan_ppt(N0, fp32, 4);
+        Launch<R>::cheby_line(s, g, h, lplan, p->coefd.as<double>(), p->k_launch, p->cp.offset, p->cp.offset_i,
                               1.0 / p->cp.scaling, p->cp.prec, psi->ptr(), ctl, p->partials.as<double>());
         p->last_nb = 1;
         p->nb_init = 1;
@@ -1012,8 +1019,14 @@ static void cheby_create_impl(qdx_ham *h, double dt, int32_t order, double prec,
         // (FP64 lines above 2048 points need more registers than a wave of a
         // 512-thread block holds; they keep the per-order kernels, and so
         // does imaginary time)
+        // the whole-step kernel keeps its line buffers, the twiddle table
+        // and the 2 * k_launch coefficients in LDS (160 KiB per CU)
+        const size_t es_line = precision == QDX_FP32_BULK ? 8 : 16;
+        const size_t L0 = (size_t)p->grid.N[0];
+        const size_t line_lds = (2 * (L0 + L0 / 8) + (L0 + L0 / 8)) * es_line + 16 * (size_t)p->k_launch;
         p->line_step = !imag && p->grid.nd == 1 && p->S == 1 &&
-                       p->grid.N[0] <= (precision == QDX_FP32_BULK ? kMaxLine : 2048);
+                       p->grid.N[0] <= (precision == QDX_FP32_BULK ? kMaxLine : 2048) &&
+                       line_lds <= 160 * 1024;
         if (p->line_step) {
             // the same products as the per-order scalars of cheby_step
             const std::complex<double> ph =

@@ -149,17 +149,25 @@ def test_batched_2026_mix_three_ticks(ctx):
         note(f"  job {i} N={p.N} vs oracle {e:.2e}")
         assert nhb[i] == nh
         assert e < 1e-10
-    # every job against the single-job path
-    worst = 0.0
+    # every job against the single-job path.  Both sum |psi|^2 for QDng's
+    # break test in their own order, so a test that lands within rounding of
+    # prec can end one series an order later than the other: such a job then
+    # differs by about that term (~prec), within the 1e-10 bar; jobs with the
+    # same series lengths agree to rounding
+    worst, nlen = 0.0, 0
     for i, p in enumerate(jobs):
         U = Q.Cheby(p.gpu_ham(ctx), 1.0, prec=1e-10)
         w = p.gpu_wf(ctx)
-        for s in range(ticks):
-            U.apply(w, s)
+        nh = sum(U.apply(w, s).n_hpsi for s in range(ticks))
         e = rel(got[i], w.download())
-        worst = max(worst, e)
-        assert e < 1e-12, (i, p.N, e)
-    note(f"  512 jobs vs single path: worst rel L2 {worst:.2e}")
+        if nh == nhb[i]:
+            worst = max(worst, e)
+            assert e < 1e-12, (i, p.N, e)
+        else:
+            nlen += 1
+            assert abs(nh - nhb[i]) == 1 and e < 1e-10, (i, p.N, nh, nhb[i], e)
+    assert nlen <= 8
+    note(f"  512 jobs vs single path: worst rel L2 {worst:.2e} ({nlen} series one order apart)")
 
 
 def serving_stream(n_jobs, lam=2000.0, seed=7, scale=50):
