@@ -402,7 +402,12 @@ typedef struct qdx_job_desc {
     int32_t nsteps;
     double dt;
     int32_t order;         /* Chebyshev: 0 = OCheby's automatic order          */
-    int32_t reserved;
+    int32_t first_step;    /* clock step of the job's first step (its laser
+                              sample index); 0 for a new job.  A snapshot
+                              (qdx_batch_fetch after k steps) resubmitted with
+                              first_step = k and the remaining steps continues
+                              the propagation bit for bit (ProgPropa's restart
+                              from a written wave function, `start`)       */
     double prec;           /* Chebyshev: < 0 = OCheby's default 1e-10          */
     const double *psi0;    /* complex128 initial state [S][N]                  */
 } qdx_job_desc;
@@ -465,8 +470,11 @@ int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *
  * lines fit (states * N0 / 4 <= 512 threads, N_d <= 2048).  At 4 a line
  * uses the context's plan for its length when that plan's first and last
  * radix are 4 or 2, else first and last radix 4.  Both give the same
- * transform up to rounding.  No QDng counterpart: FFTW lays out its own
- * threads (src/fft/fft.cpp:95-110). */
+ * transform up to rounding.  Single-state 1D lines run a whole step in one
+ * block: by default with one point per thread up to 1024 points (each
+ * thread forms one output of every transform stage), else 8 per thread;
+ * 1, 4 or 8 force that kernel's layout (1 and 4 up to 1024 points).  No
+ * QDng counterpart: FFTW lays out its own threads (src/fft/fft.cpp:95-110). */
 int qdx_ctx_set_line_points(qdx_ctx *ctx, int32_t points);
 int qdx_ctx_get_line_points(qdx_ctx *ctx, int32_t *points);
 

@@ -90,7 +90,7 @@ OCT_ROW_FIXED, OCT_ROW_PER_TARGET = 5, 5
 
 class JobDesc(C.Structure):
     _fields_ = [("ham", C.c_void_p), ("ops", C.POINTER(C.c_void_p)), ("nops", C.c_int32), ("nsteps", C.c_int32),
-                ("dt", C.c_double), ("order", C.c_int32), ("reserved", C.c_int32), ("prec", C.c_double),
+                ("dt", C.c_double), ("order", C.c_int32), ("first_step", C.c_int32), ("prec", C.c_double),
                 ("psi0", C.POINTER(C.c_double))]
 
 
@@ -903,9 +903,10 @@ class Batch:
         self.sizes = {}
 
     @staticmethod
-    def job(H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None):
+    def job(H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None, first_step=0):
         """A job description: Chebyshev with Hamiltonian H (one state), or
-        GSPO with split operators ops = [T, W]."""
+        GSPO with split operators ops = [T, W]; first_step: the clock step
+        it starts at (a restart from a snapshot)."""
         psi = np.ascontiguousarray(psi0, dtype=np.complex128).ravel()
         d = JobDesc()
         keep = [psi]
@@ -923,6 +924,7 @@ class Batch:
         d.dt = float(dt)
         d.nsteps = int(nsteps)
         d.order = int(order)
+        d.first_step = int(first_step)
         d.prec = float(prec)
         d.psi0 = psi.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))
         return d, keep
@@ -933,8 +935,8 @@ class Batch:
         _chk(lib().qdx_batch_footprint(self.h, C.byref(d), C.byref(out)))
         return out.value
 
-    def submit(self, H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None) -> int:
-        d, keep = self.job(H, dt, nsteps, psi0, order, prec, ops)
+    def submit(self, H=None, dt=1.0, nsteps=1, psi0=None, order=0, prec=-1.0, ops=None, first_step=0) -> int:
+        d, keep = self.job(H, dt, nsteps, psi0, order, prec, ops, first_step)
         jid = C.c_int64()
         _chk(lib().qdx_batch_submit(self.h, C.byref(d), C.byref(jid)))
         self.sizes[jid.value] = keep[0].size

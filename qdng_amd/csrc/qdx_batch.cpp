@@ -137,7 +137,7 @@ void add_field(std::vector<double> &dst, const std::vector<double> &v) {
 }
 
 // Chebyshev job: H = OSum of grid terms on one state
-JobHost cheby_host(qdx_ham *h, int nsteps) {
+JobHost cheby_host(qdx_ham *h, int nsteps, int first) {
     JobHost j;
     const std::vector<double> *las = nullptr;
     for (const Term &t : h->terms) {
@@ -155,15 +155,15 @@ JobHost cheby_host(qdx_ham *h, int nsteps) {
             add_kinetic(j, h->grid, t);
         }
     }
-    if (las) {
-        if ((long)las->size() < nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
-        j.laser = *las;
+    if (las) {   // the samples from the job's first clock step on
+        if ((long)las->size() < (long)first + nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
+        j.laser.assign(las->begin() + first, las->end());
     }
     return j;
 }
 
 // GSPO job: operators {T, W} (OGSPO with A = kinetic, B = potential)
-JobHost gspo_host(qdx_ham *T, qdx_ham *W, int nsteps) {
+JobHost gspo_host(qdx_ham *T, qdx_ham *W, int nsteps, int first) {
     JobHost j;
     const int S = T->S;
     for (const Term &t : T->terms) {
@@ -192,9 +192,9 @@ JobHost gspo_host(qdx_ham *T, qdx_ham *W, int nsteps) {
         }
     }
     if (c01 && c10) fail(QDX_ERR_INCOMPAT, "batched GSPO jobs take one coupling element (mirrored)");
-    if (las) {
-        if ((long)las->size() < nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
-        j.laser = *las;
+    if (las) {   // the samples from the job's first clock step on
+        if ((long)las->size() < (long)first + nsteps) fail(QDX_ERR_PARAM, "Clock values doesn't fit the laser field");
+        j.laser.assign(las->begin() + first, las->end());
     }
     return j;
 }
@@ -664,6 +664,7 @@ void check_grid(const Grid &g) {
 BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
     if (!d || !d->psi0) fail(QDX_ERR_PARAM, "NULL argument");
     if (d->nsteps < 1) fail(QDX_ERR_PARAM, "nsteps must be >= 1");
+    if (d->first_step < 0) fail(QDX_ERR_PARAM, "first_step must be >= 0");
     if (d->dt == 0) fail(QDX_ERR_PARAM, "No time step defined");
     BatchJob J;
     J.nsteps = d->nsteps;
@@ -676,7 +677,7 @@ BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
         J.S = 1;
         J.grid = h->grid;
         check_grid(J.grid);
-        host = cheby_host(h, d->nsteps);
+        host = cheby_host(h, d->nsteps, d->first_step);
         J.cp = cheby_params(h, d->dt, d->order, d->prec, 0.0);
         // fixed-length series for FP32 terms and for the compact terms of 2D
         // jobs (1D jobs of a compact batch keep their terms in registers and
@@ -693,7 +694,7 @@ BatchJob plan_job(qdx_batch *b, const qdx_job_desc *d, JobHost &host) {
         J.S = T->S;
         J.grid = T->grid;
         check_grid(J.grid);
-        host = gspo_host(T, W, d->nsteps);
+        host = gspo_host(T, W, d->nsteps, d->first_step);
     }
     const bool fp32 = b->precision == QDX_FP32_BULK;
     J.rad0 = b->ctx->radix_of(J.grid.N[0], fp32);

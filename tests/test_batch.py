@@ -398,3 +398,36 @@ def test_failed_job_is_isolated(ctx, precision):
                                                                         psi0=p.psi0)]
     S.serve(reqs, np.array([0.0, 0.0]))
     assert list(S.last_failed) == [0, -1]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "compact"])
+def test_restart_from_snapshot_is_bit_identical(ctx, precision):
+    """A job fetched after k steps and resubmitted with first_step = k (its
+    laser samples from step k on) ends bit for bit where the uninterrupted
+    job ends; also for a GSPO job."""
+    import qdng_amd as Q
+    print(f"[gpu case] restart from snapshot {precision}", flush=True)
+    steps, k = 10, 4
+    jobs = mixed_jobs(steps, seed=29)[:4]
+    g = gspo_jobs(steps, seed=31)[:2] if precision != "compact" else []
+    A = Q.Batch(ctx, precision, 1 << 30)
+    ids = [A.submit(p.gpu_ham(None), 1.0, steps, p.psi0) for p in jobs]
+    ids += [A.submit(dt=1.0, nsteps=steps, psi0=p.psi0, ops=split_ops(p, None)) for p in g]
+    A.run(steps)
+    straight = [A.fetch(j) for j in ids]
+    B = Q.Batch(ctx, precision, 1 << 30)
+    first = [B.submit(p.gpu_ham(None), 1.0, k, p.psi0) for p in jobs]
+    first += [B.submit(dt=1.0, nsteps=k, psi0=p.psi0, ops=split_ops(p, None)) for p in g]
+    B.run(k)
+    snaps = [B.fetch(j) for j in first]
+    for j in first:
+        B.release(j)
+    second = [B.submit(p.gpu_ham(None), 1.0, steps - k, s, first_step=k) for p, s in zip(jobs, snaps)]
+    second += [B.submit(dt=1.0, nsteps=steps - k, psi0=s, ops=split_ops(p, None), first_step=k)
+                for p, s in zip(g, snaps[len(jobs):])]
+    B.run(steps - k)
+    for a, j in zip(straight, second):
+        assert np.array_equal(B.fetch(j), a)
+    with pytest.raises(Q.QdxError):
+        B.submit(jobs[0].gpu_ham(None), 1.0, steps, jobs[0].psi0, first_step=steps)   # beyond the laser
