@@ -471,9 +471,8 @@ int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *
  * uses the context's plan for its length when that plan's first and last
  * radix are 4 or 2, else first and last radix 4.  Both give the same
  * transform up to rounding.  Single-state 1D lines run a whole step in one
- * block: by default with one point per thread up to 1024 points (each
- * thread forms one output of every transform stage), else 8 per thread;
- * 1, 4 or 8 force that kernel's layout (1 and 4 up to 1024 points).  No
+ * block, with 4 points per thread up to 1024 points and 8 above, unless 8
+ * is set.  No
  * QDng counterpart: FFTW lays out its own threads (src/fft/fft.cpp:95-110). */
 int qdx_ctx_set_line_points(qdx_ctx *ctx, int32_t points);
 int qdx_ctx_get_line_points(qdx_ctx *ctx, int32_t *points);

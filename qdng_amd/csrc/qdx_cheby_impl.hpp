@@ -382,155 +382,6 @@ __global__ void __launch_bounds__(NT) k_cheby_line(GridDev g, HamDev<R> h, LineP
     }
 }
 
-// ---------------------------------------------------------------------------
-// The whole step of a short single-state 1D line (L <= 1024) with one grid
-// point per thread (L threads, up to 16 waves on all four SIMDs of the CU):
-// the same order-by-order arithmetic as k_cheby_line, with the line
-// transforms as Stockham stages in which every thread forms one output as a
-// direct r-point sum, y_o = sum_u in[j + u m] W_L^(u (k L/(Ns r) + t m)) for
-// o = q Ns r + t Ns + k, j = q Ns + k, m = L / r (the outputs of
-// mid_stage's butterflies, one per thread; any radix of the plan).  Two LDS
-// buffers alternate between stages, one barrier per stage; the block sum of
-// |psi|^2 is one barrier as well (per-wave sums in a slot that alternates
-// between orders).
-template <class R>
-__device__ __forceinline__ cx<R> pt_fft(cx<R> v, cx<R> *a, cx<R> *b, const cx<R> *tw, const LinePlan &p, int o,
-                                        int sign) {
-    const int L = p.L;
-    a[o] = v;
-    __syncthreads();
-    cx<R> *in = a, *out = b;
-    int Ns = 1;
-    for (int st = 0; st < p.nst; ++st) {
-        const int r = p.rad[st];
-        const int m = L / r;
-        const int k = o % Ns;
-        const int t = (o / Ns) % r;
-        const int j = (o / (Ns * r)) * Ns + k;
-        const int e = k * (L / (Ns * r)) + t * m;   // exponent step of u
-        cx<double> acc = to_d(in[j]);
-        int idx = 0;
-        for (int u = 1; u < r; ++u) {
-            idx += e;
-            if (idx >= L) idx -= L;
-            cx<R> w = tw[idx];
-            if (sign > 0) w.i = -w.i;
-            acc = acc + to_d(cmul(in[j + u * m], w));
-        }
-        out[o] = from_d<R>(acc);
-        __syncthreads();
-        cx<R> *tmp = in;
-        in = out;
-        out = tmp;
-        Ns *= r;
-    }
-    return in[o];
-}
-
-template <class R, bool NIP = false>
-__global__ void __launch_bounds__(1024) k_cheby_line_pt(GridDev g, HamDev<R> h, LinePlan p0, const double *coef,
-                                                        int order, double offset, double offset_i, double inv_scaling,
-                                                        double prec, cx<double> *__restrict__ psi, ChebyCtl *ctl,
-                                                        double *__restrict__ partials) {
-    extern __shared__ __align__(16) char smem_raw[];
-    __shared__ double red[2][16];
-    const int L = g.N[0];
-    const int o = threadIdx.x;   // this thread's grid point
-    cx<R> *ba = reinterpret_cast<cx<R> *>(smem_raw);
-    cx<R> *bb = ba + L;
-    cx<R> *tw = bb + L;
-    double *cs = reinterpret_cast<double *>(tw + L);   // 2 * order coefficients
-    const int nw = (blockDim.x + 63) >> 6, wv_id = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = reinterpret_cast<const cx<R> *>(p0.tw)[i];
-    for (int i = threadIdx.x; i < 2 * order; i += blockDim.x) cs[i] = coef[i];
-    const double *tk0 = h.tk[0][0];
-    const bool has_t = tk0 != nullptr;
-    const R *Wp = h.W[0][0];
-    const R *Mp = h.mu[0][0];
-    const R *Wip = NIP ? h.Wi[0] : nullptr;
-    cx<double> ps = psi[o];
-    double w = h.unity[0];
-    if (Wp) w += (double)Wp[o];
-    if (Mp) w += h.Edip[0][0] * (double)Mp[o];
-    const double wi = NIP ? (offset_i + (Wip ? (double)Wip[o] : 0.0)) * inv_scaling : 0.0;
-    const double tq = has_t ? (0.0 + tk0[o]) * h.inv_nd[0] : 0.0;
-    cx<R> pm1 = from_d<R>(ps), pm2 = mk<R>(0, 0);
-    // block sum in a fixed order: wave sums, then the waves in order
-    auto bsum = [&](double x, int slot) {
-        x = wave_sum(x);
-        if (lane == 0) red[slot][wv_id] = x;
-        __syncthreads();
-        double t = 0;
-        for (int i = 0; i < nw; ++i) t += red[slot][i];
-        return t;
-    };
-    const double norm0 = bsum(ps.r * ps.r + ps.i * ps.i, 0);
-    if (threadIdx.x == 0) {
-        partials[2 * kMaxPartials] = norm0;
-        ctl->norm0 = norm0;
-        ctl->done = 0;
-    }
-    const cx<double> c0 = mk<double>(cs[0], cs[1]);
-    int last = 1;
-    double a_last = 0;
-    bool broke = false;
-    for (int k = 1; k < order; ++k) {
-        if (k >= 3 && prec > 0 && fabs(norm0 - a_last) / norm0 < prec) {   // order k - 1 (OCheby.cpp:179-182)
-            broke = true;
-            break;
-        }
-        last = k;
-        cx<R> v = pt_fft(pm1, ba, bb, tw, p0, o, -1);
-        v = mk<R>((R)((double)v.r * tq), (R)((double)v.i * tq));
-        v = pt_fft(v, ba, bb, tw, p0, o, +1);
-        int nterm = 1;
-        if (k >= 2 && !(prec > 0)) {
-            const int mm3 = (k - 1) % 3;
-            nterm = mm3 == 0 ? 3 : (k == order - 1 ? mm3 : 0);
-        }
-        const cx<double> ck = mk<double>(cs[2 * k], cs[2 * k + 1]);
-        cx<double> hp = mk<double>(0, 0);
-        if (has_t) hp = to_d(v);
-        const cx<double> p = to_d(pm1);
-        hp.r += w * p.r;
-        hp.i += w * p.i;
-        cx<double> bq = mk<double>((hp.r - offset * p.r) * inv_scaling, (hp.i - offset * p.i) * inv_scaling);
-        if (NIP) {
-            bq.r += wi * p.i;
-            bq.i -= wi * p.r;
-        }
-        cx<double> pk;
-        double acc = 0;
-        if (k == 1) {
-            pk = mk<double>(bq.i, -bq.r);
-            ps = cmul(c0, ps) + cmul(ck, pk);
-        } else {
-            const cx<double> mm = to_d(pm2);
-            pk = mk<double>(2.0 * bq.i + mm.r, -2.0 * bq.r + mm.i);
-            if (nterm > 0) {
-                cx<double> sv = ps;
-                if (nterm >= 3) sv = sv + cmul(mk<double>(cs[2 * (k - 2)], cs[2 * (k - 2) + 1]), mm);
-                if (nterm >= 2) sv = sv + cmul(mk<double>(cs[2 * (k - 1)], cs[2 * (k - 1) + 1]), p);
-                sv = sv + cmul(ck, pk);
-                acc = sv.r * sv.r + sv.i * sv.i;
-                ps = sv;
-            }
-        }
-        pm2 = pm1;
-        pm1 = from_d<R>(pk);
-        if (k >= 2) a_last = bsum(acc, k & 1);
-    }
-    if (threadIdx.x == 0) {
-        ctl->last_order = last;
-        if (last >= 2) partials[(last % 2) * kMaxPartials] = a_last;
-        if (broke) {
-            ctl->done = 1;
-            ctl->accnorm = a_last;
-        }
-    }
-    psi[o] = ps;
-}
-
 template <class R, int S>
 static int cheby_init_S(hipStream_t st, const GridDev &g, const LinePlan &plan, const double *psi, R *phi0,
                         R *X, ChebyCtl *ctl, double *partials, bool cmp) {
@@ -690,23 +541,6 @@ void Launch<R>::cheby_line(hipStream_t s, const GridDev &g, const HamDev<R> &h, 
                            double prec, double *psi, ChebyCtl *ctl, double *partials) {
     const int L = g.N[0], P = p0.ppt == 4 ? 4 : 8, TL = L / P;
     if (g.nd != 1 || h.S != 1 || L > kMaxLine) throw std::runtime_error("whole-step line kernel: one 1D line");
-    const bool nip0 = offset_i != 0 || h.Wi[0] != nullptr;
-    if (p0.ppt == 1) {   // one point per thread
-        if (L > 1024) throw std::runtime_error("whole-step line kernel: one point per thread up to 1024 points");
-        const size_t lds1 = (size_t)3 * L * sizeof(cx<R>) + (size_t)2 * order * sizeof(double);
-        auto cpsi1 = reinterpret_cast<cx<double> *>(psi);
-        if (nip0) {
-            set_lds(k_cheby_line_pt<R, true>, lds1);
-            QDX_LAUNCH((k_cheby_line_pt<R, true>), dim3(1), dim3(L), lds1, s, g, h, p0, coef, order, offset,
-                       offset_i, inv_scaling, prec, cpsi1, ctl, partials);
-        } else {
-            set_lds(k_cheby_line_pt<R, false>, lds1);
-            QDX_LAUNCH((k_cheby_line_pt<R, false>), dim3(1), dim3(L), lds1, s, g, h, p0, coef, order, offset, 0.0,
-                       inv_scaling, prec, cpsi1, ctl, partials);
-        }
-        QDX_HIP(hipGetLastError());
-        return;
-    }
     LinePlan p = p0;
     const size_t lcoef = (size_t)2 * order * sizeof(double);   // coefficients staged in LDS
     size_t lds = row_lds_bytes<R>(L, 1, true) + lcoef;

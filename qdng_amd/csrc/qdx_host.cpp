@@ -62,17 +62,13 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     double *init = part + 2 * kMaxPartials;
     if (p->line_step) {
         KTimer t(ctx, QDX_K_CHEBY_ROW);
-        // the whole-step kernel of a single-state line: one point per thread
-        // up to 1024 points (the line spreads over up to 16 waves), 8 above;
-        // qdx_ctx_set_line_points 1, 4 or 8 forces the choice (1 and 4 up to
-        // 1024 points)
+        // the whole-step kernel of a single-state line: 4 points per thread
+        // up to 1024 points (a short line spreads over more lanes: 3.7-5.9
+        // us per order against 5.2-7.4 at 8 for 128-1024 points), 8 above;
+        // qdx_ctx_set_line_points 4 or 8 forces the choice (4 up to 1024)
         const int N0 = p->grid.N[0];
-        int lp = ctx->line_ppt;
-        if (lp == 1 && N0 > 1024) lp = 8;
-        if (lp == 4 && N0 > 1024) lp = 8;
-        if (lp == 0) lp = N0 <= 1024 ? 1 : 8;
-        LinePlan lplan = lp == 8 ? plans[0] : ctx->plan_ppt(N0, fp32, lp == 1 ? 8 : 4);
-        if (lp == 1) lplan.ppt = 1;
+        const int lp = (ctx->line_ppt == 8 || N0 > 1024) ? 8 : 4;
+        const LinePlan lplan = lp == 8 ? plans[0] : ctx->plan_ppt(N0, fp32, 4);
         Launch<R>::cheby_line(s, g, h, lplan, p->coefd.as<double>(), p->k_launch, p->cp.offset, p->cp.offset_i,
                               1.0 / p->cp.scaling, p->cp.prec, psi->ptr(), ctl, p->partials.as<double>());
         p->last_nb = 1;
@@ -500,8 +496,7 @@ int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *
 int qdx_ctx_set_line_points(qdx_ctx *ctx, int32_t points) {
     return guarded([&] {
         if (!ctx) fail(QDX_ERR_PARAM, "ctx is NULL");
-        if (points != 0 && points != 1 && points != 4 && points != 8)
-            fail(QDX_ERR_PARAM, "points per thread: 0, 1, 4 or 8");
+        if (points != 0 && points != 4 && points != 8) fail(QDX_ERR_PARAM, "points per thread: 0, 4 or 8");
         ctx->line_ppt = points;
     });
 }

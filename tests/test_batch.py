@@ -421,13 +421,14 @@ def test_restart_from_snapshot_is_bit_identical(ctx, precision):
     first += [B.submit(dt=1.0, nsteps=k, psi0=p.psi0, ops=split_ops(p, None)) for p in g]
     B.run(k)
     snaps = [B.fetch(j) for j in first]
-    for j in first:
-        B.release(j)
-    second = [B.submit(p.gpu_ham(None), 1.0, steps - k, s, first_step=k) for p, s in zip(jobs, snaps)]
-    second += [B.submit(dt=1.0, nsteps=steps - k, psi0=s, ops=split_ops(p, None), first_step=k)
-                for p, s in zip(g, snaps[len(jobs):])]
-    B.run(steps - k)
+    # the restart in a new batch (as after the end of the process that held
+    # the snapshot)
+    C = Q.Batch(ctx, precision, 1 << 30)
+    second = [C.submit(p.gpu_ham(None), 1.0, steps - k, s, first_step=k) for p, s in zip(jobs, snaps)]
+    second += [C.submit(dt=1.0, nsteps=steps - k, psi0=s, ops=split_ops(p, None), first_step=k)
+               for p, s in zip(g, snaps[len(jobs):])]
+    C.run(steps - k)
     for a, j in zip(straight, second):
-        assert np.array_equal(B.fetch(j), a)
+        assert np.array_equal(C.fetch(j), a)
     with pytest.raises(Q.QdxError):
-        B.submit(jobs[0].gpu_ham(None), 1.0, steps, jobs[0].psi0, first_step=steps)   # beyond the laser
+        C.submit(jobs[0].gpu_ham(None), 1.0, steps, jobs[0].psi0, first_step=steps)   # beyond the laser

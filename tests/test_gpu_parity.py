@@ -228,11 +228,17 @@ def _two_state_3d():
 
 
 # 8 and 4 points per thread of the Chebyshev line kernels on grids with
-# nd >= 2 (the automatic choice takes 4 on grids this small).  Plans at 4:
-# 96 -> [4, 2, 3, 4] by default, a set plan with edges 4/2 is used as set,
-# one with an edge 8 gives way to the default; 24 -> [2, 3, 4].
+# nd >= 2 (the automatic choice takes 4 on grids this small) and of the
+# whole-step kernel of single-state 1D lines (4 by default up to 1024
+# points).  Plans at 4: 96 -> [4, 2, 3, 4] by default, a set plan with edges
+# 4/2 is used as set, one with an edge 8 gives way to the default;
+# 24 -> [2, 3, 4].
 @pytest.mark.parametrize("points", [4, 8])
 @pytest.mark.parametrize("name,builder,dt,precision,plan", [
+    ("harmonic_1d_256", lambda: cases.harmonic_1d(256), 10.0, "fp64", None),
+    ("harmonic_1d_1024", lambda: cases.harmonic_1d(1024, L=12.0), 10.0, "fp64", None),
+    ("laser_1d_384", lambda: cases.laser_1d(384), 1.0, "fp64", None),
+    ("harmonic_1d_512_fp32", lambda: cases.harmonic_1d(512), 10.0, "fp32", None),
     ("morse_2state_2d", lambda: cases.morse_2state_2d(96), 5.0, "fp64", None),
     ("morse_2state_2d_plan42", lambda: cases.morse_2state_2d(96), 5.0, "fp64", [4, 3, 4, 2]),
     ("morse_2state_2d_plan8", lambda: cases.morse_2state_2d(96), 5.0, "fp64", [8, 3, 4]),
