@@ -578,8 +578,8 @@ def cheby_1d_small(ctx, steps=1000):
             "l2_vs_oracle": float(np.linalg.norm(gv - psi) / np.linalg.norm(psi)),
             "norm_drift_max": float(np.abs(n2 - n2[0]).max()),
             "autocorr_max_abs_err": float(np.abs(ac - aco).max()),
-            "note": "latency bound (one 32-thread line; the whole step runs in one block): not judged on "
-                    "the roofline"}
+            "note": "latency bound (the whole step runs in one block, 4 points per thread: one wave "
+                    "holds the line): not judged on the roofline"}
 
 
 def main():

@@ -48,8 +48,11 @@ With `spectrum=` both programs write the autocorrelation spectrum
 `win="false"`).  `LC` wave functions sum their members as ChainLoader::LoadWFLC_ does (each
 twice, no coefficient).  Not supported (QdxError with QDX_ERR_INCOMPAT): other
 operator classes, reading a stored propagation (`read=`), filter actions
-other than `expeconly`, `normalize` and `apply` of a GridKick, and a
-propagation `start` other than 0.  Init filters (`filterinit`) run once
+other than `expeconly`, `normalize` and `apply` of a GridKick.  A
+propagation `start` = k restarts as ProgPropa does: the clock (laser sample
+and report time) begins at step k, the loop index runs k .. steps (so
+steps − k + 1 steps), the loaded state is not written again and written
+files are numbered from k.  Init filters (`filterinit`) run once
 before the initial state is kept for the autocorrelation.
 """
 from __future__ import annotations
@@ -561,8 +564,12 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
     wcycle = int(a.get("wcycle", 10))
     if wcycle < 1:
         raise _err("Write cycle less than one defined")
-    if int(a.get("start", 0)) != 0:
-        raise _err("propagation restart (start != 0) is not supported", ERR_INCOMPAT)
+    # restart (ProgPropa.cpp:82-86, 292-314): the clock starts at step
+    # `start`, the loop runs i = start .. steps (1 .. steps without it), the
+    # initial state is not written and the file counter starts at `start`
+    start = int(a.get("start", 0))
+    if start < 0:
+        raise _err("Only positive counter start counter numbers are allowed")
     d = _outdir(a, outdir)
     fname = a.get("fname", "WF")
     ld = Loader(srcdir, outdir)
@@ -589,30 +596,33 @@ def run_propa(ctx, el, srcdir=None, outdir=None, log=None):
         "\t\tEnergy[au]\t<Psi0|PsiT>"
     emit = (lambda s: out.write(s + "\n")) if out else (log or (lambda s: None))
     emit(header)
-    psi.save(d + fname, counter=0)
-    nwritten = 1
+    if start == 0:
+        psi.save(d + fname, counter=0)
+    nwritten = 1 if start == 0 else start
     spec = np.zeros(steps, dtype=complex) if "spectrum" in a else None
+    c = start   # clock step
     try:
-        for i in range(1, steps + 1):
-            pre.apply(psi, i - 1, (i - 1) * dt)
-            # Reporter::Analyze on the state of clock step i - 1
+        for n_an, i in enumerate(range(max(start, 1), steps + 1)):
+            pre.apply(psi, c, c * dt)
+            # Reporter::Analyze on the state of clock step c
             n, per = psi.norm()
             pops = [float(x) for x in per] if psi.S > 1 else []   # squared norms per state
-            e = H.expec(psi, i - 1)
+            e = H.expec(psi, c)
             ac = psi.overlap(psi0)
-            row = (i - 1, (i - 1) * dt, n * n, pops, e, ac)
+            row = (c, c * dt, n * n, pops, e, ac)
             if not math.isfinite(n):
                 raise _err("Norm is not a number")
             res.report.append(row)
             if spec is not None:
-                spec[i - 1] = ac
+                spec[n_an] = ac
             line = _report_line(*row)
             res.report_text.append(line)
             emit(line)
-            U.apply(psi, i - 1)
-            vals = post.apply(psi, i, i * dt)
+            U.apply(psi, c)
+            c += 1
+            vals = post.apply(psi, c, c * dt)
             if vals is not None:
-                res.expec_post.append((i * dt, vals))
+                res.expec_post.append((c * dt, vals))
             if i % wcycle == 0:
                 psi.save(d + fname, counter=nwritten)
                 nwritten += 1

@@ -442,3 +442,45 @@ def test_global_file_format_parameters(ctx, tmp_path):
     w = Q.WaveFunction.load(ctx, os.path.join(res.dir, "WF"), counter=2)
     ref = res.psi.download()
     assert np.max(np.abs(w.download() - ref)) < 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_propagation_restart(ctx, tmp_path):
+    # ProgPropa restart (src/ProgPropa.cpp:82-86, 292-326): a run from the
+    # state written after 20 steps with start="20" and steps="30" takes steps
+    # 20 .. 30 of the clock (11 steps, laser samples 20 .. 30) and ends where
+    # an uninterrupted 31-step run ends; its report rows are that run's rows
+    # 20 .. 30 and its files are numbered from 20
+    import xml.etree.ElementTree as ET
+
+    def run(tag, steps, start=None, wf=None):
+        print("[gpu case] qdi restart", tag, flush=True)
+        tree = ET.parse(qdi_path("TGPL-IR-Pulse-Cheby"))
+        pr = tree.getroot().find("propa")
+        pr.set("steps", str(steps))
+        pr.set("wcycle", "10")
+        pr.set("dir", tag)
+        if start is not None:
+            pr.set("start", str(start))
+            w = pr.find("wf")
+            w.set("file", wf)
+            del w.attrib["normalize"]
+        path = str(tmp_path / (tag + ".qdi"))
+        tree.write(path)
+        (res,) = qdi.run(path, ctx, FIX, str(tmp_path))
+        return res
+
+    first = run("first", 20)
+    full = run("full", 31)
+    rs = run("restart", 30, 20, os.path.join(first.dir, "WF_2"))
+    assert [r[:2] for r in rs.report] == [r[:2] for r in full.report[20:31]]
+    for a, b in zip(rs.report, full.report[20:31]):
+        assert abs(a[2] - b[2]) < 1e-13 and abs(a[4] - b[4]) < 1e-13 * abs(b[4]) + 1e-16
+    ref = full.psi.download()
+    assert np.linalg.norm(rs.psi.download() - ref) < 1e-12 * np.linalg.norm(ref)
+    assert not os.path.exists(os.path.join(rs.dir, "WF_0.wf"))
+    w = Q.WaveFunction.load(ctx, os.path.join(rs.dir, "WF"), counter=21)
+    assert np.array_equal(w.download(), rs.psi.download())
+    assert os.path.exists(os.path.join(rs.dir, "WF_20.wf"))
+    with pytest.raises(Q.QdxError):
+        run("bad", 30, -1, os.path.join(first.dir, "WF_2"))
