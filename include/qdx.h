@@ -60,7 +60,8 @@ typedef struct qdx_prop qdx_prop;
  * dx = (xmax - xmin) / (N - 1), both end points are grid points. */
 typedef struct qdx_grid {
     int32_t ndims;          /* 1..3 on the device path                           */
-    int32_t N[4];
+    int32_t N[4];           /* multiples of 8 up to 4096 without prime factors
+                               above 31 (QDng's FFTW takes any length)        */
     double xmin[4];
     double xmax[4];
 } qdx_grid;
@@ -387,7 +388,8 @@ int qdx_oct_run(qdx_prop *U, const qdx_oct_opts *opts, const double *shape, int6
  * Device memory of the jobs comes from one arena of `hbm_budget` bytes
  * allocated by qdx_batch_create; qdx_batch_submit fails with QDX_ERR_MEMORY
  * (nothing allocated) when the job does not fit, so callers can queue it.
- * Grids: 1D or 2D, each N <= 4096 and of the form 8 * 2^a * 3^b.  One block
+ * Grids: 1D or 2D, each N <= 4096 and of the form 8 * 2^a * 3^b (other
+ * prime factors are rejected with QDX_ERR_INCOMPAT).  One block
  * runs a 1D job's whole step with the line held in registers. */
 typedef struct qdx_batch qdx_batch;
 typedef struct qdx_job_desc {
@@ -455,7 +457,7 @@ int qdx_batch_graph_stats(qdx_batch *b, int64_t *captured, int64_t *replayed);
 
 /* Radix plan of the line transforms of length L in one precision on a
  * context: the sequence of radices (product L; first and last 8, 4 or 2;
- * middle 8, 4, 3 or 2; at most 16).  A plan set here is used by the
+ * middle 8, 4, 3, 2 or a prime 5 .. 31; at most 16).  A plan set here is used by the
  * propagators and Hamiltonians created and the batch jobs submitted
  * afterwards on this context (a batch job keeps the plans it was submitted
  * with).  Different plans give the same transform up to rounding. */

@@ -658,6 +658,9 @@ BatchJob &find_job(qdx_batch *b, int64_t id) {
 
 void check_grid(const Grid &g) {
     if (g.nd > 2) fail(QDX_ERR_INCOMPAT, "batched jobs are 1D or 2D");
+    // the batched transforms (in-place schedule) run radix 8/4/2/3 stages only
+    for (int d = 0; d < g.nd; ++d)
+        if (!is_smooth23(g.N[d])) fail(QDX_ERR_INCOMPAT, "batched jobs take line lengths 8 * 2^a * 3^b");
 }
 
 // job parameters and host arrays, without touching the device
@@ -838,7 +841,7 @@ Grid shape_grid(int n0, int n1) {
     g.N[0] = n0;
     g.N[1] = n1;
     check_grid(g);
-    radix_plan(n0);   // N = 8 * 2^a * 3^b
+    radix_plan(n0);
     radix_plan(n1);
     if (n0 > kMaxLine || n1 > kMaxLine) fail(QDX_ERR_PARAM, "batched lines hold at most 4096 points");
     return g;
@@ -1447,7 +1450,7 @@ int qdx_batch_tuning_load(qdx_batch *b, const char *path) {
                     seq.push_back(r);
                     rest += n;
                 }
-                if (bad.empty() && !radix_ok(L, seq)) bad = line;
+                if (bad.empty() && (!is_smooth23(L) || !radix_ok(L, seq))) bad = line;
                 plans[L] = seq;
             } else if (std::strcmp(kind, "lds") == 0) {
                 if (std::sscanf(rest, "%d %d", &lds_r, &lds_c) != 2 || (lds_r != 1 && lds_r != 2) ||

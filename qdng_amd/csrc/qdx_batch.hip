@@ -444,8 +444,9 @@ __device__ __forceinline__ void wsplit(int S, bool coupled, double tau, double w
     const double m = 0.5 * (w00 + w11);
     const double d = 0.5 * (w00 - w11);
     const double O = sqrt(d * d + w01 * w01);
-    const double c = cos(tau * O);
-    const double sc = O > 0 ? sin(tau * O) / O : tau;
+    double so, c;
+    sincos(tau * O, &so, &c);
+    const double sc = O > 0 ? so / O : tau;
     double sm, cm;
     sincos(-tau * m, &sm, &cm);
     const cx<double> ph = mk<double>(cm, sm);

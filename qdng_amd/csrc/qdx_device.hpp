@@ -190,7 +190,8 @@ struct Bfly<8, R> {
 // those elements and the last stage (span L/r) writes exactly those elements,
 // so both run on registers; only the middle stages go through LDS, using two
 // buffers alternately (one barrier per stage).  The plan's first and last
-// radices are 8, 4 or 2 and divide P; middle stages may also be radix 3.
+// radices are 8, 4 or 2 and divide P; middle stages may also be radix 3 or
+// (two-buffer schedule only) another prime (mid_stage_gen).
 //
 // LDS addressing of one line: element i of buffer a / b lives at
 // a[idx(i)] / b[idx(i)], idx(i) = (pad ? i + i/8 : i) * es.
@@ -240,6 +241,39 @@ __device__ __forceinline__ void mid_stage(const cx<R> *in, cx<R> *out, const Lds
             const int ob = (j - k) * RR + k;
 #pragma unroll
             for (int t = 0; t < RR; ++t) out[ln.idx(ob + t * Ns)] = x[t];
+        }
+    }
+}
+
+// A middle stage of an odd prime radix r > 3 (5, 7, 11, ...; lengths with
+// such factors only).  Output u of butterfly j sums its r inputs, each read
+// again from LDS, times one table twiddle: exp(sign 2 pi i (k t / (Ns r) +
+// (t u mod r) / r)) = table entry k t L/(Ns r) + (t u mod r) L/r (mod L).
+// Only one accumulator stays live, so the kernels keep the registers of
+// their radix 8/4/3/2 stages; the cost is r^2 LDS reads per butterfly.
+template <class R>
+__device__ __forceinline__ void mid_stage_gen(const cx<R> *in, cx<R> *out, const LdsLine<R> &ln, int L, int Ns,
+                                              int TL, int lt, int sign, const cx<R> *tw, int tsh, int r) {
+    const int m = L / r;
+    const int tstep = L / (Ns * r);
+    const int ustep = L / r;
+#pragma unroll 1
+    for (int j = lt; j < m; j += TL) {
+        const int k = j % Ns;
+        const int ob = (j - k) * r + k;
+#pragma unroll 1
+        for (int u = 0; u < r; ++u) {
+            cx<R> acc = in[ln.idx(j)];
+            int tu = 0;   // t u mod r
+#pragma unroll 1
+            for (int t = 1; t < r; ++t) {
+                tu += u;
+                if (tu >= r) tu -= r;
+                int q = k * t * tstep + tu * ustep;
+                if (q >= L) q -= L;
+                acc = acc + cmul(in[ln.idx(j + t * m)], twiddle(tw, q, sign, tsh));
+            }
+            out[ln.idx(ob + u * Ns)] = acc;
         }
     }
 }
@@ -360,6 +394,7 @@ __device__ __forceinline__ void fft_line(cx<R> (&v)[P], const LdsLine<R> &ln, co
             case 8: mid_stage_ip<8, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
             case 4: mid_stage_ip<4, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
             case 2: mid_stage_ip<2, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
+            // radix 3 (the hosts keep other primes out of in-place plans)
             default: mid_stage_ip<3, R, P>(ln.a, ln, L, Ns, TL, lv, sign, tw, tsh, active); break;
             }
             __syncthreads();
@@ -375,7 +410,8 @@ __device__ __forceinline__ void fft_line(cx<R> (&v)[P], const LdsLine<R> &ln, co
             case 8: mid_stage<8, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
             case 4: mid_stage<4, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
             case 2: mid_stage<2, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
-            default: mid_stage<3, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            case 3: mid_stage<3, R, P>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh); break;
+            default: mid_stage_gen<R>(in, out, ln, L, Ns, TL, lv, sign, tw, tsh, r); break;
             }
         }
         __syncthreads();
@@ -458,6 +494,30 @@ __device__ __forceinline__ double block_sum(double v, double *red /* >= 16 doubl
     if (threadIdx.x == 0)
         for (int i = 0; i < nw; ++i) t += red[i];
     return t;
+}
+
+// N block sums at once (one LDS exchange); the sums land in v[] of thread 0,
+// each in the order of block_sum
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double *red /* >= 16 N doubles */) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) red[w * N + i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            double t = 0;
+            for (int k = 0; k < nw; ++k) t += red[k * N + i];
+            v[i] = t;
+        }
+    }
 }
 
 // Sum of n partials in a fixed order that does not depend on the block size:

@@ -57,8 +57,9 @@ __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx
         const double d = 0.5 * (Wm[0][0] - Wm[1][1]);
         const double b = Wm[0][1];
         const double O = sqrt(d * d + b * b);
-        const double c = cos(op.tau * O);
-        const double sc = O > 0 ? sin(op.tau * O) / O : op.tau;
+        double so, c;
+        sincos(op.tau * O, &so, &c);
+        const double sc = O > 0 ? so / O : op.tau;
         double sm, cm;
         sincos(-op.tau * m, &sm, &cm);
         const cx<double> ph = mk<double>(cm, sm);
@@ -177,36 +178,86 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
         }
     }
     B.fft(v, p0, +1);
-    xspace(post);
     if (PHASE == 2) {
-        if (opart) {
-            __shared__ double red[16];
-            double n2 = 0, ar = 0, ai = 0;
+        // step n's post operators, the observables of the state between the
+        // steps (rounded to R, as a stored psi) and step n + 1's pre
+        // operators on each point in one pass: for S > 1 the S lines of a
+        // row are exchanged through LDS once around all three
+        double n2[S], ar[S], ai[S];
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2) n2[s2] = ar[s2] = ai[s2] = 0;
+        auto point = [&](long gi, cx<double> (&w)[S]) {
+            xgroup_point<R, S, IMAG>(post, gi, w);
+#pragma unroll
+            for (int s2 = 0; s2 < S; ++s2) {
+                w[s2] = to_d(from_d<R>(w[s2]));
+                if (opart) {
+                    const cx<double> a = w[s2], b = to_d(psi0[(long)s2 * g.size + gi]);
+                    n2[s2] += a.r * a.r + a.i * a.i;
+                    ar[s2] += a.r * b.r + a.i * b.i;
+                    ai[s2] += a.r * b.i - a.i * b.r;
+                }
+            }
+            xgroup_point<R, S, IMAG>(pre, gi, w);
+        };
+        if (S == 1) {
             if (B.active) {
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
-                    const cx<double> a = to_d(v[q]), b = to_d(psi0[base + B.gidx(q)]);
-                    n2 += a.r * a.r + a.i * a.i;
-                    ar += a.r * b.r + a.i * b.i;
-                    ai += a.r * b.i - a.i * b.r;
+                    cx<double> w[S];
+                    w[0] = to_d(v[q]);
+                    point(B.gidx(q), w);
+                    v[q] = from_d<R>(w[0]);
                 }
             }
-            const long nb = gridDim.x;
+        } else {
+            __syncthreads();
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL] = v[q];
+            }
+            __syncthreads();
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const int x = B.lt + q * B.TL;
+                    if (x % S != st) continue;
+                    cx<double> w[S];
+#pragma unroll
+                    for (int s2 = 0; s2 < S; ++s2) w[s2] = to_d(xa[((long)s2 * B.rb + B.rr) * B.L + x]);
+                    point(B.row * B.L + x, w);
+#pragma unroll
+                    for (int s2 = 0; s2 < S; ++s2) xa[((long)s2 * B.rb + B.rr) * B.L + x] = from_d<R>(w[s2]);
+                }
+            }
+            __syncthreads();
+            if (B.active) {
+#pragma unroll
+                for (int q = 0; q < P; ++q) v[q] = xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL];
+            }
+        }
+        if (opart) {
+            // the 3 S block sums in one reduction
+            __shared__ double red[16 * 3 * S];
+            double t[3 * S];
 #pragma unroll
             for (int s2 = 0; s2 < S; ++s2) {
-                const bool mine = B.active && st == s2;
-                const double t0 = block_sum(mine ? n2 : 0.0, red);
-                const double t1 = block_sum(mine ? ar : 0.0, red);
-                const double t2 = block_sum(mine ? ai : 0.0, red);
-                if (threadIdx.x == 0) {
-                    opart[2 * (s2 * nb + blockIdx.x)] = t0;
+                t[3 * s2] = n2[s2];
+                t[3 * s2 + 1] = ar[s2];
+                t[3 * s2 + 2] = ai[s2];
+            }
+            block_sum_n<3 * S>(t, red);
+            if (threadIdx.x == 0) {
+                const long nb = gridDim.x;
+#pragma unroll
+                for (int s2 = 0; s2 < S; ++s2) {
+                    opart[2 * (s2 * nb + blockIdx.x)] = t[3 * s2];
                     opart[2 * (s2 * nb + blockIdx.x) + 1] = 0.0;
-                    opart[2 * S * nb + 2 * (s2 * nb + blockIdx.x)] = t1;
-                    opart[2 * S * nb + 2 * (s2 * nb + blockIdx.x) + 1] = t2;
+                    opart[2 * S * nb + 2 * (s2 * nb + blockIdx.x)] = t[3 * s2 + 1];
+                    opart[2 * S * nb + 2 * (s2 * nb + blockIdx.x) + 1] = t[3 * s2 + 2];
                 }
             }
         }
-        xspace(pre);
         B.fft(v, p0, -1);
         if (B.active) {
 #pragma unroll
@@ -214,6 +265,7 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
         }
         return;
     }
+    xspace(post);
     if (B.active) {
 #pragma unroll
         for (int q = 0; q < P; ++q) psi[base + B.gidx(q)] = v[q];

@@ -475,7 +475,7 @@ int qdx_ctx_set_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, const int
         if (!ctx || !radix) fail(QDX_ERR_PARAM, "NULL argument");
         if (precision != QDX_FP64 && precision != QDX_FP32_BULK) fail(QDX_ERR_PARAM, "unknown precision");
         if (nst < 1 || nst > kMaxStages) fail(QDX_ERR_PARAM, "1 to 16 radix stages");
-        radix_plan(L);   // L = 8 * 2^a * 3^b
+        radix_plan(L);   // a valid line length
         std::vector<int> r(radix, radix + nst);
         if (!radix_ok(L, r)) fail(QDX_ERR_PARAM, "radix plan does not fit the line length");
         ctx->radix[{L, precision == QDX_FP32_BULK}] = r;

@@ -122,11 +122,33 @@ struct Grid {
     }
 };
 
-inline bool is_smooth(int n) {  // 2^a 3^b
-    if (n < 1) return false;
-    while (n % 2 == 0) n /= 2;
-    while (n % 3 == 0) n /= 3;
-    return n == 1;
+// A line length L = 2^a 3^b p_1 p_2 ... with odd primes 5 <= p_i <= kMaxPrime
+// (the device transforms run radix 8/4/2/3 stages and, for the other primes,
+// a generic stage of r^2 operations per butterfly).  Returns false when L
+// has a larger prime factor.
+constexpr int kMaxPrime = 31;
+inline bool factor_line(int L, int &a, int &b, std::vector<int> &big) {
+    a = b = 0;
+    big.clear();
+    if (L < 1) return false;
+    int m = L;
+    while (m % 2 == 0) { ++a; m /= 2; }
+    while (m % 3 == 0) { ++b; m /= 3; }
+    for (int f = 5; m > 1 && f <= kMaxPrime; f += 2)
+        while (m % f == 0) { big.push_back(f); m /= f; }
+    return m == 1;
+}
+// only factors 2 and 3 (the batched kernels' in-place transforms)
+inline bool is_smooth23(int n) {
+    int a, b;
+    std::vector<int> big;
+    return factor_line(n, a, b, big) && big.empty();
+}
+inline bool is_small_prime(int x) {
+    if (x < 5 || x > kMaxPrime || x % 2 == 0) return false;
+    for (int f = 3; f * f <= x; f += 2)
+        if (x % f == 0) return false;
+    return true;
 }
 
 inline Grid make_grid(const qdx_grid *g) {
@@ -137,8 +159,10 @@ inline Grid make_grid(const qdx_grid *g) {
     for (int d = 0; d < r.nd; ++d) {
         if (g->N[d] < 2) fail(QDX_ERR_PARAM, "Zero points grid defined");
         if (g->xmax[d] - g->xmin[d] <= 0) fail(QDX_ERR_PARAM, "Zero length grid defined");
-        if (!is_smooth(g->N[d]) || g->N[d] % 8 != 0)
-            fail(QDX_ERR_PARAM, "grid sizes must be of the form 8 * 2^a * 3^b");
+        int fa, fb;
+        std::vector<int> fbig;
+        if (!factor_line(g->N[d], fa, fb, fbig) || g->N[d] % 8 != 0)
+            fail(QDX_ERR_PARAM, "grid sizes must be multiples of 8 without prime factors above 31");
         if (g->N[d] > kMaxLine) fail(QDX_ERR_PARAM, "grid dimension longer than 4096 points");
         r.N[d] = g->N[d];
         r.xmin[d] = g->xmin[d];
@@ -148,17 +172,21 @@ inline Grid make_grid(const qdx_grid *g) {
 }
 
 // ---------------------------------------------------------------------------
-// radix plans and twiddle tables exp(-2 pi i q / L).  L = 2^a 3^b with
-// a >= 3; the first and the last stage take a radix 8, 4 or 2 (they run on
-// registers in fft8), radix-3 stages sit in the middle.
+// radix plans and twiddle tables exp(-2 pi i q / L).  L = 2^a 3^b p_1 ...
+// with a >= 3; the first and the last stage take a radix 8, 4 or 2 (they run
+// on registers in fft_line), radix-3 stages and the other primes sit in the
+// middle.
+inline void factor_or_fail(int L, int &a, int &b, std::vector<int> &big) {
+    if (!factor_line(L, a, b, big) || a < 3)
+        fail(QDX_ERR_PARAM, "line length must be a multiple of 8 without prime factors above 31");
+}
 inline std::vector<int> radix_plan(int L) {
-    int a = 0, b = 0, m = L;
-    while (m % 2 == 0) { ++a; m /= 2; }
-    while (m % 3 == 0) { ++b; m /= 3; }
-    if (m != 1 || a < 3) fail(QDX_ERR_PARAM, "line length must be 8 * 2^a * 3^b");
+    int a, b;
+    std::vector<int> big;
+    factor_or_fail(L, a, b, big);
     auto pw = [](int e) { return e >= 3 ? 8 : (e == 2 ? 4 : 2); };
     std::vector<int> r;
-    if (b == 0) {
+    if (b == 0 && big.empty()) {
         while (a >= 3) { r.push_back(8); a -= 3; }
         if (a > 0) r.push_back(pw(a));
     } else {
@@ -170,6 +198,7 @@ inline std::vector<int> radix_plan(int L) {
         while (a >= 3) { r.push_back(8); a -= 3; }
         if (a > 0) r.push_back(pw(a));
         for (int i = 0; i < b; ++i) r.push_back(3);
+        for (int f : big) r.push_back(f);
         r.push_back(last);
     }
     if ((int)r.size() > kMaxStages) fail(QDX_ERR_PARAM, "too many radix stages");
@@ -177,12 +206,12 @@ inline std::vector<int> radix_plan(int L) {
 }
 
 // Default plan of the 4-points-per-thread kernels: first and last radix 4
-// (2 where L has too few factors 2), radix-8, -4, -2 and -3 stages between.
+// (2 where L has too few factors 2), radix-8, -4, -2, -3 and other prime
+// stages between.
 inline std::vector<int> radix_plan4(int L) {
-    int a = 0, b = 0, m = L;
-    while (m % 2 == 0) { ++a; m /= 2; }
-    while (m % 3 == 0) { ++b; m /= 3; }
-    if (m != 1 || a < 3) fail(QDX_ERR_PARAM, "line length must be 8 * 2^a * 3^b");
+    int a, b;
+    std::vector<int> big;
+    factor_or_fail(L, a, b, big);
     const int first = a >= 5 ? 4 : 2;
     a -= first == 4 ? 2 : 1;
     const int last = a >= 2 ? 4 : 2;
@@ -192,20 +221,22 @@ inline std::vector<int> radix_plan4(int L) {
     if (a == 2) r.push_back(4);
     if (a == 1) r.push_back(2);
     for (int i = 0; i < b; ++i) r.push_back(3);
+    for (int f : big) r.push_back(f);
     r.push_back(last);
     if ((int)r.size() > kMaxStages) fail(QDX_ERR_PARAM, "too many radix stages");
     return r;
 }
 
-// A radix sequence fft8 can run for length L: product L, first and last
-// radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 4, 3 or 2.
+// A radix sequence fft_line can run for length L: product L, first and last
+// radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 4, 3, 2 or
+// a prime 5 .. kMaxPrime.
 inline bool radix_ok(int L, const std::vector<int> &r) {
     if (r.empty() || (int)r.size() > kMaxStages) return false;
     long prod = 1;
     for (size_t i = 0; i < r.size(); ++i) {
         const int x = r[i];
         const bool edge = i == 0 || i + 1 == r.size();
-        if (!(x == 8 || x == 4 || x == 2 || (!edge && x == 3))) return false;
+        if (!(x == 8 || x == 4 || x == 2 || (!edge && (x == 3 || is_small_prime(x))))) return false;
         prod *= x;
     }
     if (L == 8) return r.size() == 1 && r[0] == 8;
@@ -218,6 +249,11 @@ inline std::vector<std::vector<int>> radix_candidates(int L, int extra) {
     std::vector<std::vector<int>> out;
     if (L == 8) return {{8}};
     const int nmin = (int)radix_plan(L).size();
+    int a, b;
+    std::vector<int> radices{8, 4, 3, 2}, big;
+    factor_line(L, a, b, big);
+    for (int f : big)
+        if (std::find(radices.begin(), radices.end(), f) == radices.end()) radices.push_back(f);
     std::vector<int> cur;
     // depth-first over sequences whose prefix product divides L
     std::function<void(long)> rec = [&](long prod) {
@@ -227,7 +263,7 @@ inline std::vector<std::vector<int>> radix_candidates(int L, int extra) {
             return;
         }
         if (n >= nmin + extra) return;
-        for (int x : {8, 4, 3, 2}) {
+        for (int x : radices) {
             if ((L / prod) % x != 0) continue;
             cur.push_back(x);
             rec(prod * x);

@@ -36,7 +36,7 @@ def _fresh():
 # FFT
 
 @pytest.mark.parametrize("dims", [[8], [9], [12], [96], [128], [243], [100], [6, 4], [8, 12],
-                                  [4, 6, 8], [16, 9, 5]])
+                                  [4, 6, 8], [16, 9, 5], [40], [56], [440], [1000], [3080], [40, 56]])
 def test_fft_matches_numpy(dims):
     rng = np.random.default_rng(1)
     shape = dims[::-1]
@@ -47,7 +47,7 @@ def test_fft_matches_numpy(dims):
     assert np.allclose(b, np.fft.ifftn(x) * x.size, rtol=1e-13, atol=1e-12 * np.abs(x).sum())
 
 
-@pytest.mark.parametrize("n", [7, 24, 45])
+@pytest.mark.parametrize("n", [7, 24, 45, 88, 104, 248])
 def test_fft_brute_force_dft(n):
     x = np.exp(1j * np.arange(n) ** 1.5) * (1 + np.arange(n) % 3)
     F = np.exp(-2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
