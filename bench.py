@@ -597,6 +597,9 @@ def main():
     ap.add_argument("--batched-only", choices=["fp64", "fp32", "compact"],
                     help="only run the batched-jobs workload in this precision (default schedules, "
                          "--batch-steps ticks) and print its tick time: a short program for counter runs")
+    ap.add_argument("--gspo-only", action="store_true",
+                    help="only run the GSPO leg (FP64 and FP32, --steps steps after --warmup) and print its step "
+                         "time: a short program for traces")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -619,6 +622,11 @@ def main():
         return
     p = build_problem(512)
     npts = p.size * p.S
+    if args.gspo_only:
+        for prec in ("fp64", "fp32"):
+            t = run_gspo(ctx, p, prec, args.steps, args.warmup)
+            log(f"gspo {prec}: {npts * args.steps / t:.4g} grid-pt*step/s, {t * 1e6 / args.steps:.2f} us/step")
+        return
     log(f"workload 512x512 x 2 states, {args.warmup} warmup + {args.steps} timed steps")
 
     # ---- headline: FP64 Chebyshev.  Pass 1 times the steps; pass 2 repeats

@@ -59,7 +59,7 @@ typedef struct qdx_prop qdx_prop;
 /* Grid geometry, QDng GridSystem (GridSystem.h:29-118, GridSystem.proto:6-17):
  * dx = (xmax - xmin) / (N - 1), both end points are grid points. */
 typedef struct qdx_grid {
-    int32_t ndims;          /* 1..3 on the device path                           */
+    int32_t ndims;          /* 1..4 (QDng's MAX_DIMS)                            */
     int32_t N[4];           /* multiples of 8 up to 4096 without prime factors
                                above 31 (QDng's FFTW takes any length)        */
     double xmin[4];

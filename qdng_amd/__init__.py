@@ -408,6 +408,8 @@ def write_propagation_meta(directory: str, nt: int, dt: float, wf_base: str = "W
 
 
 def make_grid(N, xmin, xmax) -> Grid:
+    if not 1 <= len(N) <= 4:
+        raise QdxError(-1, "grids have 1..4 dimensions")
     g = Grid()
     g.ndims = len(N)
     for d in range(len(N)):

@@ -47,7 +47,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     if constexpr (std::is_same<R, float>::value) h = H->dev.h32;
     else h = H->dev.h64;
     H->set_fields(h, step);
-    LinePlan plans[3];
+    LinePlan plans[kMaxDims];
     const int ppt = cheby_line_points(ctx, g, p->S);
     for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan_ppt(p->grid.N[d], fp32, ppt);
     R *A = p->phiA.as<R>(), *B = p->phiB.as<R>();
@@ -193,13 +193,25 @@ XSplitDev<R> xsplit(qdx_ham *op, double tau, long step) {
     return x;
 }
 
+// Strided passes of the full kinetic factor on grids with nd >= 2 (mode 2:
+// T(k)/N, mode 3: exp(-i tau T(k))/N) on data transformed along dimension 0:
+// forward transforms along dimensions 1 .. nd-2, the last dimension's
+// transform, factor and inverse, then the inverse transforms back down to
+// dimension 1 (QDng's N-dimensional transform, OGSPO.cpp:114-139).
+template <class F>
+void full_t_passes(int nd, int mode, F col) {
+    for (int d = 1; d < nd - 1; ++d) col(d, 0);
+    col(nd - 1, mode);
+    for (int d = nd - 2; d >= 1; --d) col(d, 1);
+}
+
 template <class R>
 void gspo_step(qdx_prop *p, R *psi, long step) {
     qdx_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const bool fp32 = std::is_same<R, float>::value;
     const GridDev g = p->grid.dev();
-    LinePlan plans[3];
+    LinePlan plans[kMaxDims];
     // 4 points per thread where the grid is too small for two waves per
     // SIMD at 8 (the rule of the Chebyshev line kernels)
     const int ppt = cheby_line_points(ctx, g, p->S);
@@ -243,13 +255,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
                 KTimer t(ctx, QDX_K_LINE_PASS);
                 Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, o.tau, X, nullptr, ConvCheck(), p->imag);
             };
-            if (p->grid.nd == 2) {
-                col(1, 3);
-            } else {
-                col(1, 0);
-                col(2, 3);
-                col(1, 1);
-            }
+            full_t_passes(p->grid.nd, 3, col);
             {
                 KTimer t(ctx, QDX_K_GSPO_ROW);
                 Launch<R>::gspo_row_post(s, g, p->S, plans[0], post, X, psi, p->imag);
@@ -294,7 +300,7 @@ void gspo_run_fused(qdx_prop *p, R *psi, const R *psi0, long first, long nsteps,
     hipStream_t s = ctx->stream;
     const bool fp32 = std::is_same<R, float>::value;
     const GridDev g = p->grid.dev();
-    LinePlan plans[3];
+    LinePlan plans[kMaxDims];
     const int ppt = cheby_line_points(ctx, g, p->S);
     for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan_ppt(p->grid.N[d], fp32, ppt);
     const auto seq = gspo_sequence(p);
@@ -321,13 +327,7 @@ void gspo_run_fused(qdx_prop *p, R *psi, const R *psi0, long first, long nsteps,
             KTimer t(ctx, QDX_K_LINE_PASS);
             Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, tau, X, nullptr, ConvCheck(), p->imag);
         };
-        if (p->grid.nd == 2) {
-            col(1, 3);
-        } else {
-            col(1, 0);
-            col(2, 3);
-            col(1, 1);
-        }
+        full_t_passes(p->grid.nd, 3, col);
         const XGroupDev<R> post = group(kpos + 1, seq.size(), step);
         if (i == nsteps - 1) {
             KTimer t(ctx, QDX_K_GSPO_ROW);
@@ -870,7 +870,7 @@ static void ham_apply_impl(qdx_ham *h, qdx_wf *dst, qdx_wf *src, long step) {
     h->finalize();
     HamDev<double> hd = h->dev.h64;
     h->set_fields(hd, step);
-    LinePlan plans[3];
+    LinePlan plans[kMaxDims];
     for (int d = 0; d < h->grid.nd; ++d) plans[d] = h->ctx->plan(h->grid.N[d], false);
     if (h->grid.nd > 1 && !h->X64.p) h->X64.alloc((size_t)h->S * h->grid.size() * 2 * sizeof(double));
     Launch<double>::ham_apply(h->ctx->stream, h->grid.dev(), hd, plans, src->ptr(), dst->ptr(), h->X64.as<double>());
@@ -917,7 +917,7 @@ int qdx_ham_expec_noalloc(qdx_ham *h, qdx_wf *psi, int64_t step, double *out) {
         // set-up happens elsewhere: the operator arrays are uploaded and the
         // transform tables built by the first qdx_ham_apply / qdx_ham_expec
         if (!h->final_) fail(QDX_ERR_STATE, "qdx_ham_expec_noalloc: apply the Hamiltonian once first (set-up)");
-        LinePlan plans[3];
+        LinePlan plans[kMaxDims];
         for (int d = 0; d < h->grid.nd; ++d) {
             if (!c->tw64.count(h->grid.N[d]))
                 fail(QDX_ERR_STATE, "qdx_ham_expec_noalloc: apply the Hamiltonian once first (set-up)");

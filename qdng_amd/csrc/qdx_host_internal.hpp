@@ -116,7 +116,7 @@ struct Grid {
     GridDev dev() const {
         GridDev g;
         g.nd = nd;
-        for (int d = 0; d < 3; ++d) g.N[d] = d < nd ? N[d] : 1;
+        for (int d = 0; d < kMaxDims; ++d) g.N[d] = d < nd ? N[d] : 1;
         g.size = size();
         return g;
     }
@@ -153,7 +153,7 @@ inline bool is_small_prime(int x) {
 
 inline Grid make_grid(const qdx_grid *g) {
     if (!g) fail(QDX_ERR_PARAM, "grid is NULL");
-    if (g->ndims < 1 || g->ndims > 3) fail(QDX_ERR_PARAM, "device path supports 1..3 grid dimensions");
+    if (g->ndims < 1 || g->ndims > kMaxDims) fail(QDX_ERR_PARAM, "grids have 1..4 dimensions");
     Grid r;
     r.nd = g->ndims;
     for (int d = 0; d < r.nd; ++d) {
@@ -228,8 +228,8 @@ inline std::vector<int> radix_plan4(int L) {
 }
 
 // A radix sequence fft_line can run for length L: product L, first and last
-// radix 8, 4 or 2 (L = 8: the single radix 8), middle radices 8, 4, 3, 2 or
-// a prime 5 .. kMaxPrime.
+// radix 8, 4 or 2 (or, L = 8, the single radix 8), middle radices 8, 4, 3, 2
+// or a prime 5 .. kMaxPrime.
 inline bool radix_ok(int L, const std::vector<int> &r) {
     if (r.empty() || (int)r.size() > kMaxStages) return false;
     long prod = 1;
@@ -239,7 +239,7 @@ inline bool radix_ok(int L, const std::vector<int> &r) {
         if (!(x == 8 || x == 4 || x == 2 || (!edge && (x == 3 || is_small_prime(x))))) return false;
         prod *= x;
     }
-    if (L == 8) return r.size() == 1 && r[0] == 8;
+    if (L == 8 && r.size() == 1) return r[0] == 8;   // one register stage (8 points per thread)
     return r.size() >= 2 && prod == L;
 }
 
@@ -641,12 +641,12 @@ struct qdx_ham {
         HamDev<double> &h = dev.h64;
         h.S = S;
         h.inv_n = 1.0 / double(n);
-        for (int d = 0; d < 3; ++d) h.inv_nd[d] = d < grid.nd ? 1.0 / double(grid.N[d]) : 1.0;
+        for (int d = 0; d < kMaxDims; ++d) h.inv_nd[d] = d < grid.nd ? 1.0 / double(grid.N[d]) : 1.0;
         hostW.assign(S * S, {});
         hostMu.assign(S * S, {});
         hostWi.assign(S, {});
         std::vector<double> tkh;
-        std::vector<long> tk_off(S * 3, -1);
+        std::vector<long> tk_off(S * kMaxDims, -1);
         for (int i = 0; i < S; ++i) {
             h.unity[i] = 0;
             for (int j = 0; j < S; ++j) {
@@ -677,11 +677,11 @@ struct qdx_ham {
                         for (int d = 0; d < grid.nd; ++d) {
                             const int Nx = grid.N[d];
                             const std::vector<double> k1 = kspace_d2(grid, d, t.mass[d]);
-                            if (tk_off[i * 3 + d] < 0) {
-                                tk_off[i * 3 + d] = (long)tkh.size();
+                            if (tk_off[i * kMaxDims + d] < 0) {
+                                tk_off[i * kMaxDims + d] = (long)tkh.size();
                                 tkh.insert(tkh.end(), k1.begin(), k1.end());
                             } else {
-                                for (int q = 0; q < Nx; ++q) tkh[tk_off[i * 3 + d] + q] += k1[q];
+                                for (int q = 0; q < Nx; ++q) tkh[tk_off[i * kMaxDims + d] + q] += k1[q];
                             }
                         }
                         (void)base;
@@ -695,12 +695,13 @@ struct qdx_ham {
         tk.alloc(std::max<size_t>(tkh.size(), 1) * sizeof(double));
         if (!tkh.empty()) HIPCHK(hipMemcpy(tk.p, tkh.data(), tkh.size() * sizeof(double), hipMemcpyHostToDevice));
         for (int i = 0; i < S; ++i)
-            for (int d = 0; d < 3; ++d)
-                h.tk[i][d] = (d < grid.nd && tk_off[i * 3 + d] >= 0) ? tk.as<double>() + tk_off[i * 3 + d] : nullptr;
+            for (int d = 0; d < kMaxDims; ++d)
+                h.tk[i][d] = (d < grid.nd && tk_off[i * kMaxDims + d] >= 0) ? tk.as<double>() + tk_off[i * kMaxDims + d]
+                                                                           : nullptr;
         for (int i = 0; i < S; ++i) {
             h.Wi[i] = hostWi[i].empty() ? nullptr : upload64(hostWi[i]);
             if (h.tk[i][0] == nullptr)
-                for (int d = 0; d < 3; ++d) h.tk[i][d] = nullptr;
+                for (int d = 0; d < kMaxDims; ++d) h.tk[i][d] = nullptr;
             for (int j = 0; j < S; ++j) {
                 h.W[i][j] = nullptr;
                 h.mu[i][j] = nullptr;
@@ -732,11 +733,11 @@ struct qdx_ham {
         const HamDev<double> &d = dev.h64;
         f.S = d.S;
         f.inv_n = d.inv_n;
-        for (int k = 0; k < 3; ++k) f.inv_nd[k] = d.inv_nd[k];
+        for (int k = 0; k < kMaxDims; ++k) f.inv_nd[k] = d.inv_nd[k];
         for (int i = 0; i < S; ++i) {
             f.unity[i] = d.unity[i];
             f.Wi[i] = hostWi[i].empty() ? nullptr : upload32(hostWi[i]);
-            for (int k = 0; k < 3; ++k) f.tk[i][k] = d.tk[i][k];
+            for (int k = 0; k < kMaxDims; ++k) f.tk[i][k] = d.tk[i][k];
             for (int j = 0; j < S; ++j) {
                 f.W[i][j] = hostW[i * S + j].empty() ? nullptr : upload32(hostW[i * S + j]);
                 f.mu[i][j] = hostMu[i * S + j].empty() ? nullptr : upload32(hostMu[i * S + j]);

@@ -13,6 +13,7 @@ namespace qdx {
 constexpr int kMaxStates = 4;      // states handled by the fused kernels
 constexpr int kMaxStages = 16;     // radix stages of one line transform
 constexpr int kMaxLine = 4096;     // longest line transform held in LDS
+constexpr int kMaxDims = 4;        // grid dimensions (GridSystem's MAX_DIMS)
 
 // ---------------------------------------------------------------------------
 // Radix plan of one line length L = 2^a 3^b: radix-8 stages first, then one
@@ -32,9 +33,41 @@ struct LinePlan {
 // contiguous.  Unused dimensions have N = 1.
 struct GridDev {
     int nd = 1;
-    int N[3] = {1, 1, 1};
+    int N[kMaxDims] = {1, 1, 1, 1};
     long size = 1;
 };
+
+// Element stride of dimension d (the product of the sizes before it).
+__host__ __device__ inline long dim_stride(const GridDev &g, int d) {
+    long s = 1;
+    for (int e = 0; e < d; ++e) s *= g.N[e];
+    return s;
+}
+// Lines along dimension dim >= 1 of a column tile: besides dimension 0 (the
+// tile's columns) they run over the other dimensions, whose index o in
+// [0, col_nother) counts them in increasing dimension order.
+__host__ __device__ inline int col_nother(const GridDev &g, int dim) {
+    int n = 1;
+    for (int e = 1; e < g.nd; ++e)
+        if (e != dim) n *= g.N[e];
+    return n;
+}
+// element offset of "other" index o, and the index of each other dimension in k[]
+// (loops unrolled over kMaxDims: k[] keeps constant indices, in registers)
+__host__ __device__ inline long col_other(const GridDev &g, int dim, int o, int (&k)[kMaxDims]) {
+    long off = 0, stride = g.N[0];
+#pragma unroll
+    for (int e = 1; e < kMaxDims; ++e) {
+        if (e < g.nd && e != dim) {
+            const int i = o % g.N[e];
+            o /= g.N[e];
+            k[e] = i;
+            off += (long)i * stride;
+        }
+        stride *= g.N[e];
+    }
+    return off;
+}
 
 // Hamiltonian H = T + W(x) - E(t) mu(x) as the fused kernels apply it to a
 // multistate vector phi[S][N] (OSum / OMultistate of grid terms):
@@ -48,8 +81,8 @@ template <class R>
 struct HamDev {
     int S = 1;
     double inv_n = 1.0;
-    double inv_nd[3] = {1.0, 1.0, 1.0};   // 1 / N_d: T = sum_d T_d acts as 1D filters
-    const double *tk[kMaxStates][3] = {};
+    double inv_nd[kMaxDims] = {1.0, 1.0, 1.0, 1.0};   // 1 / N_d: T = sum_d T_d acts as 1D filters
+    const double *tk[kMaxStates][kMaxDims] = {};
     const R *W[kMaxStates][kMaxStates] = {};
     const R *mu[kMaxStates][kMaxStates] = {};
     double Edip[kMaxStates][kMaxStates] = {};

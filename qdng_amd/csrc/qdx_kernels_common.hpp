@@ -21,16 +21,15 @@ namespace qdx {
 // ---------------------------------------------------------------------------
 // helpers
 
-// kinetic value T_i(k) at k-space indices (k0, k1, k2), summed over dimensions
-// in dimension order as Diff::_InitKspace does (Diff.cpp:120-139)
+// kinetic value T_i(k) at k-space indices k[0 .. nd-1], summed over
+// dimensions in dimension order as Diff::_InitKspace does (Diff.cpp:120-139)
 template <class R>
-__device__ __forceinline__ double kin_sum(const HamDev<R> &h, int st, const GridDev &g, int k0, int k1, int k2) {
+__device__ __forceinline__ double kin_sum(const HamDev<R> &h, int st, const GridDev &g, const int (&k)[kMaxDims]) {
     const double *const *tk = h.tk[st];
-    double t = 0.0 + tk[0][k0];
-    if (g.nd > 1) {
-        t += tk[1][k1];
-        if (g.nd > 2) t += tk[2][k2];
-    }
+    double t = 0.0 + tk[0][k[0]];
+#pragma unroll
+    for (int d = 1; d < kMaxDims; ++d)
+        if (d < g.nd) t += tk[d][k[d]];
     return t;
 }
 
@@ -125,7 +124,7 @@ static size_t col_lds_bytes(int W, int L, size_t elem) {
 // tile width of the strided passes: adjacent dimension-0 positions per block.
 // Prefer >= 64-byte segments, keep >= 256 blocks when possible.
 static int col_tile(const GridDev &g, int S, int dim, int L, size_t elem, int P = 8) {
-    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
+    const int nother = col_nother(g, dim);
     int W = 16;
     auto fits = [&](int w) {
         return g.N[0] % w == 0 && col_lds_bytes(w, L, elem) <= 96 * 1024 && w * (L / P) <= 512;

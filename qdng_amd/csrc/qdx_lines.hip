@@ -26,9 +26,8 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     const int N0 = g.N[0];
     const int L = g.N[dim];
     const int TL = L / P;
-    const long estride = dim == 1 ? N0 : (long)N0 * g.N[1];
-    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
-    const long ostride = dim == 1 ? (long)N0 * g.N[1] : N0;
+    const long estride = dim_stride(g, dim);
+    const int nother = col_nother(g, dim);
     const int tiles0 = N0 / W;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int c0 = (b % tiles0) * W;
@@ -38,7 +37,8 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     const int c = threadIdx.x % W;
     const int lt = threadIdx.x / W;
     const bool active = lt < TL;
-    const long boff = (long)st * g.size + (long)oth * ostride + c0 + c;   // element of this thread's column
+    int kk[kMaxDims] = {0, 0, 0, 0};   // k-space index per dimension (dimensions 0 and dim set below)
+    const long boff = (long)st * g.size + col_other(g, dim, oth, kk) + c0 + c;   // element of this thread's column
     const long nT = (long)S * g.size;
     LdsLine<R> ln;
     ln.a = smem + c;
@@ -66,12 +66,15 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
     double tq[P];
     if (MODE >= 2 && active) {
         const bool has_t = h.tk[st][0] != nullptr;
-        const int k0 = c0 + c;
+        kk[0] = c0 + c;
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             const int kd = lt + q * TL;
+#pragma unroll
+            for (int e = 1; e < kMaxDims; ++e)
+                if (e == dim) kk[e] = kd;
             if (MODE >= 4) tq[q] = has_t ? (0.0 + h.tk[st][dim][kd]) * h.inv_nd[dim] : 0.0;
-            else tq[q] = has_t ? kin_sum<R>(h, st, g, k0, dim == 1 ? kd : oth, dim == 2 ? kd : oth) : 0.0;
+            else tq[q] = has_t ? kin_sum<R>(h, st, g, kk) : 0.0;
         }
     }
     if (tw_in_lds) stage_twiddles(twl, p);
@@ -200,8 +203,7 @@ static void col_pass_P4(hipStream_t s, const GridDev &g, int S, int dim, int mod
                         const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag) {
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>), P);
-    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
-    const int nb = (g.N[0] / W) * nother * S;
+    const int nb = (g.N[0] / W) * col_nother(g, dim) * S;
     const int threads = std::max(64, round64(W * (L / P)));
     const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *x = reinterpret_cast<cx<R> *>(X);
@@ -243,8 +245,7 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     if (plan.ppt == 4) return col_pass_P4<R, 4>(s, g, S, dim, mode, plan, h, tau, X, ctl, cc, imag);
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
-    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
-    const int nb = (g.N[0] / W) * nother * S;
+    const int nb = (g.N[0] / W) * col_nother(g, dim) * S;
     const int threads = std::max(64, round64(W * (L / 8)));
     const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *x = reinterpret_cast<cx<R> *>(X);
@@ -287,8 +288,7 @@ static void kin_pass_P(hipStream_t s, const GridDev &g, int S, int dim, bool acc
                        const HamDev<R> &h, const R *src, R *Y, ChebyCtl *ctl, const ConvCheck &cc, bool cmp) {
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>), P);
-    const int nother = g.nd == 3 ? (dim == 1 ? g.N[2] : g.N[1]) : 1;
-    const int nb = (g.N[0] / W) * nother * S;
+    const int nb = (g.N[0] / W) * col_nother(g, dim) * S;
     const int threads = std::max(64, round64(W * (L / P)));
     const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *y = reinterpret_cast<cx<R> *>(Y);
@@ -329,7 +329,8 @@ void Launch<R>::kin_pass(hipStream_t s, const GridDev &g, int S, int dim, bool a
 template <class R, int S>
 static void ham_apply_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, const LinePlan *plans_in,
                         const double *src, double *dst, R *X) {
-    LinePlan plans[3] = {plans_in[0], g.nd > 1 ? plans_in[1] : LinePlan(), g.nd > 2 ? plans_in[2] : LinePlan()};
+    LinePlan plans[kMaxDims];
+    for (int d = 0; d < g.nd; ++d) plans[d] = plans_in[d];
     RowLaunch r = row_launch<R>(g, S, plans[0]);
     auto s = reinterpret_cast<const cx<double> *>(src);
     auto d = reinterpret_cast<cx<double> *>(dst);
@@ -344,13 +345,11 @@ static void ham_apply_S(hipStream_t st, const GridDev &g, const HamDev<R> &h, co
     set_lds(k_row_fwd<R, S>, r.lds);
     QDX_LAUNCH((k_row_fwd<R, S>), dim3(r.nb), dim3(r.threads), r.lds, st, g, plans[0], r.rb, s, x);
     QDX_HIP(hipGetLastError());
-    if (g.nd == 2) {
-        Launch<R>::col_pass(st, g, S, 1, 2, plans[1], h, 0.0, X, nullptr, ConvCheck());
-    } else {
-        Launch<R>::col_pass(st, g, S, 1, 0, plans[1], h, 0.0, X, nullptr, ConvCheck());
-        Launch<R>::col_pass(st, g, S, 2, 2, plans[2], h, 0.0, X, nullptr, ConvCheck());
-        Launch<R>::col_pass(st, g, S, 1, 1, plans[1], h, 0.0, X, nullptr, ConvCheck());
-    }
+    // forward transforms along dimensions 1 .. nd-2, the last dimension's
+    // FFT * T(k)/N * IFFT, inverse transforms back down to dimension 1
+    for (int d = 1; d < g.nd - 1; ++d) Launch<R>::col_pass(st, g, S, d, 0, plans[d], h, 0.0, X, nullptr, ConvCheck());
+    Launch<R>::col_pass(st, g, S, g.nd - 1, 2, plans[g.nd - 1], h, 0.0, X, nullptr, ConvCheck());
+    for (int d = g.nd - 2; d >= 1; --d) Launch<R>::col_pass(st, g, S, d, 1, plans[d], h, 0.0, X, nullptr, ConvCheck());
     set_lds(k_apply_row<R, S, false>, r.lds);
     QDX_LAUNCH((k_apply_row<R, S, false>), dim3(r.nb), dim3(r.threads), r.lds, st, g, h, plans[0], r.rb, s, x,
                        d);
