@@ -11,6 +11,23 @@ namespace qdx {
 //   OGridDipole.cpp:83-87, ODMultistate.cpp:109-159 / TransformMSD.cpp:31-101)
 template <class R, int S, bool IMAG = false>
 __device__ __forceinline__ void xsplit_point(const XSplitDev<R> &op, long gi, cx<double> (&v)[S]) {
+    if (!IMAG && op.U) {   // the table of a time-independent operator
+        auto u = [&](int c) {
+            const double *q = op.U + 2 * ((long)c * op.n + gi);
+            return mk<double>(q[0], q[1]);
+        };
+        if (S == 2 && op.coupled) {
+            const cx<double> u00 = u(0), u01 = u(1), u11 = u(2);
+            const cx<double> a0 = cmul(u00, v[0]) + cmul(u01, v[1]);
+            const cx<double> a1 = cmul(u01, v[0]) + cmul(u11, v[1]);
+            v[0] = a0;
+            v[1] = a1;
+        } else {
+#pragma unroll
+            for (int i = 0; i < S; ++i) v[i] = cmul(v[i], u(i));
+        }
+        return;
+    }
     double Wm[S][S];
 #pragma unroll
     for (int i = 0; i < S; ++i)
@@ -284,6 +301,49 @@ __global__ void k_gspo_xonly(GridDev g, XGroupDev<R> grp, cx<R> *__restrict__ ps
     for (int st = 0; st < S; ++st) psi[(long)st * g.size + gi] = from_d<R>(v[st]);
 }
 
+// The table XSplitDev::U: the per-point arithmetic above applied to unit
+// vectors (exact zeros and ones, so each entry is that arithmetic's product
+// rounded once): coupled two states U00, U01 from e_0 and U11 from e_1,
+// otherwise the S diagonal phases from (1, ..., 1).
+template <class R, int S>
+__global__ void k_xsplit_unitary(GridDev g, XSplitDev<R> op, double *__restrict__ U) {
+    const long gi = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= g.size) return;
+    op.U = nullptr;
+    auto put = [&](int c, cx<double> z) {
+        U[2 * ((long)c * g.size + gi)] = z.r;
+        U[2 * ((long)c * g.size + gi) + 1] = z.i;
+    };
+    cx<double> v[S];
+    if (S == 2 && op.coupled) {
+        v[0] = mk<double>(1, 0);
+        v[1] = mk<double>(0, 0);
+        xsplit_point<R, S>(op, gi, v);
+        put(0, v[0]);
+        put(1, v[1]);
+        v[0] = mk<double>(0, 0);
+        v[1] = mk<double>(1, 0);
+        xsplit_point<R, S>(op, gi, v);
+        put(2, v[1]);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) v[i] = mk<double>(1, 0);
+    xsplit_point<R, S>(op, gi, v);
+#pragma unroll
+    for (int i = 0; i < S; ++i) put(i, v[i]);
+}
+
+template <class R>
+void Launch<R>::gspo_unitary(hipStream_t s, const GridDev &g, int S, const XSplitDev<R> &op, double *U) {
+    const int threads = 256;
+    const int nb = (int)((g.size + threads - 1) / threads);
+    QDX_DISPATCH_S(S, {
+        QDX_LAUNCH((k_xsplit_unitary<R, SS>), dim3(nb), dim3(threads), 0, s, g, op, U);
+    });
+    QDX_HIP(hipGetLastError());
+}
+
 // returns the number of row blocks launched
 template <class R, bool ONE_D, int PHASE, bool IMAG, int P>
 static int gspo_row_launch_IP(hipStream_t s, const GridDev &g, int S, const HamDev<R> &kin, double tau,
@@ -389,6 +449,7 @@ void Launch<R>::gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupD
     template void Launch<R>::gspo_1d(hipStream_t, const GridDev &, int, const LinePlan &, const HamDev<R> &,       \
                                      double, const XGroupDev<R> &, const XGroupDev<R> &, R *, int);                 \
     template void Launch<R>::gspo_xonly(hipStream_t, const GridDev &, int, const XGroupDev<R> &, R *, int);       \
+    template void Launch<R>::gspo_unitary(hipStream_t, const GridDev &, int, const XSplitDev<R> &, double *);     \
     template int Launch<R>::gspo_row_bridge(hipStream_t, const GridDev &, int, const LinePlan &,                   \
                                             const XGroupDev<R> &, const XGroupDev<R> &, R *, const R *, double *,  \
                                             int);

@@ -174,7 +174,7 @@ std::vector<SplitOp> gspo_sequence(qdx_prop *p) {
 }
 
 template <class R>
-XSplitDev<R> xsplit(qdx_ham *op, double tau, long step) {
+XSplitDev<R> xsplit(qdx_prop *p, qdx_ham *op, double tau, long step) {
     XSplitDev<R> x;
     x.S = op->S;
     x.tau = tau;
@@ -183,13 +183,31 @@ XSplitDev<R> xsplit(qdx_ham *op, double tau, long step) {
     if constexpr (std::is_same<R, float>::value) h = op->dev.h32;
     else h = op->dev.h64;
     op->set_fields(h, step);
+    bool dipole = false;
     for (int i = 0; i < op->S; ++i)
         for (int j = 0; j < op->S; ++j) {
             x.V[i][j] = h.W[i][j];
             x.mu[i][j] = h.mu[i][j];
             x.Edip[i][j] = h.Edip[i][j];
+            if (h.mu[i][j]) dipole = true;
             if (i != j && (h.W[i][j] || h.mu[i][j])) x.coupled = 1;
         }
+    // a real-time operator without a field is the same unitary at every
+    // step: its per-point table is computed once (the operator's terms are
+    // fixed once it is in use)
+    if (!p->imag && !dipole && (!x.coupled || op->S == 2)) {
+        const auto key = std::make_tuple((const qdx_ham *)op, tau, std::is_same<R, float>::value);
+        auto it = p->unitaries.find(key);
+        if (it == p->unitaries.end()) {
+            const GridDev g = p->grid.dev();
+            const size_t ncomp = x.coupled ? 3 : (size_t)op->S;
+            auto buf = std::make_unique<DevBuf>(ncomp * (size_t)g.size * 2 * sizeof(double));
+            Launch<R>::gspo_unitary(p->ctx->stream, g, op->S, x, buf->as<double>());
+            it = p->unitaries.emplace(key, std::move(buf)).first;
+        }
+        x.U = it->second->template as<double>();
+        x.n = p->grid.size();
+    }
     return x;
 }
 
@@ -227,7 +245,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
                 Launch<R>::gspo_xonly(s, g, p->S, pending, psi, p->imag);
                 pending.n = 0;
             }
-            pending.op[pending.n++] = xsplit<R>(o.op, o.tau, step);
+            pending.op[pending.n++] = xsplit<R>(p, o.op, o.tau, step);
             ++i;
             continue;
         }
@@ -236,7 +254,7 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
         post.n = 0;
         size_t j = i + 1;
         while (j < seq.size() && !seq[j].kinetic && post.n < kMaxXOps) {
-            post.op[post.n++] = xsplit<R>(seq[j].op, seq[j].tau, step);
+            post.op[post.n++] = xsplit<R>(p, seq[j].op, seq[j].tau, step);
             ++j;
         }
         HamDev<R> kin;
@@ -308,7 +326,7 @@ void gspo_run_fused(qdx_prop *p, R *psi, const R *psi0, long first, long nsteps,
     auto group = [&](size_t a, size_t b, long step) {
         XGroupDev<R> grp;
         grp.n = 0;
-        for (size_t i = a; i < b; ++i) grp.op[grp.n++] = xsplit<R>(seq[i].op, seq[i].tau, step);
+        for (size_t i = a; i < b; ++i) grp.op[grp.n++] = xsplit<R>(p, seq[i].op, seq[i].tau, step);
         return grp;
     };
     HamDev<R> kin;

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -976,6 +977,9 @@ struct qdx_prop {
     int arn_order = 0;    // converged order; kept between steps as OArnoldi does
     // GSPO
     std::vector<qdx_ham *> ops;
+    // exp(-i tau W) tables of time-independent x-space operators, per
+    // (operator, tau, FP32 arrays)
+    std::map<std::tuple<const qdx_ham *, double, bool>, std::unique_ptr<DevBuf>> unitaries;
     DevBuf psi32, psi0_32;
     DevBuf obs;   // device observables of qdx_prop_run
 

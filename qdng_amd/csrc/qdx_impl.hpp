@@ -168,6 +168,12 @@ struct XSplitDev {
     const R *V[kMaxStates][kMaxStates] = {};   // potential arrays
     const R *mu[kMaxStates][kMaxStates] = {};  // dipole arrays
     double Edip[kMaxStates][kMaxStates] = {};  // -laser[step]*scale
+    // exp(-i tau W) per grid point, computed once for a time-independent
+    // operator (no dipole), FP64 complex as (re, im) pairs, component-major
+    // (U + 2 (c n + x)): coupled two states c = 0, 1, 2 for U00, U01, U11,
+    // otherwise c = state (the diagonal phases).  nullptr: evaluated per point.
+    const double *U = nullptr;
+    long n = 0;
 };
 
 constexpr int kMaxXOps = 4;
@@ -246,6 +252,8 @@ struct Launch {
                         const XGroupDev<R> &post, R *psi, int imag = 0);
     static void gspo_xonly(hipStream_t s, const GridDev &g, int S, const XGroupDev<R> &grp,
                            R *psi, int imag = 0);
+    // the table XSplitDev::U of a real-time operator op (3 n or S n complex)
+    static void gspo_unitary(hipStream_t s, const GridDev &g, int S, const XSplitDev<R> &op, double *U);
     // X = the unnormalised forward transform of every state's lines along
     // `dim` (src complex128)
     static void line_fft(hipStream_t s, const GridDev &g, int S, int dim, const LinePlan &p, const double *src,
