@@ -173,6 +173,37 @@ std::vector<SplitOp> gspo_sequence(qdx_prop *p) {
     return seq;
 }
 
+// exp(-i tau T_d(k)) for every state and dimension of a kinetic split
+// operator (layout of k_col's ek: state-major, dimensions in order, (re, im)
+// pairs), from the operator's own device tables T_d; a state without a
+// kinetic term gets 1.  Built once per (operator, tau).
+const double *kinetic_exp_table(qdx_prop *p, qdx_ham *op, double tau) {
+    const auto key = std::make_pair((const qdx_ham *)op, tau);
+    auto it = p->kinexp.find(key);
+    if (it != p->kinexp.end()) return it->second->as<double>();
+    const Grid &g = p->grid;
+    long tot = 0;
+    for (int d = 0; d < g.nd; ++d) tot += g.N[d];
+    std::vector<double> e(2 * (size_t)op->S * tot);
+    const HamDev<double> &h = op->dev.h64;
+    std::vector<double> t;
+    for (int s = 0; s < op->S; ++s) {
+        long off = 0;
+        for (int d = 0; d < g.nd; ++d) {
+            t.assign(g.N[d], 0.0);
+            if (h.tk[s][0]) HIPCHK(hipMemcpy(t.data(), h.tk[s][d], g.N[d] * sizeof(double), hipMemcpyDeviceToHost));
+            for (int k = 0; k < g.N[d]; ++k) {
+                e[2 * (s * tot + off + k)] = std::cos(tau * t[k]);
+                e[2 * (s * tot + off + k) + 1] = -std::sin(tau * t[k]);
+            }
+            off += g.N[d];
+        }
+    }
+    auto buf = std::make_unique<DevBuf>(e.size() * sizeof(double));
+    HIPCHK(hipMemcpy(buf->p, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice));
+    return p->kinexp.emplace(key, std::move(buf)).first->second->as<double>();
+}
+
 template <class R>
 XSplitDev<R> xsplit(qdx_prop *p, qdx_ham *op, double tau, long step) {
     XSplitDev<R> x;
@@ -269,9 +300,11 @@ void gspo_step(qdx_prop *p, R *psi, long step) {
                 KTimer t(ctx, QDX_K_GSPO_ROW);
                 Launch<R>::gspo_row_pre(s, g, p->S, plans[0], pending, psi, X, p->imag);
             }
+            const double *ek = p->imag ? nullptr : kinetic_exp_table(p, o.op, o.tau);
             auto col = [&](int dim, int mode) {
                 KTimer t(ctx, QDX_K_LINE_PASS);
-                Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, o.tau, X, nullptr, ConvCheck(), p->imag);
+                Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, o.tau, X, nullptr, ConvCheck(), p->imag,
+                                    ek);
             };
             full_t_passes(p->grid.nd, 3, col);
             {
@@ -333,6 +366,7 @@ void gspo_run_fused(qdx_prop *p, R *psi, const R *psi0, long first, long nsteps,
     if constexpr (std::is_same<R, float>::value) kin = seq[kpos].op->dev.h32;
     else kin = seq[kpos].op->dev.h64;
     const double tau = seq[kpos].tau;
+    const double *ek = p->imag ? nullptr : kinetic_exp_table(p, seq[kpos].op, tau);
     R *X = p->X.as<R>();
     double *opart = obs ? p->partials.as<double>() : nullptr;
     {
@@ -343,7 +377,7 @@ void gspo_run_fused(qdx_prop *p, R *psi, const R *psi0, long first, long nsteps,
         const long step = first + i;
         auto col = [&](int dim, int mode) {
             KTimer t(ctx, QDX_K_LINE_PASS);
-            Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, tau, X, nullptr, ConvCheck(), p->imag);
+            Launch<R>::col_pass(s, g, p->S, dim, mode, plans[dim], kin, tau, X, nullptr, ConvCheck(), p->imag, ek);
         };
         full_t_passes(p->grid.nd, 3, col);
         const XGroupDev<R> post = group(kpos + 1, seq.size(), step);

@@ -980,6 +980,8 @@ struct qdx_prop {
     // exp(-i tau W) tables of time-independent x-space operators, per
     // (operator, tau, FP32 arrays)
     std::map<std::tuple<const qdx_ham *, double, bool>, std::unique_ptr<DevBuf>> unitaries;
+    // exp(-i tau T_d(k)) tables of kinetic split operators, per (operator, tau)
+    std::map<std::pair<const qdx_ham *, double>, std::unique_ptr<DevBuf>> kinexp;
     DevBuf psi32, psi0_32;
     DevBuf obs;   // device observables of qdx_prop_run
 

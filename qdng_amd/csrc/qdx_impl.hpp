@@ -220,9 +220,11 @@ struct Launch {
     // mode 0: forward, 1: inverse, 2: forward * T/N inverse (kinetic),
     // 3: forward * exp(-i tau T)/N inverse (GSPO kinetic split operator),
     // or exp(-tau T)/N with imag (imaginary time)
+    // ek (mode 3, real time; nullable): per-state tables exp(-i tau T_d(k))
+    // of every dimension (see k_col)
     static void col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode,
                          const LinePlan &p, const HamDev<R> &h, double tau, R *X,
-                         ChebyCtl *ctl, const ConvCheck &cc, int imag = 0);
+                         ChebyCtl *ctl, const ConvCheck &cc, int imag = 0, const double *ek = nullptr);
     // kinetic part along dimension `dim` of the separable T = sum_d T_d:
     // Y = IFFT_d(T_d/N_d FFT_d(src)), or Y += that when `accumulate`.  The
     // first pass of an order also evaluates the break test of the previous

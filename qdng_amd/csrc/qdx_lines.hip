@@ -18,7 +18,7 @@ namespace qdx {
 template <class R, int MODE, int P = 8, bool IMAG = false, bool CMP = false>
 __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, LinePlan p, HamDev<R> h, double tau,
                                               cx<R> *__restrict__ X, const cx<R> *__restrict__ src,
-                                              ChebyCtl *ctl, ConvCheck cc) {
+                                              ChebyCtl *ctl, ConvCheck cc, const double *__restrict__ ek) {
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) char smem_raw[];
     cx<R> *smem = reinterpret_cast<cx<R> *>(smem_raw);
@@ -62,9 +62,39 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
             for (int q = 0; q < P; ++q) yv[q] = tld<CMP>(X, boff + (long)(lt + q * TL) * estride, nT);
         }
     }
-    // kinetic table values of this thread's elements, requested early
+    // kinetic table values of this thread's elements, requested early.
+    // Real-time GSPO (mode 3) with ek: the factor exp(-i tau T(k)) / N as
+    // inv_n times the product over dimensions of exp(-i tau T_d(k_d)) from
+    // the per-state tables ek (state st at 2 st sum_d N_d doubles, dimension
+    // d after the tables of the dimensions before it, (re, im) pairs)
     double tq[P];
-    if (MODE >= 2 && active) {
+    cx<double> fq[P];
+    const bool etab = MODE == 3 && !IMAG && ek != nullptr;
+    if (etab && active) {
+        long tot = 0, offd = 0;
+        long off[kMaxDims] = {0, 0, 0, 0};
+#pragma unroll
+        for (int e = 0; e < kMaxDims; ++e) {
+            off[e] = tot;
+            if (e < g.nd) tot += g.N[e];
+            if (e == dim) offd = off[e];
+        }
+        const double *es = ek + 2 * (long)st * tot;
+        kk[0] = c0 + c;
+        cx<double> fo = mk<double>(h.inv_n, 0.0);
+#pragma unroll
+        for (int e = 0; e < kMaxDims; ++e) {
+            if (e < g.nd && e != dim) {
+                const double *q = es + 2 * (off[e] + kk[e]);
+                fo = cmul(fo, mk<double>(q[0], q[1]));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const double *t = es + 2 * (offd + lt + q * TL);
+            fq[q] = cmul(fo, mk<double>(t[0], t[1]));
+        }
+    } else if (MODE >= 2 && active) {
         const bool has_t = h.tk[st][0] != nullptr;
         kk[0] = c0 + c;
 #pragma unroll
@@ -106,6 +136,8 @@ __global__ void __launch_bounds__(512) k_col(GridDev g, int S, int dim, int W, L
                     // exp(-tau T) / N, imaginary time
                     const double f = exp(-tau * tt) * h.inv_n;
                     v[q] = mk<R>((R)((double)v[q].r * f), (R)((double)v[q].i * f));
+                } else if (etab) {
+                    v[q] = from_d<R>(cmul(to_d(v[q]), fq[q]));
                 } else {
                     // exp(-i tau T) / N (OGSPO.cpp:37-41 folds 1/N into the exponential)
                     double sn, cs;
@@ -200,7 +232,8 @@ __global__ void __launch_bounds__(512) k_apply_row(GridDev g, HamDev<R> h, LineP
 
 template <class R, int P>
 static void col_pass_P4(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
-                        const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag) {
+                        const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag,
+                        const double *ek) {
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>), P);
     const int nb = (g.N[0] / W) * col_nother(g, dim) * S;
@@ -210,28 +243,29 @@ static void col_pass_P4(hipStream_t s, const GridDev &g, int S, int dim, int mod
     LinePlan pl = plan;
     pl.tw_lds = 1;
     const cx<R> *none = nullptr;
+    const double *ekp = mode == 3 && !imag ? ek : nullptr;
     switch (mode) {
     case 0:
         set_lds(k_col<R, 0, P>, lds);
-        QDX_LAUNCH((k_col<R, 0, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 0, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     case 1:
         set_lds(k_col<R, 1, P>, lds);
-        QDX_LAUNCH((k_col<R, 1, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 1, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     case 3:
         if (imag) {
             if constexpr (std::is_same<R, double>::value) {
                 set_lds(k_col<R, 3, P, true>, lds);
                 QDX_LAUNCH((k_col<R, 3, P, true>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x,
-                           none, ctl, cc);
+                           none, ctl, cc, ekp);
             } else {
                 throw std::runtime_error("imaginary time: FP64 only");
             }
             break;
         }
         set_lds(k_col<R, 3, P>, lds);
-        QDX_LAUNCH((k_col<R, 3, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 3, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     default:
         throw std::runtime_error("line pass: mode 0, 1 or 3 at 4 points per thread");
@@ -241,8 +275,9 @@ static void col_pass_P4(hipStream_t s, const GridDev &g, int S, int dim, int mod
 
 template <class R>
 void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mode, const LinePlan &plan,
-                         const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag) {
-    if (plan.ppt == 4) return col_pass_P4<R, 4>(s, g, S, dim, mode, plan, h, tau, X, ctl, cc, imag);
+                         const HamDev<R> &h, double tau, R *X, ChebyCtl *ctl, const ConvCheck &cc, int imag,
+                         const double *ek) {
+    if (plan.ppt == 4) return col_pass_P4<R, 4>(s, g, S, dim, mode, plan, h, tau, X, ctl, cc, imag, ek);
     const int L = g.N[dim];
     const int W = col_tile(g, S, dim, L, sizeof(cx<R>));
     const int nb = (g.N[0] / W) * col_nother(g, dim) * S;
@@ -252,32 +287,33 @@ void Launch<R>::col_pass(hipStream_t s, const GridDev &g, int S, int dim, int mo
     LinePlan pl = plan;
     pl.tw_lds = 1;
     const cx<R> *none = nullptr;
+    const double *ekp = mode == 3 && !imag ? ek : nullptr;
     switch (mode) {
     case 0:
         set_lds(k_col<R, 0>, lds);
-        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 0>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     case 1:
         set_lds(k_col<R, 1>, lds);
-        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 1>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     case 2:
         set_lds(k_col<R, 2>, lds);
-        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 2>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     default:
         if (imag) {   // imaginary time: FP64 only
             if constexpr (std::is_same<R, double>::value) {
                 set_lds(k_col<R, 3, 8, true>, lds);
                 QDX_LAUNCH((k_col<R, 3, 8, true>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none,
-                           ctl, cc);
+                           ctl, cc, ekp);
             } else {
                 throw std::runtime_error("imaginary time: FP64 only");
             }
             break;
         }
         set_lds(k_col<R, 3>, lds);
-        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc);
+        QDX_LAUNCH((k_col<R, 3>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, tau, x, none, ctl, cc, ekp);
         break;
     }
     QDX_HIP(hipGetLastError());
@@ -293,6 +329,7 @@ static void kin_pass_P(hipStream_t s, const GridDev &g, int S, int dim, bool acc
     const size_t lds = col_lds_bytes(W, L, sizeof(cx<R>));
     cx<R> *y = reinterpret_cast<cx<R> *>(Y);
     const cx<R> *in = reinterpret_cast<const cx<R> *>(src);
+    const double *ekp = nullptr;
     LinePlan pl = plan;
     pl.tw_lds = 1;
     if (cmp) {
@@ -300,21 +337,21 @@ static void kin_pass_P(hipStream_t s, const GridDev &g, int S, int dim, bool acc
             if (accumulate) {
                 set_lds(k_col<R, 5, P, false, true>, lds);
                 QDX_LAUNCH((k_col<R, 5, P, false, true>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y,
-                           in, ctl, cc);
+                           in, ctl, cc, ekp);
             } else {
                 set_lds(k_col<R, 4, P, false, true>, lds);
                 QDX_LAUNCH((k_col<R, 4, P, false, true>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y,
-                           in, ctl, cc);
+                           in, ctl, cc, ekp);
             }
         } else {
             throw std::runtime_error("compact terms: FP64 only");
         }
     } else if (accumulate) {
         set_lds(k_col<R, 5, P>, lds);
-        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+        QDX_LAUNCH((k_col<R, 5, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc, ekp);
     } else {
         set_lds(k_col<R, 4, P>, lds);
-        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc);
+        QDX_LAUNCH((k_col<R, 4, P>), dim3(nb), dim3(threads), lds, s, g, S, dim, W, pl, h, 0.0, y, in, ctl, cc, ekp);
     }
     QDX_HIP(hipGetLastError());
 }
@@ -383,7 +420,8 @@ void Launch<R>::line_fft(hipStream_t s, const GridDev &g, int S, int dim, const 
 
 #define QDX_INST_LINES(R)                                                                                      \
     template void Launch<R>::col_pass(hipStream_t, const GridDev &, int, int, int, const LinePlan &,               \
-                                      const HamDev<R> &, double, R *, ChebyCtl *, const ConvCheck &, int);          \
+                                      const HamDev<R> &, double, R *, ChebyCtl *, const ConvCheck &, int,           \
+                                      const double *);                                                              \
     template void Launch<R>::kin_pass(hipStream_t, const GridDev &, int, int, bool, const LinePlan &,              \
                                       const HamDev<R> &, const R *, R *, ChebyCtl *, const ConvCheck &, bool);      \
     template void Launch<R>::ham_apply(hipStream_t, const GridDev &, const HamDev<R> &, const LinePlan *,          \

@@ -152,20 +152,22 @@ def test_batched_2026_mix_three_ticks(ctx):
     # every job against the single-job path.  Both sum |psi|^2 for QDng's
     # break test in their own order, so a test that lands within rounding of
     # prec can end one series an order later than the other: such a job then
-    # differs by about that term (~prec), within the 1e-10 bar; jobs with the
-    # same series lengths agree to rounding
+    # differs by about that term (~prec), within the 1e-10 bar.  The batch
+    # reports only the total count of the three steps, so two such steps of
+    # opposite sign leave the totals equal; jobs whose series have the same
+    # lengths agree to rounding (1e-12)
     worst, nlen = 0.0, 0
     for i, p in enumerate(jobs):
         U = Q.Cheby(p.gpu_ham(ctx), 1.0, prec=1e-10)
         w = p.gpu_wf(ctx)
         nh = sum(U.apply(w, s).n_hpsi for s in range(ticks))
         e = rel(got[i], w.download())
-        if nh == nhb[i]:
+        if nh == nhb[i] and e < 1e-12:
             worst = max(worst, e)
-            assert e < 1e-12, (i, p.N, e)
         else:
             nlen += 1
-            assert abs(nh - nhb[i]) == 1 and e < 1e-10, (i, p.N, nh, nhb[i], e)
+            note(f"  job {i} N={p.N}: a series one order apart ({nh} / {nhb[i]} H psi), rel {e:.2e}")
+            assert abs(nh - nhb[i]) <= 1 and e < 1e-10, (i, p.N, nh, nhb[i], e)
     assert nlen <= 8
     note(f"  512 jobs vs single path: worst rel L2 {worst:.2e} ({nlen} series one order apart)")
 
