@@ -139,12 +139,15 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
             for (int q = 0; q < P; ++q) xa[((long)st * B.rb + B.rr) * B.L + B.lt + q * B.TL] = v[q];
         }
         __syncthreads();
-        // the points of row rr are shared out among the S lines' threads
+        // the points of row rr are shared out among the S lines' threads:
+        // point lt + q TL goes to thread lt of line q mod S, so each thread
+        // takes P / S of them (x mod S would give a thread all of its P
+        // points or none when TL is a multiple of S)
         if (B.active) {
 #pragma unroll
             for (int q = 0; q < P; ++q) {
                 const int x = B.lt + q * B.TL;
-                if (x % S != st) continue;
+                if (q % S != st) continue;
                 cx<double> w[S];
 #pragma unroll
                 for (int s2 = 0; s2 < S; ++s2) w[s2] = to_d(xa[((long)s2 * B.rb + B.rr) * B.L + x]);
@@ -238,7 +241,7 @@ __global__ void __launch_bounds__(512) k_gspo_row(GridDev g, HamDev<R> kin, doub
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     const int x = B.lt + q * B.TL;
-                    if (x % S != st) continue;
+                    if (q % S != st) continue;
                     cx<double> w[S];
 #pragma unroll
                     for (int s2 = 0; s2 < S; ++s2) w[s2] = to_d(xa[((long)s2 * B.rb + B.rr) * B.L + x]);
