@@ -469,8 +469,8 @@ int qdx_ctx_get_radix_plan(qdx_ctx *ctx, int32_t precision, int32_t L, int32_t *
  * two or more dimensions (the row kernel and the kinetic passes along
  * dimensions >= 1), for steps taken afterwards: 8, 4, or 0 (the default) to
  * choose 4 where the lines fit (states * N0 / 4 <= 512 threads, N_d <= 2048)
- * and, for the FP64 and compact kernels, the grid gives fewer than two waves
- * per SIMD at 8.  At 4 a line
+ * and, for the FP64 kernels, the grid gives fewer than two waves per SIMD
+ * at 8.  At 4 a line
  * uses the context's plan for its length when that plan's first and last
  * radix are 4 or 2, else first and last radix 4.  Both give the same
  * transform up to rounding.  Single-state 1D lines run a whole step in one

@@ -25,15 +25,16 @@ qdx_wf *check_wf(qdx_wf *w) {
 }
 
 // points per thread of the Chebyshev line kernels on this grid (see
-// qdx_ctx_set_line_points).  fp32_series: the FP32 Chebyshev kernels, which
-// take 4 wherever the lines fit (3D 128^3: 2.46 -> 2.12 ms per step; the
-// FP64 kernels time the same at 4 and 8 on that grid)
-int cheby_line_points(const qdx_ctx *ctx, const GridDev &g, int S, bool fp32_series = false) {
+// qdx_ctx_set_line_points).  reduced: the FP32 and compact Chebyshev
+// kernels, which take 4 wherever the lines fit (3D 128^3: FP32 2.46 -> 2.12,
+// compact 3.41 -> 2.91 ms per step; the FP64 kernels time the same at 4 and
+// 8 on that grid)
+int cheby_line_points(const qdx_ctx *ctx, const GridDev &g, int S, bool reduced = false) {
     if (g.nd < 2 || ctx->line_ppt == 8) return 8;
     if (S * (g.N[0] / 4) > 512) return 8;
     for (int d = 1; d < g.nd; ++d)
         if (g.N[d] > 2048) return 8;
-    if (ctx->line_ppt == 4 || fp32_series) return 4;
+    if (ctx->line_ppt == 4 || reduced) return 4;
     const long waves8 = (long)S * g.size / (8 * 64);
     return waves8 < 2 * 1024 ? 4 : 8;   // 256 CUs x 4 SIMDs
 }
@@ -50,7 +51,7 @@ void cheby_step(qdx_prop *p, qdx_wf *psi, long step) {
     else h = H->dev.h64;
     H->set_fields(h, step);
     LinePlan plans[kMaxDims];
-    const int ppt = cheby_line_points(ctx, g, p->S, fp32);
+    const int ppt = cheby_line_points(ctx, g, p->S, fp32 || p->cmp);
     for (int d = 0; d < p->grid.nd; ++d) plans[d] = ctx->plan_ppt(p->grid.N[d], fp32, ppt);
     R *A = p->phiA.as<R>(), *B = p->phiB.as<R>();
     R *X = p->grid.nd > 1 ? p->X.as<R>() : nullptr;
