@@ -600,6 +600,9 @@ def main():
     ap.add_argument("--gspo-only", action="store_true",
                     help="only run the GSPO leg (FP64 and FP32, --steps steps after --warmup) and print its step "
                          "time: a short program for traces")
+    ap.add_argument("--precision-3d-only", action="store_true",
+                    help="only run the 3D 128^3 precision workload (FP64, compact, FP32 at dt 3.5 and 37) and print "
+                         "its record: a short program for traces")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -619,6 +622,13 @@ def main():
         r = run_batched(ctx, args.batched_only, args.batch_steps, 1)
         log(f"batched {args.batched_only}: {r['seconds'] * 1e3 / args.batch_steps:.3f} ms/tick "
             f"over {args.batch_steps} ticks")
+        return
+    if args.precision_3d_only:
+        rec = precision_3d(ctx)
+        for dt, r in rec["dt"].items():
+            for prec in PRECISIONS:
+                log(f"3D dt {dt} {prec}: {r[prec]['value']:.4g} grid-pt*Hpsi/s, "
+                    f"row kernel {r[prec]['roofline']['avg_launch_us']:.2f} us ({r[prec]['roofline']['frac']:.3f})")
         return
     p = build_problem(512)
     npts = p.size * p.S
